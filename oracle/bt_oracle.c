@@ -1,0 +1,704 @@
+/*
+ * CPU restatement of INMSOM's barotropic hot path (TEST INFRASTRUCTURE:
+ * the checker for the HIP implementation, never part of the product).
+ *
+ * Each function cites the reference routine it restates.  Expressions
+ * keep the reference's operand order and parenthesisation, because
+ * parity with the compiled reference is checked bit for bit; the file
+ * must be compiled with -ffp-contract=off.
+ *
+ * Single-rank semantics: the reference's halo exchange
+ * (Modules/mod_mpi_parallel_tools.f90:179-252) moves nothing when the
+ * process grid is 1x1, so the sync calls disappear.  Periodic wrap
+ * (Service/oc_algorithms.f90:50-135) is restated as the copy a
+ * decomposed run performs: column mm -> mmm-1 and mmm -> mm+1 (rows
+ * likewise).
+ */
+#include "bt_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define IX(m, n) ((size_t)((n) - 1) * (size_t)g->nx + (size_t)((m) - 1))
+#define NXS 3
+#define NXE (g->nx - 2)
+#define NYS 3
+#define NYE (g->ny - 2)
+
+/* Service/oc_algorithms.f90:50-95 (decomposed-run semantics). */
+static void cyclize_x(const bto_grid *g, double *f)
+{
+    for (int n = 1; n <= g->ny; ++n) {
+        double a = f[IX(NXE, n)], b = f[IX(NXS, n)];
+        f[IX(NXS - 1, n)] = a;
+        f[IX(NXE + 1, n)] = b;
+    }
+}
+
+/* Service/oc_algorithms.f90:118-135. */
+static void cyclize_y(const bto_grid *g, double *f)
+{
+    for (int m = 1; m <= g->nx; ++m) {
+        double a = f[IX(m, NYE)], b = f[IX(m, NYS)];
+        f[IX(m, NYS - 1)] = a;
+        f[IX(m, NYE + 1)] = b;
+    }
+}
+
+static void cyclize(const bto_grid *g, double *f)
+{
+    if (g->periodicity_x) cyclize_x(g, f);
+    if (g->periodicity_y) cyclize_y(g, f);
+}
+
+static void cyclize_xf(int nx, int ny, float *f)
+{
+    for (int n = 0; n < ny; ++n) {
+        float a = f[(size_t)n * nx + (nx - 3)], b = f[(size_t)n * nx + 2];
+        f[(size_t)n * nx + 1] = a;
+        f[(size_t)n * nx + (nx - 2)] = b;
+    }
+}
+
+static void cyclize_yf(int nx, int ny, float *f)
+{
+    for (int m = 0; m < nx; ++m) {
+        float a = f[(size_t)(ny - 3) * nx + m], b = f[(size_t)2 * nx + m];
+        f[(size_t)1 * nx + m] = a;
+        f[(size_t)(ny - 2) * nx + m] = b;
+    }
+}
+
+void bto_gridcon(int nx, int ny, int px, int py, const int *lb, float *lu,
+                 float *luu, float *luh, float *lcu, float *lcv, float *llu,
+                 float *llv)
+{
+    size_t sz = (size_t)nx * ny;
+    memset(lu, 0, sz * sizeof(float));
+    memset(luu, 0, sz * sizeof(float));
+    memset(luh, 0, sz * sizeof(float));
+    memset(lcu, 0, sz * sizeof(float));
+    memset(lcv, 0, sz * sizeof(float));
+    memset(llu, 0, sz * sizeof(float));
+    memset(llv, 0, sz * sizeof(float));
+#define J(m, n) ((size_t)((n) - 1) * (size_t)nx + (size_t)((m) - 1))
+    /* grid_construction.f90:33-39 */
+    for (int n = 1; n <= ny; ++n)
+        for (int m = 1; m <= nx; ++m)
+            if (lb[J(m, n)] == 0) lu[J(m, n)] = 1.0f;
+    if (px) cyclize_xf(nx, ny, lu);
+    if (py) cyclize_yf(nx, ny, lu);
+    /* :59-73 over nx_start-1..nx_end, ny_start-1..ny_end */
+    for (int n = 2; n <= ny - 2; ++n)
+        for (int m = 2; m <= nx - 2; ++m) {
+            if (lu[J(m, n)] + lu[J(m + 1, n)] + lu[J(m, n + 1)] + lu[J(m + 1, n + 1)] > 0.5f)
+                luh[J(m, n)] = 1.0f;
+            if (lu[J(m, n)] * lu[J(m + 1, n)] * lu[J(m, n + 1)] * lu[J(m + 1, n + 1)] > 0.5f)
+                luu[J(m, n)] = 1.0f;
+        }
+    if (px) { cyclize_xf(nx, ny, luh); cyclize_xf(nx, ny, luu); }
+    if (py) { cyclize_yf(nx, ny, luh); cyclize_yf(nx, ny, luu); }
+    /* :92-112 */
+    for (int n = 2; n <= ny - 2; ++n)
+        for (int m = 2; m <= nx - 2; ++m) {
+            if (lu[J(m, n)] + lu[J(m + 1, n)] > 0.5f) llu[J(m, n)] = 1.0f;
+            if (lu[J(m, n)] + lu[J(m, n + 1)] > 0.5f) llv[J(m, n)] = 1.0f;
+            if (lu[J(m, n)] * lu[J(m + 1, n)] > 0.5f) lcu[J(m, n)] = 1.0f;
+            if (lu[J(m, n)] * lu[J(m, n + 1)] > 0.5f) lcv[J(m, n)] = 1.0f;
+        }
+    if (px) {
+        cyclize_xf(nx, ny, lcu); cyclize_xf(nx, ny, llu);
+        cyclize_xf(nx, ny, lcv); cyclize_xf(nx, ny, llv);
+    }
+    if (py) {
+        cyclize_yf(nx, ny, lcu); cyclize_yf(nx, ny, llu);
+        cyclize_yf(nx, ny, lcv); cyclize_yf(nx, ny, llv);
+    }
+#undef J
+}
+
+/* Control/init_pars.f90:459-481 */
+void bto_rayleigh_init(const bto_grid *g, double r_fric, double *r_diss)
+{
+    const double *h = g->hhq_rest;
+#pragma omp parallel for schedule(static)
+    for (int n = NYS; n <= NYE; ++n)
+        for (int m = NXS; m <= NXE; ++m) {
+            if (g->lu[IX(m, n)] > 0.5f) {
+                double a = (h[IX(m + 1, n)] - h[IX(m, n)]) / g->dxt[IX(m, n)];
+                double b = (h[IX(m, n)] - h[IX(m - 1, n)]) / g->dxt[IX(m - 1, n)];
+                double hx2 = (a * a * (double)g->lcu[IX(m, n)] +
+                              b * b * (double)g->lcu[IX(m - 1, n)]) /
+                             (double)(g->lcu[IX(m, n)] + g->lcu[IX(m - 1, n)]);
+                double c = (h[IX(m, n + 1)] - h[IX(m, n)]) / g->dyt[IX(m, n)];
+                double e = (h[IX(m, n)] - h[IX(m, n - 1)]) / g->dyt[IX(m, n - 1)];
+                double hy2 = (c * c * (double)g->lcv[IX(m, n)] +
+                              e * e * (double)g->lcv[IX(m, n - 1)]) /
+                             (double)(g->lcv[IX(m, n)] + g->lcv[IX(m, n - 1)]);
+                r_diss[IX(m, n)] = r_fric * sqrt(hx2 + hy2);
+            }
+        }
+    cyclize(g, r_diss);
+}
+
+/* Depth on U/V/H points from a T-point depth, Function/depth.f90:29-66. */
+static inline double hu_of(const bto_grid *g, const double *hq, int m, int n)
+{
+    double slu = (double)(g->lu[IX(m, n)] + g->lu[IX(m + 1, n)]);
+    return (hq[IX(m, n)] * g->dx[IX(m, n)] * g->dy[IX(m, n)] * (double)g->lu[IX(m, n)] +
+            hq[IX(m + 1, n)] * g->dx[IX(m + 1, n)] * g->dy[IX(m + 1, n)] * (double)g->lu[IX(m + 1, n)]) /
+           slu / g->dxt[IX(m, n)] / g->dyh[IX(m, n)];
+}
+
+static inline double hv_of(const bto_grid *g, const double *hq, int m, int n)
+{
+    double slu = (double)(g->lu[IX(m, n)] + g->lu[IX(m, n + 1)]);
+    return (hq[IX(m, n)] * g->dx[IX(m, n)] * g->dy[IX(m, n)] * (double)g->lu[IX(m, n)] +
+            hq[IX(m, n + 1)] * g->dx[IX(m, n + 1)] * g->dy[IX(m, n + 1)] * (double)g->lu[IX(m, n + 1)]) /
+           slu / g->dxh[IX(m, n)] / g->dyt[IX(m, n)];
+}
+
+static inline double hh_of(const bto_grid *g, const double *hq, int m, int n)
+{
+    double slu = (double)(g->lu[IX(m, n)] + g->lu[IX(m + 1, n)] + g->lu[IX(m, n + 1)] +
+                          g->lu[IX(m + 1, n + 1)]);
+    return (hq[IX(m, n)] * g->dx[IX(m, n)] * g->dy[IX(m, n)] * (double)g->lu[IX(m, n)] +
+            hq[IX(m + 1, n)] * g->dx[IX(m + 1, n)] * g->dy[IX(m + 1, n)] * (double)g->lu[IX(m + 1, n)] +
+            hq[IX(m, n + 1)] * g->dx[IX(m, n + 1)] * g->dy[IX(m, n + 1)] * (double)g->lu[IX(m, n + 1)] +
+            hq[IX(m + 1, n + 1)] * g->dx[IX(m + 1, n + 1)] * g->dy[IX(m + 1, n + 1)] *
+                (double)g->lu[IX(m + 1, n + 1)]) /
+           slu / g->dxb[IX(m, n)] / g->dyb[IX(m, n)];
+}
+
+/* Function/depth.f90:2-105 */
+void bto_hh_init(const bto_grid *g, bto_depths *d, const double *ssh,
+                 const double *sshp)
+{
+    size_t sz = (size_t)g->nx * g->ny;
+    double ffs = (double)g->full_free_surface;
+    double *hq = d->h[0], *hqp = d->h[1], *hqn = d->h[2];
+    for (size_t i = 0; i < sz; ++i) {
+        hq[i] = g->hhq_rest[i] + ssh[i] * ffs;
+        hqp[i] = g->hhq_rest[i] + sshp[i] * ffs;
+        hqn[i] = g->hhq_rest[i];
+    }
+#pragma omp parallel for schedule(static)
+    for (int n = NYS - 1; n <= NYE; ++n)
+        for (int m = NXS - 1; m <= NXE; ++m) {
+            if (g->llu[IX(m, n)] > 0.5f) {
+                d->h[3][IX(m, n)] = hu_of(g, hq, m, n);
+                d->h[4][IX(m, n)] = hu_of(g, hqp, m, n);
+                d->h[5][IX(m, n)] = hu_of(g, hqn, m, n);
+            }
+            if (g->llv[IX(m, n)] > 0.5f) {
+                d->h[6][IX(m, n)] = hv_of(g, hq, m, n);
+                d->h[7][IX(m, n)] = hv_of(g, hqp, m, n);
+                d->h[8][IX(m, n)] = hv_of(g, hqn, m, n);
+            }
+            if (g->luh[IX(m, n)] > 0.5f) {
+                d->h[9][IX(m, n)] = hh_of(g, hq, m, n);
+                d->h[10][IX(m, n)] = hh_of(g, hqp, m, n);
+                d->h[11][IX(m, n)] = hh_of(g, hqn, m, n);
+            }
+        }
+    for (int k = 3; k < 12; ++k) cyclize(g, d->h[k]);
+}
+
+/* Function/depth.f90:108-170 */
+void bto_hh_update(const bto_grid *g, double *hqn, double *hun, double *hvn,
+                   double *hhn, const double *sh)
+{
+    size_t sz = (size_t)g->nx * g->ny;
+    for (size_t i = 0; i < sz; ++i) hqn[i] = g->hhq_rest[i] + sh[i];
+#pragma omp parallel for schedule(static)
+    for (int n = NYS - 1; n <= NYE; ++n)
+        for (int m = NXS - 1; m <= NXE; ++m) {
+            if (g->llu[IX(m, n)] > 0.5f) hun[IX(m, n)] = hu_of(g, hqn, m, n);
+            if (g->llv[IX(m, n)] > 0.5f) hvn[IX(m, n)] = hv_of(g, hqn, m, n);
+            if (g->luh[IX(m, n)] > 0.5f) hhn[IX(m, n)] = hh_of(g, hqn, m, n);
+        }
+    cyclize(g, hun);
+    cyclize(g, hvn);
+    cyclize(g, hhn);
+}
+
+/* Function/depth.f90:174-220 */
+void bto_hh_shift(const bto_grid *g, bto_depths *d, int nstep)
+{
+    const double ts = g->time_smooth, dn = (double)nstep;
+#pragma omp parallel for schedule(static)
+    for (int n = NYS - 1; n <= NYE + 1; ++n)
+        for (int m = NXS - 1; m <= NXE + 1; ++m) {
+            size_t i = IX(m, n);
+            if (g->llu[i] > 0.5f) {
+                double *h = d->h[3], *hp = d->h[4], *hn = d->h[5];
+                hp[i] = h[i] + ts * (hn[i] - 2.0 * h[i] + hp[i]) / 2.0 / dn;
+                h[i] = hn[i];
+            }
+            if (g->llv[i] > 0.5f) {
+                double *h = d->h[6], *hp = d->h[7], *hn = d->h[8];
+                hp[i] = h[i] + ts * (hn[i] - 2.0 * h[i] + hp[i]) / 2.0 / dn;
+                h[i] = hn[i];
+            }
+            if (g->lu[i] > 0.5f) {
+                double *h = d->h[0], *hp = d->h[1], *hn = d->h[2];
+                hp[i] = h[i] + ts * (hn[i] - 2.0 * h[i] + hp[i]) / 2.0 / dn;
+                h[i] = hn[i];
+            }
+            if (g->luh[i] > 0.5f) {
+                double *h = d->h[9], *hp = d->h[10], *hn = d->h[11];
+                hp[i] = h[i] + ts * (hn[i] - 2.0 * h[i] + hp[i]) / 2.0 / dn;
+                h[i] = hn[i];
+            }
+        }
+}
+
+/* Function/mixing.f90:115-168 */
+void bto_stress_components(const bto_grid *g, const double *u, const double *v,
+                           double *str_t, double *str_s)
+{
+#pragma omp parallel for schedule(static)
+    for (int n = NYS; n <= NYE; ++n)
+        for (int m = NXS; m <= NXE; ++m)
+            if (g->lu[IX(m, n)] > 0.5f)
+                str_t[IX(m, n)] =
+                    g->dy[IX(m, n)] / g->dx[IX(m, n)] *
+                        (u[IX(m, n)] / g->dyh[IX(m, n)] - u[IX(m - 1, n)] / g->dyh[IX(m - 1, n)]) -
+                    g->dx[IX(m, n)] / g->dy[IX(m, n)] *
+                        (v[IX(m, n)] / g->dxh[IX(m, n)] - v[IX(m, n - 1)] / g->dxh[IX(m, n - 1)]);
+#pragma omp parallel for schedule(static)
+    for (int n = NYS; n <= NYE; ++n)
+        for (int m = NXS; m <= NXE; ++m)
+            if (g->luu[IX(m, n)] > 0.5f)
+                str_s[IX(m, n)] =
+                    g->dxb[IX(m, n)] / g->dyb[IX(m, n)] *
+                        (u[IX(m, n + 1)] / g->dxt[IX(m, n + 1)] - u[IX(m, n)] / g->dxt[IX(m, n)]) +
+                    g->dyb[IX(m, n)] / g->dxb[IX(m, n)] *
+                        (v[IX(m + 1, n)] / g->dyt[IX(m + 1, n)] - v[IX(m, n)] / g->dyt[IX(m, n)]);
+    cyclize(g, str_t);
+    cyclize(g, str_s);
+}
+
+/* Function/vel_ssh.f90:2-114 */
+void bto_uv_trans(const bto_grid *g, const double *u, const double *v,
+                  double *vort, const double *hq, const double *hu,
+                  const double *hv, const double *hh, double *rhsx,
+                  double *rhsy)
+{
+    (void)hq;
+    const double *dxt = g->dxt, *dyt = g->dyt, *dxb = g->dxb, *dyb = g->dyb;
+    const double *dxh = g->dxh, *dyh = g->dyh;
+#pragma omp parallel for schedule(static)
+    for (int n = NYS; n <= NYE; ++n)
+        for (int m = NXS; m <= NXE; ++m)
+            if (g->luu[IX(m, n)] > 0.5f)
+                vort[IX(m, n)] =
+                    (v[IX(m + 1, n)] * dyt[IX(m + 1, n)] - v[IX(m, n)] * dyt[IX(m, n)]) -
+                    (u[IX(m, n + 1)] * dxt[IX(m, n + 1)] - u[IX(m, n)] * dxt[IX(m, n)]) -
+                    ((v[IX(m + 1, n)] - v[IX(m, n)]) * dyb[IX(m, n)] -
+                     (u[IX(m, n + 1)] - u[IX(m, n)]) * dxb[IX(m, n)]);
+    cyclize(g, vort);
+
+#pragma omp parallel for schedule(static)
+    for (int n = NYS; n <= NYE; ++n)
+        for (int m = NXS; m <= NXE; ++m) {
+            if (g->lcu[IX(m, n)] > 0.5f) {
+                double fx_p = (u[IX(m, n)] * dyh[IX(m, n)] * hu[IX(m, n)] +
+                               u[IX(m + 1, n)] * dyh[IX(m + 1, n)] * hu[IX(m + 1, n)]) / 2.0 *
+                              (u[IX(m, n)] + u[IX(m + 1, n)]) / 2.0;
+                double fx_m = (u[IX(m, n)] * dyh[IX(m, n)] * hu[IX(m, n)] +
+                               u[IX(m - 1, n)] * dyh[IX(m - 1, n)] * hu[IX(m - 1, n)]) / 2.0 *
+                              (u[IX(m, n)] + u[IX(m - 1, n)]) / 2.0;
+                double fy_p = (v[IX(m, n)] * dxh[IX(m, n)] * hv[IX(m, n)] +
+                               v[IX(m + 1, n)] * dxh[IX(m + 1, n)] * hv[IX(m + 1, n)]) / 2.0 *
+                              (u[IX(m, n + 1)] + u[IX(m, n)]) / 2.0 * (double)g->luu[IX(m, n)];
+                double fy_m = (v[IX(m, n - 1)] * dxh[IX(m, n - 1)] * hv[IX(m, n - 1)] +
+                               v[IX(m + 1, n - 1)] * dxh[IX(m + 1, n - 1)] * hv[IX(m + 1, n - 1)]) / 2.0 *
+                              (u[IX(m, n - 1)] + u[IX(m, n)]) / 2.0 * (double)g->luu[IX(m, n - 1)];
+                rhsx[IX(m, n)] = -(fx_p - fx_m + fy_p - fy_m) +
+                                 (vort[IX(m, n)] * hh[IX(m, n)] * (v[IX(m + 1, n)] + v[IX(m, n)]) +
+                                  vort[IX(m, n - 1)] * hh[IX(m, n - 1)] *
+                                      (v[IX(m + 1, n - 1)] + v[IX(m, n - 1)])) / 4.0;
+            }
+            if (g->lcv[IX(m, n)] > 0.5f) {
+                double fy_p = (v[IX(m, n)] * dxh[IX(m, n)] * hv[IX(m, n)] +
+                               v[IX(m, n + 1)] * dxh[IX(m, n + 1)] * hv[IX(m, n + 1)]) / 2.0 *
+                              (v[IX(m, n)] + v[IX(m, n + 1)]) / 2.0;
+                double fy_m = (v[IX(m, n)] * dxh[IX(m, n)] * hv[IX(m, n)] +
+                               v[IX(m, n - 1)] * dxh[IX(m, n - 1)] * hv[IX(m, n - 1)]) / 2.0 *
+                              (v[IX(m, n)] + v[IX(m, n - 1)]) / 2.0;
+                double fx_p = (u[IX(m, n)] * dyh[IX(m, n)] * hu[IX(m, n)] +
+                               u[IX(m, n + 1)] * dyh[IX(m, n + 1)] * hu[IX(m, n + 1)]) / 2.0 *
+                              (v[IX(m + 1, n)] + v[IX(m, n)]) / 2.0;
+                double fx_m = (u[IX(m - 1, n)] * dyh[IX(m - 1, n)] * hu[IX(m - 1, n)] +
+                               u[IX(m - 1, n + 1)] * dyh[IX(m - 1, n + 1)] * hu[IX(m - 1, n + 1)]) / 2.0 *
+                              (v[IX(m - 1, n)] + v[IX(m, n)]) / 2.0;
+                rhsy[IX(m, n)] = -(fx_p - fx_m + fy_p - fy_m) -
+                                 (vort[IX(m, n)] * hh[IX(m, n)] * (u[IX(m, n + 1)] + u[IX(m, n)]) +
+                                  vort[IX(m - 1, n)] * hh[IX(m - 1, n)] *
+                                      (u[IX(m - 1, n + 1)] + u[IX(m - 1, n)])) / 4.0;
+            }
+        }
+}
+
+/* The viscous sweeps of uv_diff2/uv_diff4 share one stencil
+ * (Function/vel_ssh.f90:150-156, 166-172).  The tension part p and the
+ * shear part q are returned separately because the reference adds them
+ * to an existing RHS one after the other (:286-289, :302-305). */
+static inline void visc_x(const bto_grid *g, const double *mu, const double *str_t,
+                          const double *str_s, const double *hq, const double *hh,
+                          int m, int n, double *p, double *q)
+{
+    double muh_p = (mu[IX(m, n)] + mu[IX(m + 1, n)] + mu[IX(m, n + 1)] + mu[IX(m + 1, n + 1)]) / 4.0;
+    double muh_m = (mu[IX(m, n)] + mu[IX(m + 1, n)] + mu[IX(m, n - 1)] + mu[IX(m + 1, n - 1)]) / 4.0;
+    double dy1 = g->dy[IX(m + 1, n)], dy0 = g->dy[IX(m, n)];
+    double b1 = g->dxb[IX(m, n)], b0 = g->dxb[IX(m, n - 1)];
+    *p = (dy1 * dy1 * mu[IX(m + 1, n)] * hq[IX(m + 1, n)] * str_t[IX(m + 1, n)] -
+          dy0 * dy0 * mu[IX(m, n)] * hq[IX(m, n)] * str_t[IX(m, n)]) / g->dyh[IX(m, n)];
+    *q = (b1 * b1 * muh_p * hh[IX(m, n)] * str_s[IX(m, n)] -
+          b0 * b0 * muh_m * hh[IX(m, n - 1)] * str_s[IX(m, n - 1)]) / g->dxt[IX(m, n)];
+}
+
+/* p is the tension part the reference subtracts, q the shear part. */
+static inline void visc_y(const bto_grid *g, const double *mu, const double *str_t,
+                          const double *str_s, const double *hq, const double *hh,
+                          int m, int n, double *p, double *q)
+{
+    double muh_p = (mu[IX(m, n)] + mu[IX(m + 1, n)] + mu[IX(m, n + 1)] + mu[IX(m + 1, n + 1)]) / 4.0;
+    double muh_m = (mu[IX(m, n)] + mu[IX(m - 1, n)] + mu[IX(m, n + 1)] + mu[IX(m - 1, n + 1)]) / 4.0;
+    double dx1 = g->dx[IX(m, n + 1)], dx0 = g->dx[IX(m, n)];
+    double b1 = g->dyb[IX(m, n)], b0 = g->dyb[IX(m - 1, n)];
+    *p = (dx1 * dx1 * mu[IX(m, n + 1)] * hq[IX(m, n + 1)] * str_t[IX(m, n + 1)] -
+          dx0 * dx0 * mu[IX(m, n)] * hq[IX(m, n)] * str_t[IX(m, n)]) / g->dxh[IX(m, n)];
+    *q = (b1 * b1 * muh_p * hh[IX(m, n)] * str_s[IX(m, n)] -
+          b0 * b0 * muh_m * hh[IX(m - 1, n)] * str_s[IX(m - 1, n)]) / g->dyt[IX(m, n)];
+}
+
+/* Function/vel_ssh.f90:117-184 */
+void bto_uv_diff2(const bto_grid *g, const double *mu, const double *str_t,
+                  const double *str_s, const double *hq, const double *hu,
+                  const double *hv, const double *hh, double *rhsx,
+                  double *rhsy)
+{
+    (void)hu; (void)hv;
+#pragma omp parallel for schedule(static)
+    for (int n = NYS; n <= NYE; ++n)
+        for (int m = NXS; m <= NXE; ++m) {
+            double p, q;
+            if (g->lcu[IX(m, n)] > 0.5f) {
+                visc_x(g, mu, str_t, str_s, hq, hh, m, n, &p, &q);
+                rhsx[IX(m, n)] = p + q;
+            }
+            if (g->lcv[IX(m, n)] > 0.5f) {
+                visc_y(g, mu, str_t, str_s, hq, hh, m, n, &p, &q);
+                rhsy[IX(m, n)] = -p + q;
+            }
+        }
+}
+
+/* Function/vel_ssh.f90:187-317 */
+void bto_uv_diff4(const bto_grid *g, const double *mu, double *str_t,
+                  double *str_s, double *fx, double *fy, const double *hq,
+                  const double *hu, const double *hv, const double *hh,
+                  double *rhsx, double *rhsy)
+{
+    size_t sz = (size_t)g->nx * g->ny;
+    memset(fx, 0, sz * sizeof(double));
+    memset(fy, 0, sz * sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int n = NYS; n <= NYE; ++n)
+        for (int m = NXS; m <= NXE; ++m) {
+            double p, q;
+            if (g->lcu[IX(m, n)] > 0.5f) {
+                visc_x(g, mu, str_t, str_s, hq, hh, m, n, &p, &q);
+                double f = p + q;
+                fx[IX(m, n)] = -f / hu[IX(m, n)] / g->dxt[IX(m, n)] / g->dyh[IX(m, n)];
+            }
+            if (g->lcv[IX(m, n)] > 0.5f) {
+                visc_y(g, mu, str_t, str_s, hq, hh, m, n, &p, &q);
+                double f = -p + q;
+                fy[IX(m, n)] = -f / hv[IX(m, n)] / g->dxh[IX(m, n)] / g->dyt[IX(m, n)];
+            }
+        }
+    cyclize(g, fx);
+    cyclize(g, fy);
+    bto_stress_components(g, fx, fy, str_t, str_s);
+#pragma omp parallel for schedule(static)
+    for (int n = NYS; n <= NYE; ++n)
+        for (int m = NXS; m <= NXE; ++m) {
+            double p, q;
+            if (g->lcu[IX(m, n)] > 0.5f) {
+                visc_x(g, mu, str_t, str_s, hq, hh, m, n, &p, &q);
+                rhsx[IX(m, n)] = rhsx[IX(m, n)] + p + q;
+            }
+            if (g->lcv[IX(m, n)] > 0.5f) {
+                visc_y(g, mu, str_t, str_s, hq, hh, m, n, &p, &q);
+                rhsy[IX(m, n)] = rhsy[IX(m, n)] - p + q;
+            }
+        }
+}
+
+/* Function/vel_ssh.f90:379-679 */
+void bto_barotropic_dynamics(const bto_grid *g, bto_depths *d, double tau,
+                             int nstep, int ksw4, double *ubrtr_e,
+                             double *ubrtrp_e, double *vbrtr_e,
+                             double *vbrtrp_e, double *ssh_e, double *sshp_e,
+                             double *ubrtr_i, double *vbrtr_i,
+                             double *ssh4gradx, double *ssh4grady,
+                             const double *RHSx, const double *RHSy,
+                             const double *wflux, const double *mu,
+                             const double *mu4, double *vort, double *str_t2d,
+                             double *str_s2d, double *fx, double *fy,
+                             const double *rdis, double *RHSx_adv,
+                             double *RHSy_adv, double *RHSx_dif,
+                             double *RHSy_dif)
+{
+    const size_t sz = (size_t)g->nx * g->ny;
+    const size_t bytes = sz * sizeof(double);
+    double *u = malloc(bytes), *v = malloc(bytes), *ssh = malloc(bytes);
+    double *up = malloc(bytes), *vp = malloc(bytes), *sshp = malloc(bytes);
+    double *un = calloc(sz, sizeof(double)), *vn = calloc(sz, sizeof(double));
+    double *sshn = calloc(sz, sizeof(double));
+    double *hhq_e = d->h[0], *hhqn_e = d->h[2];
+    double *hhu_e = d->h[3], *hhup_e = d->h[4], *hhun_e = d->h[5];
+    double *hhv_e = d->h[6], *hhvp_e = d->h[7], *hhvn_e = d->h[8];
+    double *hhh_e = d->h[9], *hhhn_e = d->h[11];
+    const double tau_inner = tau / (double)nstep;
+    const double g_acc = g->free_fall_acc, rho0 = g->ref_den, ts = g->time_smooth;
+    const double ffs = (double)g->full_free_surface;
+    const double d4n = (double)(4 * nstep), dn = (double)nstep;
+
+    memcpy(up, ubrtrp_e, bytes);
+    memcpy(vp, vbrtrp_e, bytes);
+    memcpy(sshp, sshp_e, bytes);
+    memcpy(u, ubrtr_e, bytes);
+    memcpy(v, vbrtr_e, bytes);
+    memcpy(ssh, ssh_e, bytes);
+    memset(ubrtr_i, 0, bytes);
+    memset(vbrtr_i, 0, bytes);
+    memset(ssh4gradx, 0, bytes);
+    memset(ssh4grady, 0, bytes);
+
+    for (int step = 1; step <= 2 * nstep; ++step) {
+        /* continuity, :487-499 */
+#pragma omp parallel for schedule(static)
+        for (int n = NYS; n <= NYE; ++n)
+            for (int m = NXS; m <= NXE; ++m)
+                if (g->lu[IX(m, n)] > 0.5f)
+                    sshn[IX(m, n)] =
+                        sshp[IX(m, n)] +
+                        2.0 * tau_inner *
+                            (wflux[IX(m, n)] / rho0 * ffs -
+                             (u[IX(m, n)] * hhu_e[IX(m, n)] * g->dyh[IX(m, n)] -
+                              u[IX(m - 1, n)] * hhu_e[IX(m - 1, n)] * g->dyh[IX(m - 1, n)] +
+                              v[IX(m, n)] * hhv_e[IX(m, n)] * g->dxh[IX(m, n)] -
+                              v[IX(m, n - 1)] * hhv_e[IX(m, n - 1)] * g->dxh[IX(m, n - 1)]) /
+                                 (g->dx[IX(m, n)] * g->dy[IX(m, n)]));
+        cyclize(g, sshn);
+
+        if (g->full_free_surface > 0) bto_hh_update(g, hhqn_e, hhun_e, hhvn_e, hhhn_e, sshn);
+
+        bto_stress_components(g, up, vp, str_t2d, str_s2d);
+        bto_uv_trans(g, u, v, vort, hhq_e, hhu_e, hhv_e, hhh_e, RHSx_adv, RHSy_adv);
+        bto_uv_diff2(g, mu, str_t2d, str_s2d, hhq_e, hhu_e, hhv_e, hhh_e, RHSx_dif, RHSy_dif);
+        if (ksw4 > 0)
+            bto_uv_diff4(g, mu4, str_t2d, str_s2d, fx, fy, hhq_e, hhu_e, hhv_e, hhh_e,
+                         RHSx_dif, RHSy_dif);
+
+        /* momentum, :531-573 */
+#pragma omp parallel for schedule(static)
+        for (int n = NYS; n <= NYE; ++n)
+            for (int m = NXS; m <= NXE; ++m) {
+                size_t i = IX(m, n);
+                if (g->lcu[i] > 0.5f) {
+                    double bp = hhun_e[i] * g->dxt[i] * g->dyh[i] / 2.0 / tau_inner;
+                    double bp0 = hhup_e[i] * g->dxt[i] * g->dyh[i] / 2.0 / tau_inner;
+                    double slx = -g_acc * (ssh[IX(m + 1, n)] - ssh[i]) * g->dyh[i] * hhu_e[i];
+                    double slxn = -g_acc * (sshn[IX(m + 1, n)] - sshn[i]) * g->dyh[i] * hhun_e[i];
+                    double grx = RHSx[i] + slx + RHSx_dif[i] + RHSx_adv[i] -
+                                 (rdis[i] + rdis[IX(m + 1, n)]) / 2.0 * up[i] * g->dxt[i] *
+                                     g->dyh[i] * hhu_e[i] +
+                                 (g->rlh_s[i] * hhh_e[i] * g->dxb[i] * g->dyb[i] *
+                                      (v[IX(m + 1, n)] + v[i]) +
+                                  g->rlh_s[IX(m, n - 1)] * hhh_e[IX(m, n - 1)] * g->dxb[IX(m, n - 1)] *
+                                      g->dyb[IX(m, n - 1)] * (v[IX(m + 1, n - 1)] + v[IX(m, n - 1)])) /
+                                     4.0;
+                    un[i] = (up[i] * bp0 + grx) / bp;
+                    ubrtr_i[i] = ubrtr_i[i] + (u[i] * hhu_e[i] + un[i] * hhun_e[i]) / d4n;
+                    ssh4gradx[i] = ssh4gradx[i] + (slx + slxn) / d4n;
+                }
+                if (g->lcv[i] > 0.5f) {
+                    double bp = hhvn_e[i] * g->dyt[i] * g->dxh[i] / 2.0 / tau_inner;
+                    double bp0 = hhvp_e[i] * g->dyt[i] * g->dxh[i] / 2.0 / tau_inner;
+                    double sly = -g_acc * (ssh[IX(m, n + 1)] - ssh[i]) * g->dxh[i] * hhv_e[i];
+                    double slyn = -g_acc * (sshn[IX(m, n + 1)] - sshn[i]) * g->dxh[i] * hhvn_e[i];
+                    double gry = RHSy[i] + sly + RHSy_dif[i] + RHSy_adv[i] -
+                                 (rdis[i] + rdis[IX(m, n + 1)]) / 2.0 * vp[i] * g->dxh[i] *
+                                     g->dyt[i] * hhv_e[i] -
+                                 (g->rlh_s[i] * hhh_e[i] * g->dxb[i] * g->dyb[i] *
+                                      (u[IX(m, n + 1)] + u[i]) +
+                                  g->rlh_s[IX(m - 1, n)] * hhh_e[IX(m - 1, n)] * g->dxb[IX(m - 1, n)] *
+                                      g->dyb[IX(m - 1, n)] * (u[IX(m - 1, n + 1)] + u[IX(m - 1, n)])) /
+                                     4.0;
+                    vn[i] = (vp[i] * bp0 + gry) / bp;
+                    vbrtr_i[i] = vbrtr_i[i] + (v[i] * hhv_e[i] + vn[i] * hhvn_e[i]) / d4n;
+                    ssh4grady[i] = ssh4grady[i] + (sly + slyn) / d4n;
+                }
+            }
+        cyclize(g, un);
+        cyclize(g, vn);
+
+        /* time-level shift with the Robert-Asselin filter, :588-612 */
+#pragma omp parallel for schedule(static)
+        for (int n = NYS - 1; n <= NYE + 1; ++n)
+            for (int m = NXS - 1; m <= NXE + 1; ++m) {
+                size_t i = IX(m, n);
+                if (g->lu[i] > 0.5f) {
+                    sshp[i] = ssh[i] + ts * (sshn[i] - 2.0 * ssh[i] + sshp[i]) / 2.0 / dn;
+                    ssh[i] = sshn[i];
+                }
+                if (g->lcu[i] > 0.5f) {
+                    up[i] = u[i] + ts * (un[i] - 2.0 * u[i] + up[i]) / 2.0 / dn;
+                    u[i] = un[i];
+                }
+                if (g->lcv[i] > 0.5f) {
+                    vp[i] = v[i] + ts * (vn[i] - 2.0 * v[i] + vp[i]) / 2.0 / dn;
+                    v[i] = vn[i];
+                }
+            }
+
+        if (g->full_free_surface > 0) bto_hh_shift(g, d, nstep);
+
+        if (step == nstep) {
+            memcpy(ubrtrp_e, up, bytes);
+            memcpy(vbrtrp_e, vp, bytes);
+            memcpy(sshp_e, sshp, bytes);
+            memcpy(ubrtr_e, u, bytes);
+            memcpy(vbrtr_e, v, bytes);
+            memcpy(ssh_e, ssh, bytes);
+        }
+    }
+    cyclize(g, ubrtr_i);
+    cyclize(g, vbrtr_i);
+    if (g->full_free_surface > 0) bto_hh_init(g, d, ssh_e, sshp_e);
+    (void)hhq_e;
+    free(sshn); free(vn); free(un); free(sshp); free(vp); free(up);
+    free(ssh); free(v); free(u);
+}
+
+/* Function/fluxes.f90:29-37 */
+void bto_surface_stress(const bto_grid *g, const double *taux,
+                        const double *tauy, double *surf_x, double *surf_y)
+{
+    const double rho0 = g->ref_den;
+#pragma omp parallel for schedule(static)
+    for (int n = NYS; n <= NYE; ++n)
+        for (int m = NXS; m <= NXE; ++m) {
+            size_t i = IX(m, n);
+            if (g->lcu[i] > 0.5f)
+                surf_x[i] = (taux[i] * g->dx[i] * g->dy[i] +
+                             taux[IX(m + 1, n)] * g->dx[IX(m + 1, n)] * g->dy[IX(m + 1, n)]) /
+                            rho0 / 2.0 / g->dxt[i] / g->dyh[i];
+            if (g->lcv[i] > 0.5f)
+                surf_y[i] = (tauy[i] * g->dx[i] * g->dy[i] +
+                             tauy[IX(m, n + 1)] * g->dx[IX(m, n + 1)] * g->dy[IX(m, n + 1)]) /
+                            rho0 / 2.0 / g->dxh[i] / g->dyt[i];
+        }
+}
+
+/* Function/fluxes.f90:226-272 (type_fric = 2) */
+void bto_bottom_stress(const bto_grid *g, double cb_nl, double ebottom,
+                       const double *uu, const double *vv, const double *uup,
+                       const double *vvp, const double *hhu, const double *hhv,
+                       const double *hhq, double *bf, double *bot_x,
+                       double *bot_y)
+{
+#pragma omp parallel for schedule(static)
+    for (int n = NYS; n <= NYE; ++n)
+        for (int m = NXS; m <= NXE; ++m) {
+            size_t i = IX(m, n);
+            if (g->lu[i] > 0.5f) {
+                double ut = (uu[i] * g->dxt[i] * g->dyh[i] * hhu[i] +
+                             uu[IX(m - 1, n)] * g->dxt[IX(m - 1, n)] * g->dyh[IX(m - 1, n)] *
+                                 hhu[IX(m - 1, n)]) /
+                            2.0 / g->dx[i] / g->dy[i] / hhq[i];
+                double vt = (vv[i] * g->dxh[i] * g->dyt[i] * hhv[i] +
+                             vv[IX(m, n - 1)] * g->dxh[IX(m, n - 1)] * g->dyt[IX(m, n - 1)] *
+                                 hhv[IX(m, n - 1)]) /
+                            2.0 / g->dx[i] / g->dy[i] / hhq[i];
+                bf[i] = cb_nl * sqrt(ut * ut + vt * vt + ebottom);
+            }
+        }
+    cyclize(g, bf);
+#pragma omp parallel for schedule(static)
+    for (int n = NYS; n <= NYE; ++n)
+        for (int m = NXS; m <= NXE; ++m) {
+            size_t i = IX(m, n);
+            if (g->lcu[i] > 0.5f) bot_x[i] = -(bf[i] + bf[IX(m + 1, n)]) / 2.0 * uup[i];
+            if (g->lcv[i] > 0.5f) bot_y[i] = -(bf[i] + bf[IX(m, n + 1)]) / 2.0 * vvp[i];
+        }
+}
+
+/* Control/dyncall.f90:110-130 for a 2-D flow (all 3-D terms zero). */
+void bto_rhs_assembly(const bto_grid *g, const double *surf_x,
+                      const double *surf_y, const double *bot_x,
+                      const double *bot_y, double *rhsx2d, double *rhsy2d)
+{
+    size_t sz = (size_t)g->nx * g->ny;
+    /* depth_ave of a zero 3-D RHS leaves zero on llu/llv (dyncall.f90:103-104) */
+    for (size_t i = 0; i < sz; ++i) {
+        if (g->llu[i] > 0.5f) rhsx2d[i] = 0.0;
+        if (g->llv[i] > 0.5f) rhsy2d[i] = 0.0;
+    }
+#pragma omp parallel for schedule(static)
+    for (int n = NYS; n <= NYE; ++n)
+        for (int m = NXS; m <= NXE; ++m) {
+            size_t i = IX(m, n);
+            if (g->lcu[i] > 0.5f)
+                rhsx2d[i] = rhsx2d[i] + (surf_x[i] + bot_x[i]) * g->dxt[i] * g->dyh[i];
+            if (g->lcv[i] > 0.5f)
+                rhsy2d[i] = rhsy2d[i] + (surf_y[i] + bot_y[i]) * g->dyt[i] * g->dxh[i];
+        }
+}
+
+void bto_model_step(const bto_grid *g, bto_depths *d, double tau, int nstep,
+                    int ksw4, double cb_nl, double ebottom, double *ubrtr_e,
+                    double *ubrtrp_e, double *vbrtr_e, double *vbrtrp_e,
+                    double *ssh_e, double *sshp_e, double *ubrtr_i,
+                    double *vbrtr_i, double *ssh4gradx, double *ssh4grady,
+                    const double *taux, const double *tauy,
+                    const double *wflux, const double *mu, const double *mu4,
+                    const double *rdis, double *work)
+{
+    const size_t sz = (size_t)g->nx * g->ny;
+    double *surf_x = work, *surf_y = work + sz, *bot_x = work + 2 * sz;
+    double *bot_y = work + 3 * sz, *bf = work + 4 * sz, *rhsx = work + 5 * sz;
+    double *rhsy = work + 6 * sz, *vort = work + 7 * sz, *str_t = work + 8 * sz;
+    double *str_s = work + 9 * sz, *fx = work + 10 * sz, *fy = work + 11 * sz;
+    double *radv = work + 12 * sz, *rady = work + 13 * sz, *rdfx = work + 14 * sz;
+    double *rdfy = work + 15 * sz;
+    bto_surface_stress(g, taux, tauy, surf_x, surf_y);
+    /* the drag sees the external-mode velocities and depths of this step */
+    bto_bottom_stress(g, cb_nl, ebottom, ubrtr_e, vbrtr_e, ubrtrp_e, vbrtrp_e,
+                      d->h[3], d->h[6], d->h[0], bf, bot_x, bot_y);
+    bto_rhs_assembly(g, surf_x, surf_y, bot_x, bot_y, rhsx, rhsy);
+    bto_barotropic_dynamics(g, d, tau, nstep, ksw4, ubrtr_e, ubrtrp_e, vbrtr_e,
+                            vbrtrp_e, ssh_e, sshp_e, ubrtr_i, vbrtr_i,
+                            ssh4gradx, ssh4grady, rhsx, rhsy, wflux, mu, mu4,
+                            vort, str_t, str_s, fx, fy, rdis, radv, rady, rdfx,
+                            rdfy);
+}
+
+double bto_volume(const bto_grid *g, const double *ssh)
+{
+    double total = 0.0;
+    for (int n = NYS; n <= NYE; ++n) {
+        double row = 0.0;
+        for (int m = NXS; m <= NXE; ++m)
+            if (g->lu[IX(m, n)] > 0.5f)
+                row += ssh[IX(m, n)] * g->dx[IX(m, n)] * g->dy[IX(m, n)];
+        total += row;
+    }
+    return total;
+}

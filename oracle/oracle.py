@@ -1,0 +1,228 @@
+"""ctypes front-end of the C restatement in bt_oracle.c.
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP implementation.  Only
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+
+Arrays follow inmsom_bt.cases: float64 (ny, nx) C order == the
+reference's column-major (m, n).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_HARNESS = os.path.join(HERE, "_ref", "ref_harness")
+
+MASKS = ("lu", "luu", "luh", "lcu", "lcv", "llu", "llv")
+METRICS = ("dx", "dy", "dxt", "dyt", "dxh", "dyh", "dxb", "dyb")
+DEPTHS = ("hhq_e", "hhqp_e", "hhqn_e", "hhu_e", "hhup_e", "hhun_e",
+          "hhv_e", "hhvp_e", "hhvn_e", "hhh_e", "hhhp_e", "hhhn_e")
+OUT_FIELDS = ("ssh", "sshp", "u", "up", "v", "vp", "ubrtr_i", "vbrtr_i",
+              "ssh4gradx", "ssh4grady")
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+
+
+class _Grid(C.Structure):
+    _fields_ = ([("nx", C.c_int), ("ny", C.c_int), ("periodicity_x", C.c_int),
+                 ("periodicity_y", C.c_int), ("full_free_surface", C.c_int),
+                 ("free_fall_acc", C.c_double), ("ref_den", C.c_double),
+                 ("time_smooth", C.c_double)]
+                + [(k, _fp) for k in MASKS]
+                + [(k, _dp) for k in METRICS]
+                + [("rlh_s", _dp), ("hhq_rest", _dp)])
+
+
+class _Depths(C.Structure):
+    _fields_ = [("h", _dp * 12)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+        _lib = C.CDLL(LIB_PATH)
+        _lib.bto_volume.restype = C.c_double
+    return _lib
+
+
+def _d(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _f(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_fp)
+
+
+def gridcon(lbasins: np.ndarray, px: int = 0, py: int = 0) -> Dict[str, np.ndarray]:
+    """Service/grid_construction.f90:33-112."""
+    ny, nx = lbasins.shape
+    lb = np.ascontiguousarray(lbasins, dtype=np.int32)
+    out = {k: np.zeros((ny, nx), dtype=np.float32) for k in MASKS}
+    lib().bto_gridcon(nx, ny, px, py, lb.ctypes.data_as(_ip), *[_f(out[k]) for k in MASKS])
+    return out
+
+
+class OracleGrid:
+    """Static fields of one basin, held for the C oracle."""
+
+    def __init__(self, b, periodicity=(0, 0), full_free_surface=1):
+        self.b = b
+        self.masks = gridcon(b.lbasins, *periodicity)
+        self.keep = {k: np.ascontiguousarray(getattr(b, k), dtype=np.float64)
+                     for k in METRICS + ("rlh_s", "hhq_rest")}
+        g = _Grid()
+        g.nx, g.ny = b.nx, b.ny
+        g.periodicity_x, g.periodicity_y = periodicity
+        g.full_free_surface = full_free_surface
+        g.free_fall_acc, g.ref_den, g.time_smooth = b.g, b.rho0, b.time_smooth
+        for k in MASKS:
+            setattr(g, k, _f(self.masks[k]))
+        for k, a in self.keep.items():
+            setattr(g, k, _d(a))
+        self.g = g
+        self.ref = C.byref(g)
+
+    def rayleigh(self, r_fric: Optional[float] = None) -> np.ndarray:
+        out = np.zeros((self.b.ny, self.b.nx))
+        lib().bto_rayleigh_init(self.ref, C.c_double(self.b.r_fric if r_fric is None else r_fric),
+                                _d(out))
+        return out
+
+    def volume(self, ssh: np.ndarray) -> float:
+        return float(lib().bto_volume(self.ref, _d(np.ascontiguousarray(ssh))))
+
+    def new_depths(self):
+        arrs = [np.zeros((self.b.ny, self.b.nx)) for _ in range(12)]
+        d = _Depths()
+        for i, a in enumerate(arrs):
+            d.h[i] = _d(a)
+        return d, arrs
+
+
+def run(b, nsteps: int, nstep: int = 1, ksw4: int = 0, out_steps: Sequence[int] = (),
+        rdis: Optional[np.ndarray] = None, periodicity=(0, 0), grid: Optional[OracleGrid] = None):
+    """Time loop of bto_model_step; returns {step: {field: array}} for the
+    steps in out_steps (fields: OUT_FIELDS + DEPTHS)."""
+    L = lib()
+    og = grid or OracleGrid(b, periodicity)
+    if rdis is None:
+        rdis = og.rayleigh()
+    st = {k: np.ascontiguousarray(getattr(b, k), dtype=np.float64).copy()
+          for k in ("ssh", "sshp", "u", "up", "v", "vp")}
+    acc = {k: np.zeros((b.ny, b.nx)) for k in ("ubrtr_i", "vbrtr_i", "ssh4gradx", "ssh4grady")}
+    dstruct, darr = og.new_depths()
+    L.bto_hh_init(og.ref, C.byref(dstruct), _d(st["ssh"]), _d(st["sshp"]))
+    work = np.zeros(16 * b.nx * b.ny)
+    forc = [np.ascontiguousarray(getattr(b, k), dtype=np.float64)
+            for k in ("taux", "tauy", "wflux", "mu", "mu4")]
+    rd = np.ascontiguousarray(rdis, dtype=np.float64)
+    outs = {}
+    want = set(out_steps)
+    for step in range(1, nsteps + 1):
+        L.bto_model_step(og.ref, C.byref(dstruct), C.c_double(b.dt), nstep, ksw4,
+                         C.c_double(b.cb_nl), C.c_double(b.ebottom),
+                         _d(st["u"]), _d(st["up"]), _d(st["v"]), _d(st["vp"]),
+                         _d(st["ssh"]), _d(st["sshp"]),
+                         _d(acc["ubrtr_i"]), _d(acc["vbrtr_i"]), _d(acc["ssh4gradx"]),
+                         _d(acc["ssh4grady"]), *[_d(a) for a in forc], _d(rd), _d(work))
+        if step in want:
+            o = {k: v.copy() for k, v in st.items()}
+            o.update({k: v.copy() for k, v in acc.items()})
+            o.update({k: a.copy() for k, a in zip(DEPTHS, darr)})
+            outs[step] = o
+    return outs
+
+
+def units(b, nstep: int = 1, rdis: Optional[np.ndarray] = None) -> Dict[str, np.ndarray]:
+    """The helper-routine sequence of the harness's mode 1."""
+    L = lib()
+    og = OracleGrid(b)
+    nx, ny = b.nx, b.ny
+    z = lambda: np.zeros((ny, nx))
+    ssh = np.ascontiguousarray(b.ssh).copy()
+    sshp = np.ascontiguousarray(b.sshp).copy()
+    u, up = np.ascontiguousarray(b.u).copy(), np.ascontiguousarray(b.up).copy()
+    v, vp = np.ascontiguousarray(b.v).copy(), np.ascontiguousarray(b.vp).copy()
+    ds, da = og.new_depths()
+    out = {}
+    L.bto_hh_init(og.ref, C.byref(ds), _d(ssh), _d(sshp))
+    out.update({"init_" + k: a.copy() for k, a in zip(DEPTHS, da)})
+    L.bto_hh_update(og.ref, _d(da[2]), _d(da[5]), _d(da[8]), _d(da[11]), _d(u))
+    for k, i in (("hhqn_e", 2), ("hhun_e", 5), ("hhvn_e", 8), ("hhhn_e", 11)):
+        out["update_" + k] = da[i].copy()
+    L.bto_hh_shift(og.ref, C.byref(ds), nstep)
+    out.update({"shift_" + k: a.copy() for k, a in zip(DEPTHS, da)})
+    strt, strs = z(), z()
+    L.bto_stress_components(og.ref, _d(up), _d(vp), _d(strt), _d(strs))
+    out["str_t"], out["str_s"] = strt.copy(), strs.copy()
+    vort, radv, rady = z(), z(), z()
+    L.bto_uv_trans(og.ref, _d(u), _d(v), _d(vort), _d(da[0]), _d(da[3]), _d(da[6]), _d(da[9]),
+                   _d(radv), _d(rady))
+    out["vort"], out["rhsx_adv"], out["rhsy_adv"] = vort, radv, rady
+    rdfx, rdfy = z(), z()
+    mu = np.ascontiguousarray(b.mu)
+    L.bto_uv_diff2(og.ref, _d(mu), _d(strt), _d(strs), _d(da[0]), _d(da[3]), _d(da[6]),
+                   _d(da[9]), _d(rdfx), _d(rdfy))
+    out["rhsx_dif2"], out["rhsy_dif2"] = rdfx.copy(), rdfy.copy()
+    fx, fy = z(), z()
+    mu4 = np.ascontiguousarray(b.mu4)
+    L.bto_uv_diff4(og.ref, _d(mu4), _d(strt), _d(strs), _d(fx), _d(fy), _d(da[0]), _d(da[3]),
+                   _d(da[6]), _d(da[9]), _d(rdfx), _d(rdfy))
+    out.update({"d4_fx": fx, "d4_fy": fy, "d4_str_t": strt.copy(), "d4_str_s": strs.copy(),
+                "rhsx_dif4": rdfx.copy(), "rhsy_dif4": rdfy.copy()})
+    sx, sy, bf, bx, by = z(), z(), z(), z(), z()
+    taux, tauy = np.ascontiguousarray(b.taux), np.ascontiguousarray(b.tauy)
+    L.bto_surface_stress(og.ref, _d(taux), _d(tauy), _d(sx), _d(sy))
+    L.bto_bottom_stress(og.ref, C.c_double(b.cb_nl), C.c_double(b.ebottom), _d(u), _d(v),
+                        _d(up), _d(vp), _d(da[3]), _d(da[6]), _d(da[0]), _d(bf), _d(bx), _d(by))
+    out.update({"surf_x": sx, "surf_y": sy, "bottom_friction": bf, "bot_x": bx, "bot_y": by})
+    return out
+
+
+# ------------------------------------------------------- reference harness
+UNIT_LAYOUT = ([("init_" + k) for k in DEPTHS]
+               + ["update_hhqn_e", "update_hhun_e", "update_hhvn_e", "update_hhhn_e"]
+               + [("shift_" + k) for k in DEPTHS]
+               + ["str_t", "str_s", "vort", "rhsx_adv", "rhsy_adv", "rhsx_dif2", "rhsy_dif2",
+                  "d4_fx", "d4_fy", "d4_str_t", "d4_str_s", "rhsx_dif4", "rhsy_dif4",
+                  "surf_x", "surf_y", "bottom_friction", "bot_x", "bot_y"])
+
+
+def have_reference() -> bool:
+    return os.path.exists(REF_HARNESS)
+
+
+def run_reference(case_path: str, out_path: str, nx: int, ny: int, mode: int,
+                  out_steps: Sequence[int] = (), threads: int = 4) -> dict:
+    """Run the compiled reference harness on a case file and parse its output."""
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    subprocess.run([REF_HARNESS, case_path, out_path], check=True, env=env,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, cwd=os.path.dirname(out_path))
+    raw = np.fromfile(out_path, dtype="<f8")
+    per = nx * ny
+    arrs = [raw[i * per:(i + 1) * per].reshape(ny, nx) for i in range(raw.size // per)]
+    res = {"masks": dict(zip(MASKS, arrs[:7]))}
+    arrs = arrs[7:]
+    if mode == 0:
+        nf = len(OUT_FIELDS) + len(DEPTHS)
+        for k, step in enumerate(out_steps):
+            blk = arrs[k * nf:(k + 1) * nf]
+            res[step] = dict(zip(OUT_FIELDS + DEPTHS, blk))
+    else:
+        res["units"] = dict(zip(UNIT_LAYOUT, arrs))
+    return res

@@ -1,0 +1,188 @@
+! Test harness that drives the REFERENCE Fortran hot path on synthetic
+! inputs.  TEST INFRASTRUCTURE ONLY: it links the reference's unmodified
+! sources (built by oracle/Makefile into oracle/_ref/) and writes the
+! vectors that pin the C restatement in oracle/bt_oracle.c.
+!
+! usage: ref_harness <case.bin> <out.bin>
+!
+! case.bin (little-endian stream, written by inmsom_bt.cases.write_case):
+!   int32 magic ('BTC1'), version, nx, ny, nsteps, nstep, ksw4, mode, nout
+!   int32 out_steps(nout)
+!   real8 tau, cb_nl, ebottom
+!   int32 lbasins(nx,ny)
+!   real8 hhq_rest, dx, dy, dxt, dyt, dxh, dyh, dxb, dyb, rlh_s, taux,
+!         tauy, wflux, mu, mu4, rdis, ssh, sshp, u, up, v, vp   (nx,ny each)
+!
+! The output starts with the 7 masks lu, luu, luh, lcu, lcv, llu, llv
+! (as real8 (nx,ny)) built by the reference's gridcon.
+! mode 0: time loop.  Each step is the 2-D path of Control/dyncall.f90:
+!   sea_surface_fluxes (ksw_ssbc=1) and sea_bottom_fluxes on the
+!   external-mode state, RHS assembly (:114-126 with every 3-D term
+!   zero), then barotropic_dynamics(tau, nstep, ksw4).  After each step
+!   listed in out_steps it writes, as real8 (nx,ny): ssh_e, sshp_e, ubrtr_e, ubrtrp_e, vbrtr_e, vbrtrp_e,
+!   ubrtr_i, vbrtr_i, ssh4gradx, ssh4grady and the 12 hh*_e arrays.
+! mode 1: one call of each helper routine on the case's state arrays
+!   (see the write sequence at the end of this file).
+program ref_harness
+use main_basin_pars
+use mpi_parallel_tools
+use basin_grid
+use ocean_variables
+implicit none
+
+character(512) :: case_path, out_path, mask_path
+integer :: ierr, magic, version, cnx, cny, nsteps, nstep, ksw4, mode, nout
+integer :: step, k, m, n, io
+integer, allocatable :: out_steps(:), lb(:,:)
+real(8) :: tau, cb_nl_in, ebottom_in
+real(8), allocatable :: wflux(:,:), mu(:,:), mu4(:,:), rdis(:,:)
+real(8), allocatable :: ssh4gx(:,:), ssh4gy(:,:), vort(:,:), strt(:,:), strs(:,:)
+real(8), allocatable :: fxa(:,:), fya(:,:), radv(:,:), rady(:,:), rdfx(:,:), rdfy(:,:)
+real(8), allocatable :: ax(:,:), ay(:,:)
+
+call get_command_argument(1, case_path)
+call get_command_argument(2, out_path)
+
+call mpi_array_boundary_definition
+call model_grid_allocate
+call ocean_variables_allocate
+call oceanbc_arrays_allocate
+
+open(21, file=trim(case_path), access='stream', form='unformatted', status='old', iostat=io)
+if (io /= 0) then
+  print *, 'cannot open case file'
+  call mpi_abort(cart_comm, 5, ierr)
+endif
+read(21) magic, version, cnx, cny, nsteps, nstep, ksw4, mode, nout
+if (cnx /= nx .or. cny /= ny) then
+  print *, 'case dims', cnx, cny, 'differ from the compiled nx, ny', nx, ny
+  call mpi_abort(cart_comm, 6, ierr)
+endif
+allocate(out_steps(max(nout,1)), lb(nx,ny))
+if (nout > 0) read(21) out_steps(1:nout)
+read(21) tau, cb_nl_in, ebottom_in
+read(21) lb
+
+allocate(wflux(nx,ny), mu(nx,ny), mu4(nx,ny), rdis(nx,ny), ssh4gx(nx,ny), ssh4gy(nx,ny))
+allocate(vort(nx,ny), strt(nx,ny), strs(nx,ny), fxa(nx,ny), fya(nx,ny))
+allocate(radv(nx,ny), rady(nx,ny), rdfx(nx,ny), rdfy(nx,ny), ax(nx,ny), ay(nx,ny))
+vort = 0.0d0; strt = 0.0d0; strs = 0.0d0; fxa = 0.0d0; fya = 0.0d0
+radv = 0.0d0; rady = 0.0d0; rdfx = 0.0d0; rdfy = 0.0d0; ssh4gx = 0.0d0; ssh4gy = 0.0d0
+
+read(21) hhq_rest, dx, dy, dxt, dyt, dxh, dyh, dxb, dyb, rlh_s, taux, tauy, &
+         wflux, mu, mu4, rdis, ssh_e, sshp_e, ubrtr_e, ubrtrp_e, vbrtr_e, vbrtrp_e
+close(21)
+
+! masks through the reference's own gridcon, from a mask file in its format
+mask_path = trim(out_path)//'.mask.txt'
+open(22, file=trim(mask_path), status='replace')
+write(22, '(a)') 'synthetic mask (0 - ocean, 1 - land)'
+do n = ny, 1, -1
+  write(22, '(*(i1))') (lb(m,n), m = 1, nx)
+enddo
+close(22)
+call gridcon(trim(mask_path))
+
+open(23, file=trim(out_path), access='stream', form='unformatted', status='replace')
+call write_masks()
+
+if (mode == 0) then
+  ksw_ssbc = 1
+  ksw_lat4 = ksw4
+  call hh_init(hhq_e, hhqp_e, hhqn_e, hhu_e, hhup_e, hhun_e, hhv_e, hhvp_e, hhvn_e, &
+               hhh_e, hhhp_e, hhhn_e, ssh_e, sshp_e, hhq_rest)
+  r_diss = rdis
+  k = 1
+  do step = 1, nsteps
+    ! the 2-D flow is its own bottom layer; depths are the external-mode ones
+    uu(:,:,nz)  = ubrtr_e
+    vv(:,:,nz)  = vbrtr_e
+    uup(:,:,nz) = ubrtrp_e
+    vvp(:,:,nz) = vbrtrp_e
+    hhu = hhu_e
+    hhv = hhv_e
+    hhq = hhq_e
+    call sea_surface_fluxes
+    call sea_bottom_fluxes
+    RHSx2d = 0.0d0
+    RHSy2d = 0.0d0
+    do n = ny_start, ny_end
+      do m = nx_start, nx_end
+        if (lcu(m,n) > 0.5) then
+          RHSx2d(m,n) = RHSx2d(m,n) + (surf_stress_x(m,n) + bot_stress_x(m,n))*dxt(m,n)*dyh(m,n)
+        endif
+        if (lcv(m,n) > 0.5) then
+          RHSy2d(m,n) = RHSy2d(m,n) + (surf_stress_y(m,n) + bot_stress_y(m,n))*dyt(m,n)*dxh(m,n)
+        endif
+      enddo
+    enddo
+    call barotropic_dynamics(tau, nstep, ksw4, ubrtr_e, ubrtrp_e, vbrtr_e, vbrtrp_e, &
+                             ssh_e, sshp_e, ubrtr_i, vbrtr_i, ssh4gx, ssh4gy,       &
+                             RHSx2d, RHSy2d, wflux, mu, mu4, vort, strt, strs,      &
+                             fxa, fya, r_diss, radv, rady, rdfx, rdfy)
+    if (k <= nout) then
+      if (out_steps(k) == step) then
+        write(23) ssh_e, sshp_e, ubrtr_e, ubrtrp_e, vbrtr_e, vbrtrp_e, &
+                  ubrtr_i, vbrtr_i, ssh4gx, ssh4gy
+        write(23) hhq_e, hhqp_e, hhqn_e, hhu_e, hhup_e, hhun_e, hhv_e, hhvp_e, hhvn_e, &
+                  hhh_e, hhhp_e, hhhn_e
+        k = k + 1
+      endif
+    endif
+  enddo
+else
+  ! helper routines, one call each, on the case's state arrays
+  ! hh_init from (ssh_e, sshp_e)
+  call hh_init(hhq_e, hhqp_e, hhqn_e, hhu_e, hhup_e, hhun_e, hhv_e, hhvp_e, hhvn_e, &
+               hhh_e, hhhp_e, hhhn_e, ssh_e, sshp_e, hhq_rest)
+  write(23) hhq_e, hhqp_e, hhqn_e, hhu_e, hhup_e, hhun_e, hhv_e, hhvp_e, hhvn_e, &
+            hhh_e, hhhp_e, hhhn_e
+  ! hh_update from ubrtr_e (used as a surface height field)
+  call hh_update(hhqn_e, hhun_e, hhvn_e, hhhn_e, ubrtr_e, hhq_rest)
+  write(23) hhqn_e, hhun_e, hhvn_e, hhhn_e
+  ! hh_shift with nstep
+  call hh_shift(hhq_e, hhqp_e, hhqn_e, hhu_e, hhup_e, hhun_e, hhv_e, hhvp_e, hhvn_e, &
+                hhh_e, hhhp_e, hhhn_e, nstep)
+  write(23) hhq_e, hhqp_e, hhqn_e, hhu_e, hhup_e, hhun_e, hhv_e, hhvp_e, hhvn_e, &
+            hhh_e, hhhp_e, hhhn_e
+  ! stress_components(ubrtrp_e, vbrtrp_e)
+  call stress_components(ubrtrp_e, vbrtrp_e, strt, strs, 1)
+  write(23) strt, strs
+  ! uv_trans(ubrtr_e, vbrtr_e) with the current depths
+  call uv_trans(ubrtr_e, vbrtr_e, vort, hhq_e, hhu_e, hhv_e, hhh_e, radv, rady, 1)
+  write(23) vort, radv, rady
+  ! uv_diff2(mu) on the stresses above
+  call uv_diff2(mu, strt, strs, hhq_e, hhu_e, hhv_e, hhh_e, rdfx, rdfy, 1)
+  write(23) rdfx, rdfy
+  ! uv_diff4(mu4) accumulated onto the uv_diff2 result
+  call uv_diff4(mu4, strt, strs, fxa, fya, hhq_e, hhu_e, hhv_e, hhh_e, rdfx, rdfy, 1)
+  write(23) fxa, fya, strt, strs, rdfx, rdfy
+  ! wind stress and quadratic drag
+  ksw_ssbc = 1
+  uu(:,:,nz)  = ubrtr_e
+  vv(:,:,nz)  = vbrtr_e
+  uup(:,:,nz) = ubrtrp_e
+  vvp(:,:,nz) = vbrtrp_e
+  hhu = hhu_e
+  hhv = hhv_e
+  hhq = hhq_e
+  call sea_surface_fluxes
+  call sea_bottom_fluxes
+  write(23) surf_stress_x, surf_stress_y, BottomFriction, bot_stress_x, bot_stress_y
+endif
+close(23)
+call mpi_finalize(ierr)
+
+contains
+
+subroutine write_masks()
+  ax = dble(lu);  write(23) ax
+  ax = dble(luu); write(23) ax
+  ax = dble(luh); write(23) ax
+  ax = dble(lcu); write(23) ax
+  ax = dble(lcv); write(23) ax
+  ax = dble(llu); write(23) ax
+  ax = dble(llv); write(23) ax
+end subroutine write_masks
+
+end program ref_harness
