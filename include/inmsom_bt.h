@@ -1,0 +1,164 @@
+/*
+ * inmsom_bt: MI355X (gfx950) implementation of INMSOM's explicit
+ * leap-frog barotropic (external-mode shallow-water) step.
+ *
+ * C ABI.  Every 2-D field is a column-major (m fastest) array of nx*ny
+ * values with the reference's bounds bnd_x1..bnd_x2 x bnd_y1..bnd_y2 on a
+ * single process (Control/init_arrays.f90:44-70): nx, ny include two halo
+ * cells per side and the computed interior is m = 3..nx-2, n = 3..ny-2
+ * (1-based).  Host arrays are passed as plain pointers; all device memory
+ * is owned by the context.  Every function returns an INMSOM_BT_* status.
+ *
+ * Calls on one context must come from one host thread at a time; the
+ * context owns one HIP stream and all work is ordered on it.
+ */
+#ifndef INMSOM_BT_H
+#define INMSOM_BT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define INMSOM_BT_OK 0
+#define INMSOM_BT_EINVAL (-1)  /* bad argument: null pointer, size or flag */
+#define INMSOM_BT_EDEVICE (-2) /* HIP runtime error (message in inmsom_bt_last_error) */
+#define INMSOM_BT_ERANGE (-3)  /* state failed the range check (NaN or out of bounds) */
+#define INMSOM_BT_ENOMEM (-4)  /* device allocation failed */
+#define INMSOM_BT_ESTATE (-5)  /* call not valid in the context's current state */
+
+/* Arithmetic of the fused step (inmsom_bt_run). */
+#define INMSOM_BT_FP64_EXACT 0 /* FP64, the reference's operation order, no fused multiply-add */
+#define INMSOM_BT_FP64 1       /* FP64 with reciprocal metrics and fused multiply-add */
+#define INMSOM_BT_FP32 2       /* FP32 stencils; FP64 filtered level and FP64 Robert-Asselin filter */
+
+typedef struct inmsom_bt_ctx inmsom_bt_ctx;
+
+/* Static grid: the reference's module globals that barotropic_dynamics
+ * reads (Modules/mod_basin_grid.f90:133-190, Inc/basinpar.fi).
+ * Masks: either the seven real(4) masks of gridcon, or (when lu is NULL)
+ * the integer mask lbasins (0 = ocean) from which they are built exactly
+ * as Service/grid_construction.f90:33-133 does. */
+typedef struct inmsom_bt_grid {
+    int nx, ny;
+    int periodicity_x, periodicity_y; /* basinpar.fi:13-14 */
+    int full_free_surface;            /* basinpar.fi:46 */
+    double free_fall_acc;             /* basinpar.fi:77 (FreeFallAcc) */
+    double ref_den;                   /* basinpar.fi:76 (RefDen) */
+    double time_smooth;               /* basinpar.fi:3 */
+    const int *lbasins;
+    const float *lu, *luu, *luh, *lcu, *lcv, *llu, *llv;
+    const double *dx, *dy, *dxt, *dyt, *dxh, *dyh, *dxb, *dyb;
+    const double *rlh_s;    /* Coriolis parameter on H points */
+    const double *hhq_rest; /* depth at rest on T points */
+} inmsom_bt_grid;
+
+/* Per-run forcing of the 2-D model step (Function/fluxes.f90 inputs and
+ * the barotropic_dynamics arrays that do not change between steps). */
+typedef struct inmsom_bt_forcing {
+    const double *taux, *tauy; /* T-point wind stress [Pa] */
+    const double *wflux;       /* surface water flux, barotropic_dynamics arg wflux */
+    const double *mu, *mu4;    /* 2-D lateral viscosities, args mu, mu4 */
+    const double *rdis;        /* Rayleigh friction, arg rdis (r_diss) */
+    double cb_nl, ebottom;     /* quadratic drag, basinpar.fi:66-68 */
+} inmsom_bt_forcing;
+
+/* Last error message of the calling thread ("" if none). */
+const char *inmsom_bt_last_error(void);
+
+/* Builds the seven C-grid masks from lbasins on the host, as
+ * Service/grid_construction.f90:33-133 (one process). */
+int inmsom_bt_gridcon(int nx, int ny, int periodicity_x, int periodicity_y,
+                      const int *lbasins, float *lu, float *luu, float *luh,
+                      float *lcu, float *lcv, float *llu, float *llv);
+
+/* Rayleigh friction r_diss from the bottom slope on the host,
+ * Control/init_pars.f90:459-481 (r_fric: basinpar.fi:70). */
+int inmsom_bt_rayleigh(const inmsom_bt_grid *grid, double r_fric, double *r_diss);
+
+/* Uploads the static grid to device 'device' and returns a context.
+ * Replaces the module initialisation the reference performs in
+ * Control/init_pars.f90 (ocean_model_parameters/ocinicond) for the
+ * fields the barotropic step reads. */
+int inmsom_bt_init(const inmsom_bt_grid *grid, int device, inmsom_bt_ctx **out);
+int inmsom_bt_finalize(inmsom_bt_ctx *ctx);
+
+/* Drop-in for barotropic_dynamics (Function/vel_ssh.f90:379-679).
+ * Arguments keep the reference's order and meaning; arrays are host
+ * column-major nx*ny.  In: *_e state, RHSx/RHSy, wflux, mu, mu4, rdis.
+ * Out: *_e (state after nstep substeps), ubrtr_i, vbrtr_i, ssh4gradx,
+ * ssh4grady.  The reference's scratch outputs (vort, str_t2d, str_s2d,
+ * fx, fy, RHS*_adv, RHS*_dif) are recomputed before any later use by its
+ * caller (Control/dyncall.f90:39,65,72-84) and are not returned.  The
+ * context's external-mode depths are re-initialised from ssh_e/sshp_e at
+ * the end, as hh_init does (:668-675); read them with
+ * inmsom_bt_get_depths. */
+int inmsom_bt_barotropic_dynamics(inmsom_bt_ctx *ctx, double tau, int nstep, int ksw4,
+                                  double *ubrtr_e, double *ubrtrp_e, double *vbrtr_e,
+                                  double *vbrtrp_e, double *ssh_e, double *sshp_e,
+                                  double *ubrtr_i, double *vbrtr_i, double *ssh4gradx,
+                                  double *ssh4grady, const double *RHSx, const double *RHSy,
+                                  const double *wflux, const double *mu, const double *mu4,
+                                  const double *rdis);
+
+/* The twelve external-mode depths hhq_e, hhqp_e, hhqn_e, hhu_e, hhup_e,
+ * hhun_e, hhv_e, hhvp_e, hhvn_e, hhh_e, hhhp_e, hhhn_e
+ * (Modules/mod_basin_grid.f90:179-190) as the context holds them. */
+int inmsom_bt_get_depths(inmsom_bt_ctx *ctx, double *const h[12]);
+
+/* ---- device-resident 2-D model ------------------------------------- */
+
+int inmsom_bt_set_forcing(inmsom_bt_ctx *ctx, const inmsom_bt_forcing *f);
+/* Uploads (u, up, v, vp, ssh, sshp) = (ubrtr_e, ubrtrp_e, vbrtr_e,
+ * vbrtrp_e, ssh_e, sshp_e) and re-initialises the depths (hh_init). */
+int inmsom_bt_set_state(inmsom_bt_ctx *ctx, const double *u, const double *up,
+                        const double *v, const double *vp, const double *ssh,
+                        const double *sshp);
+int inmsom_bt_get_state(inmsom_bt_ctx *ctx, double *u, double *up, double *v, double *vp,
+                        double *ssh, double *sshp);
+/* ubrtr_i, vbrtr_i, ssh4gradx, ssh4grady of the last literal step. */
+int inmsom_bt_get_means(inmsom_bt_ctx *ctx, double *ubrtr_i, double *vbrtr_i,
+                        double *ssh4gradx, double *ssh4grady);
+
+/* nsteps outer steps of the 2-D model, each the sequence
+ * Control/dyncall.f90:52-164 runs for a single-level flow: wind stress
+ * (Function/fluxes.f90:29-37), quadratic bottom drag from the current
+ * external-mode state (:226-272), RHS assembly (dyncall.f90:114-126 with
+ * the 3-D terms zero) and barotropic_dynamics(tau, nstep, ksw4), all on
+ * the device in the reference's operation order. */
+int inmsom_bt_model_steps(inmsom_bt_ctx *ctx, double tau, int nstep, int ksw4, int nsteps);
+
+/* nsteps outer steps with ksw4 = 0, each through nstep launches of one
+ * fused kernel (one per state-advancing substep of tau/nstep).  The
+ * state after each outer step is that of
+ * inmsom_bt_model_steps(ctx, tau, nstep, 0, 1): the look-ahead substeps
+ * nstep+1..2*nstep, which the reference runs only to form the averages
+ * ubrtr_i/vbrtr_i/ssh4grad*, are not run; the forcing is formed at the
+ * first substep and frozen over the call, as the reference forms it once
+ * per outer step.  Requires uniform metrics (all eight metric arrays one
+ * constant) and wind, water flux and Coriolis parameter uniform along x,
+ * mu = constant * lu (checked at the first call).  mode is an
+ * INMSOM_BT_FP* value; the first call in a mode converts the state. */
+int inmsom_bt_run(inmsom_bt_ctx *ctx, double tau, int nstep, int nsteps, int mode);
+
+/* Same as inmsom_bt_run, timed on the context's stream with HIP events:
+ * *ms receives the elapsed time of the nsteps*nstep launches. */
+int inmsom_bt_run_timed(inmsom_bt_ctx *ctx, double tau, int nstep, int nsteps, int mode,
+                        double *ms);
+
+/* Range check of Control/dyncall.f90:452-516 on the device state:
+ * *bad receives the number of wet cells with NaN, |ssh| >= 10 or
+ * |u|,|v| >= 100.  Returns INMSOM_BT_ERANGE when *bad > 0. */
+int inmsom_bt_range_check(inmsom_bt_ctx *ctx, long long *bad);
+
+/* Volume anomaly sum(ssh * dx * dy) over wet interior cells, FP64,
+ * summed in a fixed order (deterministic). */
+int inmsom_bt_volume(inmsom_bt_ctx *ctx, double *volume);
+
+int inmsom_bt_synchronize(inmsom_bt_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

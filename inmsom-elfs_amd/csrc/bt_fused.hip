@@ -1,0 +1,941 @@
+// Fused barotropic step: one kernel launch advances the 2-D model one
+// outer step (Control/dyncall.f90:52-164 with nstep = 1, ksw4 = 0):
+// wind stress and quadratic drag (Function/fluxes.f90:29-37, 226-272),
+// continuity, depth update, stress tensor, metric vorticity, flux-form
+// advection, harmonic viscosity, momentum with Coriolis, pressure
+// gradient and Rayleigh friction, and the Robert-Asselin shift
+// (Function/vel_ssh.f90:484-648, depth.f90, mixing.f90:115-168).
+//
+// Work decomposition.  One 64-lane wave owns a strip of 60 output
+// columns (lane L holds column m = 60*strip + L - 1, lanes 2..61 write)
+// and marches down `chunk` output rows.  Everything a row needs from the
+// rows above and below is kept in registers (a rolling window of input
+// rows and of the T- and H-point intermediates); x-neighbours come from
+// the adjacent lanes by wave shuffles.  No LDS tiles and no barriers:
+// each input row is read once per strip and each intermediate is
+// computed once per column.  Work items are dealt to the 8 XCDs in
+// contiguous bands so that vertically adjacent chunks (which share three
+// input rows) run on the same L2.
+//
+// Arithmetic modes (template parameter EXACT / the stencil type R):
+//   FP64_EXACT  the reference's expressions, operation by operation, with
+//               the uniform metric substituted; compiled without FMA
+//               contraction it reproduces bt_literal.hip bit for bit;
+//   FP64        the same algebra simplified for uniform metrics
+//               (reciprocals, factored metric products, FMA);
+//   FP32        FP64's algebra in FP32 for the level-n fields and all
+//               stencils; the filtered (n-1) level, the continuity
+//               accumulation and the Robert-Asselin filter stay FP64.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "bt_fused.h"
+#include "bt_literal.h"
+
+namespace ibt {
+
+constexpr int WAVE = 64;
+constexpr int SW = 60;  // output columns per wave
+constexpr int NXCD = 8;
+
+struct FusedParams {
+    int nx, ny, strips, chunk, nchunks, total, per_xcd;
+    const uint8_t *mb;
+    const void *H, *rd;     // stencil type
+    const void *u, *v, *s;  // level n, stencil type
+    const double *up, *vp, *sp;
+    void *uo, *vo, *so;
+    double *upo, *vpo, *spo;
+    const double *row_f, *row_sx, *row_sy, *row_wf;  // per row, index n-1
+    double g, rho0, ts, tau, cb, eb, nu, ffs, d;
+    // sub-cycling: substep `sub` of `nsub` (tau = tau_inner = outer tau / nsub)
+    int sub, nsub;
+    double dn;             // dfloat(nstep)
+    void *frx, *fry;       // forcing frozen over the substeps of one call (stencil type)
+    double *hup_c, *hvp_c; // filtered U/V depths carried between substeps (FP64_EXACT only)
+};
+
+template <class T>
+__device__ __forceinline__ T xr(T x)  // value of column m+1
+{
+    return __shfl_down(x, 1);
+}
+template <class T>
+__device__ __forceinline__ T xl(T x)  // value of column m-1
+{
+    return __shfl_up(x, 1);
+}
+
+template <class R>
+__device__ __forceinline__ R bitf(int mb, int bit)
+{
+    return (mb & bit) ? R(1) : R(0);
+}
+
+template <typename R, bool EXACT>
+struct Ops {
+    const FusedParams &p;
+    R d, inv_d;
+    __device__ Ops(const FusedParams &pp) : p(pp), d(R(pp.d)), inv_d(R(1.0 / pp.d)) {}
+
+    // depth on a U or V point from its two T points (Function/depth.f90:29-49)
+    __device__ R avg2(R qa, R qb, R la, R lb) const
+    {
+        R slu = la + lb;
+        if constexpr (EXACT) {
+            return (((qa * d) * d) * la + ((qb * d) * d) * lb) / slu / d / d;
+        } else {
+#pragma clang fp contract(fast)
+            return (la * qa + lb * qb) * (slu > R(1.5) ? R(0.5) : R(1));
+        }
+    }
+    // depth on an H point from its four T points (:51-66)
+    __device__ R avg4(R q1, R q2, R q3, R q4, R l1, R l2, R l3, R l4) const
+    {
+        R slu = l1 + l2 + l3 + l4;
+        if constexpr (EXACT) {
+            return (((q1 * d) * d) * l1 + ((q2 * d) * d) * l2 + ((q3 * d) * d) * l3 +
+                    ((q4 * d) * d) * l4) /
+                   slu / d / d;
+        } else {
+#pragma clang fp contract(fast)
+            R rs = slu > R(3.5) ? R(0.25) : slu > R(2.5) ? R(1.0 / 3.0) : slu > R(1.5) ? R(0.5) : R(1);
+            return (l1 * q1 + l2 * q2 + l3 * q3 + l4 * q4) * rs;
+        }
+    }
+};
+
+template <typename R>
+__device__ __forceinline__ R ldr(const void *a, const FusedParams &p, bool colok, int m, int row)
+{
+    return (colok && row >= 1 && row <= p.ny)
+               ? static_cast<const R *>(a)[(size_t)(row - 1) * p.nx + (m - 1)]
+               : R(0);
+}
+__device__ __forceinline__ double ldd(const double *a, const FusedParams &p, bool colok, int m,
+                                      int row)
+{
+    return (colok && row >= 1 && row <= p.ny) ? a[(size_t)(row - 1) * p.nx + (m - 1)] : 0.0;
+}
+__device__ __forceinline__ int ldm(const FusedParams &p, bool colok, int m, int row)
+{
+    return (colok && row >= 1 && row <= p.ny) ? (int)p.mb[(size_t)(row - 1) * p.nx + (m - 1)] : 0;
+}
+
+template <typename R, bool EXACT>
+__global__ void __launch_bounds__(WAVE) k_fused_step(FusedParams p)
+{
+    // XCD-aware work mapping: workgroup w runs on XCD w % 8; XCD k takes
+    // the contiguous band of work items [k*per_xcd, (k+1)*per_xcd).
+    const int w = blockIdx.x;
+    const int t = (w % NXCD) * p.per_xcd + (w / NXCD);
+    if (t >= p.total) return;
+    const int chunk_id = t / p.strips, strip = t - chunk_id * p.strips;
+    const int lane = threadIdx.x;
+    const int m = strip * SW + lane - 1;
+    const bool colok = (m >= 1 && m <= p.nx);
+    const bool writer = colok && lane >= 2 && lane <= SW + 1;
+    const int r0 = chunk_id * p.chunk + 1;
+    const int r1 = min(r0 + p.chunk, p.ny + 1);
+    const bool inx = (m >= 3 && m <= p.nx - 2);
+    const bool inx1 = (m >= 2 && m <= p.nx - 1);
+
+    Ops<R, EXACT> op(p);
+    const R d = op.d;
+    const R tau = R(p.tau);
+    const R G = R(p.g);
+    const R nu = R(p.nu);
+
+    // rolling window (suffix m1 = row n-1, 0 = row n, p1 = n+1, p2 = n+2)
+    R u_m1 = 0, u_0 = 0, u_p1 = 0, v_m1 = 0, v_0 = 0, v_p1 = 0;
+    R s_0 = 0, s_p1 = 0, s_p2 = 0, H_p1 = 0, H_p2 = 0;
+    double up_0 = 0, up_p1 = 0, vp_0 = 0, vp_p1 = 0, sp_0 = 0, sp_p1 = 0;
+    R rd_0 = 0, rd_p1 = 0;
+    int mb_m1 = 0, mb_0 = 0, mb_p1 = 0, mb_p2 = 0;
+    R hq_0 = 0, hq_p1 = 0, hq_p2 = 0, hqp_0 = 0, hqp_p1 = 0, hqn_0 = 0, hqn_p1 = 0;
+    R hu_0 = 0, hu_p1 = 0, hup_0 = 0, hup_p1 = 0;
+    R hv_m1 = 0, hv_0 = 0, hv_p1 = 0, hvp_0 = 0, hvn_0 = 0;
+    R hh_m1 = 0, hh_0 = 0;
+    double sn_0 = 0, sn_p1 = 0;
+    R hun_0 = 0, hun_p1 = 0, stt_0 = 0, stt_p1 = 0, bf_0 = 0, bf_p1 = 0;
+    R sts_m1 = 0, sts_0 = 0, vo_m1 = 0, vo_0 = 0;
+
+    for (int n = r0 - 3; n < r1; ++n) {
+        const int k1 = n + 1, k2 = n + 2;
+        // ---- loads: surface height, depth and masks two rows ahead, the rest one row ahead
+        s_p2 = ldr<R>(p.s, p, colok, m, k2);
+        H_p2 = ldr<R>(p.H, p, colok, m, k2);
+        mb_p2 = ldm(p, colok, m, k2);
+        u_p1 = ldr<R>(p.u, p, colok, m, k1);
+        v_p1 = ldr<R>(p.v, p, colok, m, k1);
+        up_p1 = ldd(p.up, p, colok, m, k1);
+        vp_p1 = ldd(p.vp, p, colok, m, k1);
+        sp_p1 = ldd(p.sp, p, colok, m, k1);
+        rd_p1 = ldr<R>(p.rd, p, colok, m, k1);
+
+        const R lu_0 = bitf<R>(mb_0, MB_LU), lu_p1 = bitf<R>(mb_p1, MB_LU), lu_p2 = bitf<R>(mb_p2, MB_LU);
+        const R lu_0r = xr(lu_0), lu_p1r = xr(lu_p1);
+        const bool iny1 = (k1 >= 3 && k1 <= p.ny - 2);
+
+        // ---- depths (hh_init / hh_update, Function/depth.f90)
+        hq_p2 = H_p2 + s_p2;
+        hqp_p1 = H_p1 + R(sp_p1);
+        {
+            const R hq1r = xr(hq_p1), hqp1r = xr(hqp_p1), hqp0 = hqp_0;
+            hu_p1 = (mb_p1 & MB_LLU) ? op.avg2(hq_p1, hq1r, lu_p1, lu_p1r) : R(0);
+            hup_p1 = (mb_p1 & MB_LLU) ? op.avg2(hqp_p1, hqp1r, lu_p1, lu_p1r) : R(0);
+            hv_p1 = (mb_p1 & MB_LLV) ? op.avg2(hq_p1, hq_p2, lu_p1, lu_p2) : R(0);
+            hvp_0 = (mb_0 & MB_LLV) ? op.avg2(hqp0, hqp_p1, lu_0, lu_p1) : R(0);
+            const R hq0r = xr(hq_0);
+            hh_0 = (mb_0 & MB_LUH) ? op.avg4(hq_0, hq0r, hq_p1, hq1r, lu_0, lu_0r, lu_p1, lu_p1r) : R(0);
+        }
+
+        // ---- T-point stage of row n+1: continuity, new depths, tension, drag
+        const R u1l = xl(u_p1), hu1l = xl(hu_p1), up1l = xl(R(up_p1));
+        {
+            const bool tw = (mb_p1 & MB_LU) && inx && iny1;
+            const R wf = (k1 >= 1 && k1 <= p.ny) ? R(p.row_wf[k1 - 1]) : R(0);
+            double sn;
+            if constexpr (EXACT) {
+                sn = sp_p1 + 2.0 * p.tau *
+                                 (wf / p.rho0 * p.ffs -
+                                  (u_p1 * hu_p1 * d - u1l * hu1l * d + v_p1 * hv_p1 * d -
+                                   v_0 * hv_0 * d) /
+                                      (d * d));
+            } else {
+#pragma clang fp contract(fast)
+                const R inc = R(2.0 * p.tau) * (wf * R(p.ffs / p.rho0) -
+                                                (u_p1 * hu_p1 - u1l * hu1l + v_p1 * hv_p1 - v_0 * hv_0) *
+                                                    op.inv_d);
+                sn = sp_p1 + double(inc);
+            }
+            sn_p1 = tw ? sn : 0.0;
+        }
+        hqn_p1 = H_p1 + R(sn_p1);
+        {
+            const R hqn1r = xr(hqn_p1);
+            hun_p1 = (mb_p1 & MB_LLU) ? op.avg2(hqn_p1, hqn1r, lu_p1, lu_p1r) : R(0);
+            hvn_0 = (mb_0 & MB_LLV) ? op.avg2(hqn_0, hqn_p1, lu_0, lu_p1) : R(0);
+        }
+        {
+            const bool tw = (mb_p1 & MB_LU) && inx && iny1;
+            R st, bf;
+            const R upr = R(up_p1), vpr = R(vp_p1), vp0r = R(vp_0);
+            if constexpr (EXACT) {
+                st = d / d * (upr / d - up1l / d) - d / d * (vpr / d - vp0r / d);
+                R ut = (u_p1 * d * d * hu_p1 + u1l * d * d * hu1l) / 2.0 / d / d / hq_p1;
+                R vt = (v_p1 * d * d * hv_p1 + v_0 * d * d * hv_0) / 2.0 / d / d / hq_p1;
+                bf = R(p.cb) * sqrt(ut * ut + vt * vt + R(p.eb));
+            } else {
+#pragma clang fp contract(fast)
+                st = op.inv_d * ((upr - up1l) - (vpr - vp0r));
+                const R rh = R(0.5) / hq_p1;
+                R ut = (u_p1 * hu_p1 + u1l * hu1l) * rh;
+                R vt = (v_p1 * hv_p1 + v_0 * hv_0) * rh;
+                bf = R(p.cb) * sqrt(ut * ut + vt * vt + R(p.eb));
+            }
+            stt_p1 = tw ? st : R(0);
+            bf_p1 = (tw && p.sub == 1) ? bf : R(0);
+        }
+
+        // ---- H-point stage of row n: shear and metric vorticity
+        {
+            const bool hw = (mb_0 & MB_LUU) && inx && (n >= 3 && n <= p.ny - 2);
+            const R vp0 = R(vp_0), vp0r = xr(vp0);
+            const R v0r = xr(v_0);
+            R ss, vo;
+            if constexpr (EXACT) {
+                ss = d / d * (R(up_p1) / d - R(up_0) / d) + d / d * (vp0r / d - vp0 / d);
+                vo = (v0r * d - v_0 * d) - (u_p1 * d - u_0 * d) - ((v0r - v_0) * d - (u_p1 - u_0) * d);
+            } else {
+#pragma clang fp contract(fast)
+                ss = op.inv_d * ((R(up_p1) - R(up_0)) + (vp0r - vp0));
+                vo = R(0);  // identically zero for uniform metrics (vel_ssh.f90:35-37)
+            }
+            sts_0 = hw ? ss : R(0);
+            vo_0 = hw ? vo : R(0);
+        }
+
+        // ---- outputs of row n
+        if (n >= r0) {
+            const bool iny = (n >= 3 && n <= p.ny - 2);
+            const R lu_m1 = bitf<R>(mb_m1, MB_LU);
+            const R mu_m1 = lu_m1 * nu, mu_0 = lu_0 * nu, mu_p1 = lu_p1 * nu;
+            const R mu_m1r = xr(mu_m1), mu_0r = xr(mu_0), mu_p1r = xr(mu_p1);
+            const R mu_0l = xl(mu_0), mu_p1l = xl(mu_p1);
+            const R u0r = xr(u_0), u0l = xl(u_0), hu0r = xr(hu_0), hu0l = xl(hu_0);
+            const R v0r = xr(v_0), v0l = xl(v_0), hv0r = xr(hv_0);
+            const R vm1r = xr(v_m1), hvm1r = xr(hv_m1);
+            const R up1l_ = xl(u_p1), hup1l = xl(hu_p1);
+            const R s0r = xr(s_0), rd0r = xr(rd_0), bf0r = xr(bf_0);
+            const R hq0r = xr(hq_0), stt0r = xr(stt_0);
+            const R hh0l = xl(hh_0), sts0l = xl(sts_0), vo0l = xl(vo_0);
+            const R up0 = R(up_0), vp0 = R(vp_0);
+            const R luu0 = bitf<R>(mb_0, MB_LUU), luum1 = bitf<R>(mb_m1, MB_LUU);
+            const R sx = R(p.row_sx[n - 1]), sy = R(p.row_sy[n - 1]);
+            const R f_0 = R(p.row_f[n - 1]), f_m1 = n >= 2 ? R(p.row_f[n - 2]) : R(0);
+            const size_t io = (size_t)(n - 1) * p.nx + (m - 1);
+            const bool own = writer && n <= p.ny;
+            // the RHS of Control/dyncall.f90:114-126 is formed once per call
+            // (first substep) and frozen over the call's substeps
+            R rhs_u, rhs_v;
+            if constexpr (EXACT) {
+                rhs_u = 0.0 + (sx + (-(bf_0 + bf0r) / 2.0 * up0)) * d * d;
+                rhs_v = 0.0 + (sy + (-(bf_0 + bf_p1) / 2.0 * vp0)) * d * d;
+            } else {
+#pragma clang fp contract(fast)
+                rhs_u = sx - R(0.5) * (bf_0 + bf0r) * up0;
+                rhs_v = sy - R(0.5) * (bf_0 + bf_p1) * vp0;
+            }
+            if (p.sub > 1) {
+                rhs_u = own ? static_cast<const R *>(p.frx)[io] : R(0);
+                rhs_v = own ? static_cast<const R *>(p.fry)[io] : R(0);
+            } else if (p.nsub > 1 && own) {
+                static_cast<R *>(p.frx)[io] = rhs_u;
+                static_cast<R *>(p.fry)[io] = rhs_v;
+            }
+            // level n-1 U/V depths: hh_init values at the first substep,
+            // the hh_shift-filtered carry afterwards (depth.f90:195-203)
+            R hupc = hup_0, hvpc = hvp_0;
+            if constexpr (EXACT) {
+                if (p.sub > 1) {
+                    hupc = own ? R(p.hup_c[io]) : R(0);
+                    hvpc = own ? R(p.hvp_c[io]) : R(0);
+                }
+            }
+            R un, vn;
+            if constexpr (EXACT) {
+                // zonal momentum (vel_ssh.f90:536-551) with uv_trans (:59-81) and uv_diff2 (:146-159)
+                R rhsx = rhs_u;
+                R fx_p = (u_0 * d * hu_0 + u0r * d * hu0r) / 2.0 * (u_0 + u0r) / 2.0;
+                R fx_m = (u_0 * d * hu_0 + u0l * d * hu0l) / 2.0 * (u_0 + u0l) / 2.0;
+                R fy_p = (v_0 * d * hv_0 + v0r * d * hv0r) / 2.0 * (u_p1 + u_0) / 2.0 * luu0;
+                R fy_m = (v_m1 * d * hv_m1 + vm1r * d * hvm1r) / 2.0 * (u_m1 + u_0) / 2.0 * luum1;
+                R radv = -(fx_p - fx_m + fy_p - fy_m) +
+                         (vo_0 * hh_0 * (v0r + v_0) + vo_m1 * hh_m1 * (vm1r + v_m1)) / 4.0;
+                R muh_p = (mu_0 + mu_0r + mu_p1 + mu_p1r) / 4.0;
+                R muh_m = (mu_0 + mu_0r + mu_m1 + mu_m1r) / 4.0;
+                R pp = (d * d * mu_0r * hq0r * stt0r - d * d * mu_0 * hq_0 * stt_0) / d;
+                R qq = (d * d * muh_p * hh_0 * sts_0 - d * d * muh_m * hh_m1 * sts_m1) / d;
+                R rdfx = pp + qq;
+                R bp = hun_0 * d * d / 2.0 / tau;
+                R bp0 = hupc * d * d / 2.0 / tau;
+                R slx = -G * (s0r - s_0) * d * hu_0;
+                R grx = rhsx + slx + rdfx + radv - (rd_0 + rd0r) / 2.0 * up0 * d * d * hu_0 +
+                        (f_0 * hh_0 * d * d * (v0r + v_0) + f_m1 * hh_m1 * d * d * (vm1r + v_m1)) / 4.0;
+                un = (up0 * bp0 + grx) / bp;
+
+                // meridional momentum (vel_ssh.f90:554-569) with uv_trans (:84-105) and uv_diff2 (:162-175)
+                R rhsy = rhs_v;
+                R gy_p = (v_0 * d * hv_0 + v_p1 * d * hv_p1) / 2.0 * (v_0 + v_p1) / 2.0;
+                R gy_m = (v_0 * d * hv_0 + v_m1 * d * hv_m1) / 2.0 * (v_0 + v_m1) / 2.0;
+                R gx_p = (u_0 * d * hu_0 + u_p1 * d * hu_p1) / 2.0 * (v0r + v_0) / 2.0;
+                R gx_m = (u0l * d * hu0l + up1l_ * d * hup1l) / 2.0 * (v0l + v_0) / 2.0;
+                R rady = -(gx_p - gx_m + gy_p - gy_m) -
+                         (vo_0 * hh_0 * (u_p1 + u_0) + vo0l * hh0l * (up1l_ + u0l)) / 4.0;
+                R nuh_p = (mu_0 + mu_0r + mu_p1 + mu_p1r) / 4.0;
+                R nuh_m = (mu_0 + mu_0l + mu_p1 + mu_p1l) / 4.0;
+                R py = (d * d * mu_p1 * hq_p1 * stt_p1 - d * d * mu_0 * hq_0 * stt_0) / d;
+                R qy = (d * d * nuh_p * hh_0 * sts_0 - d * d * nuh_m * hh0l * sts0l) / d;
+                R rdfy = -py + qy;
+                R bq = hvn_0 * d * d / 2.0 / tau;
+                R bq0 = hvpc * d * d / 2.0 / tau;
+                R sly = -G * (s_p1 - s_0) * d * hv_0;
+                R gry = rhsy + sly + rdfy + rady - (rd_0 + rd_p1) / 2.0 * vp0 * d * d * hv_0 -
+                        (f_0 * hh_0 * d * d * (u_p1 + u_0) + f_0 * hh0l * d * d * (up1l_ + u0l)) / 4.0;
+                vn = (vp0 * bq0 + gry) / bq;
+            } else {
+#pragma clang fp contract(fast)
+                const R id = op.inv_d;
+                const R two_tau = R(2.0) * tau;
+                // forces per unit area (the reference's terms divided by dxt*dyh)
+                R adv = R(-0.25) * ((u_0 * hu_0 + u0r * hu0r) * (u_0 + u0r) -
+                                    (u_0 * hu_0 + u0l * hu0l) * (u_0 + u0l) +
+                                    luu0 * (v_0 * hv_0 + v0r * hv0r) * (u_p1 + u_0) -
+                                    luum1 * (v_m1 * hv_m1 + vm1r * hvm1r) * (u_m1 + u_0));
+                R muh_p = R(0.25) * (mu_0 + mu_0r + mu_p1 + mu_p1r);
+                R muh_m = R(0.25) * (mu_0 + mu_0r + mu_m1 + mu_m1r);
+                R dif = (mu_0r * hq0r * stt0r - mu_0 * hq_0 * stt_0) +
+                        (muh_p * hh_0 * sts_0 - muh_m * hh_m1 * sts_m1);
+                R F = rhs_u - G * id * (s0r - s_0) * hu_0 +
+                      id * (adv + dif) - R(0.5) * (rd_0 + rd0r) * up0 * hu_0 +
+                      R(0.25) * (f_0 * hh_0 * (v0r + v_0) + f_m1 * hh_m1 * (vm1r + v_m1));
+                un = (up0 * hupc + two_tau * F) / hun_0;
+
+                R advy = R(-0.25) * ((u_0 * hu_0 + u_p1 * hu_p1) * (v0r + v_0) -
+                                     (u0l * hu0l + up1l_ * hup1l) * (v0l + v_0) +
+                                     (v_0 * hv_0 + v_p1 * hv_p1) * (v_0 + v_p1) -
+                                     (v_0 * hv_0 + v_m1 * hv_m1) * (v_0 + v_m1));
+                R nuh_p = R(0.25) * (mu_0 + mu_0r + mu_p1 + mu_p1r);
+                R nuh_m = R(0.25) * (mu_0 + mu_0l + mu_p1 + mu_p1l);
+                R dify = -(mu_p1 * hq_p1 * stt_p1 - mu_0 * hq_0 * stt_0) +
+                         (nuh_p * hh_0 * sts_0 - nuh_m * hh0l * sts0l);
+                R Fy = rhs_v - G * id * (s_p1 - s_0) * hv_0 +
+                       id * (advy + dify) - R(0.5) * (rd_0 + rd_p1) * vp0 * hv_0 -
+                       R(0.25) * f_0 * (hh_0 * (u_p1 + u_0) + hh0l * (up1l_ + u0l));
+                vn = (vp0 * hvpc + two_tau * Fy) / hvn_0;
+            }
+            const bool uw = (mb_0 & MB_LCU) && inx && iny;
+            const bool vw = (mb_0 & MB_LCV) && inx && iny;
+            un = uw ? un : R(0);
+            vn = vw ? vn : R(0);
+
+            // time-level shift with the Robert-Asselin filter (vel_ssh.f90:588-612), nstep = 1
+            R s_new = s_0, u_new = u_0, v_new = v_0;
+            double sp_new = sp_0, up_new = up_0, vp_new = vp_0;
+            if (inx1 && n >= 2 && n <= p.ny - 1) {
+                const double ts = p.ts, cts = 0.5 * p.ts / p.dn;
+                if (mb_0 & MB_LU) {
+                    if constexpr (EXACT) {
+                        sp_new = s_0 + ts * (sn_0 - 2.0 * s_0 + sp_0) / 2.0 / p.dn;
+                    } else {
+#pragma clang fp contract(fast)
+                        sp_new = double(s_0) + cts * (sn_0 - 2.0 * double(s_0) + sp_0);
+                    }
+                    s_new = R(sn_0);
+                }
+                if (mb_0 & MB_LCU) {
+                    if constexpr (EXACT) {
+                        up_new = u_0 + ts * (un - 2.0 * u_0 + up_0) / 2.0 / p.dn;
+                    } else {
+#pragma clang fp contract(fast)
+                        up_new = double(u_0) + cts * (double(un) - 2.0 * double(u_0) + up_0);
+                    }
+                    u_new = un;
+                }
+                if (mb_0 & MB_LCV) {
+                    if constexpr (EXACT) {
+                        vp_new = v_0 + ts * (vn - 2.0 * v_0 + vp_0) / 2.0 / p.dn;
+                    } else {
+#pragma clang fp contract(fast)
+                        vp_new = double(v_0) + cts * (double(vn) - 2.0 * double(v_0) + vp_0);
+                    }
+                    v_new = vn;
+                }
+            }
+            if constexpr (EXACT) {
+                // hh_shift of the U/V depths for the next substep (depth.f90:195-203)
+                if (p.nsub > 1 && p.sub < p.nsub && own) {
+                    const double ts = p.ts;
+                    double hu_new = hupc, hv_new = hvpc;
+                    if (inx1 && n >= 2 && n <= p.ny - 1) {
+                        if (mb_0 & MB_LLU) hu_new = hu_0 + ts * (hun_0 - 2.0 * hu_0 + hupc) / 2.0 / p.dn;
+                        if (mb_0 & MB_LLV) hv_new = hv_0 + ts * (hvn_0 - 2.0 * hv_0 + hvpc) / 2.0 / p.dn;
+                    }
+                    p.hup_c[io] = hu_new;
+                    p.hvp_c[io] = hv_new;
+                }
+            }
+            if (own) {
+                const size_t i = io;
+                static_cast<R *>(p.so)[i] = s_new;
+                static_cast<R *>(p.uo)[i] = u_new;
+                static_cast<R *>(p.vo)[i] = v_new;
+                p.spo[i] = sp_new;
+                p.upo[i] = up_new;
+                p.vpo[i] = vp_new;
+            }
+        }
+
+        // ---- advance the window one row
+        u_m1 = u_0; u_0 = u_p1;
+        v_m1 = v_0; v_0 = v_p1;
+        s_0 = s_p1; s_p1 = s_p2;
+        H_p1 = H_p2;
+        up_0 = up_p1; vp_0 = vp_p1; sp_0 = sp_p1; rd_0 = rd_p1;
+        mb_m1 = mb_0; mb_0 = mb_p1; mb_p1 = mb_p2;
+        hq_0 = hq_p1; hq_p1 = hq_p2;
+        hqp_0 = hqp_p1; hqn_0 = hqn_p1;
+        hu_0 = hu_p1; hup_0 = hup_p1;
+        hv_m1 = hv_0; hv_0 = hv_p1;
+        hh_m1 = hh_0;
+        sn_0 = sn_p1; hun_0 = hun_p1;
+        stt_0 = stt_p1; bf_0 = bf_p1;
+        sts_m1 = sts_0; vo_m1 = vo_0;
+    }
+}
+
+// -------------------------------------------------------------- helpers
+__global__ void k_pack_masks(int n, const float *lu, const float *luu, const float *luh,
+                             const float *lcu, const float *lcv, const float *llu, const float *llv,
+                             uint8_t *mb)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n) return;
+    uint8_t b = 0;
+    if (lu[i] > 0.5f) b |= MB_LU;
+    if (lcu[i] > 0.5f) b |= MB_LCU;
+    if (lcv[i] > 0.5f) b |= MB_LCV;
+    if (luu[i] > 0.5f) b |= MB_LUU;
+    if (luh[i] > 0.5f) b |= MB_LUH;
+    if (llu[i] > 0.5f) b |= MB_LLU;
+    if (llv[i] > 0.5f) b |= MB_LLV;
+    mb[i] = b;
+}
+
+__global__ void k_to_f32(size_t n, const double *a, float *b)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+__global__ void k_to_f64(size_t n, const float *a, double *b)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+__global__ void k_copy_d(size_t n, const double *a, double *b)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+// Volume anomaly: one block per row, fixed-shape tree in LDS, then the
+// row sums are added in row order by one thread -> deterministic.
+template <class R>
+__global__ void k_row_volume(int nx, int ny, const R *ssh, const float *lu, const double *dx,
+                             const double *dy, double *rows)
+{
+    __shared__ double part[256];
+    const int n = blockIdx.x + 3;  // interior rows 3..ny-2
+    double acc = 0.0;
+    for (int m = threadIdx.x + 3; m <= nx - 2; m += blockDim.x) {
+        size_t i = (size_t)(n - 1) * nx + (m - 1);
+        if (lu[i] > 0.5f) acc += (double)ssh[i] * dx[i] * dy[i];
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) rows[blockIdx.x] = part[0];
+}
+__global__ void k_sum_rows(int nrows, const double *rows, double *out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double t = 0.0;
+        for (int r = 0; r < nrows; ++r) t += rows[r];
+        *out = t;
+    }
+}
+
+// Range check of Control/dyncall.f90:452-516 on the 2-D state.
+template <class R>
+__global__ void k_range(int nx, int ny, const float *lu, const float *lcu, const float *lcv,
+                        const R *ssh, const R *u, const R *v, unsigned long long *bad)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nx * ny) return;
+    int m = (int)(i % nx) + 1, n = (int)(i / nx) + 1;
+    if (m < 3 || m > nx - 2 || n < 3 || n > ny - 2) return;
+    bool b = false;
+    if (lu[i] > 0.5f) {
+        double s = (double)ssh[i];
+        b |= !(fabs(s) < 10.0);
+    }
+    if (lcu[i] > 0.5f) b |= !(fabs((double)u[i]) < 100.0);
+    if (lcv[i] > 0.5f) b |= !(fabs((double)v[i]) < 100.0);
+    if (b) atomicAdd(bad, 1ull);
+}
+
+// ------------------------------------------------------------- host side
+static int hipfail(hipError_t e, const char *what)
+{
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return INMSOM_BT_EDEVICE;
+}
+#define HC(expr)                                         \
+    do {                                                 \
+        hipError_t e_ = (expr);                          \
+        if (e_ != hipSuccess) return hipfail(e_, #expr); \
+    } while (0)
+
+static inline unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
+
+static int g_chunk = 32;
+
+int fused_set_shape(inmsom_bt_ctx *c, int chunk)
+{
+    (void)c;
+    if (chunk < 1 || chunk > 4096) {
+        set_error("chunk out of range");
+        return INMSOM_BT_EINVAL;
+    }
+    g_chunk = chunk;
+    return INMSOM_BT_OK;
+}
+
+// Per-row tables and the packed mask; validates that the fused step
+// applies (uniform metrics, forcing uniform along x, mu = nu * lu).
+static int fused_prepare(inmsom_bt_ctx *c)
+{
+    if (c->fused_ready) return INMSOM_BT_OK;
+    if (!c->uniform_metrics) {
+        set_error("the fused step needs uniform metrics; use inmsom_bt_model_steps");
+        return INMSOM_BT_ESTATE;
+    }
+    if (!c->have_forcing) {
+        set_error("inmsom_bt_set_forcing must be called first");
+        return INMSOM_BT_ESTATE;
+    }
+    const int nx = c->nx, ny = c->ny;
+    const size_t n = c->n;
+    std::vector<double> taux(n), tauy(n), wf(n), mu(n), f(n);
+    std::vector<float> lu(n);
+    HC(hipMemcpy(taux.data(), c->taux, n * 8, hipMemcpyDeviceToHost));
+    HC(hipMemcpy(tauy.data(), c->tauy, n * 8, hipMemcpyDeviceToHost));
+    HC(hipMemcpy(wf.data(), c->wflux, n * 8, hipMemcpyDeviceToHost));
+    HC(hipMemcpy(mu.data(), c->mu, n * 8, hipMemcpyDeviceToHost));
+    HC(hipMemcpy(f.data(), c->rlh_s, n * 8, hipMemcpyDeviceToHost));
+    HC(hipMemcpy(lu.data(), c->mask[0], n * 4, hipMemcpyDeviceToHost));
+    double nu = -1.0;
+    for (size_t i = 0; i < n; ++i) {
+        if (lu[i] > 0.5f) {
+            if (nu < 0.0) nu = mu[i];
+            if (mu[i] != nu) {
+                set_error("the fused step needs mu = constant on wet cells and 0 on land");
+                return INMSOM_BT_ESTATE;
+            }
+        } else if (mu[i] != 0.0) {
+            set_error("the fused step needs mu = 0 on land");
+            return INMSOM_BT_ESTATE;
+        }
+    }
+    c->nu = nu < 0.0 ? 0.0 : nu;
+    const double d = c->metric0, rho0 = c->g.rho0;
+    std::vector<double> rf(ny), rsx(ny), rsy(ny), rwf(ny);
+    for (int r = 0; r < ny; ++r) {
+        const double *tx = &taux[(size_t)r * nx], *ty = &tauy[(size_t)r * nx];
+        const double *w = &wf[(size_t)r * nx], *fr = &f[(size_t)r * nx];
+        for (int m = 1; m < nx; ++m)
+            if (tx[m] != tx[0] || ty[m] != ty[0] || w[m] != w[0] || fr[m] != fr[0]) {
+                set_error("the fused step needs taux, tauy, wflux and rlh_s uniform along x");
+                return INMSOM_BT_ESTATE;
+            }
+        rf[r] = fr[0];
+        rwf[r] = w[0];
+        // Function/fluxes.f90:30-31 and :35-36 with the row's uniform values
+        rsx[r] = (tx[0] * d * d + tx[0] * d * d) / rho0 / 2.0 / d / d;
+        rsy[r] = (ty[0] * d * d + ty[0] * d * d) / rho0 / 2.0 / d / d;
+    }
+    auto upl = [&](double **dst, const std::vector<double> &src) -> int {
+        if (!*dst) HC(hipMalloc(dst, src.size() * 8));
+        HC(hipMemcpy(*dst, src.data(), src.size() * 8, hipMemcpyHostToDevice));
+        return INMSOM_BT_OK;
+    };
+    int rc;
+    if ((rc = upl(&c->row_f, rf)) || (rc = upl(&c->row_wx, rsx)) || (rc = upl(&c->row_wy, rsy)) ||
+        (rc = upl(&c->row_wf, rwf)))
+        return rc;
+    if (!c->mbits) HC(hipMalloc(&c->mbits, n));
+    hipLaunchKernelGGL(k_pack_masks, dim3(nblk(n)), dim3(256), 0, c->stream, (int)n, c->mask[0],
+                       c->mask[1], c->mask[2], c->mask[3], c->mask[4], c->mask[5], c->mask[6],
+                       c->mbits);
+    HC(hipGetLastError());
+    HC(hipStreamSynchronize(c->stream));
+    c->fused_ready = true;
+    return INMSOM_BT_OK;
+}
+
+// Fused state buffers: level n (u, v, ssh) in the stencil type and the
+// filtered level (up, vp, sshp) in FP64, twice (ping-pong), plus the
+// static depth and Rayleigh coefficient in the stencil type.
+struct FusedAlloc {
+    void *lvl[2][3] = {};
+    double *flt[2][3] = {};
+    void *H = nullptr, *rd = nullptr;
+    void *frx = nullptr, *fry = nullptr;     // frozen forcing (stencil type)
+    double *hup_c = nullptr, *hvp_c = nullptr;  // carried filtered depths
+    int mode = -1;
+};
+static FusedAlloc *falloc(inmsom_bt_ctx *c) { return c->fa; }
+
+void fused_release(inmsom_bt_ctx *c)
+{
+    FusedAlloc *a = falloc(c);
+    if (a) {
+        for (int b = 0; b < 2; ++b)
+            for (int k = 0; k < 3; ++k) {
+                if (a->lvl[b][k]) hipFree(a->lvl[b][k]);
+                if (a->flt[b][k]) hipFree(a->flt[b][k]);
+            }
+        for (void *q : {a->H, a->rd, a->frx, a->fry, (void *)a->hup_c, (void *)a->hvp_c})
+            if (q) hipFree(q);
+        delete a;
+        c->fa = nullptr;
+    }
+    for (double **p : {&c->row_f, &c->row_wx, &c->row_wy, &c->row_wf})
+        if (*p) {
+            hipFree(*p);
+            *p = nullptr;
+        }
+    if (c->mbits) hipFree(c->mbits);
+    c->mbits = nullptr;
+    c->fused_ready = false;
+    c->state_on_fused = false;
+}
+
+// Moves the canonical FP64 state into the fused buffers of `mode`.
+static int fused_load(inmsom_bt_ctx *c, int mode)
+{
+    FusedAlloc *a = falloc(c);
+    const size_t n = c->n;
+    const size_t es = (mode == INMSOM_BT_FP32) ? 4 : 8;
+    if (a && a->mode != mode) {
+        // different element size: reallocate
+        fused_release(c);
+        c->fused_ready = false;
+        int rc = fused_prepare(c);
+        if (rc) return rc;
+        a = nullptr;
+    }
+    if (!a) {
+        a = new FusedAlloc();
+        c->fa = a;
+        a->mode = mode;
+        for (int b = 0; b < 2; ++b)
+            for (int k = 0; k < 3; ++k) {
+                HC(hipMalloc(&a->lvl[b][k], n * es));
+                HC(hipMalloc(&a->flt[b][k], n * 8));
+            }
+        HC(hipMalloc(&a->H, n * es));
+        HC(hipMalloc(&a->rd, n * es));
+        HC(hipMalloc(&a->frx, n * es));
+        HC(hipMalloc(&a->fry, n * es));
+        HC(hipMalloc(&a->hup_c, n * 8));
+        HC(hipMalloc(&a->hvp_c, n * 8));
+        if (es == 4) {
+            hipLaunchKernelGGL(k_to_f32, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->hr, (float *)a->H);
+            hipLaunchKernelGGL(k_to_f32, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->rdis, (float *)a->rd);
+        } else {
+#pragma clang fp contract(fast)
+            hipLaunchKernelGGL(k_copy_d, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->hr, (double *)a->H);
+            hipLaunchKernelGGL(k_copy_d, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->rdis, (double *)a->rd);
+        }
+    } else {
+#pragma clang fp contract(fast)
+        // forcing may have changed: refresh the static copies
+        if (es == 4)
+            hipLaunchKernelGGL(k_to_f32, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->rdis, (float *)a->rd);
+        else
+            hipLaunchKernelGGL(k_copy_d, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->rdis, (double *)a->rd);
+    }
+    // canonical order u, up, v, vp, ssh, sshp -> level (u, v, ssh), filtered (up, vp, sshp)
+    const int lv[3] = {0, 2, 4}, fl[3] = {1, 3, 5};
+    for (int k = 0; k < 3; ++k) {
+        if (es == 4)
+            hipLaunchKernelGGL(k_to_f32, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->st[lv[k]],
+                               (float *)a->lvl[0][k]);
+        else
+            hipLaunchKernelGGL(k_copy_d, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->st[lv[k]],
+                               (double *)a->lvl[0][k]);
+        hipLaunchKernelGGL(k_copy_d, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->st[fl[k]], a->flt[0][k]);
+    }
+    HC(hipGetLastError());
+    c->fz_cur = 0;
+    c->fused_mode = mode;
+    c->state_on_fused = true;
+    return INMSOM_BT_OK;
+}
+
+int fused_flush(inmsom_bt_ctx *c)
+{
+    if (!c->state_on_fused) return INMSOM_BT_OK;
+    FusedAlloc *a = falloc(c);
+    const size_t n = c->n;
+    const int b = c->fz_cur;
+    const int lv[3] = {0, 2, 4}, fl[3] = {1, 3, 5};
+    for (int k = 0; k < 3; ++k) {
+        if (a->mode == INMSOM_BT_FP32)
+            hipLaunchKernelGGL(k_to_f64, dim3(nblk(n)), dim3(256), 0, c->stream, n,
+                               (const float *)a->lvl[b][k], c->st[lv[k]]);
+        else
+            hipLaunchKernelGGL(k_copy_d, dim3(nblk(n)), dim3(256), 0, c->stream, n,
+                               (const double *)a->lvl[b][k], c->st[lv[k]]);
+        hipLaunchKernelGGL(k_copy_d, dim3(nblk(n)), dim3(256), 0, c->stream, n, a->flt[b][k], c->st[fl[k]]);
+    }
+    // the reference's callers see depths re-initialised from the new state
+    lit_hh_init(c, c->st[4], c->st[5]);
+    HC(hipGetLastError());
+    HC(hipStreamSynchronize(c->stream));
+    c->state_on_fused = false;
+    c->fused_mode = -1;
+    return INMSOM_BT_OK;
+}
+
+static FusedParams make_params(inmsom_bt_ctx *c, FusedAlloc *a, int b, double tau, int nstep,
+                               int sub)
+{
+    FusedParams p{};
+    p.nx = c->nx;
+    p.ny = c->ny;
+    p.strips = (c->nx + 2 + SW - 1) / SW;  // columns 1..nx need strips covering m <= nx
+    p.chunk = g_chunk;
+    p.nchunks = (c->ny + p.chunk - 1) / p.chunk;
+    p.total = p.strips * p.nchunks;
+    p.per_xcd = (p.total + NXCD - 1) / NXCD;
+    p.mb = c->mbits;
+    p.H = a->H;
+    p.rd = a->rd;
+    p.u = a->lvl[b][0];
+    p.v = a->lvl[b][1];
+    p.s = a->lvl[b][2];
+    p.up = a->flt[b][0];
+    p.vp = a->flt[b][1];
+    p.sp = a->flt[b][2];
+    p.uo = a->lvl[1 - b][0];
+    p.vo = a->lvl[1 - b][1];
+    p.so = a->lvl[1 - b][2];
+    p.upo = a->flt[1 - b][0];
+    p.vpo = a->flt[1 - b][1];
+    p.spo = a->flt[1 - b][2];
+    p.row_f = c->row_f;
+    p.row_sx = c->row_wx;
+    p.row_sy = c->row_wy;
+    p.row_wf = c->row_wf;
+    p.g = c->g.g;
+    p.rho0 = c->g.rho0;
+    p.ts = c->g.ts;
+    p.tau = tau / (double)nstep;  // tau_inner, vel_ssh.f90:465
+    p.sub = sub;
+    p.nsub = nstep;
+    p.dn = (double)nstep;
+    p.frx = a->frx;
+    p.fry = a->fry;
+    p.hup_c = a->hup_c;
+    p.hvp_c = a->hvp_c;
+    p.cb = c->cb_nl;
+    p.eb = c->ebottom;
+    p.nu = c->nu;
+    p.ffs = (double)c->g.ffs;
+    p.d = c->metric0;
+    return p;
+}
+
+static void launch(inmsom_bt_ctx *c, const FusedParams &p, int mode)
+{
+    dim3 grid((unsigned)(p.per_xcd * NXCD)), blk(WAVE);
+    if (mode == INMSOM_BT_FP64_EXACT)
+        hipLaunchKernelGGL((k_fused_step<double, true>), grid, blk, 0, c->stream, p);
+    else if (mode == INMSOM_BT_FP64)
+        hipLaunchKernelGGL((k_fused_step<double, false>), grid, blk, 0, c->stream, p);
+    else
+        hipLaunchKernelGGL((k_fused_step<float, false>), grid, blk, 0, c->stream, p);
+}
+
+int fused_run(inmsom_bt_ctx *c, double tau, int nstep, int nsteps, int mode, double *ms)
+{
+    if (nstep < 1 || nstep > 1000) {
+        set_error("nstep must be in 1..1000");
+        return INMSOM_BT_EINVAL;
+    }
+    if (mode != INMSOM_BT_FP64_EXACT && mode != INMSOM_BT_FP64 && mode != INMSOM_BT_FP32) {
+        set_error("unknown mode");
+        return INMSOM_BT_EINVAL;
+    }
+    if (!(tau > 0.0) || nsteps < 0) {
+        set_error("bad tau / nsteps");
+        return INMSOM_BT_EINVAL;
+    }
+    if (c->g.ffs != 1 || c->g.px || c->g.py) {
+        set_error("the fused step covers full_free_surface = 1 without periodicity");
+        return INMSOM_BT_ESTATE;
+    }
+    int rc = fused_prepare(c);
+    if (rc) return rc;
+    if (c->state_on_fused && c->fused_mode != mode) {
+        rc = fused_flush(c);
+        if (rc) return rc;
+    }
+    if (!c->state_on_fused) {
+        rc = fused_load(c, mode);
+        if (rc) return rc;
+    }
+    FusedAlloc *a = falloc(c);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ms) {
+        HC(hipEventCreate(&e0));
+        HC(hipEventCreate(&e1));
+        HC(hipEventRecord(e0, c->stream));
+    }
+    for (int s = 0; s < nsteps; ++s)
+        for (int sub = 1; sub <= nstep; ++sub) {
+            FusedParams p = make_params(c, a, c->fz_cur, tau, nstep, sub);
+            launch(c, p, mode);
+            c->fz_cur = 1 - c->fz_cur;
+        }
+    HC(hipGetLastError());
+    if (ms) {
+        HC(hipEventRecord(e1, c->stream));
+        HC(hipEventSynchronize(e1));
+        float t = 0.f;
+        HC(hipEventElapsedTime(&t, e0, e1));
+        *ms = t;
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+    } else {
+#pragma clang fp contract(fast)
+        HC(hipStreamSynchronize(c->stream));
+    }
+    return INMSOM_BT_OK;
+}
+
+int fused_range_check(inmsom_bt_ctx *c, long long *bad)
+{
+    const size_t n = c->n;
+    HC(hipMemsetAsync(c->cnt, 0, sizeof(long long), c->stream));
+    unsigned long long *cnt = reinterpret_cast<unsigned long long *>(c->cnt);
+    if (c->state_on_fused) {
+        FusedAlloc *a = falloc(c);
+        const int b = c->fz_cur;
+        if (a->mode == INMSOM_BT_FP32)
+            hipLaunchKernelGGL(k_range<float>, dim3(nblk(n)), dim3(256), 0, c->stream, c->nx, c->ny,
+                               c->mask[0], c->mask[3], c->mask[4], (const float *)a->lvl[b][2],
+                               (const float *)a->lvl[b][0], (const float *)a->lvl[b][1], cnt);
+        else
+            hipLaunchKernelGGL(k_range<double>, dim3(nblk(n)), dim3(256), 0, c->stream, c->nx, c->ny,
+                               c->mask[0], c->mask[3], c->mask[4], (const double *)a->lvl[b][2],
+                               (const double *)a->lvl[b][0], (const double *)a->lvl[b][1], cnt);
+    } else {
+#pragma clang fp contract(fast)
+        hipLaunchKernelGGL(k_range<double>, dim3(nblk(n)), dim3(256), 0, c->stream, c->nx, c->ny,
+                           c->mask[0], c->mask[3], c->mask[4], c->st[4], c->st[0], c->st[2], cnt);
+    }
+    HC(hipGetLastError());
+    long long h = 0;
+    HC(hipMemcpyAsync(&h, c->cnt, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    *bad = h;
+    return INMSOM_BT_OK;
+}
+
+int fused_volume(inmsom_bt_ctx *c, double *vol)
+{
+    const int rows = c->ny - 4;
+    if (rows <= 0 || rows > DEP_TABLE_OFFSET * 1024) {
+        set_error("grid too small");
+        return INMSOM_BT_EINVAL;
+    }
+    double *rowbuf = nullptr;
+    HC(hipMallocAsync((void **)&rowbuf, (size_t)rows * 8, c->stream));
+    if (c->state_on_fused && falloc(c)->mode == INMSOM_BT_FP32)
+        hipLaunchKernelGGL(k_row_volume<float>, dim3(rows), dim3(256), 0, c->stream, c->nx, c->ny,
+                           (const float *)falloc(c)->lvl[c->fz_cur][2], c->mask[0], c->metric[0],
+                           c->metric[1], rowbuf);
+    else {
+        const double *s = c->state_on_fused ? (const double *)falloc(c)->lvl[c->fz_cur][2] : c->st[4];
+        hipLaunchKernelGGL(k_row_volume<double>, dim3(rows), dim3(256), 0, c->stream, c->nx, c->ny, s,
+                           c->mask[0], c->metric[0], c->metric[1], rowbuf);
+    }
+    hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(64), 0, c->stream, rows, rowbuf, c->scal);
+    HC(hipGetLastError());
+    double h = 0.0;
+    HC(hipMemcpyAsync(&h, c->scal, 8, hipMemcpyDeviceToHost, c->stream));
+    HC(hipFreeAsync(rowbuf, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    *vol = h;
+    return INMSOM_BT_OK;
+}
+
+}  // namespace ibt

@@ -1,0 +1,88 @@
+// Internal declarations shared by the inmsom_bt HIP sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "inmsom_bt.h"
+
+namespace ibt {
+
+// Static grid as the device kernels see it.  Indices are the reference's
+// 1-based (m, n); element (m, n) lives at (n-1)*nx + (m-1).
+struct DevGrid {
+    int nx, ny;
+    int px, py;  // periodicity flags
+    int ffs;     // full_free_surface
+    double g, rho0, ts;
+    const float *lu, *luu, *luh, *lcu, *lcv, *llu, *llv;
+    const double *dx, *dy, *dxt, *dyt, *dxh, *dyh, *dxb, *dyb;
+    const double *rlh_s, *hr;
+};
+
+__host__ __device__ inline size_t IDX(const DevGrid &g, int m, int n)
+{
+    return (size_t)(n - 1) * (size_t)g.nx + (size_t)(m - 1);
+}
+
+// Mask bits of the fused kernels (one byte per cell).
+enum : uint8_t {
+    MB_LU = 1, MB_LCU = 2, MB_LCV = 4, MB_LUU = 8, MB_LUH = 16, MB_LLU = 32, MB_LLV = 64
+};
+
+void set_error(const std::string &msg);
+
+}  // namespace ibt
+
+namespace ibt { struct FusedAlloc; }
+
+// Context: every device allocation the library makes for one basin.
+struct inmsom_bt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int nx = 0, ny = 0;
+    size_t n = 0;  // nx*ny
+    ibt::DevGrid g{};
+
+    // static grid (device)
+    float *mask[7] = {};
+    double *metric[8] = {};
+    double *rlh_s = nullptr, *hr = nullptr;
+    bool uniform_metrics = false;
+    double metric0 = 0.0;
+
+    // forcing (device)
+    double *taux = nullptr, *tauy = nullptr, *wflux = nullptr, *mu = nullptr, *mu4 = nullptr,
+           *rdis = nullptr;
+    double cb_nl = 2.5e-3, ebottom = 25.0e-4;
+    bool have_forcing = false;
+
+    // external-mode state: u, up, v, vp, ssh, sshp
+    double *st[6] = {};
+    // barotropic_dynamics working arrays: u, v, ssh, up, vp, sshp, un, vn, sshn
+    double *w[9] = {};
+    double *acc[4] = {};   // ubrtr_i, vbrtr_i, ssh4gradx, ssh4grady
+    double *dep[12] = {};  // hhq_e .. hhhn_e
+    double *vort = nullptr, *str_t = nullptr, *str_s = nullptr, *fx = nullptr, *fy = nullptr;
+    double *radv = nullptr, *rady = nullptr, *rdfx = nullptr, *rdfy = nullptr;
+    double *rhsx = nullptr, *rhsy = nullptr, *bf = nullptr;
+    double *surf_x = nullptr, *surf_y = nullptr, *bot_x = nullptr, *bot_y = nullptr;
+
+    // fused path
+    int fused_mode = -1;           // mode the fused state is held in, -1 = none
+    bool fused_ready = false;      // per-row tables built
+    bool state_on_fused = false;   // authoritative state lives in the fused buffers
+    uint8_t *mbits = nullptr;
+    double *row_f = nullptr;       // Coriolis per H row
+    double *row_wx = nullptr;      // wind stress on U faces per row (fluxes.f90:30-31)
+    double *row_wy = nullptr;      // wind stress on V faces per row
+    double nu = 0.0;               // constant viscosity on wet cells
+    double *row_wf = nullptr;      // surface water flux per T row
+    ibt::FusedAlloc *fa = nullptr;  // ping-pong state buffers of the fused step
+    int fz_cur = 0;
+    double *scal = nullptr;        // device scalars (reductions)
+    long long *cnt = nullptr;
+};

@@ -1,0 +1,19 @@
+// Host entry points of the stage-by-stage (reference-order) path.
+#pragma once
+
+#include "bt_internal.h"
+
+namespace ibt {
+
+// c->scal layout: [0, 64) reduction scratch, [64, 76) depth pointer table
+constexpr int DEP_TABLE_OFFSET = 64;
+constexpr int SCAL_DOUBLES = 128;
+
+void lit_copy(inmsom_bt_ctx *c, double *dst, const double *src);
+void lit_hh_init(inmsom_bt_ctx *c, const double *ssh, const double *sshp);
+void lit_barotropic_dynamics(inmsom_bt_ctx *c, double tau, int nstep, int ksw4, const double *RHSx,
+                             const double *RHSy, const double *wflux, const double *mu,
+                             const double *mu4, const double *rdis);
+void lit_forcing(inmsom_bt_ctx *c);
+
+}  // namespace ibt

@@ -1,0 +1,586 @@
+// Stage-by-stage device implementation of INMSOM's barotropic step.
+//
+// Each kernel is one sweep of the reference with the reference's index
+// range, mask test and operation order, so that the result is the
+// reference's bit for bit.  This file is compiled with
+// -ffp-contract=off: a fused multiply-add would round differently.
+// These kernels serve the general drop-in (any metrics, nstep, ksw4,
+// periodicity); the fused single-kernel step for the common case lives
+// in bt_fused.hip.
+#include "bt_internal.h"
+#include "bt_literal.h"
+
+namespace ibt {
+
+#define KERNEL_MN                                        \
+    const int m = blockIdx.x * blockDim.x + threadIdx.x + 1; \
+    const int n = blockIdx.y * blockDim.y + threadIdx.y + 1;
+
+#define IN_INTERIOR(g) (m >= 3 && m <= (g).nx - 2 && n >= 3 && n <= (g).ny - 2)
+#define I(mm, nn) IDX(g, (mm), (nn))
+
+// ---------------------------------------------------------------- helpers
+__global__ void k_fill(double *a, size_t n, double v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ void k_copy(double *__restrict__ dst, const double *__restrict__ src, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// Service/oc_algorithms.f90:50-95 as a decomposed run performs it:
+// column mm -> mmm-1, mmm -> mm+1 over all rows.
+__global__ void k_cyclize_x(DevGrid g, double *f)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (n > g.ny) return;
+    double a = f[I(g.nx - 2, n)], b = f[I(3, n)];
+    f[I(2, n)] = a;
+    f[I(g.nx - 1, n)] = b;
+}
+
+// Service/oc_algorithms.f90:118-135.
+__global__ void k_cyclize_y(DevGrid g, double *f)
+{
+    int m = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (m > g.nx) return;
+    double a = f[I(m, g.ny - 2)], b = f[I(m, 3)];
+    f[I(m, 2)] = a;
+    f[I(m, g.ny - 1)] = b;
+}
+
+// Depth on U, V and H points from T-point depths hq(m, n) given by Q,
+// Function/depth.f90:29-66.
+template <class Q>
+__device__ inline double avg_u(const DevGrid &g, Q q, int m, int n)
+{
+    double slu = (double)(g.lu[I(m, n)] + g.lu[I(m + 1, n)]);
+    return (q(m, n) * g.dx[I(m, n)] * g.dy[I(m, n)] * (double)g.lu[I(m, n)] +
+            q(m + 1, n) * g.dx[I(m + 1, n)] * g.dy[I(m + 1, n)] * (double)g.lu[I(m + 1, n)]) /
+           slu / g.dxt[I(m, n)] / g.dyh[I(m, n)];
+}
+
+template <class Q>
+__device__ inline double avg_v(const DevGrid &g, Q q, int m, int n)
+{
+    double slu = (double)(g.lu[I(m, n)] + g.lu[I(m, n + 1)]);
+    return (q(m, n) * g.dx[I(m, n)] * g.dy[I(m, n)] * (double)g.lu[I(m, n)] +
+            q(m, n + 1) * g.dx[I(m, n + 1)] * g.dy[I(m, n + 1)] * (double)g.lu[I(m, n + 1)]) /
+           slu / g.dxh[I(m, n)] / g.dyt[I(m, n)];
+}
+
+template <class Q>
+__device__ inline double avg_h(const DevGrid &g, Q q, int m, int n)
+{
+    double slu = (double)(g.lu[I(m, n)] + g.lu[I(m + 1, n)] + g.lu[I(m, n + 1)] +
+                          g.lu[I(m + 1, n + 1)]);
+    return (q(m, n) * g.dx[I(m, n)] * g.dy[I(m, n)] * (double)g.lu[I(m, n)] +
+            q(m + 1, n) * g.dx[I(m + 1, n)] * g.dy[I(m + 1, n)] * (double)g.lu[I(m + 1, n)] +
+            q(m, n + 1) * g.dx[I(m, n + 1)] * g.dy[I(m, n + 1)] * (double)g.lu[I(m, n + 1)] +
+            q(m + 1, n + 1) * g.dx[I(m + 1, n + 1)] * g.dy[I(m + 1, n + 1)] *
+                (double)g.lu[I(m + 1, n + 1)]) /
+           slu / g.dxb[I(m, n)] / g.dyb[I(m, n)];
+}
+
+// ------------------------------------------------------------ depth.f90
+// hh_init (Function/depth.f90:2-105): T depths over the whole array,
+// U/V/H depths over nx_start-1..nx_end x ny_start-1..ny_end.
+__global__ void k_hh_init(DevGrid g, const double *__restrict__ sh, const double *__restrict__ shp,
+                          double *const *__restrict__ d)
+{
+    KERNEL_MN
+    if (m > g.nx || n > g.ny) return;
+    const double ffs = (double)g.ffs;
+    const size_t i = I(m, n);
+    d[0][i] = g.hr[i] + sh[i] * ffs;
+    d[1][i] = g.hr[i] + shp[i] * ffs;
+    d[2][i] = g.hr[i];
+    if (m < 2 || m > g.nx - 2 || n < 2 || n > g.ny - 2) return;
+    auto q = [&](int a, int b) { size_t j = I(a, b); return g.hr[j] + sh[j] * ffs; };
+    auto qp = [&](int a, int b) { size_t j = I(a, b); return g.hr[j] + shp[j] * ffs; };
+    auto qn = [&](int a, int b) { return g.hr[I(a, b)]; };
+    if (g.llu[i] > 0.5f) {
+        d[3][i] = avg_u(g, q, m, n);
+        d[4][i] = avg_u(g, qp, m, n);
+        d[5][i] = avg_u(g, qn, m, n);
+    }
+    if (g.llv[i] > 0.5f) {
+        d[6][i] = avg_v(g, q, m, n);
+        d[7][i] = avg_v(g, qp, m, n);
+        d[8][i] = avg_v(g, qn, m, n);
+    }
+    if (g.luh[i] > 0.5f) {
+        d[9][i] = avg_h(g, q, m, n);
+        d[10][i] = avg_h(g, qp, m, n);
+        d[11][i] = avg_h(g, qn, m, n);
+    }
+}
+
+// hh_update (Function/depth.f90:108-170)
+__global__ void k_hh_update(DevGrid g, const double *__restrict__ sh, double *__restrict__ hqn,
+                            double *__restrict__ hun, double *__restrict__ hvn,
+                            double *__restrict__ hhn)
+{
+    KERNEL_MN
+    if (m > g.nx || n > g.ny) return;
+    const size_t i = I(m, n);
+    hqn[i] = g.hr[i] + sh[i];
+    if (m < 2 || m > g.nx - 2 || n < 2 || n > g.ny - 2) return;
+    auto q = [&](int a, int b) { size_t j = I(a, b); return g.hr[j] + sh[j]; };
+    if (g.llu[i] > 0.5f) hun[i] = avg_u(g, q, m, n);
+    if (g.llv[i] > 0.5f) hvn[i] = avg_v(g, q, m, n);
+    if (g.luh[i] > 0.5f) hhn[i] = avg_h(g, q, m, n);
+}
+
+// time-level shift of ssh/u/v with the Robert-Asselin filter
+// (Function/vel_ssh.f90:588-612) and of the depths (depth.f90:174-220),
+// over nx_start-1..nx_end+1 x ny_start-1..ny_end+1.
+__global__ void k_shift(DevGrid g, double dn, int with_depths, double *__restrict__ ssh,
+                        double *__restrict__ sshp, const double *__restrict__ sshn,
+                        double *__restrict__ u, double *__restrict__ up, const double *__restrict__ un,
+                        double *__restrict__ v, double *__restrict__ vp, const double *__restrict__ vn,
+                        double *const *__restrict__ d)
+{
+    KERNEL_MN
+    if (m < 2 || m > g.nx - 1 || n < 2 || n > g.ny - 1) return;
+    const size_t i = I(m, n);
+    const double ts = g.ts;
+    if (g.lu[i] > 0.5f) {
+        sshp[i] = ssh[i] + ts * (sshn[i] - 2.0 * ssh[i] + sshp[i]) / 2.0 / dn;
+        ssh[i] = sshn[i];
+    }
+    if (g.lcu[i] > 0.5f) {
+        up[i] = u[i] + ts * (un[i] - 2.0 * u[i] + up[i]) / 2.0 / dn;
+        u[i] = un[i];
+    }
+    if (g.lcv[i] > 0.5f) {
+        vp[i] = v[i] + ts * (vn[i] - 2.0 * v[i] + vp[i]) / 2.0 / dn;
+        v[i] = vn[i];
+    }
+    if (!with_depths) return;
+    auto shift = [&](int a) {
+        double *h = d[a], *hp = d[a + 1], *hn = d[a + 2];
+        hp[i] = h[i] + ts * (hn[i] - 2.0 * h[i] + hp[i]) / 2.0 / dn;
+        h[i] = hn[i];
+    };
+    if (g.llu[i] > 0.5f) shift(3);
+    if (g.llv[i] > 0.5f) shift(6);
+    if (g.lu[i] > 0.5f) shift(0);
+    if (g.luh[i] > 0.5f) shift(9);
+}
+
+// ------------------------------------------------------------ vel_ssh.f90
+// continuity (Function/vel_ssh.f90:487-499)
+__global__ void k_continuity(DevGrid g, double tau_inner, const double *__restrict__ wflux,
+                             const double *__restrict__ u, const double *__restrict__ v,
+                             const double *__restrict__ hhu, const double *__restrict__ hhv,
+                             const double *__restrict__ sshp, double *__restrict__ sshn)
+{
+    KERNEL_MN
+    if (!IN_INTERIOR(g)) return;
+    const size_t i = I(m, n);
+    if (g.lu[i] > 0.5f)
+        sshn[i] = sshp[i] +
+                  2.0 * tau_inner *
+                      (wflux[i] / g.rho0 * (double)g.ffs -
+                       (u[i] * hhu[i] * g.dyh[i] - u[I(m - 1, n)] * hhu[I(m - 1, n)] * g.dyh[I(m - 1, n)] +
+                        v[i] * hhv[i] * g.dxh[i] - v[I(m, n - 1)] * hhv[I(m, n - 1)] * g.dxh[I(m, n - 1)]) /
+                           (g.dx[i] * g.dy[i]));
+}
+
+// stress_components (Function/mixing.f90:115-168)
+__global__ void k_stress(DevGrid g, const double *__restrict__ u, const double *__restrict__ v,
+                         double *__restrict__ str_t, double *__restrict__ str_s)
+{
+    KERNEL_MN
+    if (!IN_INTERIOR(g)) return;
+    const size_t i = I(m, n);
+    if (g.lu[i] > 0.5f)
+        str_t[i] = g.dy[i] / g.dx[i] * (u[i] / g.dyh[i] - u[I(m - 1, n)] / g.dyh[I(m - 1, n)]) -
+                   g.dx[i] / g.dy[i] * (v[i] / g.dxh[i] - v[I(m, n - 1)] / g.dxh[I(m, n - 1)]);
+    if (g.luu[i] > 0.5f)
+        str_s[i] = g.dxb[i] / g.dyb[i] * (u[I(m, n + 1)] / g.dxt[I(m, n + 1)] - u[i] / g.dxt[i]) +
+                   g.dyb[i] / g.dxb[i] * (v[I(m + 1, n)] / g.dyt[I(m + 1, n)] - v[i] / g.dyt[i]);
+}
+
+// metric vorticity of uv_trans (Function/vel_ssh.f90:30-42)
+__global__ void k_vort(DevGrid g, const double *__restrict__ u, const double *__restrict__ v,
+                       double *__restrict__ vort)
+{
+    KERNEL_MN
+    if (!IN_INTERIOR(g)) return;
+    const size_t i = I(m, n);
+    if (g.luu[i] > 0.5f)
+        vort[i] = (v[I(m + 1, n)] * g.dyt[I(m + 1, n)] - v[i] * g.dyt[i]) -
+                  (u[I(m, n + 1)] * g.dxt[I(m, n + 1)] - u[i] * g.dxt[i]) -
+                  ((v[I(m + 1, n)] - v[i]) * g.dyb[i] - (u[I(m, n + 1)] - u[i]) * g.dxb[i]);
+}
+
+// Tension part p and shear part q of the viscous stencil,
+// Function/vel_ssh.f90:150-156 (u) and :166-172 (v).
+__device__ inline void visc_u(const DevGrid &g, const double *mu, const double *str_t,
+                              const double *str_s, const double *hq, const double *hh, int m, int n,
+                              double &p, double &q)
+{
+    double muh_p = (mu[I(m, n)] + mu[I(m + 1, n)] + mu[I(m, n + 1)] + mu[I(m + 1, n + 1)]) / 4.0;
+    double muh_m = (mu[I(m, n)] + mu[I(m + 1, n)] + mu[I(m, n - 1)] + mu[I(m + 1, n - 1)]) / 4.0;
+    double dy1 = g.dy[I(m + 1, n)], dy0 = g.dy[I(m, n)];
+    double b1 = g.dxb[I(m, n)], b0 = g.dxb[I(m, n - 1)];
+    p = (dy1 * dy1 * mu[I(m + 1, n)] * hq[I(m + 1, n)] * str_t[I(m + 1, n)] -
+         dy0 * dy0 * mu[I(m, n)] * hq[I(m, n)] * str_t[I(m, n)]) / g.dyh[I(m, n)];
+    q = (b1 * b1 * muh_p * hh[I(m, n)] * str_s[I(m, n)] -
+         b0 * b0 * muh_m * hh[I(m, n - 1)] * str_s[I(m, n - 1)]) / g.dxt[I(m, n)];
+}
+
+__device__ inline void visc_v(const DevGrid &g, const double *mu, const double *str_t,
+                              const double *str_s, const double *hq, const double *hh, int m, int n,
+                              double &p, double &q)
+{
+    double muh_p = (mu[I(m, n)] + mu[I(m + 1, n)] + mu[I(m, n + 1)] + mu[I(m + 1, n + 1)]) / 4.0;
+    double muh_m = (mu[I(m, n)] + mu[I(m - 1, n)] + mu[I(m, n + 1)] + mu[I(m - 1, n + 1)]) / 4.0;
+    double dx1 = g.dx[I(m, n + 1)], dx0 = g.dx[I(m, n)];
+    double b1 = g.dyb[I(m, n)], b0 = g.dyb[I(m - 1, n)];
+    p = (dx1 * dx1 * mu[I(m, n + 1)] * hq[I(m, n + 1)] * str_t[I(m, n + 1)] -
+         dx0 * dx0 * mu[I(m, n)] * hq[I(m, n)] * str_t[I(m, n)]) / g.dxh[I(m, n)];
+    q = (b1 * b1 * muh_p * hh[I(m, n)] * str_s[I(m, n)] -
+         b0 * b0 * muh_m * hh[I(m - 1, n)] * str_s[I(m - 1, n)]) / g.dyt[I(m, n)];
+}
+
+// flux-form advection of uv_trans (Function/vel_ssh.f90:54-109) and the
+// harmonic viscosity of uv_diff2 (:141-179) in one sweep.
+__global__ void k_trans_diff2(DevGrid g, const double *__restrict__ u, const double *__restrict__ v,
+                              const double *__restrict__ vort, const double *__restrict__ hq,
+                              const double *__restrict__ hu, const double *__restrict__ hv,
+                              const double *__restrict__ hh, const double *__restrict__ mu,
+                              const double *__restrict__ str_t, const double *__restrict__ str_s,
+                              double *__restrict__ radv, double *__restrict__ rady,
+                              double *__restrict__ rdfx, double *__restrict__ rdfy)
+{
+    KERNEL_MN
+    if (!IN_INTERIOR(g)) return;
+    const size_t i = I(m, n);
+    const double *dyh = g.dyh, *dxh = g.dxh;
+    if (g.lcu[i] > 0.5f) {
+        double fx_p = (u[i] * dyh[i] * hu[i] + u[I(m + 1, n)] * dyh[I(m + 1, n)] * hu[I(m + 1, n)]) /
+                      2.0 * (u[i] + u[I(m + 1, n)]) / 2.0;
+        double fx_m = (u[i] * dyh[i] * hu[i] + u[I(m - 1, n)] * dyh[I(m - 1, n)] * hu[I(m - 1, n)]) /
+                      2.0 * (u[i] + u[I(m - 1, n)]) / 2.0;
+        double fy_p = (v[i] * dxh[i] * hv[i] + v[I(m + 1, n)] * dxh[I(m + 1, n)] * hv[I(m + 1, n)]) /
+                      2.0 * (u[I(m, n + 1)] + u[i]) / 2.0 * (double)g.luu[i];
+        double fy_m = (v[I(m, n - 1)] * dxh[I(m, n - 1)] * hv[I(m, n - 1)] +
+                       v[I(m + 1, n - 1)] * dxh[I(m + 1, n - 1)] * hv[I(m + 1, n - 1)]) /
+                      2.0 * (u[I(m, n - 1)] + u[i]) / 2.0 * (double)g.luu[I(m, n - 1)];
+        radv[i] = -(fx_p - fx_m + fy_p - fy_m) +
+                  (vort[i] * hh[i] * (v[I(m + 1, n)] + v[i]) +
+                   vort[I(m, n - 1)] * hh[I(m, n - 1)] * (v[I(m + 1, n - 1)] + v[I(m, n - 1)])) / 4.0;
+        double p, q;
+        visc_u(g, mu, str_t, str_s, hq, hh, m, n, p, q);
+        rdfx[i] = p + q;
+    }
+    if (g.lcv[i] > 0.5f) {
+        double fy_p = (v[i] * dxh[i] * hv[i] + v[I(m, n + 1)] * dxh[I(m, n + 1)] * hv[I(m, n + 1)]) /
+                      2.0 * (v[i] + v[I(m, n + 1)]) / 2.0;
+        double fy_m = (v[i] * dxh[i] * hv[i] + v[I(m, n - 1)] * dxh[I(m, n - 1)] * hv[I(m, n - 1)]) /
+                      2.0 * (v[i] + v[I(m, n - 1)]) / 2.0;
+        double fx_p = (u[i] * dyh[i] * hu[i] + u[I(m, n + 1)] * dyh[I(m, n + 1)] * hu[I(m, n + 1)]) /
+                      2.0 * (v[I(m + 1, n)] + v[i]) / 2.0;
+        double fx_m = (u[I(m - 1, n)] * dyh[I(m - 1, n)] * hu[I(m - 1, n)] +
+                       u[I(m - 1, n + 1)] * dyh[I(m - 1, n + 1)] * hu[I(m - 1, n + 1)]) /
+                      2.0 * (v[I(m - 1, n)] + v[i]) / 2.0;
+        rady[i] = -(fx_p - fx_m + fy_p - fy_m) -
+                  (vort[i] * hh[i] * (u[I(m, n + 1)] + u[i]) +
+                   vort[I(m - 1, n)] * hh[I(m - 1, n)] * (u[I(m - 1, n + 1)] + u[I(m - 1, n)])) / 4.0;
+        double p, q;
+        visc_v(g, mu, str_t, str_s, hq, hh, m, n, p, q);
+        rdfy[i] = -p + q;
+    }
+}
+
+// first sweep of uv_diff4 (Function/vel_ssh.f90:216-257); fx, fy are
+// zero-filled beforehand (:213-214).
+__global__ void k_diff4_f(DevGrid g, const double *__restrict__ mu, const double *__restrict__ str_t,
+                          const double *__restrict__ str_s, const double *__restrict__ hq,
+                          const double *__restrict__ hu, const double *__restrict__ hv,
+                          const double *__restrict__ hh, double *__restrict__ fx,
+                          double *__restrict__ fy)
+{
+    KERNEL_MN
+    if (!IN_INTERIOR(g)) return;
+    const size_t i = I(m, n);
+    double p, q;
+    if (g.lcu[i] > 0.5f) {
+        visc_u(g, mu, str_t, str_s, hq, hh, m, n, p, q);
+        double f = p + q;
+        fx[i] = -f / hu[i] / g.dxt[i] / g.dyh[i];
+    }
+    if (g.lcv[i] > 0.5f) {
+        visc_v(g, mu, str_t, str_s, hq, hh, m, n, p, q);
+        double f = -p + q;
+        fy[i] = -f / hv[i] / g.dxh[i] / g.dyt[i];
+    }
+}
+
+// last sweep of uv_diff4 (Function/vel_ssh.f90:274-312)
+__global__ void k_diff4_add(DevGrid g, const double *__restrict__ mu, const double *__restrict__ str_t,
+                            const double *__restrict__ str_s, const double *__restrict__ hq,
+                            const double *__restrict__ hh, double *__restrict__ rdfx,
+                            double *__restrict__ rdfy)
+{
+    KERNEL_MN
+    if (!IN_INTERIOR(g)) return;
+    const size_t i = I(m, n);
+    double p, q;
+    if (g.lcu[i] > 0.5f) {
+        visc_u(g, mu, str_t, str_s, hq, hh, m, n, p, q);
+        rdfx[i] = rdfx[i] + p + q;
+    }
+    if (g.lcv[i] > 0.5f) {
+        visc_v(g, mu, str_t, str_s, hq, hh, m, n, p, q);
+        rdfy[i] = rdfy[i] - p + q;
+    }
+}
+
+// momentum with Coriolis, pressure gradient, Rayleigh friction and the
+// time-mean accumulators (Function/vel_ssh.f90:531-573)
+struct MomArgs {
+    const double *RHSx, *RHSy, *rdis, *rdfx, *rdfy, *radv, *rady;
+    const double *u, *v, *up, *vp, *ssh, *sshn;
+    const double *hhu, *hhup, *hhun, *hhv, *hhvp, *hhvn, *hhh;
+    double *un, *vn, *ubrtr_i, *vbrtr_i, *ssh4gx, *ssh4gy;
+};
+
+__global__ void k_momentum(DevGrid g, double tau_inner, double d4n, MomArgs a)
+{
+    KERNEL_MN
+    if (!IN_INTERIOR(g)) return;
+    const size_t i = I(m, n);
+    const double G = g.g;
+    if (g.lcu[i] > 0.5f) {
+        double bp = a.hhun[i] * g.dxt[i] * g.dyh[i] / 2.0 / tau_inner;
+        double bp0 = a.hhup[i] * g.dxt[i] * g.dyh[i] / 2.0 / tau_inner;
+        double slx = -G * (a.ssh[I(m + 1, n)] - a.ssh[i]) * g.dyh[i] * a.hhu[i];
+        double slxn = -G * (a.sshn[I(m + 1, n)] - a.sshn[i]) * g.dyh[i] * a.hhun[i];
+        double grx = a.RHSx[i] + slx + a.rdfx[i] + a.radv[i] -
+                     (a.rdis[i] + a.rdis[I(m + 1, n)]) / 2.0 * a.up[i] * g.dxt[i] * g.dyh[i] *
+                         a.hhu[i] +
+                     (g.rlh_s[i] * a.hhh[i] * g.dxb[i] * g.dyb[i] * (a.v[I(m + 1, n)] + a.v[i]) +
+                      g.rlh_s[I(m, n - 1)] * a.hhh[I(m, n - 1)] * g.dxb[I(m, n - 1)] *
+                          g.dyb[I(m, n - 1)] * (a.v[I(m + 1, n - 1)] + a.v[I(m, n - 1)])) /
+                         4.0;
+        double unew = (a.up[i] * bp0 + grx) / bp;
+        a.un[i] = unew;
+        a.ubrtr_i[i] = a.ubrtr_i[i] + (a.u[i] * a.hhu[i] + unew * a.hhun[i]) / d4n;
+        a.ssh4gx[i] = a.ssh4gx[i] + (slx + slxn) / d4n;
+    }
+    if (g.lcv[i] > 0.5f) {
+        double bp = a.hhvn[i] * g.dyt[i] * g.dxh[i] / 2.0 / tau_inner;
+        double bp0 = a.hhvp[i] * g.dyt[i] * g.dxh[i] / 2.0 / tau_inner;
+        double sly = -G * (a.ssh[I(m, n + 1)] - a.ssh[i]) * g.dxh[i] * a.hhv[i];
+        double slyn = -G * (a.sshn[I(m, n + 1)] - a.sshn[i]) * g.dxh[i] * a.hhvn[i];
+        double gry = a.RHSy[i] + sly + a.rdfy[i] + a.rady[i] -
+                     (a.rdis[i] + a.rdis[I(m, n + 1)]) / 2.0 * a.vp[i] * g.dxh[i] * g.dyt[i] *
+                         a.hhv[i] -
+                     (g.rlh_s[i] * a.hhh[i] * g.dxb[i] * g.dyb[i] * (a.u[I(m, n + 1)] + a.u[i]) +
+                      g.rlh_s[I(m - 1, n)] * a.hhh[I(m - 1, n)] * g.dxb[I(m - 1, n)] *
+                          g.dyb[I(m - 1, n)] * (a.u[I(m - 1, n + 1)] + a.u[I(m - 1, n)])) /
+                         4.0;
+        double vnew = (a.vp[i] * bp0 + gry) / bp;
+        a.vn[i] = vnew;
+        a.vbrtr_i[i] = a.vbrtr_i[i] + (a.v[i] * a.hhv[i] + vnew * a.hhvn[i]) / d4n;
+        a.ssh4gy[i] = a.ssh4gy[i] + (sly + slyn) / d4n;
+    }
+}
+
+// ------------------------------------------------------------ fluxes.f90
+// wind stress on U/V faces (Function/fluxes.f90:29-37)
+__global__ void k_surface_stress(DevGrid g, const double *__restrict__ taux,
+                                 const double *__restrict__ tauy, double *__restrict__ sx,
+                                 double *__restrict__ sy)
+{
+    KERNEL_MN
+    if (!IN_INTERIOR(g)) return;
+    const size_t i = I(m, n);
+    if (g.lcu[i] > 0.5f)
+        sx[i] = (taux[i] * g.dx[i] * g.dy[i] + taux[I(m + 1, n)] * g.dx[I(m + 1, n)] * g.dy[I(m + 1, n)]) /
+                g.rho0 / 2.0 / g.dxt[i] / g.dyh[i];
+    if (g.lcv[i] > 0.5f)
+        sy[i] = (tauy[i] * g.dx[i] * g.dy[i] + tauy[I(m, n + 1)] * g.dx[I(m, n + 1)] * g.dy[I(m, n + 1)]) /
+                g.rho0 / 2.0 / g.dxh[i] / g.dyt[i];
+}
+
+// quadratic bottom friction rate on T points (Function/fluxes.f90:231-243)
+__global__ void k_bottom_friction(DevGrid g, double cb_nl, double ebottom,
+                                  const double *__restrict__ uu, const double *__restrict__ vv,
+                                  const double *__restrict__ hhu, const double *__restrict__ hhv,
+                                  const double *__restrict__ hhq, double *__restrict__ bf)
+{
+    KERNEL_MN
+    if (!IN_INTERIOR(g)) return;
+    const size_t i = I(m, n);
+    if (g.lu[i] > 0.5f) {
+        double ut = (uu[i] * g.dxt[i] * g.dyh[i] * hhu[i] +
+                     uu[I(m - 1, n)] * g.dxt[I(m - 1, n)] * g.dyh[I(m - 1, n)] * hhu[I(m - 1, n)]) /
+                    2.0 / g.dx[i] / g.dy[i] / hhq[i];
+        double vt = (vv[i] * g.dxh[i] * g.dyt[i] * hhv[i] +
+                     vv[I(m, n - 1)] * g.dxh[I(m, n - 1)] * g.dyt[I(m, n - 1)] * hhv[I(m, n - 1)]) /
+                    2.0 / g.dx[i] / g.dy[i] / hhq[i];
+        bf[i] = cb_nl * sqrt(ut * ut + vt * vt + ebottom);
+    }
+}
+
+// bottom stress (Function/fluxes.f90:261-272) and the 2-D RHS assembly
+// of Control/dyncall.f90:103-126 with every 3-D term zero.
+__global__ void k_bottom_stress_rhs(DevGrid g, const double *__restrict__ bf,
+                                    const double *__restrict__ uup, const double *__restrict__ vvp,
+                                    const double *__restrict__ sx, const double *__restrict__ sy,
+                                    double *__restrict__ bx, double *__restrict__ by,
+                                    double *__restrict__ rhsx, double *__restrict__ rhsy)
+{
+    KERNEL_MN
+    if (m > g.nx || n > g.ny) return;
+    const size_t i = I(m, n);
+    if (g.llu[i] > 0.5f) rhsx[i] = 0.0;
+    if (g.llv[i] > 0.5f) rhsy[i] = 0.0;
+    if (!IN_INTERIOR(g)) return;
+    if (g.lcu[i] > 0.5f) {
+        bx[i] = -(bf[i] + bf[I(m + 1, n)]) / 2.0 * uup[i];
+        rhsx[i] = rhsx[i] + (sx[i] + bx[i]) * g.dxt[i] * g.dyh[i];
+    }
+    if (g.lcv[i] > 0.5f) {
+        by[i] = -(bf[i] + bf[I(m, n + 1)]) / 2.0 * vvp[i];
+        rhsy[i] = rhsy[i] + (sy[i] + by[i]) * g.dyt[i] * g.dxh[i];
+    }
+}
+
+// ------------------------------------------------------------ host side
+static inline dim3 grid2(const DevGrid &g) { return dim3((g.nx + 63) / 64, (g.ny + 3) / 4); }
+static const dim3 BLK2(64, 4);
+
+static void fill(inmsom_bt_ctx *c, double *a, double v)
+{
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0, c->stream, a, c->n, v);
+}
+
+void lit_copy(inmsom_bt_ctx *c, double *dst, const double *src)
+{
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0, c->stream, dst, src, c->n);
+}
+
+static void cyclize(inmsom_bt_ctx *c, double *f)
+{
+    if (c->g.px)
+        hipLaunchKernelGGL(k_cyclize_x, dim3((c->ny + 255) / 256), dim3(256), 0, c->stream, c->g, f);
+    if (c->g.py)
+        hipLaunchKernelGGL(k_cyclize_y, dim3((c->nx + 255) / 256), dim3(256), 0, c->stream, c->g, f);
+}
+
+// Device array of the 12 depth pointers (k_hh_init and k_shift index
+// it); written once when the context is created.
+static inline double *const *dep_table(inmsom_bt_ctx *c)
+{
+    return reinterpret_cast<double *const *>(c->scal + DEP_TABLE_OFFSET);
+}
+
+void lit_hh_init(inmsom_bt_ctx *c, const double *ssh, const double *sshp)
+{
+    hipLaunchKernelGGL(k_hh_init, grid2(c->g), BLK2, 0, c->stream, c->g, ssh, sshp, dep_table(c));
+    for (int k = 3; k < 12; ++k) cyclize(c, c->dep[k]);
+}
+
+// barotropic_dynamics (Function/vel_ssh.f90:379-679) on the context's
+// device arrays: state c->st, accumulators c->acc, depths c->dep.
+void lit_barotropic_dynamics(inmsom_bt_ctx *c, double tau, int nstep, int ksw4, const double *RHSx,
+                             const double *RHSy, const double *wflux, const double *mu,
+                             const double *mu4, const double *rdis)
+{
+    const DevGrid &g = c->g;
+    double *u = c->w[0], *v = c->w[1], *ssh = c->w[2], *up = c->w[3], *vp = c->w[4];
+    double *sshp = c->w[5], *un = c->w[6], *vn = c->w[7], *sshn = c->w[8];
+    double **d = c->dep;
+    const double tau_inner = tau / (double)nstep;
+    const double d4n = (double)(4 * nstep), dn = (double)nstep;
+    double *const *dt = dep_table(c);
+
+    fill(c, un, 0.0);
+    fill(c, vn, 0.0);
+    fill(c, sshn, 0.0);
+    lit_copy(c, up, c->st[1]);
+    lit_copy(c, vp, c->st[3]);
+    lit_copy(c, sshp, c->st[5]);
+    lit_copy(c, u, c->st[0]);
+    lit_copy(c, v, c->st[2]);
+    lit_copy(c, ssh, c->st[4]);
+    for (int k = 0; k < 4; ++k) fill(c, c->acc[k], 0.0);
+
+    for (int step = 1; step <= 2 * nstep; ++step) {
+        hipLaunchKernelGGL(k_continuity, grid2(g), BLK2, 0, c->stream, g, tau_inner, wflux, u, v, d[3],
+                           d[6], sshp, sshn);
+        cyclize(c, sshn);
+        if (g.ffs > 0) {
+            hipLaunchKernelGGL(k_hh_update, grid2(g), BLK2, 0, c->stream, g, sshn, d[2], d[5], d[8],
+                               d[11]);
+            cyclize(c, d[5]);
+            cyclize(c, d[8]);
+            cyclize(c, d[11]);
+        }
+        hipLaunchKernelGGL(k_stress, grid2(g), BLK2, 0, c->stream, g, up, vp, c->str_t, c->str_s);
+        cyclize(c, c->str_t);
+        cyclize(c, c->str_s);
+        hipLaunchKernelGGL(k_vort, grid2(g), BLK2, 0, c->stream, g, u, v, c->vort);
+        cyclize(c, c->vort);
+        hipLaunchKernelGGL(k_trans_diff2, grid2(g), BLK2, 0, c->stream, g, u, v, c->vort, d[0], d[3],
+                           d[6], d[9], mu, c->str_t, c->str_s, c->radv, c->rady, c->rdfx, c->rdfy);
+        if (ksw4 > 0) {
+            fill(c, c->fx, 0.0);
+            fill(c, c->fy, 0.0);
+            hipLaunchKernelGGL(k_diff4_f, grid2(g), BLK2, 0, c->stream, g, mu4, c->str_t, c->str_s,
+                               d[0], d[3], d[6], d[9], c->fx, c->fy);
+            cyclize(c, c->fx);
+            cyclize(c, c->fy);
+            hipLaunchKernelGGL(k_stress, grid2(g), BLK2, 0, c->stream, g, c->fx, c->fy, c->str_t,
+                               c->str_s);
+            cyclize(c, c->str_t);
+            cyclize(c, c->str_s);
+            hipLaunchKernelGGL(k_diff4_add, grid2(g), BLK2, 0, c->stream, g, mu4, c->str_t, c->str_s,
+                               d[0], d[9], c->rdfx, c->rdfy);
+        }
+        MomArgs a{RHSx, RHSy, rdis, c->rdfx, c->rdfy, c->radv, c->rady, u, v, up, vp, ssh, sshn,
+                  d[3], d[4], d[5], d[6], d[7], d[8], d[9], un, vn, c->acc[0], c->acc[1],
+                  c->acc[2], c->acc[3]};
+        hipLaunchKernelGGL(k_momentum, grid2(g), BLK2, 0, c->stream, g, tau_inner, d4n, a);
+        cyclize(c, un);
+        cyclize(c, vn);
+        hipLaunchKernelGGL(k_shift, grid2(g), BLK2, 0, c->stream, g, dn, g.ffs > 0 ? 1 : 0, ssh, sshp,
+                           sshn, u, up, un, v, vp, vn, dt);
+        if (step == nstep) {
+            lit_copy(c, c->st[1], up);
+            lit_copy(c, c->st[3], vp);
+            lit_copy(c, c->st[5], sshp);
+            lit_copy(c, c->st[0], u);
+            lit_copy(c, c->st[2], v);
+            lit_copy(c, c->st[4], ssh);
+        }
+    }
+    cyclize(c, c->acc[0]);
+    cyclize(c, c->acc[1]);
+    if (g.ffs > 0) lit_hh_init(c, c->st[4], c->st[5]);
+}
+
+// wind stress, bottom drag and RHS assembly of one 2-D model step
+void lit_forcing(inmsom_bt_ctx *c)
+{
+    const DevGrid &g = c->g;
+    hipLaunchKernelGGL(k_surface_stress, grid2(g), BLK2, 0, c->stream, g, c->taux, c->tauy, c->surf_x,
+                       c->surf_y);
+    hipLaunchKernelGGL(k_bottom_friction, grid2(g), BLK2, 0, c->stream, g, c->cb_nl, c->ebottom,
+                       c->st[0], c->st[2], c->dep[3], c->dep[6], c->dep[0], c->bf);
+    cyclize(c, c->bf);
+    hipLaunchKernelGGL(k_bottom_stress_rhs, grid2(g), BLK2, 0, c->stream, g, c->bf, c->st[1], c->st[3],
+                       c->surf_x, c->surf_y, c->bot_x, c->bot_y, c->rhsx, c->rhsy);
+}
+
+}  // namespace ibt

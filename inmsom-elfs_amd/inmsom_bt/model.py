@@ -1,0 +1,206 @@
+"""Host-side mirror of the reference's barotropic interface.
+
+``BarotropicModel`` owns one device context (one basin on one GPU).  Its
+methods follow the reference's routines:
+
+* ``barotropic_dynamics(...)``  - Function/vel_ssh.f90:379-679, same
+  argument order and meaning, numpy arrays updated in place;
+* ``model_steps(n, nstep, ksw4)`` - the 2-D part of
+  Control/dyncall.f90:ocean_model_step (wind stress, bottom drag, RHS
+  assembly, barotropic_dynamics), n times, in the reference's operation
+  order;
+* ``run(n, mode)`` - the same step with nstep = 1 through one fused kernel
+  per step (modes 'fp64_exact', 'fp64', 'fp32').
+
+All numpy fields are float64 (ny, nx) C-order arrays, i.e. the reference's
+column-major (m, n) layout (see inmsom_bt.cases).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+from . import _capi
+from ._capi import check, lib
+from .cases import Basin
+
+MASKS = ("lu", "luu", "luh", "lcu", "lcv", "llu", "llv")
+METRICS = ("dx", "dy", "dxt", "dyt", "dxh", "dyh", "dxb", "dyb")
+STATE = ("u", "up", "v", "vp", "ssh", "sshp")
+DEPTHS = ("hhq_e", "hhqp_e", "hhqn_e", "hhu_e", "hhup_e", "hhun_e",
+          "hhv_e", "hhvp_e", "hhvn_e", "hhh_e", "hhhp_e", "hhhn_e")
+
+
+def _d(a: np.ndarray):
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise TypeError("expected a C-contiguous float64 array")
+    return a.ctypes.data_as(_capi._dp)
+
+
+def _f(a: np.ndarray):
+    return a.ctypes.data_as(_capi._fp)
+
+
+def gridcon(lbasins: np.ndarray, periodicity=(0, 0)) -> Dict[str, np.ndarray]:
+    """Service/grid_construction.f90:33-133 (host code of the library)."""
+    ny, nx = lbasins.shape
+    lb = np.ascontiguousarray(lbasins, dtype=np.int32)
+    out = {k: np.zeros((ny, nx), dtype=np.float32) for k in MASKS}
+    check(lib().inmsom_bt_gridcon(nx, ny, periodicity[0], periodicity[1],
+                                  lb.ctypes.data_as(_capi._ip), *[_f(out[k]) for k in MASKS]),
+          "inmsom_bt_gridcon")
+    return out
+
+
+def _grid_struct(b: Basin, keep: list, periodicity=(0, 0), full_free_surface: int = 1,
+                 masks: Optional[Dict[str, np.ndarray]] = None) -> _capi.Grid:
+    g = _capi.Grid()
+    g.nx, g.ny = b.nx, b.ny
+    g.periodicity_x, g.periodicity_y = periodicity
+    g.full_free_surface = full_free_surface
+    g.free_fall_acc, g.ref_den, g.time_smooth = b.g, b.rho0, b.time_smooth
+    lb = np.ascontiguousarray(b.lbasins, dtype=np.int32)
+    keep.append(lb)
+    g.lbasins = lb.ctypes.data_as(_capi._ip)
+    if masks is not None:
+        for k in MASKS:
+            a = np.ascontiguousarray(masks[k], dtype=np.float32)
+            keep.append(a)
+            setattr(g, k, _f(a))
+    for k in METRICS + ("rlh_s", "hhq_rest"):
+        a = np.ascontiguousarray(getattr(b, k), dtype=np.float64)
+        keep.append(a)
+        setattr(g, k, _d(a))
+    return g
+
+
+def rayleigh(b: Basin, r_fric: Optional[float] = None, periodicity=(0, 0)) -> np.ndarray:
+    """Control/init_pars.f90:459-481 (host code of the library)."""
+    keep: list = []
+    g = _grid_struct(b, keep, periodicity)
+    out = np.zeros((b.ny, b.nx))
+    check(lib().inmsom_bt_rayleigh(C.byref(g), b.r_fric if r_fric is None else r_fric, _d(out)),
+          "inmsom_bt_rayleigh")
+    return out
+
+
+class BarotropicModel:
+    """One basin resident on one GPU."""
+
+    def __init__(self, b: Basin, device: int = 0, rdis: Optional[np.ndarray] = None,
+                 periodicity=(0, 0), full_free_surface: int = 1,
+                 masks: Optional[Dict[str, np.ndarray]] = None):
+        self.b = b
+        self.nx, self.ny = b.nx, b.ny
+        self._keep: list = []
+        g = _grid_struct(b, self._keep, periodicity, full_free_surface, masks)
+        h = C.c_void_p()
+        check(lib().inmsom_bt_init(C.byref(g), device, C.byref(h)), "inmsom_bt_init")
+        self._h = h
+        self.rdis = rayleigh(b, periodicity=periodicity) if rdis is None else np.ascontiguousarray(rdis)
+        self.set_forcing()
+        self.set_state(b.u, b.up, b.v, b.vp, b.ssh, b.sshp)
+
+    # ------------------------------------------------------------ lifecycle
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().inmsom_bt_finalize(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------ state
+    def set_forcing(self, taux=None, tauy=None, wflux=None, mu=None, mu4=None, rdis=None,
+                    cb_nl=None, ebottom=None):
+        b = self.b
+        arrs = {k: np.ascontiguousarray(v if v is not None else getattr(b, k), dtype=np.float64)
+                for k, v in (("taux", taux), ("tauy", tauy), ("wflux", wflux), ("mu", mu),
+                             ("mu4", mu4))}
+        arrs["rdis"] = np.ascontiguousarray(self.rdis if rdis is None else rdis, dtype=np.float64)
+        f = _capi.Forcing()
+        for k, a in arrs.items():
+            setattr(f, k, _d(a))
+        f.cb_nl = b.cb_nl if cb_nl is None else cb_nl
+        f.ebottom = b.ebottom if ebottom is None else ebottom
+        check(lib().inmsom_bt_set_forcing(self._h, C.byref(f)), "inmsom_bt_set_forcing")
+        self._forcing = arrs
+
+    def set_state(self, u, up, v, vp, ssh, sshp):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (u, up, v, vp, ssh, sshp)]
+        check(lib().inmsom_bt_set_state(self._h, *[_d(x) for x in a]), "inmsom_bt_set_state")
+
+    def get_state(self) -> Dict[str, np.ndarray]:
+        out = {k: np.empty((self.ny, self.nx)) for k in STATE}
+        check(lib().inmsom_bt_get_state(self._h, *[_d(out[k]) for k in STATE]), "inmsom_bt_get_state")
+        return out
+
+    def means(self) -> Dict[str, np.ndarray]:
+        keys = ("ubrtr_i", "vbrtr_i", "ssh4gradx", "ssh4grady")
+        out = {k: np.empty((self.ny, self.nx)) for k in keys}
+        check(lib().inmsom_bt_get_means(self._h, *[_d(out[k]) for k in keys]), "inmsom_bt_get_means")
+        return out
+
+    def depths(self) -> Dict[str, np.ndarray]:
+        out = {k: np.empty((self.ny, self.nx)) for k in DEPTHS}
+        arr = (_capi._dp * 12)(*[_d(out[k]) for k in DEPTHS])
+        check(lib().inmsom_bt_get_depths(self._h, arr), "inmsom_bt_get_depths")
+        return out
+
+    # ------------------------------------------------------------ steps
+    def barotropic_dynamics(self, tau, nstep, ksw4, ubrtr_e, ubrtrp_e, vbrtr_e, vbrtrp_e, ssh_e,
+                            sshp_e, ubrtr_i, vbrtr_i, ssh4gradx, ssh4grady, RHSx, RHSy, wflux,
+                            mu, mu4, rdis):
+        """Function/vel_ssh.f90:379-679; the first ten arrays are updated in place."""
+        arrs = [ubrtr_e, ubrtrp_e, vbrtr_e, vbrtrp_e, ssh_e, sshp_e, ubrtr_i, vbrtr_i, ssh4gradx,
+                ssh4grady, RHSx, RHSy, wflux, mu, mu4, rdis]
+        check(lib().inmsom_bt_barotropic_dynamics(self._h, float(tau), int(nstep), int(ksw4),
+                                                  *[_d(a) for a in arrs]),
+              "inmsom_bt_barotropic_dynamics")
+
+    def model_steps(self, nsteps: int, nstep: int = 1, ksw4: int = 0, tau: Optional[float] = None):
+        check(lib().inmsom_bt_model_steps(self._h, float(self.b.dt if tau is None else tau),
+                                          int(nstep), int(ksw4), int(nsteps)),
+              "inmsom_bt_model_steps")
+
+    def run(self, nsteps: int, mode: str = "fp64", nstep: int = 1, tau: Optional[float] = None):
+        """nsteps outer steps of tau, each sub-cycled in nstep fused launches."""
+        check(lib().inmsom_bt_run(self._h, float(self.b.dt if tau is None else tau), int(nstep),
+                                  int(nsteps), _capi.MODES[mode]), "inmsom_bt_run")
+
+    def run_timed(self, nsteps: int, mode: str = "fp64", nstep: int = 1,
+                  tau: Optional[float] = None) -> float:
+        """As run(); returns the HIP-event time of the launches in ms."""
+        ms = C.c_double()
+        check(lib().inmsom_bt_run_timed(self._h, float(self.b.dt if tau is None else tau),
+                                        int(nstep), int(nsteps), _capi.MODES[mode], C.byref(ms)),
+              "inmsom_bt_run_timed")
+        return ms.value
+
+    def synchronize(self):
+        check(lib().inmsom_bt_synchronize(self._h), "inmsom_bt_synchronize")
+
+    # ------------------------------------------------------------ diagnostics
+    def volume(self) -> float:
+        v = C.c_double()
+        check(lib().inmsom_bt_volume(self._h, C.byref(v)), "inmsom_bt_volume")
+        return v.value
+
+    def range_check(self) -> int:
+        """Number of wet cells failing Control/dyncall.f90:452-516's bounds."""
+        bad = C.c_longlong()
+        code = lib().inmsom_bt_range_check(self._h, C.byref(bad))
+        if code not in (_capi.OK, _capi.ERANGE):
+            check(code, "inmsom_bt_range_check")
+        return bad.value

@@ -148,6 +148,29 @@ def run(b, nsteps: int, nstep: int = 1, ksw4: int = 0, out_steps: Sequence[int] 
     return outs
 
 
+def barotropic_dynamics(og: OracleGrid, tau: float, nstep: int, ksw4: int, st: dict, acc: dict,
+                        rhsx, rhsy, wflux, mu, mu4, rdis) -> Dict[str, np.ndarray]:
+    """One call of Function/vel_ssh.f90:379-679 on depths hh_init(ssh, sshp);
+    st (u, up, v, vp, ssh, sshp) and acc are updated in place; returns the
+    depths after the call."""
+    L = lib()
+    b = og.b
+    ds, da = og.new_depths()
+    L.bto_hh_init(og.ref, C.byref(ds), _d(st["ssh"]), _d(st["sshp"]))
+    z = lambda: np.zeros((b.ny, b.nx))
+    scratch = [z() for _ in range(9)]
+    ins = [np.ascontiguousarray(a, dtype=np.float64) for a in (rhsx, rhsy, wflux, mu, mu4)]
+    rd = np.ascontiguousarray(rdis, dtype=np.float64)
+    L.bto_barotropic_dynamics(og.ref, C.byref(ds), C.c_double(tau), nstep, ksw4,
+                              _d(st["u"]), _d(st["up"]), _d(st["v"]), _d(st["vp"]), _d(st["ssh"]),
+                              _d(st["sshp"]), _d(acc["ubrtr_i"]), _d(acc["vbrtr_i"]),
+                              _d(acc["ssh4gradx"]), _d(acc["ssh4grady"]), *[_d(a) for a in ins],
+                              _d(scratch[0]), _d(scratch[1]), _d(scratch[2]), _d(scratch[3]),
+                              _d(scratch[4]), _d(rd), _d(scratch[5]), _d(scratch[6]),
+                              _d(scratch[7]), _d(scratch[8]))
+    return {k: a.copy() for k, a in zip(DEPTHS, da)}
+
+
 def units(b, nstep: int = 1, rdis: Optional[np.ndarray] = None) -> Dict[str, np.ndarray]:
     """The helper-routine sequence of the harness's mode 1."""
     L = lib()

@@ -1,0 +1,112 @@
+"""GPU parity of the HIP path against the oracle (C restatement pinned to
+the compiled reference).  Integer-exact comparisons (bit patterns) for the
+reference-order paths, relative L2 for the reassociated FP64 and the FP32
+paths, with the tolerances written in each test."""
+import numpy as np
+import pytest
+
+import oracle
+from _util import STATE, assert_bitwise, rel_l2
+from inmsom_bt import BarotropicModel, make_basin, randomize_state
+
+pytestmark = pytest.mark.gpu
+
+ACC = ("ubrtr_i", "vbrtr_i", "ssh4gradx", "ssh4grady")
+DEPTHS = ("hhq_e", "hhqp_e", "hhqn_e", "hhu_e", "hhup_e", "hhun_e",
+          "hhv_e", "hhvp_e", "hhvn_e", "hhh_e", "hhhp_e", "hhhn_e")
+
+
+def _case(kind):
+    if kind == "uniform":
+        return make_basin(150, 110, seed=21, sigma=6.0), 1, 0
+    if kind == "stretched_ksw4_nstep3":
+        b = make_basin(130, 97, seed=22, sigma=5.0, metrics="stretched", lvisc_4=1e-3)
+        return randomize_state(b, 7, wflux_amp=1e-5), 3, 1
+    if kind == "ragged_small":
+        return randomize_state(make_basin(7, 5, seed=23, sigma=1.0, hland=60.0), 8), 2, 0
+    raise KeyError(kind)
+
+
+@pytest.mark.parametrize("kind", ["uniform", "stretched_ksw4_nstep3", "ragged_small"])
+def test_model_steps_bitwise(kind):
+    b, nstep, ksw4 = _case(kind)
+    steps = 12
+    print(f"case {kind}: {b.nx}x{b.ny} nstep={nstep} ksw4={ksw4}", flush=True)
+    want = oracle.run(b, steps, nstep, ksw4, out_steps=[steps])[steps]
+    with BarotropicModel(b) as m:
+        m.model_steps(steps, nstep=nstep, ksw4=ksw4)
+        got = m.get_state()
+        got.update(m.means())
+        got.update(m.depths())
+    assert_bitwise(got, want, STATE + ACC + DEPTHS, f"model_steps[{kind}]")
+
+
+def test_barotropic_dynamics_dropin_bitwise():
+    """The drop-in entry point with caller-supplied RHS, wflux, mu, mu4."""
+    b, nstep, ksw4 = _case("stretched_ksw4_nstep3")
+    og = oracle.OracleGrid(b)
+    rdis = og.rayleigh()
+    rng = np.random.default_rng(5)
+    rhsx = rng.standard_normal((b.ny, b.nx)) * 1e3
+    rhsy = rng.standard_normal((b.ny, b.nx)) * 1e3
+    args_o = {k: np.ascontiguousarray(getattr(b, k)).copy() for k in ("u", "up", "v", "vp", "ssh", "sshp")}
+    args_g = {k: v.copy() for k, v in args_o.items()}
+    acc_o = {k: np.zeros((b.ny, b.nx)) for k in ACC}
+    acc_g = {k: np.zeros((b.ny, b.nx)) for k in ACC}
+    want = oracle.barotropic_dynamics(og, b.dt, nstep, ksw4, args_o, acc_o, rhsx, rhsy,
+                                      b.wflux, b.mu, b.mu4, rdis)
+    with BarotropicModel(b, rdis=rdis) as m:
+        m.barotropic_dynamics(b.dt, nstep, ksw4, args_g["u"], args_g["up"], args_g["v"],
+                              args_g["vp"], args_g["ssh"], args_g["sshp"], acc_g["ubrtr_i"],
+                              acc_g["vbrtr_i"], acc_g["ssh4gradx"], acc_g["ssh4grady"], rhsx, rhsy,
+                              np.ascontiguousarray(b.wflux), np.ascontiguousarray(b.mu),
+                              np.ascontiguousarray(b.mu4), rdis)
+        dep = m.depths()
+    got = dict(args_g)
+    got.update(acc_g)
+    got.update(dep)
+    want_all = dict(args_o)
+    want_all.update(acc_o)
+    want_all.update(want)
+    assert_bitwise(got, want_all, STATE + ACC + DEPTHS, "barotropic_dynamics")
+
+
+@pytest.mark.parametrize("shape,nstep", [((150, 110), 1), ((150, 110), 2), ((61, 7), 3),
+                                         ((300, 64), 2)])
+def test_fused_exact_bitwise(shape, nstep):
+    b = make_basin(*shape, seed=31, sigma=5.0)
+    steps = 20
+    print(f"fused exact {shape} nstep={nstep}", flush=True)
+    want = oracle.run(b, steps, nstep, out_steps=[steps])[steps]
+    with BarotropicModel(b) as m:
+        m.run(steps, "fp64_exact", nstep=nstep)
+        got = m.get_state()
+    assert_bitwise(got, want, STATE, f"fused fp64_exact {shape} nstep={nstep}")
+
+
+def test_fused_fp64_within_1e12():
+    """Reassociated FP64 step: relative L2 <= 1e-12 after 200 outer steps."""
+    b = make_basin(200, 150, seed=41, sigma=8.0)
+    steps, nstep = 200, 2
+    want = oracle.run(b, steps, nstep, out_steps=[steps])[steps]
+    with BarotropicModel(b) as m:
+        m.run(steps, "fp64", nstep=nstep)
+        got = m.get_state()
+    for k in ("ssh", "u", "v", "sshp", "up", "vp"):
+        e = rel_l2(got[k], want[k])
+        print(f"fp64 {k}: rel L2 {e:.3e}")
+        assert e <= 1e-12, (k, e)
+
+
+def test_fused_fp32_tolerance():
+    """FP32 stencils, FP64 filtered level: rel L2 <= 1e-5 after 200 steps."""
+    b = make_basin(200, 150, seed=42, sigma=8.0)
+    steps, nstep = 200, 2
+    want = oracle.run(b, steps, nstep, out_steps=[steps])[steps]
+    with BarotropicModel(b) as m:
+        m.run(steps, "fp32", nstep=nstep)
+        got = m.get_state()
+    for k in ("ssh", "u", "v"):
+        e = rel_l2(got[k], want[k])
+        print(f"fp32 {k}: rel L2 {e:.3e}")
+        assert e <= 1e-5, (k, e)
