@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Throughput of the fused barotropic step on one MI355X.
+
+Workload (inmsom_bt.workloads.single_basin('cfg1')): a 2048 x 1536 closed
+basin at 1 km, 10 s outer steps, barotropic mode sub-cycled twice
+(5 s leap-frog substeps).  A bench "step" is one outer step = two fused
+launches; the reported value counts state-advancing leap-frog substeps:
+
+    value = wet cells * steps * nstep_barotrop / timed seconds
+
+Printed (stdout, one JSON line): FP64 (the reference's precision) as the
+headline value, with FP32 and FP64_EXACT alongside, each with its roofline
+figure, the FP32 / FP64 errors after the timed run, and the oracle (the
+C restatement of the reference, OpenMP) timed on a bounded sample of the
+same workload on this host's cores.  Progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "inmsom-elfs_amd"))
+
+import numpy as np  # noqa: E402
+
+from inmsom_bt import BarotropicModel  # noqa: E402
+from inmsom_bt.workloads import NSTEP_BAROTROP, single_basin  # noqa: E402
+
+METRIC = "wet-cell steps/s on 1 GPU: plain FP64 vs FP32/batched/learned at fixed SSH err"
+PEAK_HBM_GBS = 8000.0
+
+# Algorithmic bytes per streamed cell per launch (substep), nstep_barotrop = 2:
+# level n and filtered level n-1 of (u, v, ssh) read and written, the depth at
+# rest and the Rayleigh coefficient read, and the two RHS components frozen
+# over the call (written by the first substep, read by the second: one
+# 8-byte (FP32: 4-byte) array access per component per launch on average).
+BYTES_PER_CELL = {
+    "fp64": 12 * 8 + 8 + 8 + 2 * 8,          # 128
+    "fp64_exact": 12 * 8 + 8 + 8 + 2 * 8,    # 128
+    "fp32": 6 * 4 + 6 * 8 + 4 + 4 + 2 * 4,   # 88
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def traffic_from_profiles(mode):
+    path = os.path.join(ROOT, "profiles", "fused_step_cfg1_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        e = t[mode]
+        return float(e["fetch_bytes_per_launch"] + e["write_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def timed_mode(m, b, mode, steps, warmup, ncell):
+    m.set_state(b.u, b.up, b.v, b.vp, b.ssh, b.sshp)
+    vol0 = m.volume()
+    if warmup:
+        m.run(warmup, mode, nstep=NSTEP_BAROTROP)
+    m.synchronize()
+    t0 = time.perf_counter()
+    ms = m.run_timed(steps, mode, nstep=NSTEP_BAROTROP)
+    wall = time.perf_counter() - t0
+    bad = m.range_check()
+    vol1 = m.volume()
+    st = m.get_state()
+    launches = steps * NSTEP_BAROTROP
+    avg_launch_s = ms * 1e-3 / launches
+    bpc = BYTES_PER_CELL[mode]
+    achieved = bpc * ncell / avg_launch_s / 1e9
+    traffic = traffic_from_profiles(mode)
+    res = {
+        "value": b.n_wet * launches / (ms * 1e-3),
+        "ms_per_step": ms / steps,
+        "wall_s": wall,
+        "avg_launch_us": avg_launch_s * 1e6,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                     "bytes_per_cell": bpc, "cells_per_launch": ncell},
+        "range_check_bad_cells": bad,
+        "volume_rel_drift": abs(vol1 - vol0) / max(abs(vol0), 1e-300),
+    }
+    log(f"{mode}: {ms:.1f} ms for {steps} steps ({avg_launch_s*1e6:.1f} us/launch), "
+        f"{res['value']:.3e} wet-cell steps/s, {achieved:.0f} GB/s")
+    return res, st
+
+
+def cpu_baseline(b, steps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    og = oracle.OracleGrid(b)
+    rdis = og.rayleigh()
+    threads = int(oracle.lib().bto_num_threads())
+    t0 = time.perf_counter()
+    oracle.run(b, steps, NSTEP_BAROTROP, out_steps=[], rdis=rdis, grid=og)
+    dt = time.perf_counter() - t0
+    return {"value": b.n_wet * steps * NSTEP_BAROTROP / dt, "unit": "wet-cell steps/s",
+            "cores": threads, "kind": "port",
+            "sample": f"{steps} outer steps (nstep_barotrop={NSTEP_BAROTROP}, with the reference's "
+                      f"look-ahead substeps) of the same 2048x1536 basin, {dt:.1f} s, oracle/bt_oracle.c "
+                      f"with OpenMP"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000, help="timed outer steps (10 s each)")
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1"])
+    ap.add_argument("--cpu-steps", type=int, default=16, help="outer steps timed on the CPU oracle")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--modes", default="fp64,fp32,fp64_exact")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.perf_counter()
+    b = single_basin(args.workload)
+    ncell = b.nx * b.ny
+    log(f"workload {args.workload}: {b.nx - 4}x{b.ny - 4} interior, {b.n_wet} wet cells, "
+        f"generated in {time.perf_counter() - t0:.1f} s")
+    modes = [x for x in args.modes.split(",") if x]
+    results, states = {}, {}
+    with BarotropicModel(b, device=0) as m:
+        for mode in modes:
+            results[mode], states[mode] = timed_mode(m, b, mode, args.steps, args.warmup, ncell)
+
+    errs = {}
+    if "fp64" in states:
+        ref = states["fp64"]
+        for mode in ("fp32", "fp64_exact"):
+            if mode in states:
+                errs[mode] = {k: {"rel_l2": rel_l2(states[mode][k], ref[k]),
+                                  "max_abs": float(np.max(np.abs(states[mode][k] - ref[k])))}
+                              for k in ("ssh", "u", "v")}
+                results[mode]["err_vs_fp64"] = errs[mode]
+
+    head = results.get("fp64") or results[modes[0]]
+    head_mode = "fp64" if "fp64" in results else modes[0]
+    line = {
+        "metric": METRIC,
+        "value": head["value"],
+        "unit": "wet-cell steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp64_exact": "f64", "fp32": "f32+f64"}[head_mode],
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {b.nx - 4}x{b.ny - 4} closed basin, dx=1 km, "
+                               "dt=10 s outer step, nstep_barotrop=2 (5 s leap-frog substeps), "
+                               f"{head_mode} fused step",
+                   "wet_cells": b.n_wet, "array_cells": ncell,
+                   "step": "one 10 s outer step = 2 state-advancing leap-frog substeps; value "
+                           "counts substeps x wet cells"},
+        "roofline": head["roofline"],
+    }
+    for mode in modes:
+        if mode != head_mode:
+            line[mode] = results[mode]
+    line["fp64_details"] = {k: head[k] for k in ("avg_launch_us", "range_check_bad_cells",
+                                                 "volume_rel_drift", "wall_s")}
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle) ...")
+        line["cpu_baseline"] = cpu_baseline(b, args.cpu_steps)
+        log(f"cpu baseline: {line['cpu_baseline']['value']:.3e} wet-cell steps/s on "
+            f"{line['cpu_baseline']['cores']} threads")
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,31 @@
+"""The shallow-water workloads the barotropic step is measured on.
+
+single_basin(...) is the plain leap-frog configuration: a seeded
+Gaussian-smoothed bathymetry (sigma 8 cells) mapped to 0..200 m with land
+below 10 m, dx = dy = 1 km, f = 1e-4 + 2e-11 y, g = 9.81,
+tau_x = 0.1 cos(pi y / L_y), nu = 10 m^2/s, quadratic drag 2.5e-3,
+Robert-Asselin 0.05, a 0.5 m Gaussian bump of radius 20 km at the centre.
+
+Outer step dt = 10 s.  The barotropic mode is sub-cycled with
+nstep_barotrop = 2 (leap-frog substep 5 s): a 10 s leap-frog step violates
+the C-grid leap-frog limit c dt / dx <= 1/(2 sqrt 2) for depths above
+~128 m at 1 km, and the reference scheme (run through the bit-exact
+oracle) diverges within ~40 steps at nstep_barotrop = 1.
+"""
+from __future__ import annotations
+
+from .cases import Basin, make_basin
+
+DT = 10.0
+NSTEP_BAROTROP = 2
+
+SINGLE = {
+    "cfg0": dict(nxi=256, nyi=256, steps=1000),
+    "cfg1": dict(nxi=2048, nyi=1536, steps=10000),
+}
+
+
+def single_basin(name: str = "cfg1", seed: int = 2024) -> Basin:
+    spec = SINGLE[name]
+    return make_basin(spec["nxi"], spec["nyi"], seed=seed, sigma=8.0, hmax=200.0, hland=10.0,
+                      f0=1.0e-4, beta=2.0e-11, tau0=0.1, nu=10.0, g=9.81, dt=DT)

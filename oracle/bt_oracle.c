@@ -19,6 +19,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define IX(m, n) ((size_t)((n) - 1) * (size_t)g->nx + (size_t)((m) - 1))
 #define NXS 3
@@ -701,4 +704,13 @@ double bto_volume(const bto_grid *g, const double *ssh)
         total += row;
     }
     return total;
+}
+
+int bto_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }

@@ -136,6 +136,9 @@ void bto_model_step(const bto_grid *g, bto_depths *d, double tau, int nstep,
  * summed row by row in a fixed order. */
 double bto_volume(const bto_grid *g, const double *ssh);
 
+/* OpenMP threads the oracle's loops run on. */
+int bto_num_threads(void);
+
 #ifdef __cplusplus
 }
 #endif
