@@ -147,6 +147,12 @@ int inmsom_bt_run(inmsom_bt_ctx *ctx, double tau, int nstep, int nsteps, int mod
 int inmsom_bt_run_timed(inmsom_bt_ctx *ctx, double tau, int nstep, int nsteps, int mode,
                         double *ms);
 
+/* Launch shape of the fused step: output rows marched by one 64-lane
+ * wave (0 = default: 32 for the FP64 modes, 20 for FP32).  Results do
+ * not depend on it. */
+int inmsom_bt_set_launch(inmsom_bt_ctx *ctx, int rows_per_wave);
+int inmsom_bt_get_launch(inmsom_bt_ctx *ctx, int *rows_per_wave);
+
 /* Range check of Control/dyncall.f90:452-516 on the device state:
  * *bad receives the number of wet cells with NaN, |ssh| >= 10 or
  * |u|,|v| >= 100.  Returns INMSOM_BT_ERANGE when *bad > 0. */

@@ -489,6 +489,20 @@ int inmsom_bt_run_timed(inmsom_bt_ctx *c, double tau, int nstep, int nsteps, int
     return INMSOM_BT_OK;
 }
 
+int inmsom_bt_set_launch(inmsom_bt_ctx *c, int rows_per_wave)
+{
+    REQUIRE(c, "null context");
+    TRY(fused_set_shape(c, rows_per_wave));
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_get_launch(inmsom_bt_ctx *c, int *rows_per_wave)
+{
+    REQUIRE(c && rows_per_wave, "null pointer");
+    *rows_per_wave = c->chunk;
+    return INMSOM_BT_OK;
+}
+
 int inmsom_bt_range_check(inmsom_bt_ctx *c, long long *bad)
 {
     REQUIRE(c && bad, "null pointer");

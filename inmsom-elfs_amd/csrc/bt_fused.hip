@@ -56,16 +56,27 @@ struct FusedParams {
     double *hup_c, *hvp_c; // filtered U/V depths carried between substeps (FP64_EXACT only)
 };
 
-template <class T>
-__device__ __forceinline__ T xr(T x)  // value of column m+1
+// Lane shifts by DPP (no LDS traffic).  wave_shl:1 gives lane L the value
+// of lane L+1 (column m+1), wave_shr:1 the value of lane L-1 (column m-1);
+// the lanes at the wave's ends receive 0 and only ever feed halo lanes.
+__device__ __forceinline__ int dpp_from_right(int x)
 {
-    return __shfl_down(x, 1);
+    return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xF, 0xF, true);
 }
-template <class T>
-__device__ __forceinline__ T xl(T x)  // value of column m-1
+__device__ __forceinline__ int dpp_from_left(int x)
 {
-    return __shfl_up(x, 1);
+    return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xF, 0xF, true);
 }
+__device__ __forceinline__ double xr(double x)  // value of column m+1
+{
+    return __hiloint2double(dpp_from_right(__double2hiint(x)), dpp_from_right(__double2loint(x)));
+}
+__device__ __forceinline__ double xl(double x)  // value of column m-1
+{
+    return __hiloint2double(dpp_from_left(__double2hiint(x)), dpp_from_left(__double2loint(x)));
+}
+__device__ __forceinline__ float xr(float x) { return __int_as_float(dpp_from_right(__float_as_int(x))); }
+__device__ __forceinline__ float xl(float x) { return __int_as_float(dpp_from_left(__float_as_int(x))); }
 
 template <class R>
 __device__ __forceinline__ R bitf(int mb, int bit)
@@ -106,22 +117,17 @@ struct Ops {
     }
 };
 
-template <typename R>
-__device__ __forceinline__ R ldr(const void *a, const FusedParams &p, bool colok, int m, int row)
-{
-    return (colok && row >= 1 && row <= p.ny)
-               ? static_cast<const R *>(a)[(size_t)(row - 1) * p.nx + (m - 1)]
-               : R(0);
-}
-__device__ __forceinline__ double ldd(const double *a, const FusedParams &p, bool colok, int m,
-                                      int row)
-{
-    return (colok && row >= 1 && row <= p.ny) ? a[(size_t)(row - 1) * p.nx + (m - 1)] : 0.0;
-}
-__device__ __forceinline__ int ldm(const FusedParams &p, bool colok, int m, int row)
-{
-    return (colok && row >= 1 && row <= p.ny) ? (int)p.mb[(size_t)(row - 1) * p.nx + (m - 1)] : 0;
-}
+// Loads are branch-free: the address is clamped into the array and the
+// value is zeroed afterwards when the cell lies outside it.  Straight-line
+// loads let the compiler wait for exactly the older (prefetched) loads.
+struct RowAddr {
+    int nx, ny, mc;  // mc = column clamped to 1..nx
+    __device__ __forceinline__ size_t at(int row) const
+    {
+        const int rc = min(max(row, 1), ny);
+        return (size_t)(rc - 1) * nx + (mc - 1);
+    }
+};
 
 template <typename R, bool EXACT>
 __global__ void __launch_bounds__(WAVE) k_fused_step(FusedParams p)
@@ -161,18 +167,74 @@ __global__ void __launch_bounds__(WAVE) k_fused_step(FusedParams p)
     R hun_0 = 0, hun_p1 = 0, stt_0 = 0, stt_p1 = 0, bf_0 = 0, bf_p1 = 0;
     R sts_m1 = 0, sts_0 = 0, vo_m1 = 0, vo_0 = 0;
 
+    // Loads run one iteration ahead of their use (software prefetch): the
+    // nx_* registers hold what the next iteration consumes.
+    const RowAddr ra{p.nx, p.ny, min(max(m, 1), p.nx)};
+    const R *S = static_cast<const R *>(p.s), *Hh = static_cast<const R *>(p.H);
+    const R *U = static_cast<const R *>(p.u), *V = static_cast<const R *>(p.v);
+    const R *RD = static_cast<const R *>(p.rd);
+    const R *FX = static_cast<const R *>(p.frx), *FY = static_cast<const R *>(p.fry);
+    const bool frozen = p.sub > 1;
+    auto rowok = [&](int row) { return colok && row >= 1 && row <= p.ny; };
+    auto tab = [&](const double *t, int row) { return t[min(max(row, 1), p.ny) - 1]; };
+
+    R nx_s, nx_H, nx_u, nx_v, nx_rd, nx_fx, nx_fy;
+    double nx_up, nx_vp, nx_sp, nx_hupc, nx_hvpc, nx_f, nx_sx, nx_sy, nx_wf;
+    int nx_mb;
+    auto issue = [&](int n) {  // the loads iteration n consumes
+        const size_t i2 = ra.at(n + 2), i1 = ra.at(n + 1), i0 = ra.at(n);
+        nx_s = S[i2];
+        nx_H = Hh[i2];
+        nx_mb = p.mb[i2];
+        nx_u = U[i1];
+        nx_v = V[i1];
+        nx_up = p.up[i1];
+        nx_vp = p.vp[i1];
+        nx_sp = p.sp[i1];
+        nx_rd = RD[i1];
+        // per-cell frozen forcing and carried depths of the output row n:
+        // element 0 (one cache line per wave) when they are not used
+        const size_t iff = frozen ? i0 : 0;
+        nx_fx = FX[iff];
+        nx_fy = FY[iff];
+        if constexpr (EXACT) {
+            nx_hupc = p.hup_c[iff];
+            nx_hvpc = p.hvp_c[iff];
+        } else {
+            nx_hupc = nx_hvpc = 0.0;
+        }
+        nx_f = tab(p.row_f, n);
+        nx_sx = tab(p.row_sx, n);
+        nx_sy = tab(p.row_sy, n);
+        nx_wf = tab(p.row_wf, n + 1);
+    };
+    issue(r0 - 3);
+    R f_m1 = 0, f_0 = 0, fx_0 = 0, fy_0 = 0, sx_0 = 0, sy_0 = 0, wf_p1 = 0;
+    double hupc_0 = 0, hvpc_0 = 0;
+
     for (int n = r0 - 3; n < r1; ++n) {
-        const int k1 = n + 1, k2 = n + 2;
-        // ---- loads: surface height, depth and masks two rows ahead, the rest one row ahead
-        s_p2 = ldr<R>(p.s, p, colok, m, k2);
-        H_p2 = ldr<R>(p.H, p, colok, m, k2);
-        mb_p2 = ldm(p, colok, m, k2);
-        u_p1 = ldr<R>(p.u, p, colok, m, k1);
-        v_p1 = ldr<R>(p.v, p, colok, m, k1);
-        up_p1 = ldd(p.up, p, colok, m, k1);
-        vp_p1 = ldd(p.vp, p, colok, m, k1);
-        sp_p1 = ldd(p.sp, p, colok, m, k1);
-        rd_p1 = ldr<R>(p.rd, p, colok, m, k1);
+        const int k1 = n + 1;
+        // ---- take the prefetched rows: surface height, depth and masks two
+        //      rows ahead, the rest one row ahead
+        s_p2 = rowok(n + 2) ? nx_s : R(0);
+        H_p2 = rowok(n + 2) ? nx_H : R(0);
+        mb_p2 = rowok(n + 2) ? nx_mb : 0;
+        u_p1 = rowok(k1) ? nx_u : R(0);
+        v_p1 = rowok(k1) ? nx_v : R(0);
+        up_p1 = rowok(k1) ? nx_up : 0.0;
+        vp_p1 = rowok(k1) ? nx_vp : 0.0;
+        sp_p1 = rowok(k1) ? nx_sp : 0.0;
+        rd_p1 = rowok(k1) ? nx_rd : R(0);
+        fx_0 = nx_fx;
+        fy_0 = nx_fy;
+        hupc_0 = nx_hupc;
+        hvpc_0 = nx_hvpc;
+        f_m1 = f_0;
+        f_0 = (n >= 1 && n <= p.ny) ? R(nx_f) : R(0);
+        sx_0 = R(nx_sx);
+        sy_0 = R(nx_sy);
+        wf_p1 = (k1 >= 1 && k1 <= p.ny) ? R(nx_wf) : R(0);
+        if (n + 1 < r1) issue(n + 1);
 
         const R lu_0 = bitf<R>(mb_0, MB_LU), lu_p1 = bitf<R>(mb_p1, MB_LU), lu_p2 = bitf<R>(mb_p2, MB_LU);
         const R lu_0r = xr(lu_0), lu_p1r = xr(lu_p1);
@@ -195,7 +257,7 @@ __global__ void __launch_bounds__(WAVE) k_fused_step(FusedParams p)
         const R u1l = xl(u_p1), hu1l = xl(hu_p1), up1l = xl(R(up_p1));
         {
             const bool tw = (mb_p1 & MB_LU) && inx && iny1;
-            const R wf = (k1 >= 1 && k1 <= p.ny) ? R(p.row_wf[k1 - 1]) : R(0);
+            const R wf = wf_p1;
             double sn;
             if constexpr (EXACT) {
                 sn = sp_p1 + 2.0 * p.tau *
@@ -222,18 +284,23 @@ __global__ void __launch_bounds__(WAVE) k_fused_step(FusedParams p)
             const bool tw = (mb_p1 & MB_LU) && inx && iny1;
             R st, bf;
             const R upr = R(up_p1), vpr = R(vp_p1), vp0r = R(vp_0);
+            bf = R(0);
             if constexpr (EXACT) {
                 st = d / d * (upr / d - up1l / d) - d / d * (vpr / d - vp0r / d);
-                R ut = (u_p1 * d * d * hu_p1 + u1l * d * d * hu1l) / 2.0 / d / d / hq_p1;
-                R vt = (v_p1 * d * d * hv_p1 + v_0 * d * d * hv_0) / 2.0 / d / d / hq_p1;
-                bf = R(p.cb) * sqrt(ut * ut + vt * vt + R(p.eb));
+                if (p.sub == 1) {
+                    R ut = (u_p1 * d * d * hu_p1 + u1l * d * d * hu1l) / 2.0 / d / d / hq_p1;
+                    R vt = (v_p1 * d * d * hv_p1 + v_0 * d * d * hv_0) / 2.0 / d / d / hq_p1;
+                    bf = R(p.cb) * sqrt(ut * ut + vt * vt + R(p.eb));
+                }
             } else {
 #pragma clang fp contract(fast)
                 st = op.inv_d * ((upr - up1l) - (vpr - vp0r));
-                const R rh = R(0.5) / hq_p1;
-                R ut = (u_p1 * hu_p1 + u1l * hu1l) * rh;
-                R vt = (v_p1 * hv_p1 + v_0 * hv_0) * rh;
-                bf = R(p.cb) * sqrt(ut * ut + vt * vt + R(p.eb));
+                if (p.sub == 1) {
+                    const R rh = R(0.5) / hq_p1;
+                    R ut = (u_p1 * hu_p1 + u1l * hu1l) * rh;
+                    R vt = (v_p1 * hv_p1 + v_0 * hv_0) * rh;
+                    bf = R(p.cb) * sqrt(ut * ut + vt * vt + R(p.eb));
+                }
             }
             stt_p1 = tw ? st : R(0);
             bf_p1 = (tw && p.sub == 1) ? bf : R(0);
@@ -273,8 +340,7 @@ __global__ void __launch_bounds__(WAVE) k_fused_step(FusedParams p)
             const R hh0l = xl(hh_0), sts0l = xl(sts_0), vo0l = xl(vo_0);
             const R up0 = R(up_0), vp0 = R(vp_0);
             const R luu0 = bitf<R>(mb_0, MB_LUU), luum1 = bitf<R>(mb_m1, MB_LUU);
-            const R sx = R(p.row_sx[n - 1]), sy = R(p.row_sy[n - 1]);
-            const R f_0 = R(p.row_f[n - 1]), f_m1 = n >= 2 ? R(p.row_f[n - 2]) : R(0);
+            const R sx = sx_0, sy = sy_0;
             const size_t io = (size_t)(n - 1) * p.nx + (m - 1);
             const bool own = writer && n <= p.ny;
             // the RHS of Control/dyncall.f90:114-126 is formed once per call
@@ -288,9 +354,9 @@ __global__ void __launch_bounds__(WAVE) k_fused_step(FusedParams p)
                 rhs_u = sx - R(0.5) * (bf_0 + bf0r) * up0;
                 rhs_v = sy - R(0.5) * (bf_0 + bf_p1) * vp0;
             }
-            if (p.sub > 1) {
-                rhs_u = own ? static_cast<const R *>(p.frx)[io] : R(0);
-                rhs_v = own ? static_cast<const R *>(p.fry)[io] : R(0);
+            if (frozen) {
+                rhs_u = fx_0;
+                rhs_v = fy_0;
             } else if (p.nsub > 1 && own) {
                 static_cast<R *>(p.frx)[io] = rhs_u;
                 static_cast<R *>(p.fry)[io] = rhs_v;
@@ -299,9 +365,9 @@ __global__ void __launch_bounds__(WAVE) k_fused_step(FusedParams p)
             // the hh_shift-filtered carry afterwards (depth.f90:195-203)
             R hupc = hup_0, hvpc = hvp_0;
             if constexpr (EXACT) {
-                if (p.sub > 1) {
-                    hupc = own ? R(p.hup_c[io]) : R(0);
-                    hvpc = own ? R(p.hvp_c[io]) : R(0);
+                if (frozen) {
+                    hupc = R(hupc_0);
+                    hvpc = R(hvpc_0);
                 }
             }
             R un, vn;
@@ -553,16 +619,13 @@ static int hipfail(hipError_t e, const char *what)
 
 static inline unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
 
-static int g_chunk = 32;
-
 int fused_set_shape(inmsom_bt_ctx *c, int chunk)
 {
-    (void)c;
-    if (chunk < 1 || chunk > 4096) {
-        set_error("chunk out of range");
+    if (chunk < 0 || chunk > 4096) {
+        set_error("rows per wave out of range (0 = default, 1..4096)");
         return INMSOM_BT_EINVAL;
     }
-    g_chunk = chunk;
+    c->chunk = chunk;
     return INMSOM_BT_OK;
 }
 
@@ -771,7 +834,8 @@ static FusedParams make_params(inmsom_bt_ctx *c, FusedAlloc *a, int b, double ta
     p.nx = c->nx;
     p.ny = c->ny;
     p.strips = (c->nx + 2 + SW - 1) / SW;  // columns 1..nx need strips covering m <= nx
-    p.chunk = g_chunk;
+    // measured on MI355X at 2048x1536: FP64 best at 32 rows, FP32 at 20
+    p.chunk = c->chunk > 0 ? c->chunk : (c->fa && c->fa->mode == INMSOM_BT_FP32 ? 20 : 32);
     p.nchunks = (c->ny + p.chunk - 1) / p.chunk;
     p.total = p.strips * p.nchunks;
     p.per_xcd = (p.total + NXCD - 1) / NXCD;

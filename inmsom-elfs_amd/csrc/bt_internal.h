@@ -83,6 +83,7 @@ struct inmsom_bt_ctx {
     double *row_wf = nullptr;      // surface water flux per T row
     ibt::FusedAlloc *fa = nullptr;  // ping-pong state buffers of the fused step
     int fz_cur = 0;
+    int chunk = 0;                 // output rows per wave of the fused step (0 = per-mode default)
     double *scal = nullptr;        // device scalars (reductions)
     long long *cnt = nullptr;
 };

@@ -188,6 +188,14 @@ class BarotropicModel:
               "inmsom_bt_run_timed")
         return ms.value
 
+    def set_launch(self, rows_per_wave: int):
+        check(lib().inmsom_bt_set_launch(self._h, int(rows_per_wave)), "inmsom_bt_set_launch")
+
+    def get_launch(self) -> int:
+        r = C.c_int()
+        check(lib().inmsom_bt_get_launch(self._h, C.byref(r)), "inmsom_bt_get_launch")
+        return r.value
+
     def synchronize(self):
         check(lib().inmsom_bt_synchronize(self._h), "inmsom_bt_synchronize")
 
