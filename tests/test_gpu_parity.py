@@ -110,3 +110,50 @@ def test_fused_fp32_tolerance():
         e = rel_l2(got[k], want[k])
         print(f"fp32 {k}: rel L2 {e:.3e}")
         assert e <= 1e-5, (k, e)
+
+
+# ---------------------------------------------------------------- golden vectors
+import hashlib  # noqa: E402
+import json  # noqa: E402
+import os  # noqa: E402
+import sys  # noqa: E402
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(_HERE, "golden"))
+import make_golden  # noqa: E402
+
+with open(os.path.join(_HERE, "golden", "reference_vectors.json")) as _f:
+    _GOLD = json.load(_f)
+
+
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("name", [k for k, v in sorted(_GOLD["cases"].items())
+                                  if v["run"].get("mode", 0) == 0])
+@pytest.mark.parametrize("path", ["model_steps", "fused_exact"])
+def test_gpu_against_reference_vectors(name, path):
+    """The HIP paths against the vectors the compiled reference wrote."""
+    rec = _GOLD["cases"][name]
+    run = rec["run"]
+    b = make_golden.build_case(name)
+    if path == "fused_exact" and (run["ksw4"] or rec["basin"].get("metrics") == "stretched"):
+        pytest.skip("the fused step covers uniform metrics without the biharmonic term")
+    print(f"golden {name} via {path}", flush=True)
+    with BarotropicModel(b) as m:
+        last = 0
+        for step in sorted(int(s) for s in rec["outputs"]):
+            if path == "model_steps":
+                m.model_steps(step - last, nstep=run["nstep"], ksw4=run["ksw4"])
+            else:
+                m.run(step - last, "fp64_exact", nstep=run["nstep"])
+            last = step
+            got = m.get_state()
+            names = {"ssh": "ssh", "sshp": "sshp", "u": "u", "up": "up", "v": "v", "vp": "vp"}
+            if path == "model_steps":
+                got.update(m.means())
+                got.update(m.depths())
+            bad = [k for k, d in rec["outputs"][str(step)].items()
+                   if k in got and _sha(got[k]) != d["sha256"]]
+            assert not bad, f"{name} step {step} via {path}: {bad}"

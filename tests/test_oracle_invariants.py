@@ -1,0 +1,222 @@
+"""Checks that pin the oracle by what the mathematics fixes, independent of
+the compiled reference: rest state, volume conservation, closed forms of
+the depth averages, stress tensor and Rayleigh coefficient, hand-derived
+continuity increments, the C-grid mask definitions, gravity-wave speed,
+mirror symmetry and (for the periodic wrap, which the shipped reference
+build does not exercise) translation invariance."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from inmsom_bt.cases import make_basin, randomize_state
+
+
+def _flat(nxi, nyi, h=50.0, bumps=(), **kw):
+    """All-wet rectangle of constant depth h; surface bumps (amplitude,
+    x-centre, y-centre [cells], radius [m]) on the wet cells."""
+    b = make_basin(nxi, nyi, seed=0, sigma=2.0, bumps=[], **kw)
+    wet = np.zeros((b.ny, b.nx), dtype=bool)
+    wet[2:-2, 2:-2] = True
+    b.lbasins = np.where(wet, 0, 1).astype(np.int32)
+    b.hhq_rest = np.where(wet, h, 0.0)
+    b.mu = b.mu * 0 + 10.0 * wet
+    xc = (np.arange(b.nx) - 2 + 0.5)[None, :]
+    yc = (np.arange(b.ny) - 2 + 0.5)[:, None]
+    ssh = np.zeros((b.ny, b.nx))
+    for amp, x0, y0, rad in bumps:
+        ssh += amp * np.exp(-(((xc - x0) * 1000.0) ** 2 + ((yc - y0) * 1000.0) ** 2) / rad ** 2)
+    b.ssh = ssh * wet
+    b.sshp = b.ssh.copy()
+    return b
+
+
+def test_rest_state_stays_exactly_at_rest():
+    b = make_basin(60, 40, seed=5, sigma=4.0, tau0=0.0, bumps=[])
+    out = oracle.run(b, 25, 2, out_steps=[25])[25]
+    for k in ("ssh", "sshp", "u", "up", "v", "vp", "ubrtr_i", "vbrtr_i"):
+        assert not np.any(out[k]), k
+
+
+def test_volume_conserved_in_closed_basin():
+    b = make_basin(80, 60, seed=6, sigma=5.0)
+    og = oracle.OracleGrid(b)
+    v0 = og.volume(b.ssh)
+    outs = oracle.run(b, 60, 2, out_steps=[20, 60], grid=og)
+    for s, o in outs.items():
+        v = og.volume(o["ssh"])
+        assert abs(v - v0) <= 1e-12 * abs(v0), (s, v, v0)
+
+
+def test_hh_init_closed_form_uniform_metrics():
+    b = randomize_state(make_basin(40, 30, seed=7, sigma=3.0), 3)
+    og = oracle.OracleGrid(b)
+    ds, da = og.new_depths()
+    import ctypes as C
+    oracle.lib().bto_hh_init(og.ref, C.byref(ds), oracle._d(np.ascontiguousarray(b.ssh)),
+                             oracle._d(np.ascontiguousarray(b.sshp)))
+    hq = b.hhq_rest + b.ssh
+    lu = og.masks["lu"].astype(np.float64)
+    llu = og.masks["llu"] > 0.5
+    num = hq[:, :-1] * lu[:, :-1] + hq[:, 1:] * lu[:, 1:]
+    den = lu[:, :-1] + lu[:, 1:]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        want = np.where(llu[:, :-1], num / den, 0.0)
+    got = da[3][:, :-1]
+    assert np.allclose(got, want, rtol=1e-15, atol=0), np.max(np.abs(got - want))
+    # the H-point depth is the wet-weighted mean of the 4 surrounding T depths
+    luh = og.masks["luh"] > 0.5
+    num4 = hq[:-1, :-1] * lu[:-1, :-1] + hq[:-1, 1:] * lu[:-1, 1:] + hq[1:, :-1] * lu[1:, :-1] + \
+        hq[1:, 1:] * lu[1:, 1:]
+    den4 = lu[:-1, :-1] + lu[:-1, 1:] + lu[1:, :-1] + lu[1:, 1:]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        want4 = np.where(luh[:-1, :-1], num4 / den4, 0.0)
+    assert np.allclose(da[9][:-1, :-1], want4, rtol=1e-15, atol=0)
+
+
+def test_stress_of_linear_flow_and_zero_viscous_rhs():
+    """u = a*x, v = b*y on a flat, all-wet basin: tension = a - b exactly in the
+    interior, shear = 0, so the harmonic viscous force vanishes wherever
+    the stencil sees only interior wet points."""
+    b = _flat(30, 24)
+    a, c = 2e-5, -3e-5
+    x_u = (np.arange(b.nx) + 0.5) * 1000.0
+    y_v = (np.arange(b.ny) + 0.5) * 1000.0
+    og = oracle.OracleGrid(b)
+    lcu, lcv = og.masks["lcu"] > 0.5, og.masks["lcv"] > 0.5
+    b.up = np.where(lcu, a * x_u[None, :], 0.0)
+    b.vp = np.where(lcv, c * y_v[:, None], 0.0)
+    L, d = oracle.lib(), oracle._d
+    z = lambda: np.zeros((b.ny, b.nx))
+    strt, strs, rx, ry = z(), z(), z(), z()
+    L.bto_stress_components(og.ref, d(b.up), d(b.vp), d(strt), d(strs))
+    hq = np.ascontiguousarray(b.hhq_rest)  # flat depth at every point type
+    mu = np.ascontiguousarray(b.mu)
+    L.bto_uv_diff2(og.ref, d(mu), d(strt), d(strs), d(hq), d(hq), d(hq), d(hq), d(rx), d(ry))
+    assert np.allclose(strt[4:-4, 4:-4], a - c, rtol=1e-12, atol=0)
+    assert np.max(np.abs(strs[4:-4, 4:-4])) < 1e-18
+    scale = 10.0 * 50.0 * abs(a - c) * 1000.0  # mu * H * |stress| * dy
+    assert np.max(np.abs(rx[5:-5, 5:-5])) < 1e-9 * scale
+    assert np.max(np.abs(ry[5:-5, 5:-5])) < 1e-9 * scale
+
+
+def test_rayleigh_closed_form_linear_slope():
+    b = _flat(20, 16)
+    gx, gy = 3e-3, -4e-3
+    xx = np.arange(b.nx)[None, :] * 1000.0
+    yy = np.arange(b.ny)[:, None] * 1000.0
+    wet = b.lbasins == 0
+    b.hhq_rest = np.where(wet, 100.0 + gx * xx + gy * yy, 0.0)
+    r = oracle.OracleGrid(b).rayleigh(1e-4)
+    want = 1e-4 * math.hypot(gx, gy)
+    assert np.allclose(r[3:-3, 3:-3], want, rtol=1e-9)
+    # at a coast the one-sided slope is used: west column sees only its east face
+    assert np.isclose(r[5, 2], 1e-4 * math.hypot(gx, gy), rtol=1e-9)
+
+
+def test_continuity_single_transport_by_hand():
+    """One nonzero u on a flat basin at rest: after one substep the two T
+    cells beside that U face change by -/+ 2 tau u h dy / (dx dy)."""
+    b = _flat(12, 10, h=40.0, tau0=0.0, bumps=[])
+    b.cb_nl = 0.0
+    m0, n0 = 7, 6  # 0-based U point between T(7,6) and T(8,6)
+    b.u[n0, m0] = b.up[n0, m0] = 0.3
+    rd = np.zeros((b.ny, b.nx))
+    out = oracle.run(b, 1, 1, out_steps=[1], rdis=rd)[1]
+    tau = b.dt
+    d = 2 * tau * 0.3 * 40.0 * 1000.0 / (1000.0 * 1000.0)
+    # Asselin: sshp = ssh + gamma/2 (sshn - 2 ssh + sshp) with ssh = sshp = 0
+    assert math.isclose(out["ssh"][n0, m0], -d, rel_tol=1e-14)
+    assert math.isclose(out["ssh"][n0, m0 + 1], d, rel_tol=1e-14)
+    assert math.isclose(out["sshp"][n0, m0 + 1], 0.025 * d, rel_tol=1e-13)
+    others = out["ssh"].copy()
+    others[n0, m0] = others[n0, m0 + 1] = 0.0
+    assert not np.any(others)
+
+
+def test_gridcon_brute_force():
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        lb = (rng.random((9, 11)) < 0.35).astype(np.int32)
+        m = oracle.gridcon(lb)
+        lu = (lb == 0).astype(np.float32)
+        ny, nx = lb.shape
+        for n in range(ny):
+            for mm in range(nx):
+                inr = 1 <= mm <= nx - 3 and 1 <= n <= ny - 3
+                four = [lu[n, mm], lu[n, min(mm + 1, nx - 1)], lu[min(n + 1, ny - 1), mm],
+                        lu[min(n + 1, ny - 1), min(mm + 1, nx - 1)]]
+                assert m["lu"][n, mm] == lu[n, mm]
+                assert m["luh"][n, mm] == (1.0 if inr and sum(four) > 0 else 0.0)
+                assert m["luu"][n, mm] == (1.0 if inr and min(four) > 0 else 0.0)
+                assert m["llu"][n, mm] == (1.0 if inr and four[0] + four[1] > 0 else 0.0)
+                assert m["lcu"][n, mm] == (1.0 if inr and four[0] * four[1] > 0 else 0.0)
+                assert m["llv"][n, mm] == (1.0 if inr and four[0] + four[2] > 0 else 0.0)
+                assert m["lcv"][n, mm] == (1.0 if inr and four[0] * four[2] > 0 else 0.0)
+
+
+def test_gravity_wave_speed():
+    """A 1-D ridge of surface height in a long flat channel (f = 0, no
+    friction, no viscosity) splits into two fronts moving at sqrt(g H)."""
+    h = 40.0
+    b = _flat(400, 6, h=h, tau0=0.0, f0=0.0, beta=0.0, nu=0.0, bumps=[])
+    b.mu[:] = 0.0
+    b.cb_nl = 0.0
+    x = (np.arange(b.nx) - 2 + 0.5) * 1000.0
+    eta = 0.01 * np.exp(-((x - 200e3) / 8e3) ** 2)
+    b.ssh = np.where(b.lbasins == 0, eta[None, :], 0.0)
+    b.sshp = b.ssh.copy()
+    steps, nstep = 150, 2
+    rd = np.zeros((b.ny, b.nx))
+    out = oracle.run(b, steps, nstep, out_steps=[steps], rdis=rd)[steps]
+    row = out["ssh"][4, 2:-2]
+    right = np.argmax(row[200:]) + 200
+    dist = (right + 0.5) * 1000.0 - 200e3
+    c = dist / (steps * b.dt)
+    assert abs(c - math.sqrt(b.g * h)) / math.sqrt(b.g * h) < 0.03, c
+
+
+def test_mirror_symmetry_without_rotation():
+    b = _flat(40, 30, h=30.0, tau0=0.0, f0=0.0, beta=0.0,
+              bumps=[(0.3, 20.0, 15.0, 4e3)])
+    steps = 20
+    out = oracle.run(b, steps, 2, out_steps=[steps], rdis=np.zeros((b.ny, b.nx)))[steps]
+    s = out["ssh"][2:-2, 2:-2]
+    assert np.allclose(s, s[:, ::-1], rtol=0, atol=1e-13 * np.max(np.abs(s)))
+    assert np.allclose(s, s[::-1, :], rtol=0, atol=1e-13 * np.max(np.abs(s)))
+
+
+@pytest.mark.parametrize("shift", [3, 11])
+def test_periodic_x_translation_invariance(shift):
+    """Periodic wrap (Service/oc_algorithms.f90:50-95; parity with the
+    compiled reference is not available, its build has periodicity 0):
+    in a zonally periodic channel with forcing uniform along x, shifting the
+    initial state by k columns shifts the solution by k columns."""
+    b = make_basin(48, 20, seed=9, sigma=3.0, bumps=[(0.2, 10.0, 10.0, 5e3)])
+    wet = np.zeros((b.ny, b.nx), dtype=bool)
+    wet[2:-2, :] = True
+    b.lbasins = np.where(wet, 0, 1).astype(np.int32)
+    b.hhq_rest = np.where(wet, 60.0, 0.0)
+    b.mu = 10.0 * wet
+    x = np.arange(b.nx) - 2
+    b.ssh = np.where(wet, 0.2 * np.exp(-(((x % 48) - 10.0) ** 2) / 9.0)[None, :], 0.0)
+    b.sshp = b.ssh.copy()
+    rd = np.zeros((b.ny, b.nx))
+
+    def shifted(a):
+        inner = np.roll(a[:, 2:-2], shift, axis=1)
+        out = a.copy()
+        out[:, 2:-2] = inner
+        out[:, 1] = inner[:, -1]
+        out[:, -2] = inner[:, 0]
+        return out
+
+    b2 = b.copy()
+    for k in ("ssh", "sshp", "u", "up", "v", "vp"):
+        setattr(b2, k, shifted(getattr(b, k)))
+    o1 = oracle.run(b, 12, 2, out_steps=[12], rdis=rd, periodicity=(1, 0))[12]
+    o2 = oracle.run(b2, 12, 2, out_steps=[12], rdis=rd, periodicity=(1, 0))[12]
+    for k in ("ssh", "u", "v"):
+        want = np.roll(o1[k][:, 2:-2], shift, axis=1)
+        assert np.allclose(o2[k][:, 2:-2], want, rtol=0, atol=1e-12 * max(1.0, np.max(np.abs(want))))

@@ -1,0 +1,34 @@
+"""Live comparison of the oracle with the compiled reference harness on
+further random cases (runs where oracle/_ref has been built, i.e. where the
+reference sources are mounted; skipped elsewhere)."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle
+from _util import assert_bitwise
+from inmsom_bt.cases import make_basin, randomize_state, write_case
+
+pytestmark = pytest.mark.skipif(not oracle.have_reference(),
+                                reason="oracle/_ref not built (reference sources not mounted)")
+
+FIELDS = ("ssh", "sshp", "u", "up", "v", "vp", "ubrtr_i", "vbrtr_i", "ssh4gradx", "ssh4grady")
+
+
+@pytest.mark.parametrize("seed,nstep,ksw4,metrics", [(201, 2, 0, "uniform"), (202, 4, 1, "stretched"),
+                                                    (203, 1, 1, "uniform")])
+def test_random_case_bitwise(seed, nstep, ksw4, metrics):
+    b = make_basin(285, 159, seed=seed, sigma=4.0, g=9.8, hmax=120.0, metrics=metrics,
+                   lvisc_4=1e-3 if ksw4 else 0.0)
+    b = randomize_state(b, seed + 1, amp_uv=0.05, wflux_amp=1e-6)
+    rd = oracle.OracleGrid(b).rayleigh()
+    steps = [1, 5]
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "case.bin")
+        write_case(path, b, rd, 5, nstep, ksw4, 0, steps)
+        ref = oracle.run_reference(path, path + ".out", b.nx, b.ny, 0, steps)
+    got = oracle.run(b, 5, nstep, ksw4, out_steps=steps, rdis=rd)
+    for s in steps:
+        assert_bitwise(got[s], ref[s], FIELDS + oracle.DEPTHS, f"seed {seed} step {s}")
