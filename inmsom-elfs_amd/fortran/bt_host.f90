@@ -1,0 +1,175 @@
+! Fortran host for the GPU barotropic step, in the shape of the reference's
+! driver (inmsom_head.f90:1-333 with the 2-D part of Control/dyncall.f90):
+! it reads the run parameters from ocean_run.par / phys_proc.par in the
+! reference's positional format, uploads the grid once, runs the time loop
+! with the range check after every step and reports the same timer the
+! reference prints (time_barotrop).
+!
+! usage: inmsom_bt_host <par dir> <case.bin> <out.bin> [literal|fp64_exact|fp64|fp32] [g]
+!
+!   ocean_run.par  line 2 time_step [s], line 3 run_duration [days],
+!                  line 9 nstep_barotrop                 (inmsom_head.f90:47-55)
+!   phys_proc.par  line 6 ksw_lat4                       (init_pars.f90:17-57)
+!   case.bin       grid, forcing and initial state, written by
+!                  inmsom_bt.cases.write_case (layout in oracle/ref_harness.f90);
+!                  its nsteps field, when positive, overrides run_duration.
+!   out.bin        final ssh_e, sshp_e, ubrtr_e, ubrtrp_e, vbrtr_e, vbrtrp_e (real8)
+!
+! Mode 'literal' runs the stage-by-stage device path (any metrics, the
+! biharmonic term); the fused modes need uniform metrics.
+program inmsom_bt_host
+use iso_c_binding
+use inmsom_bt
+implicit none
+
+character(512) :: pardir, case_path, out_path, arg
+character(256) :: lines(64)
+character(16) :: mode_name
+integer :: nl, ierr, magic, version, nx, ny, nsteps_case, nstep_case, ksw4_case, cmode, nout
+integer :: nstep_barotrop, ksw_lat4, nsteps, step, mode
+integer, allocatable, target :: lbasins(:,:), out_steps(:)
+real(8) :: time_step, run_duration, tau_case, cb_nl, ebottom, g
+real(8), allocatable, target :: f(:,:,:)
+real(8), allocatable, target :: st(:,:,:)
+real(8), allocatable, target :: rdis(:,:)
+integer(c_long_long) :: bad
+integer(8) :: c0, c1, crate
+real(8) :: time_barotrop, vol0, vol1
+type(bt_grid) :: grid
+type(bt_forcing) :: forc
+integer(c_int) :: rc
+! field order inside f: hhq_rest dx dy dxt dyt dxh dyh dxb dyb rlh_s taux tauy wflux mu mu4
+integer, parameter :: NF = 15
+
+call get_command_argument(1, pardir)
+call get_command_argument(2, case_path)
+call get_command_argument(3, out_path)
+mode_name = 'literal'
+if (command_argument_count() >= 4) call get_command_argument(4, mode_name)
+g = 9.81d0
+if (command_argument_count() >= 5) then
+  call get_command_argument(5, arg)
+  read(arg, *) g
+endif
+
+! ---- parameters (positional text files as the reference reads them)
+call read_lines(trim(pardir)//'/ocean_run.par', lines, nl)
+read(lines(2), *) time_step
+read(lines(3), *) run_duration
+read(lines(9), *) nstep_barotrop
+call read_lines(trim(pardir)//'/phys_proc.par', lines, nl)
+read(lines(6), *) ksw_lat4
+
+! ---- grid, forcing and initial state
+open(21, file=trim(case_path), access='stream', form='unformatted', status='old', iostat=ierr)
+if (ierr /= 0) then
+  write(*,*) 'cannot open ', trim(case_path)
+  error stop 5
+endif
+read(21) magic, version, nx, ny, nsteps_case, nstep_case, ksw4_case, cmode, nout
+allocate(out_steps(max(nout, 1)), lbasins(nx, ny), f(nx, ny, NF), st(nx, ny, 6), rdis(nx, ny))
+if (nout > 0) read(21) out_steps(1:nout)
+read(21) tau_case, cb_nl, ebottom
+read(21) lbasins
+read(21) f
+read(21) rdis
+read(21) st            ! ssh, sshp, u, up, v, vp
+close(21)
+
+nsteps = nint(run_duration * 86400.0d0 / time_step)
+if (nsteps_case > 0) nsteps = nsteps_case
+select case (trim(mode_name))
+case ('literal');    mode = -1
+case ('fp64_exact'); mode = INMSOM_BT_FP64_EXACT
+case ('fp64');       mode = INMSOM_BT_FP64
+case ('fp32');       mode = INMSOM_BT_FP32
+case default
+  write(*,*) 'unknown mode ', trim(mode_name)
+  error stop 2
+end select
+
+grid%nx = nx; grid%ny = ny
+grid%periodicity_x = 0; grid%periodicity_y = 0
+grid%full_free_surface = 1
+grid%free_fall_acc = g
+grid%ref_den = 1025.0d0
+grid%time_smooth = 0.05d0
+grid%lbasins = c_loc(lbasins)
+grid%lu = c_null_ptr; grid%luu = c_null_ptr; grid%luh = c_null_ptr; grid%lcu = c_null_ptr
+grid%lcv = c_null_ptr; grid%llu = c_null_ptr; grid%llv = c_null_ptr
+grid%hhq_rest = c_loc(f(1,1,1))
+grid%dx = c_loc(f(1,1,2));  grid%dy = c_loc(f(1,1,3))
+grid%dxt = c_loc(f(1,1,4)); grid%dyt = c_loc(f(1,1,5))
+grid%dxh = c_loc(f(1,1,6)); grid%dyh = c_loc(f(1,1,7))
+grid%dxb = c_loc(f(1,1,8)); grid%dyb = c_loc(f(1,1,9))
+grid%rlh_s = c_loc(f(1,1,10))
+call bt_check(inmsom_bt_init(grid, 0_c_int, bt_ctx), 'inmsom_bt_init')
+
+forc%taux = c_loc(f(1,1,11)); forc%tauy = c_loc(f(1,1,12)); forc%wflux = c_loc(f(1,1,13))
+forc%mu = c_loc(f(1,1,14)); forc%mu4 = c_loc(f(1,1,15)); forc%rdis = c_loc(rdis)
+forc%cb_nl = cb_nl; forc%ebottom = ebottom
+call bt_check(inmsom_bt_set_forcing(bt_ctx, forc), 'inmsom_bt_set_forcing')
+call bt_check(inmsom_bt_set_state(bt_ctx, c_loc(st(1,1,3)), c_loc(st(1,1,4)), c_loc(st(1,1,5)), &
+              c_loc(st(1,1,6)), c_loc(st(1,1,1)), c_loc(st(1,1,2))), 'inmsom_bt_set_state')
+call bt_check(inmsom_bt_volume(bt_ctx, vol0), 'inmsom_bt_volume')
+
+write(*,'(a,i0,a,i0,a,f8.2,a,i0,a,i0,a,a)') ' grid ', nx, ' x ', ny, ', time step ', time_step, &
+      ' s, nstep_barotrop ', nstep_barotrop, ', steps ', nsteps, ', mode ', trim(mode_name)
+
+! ---- time loop (inmsom_head.f90:236-316, dyncall.f90:109-166)
+time_barotrop = 0.0d0
+call system_clock(count_rate=crate)
+do step = 1, nsteps
+  call system_clock(c0)
+  if (mode < 0) then
+    rc = inmsom_bt_model_steps(bt_ctx, time_step, int(nstep_barotrop, c_int), &
+                               int(ksw_lat4, c_int), 1_c_int)
+  else
+    rc = inmsom_bt_run(bt_ctx, time_step, int(nstep_barotrop, c_int), 1_c_int, int(mode, c_int))
+  endif
+  call bt_check(rc, 'barotropic step')
+  call system_clock(c1)
+  time_barotrop = time_barotrop + dble(c1 - c0) / dble(crate)
+  ! range check after every step (dyncall.f90:452-516)
+  rc = inmsom_bt_range_check(bt_ctx, bad)
+  if (rc == INMSOM_BT_ERANGE) then
+    write(*,'(a,i0,a,i0)') ' range check failed at step ', step, ': bad cells ', bad
+    error stop 3
+  endif
+  call bt_check(rc, 'inmsom_bt_range_check')
+enddo
+
+call bt_check(inmsom_bt_get_state(bt_ctx, c_loc(st(1,1,3)), c_loc(st(1,1,4)), c_loc(st(1,1,5)), &
+              c_loc(st(1,1,6)), c_loc(st(1,1,1)), c_loc(st(1,1,2))), 'inmsom_bt_get_state')
+call bt_check(inmsom_bt_volume(bt_ctx, vol1), 'inmsom_bt_volume')
+open(23, file=trim(out_path), access='stream', form='unformatted', status='replace')
+write(23) st
+close(23)
+write(*,'(a,es12.5)') ' Time barotropic: ', time_barotrop
+write(*,'(a,es12.5)') ' volume drift (relative): ', abs(vol1 - vol0) / max(abs(vol0), 1.0d-300)
+call bt_check(inmsom_bt_finalize(bt_ctx), 'inmsom_bt_finalize')
+
+contains
+
+! Service/read_write_parameters.f90:2-41: the file's lines, one per entry.
+subroutine read_lines(path, buf, n)
+  character(*), intent(in) :: path
+  character(256), intent(out) :: buf(:)
+  integer, intent(out) :: n
+  integer :: io
+  n = 0
+  open(31, file=path, status='old', iostat=io)
+  if (io /= 0) then
+    write(*,*) 'cannot open ', path
+    error stop 5
+  endif
+  do
+    if (n >= size(buf)) exit
+    read(31, '(a)', iostat=io) buf(n + 1)
+    if (io /= 0) exit
+    n = n + 1
+  enddo
+  close(31)
+end subroutine read_lines
+
+end program inmsom_bt_host

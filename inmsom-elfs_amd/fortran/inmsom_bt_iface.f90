@@ -1,0 +1,188 @@
+! ISO_C_BINDING interface of libinmsom_bt (include/inmsom_bt.h).
+!
+! A Fortran host uses this module in place of the reference's
+! Function/vel_ssh.f90 barotropic part:
+!   * bt_gpu_init(...)            uploads the static grid once
+!                                 (the module globals of Modules/mod_basin_grid.f90);
+!   * barotropic_dynamics_gpu(...) has the argument list of the reference's
+!                                 barotropic_dynamics (Function/vel_ssh.f90:379-406)
+!                                 and runs it on the GPU;
+!   * the inmsom_bt_* interfaces expose the whole C ABI.
+module inmsom_bt
+use iso_c_binding
+implicit none
+
+integer(c_int), parameter :: INMSOM_BT_OK = 0, INMSOM_BT_EINVAL = -1, INMSOM_BT_EDEVICE = -2, &
+                             INMSOM_BT_ERANGE = -3, INMSOM_BT_ENOMEM = -4, INMSOM_BT_ESTATE = -5
+integer(c_int), parameter :: INMSOM_BT_FP64_EXACT = 0, INMSOM_BT_FP64 = 1, INMSOM_BT_FP32 = 2
+
+! struct inmsom_bt_grid
+type, bind(C) :: bt_grid
+  integer(c_int) :: nx, ny, periodicity_x, periodicity_y, full_free_surface
+  real(c_double) :: free_fall_acc, ref_den, time_smooth
+  type(c_ptr) :: lbasins
+  type(c_ptr) :: lu, luu, luh, lcu, lcv, llu, llv
+  type(c_ptr) :: dx, dy, dxt, dyt, dxh, dyh, dxb, dyb
+  type(c_ptr) :: rlh_s, hhq_rest
+end type bt_grid
+
+! struct inmsom_bt_forcing
+type, bind(C) :: bt_forcing
+  type(c_ptr) :: taux, tauy, wflux, mu, mu4, rdis
+  real(c_double) :: cb_nl, ebottom
+end type bt_forcing
+
+! the context barotropic_dynamics_gpu works on
+type(c_ptr), save :: bt_ctx = c_null_ptr
+
+interface
+  function inmsom_bt_last_error() bind(C, name="inmsom_bt_last_error")
+    import :: c_ptr
+    type(c_ptr) :: inmsom_bt_last_error
+  end function
+  integer(c_int) function inmsom_bt_gridcon(nx, ny, px, py, lbasins, lu, luu, luh, lcu, lcv, llu, llv) &
+      bind(C, name="inmsom_bt_gridcon")
+    import :: c_int, c_ptr
+    integer(c_int), value :: nx, ny, px, py
+    type(c_ptr), value :: lbasins, lu, luu, luh, lcu, lcv, llu, llv
+  end function
+  integer(c_int) function inmsom_bt_rayleigh(grid, r_fric, r_diss) bind(C, name="inmsom_bt_rayleigh")
+    import :: c_int, c_double, c_ptr, bt_grid
+    type(bt_grid), intent(in) :: grid
+    real(c_double), value :: r_fric
+    type(c_ptr), value :: r_diss
+  end function
+  integer(c_int) function inmsom_bt_init(grid, device, ctx) bind(C, name="inmsom_bt_init")
+    import :: c_int, c_ptr, bt_grid
+    type(bt_grid), intent(in) :: grid
+    integer(c_int), value :: device
+    type(c_ptr), intent(out) :: ctx
+  end function
+  integer(c_int) function inmsom_bt_finalize(ctx) bind(C, name="inmsom_bt_finalize")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+  end function
+  integer(c_int) function inmsom_bt_barotropic_dynamics(ctx, tau, nstep, ksw4, ubrtr_e, ubrtrp_e, &
+      vbrtr_e, vbrtrp_e, ssh_e, sshp_e, ubrtr_i, vbrtr_i, ssh4gradx, ssh4grady, rhsx, rhsy, wflux, &
+      mu, mu4, rdis) bind(C, name="inmsom_bt_barotropic_dynamics")
+    import :: c_int, c_double, c_ptr
+    type(c_ptr), value :: ctx
+    real(c_double), value :: tau
+    integer(c_int), value :: nstep, ksw4
+    type(c_ptr), value :: ubrtr_e, ubrtrp_e, vbrtr_e, vbrtrp_e, ssh_e, sshp_e, ubrtr_i, vbrtr_i, &
+                          ssh4gradx, ssh4grady, rhsx, rhsy, wflux, mu, mu4, rdis
+  end function
+  integer(c_int) function inmsom_bt_get_depths(ctx, h) bind(C, name="inmsom_bt_get_depths")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+    type(c_ptr), intent(in) :: h(12)
+  end function
+  integer(c_int) function inmsom_bt_set_forcing(ctx, f) bind(C, name="inmsom_bt_set_forcing")
+    import :: c_int, c_ptr, bt_forcing
+    type(c_ptr), value :: ctx
+    type(bt_forcing), intent(in) :: f
+  end function
+  integer(c_int) function inmsom_bt_set_state(ctx, u, up, v, vp, ssh, sshp) bind(C, name="inmsom_bt_set_state")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx, u, up, v, vp, ssh, sshp
+  end function
+  integer(c_int) function inmsom_bt_get_state(ctx, u, up, v, vp, ssh, sshp) bind(C, name="inmsom_bt_get_state")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx, u, up, v, vp, ssh, sshp
+  end function
+  integer(c_int) function inmsom_bt_get_means(ctx, ubrtr_i, vbrtr_i, ssh4gradx, ssh4grady) &
+      bind(C, name="inmsom_bt_get_means")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx, ubrtr_i, vbrtr_i, ssh4gradx, ssh4grady
+  end function
+  integer(c_int) function inmsom_bt_model_steps(ctx, tau, nstep, ksw4, nsteps) bind(C, name="inmsom_bt_model_steps")
+    import :: c_int, c_double, c_ptr
+    type(c_ptr), value :: ctx
+    real(c_double), value :: tau
+    integer(c_int), value :: nstep, ksw4, nsteps
+  end function
+  integer(c_int) function inmsom_bt_run(ctx, tau, nstep, nsteps, mode) bind(C, name="inmsom_bt_run")
+    import :: c_int, c_double, c_ptr
+    type(c_ptr), value :: ctx
+    real(c_double), value :: tau
+    integer(c_int), value :: nstep, nsteps, mode
+  end function
+  integer(c_int) function inmsom_bt_run_timed(ctx, tau, nstep, nsteps, mode, ms) bind(C, name="inmsom_bt_run_timed")
+    import :: c_int, c_double, c_ptr
+    type(c_ptr), value :: ctx
+    real(c_double), value :: tau
+    integer(c_int), value :: nstep, nsteps, mode
+    real(c_double), intent(out) :: ms
+  end function
+  integer(c_int) function inmsom_bt_set_launch(ctx, rows) bind(C, name="inmsom_bt_set_launch")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: rows
+  end function
+  integer(c_int) function inmsom_bt_get_launch(ctx, rows) bind(C, name="inmsom_bt_get_launch")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int), intent(out) :: rows
+  end function
+  integer(c_int) function inmsom_bt_range_check(ctx, bad) bind(C, name="inmsom_bt_range_check")
+    import :: c_int, c_long_long, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_long_long), intent(out) :: bad
+  end function
+  integer(c_int) function inmsom_bt_volume(ctx, vol) bind(C, name="inmsom_bt_volume")
+    import :: c_int, c_double, c_ptr
+    type(c_ptr), value :: ctx
+    real(c_double), intent(out) :: vol
+  end function
+  integer(c_int) function inmsom_bt_synchronize(ctx) bind(C, name="inmsom_bt_synchronize")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+  end function
+  function c_strlen(s) bind(C, name="strlen")
+    import :: c_ptr, c_size_t
+    type(c_ptr), value :: s
+    integer(c_size_t) :: c_strlen
+  end function
+end interface
+
+contains
+
+! Stops with the library's message when status is not INMSOM_BT_OK
+! (the reference stops the run on a failed range check, dyncall.f90:452-516).
+subroutine bt_check(status, where)
+  integer(c_int), intent(in) :: status
+  character(*), intent(in) :: where
+  character(kind=c_char), pointer :: msg(:)
+  type(c_ptr) :: p
+  integer :: n
+  if (status == INMSOM_BT_OK) return
+  p = inmsom_bt_last_error()
+  n = int(c_strlen(p))
+  call c_f_pointer(p, msg, [max(n, 1)])
+  write(*, '(a,a,i0,a,*(a))') trim(where), ' failed, status ', status, ': ', msg(1:n)
+  error stop 1
+end subroutine bt_check
+
+! The reference's barotropic_dynamics argument list (Function/vel_ssh.f90:379-439)
+! on the context bt_ctx.  The scratch arguments vort .. RHSy_dif are not
+! written: the reference's caller recomputes them before any use
+! (Control/dyncall.f90:39,65,72-84).
+subroutine barotropic_dynamics_gpu(tau, nstep, ksw4, ubrtr_e, ubrtrp_e, vbrtr_e, vbrtrp_e, ssh_e, &
+                                   sshp_e, ubrtr_i, vbrtr_i, ssh4gradx, ssh4grady, RHSx, RHSy, wflux, &
+                                   mu, mu4, vort, str_t2d, str_s2d, fx, fy, rdis, RHSx_adv, RHSy_adv, &
+                                   RHSx_dif, RHSy_dif)
+  real(8), intent(in) :: tau
+  integer, intent(in) :: nstep, ksw4
+  real(8), target, intent(inout) :: ubrtr_e(*), ubrtrp_e(*), vbrtr_e(*), vbrtrp_e(*), ssh_e(*), sshp_e(*)
+  real(8), target, intent(inout) :: ubrtr_i(*), vbrtr_i(*), ssh4gradx(*), ssh4grady(*)
+  real(8), target, intent(in) :: RHSx(*), RHSy(*), wflux(*), mu(*), mu4(*), rdis(*)
+  real(8), intent(inout) :: vort(*), str_t2d(*), str_s2d(*), fx(*), fy(*)
+  real(8), intent(inout) :: RHSx_adv(*), RHSy_adv(*), RHSx_dif(*), RHSy_dif(*)
+  call bt_check(inmsom_bt_barotropic_dynamics(bt_ctx, real(tau, c_double), int(nstep, c_int), &
+                int(ksw4, c_int), c_loc(ubrtr_e), c_loc(ubrtrp_e), c_loc(vbrtr_e), c_loc(vbrtrp_e), &
+                c_loc(ssh_e), c_loc(sshp_e), c_loc(ubrtr_i), c_loc(vbrtr_i), c_loc(ssh4gradx), &
+                c_loc(ssh4grady), c_loc(RHSx), c_loc(RHSy), c_loc(wflux), c_loc(mu), c_loc(mu4), &
+                c_loc(rdis)), 'barotropic_dynamics_gpu')
+end subroutine barotropic_dynamics_gpu
+
+end module inmsom_bt
