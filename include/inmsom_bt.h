@@ -164,6 +164,46 @@ int inmsom_bt_volume(inmsom_bt_ctx *ctx, double *volume);
 
 int inmsom_bt_synchronize(inmsom_bt_ctx *ctx);
 
+/* ---- batched ensembles --------------------------------------------- */
+
+/* Many independent closed basins of different sizes advanced together:
+ * every substep of every member is one persistent launch over a work list
+ * of (member, strip, rows) items, and the launches of one outer step are
+ * replayed from a HIP graph.  Members need what inmsom_bt_run needs
+ * (uniform metrics, forcing uniform along x, mu = constant * lu) and must
+ * share g, rho0, time_smooth and the drag parameters.  grids[k] and
+ * forcing[k] describe member k (forcing[k].rdis must be set).  State per
+ * member follows inmsom_bt_set_state/get_state. */
+typedef struct inmsom_bt_ens inmsom_bt_ens;
+
+int inmsom_bt_ens_create(int nmembers, const inmsom_bt_grid *grids,
+                         const inmsom_bt_forcing *forcing, int mode, int device,
+                         inmsom_bt_ens **out);
+int inmsom_bt_ens_destroy(inmsom_bt_ens *ens);
+/* member count, wet interior cells and array cells summed over members */
+int inmsom_bt_ens_size(inmsom_bt_ens *ens, int *nmembers, long long *wet_cells, long long *cells);
+int inmsom_bt_ens_set_state(inmsom_bt_ens *ens, int member, const double *u, const double *up,
+                            const double *v, const double *vp, const double *ssh,
+                            const double *sshp);
+int inmsom_bt_ens_get_state(inmsom_bt_ens *ens, int member, double *u, double *up, double *v,
+                            double *vp, double *ssh, double *sshp);
+/* launch shape: output rows per wave (0 = default) and waves per XCD of
+ * the persistent grid (0 = default); results do not depend on it */
+int inmsom_bt_ens_set_launch(inmsom_bt_ens *ens, int rows_per_wave, int waves_per_xcd);
+/* nsteps outer steps of tau, nstep substeps each; use_graph = 1 replays a
+ * captured graph per outer step, 0 launches directly */
+int inmsom_bt_ens_run(inmsom_bt_ens *ens, double tau, int nstep, int nsteps, int use_graph);
+int inmsom_bt_ens_run_timed(inmsom_bt_ens *ens, double tau, int nstep, int nsteps, int use_graph,
+                            double *ms);
+/* device-side snapshot of all members' state and its restore (used by the
+ * launch-shape autotuner) */
+int inmsom_bt_ens_save(inmsom_bt_ens *ens);
+int inmsom_bt_ens_restore(inmsom_bt_ens *ens);
+/* range check over all members (INMSOM_BT_ERANGE when *bad > 0) */
+int inmsom_bt_ens_range_check(inmsom_bt_ens *ens, long long *bad);
+/* FP64 volume anomaly of one member (fixed-order sum) */
+int inmsom_bt_ens_volume(inmsom_bt_ens *ens, int member, double *volume);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,692 @@
+// Batched ensembles: many independent basins of uneven size advanced by
+// one persistent launch per substep, replayed from a HIP graph.
+//
+// Every member keeps its own fields (the fused step's layout, see
+// bt_fused.hip) and a descriptor (FusedParams) in a device table, one
+// table per ping-pong phase.  The work list enumerates (member, strip,
+// chunk) items of all members; a fixed grid of waves walks it with a
+// static stride, so every wave ends after its last item.  Items are dealt
+// to the 8 XCDs in contiguous bands (a member's neighbouring chunks meet in
+// one L2).  One outer step = nstep launches (substeps); its launches are
+// captured once into a graph and the graph is replayed per outer step.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bt_fused_body.h"
+#include "bt_internal.h"
+
+namespace ibt {
+
+template <typename R, bool EXACT>
+__global__ void __launch_bounds__(WAVE)
+    k_fused_batch(const FusedParams *__restrict__ table, const int *__restrict__ item_member,
+                  const int *__restrict__ item_local, int total, int per_xcd, int waves_per_xcd,
+                  double tau_inner, int sub, int nsub)
+{
+    const int w = blockIdx.x;
+    const int xcd = w % NXCD, j = w / NXCD;
+    const int t0 = xcd * per_xcd, t1 = min(t0 + per_xcd, total);
+    for (int t = t0 + j; t < t1; t += waves_per_xcd) {
+        FusedParams p = table[item_member[t]];
+        p.tau = tau_inner;
+        p.sub = sub;
+        p.nsub = nsub;
+        p.dn = (double)nsub;
+        const int local = item_local[t];
+        const int chunk_id = local / p.strips, strip = local - chunk_id * p.strips;
+        fused_chunk<R, EXACT>(p, chunk_id, strip);
+    }
+}
+
+template <class R>
+__global__ void k_member_range(int nx, int ny, const uint8_t *mb, const R *ssh, const R *u, const R *v,
+                               unsigned long long *bad)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nx * ny) return;
+    int m = (int)(i % nx) + 1, n = (int)(i / nx) + 1;
+    if (m < 3 || m > nx - 2 || n < 3 || n > ny - 2) return;
+    bool b = false;
+    if (mb[i] & MB_LU) b |= !(fabs((double)ssh[i]) < 10.0);
+    if (mb[i] & MB_LCU) b |= !(fabs((double)u[i]) < 100.0);
+    if (mb[i] & MB_LCV) b |= !(fabs((double)v[i]) < 100.0);
+    if (b) atomicAdd(bad, 1ull);
+}
+
+// per-row partial sums of ssh over wet interior cells (fixed order)
+template <class R>
+__global__ void k_member_rows(int nx, int ny, const uint8_t *mb, const R *ssh, double area,
+                              double *rows)
+{
+    __shared__ double part[256];
+    const int n = blockIdx.x + 3;
+    double acc = 0.0;
+    for (int m = threadIdx.x + 3; m <= nx - 2; m += blockDim.x) {
+        size_t i = (size_t)(n - 1) * nx + (m - 1);
+        if (mb[i] & MB_LU) acc += (double)ssh[i] * area;
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) rows[blockIdx.x] = part[0];
+}
+
+}  // namespace ibt
+
+using namespace ibt;
+
+struct EnsMember {
+    int nx = 0, ny = 0;
+    size_t n = 0;
+    double d = 0.0, area = 0.0;
+    uint8_t *mb = nullptr;
+    void *H = nullptr, *rd = nullptr;
+    void *lvl[2][3] = {};
+    double *flt[2][3] = {};
+    void *frx = nullptr, *fry = nullptr;
+    double *hup_c = nullptr, *hvp_c = nullptr;
+    double *row_f = nullptr, *row_sx = nullptr, *row_sy = nullptr, *row_wf = nullptr;
+    double nu = 0.0;
+    int items = 0;
+};
+
+struct inmsom_bt_ens {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int mode = INMSOM_BT_FP32;
+    double g = 9.81, rho0 = 1025.0, ts = 0.05, cb = 2.5e-3, eb = 25e-4, ffs = 1.0;
+    std::vector<EnsMember> mem;
+    FusedParams *table[2] = {};
+    int *item_member = nullptr, *item_local = nullptr;
+    int total = 0;
+    int chunk = 0;          // rows per wave, 0 = per-mode default
+    int waves_per_xcd = 0;  // persistent grid, 0 = default
+    int built_chunk = -1;
+    int phase = 0;          // buffer set holding the current state
+    // graph of one outer step for (tau, nstep, start phase)
+    hipGraphExec_t gexec[2] = {};
+    double g_tau = -1.0;
+    int g_nstep = -1, g_chunk = -1, g_wpx = -1;
+    double *rows = nullptr;
+    unsigned long long *cnt = nullptr;
+    // snapshot of every member's state (inmsom_bt_ens_save / _restore)
+    std::vector<void *> snap;
+};
+
+namespace {
+
+int fail(const std::string &m, int code)
+{
+    set_error(m);
+    return code;
+}
+
+#define EHC(expr)                                                                            \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) return fail(std::string(#expr) + ": " + hipGetErrorString(e_), \
+                                          INMSOM_BT_EDEVICE);                                \
+    } while (0)
+
+size_t esz(int mode) { return mode == INMSOM_BT_FP32 ? 4 : 8; }
+
+int default_chunk(int mode) { return mode == INMSOM_BT_FP32 ? 20 : 32; }
+
+int default_waves(int mode)
+{
+    // 32 CUs per XCD x 4 SIMDs x the fused kernel's occupancy (2 FP64, 3 FP32)
+    return mode == INMSOM_BT_FP32 ? 384 : 256;
+}
+
+template <class T>
+int upload_vec(inmsom_bt_ens *e, T **dst, const std::vector<T> &src)
+{
+    EHC(hipMalloc(dst, src.size() * sizeof(T)));
+    EHC(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    (void)e;
+    return INMSOM_BT_OK;
+}
+
+int upload_field(inmsom_bt_ens *e, void **dst, const double *src, size_t n)
+{
+    if (e->mode == INMSOM_BT_FP32) {
+        std::vector<float> t(n);
+        for (size_t i = 0; i < n; ++i) t[i] = (float)src[i];
+        return upload_vec(e, reinterpret_cast<float **>(dst), t);
+    }
+    std::vector<double> t(src, src + n);
+    return upload_vec(e, reinterpret_cast<double **>(dst), t);
+}
+
+// Builds one member from its grid and forcing (host side, as fused_prepare).
+int build_member(inmsom_bt_ens *e, const inmsom_bt_grid *gr, const inmsom_bt_forcing *f,
+                 EnsMember &mb)
+{
+    if (!gr || !f) return fail("null member grid/forcing", INMSOM_BT_EINVAL);
+    if (gr->nx < 5 || gr->ny < 5) return fail("member too small", INMSOM_BT_EINVAL);
+    if (!gr->lbasins && !gr->lu) return fail("member needs lbasins or masks", INMSOM_BT_EINVAL);
+    if (gr->periodicity_x || gr->periodicity_y || gr->full_free_surface != 1)
+        return fail("ensemble members must be closed basins with full_free_surface = 1",
+                    INMSOM_BT_EINVAL);
+    const int nx = gr->nx, ny = gr->ny;
+    const size_t n = (size_t)nx * ny;
+    mb.nx = nx;
+    mb.ny = ny;
+    mb.n = n;
+    std::vector<float> mk[7];
+    const float *m7[7] = {gr->lu, gr->luu, gr->luh, gr->lcu, gr->lcv, gr->llu, gr->llv};
+    if (!gr->lu) {
+        float *p7[7];
+        for (int k = 0; k < 7; ++k) {
+            mk[k].assign(n, 0.0f);
+            p7[k] = mk[k].data();
+        }
+        int rc = inmsom_bt_gridcon(nx, ny, 0, 0, gr->lbasins, p7[0], p7[1], p7[2], p7[3], p7[4], p7[5],
+                                   p7[6]);
+        if (rc) return rc;
+        for (int k = 0; k < 7; ++k) m7[k] = p7[k];
+    }
+    std::vector<uint8_t> bits(n, 0);
+    const uint8_t bit[7] = {MB_LU, MB_LUU, MB_LUH, MB_LCU, MB_LCV, MB_LLU, MB_LLV};
+    for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < 7; ++k)
+            if (m7[k][i] > 0.5f) bits[i] |= bit[k];
+    // uniform metrics
+    const double *met[8] = {gr->dx, gr->dy, gr->dxt, gr->dyt, gr->dxh, gr->dyh, gr->dxb, gr->dyb};
+    const double d = gr->dx[0];
+    for (int k = 0; k < 8; ++k)
+        for (size_t i = 0; i < n; ++i)
+            if (met[k][i] != d) return fail("ensemble members need uniform metrics", INMSOM_BT_EINVAL);
+    mb.d = d;
+    mb.area = d * d;
+    // viscosity = nu on wet cells
+    double nu = -1.0;
+    for (size_t i = 0; i < n; ++i) {
+        if (bits[i] & MB_LU) {
+            if (nu < 0.0) nu = f->mu[i];
+            if (f->mu[i] != nu) return fail("member mu must be constant on wet cells", INMSOM_BT_EINVAL);
+        } else if (f->mu[i] != 0.0) {
+            return fail("member mu must be 0 on land", INMSOM_BT_EINVAL);
+        }
+    }
+    mb.nu = nu < 0.0 ? 0.0 : nu;
+    // row tables (Function/fluxes.f90:30-36 with x-uniform wind)
+    std::vector<double> rf(ny), rsx(ny), rsy(ny), rwf(ny);
+    for (int r = 0; r < ny; ++r) {
+        const size_t o = (size_t)r * nx;
+        for (int m = 1; m < nx; ++m)
+            if (f->taux[o + m] != f->taux[o] || f->tauy[o + m] != f->tauy[o] ||
+                f->wflux[o + m] != f->wflux[o] || gr->rlh_s[o + m] != gr->rlh_s[o])
+                return fail("member wind, water flux and Coriolis must be uniform along x",
+                            INMSOM_BT_EINVAL);
+        rf[r] = gr->rlh_s[o];
+        rwf[r] = f->wflux[o];
+        rsx[r] = (f->taux[o] * d * d + f->taux[o] * d * d) / gr->ref_den / 2.0 / d / d;
+        rsy[r] = (f->tauy[o] * d * d + f->tauy[o] * d * d) / gr->ref_den / 2.0 / d / d;
+    }
+    int rc;
+    if ((rc = upload_vec(e, &mb.mb, bits)) || (rc = upload_vec(e, &mb.row_f, rf)) ||
+        (rc = upload_vec(e, &mb.row_sx, rsx)) || (rc = upload_vec(e, &mb.row_sy, rsy)) ||
+        (rc = upload_vec(e, &mb.row_wf, rwf)))
+        return rc;
+    if ((rc = upload_field(e, &mb.H, gr->hhq_rest, n)) || (rc = upload_field(e, &mb.rd, f->rdis, n)))
+        return rc;
+    const size_t es = esz(e->mode);
+    for (int b = 0; b < 2; ++b)
+        for (int k = 0; k < 3; ++k) {
+            EHC(hipMalloc(&mb.lvl[b][k], n * es));
+            EHC(hipMemset(mb.lvl[b][k], 0, n * es));
+            EHC(hipMalloc(&mb.flt[b][k], n * 8));
+            EHC(hipMemset(mb.flt[b][k], 0, n * 8));
+        }
+    EHC(hipMalloc(&mb.frx, n * es));
+    EHC(hipMalloc(&mb.fry, n * es));
+    EHC(hipMalloc(&mb.hup_c, n * 8));
+    EHC(hipMalloc(&mb.hvp_c, n * 8));
+    return INMSOM_BT_OK;
+}
+
+FusedParams member_params(const inmsom_bt_ens *e, const EnsMember &mb, int phase, int chunk)
+{
+    FusedParams p{};
+    p.nx = mb.nx;
+    p.ny = mb.ny;
+    p.strips = (mb.nx + 2 + SW - 1) / SW;
+    p.chunk = chunk;
+    p.nchunks = (mb.ny + chunk - 1) / chunk;
+    p.total = p.strips * p.nchunks;
+    p.per_xcd = 0;
+    p.mb = mb.mb;
+    p.H = mb.H;
+    p.rd = mb.rd;
+    const int b = phase, o = 1 - phase;
+    p.u = mb.lvl[b][0];
+    p.v = mb.lvl[b][1];
+    p.s = mb.lvl[b][2];
+    p.up = mb.flt[b][0];
+    p.vp = mb.flt[b][1];
+    p.sp = mb.flt[b][2];
+    p.uo = mb.lvl[o][0];
+    p.vo = mb.lvl[o][1];
+    p.so = mb.lvl[o][2];
+    p.upo = mb.flt[o][0];
+    p.vpo = mb.flt[o][1];
+    p.spo = mb.flt[o][2];
+    p.row_f = mb.row_f;
+    p.row_sx = mb.row_sx;
+    p.row_sy = mb.row_sy;
+    p.row_wf = mb.row_wf;
+    p.g = e->g;
+    p.rho0 = e->rho0;
+    p.ts = e->ts;
+    p.cb = e->cb;
+    p.eb = e->eb;
+    p.nu = mb.nu;
+    p.ffs = e->ffs;
+    p.d = mb.d;
+    p.frx = mb.frx;
+    p.fry = mb.fry;
+    p.hup_c = mb.hup_c;
+    p.hvp_c = mb.hvp_c;
+    return p;
+}
+
+// (Re)builds the descriptor tables and the work list for a launch shape.
+int build_tables(inmsom_bt_ens *e, int chunk)
+{
+    if (e->built_chunk == chunk) return INMSOM_BT_OK;
+    std::vector<FusedParams> t[2];
+    std::vector<int> im, il;
+    for (size_t k = 0; k < e->mem.size(); ++k) {
+        for (int ph = 0; ph < 2; ++ph) t[ph].push_back(member_params(e, e->mem[k], ph, chunk));
+        const int items = t[0].back().total;
+        e->mem[k].items = items;
+        for (int i = 0; i < items; ++i) {
+            im.push_back((int)k);
+            il.push_back(i);
+        }
+    }
+    for (int ph = 0; ph < 2; ++ph) {
+        if (e->table[ph]) EHC(hipFree(e->table[ph]));
+        e->table[ph] = nullptr;
+        int rc = upload_vec(e, &e->table[ph], t[ph]);
+        if (rc) return rc;
+    }
+    if (e->item_member) EHC(hipFree(e->item_member));
+    if (e->item_local) EHC(hipFree(e->item_local));
+    e->item_member = e->item_local = nullptr;
+    int rc;
+    if ((rc = upload_vec(e, &e->item_member, im)) || (rc = upload_vec(e, &e->item_local, il))) return rc;
+    e->total = (int)im.size();
+    e->built_chunk = chunk;
+    return INMSOM_BT_OK;
+}
+
+void launch_batch(inmsom_bt_ens *e, int phase, double tau_inner, int sub, int nsub, int wpx)
+{
+    const int per_xcd = (e->total + NXCD - 1) / NXCD;
+    dim3 grid((unsigned)(wpx * NXCD)), blk(WAVE);
+    const FusedParams *tb = e->table[phase];
+    if (e->mode == INMSOM_BT_FP64_EXACT)
+        hipLaunchKernelGGL((k_fused_batch<double, true>), grid, blk, 0, e->stream, tb, e->item_member,
+                           e->item_local, e->total, per_xcd, wpx, tau_inner, sub, nsub);
+    else if (e->mode == INMSOM_BT_FP64)
+        hipLaunchKernelGGL((k_fused_batch<double, false>), grid, blk, 0, e->stream, tb, e->item_member,
+                           e->item_local, e->total, per_xcd, wpx, tau_inner, sub, nsub);
+    else
+        hipLaunchKernelGGL((k_fused_batch<float, false>), grid, blk, 0, e->stream, tb, e->item_member,
+                           e->item_local, e->total, per_xcd, wpx, tau_inner, sub, nsub);
+}
+
+// launches of one outer step starting from buffer set `phase`
+void outer_step(inmsom_bt_ens *e, int phase, double tau, int nstep, int wpx)
+{
+    for (int sub = 1; sub <= nstep; ++sub) {
+        launch_batch(e, phase, tau / (double)nstep, sub, nstep, wpx);
+        phase = 1 - phase;
+    }
+}
+
+void drop_graphs(inmsom_bt_ens *e)
+{
+    for (auto &g : e->gexec)
+        if (g) {
+            hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    e->g_nstep = -1;
+}
+
+int ensure_graphs(inmsom_bt_ens *e, double tau, int nstep, int chunk, int wpx)
+{
+    if (e->gexec[0] && e->g_tau == tau && e->g_nstep == nstep && e->g_chunk == chunk &&
+        e->g_wpx == wpx)
+        return INMSOM_BT_OK;
+    drop_graphs(e);
+    const int nph = (nstep % 2) ? 2 : 1;
+    for (int ph = 0; ph < nph; ++ph) {
+        hipGraph_t graph = nullptr;
+        EHC(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        outer_step(e, ph, tau, nstep, wpx);
+        EHC(hipStreamEndCapture(e->stream, &graph));
+        hipError_t r = hipGraphInstantiate(&e->gexec[ph], graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (r != hipSuccess) return fail(std::string("hipGraphInstantiate: ") + hipGetErrorString(r),
+                                         INMSOM_BT_EDEVICE);
+    }
+    e->g_tau = tau;
+    e->g_nstep = nstep;
+    e->g_chunk = chunk;
+    e->g_wpx = wpx;
+    return INMSOM_BT_OK;
+}
+
+int run_steps(inmsom_bt_ens *e, double tau, int nstep, int nsteps, int use_graph)
+{
+    const int chunk = e->chunk > 0 ? e->chunk : default_chunk(e->mode);
+    const int wpx = e->waves_per_xcd > 0 ? e->waves_per_xcd : default_waves(e->mode);
+    int rc = build_tables(e, chunk);
+    if (rc) return rc;
+    if (use_graph) {
+        rc = ensure_graphs(e, tau, nstep, chunk, wpx);
+        if (rc) return rc;
+        for (int s = 0; s < nsteps; ++s) {
+            EHC(hipGraphLaunch(e->gexec[(nstep % 2) ? e->phase : 0], e->stream));
+            if (nstep % 2) e->phase = 1 - e->phase;
+        }
+    } else {
+        for (int s = 0; s < nsteps; ++s) {
+            outer_step(e, e->phase, tau, nstep, wpx);
+            if (nstep % 2) e->phase = 1 - e->phase;
+        }
+    }
+    EHC(hipGetLastError());
+    return INMSOM_BT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int inmsom_bt_ens_create(int nmembers, const inmsom_bt_grid *grids, const inmsom_bt_forcing *forcing,
+                         int mode, int device, inmsom_bt_ens **out)
+{
+    if (!out || !grids || !forcing || nmembers < 1) return fail("bad arguments", INMSOM_BT_EINVAL);
+    if (mode != INMSOM_BT_FP64_EXACT && mode != INMSOM_BT_FP64 && mode != INMSOM_BT_FP32)
+        return fail("unknown mode", INMSOM_BT_EINVAL);
+    *out = nullptr;
+    int ndev = 0;
+    EHC(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail("device index out of range", INMSOM_BT_EINVAL);
+    EHC(hipSetDevice(device));
+    auto *e = new inmsom_bt_ens();
+    e->device = device;
+    e->mode = mode;
+    e->g = grids[0].free_fall_acc;
+    e->rho0 = grids[0].ref_den;
+    e->ts = grids[0].time_smooth;
+    e->cb = forcing[0].cb_nl;
+    e->eb = forcing[0].ebottom;
+    int rc = INMSOM_BT_OK;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
+        rc = fail("hipStreamCreate failed", INMSOM_BT_EDEVICE);
+    for (int k = 0; k < nmembers && rc == INMSOM_BT_OK; ++k) {
+        if (grids[k].free_fall_acc != e->g || grids[k].ref_den != e->rho0 ||
+            grids[k].time_smooth != e->ts || forcing[k].cb_nl != e->cb || forcing[k].ebottom != e->eb) {
+            rc = fail("members must share g, rho0, time_smooth and the drag parameters", INMSOM_BT_EINVAL);
+            break;
+        }
+        e->mem.emplace_back();
+        rc = build_member(e, &grids[k], &forcing[k], e->mem.back());
+    }
+    if (rc == INMSOM_BT_OK && hipMalloc(&e->cnt, 8) != hipSuccess)
+        rc = fail("hipMalloc failed", INMSOM_BT_ENOMEM);
+    if (rc != INMSOM_BT_OK) {
+        inmsom_bt_ens_destroy(e);
+        return rc;
+    }
+    *out = e;
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_ens_destroy(inmsom_bt_ens *e)
+{
+    if (!e) return INMSOM_BT_OK;
+    hipSetDevice(e->device);
+    if (e->stream) hipStreamSynchronize(e->stream);
+    drop_graphs(e);
+    for (auto &mb : e->mem) {
+        for (void *q : {(void *)mb.mb, mb.H, mb.rd, mb.frx, mb.fry, (void *)mb.hup_c, (void *)mb.hvp_c,
+                        (void *)mb.row_f, (void *)mb.row_sx, (void *)mb.row_sy, (void *)mb.row_wf})
+            if (q) hipFree(q);
+        for (int b = 0; b < 2; ++b)
+            for (int k = 0; k < 3; ++k) {
+                if (mb.lvl[b][k]) hipFree(mb.lvl[b][k]);
+                if (mb.flt[b][k]) hipFree(mb.flt[b][k]);
+            }
+    }
+    for (void *q : e->snap)
+        if (q) hipFree(q);
+    for (void *q : {(void *)e->table[0], (void *)e->table[1], (void *)e->item_member,
+                    (void *)e->item_local, (void *)e->rows, (void *)e->cnt})
+        if (q) hipFree(q);
+    if (e->stream) hipStreamDestroy(e->stream);
+    delete e;
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_ens_size(inmsom_bt_ens *e, int *nmembers, long long *wet_cells, long long *cells)
+{
+    if (!e || !nmembers || !cells) return fail("null pointer", INMSOM_BT_EINVAL);
+    *nmembers = (int)e->mem.size();
+    long long c = 0;
+    for (auto &mb : e->mem) c += (long long)mb.n;
+    *cells = c;
+    if (wet_cells) {
+        long long w = 0;
+        for (auto &mb : e->mem) {
+            std::vector<uint8_t> bits(mb.n);
+            EHC(hipMemcpy(bits.data(), mb.mb, mb.n, hipMemcpyDeviceToHost));
+            for (int n = 3; n <= mb.ny - 2; ++n)
+                for (int m = 3; m <= mb.nx - 2; ++m)
+                    if (bits[(size_t)(n - 1) * mb.nx + (m - 1)] & MB_LU) ++w;
+        }
+        *wet_cells = w;
+    }
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_ens_set_state(inmsom_bt_ens *e, int k, const double *u, const double *up, const double *v,
+                            const double *vp, const double *ssh, const double *sshp)
+{
+    if (!e || k < 0 || k >= (int)e->mem.size()) return fail("bad member index", INMSOM_BT_EINVAL);
+    if (!u || !up || !v || !vp || !ssh || !sshp) return fail("null pointer", INMSOM_BT_EINVAL);
+    EHC(hipSetDevice(e->device));
+    EHC(hipStreamSynchronize(e->stream));
+    EnsMember &mb = e->mem[k];
+    const double *lv[3] = {u, v, ssh}, *fl[3] = {up, vp, sshp};
+    const int b = e->phase;
+    for (int q = 0; q < 3; ++q) {
+        if (e->mode == INMSOM_BT_FP32) {
+            std::vector<float> t(mb.n);
+            for (size_t i = 0; i < mb.n; ++i) t[i] = (float)lv[q][i];
+            EHC(hipMemcpy(mb.lvl[b][q], t.data(), mb.n * 4, hipMemcpyHostToDevice));
+        } else {
+            EHC(hipMemcpy(mb.lvl[b][q], lv[q], mb.n * 8, hipMemcpyHostToDevice));
+        }
+        EHC(hipMemcpy(mb.flt[b][q], fl[q], mb.n * 8, hipMemcpyHostToDevice));
+    }
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_ens_get_state(inmsom_bt_ens *e, int k, double *u, double *up, double *v, double *vp,
+                            double *ssh, double *sshp)
+{
+    if (!e || k < 0 || k >= (int)e->mem.size()) return fail("bad member index", INMSOM_BT_EINVAL);
+    if (!u || !up || !v || !vp || !ssh || !sshp) return fail("null pointer", INMSOM_BT_EINVAL);
+    EHC(hipSetDevice(e->device));
+    EHC(hipStreamSynchronize(e->stream));
+    EnsMember &mb = e->mem[k];
+    double *lv[3] = {u, v, ssh}, *fl[3] = {up, vp, sshp};
+    const int b = e->phase;
+    for (int q = 0; q < 3; ++q) {
+        if (e->mode == INMSOM_BT_FP32) {
+            std::vector<float> t(mb.n);
+            EHC(hipMemcpy(t.data(), mb.lvl[b][q], mb.n * 4, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < mb.n; ++i) lv[q][i] = (double)t[i];
+        } else {
+            EHC(hipMemcpy(lv[q], mb.lvl[b][q], mb.n * 8, hipMemcpyDeviceToHost));
+        }
+        EHC(hipMemcpy(fl[q], mb.flt[b][q], mb.n * 8, hipMemcpyDeviceToHost));
+    }
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_ens_set_launch(inmsom_bt_ens *e, int rows_per_wave, int waves_per_xcd)
+{
+    if (!e) return fail("null ensemble", INMSOM_BT_EINVAL);
+    if (rows_per_wave < 0 || rows_per_wave > 4096 || waves_per_xcd < 0 || waves_per_xcd > 65536)
+        return fail("launch shape out of range", INMSOM_BT_EINVAL);
+    e->chunk = rows_per_wave;
+    e->waves_per_xcd = waves_per_xcd;
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_ens_run(inmsom_bt_ens *e, double tau, int nstep, int nsteps, int use_graph)
+{
+    if (!e) return fail("null ensemble", INMSOM_BT_EINVAL);
+    if (!(tau > 0.0) || nstep < 1 || nstep > 1000 || nsteps < 0)
+        return fail("bad tau / nstep / nsteps", INMSOM_BT_EINVAL);
+    EHC(hipSetDevice(e->device));
+    int rc = run_steps(e, tau, nstep, nsteps, use_graph);
+    if (rc) return rc;
+    EHC(hipStreamSynchronize(e->stream));
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_ens_run_timed(inmsom_bt_ens *e, double tau, int nstep, int nsteps, int use_graph,
+                            double *ms)
+{
+    if (!e || !ms) return fail("null pointer", INMSOM_BT_EINVAL);
+    if (!(tau > 0.0) || nstep < 1 || nstep > 1000 || nsteps < 0)
+        return fail("bad tau / nstep / nsteps", INMSOM_BT_EINVAL);
+    EHC(hipSetDevice(e->device));
+    // build tables and graphs before the timed region
+    const int chunk = e->chunk > 0 ? e->chunk : default_chunk(e->mode);
+    const int wpx = e->waves_per_xcd > 0 ? e->waves_per_xcd : default_waves(e->mode);
+    int rc = build_tables(e, chunk);
+    if (rc) return rc;
+    if (use_graph && (rc = ensure_graphs(e, tau, nstep, chunk, wpx))) return rc;
+    hipEvent_t e0, e1;
+    EHC(hipEventCreate(&e0));
+    EHC(hipEventCreate(&e1));
+    EHC(hipEventRecord(e0, e->stream));
+    rc = run_steps(e, tau, nstep, nsteps, use_graph);
+    if (rc) return rc;
+    EHC(hipEventRecord(e1, e->stream));
+    EHC(hipEventSynchronize(e1));
+    float t = 0.f;
+    EHC(hipEventElapsedTime(&t, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *ms = t;
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_ens_save(inmsom_bt_ens *e)
+{
+    if (!e) return fail("null ensemble", INMSOM_BT_EINVAL);
+    EHC(hipSetDevice(e->device));
+    const size_t es = esz(e->mode);
+    if (e->snap.empty()) {
+        for (auto &mb : e->mem)
+            for (int q = 0; q < 6; ++q) {
+                void *p = nullptr;
+                EHC(hipMalloc(&p, mb.n * (q < 3 ? es : 8)));
+                e->snap.push_back(p);
+            }
+    }
+    size_t j = 0;
+    for (auto &mb : e->mem)
+        for (int q = 0; q < 6; ++q, ++j) {
+            const void *src = q < 3 ? mb.lvl[e->phase][q] : (const void *)mb.flt[e->phase][q - 3];
+            EHC(hipMemcpyAsync(e->snap[j], src, mb.n * (q < 3 ? es : 8), hipMemcpyDeviceToDevice,
+                               e->stream));
+        }
+    EHC(hipStreamSynchronize(e->stream));
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_ens_restore(inmsom_bt_ens *e)
+{
+    if (!e) return fail("null ensemble", INMSOM_BT_EINVAL);
+    if (e->snap.empty()) return fail("inmsom_bt_ens_save was not called", INMSOM_BT_ESTATE);
+    EHC(hipSetDevice(e->device));
+    const size_t es = esz(e->mode);
+    size_t j = 0;
+    for (auto &mb : e->mem)
+        for (int q = 0; q < 6; ++q, ++j) {
+            void *dst = q < 3 ? mb.lvl[e->phase][q] : (void *)mb.flt[e->phase][q - 3];
+            EHC(hipMemcpyAsync(dst, e->snap[j], mb.n * (q < 3 ? es : 8), hipMemcpyDeviceToDevice,
+                               e->stream));
+        }
+    EHC(hipStreamSynchronize(e->stream));
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_ens_range_check(inmsom_bt_ens *e, long long *bad)
+{
+    if (!e || !bad) return fail("null pointer", INMSOM_BT_EINVAL);
+    EHC(hipSetDevice(e->device));
+    EHC(hipMemsetAsync(e->cnt, 0, 8, e->stream));
+    for (auto &mb : e->mem) {
+        const int b = e->phase;
+        dim3 g((unsigned)((mb.n + 255) / 256)), blk(256);
+        if (e->mode == INMSOM_BT_FP32)
+            hipLaunchKernelGGL(k_member_range<float>, g, blk, 0, e->stream, mb.nx, mb.ny, mb.mb,
+                               (const float *)mb.lvl[b][2], (const float *)mb.lvl[b][0],
+                               (const float *)mb.lvl[b][1], e->cnt);
+        else
+            hipLaunchKernelGGL(k_member_range<double>, g, blk, 0, e->stream, mb.nx, mb.ny, mb.mb,
+                               (const double *)mb.lvl[b][2], (const double *)mb.lvl[b][0],
+                               (const double *)mb.lvl[b][1], e->cnt);
+    }
+    EHC(hipGetLastError());
+    unsigned long long h = 0;
+    EHC(hipMemcpyAsync(&h, e->cnt, 8, hipMemcpyDeviceToHost, e->stream));
+    EHC(hipStreamSynchronize(e->stream));
+    *bad = (long long)h;
+    return h ? INMSOM_BT_ERANGE : INMSOM_BT_OK;
+}
+
+int inmsom_bt_ens_volume(inmsom_bt_ens *e, int k, double *vol)
+{
+    if (!e || !vol || k < 0 || k >= (int)e->mem.size()) return fail("bad arguments", INMSOM_BT_EINVAL);
+    EHC(hipSetDevice(e->device));
+    EnsMember &mb = e->mem[k];
+    const int rows = mb.ny - 4;
+    double *rb = nullptr;
+    EHC(hipMalloc(&rb, (size_t)rows * 8));
+    const int b = e->phase;
+    if (e->mode == INMSOM_BT_FP32)
+        hipLaunchKernelGGL(k_member_rows<float>, dim3(rows), dim3(256), 0, e->stream, mb.nx, mb.ny, mb.mb,
+                           (const float *)mb.lvl[b][2], mb.area, rb);
+    else
+        hipLaunchKernelGGL(k_member_rows<double>, dim3(rows), dim3(256), 0, e->stream, mb.nx, mb.ny, mb.mb,
+                           (const double *)mb.lvl[b][2], mb.area, rb);
+    std::vector<double> h(rows);
+    EHC(hipMemcpyAsync(h.data(), rb, (size_t)rows * 8, hipMemcpyDeviceToHost, e->stream));
+    EHC(hipStreamSynchronize(e->stream));
+    hipFree(rb);
+    double t = 0.0;
+    for (double x : h) t += x;
+    *vol = t;
+    return INMSOM_BT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,494 @@
+// Device body of the fused barotropic step, shared by the single-basin
+// kernel (bt_fused.hip) and the batched-ensemble kernel (bt_ensemble.hip).
+// The algorithm and its modes are described at the top of bt_fused.hip.
+#pragma once
+
+#include "bt_internal.h"
+
+namespace ibt {
+
+constexpr int WAVE = 64;
+constexpr int SW = 60;  // output columns per wave
+constexpr int NXCD = 8;
+
+struct FusedParams {
+    int nx, ny, strips, chunk, nchunks, total, per_xcd;
+    const uint8_t *mb;
+    const void *H, *rd;     // stencil type
+    const void *u, *v, *s;  // level n, stencil type
+    const double *up, *vp, *sp;
+    void *uo, *vo, *so;
+    double *upo, *vpo, *spo;
+    const double *row_f, *row_sx, *row_sy, *row_wf;  // per row, index n-1
+    double g, rho0, ts, tau, cb, eb, nu, ffs, d;
+    // sub-cycling: substep `sub` of `nsub` (tau = tau_inner = outer tau / nsub)
+    int sub, nsub;
+    double dn;             // dfloat(nstep)
+    void *frx, *fry;       // forcing frozen over the substeps of one call (stencil type)
+    double *hup_c, *hvp_c; // filtered U/V depths carried between substeps (FP64_EXACT only)
+};
+
+// Lane shifts by DPP (no LDS traffic).  wave_shl:1 gives lane L the value
+// of lane L+1 (column m+1), wave_shr:1 the value of lane L-1 (column m-1);
+// the lanes at the wave's ends receive 0 and only ever feed halo lanes.
+__device__ __forceinline__ int dpp_from_right(int x)
+{
+    return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xF, 0xF, true);
+}
+__device__ __forceinline__ int dpp_from_left(int x)
+{
+    return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xF, 0xF, true);
+}
+__device__ __forceinline__ double xr(double x)  // value of column m+1
+{
+    return __hiloint2double(dpp_from_right(__double2hiint(x)), dpp_from_right(__double2loint(x)));
+}
+__device__ __forceinline__ double xl(double x)  // value of column m-1
+{
+    return __hiloint2double(dpp_from_left(__double2hiint(x)), dpp_from_left(__double2loint(x)));
+}
+__device__ __forceinline__ float xr(float x) { return __int_as_float(dpp_from_right(__float_as_int(x))); }
+__device__ __forceinline__ float xl(float x) { return __int_as_float(dpp_from_left(__float_as_int(x))); }
+
+template <class R>
+__device__ __forceinline__ R bitf(int mb, int bit)
+{
+    return (mb & bit) ? R(1) : R(0);
+}
+
+template <typename R, bool EXACT>
+struct Ops {
+    const FusedParams &p;
+    R d, inv_d;
+    __device__ Ops(const FusedParams &pp) : p(pp), d(R(pp.d)), inv_d(R(1.0 / pp.d)) {}
+
+    // depth on a U or V point from its two T points (Function/depth.f90:29-49)
+    __device__ R avg2(R qa, R qb, R la, R lb) const
+    {
+        R slu = la + lb;
+        if constexpr (EXACT) {
+            return (((qa * d) * d) * la + ((qb * d) * d) * lb) / slu / d / d;
+        } else {
+#pragma clang fp contract(fast)
+            return (la * qa + lb * qb) * (slu > R(1.5) ? R(0.5) : R(1));
+        }
+    }
+    // depth on an H point from its four T points (:51-66)
+    __device__ R avg4(R q1, R q2, R q3, R q4, R l1, R l2, R l3, R l4) const
+    {
+        R slu = l1 + l2 + l3 + l4;
+        if constexpr (EXACT) {
+            return (((q1 * d) * d) * l1 + ((q2 * d) * d) * l2 + ((q3 * d) * d) * l3 +
+                    ((q4 * d) * d) * l4) /
+                   slu / d / d;
+        } else {
+#pragma clang fp contract(fast)
+            R rs = slu > R(3.5) ? R(0.25) : slu > R(2.5) ? R(1.0 / 3.0) : slu > R(1.5) ? R(0.5) : R(1);
+            return (l1 * q1 + l2 * q2 + l3 * q3 + l4 * q4) * rs;
+        }
+    }
+};
+
+// Loads are branch-free: the address is clamped into the array and the
+// value is zeroed afterwards when the cell lies outside it.  Straight-line
+// loads let the compiler wait for exactly the older (prefetched) loads.
+struct RowAddr {
+    int nx, ny, mc;  // mc = column clamped to 1..nx
+    __device__ __forceinline__ size_t at(int row) const
+    {
+        const int rc = min(max(row, 1), ny);
+        return (size_t)(rc - 1) * nx + (mc - 1);
+    }
+};
+
+// One work item: `chunk` output rows of one 60-column strip of one basin.
+// Called by every lane of a wave with wave-uniform arguments.
+template <typename R, bool EXACT>
+__device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, int strip)
+{
+    const int lane = threadIdx.x;
+    const int m = strip * SW + lane - 1;
+    const bool colok = (m >= 1 && m <= p.nx);
+    const bool writer = colok && lane >= 2 && lane <= SW + 1;
+    const int r0 = chunk_id * p.chunk + 1;
+    const int r1 = min(r0 + p.chunk, p.ny + 1);
+    const bool inx = (m >= 3 && m <= p.nx - 2);
+    const bool inx1 = (m >= 2 && m <= p.nx - 1);
+
+    Ops<R, EXACT> op(p);
+    const R d = op.d;
+    const R tau = R(p.tau);
+    const R G = R(p.g);
+    const R nu = R(p.nu);
+
+    // rolling window (suffix m1 = row n-1, 0 = row n, p1 = n+1, p2 = n+2)
+    R u_m1 = 0, u_0 = 0, u_p1 = 0, v_m1 = 0, v_0 = 0, v_p1 = 0;
+    R s_0 = 0, s_p1 = 0, s_p2 = 0, H_p1 = 0, H_p2 = 0;
+    double up_0 = 0, up_p1 = 0, vp_0 = 0, vp_p1 = 0, sp_0 = 0, sp_p1 = 0;
+    R rd_0 = 0, rd_p1 = 0;
+    int mb_m1 = 0, mb_0 = 0, mb_p1 = 0, mb_p2 = 0;
+    R hq_0 = 0, hq_p1 = 0, hq_p2 = 0, hqp_0 = 0, hqp_p1 = 0, hqn_0 = 0, hqn_p1 = 0;
+    R hu_0 = 0, hu_p1 = 0, hup_0 = 0, hup_p1 = 0;
+    R hv_m1 = 0, hv_0 = 0, hv_p1 = 0, hvp_0 = 0, hvn_0 = 0;
+    R hh_m1 = 0, hh_0 = 0;
+    double sn_0 = 0, sn_p1 = 0;
+    R hun_0 = 0, hun_p1 = 0, stt_0 = 0, stt_p1 = 0, bf_0 = 0, bf_p1 = 0;
+    R sts_m1 = 0, sts_0 = 0, vo_m1 = 0, vo_0 = 0;
+
+    // Loads run one iteration ahead of their use (software prefetch): the
+    // nx_* registers hold what the next iteration consumes.
+    const RowAddr ra{p.nx, p.ny, min(max(m, 1), p.nx)};
+    const R *S = static_cast<const R *>(p.s), *Hh = static_cast<const R *>(p.H);
+    const R *U = static_cast<const R *>(p.u), *V = static_cast<const R *>(p.v);
+    const R *RD = static_cast<const R *>(p.rd);
+    const R *FX = static_cast<const R *>(p.frx), *FY = static_cast<const R *>(p.fry);
+    const bool frozen = p.sub > 1;
+    auto rowok = [&](int row) { return colok && row >= 1 && row <= p.ny; };
+    auto tab = [&](const double *t, int row) { return t[min(max(row, 1), p.ny) - 1]; };
+
+    R nx_s, nx_H, nx_u, nx_v, nx_rd, nx_fx, nx_fy;
+    double nx_up, nx_vp, nx_sp, nx_hupc, nx_hvpc, nx_f, nx_sx, nx_sy, nx_wf;
+    int nx_mb;
+    auto issue = [&](int n) {  // the loads iteration n consumes
+        const size_t i2 = ra.at(n + 2), i1 = ra.at(n + 1), i0 = ra.at(n);
+        nx_s = S[i2];
+        nx_H = Hh[i2];
+        nx_mb = p.mb[i2];
+        nx_u = U[i1];
+        nx_v = V[i1];
+        nx_up = p.up[i1];
+        nx_vp = p.vp[i1];
+        nx_sp = p.sp[i1];
+        nx_rd = RD[i1];
+        // per-cell frozen forcing and carried depths of the output row n:
+        // element 0 (one cache line per wave) when they are not used
+        const size_t iff = frozen ? i0 : 0;
+        nx_fx = FX[iff];
+        nx_fy = FY[iff];
+        if constexpr (EXACT) {
+            nx_hupc = p.hup_c[iff];
+            nx_hvpc = p.hvp_c[iff];
+        } else {
+            nx_hupc = nx_hvpc = 0.0;
+        }
+        nx_f = tab(p.row_f, n);
+        nx_sx = tab(p.row_sx, n);
+        nx_sy = tab(p.row_sy, n);
+        nx_wf = tab(p.row_wf, n + 1);
+    };
+    issue(r0 - 3);
+    R f_m1 = 0, f_0 = 0, fx_0 = 0, fy_0 = 0, sx_0 = 0, sy_0 = 0, wf_p1 = 0;
+    double hupc_0 = 0, hvpc_0 = 0;
+
+    for (int n = r0 - 3; n < r1; ++n) {
+        const int k1 = n + 1;
+        // ---- take the prefetched rows: surface height, depth and masks two
+        //      rows ahead, the rest one row ahead
+        s_p2 = rowok(n + 2) ? nx_s : R(0);
+        H_p2 = rowok(n + 2) ? nx_H : R(0);
+        mb_p2 = rowok(n + 2) ? nx_mb : 0;
+        u_p1 = rowok(k1) ? nx_u : R(0);
+        v_p1 = rowok(k1) ? nx_v : R(0);
+        up_p1 = rowok(k1) ? nx_up : 0.0;
+        vp_p1 = rowok(k1) ? nx_vp : 0.0;
+        sp_p1 = rowok(k1) ? nx_sp : 0.0;
+        rd_p1 = rowok(k1) ? nx_rd : R(0);
+        fx_0 = nx_fx;
+        fy_0 = nx_fy;
+        hupc_0 = nx_hupc;
+        hvpc_0 = nx_hvpc;
+        f_m1 = f_0;
+        f_0 = (n >= 1 && n <= p.ny) ? R(nx_f) : R(0);
+        sx_0 = R(nx_sx);
+        sy_0 = R(nx_sy);
+        wf_p1 = (k1 >= 1 && k1 <= p.ny) ? R(nx_wf) : R(0);
+        if (n + 1 < r1) issue(n + 1);
+
+        const R lu_0 = bitf<R>(mb_0, MB_LU), lu_p1 = bitf<R>(mb_p1, MB_LU), lu_p2 = bitf<R>(mb_p2, MB_LU);
+        const R lu_0r = xr(lu_0), lu_p1r = xr(lu_p1);
+        const bool iny1 = (k1 >= 3 && k1 <= p.ny - 2);
+
+        // ---- depths (hh_init / hh_update, Function/depth.f90)
+        hq_p2 = H_p2 + s_p2;
+        hqp_p1 = H_p1 + R(sp_p1);
+        {
+            const R hq1r = xr(hq_p1), hqp1r = xr(hqp_p1), hqp0 = hqp_0;
+            hu_p1 = (mb_p1 & MB_LLU) ? op.avg2(hq_p1, hq1r, lu_p1, lu_p1r) : R(0);
+            hup_p1 = (mb_p1 & MB_LLU) ? op.avg2(hqp_p1, hqp1r, lu_p1, lu_p1r) : R(0);
+            hv_p1 = (mb_p1 & MB_LLV) ? op.avg2(hq_p1, hq_p2, lu_p1, lu_p2) : R(0);
+            hvp_0 = (mb_0 & MB_LLV) ? op.avg2(hqp0, hqp_p1, lu_0, lu_p1) : R(0);
+            const R hq0r = xr(hq_0);
+            hh_0 = (mb_0 & MB_LUH) ? op.avg4(hq_0, hq0r, hq_p1, hq1r, lu_0, lu_0r, lu_p1, lu_p1r) : R(0);
+        }
+
+        // ---- T-point stage of row n+1: continuity, new depths, tension, drag
+        const R u1l = xl(u_p1), hu1l = xl(hu_p1), up1l = xl(R(up_p1));
+        {
+            const bool tw = (mb_p1 & MB_LU) && inx && iny1;
+            const R wf = wf_p1;
+            double sn;
+            if constexpr (EXACT) {
+                sn = sp_p1 + 2.0 * p.tau *
+                                 (wf / p.rho0 * p.ffs -
+                                  (u_p1 * hu_p1 * d - u1l * hu1l * d + v_p1 * hv_p1 * d -
+                                   v_0 * hv_0 * d) /
+                                      (d * d));
+            } else {
+#pragma clang fp contract(fast)
+                const R inc = R(2.0 * p.tau) * (wf * R(p.ffs / p.rho0) -
+                                                (u_p1 * hu_p1 - u1l * hu1l + v_p1 * hv_p1 - v_0 * hv_0) *
+                                                    op.inv_d);
+                sn = sp_p1 + double(inc);
+            }
+            sn_p1 = tw ? sn : 0.0;
+        }
+        hqn_p1 = H_p1 + R(sn_p1);
+        {
+            const R hqn1r = xr(hqn_p1);
+            hun_p1 = (mb_p1 & MB_LLU) ? op.avg2(hqn_p1, hqn1r, lu_p1, lu_p1r) : R(0);
+            hvn_0 = (mb_0 & MB_LLV) ? op.avg2(hqn_0, hqn_p1, lu_0, lu_p1) : R(0);
+        }
+        {
+            const bool tw = (mb_p1 & MB_LU) && inx && iny1;
+            R st, bf;
+            const R upr = R(up_p1), vpr = R(vp_p1), vp0r = R(vp_0);
+            bf = R(0);
+            if constexpr (EXACT) {
+                st = d / d * (upr / d - up1l / d) - d / d * (vpr / d - vp0r / d);
+                if (p.sub == 1) {
+                    R ut = (u_p1 * d * d * hu_p1 + u1l * d * d * hu1l) / 2.0 / d / d / hq_p1;
+                    R vt = (v_p1 * d * d * hv_p1 + v_0 * d * d * hv_0) / 2.0 / d / d / hq_p1;
+                    bf = R(p.cb) * sqrt(ut * ut + vt * vt + R(p.eb));
+                }
+            } else {
+#pragma clang fp contract(fast)
+                st = op.inv_d * ((upr - up1l) - (vpr - vp0r));
+                if (p.sub == 1) {
+                    const R rh = R(0.5) / hq_p1;
+                    R ut = (u_p1 * hu_p1 + u1l * hu1l) * rh;
+                    R vt = (v_p1 * hv_p1 + v_0 * hv_0) * rh;
+                    bf = R(p.cb) * sqrt(ut * ut + vt * vt + R(p.eb));
+                }
+            }
+            stt_p1 = tw ? st : R(0);
+            bf_p1 = (tw && p.sub == 1) ? bf : R(0);
+        }
+
+        // ---- H-point stage of row n: shear and metric vorticity
+        {
+            const bool hw = (mb_0 & MB_LUU) && inx && (n >= 3 && n <= p.ny - 2);
+            const R vp0 = R(vp_0), vp0r = xr(vp0);
+            const R v0r = xr(v_0);
+            R ss, vo;
+            if constexpr (EXACT) {
+                ss = d / d * (R(up_p1) / d - R(up_0) / d) + d / d * (vp0r / d - vp0 / d);
+                vo = (v0r * d - v_0 * d) - (u_p1 * d - u_0 * d) - ((v0r - v_0) * d - (u_p1 - u_0) * d);
+            } else {
+#pragma clang fp contract(fast)
+                ss = op.inv_d * ((R(up_p1) - R(up_0)) + (vp0r - vp0));
+                vo = R(0);  // identically zero for uniform metrics (vel_ssh.f90:35-37)
+            }
+            sts_0 = hw ? ss : R(0);
+            vo_0 = hw ? vo : R(0);
+        }
+
+        // ---- outputs of row n
+        if (n >= r0) {
+            const bool iny = (n >= 3 && n <= p.ny - 2);
+            const R lu_m1 = bitf<R>(mb_m1, MB_LU);
+            const R mu_m1 = lu_m1 * nu, mu_0 = lu_0 * nu, mu_p1 = lu_p1 * nu;
+            const R mu_m1r = xr(mu_m1), mu_0r = xr(mu_0), mu_p1r = xr(mu_p1);
+            const R mu_0l = xl(mu_0), mu_p1l = xl(mu_p1);
+            const R u0r = xr(u_0), u0l = xl(u_0), hu0r = xr(hu_0), hu0l = xl(hu_0);
+            const R v0r = xr(v_0), v0l = xl(v_0), hv0r = xr(hv_0);
+            const R vm1r = xr(v_m1), hvm1r = xr(hv_m1);
+            const R up1l_ = xl(u_p1), hup1l = xl(hu_p1);
+            const R s0r = xr(s_0), rd0r = xr(rd_0), bf0r = xr(bf_0);
+            const R hq0r = xr(hq_0), stt0r = xr(stt_0);
+            const R hh0l = xl(hh_0), sts0l = xl(sts_0), vo0l = xl(vo_0);
+            const R up0 = R(up_0), vp0 = R(vp_0);
+            const R luu0 = bitf<R>(mb_0, MB_LUU), luum1 = bitf<R>(mb_m1, MB_LUU);
+            const R sx = sx_0, sy = sy_0;
+            const size_t io = (size_t)(n - 1) * p.nx + (m - 1);
+            const bool own = writer && n <= p.ny;
+            // the RHS of Control/dyncall.f90:114-126 is formed once per call
+            // (first substep) and frozen over the call's substeps
+            R rhs_u, rhs_v;
+            if constexpr (EXACT) {
+                rhs_u = 0.0 + (sx + (-(bf_0 + bf0r) / 2.0 * up0)) * d * d;
+                rhs_v = 0.0 + (sy + (-(bf_0 + bf_p1) / 2.0 * vp0)) * d * d;
+            } else {
+#pragma clang fp contract(fast)
+                rhs_u = sx - R(0.5) * (bf_0 + bf0r) * up0;
+                rhs_v = sy - R(0.5) * (bf_0 + bf_p1) * vp0;
+            }
+            if (frozen) {
+                rhs_u = fx_0;
+                rhs_v = fy_0;
+            } else if (p.nsub > 1 && own) {
+                static_cast<R *>(p.frx)[io] = rhs_u;
+                static_cast<R *>(p.fry)[io] = rhs_v;
+            }
+            // level n-1 U/V depths: hh_init values at the first substep,
+            // the hh_shift-filtered carry afterwards (depth.f90:195-203)
+            R hupc = hup_0, hvpc = hvp_0;
+            if constexpr (EXACT) {
+                if (frozen) {
+                    hupc = R(hupc_0);
+                    hvpc = R(hvpc_0);
+                }
+            }
+            R un, vn;
+            if constexpr (EXACT) {
+                // zonal momentum (vel_ssh.f90:536-551) with uv_trans (:59-81) and uv_diff2 (:146-159)
+                R rhsx = rhs_u;
+                R fx_p = (u_0 * d * hu_0 + u0r * d * hu0r) / 2.0 * (u_0 + u0r) / 2.0;
+                R fx_m = (u_0 * d * hu_0 + u0l * d * hu0l) / 2.0 * (u_0 + u0l) / 2.0;
+                R fy_p = (v_0 * d * hv_0 + v0r * d * hv0r) / 2.0 * (u_p1 + u_0) / 2.0 * luu0;
+                R fy_m = (v_m1 * d * hv_m1 + vm1r * d * hvm1r) / 2.0 * (u_m1 + u_0) / 2.0 * luum1;
+                R radv = -(fx_p - fx_m + fy_p - fy_m) +
+                         (vo_0 * hh_0 * (v0r + v_0) + vo_m1 * hh_m1 * (vm1r + v_m1)) / 4.0;
+                R muh_p = (mu_0 + mu_0r + mu_p1 + mu_p1r) / 4.0;
+                R muh_m = (mu_0 + mu_0r + mu_m1 + mu_m1r) / 4.0;
+                R pp = (d * d * mu_0r * hq0r * stt0r - d * d * mu_0 * hq_0 * stt_0) / d;
+                R qq = (d * d * muh_p * hh_0 * sts_0 - d * d * muh_m * hh_m1 * sts_m1) / d;
+                R rdfx = pp + qq;
+                R bp = hun_0 * d * d / 2.0 / tau;
+                R bp0 = hupc * d * d / 2.0 / tau;
+                R slx = -G * (s0r - s_0) * d * hu_0;
+                R grx = rhsx + slx + rdfx + radv - (rd_0 + rd0r) / 2.0 * up0 * d * d * hu_0 +
+                        (f_0 * hh_0 * d * d * (v0r + v_0) + f_m1 * hh_m1 * d * d * (vm1r + v_m1)) / 4.0;
+                un = (up0 * bp0 + grx) / bp;
+
+                // meridional momentum (vel_ssh.f90:554-569) with uv_trans (:84-105) and uv_diff2 (:162-175)
+                R rhsy = rhs_v;
+                R gy_p = (v_0 * d * hv_0 + v_p1 * d * hv_p1) / 2.0 * (v_0 + v_p1) / 2.0;
+                R gy_m = (v_0 * d * hv_0 + v_m1 * d * hv_m1) / 2.0 * (v_0 + v_m1) / 2.0;
+                R gx_p = (u_0 * d * hu_0 + u_p1 * d * hu_p1) / 2.0 * (v0r + v_0) / 2.0;
+                R gx_m = (u0l * d * hu0l + up1l_ * d * hup1l) / 2.0 * (v0l + v_0) / 2.0;
+                R rady = -(gx_p - gx_m + gy_p - gy_m) -
+                         (vo_0 * hh_0 * (u_p1 + u_0) + vo0l * hh0l * (up1l_ + u0l)) / 4.0;
+                R nuh_p = (mu_0 + mu_0r + mu_p1 + mu_p1r) / 4.0;
+                R nuh_m = (mu_0 + mu_0l + mu_p1 + mu_p1l) / 4.0;
+                R py = (d * d * mu_p1 * hq_p1 * stt_p1 - d * d * mu_0 * hq_0 * stt_0) / d;
+                R qy = (d * d * nuh_p * hh_0 * sts_0 - d * d * nuh_m * hh0l * sts0l) / d;
+                R rdfy = -py + qy;
+                R bq = hvn_0 * d * d / 2.0 / tau;
+                R bq0 = hvpc * d * d / 2.0 / tau;
+                R sly = -G * (s_p1 - s_0) * d * hv_0;
+                R gry = rhsy + sly + rdfy + rady - (rd_0 + rd_p1) / 2.0 * vp0 * d * d * hv_0 -
+                        (f_0 * hh_0 * d * d * (u_p1 + u_0) + f_0 * hh0l * d * d * (up1l_ + u0l)) / 4.0;
+                vn = (vp0 * bq0 + gry) / bq;
+            } else {
+#pragma clang fp contract(fast)
+                const R id = op.inv_d;
+                const R two_tau = R(2.0) * tau;
+                // forces per unit area (the reference's terms divided by dxt*dyh)
+                R adv = R(-0.25) * ((u_0 * hu_0 + u0r * hu0r) * (u_0 + u0r) -
+                                    (u_0 * hu_0 + u0l * hu0l) * (u_0 + u0l) +
+                                    luu0 * (v_0 * hv_0 + v0r * hv0r) * (u_p1 + u_0) -
+                                    luum1 * (v_m1 * hv_m1 + vm1r * hvm1r) * (u_m1 + u_0));
+                R muh_p = R(0.25) * (mu_0 + mu_0r + mu_p1 + mu_p1r);
+                R muh_m = R(0.25) * (mu_0 + mu_0r + mu_m1 + mu_m1r);
+                R dif = (mu_0r * hq0r * stt0r - mu_0 * hq_0 * stt_0) +
+                        (muh_p * hh_0 * sts_0 - muh_m * hh_m1 * sts_m1);
+                R F = rhs_u - G * id * (s0r - s_0) * hu_0 +
+                      id * (adv + dif) - R(0.5) * (rd_0 + rd0r) * up0 * hu_0 +
+                      R(0.25) * (f_0 * hh_0 * (v0r + v_0) + f_m1 * hh_m1 * (vm1r + v_m1));
+                un = (up0 * hupc + two_tau * F) / hun_0;
+
+                R advy = R(-0.25) * ((u_0 * hu_0 + u_p1 * hu_p1) * (v0r + v_0) -
+                                     (u0l * hu0l + up1l_ * hup1l) * (v0l + v_0) +
+                                     (v_0 * hv_0 + v_p1 * hv_p1) * (v_0 + v_p1) -
+                                     (v_0 * hv_0 + v_m1 * hv_m1) * (v_0 + v_m1));
+                R nuh_p = R(0.25) * (mu_0 + mu_0r + mu_p1 + mu_p1r);
+                R nuh_m = R(0.25) * (mu_0 + mu_0l + mu_p1 + mu_p1l);
+                R dify = -(mu_p1 * hq_p1 * stt_p1 - mu_0 * hq_0 * stt_0) +
+                         (nuh_p * hh_0 * sts_0 - nuh_m * hh0l * sts0l);
+                R Fy = rhs_v - G * id * (s_p1 - s_0) * hv_0 +
+                       id * (advy + dify) - R(0.5) * (rd_0 + rd_p1) * vp0 * hv_0 -
+                       R(0.25) * f_0 * (hh_0 * (u_p1 + u_0) + hh0l * (up1l_ + u0l));
+                vn = (vp0 * hvpc + two_tau * Fy) / hvn_0;
+            }
+            const bool uw = (mb_0 & MB_LCU) && inx && iny;
+            const bool vw = (mb_0 & MB_LCV) && inx && iny;
+            un = uw ? un : R(0);
+            vn = vw ? vn : R(0);
+
+            // time-level shift with the Robert-Asselin filter (vel_ssh.f90:588-612), nstep = 1
+            R s_new = s_0, u_new = u_0, v_new = v_0;
+            double sp_new = sp_0, up_new = up_0, vp_new = vp_0;
+            if (inx1 && n >= 2 && n <= p.ny - 1) {
+                const double ts = p.ts, cts = 0.5 * p.ts / p.dn;
+                if (mb_0 & MB_LU) {
+                    if constexpr (EXACT) {
+                        sp_new = s_0 + ts * (sn_0 - 2.0 * s_0 + sp_0) / 2.0 / p.dn;
+                    } else {
+#pragma clang fp contract(fast)
+                        sp_new = double(s_0) + cts * (sn_0 - 2.0 * double(s_0) + sp_0);
+                    }
+                    s_new = R(sn_0);
+                }
+                if (mb_0 & MB_LCU) {
+                    if constexpr (EXACT) {
+                        up_new = u_0 + ts * (un - 2.0 * u_0 + up_0) / 2.0 / p.dn;
+                    } else {
+#pragma clang fp contract(fast)
+                        up_new = double(u_0) + cts * (double(un) - 2.0 * double(u_0) + up_0);
+                    }
+                    u_new = un;
+                }
+                if (mb_0 & MB_LCV) {
+                    if constexpr (EXACT) {
+                        vp_new = v_0 + ts * (vn - 2.0 * v_0 + vp_0) / 2.0 / p.dn;
+                    } else {
+#pragma clang fp contract(fast)
+                        vp_new = double(v_0) + cts * (double(vn) - 2.0 * double(v_0) + vp_0);
+                    }
+                    v_new = vn;
+                }
+            }
+            if constexpr (EXACT) {
+                // hh_shift of the U/V depths for the next substep (depth.f90:195-203)
+                if (p.nsub > 1 && p.sub < p.nsub && own) {
+                    const double ts = p.ts;
+                    double hu_new = hupc, hv_new = hvpc;
+                    if (inx1 && n >= 2 && n <= p.ny - 1) {
+                        if (mb_0 & MB_LLU) hu_new = hu_0 + ts * (hun_0 - 2.0 * hu_0 + hupc) / 2.0 / p.dn;
+                        if (mb_0 & MB_LLV) hv_new = hv_0 + ts * (hvn_0 - 2.0 * hv_0 + hvpc) / 2.0 / p.dn;
+                    }
+                    p.hup_c[io] = hu_new;
+                    p.hvp_c[io] = hv_new;
+                }
+            }
+            if (own) {
+                const size_t i = io;
+                static_cast<R *>(p.so)[i] = s_new;
+                static_cast<R *>(p.uo)[i] = u_new;
+                static_cast<R *>(p.vo)[i] = v_new;
+                p.spo[i] = sp_new;
+                p.upo[i] = up_new;
+                p.vpo[i] = vp_new;
+            }
+        }
+
+        // ---- advance the window one row
+        u_m1 = u_0; u_0 = u_p1;
+        v_m1 = v_0; v_0 = v_p1;
+        s_0 = s_p1; s_p1 = s_p2;
+        H_p1 = H_p2;
+        up_0 = up_p1; vp_0 = vp_p1; sp_0 = sp_p1; rd_0 = rd_p1;
+        mb_m1 = mb_0; mb_0 = mb_p1; mb_p1 = mb_p2;
+        hq_0 = hq_p1; hq_p1 = hq_p2;
+        hqp_0 = hqp_p1; hqn_0 = hqn_p1;
+        hu_0 = hu_p1; hup_0 = hup_p1;
+        hv_m1 = hv_0; hv_0 = hv_p1;
+        hh_m1 = hh_0;
+        sn_0 = sn_p1; hun_0 = hun_p1;
+        stt_0 = stt_p1; bf_0 = bf_p1;
+        sts_m1 = sts_0; vo_m1 = vo_0;
+    }
+}
+
+
+}  // namespace ibt

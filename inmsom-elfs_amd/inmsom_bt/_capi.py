@@ -58,6 +58,20 @@ SIGNATURES = {
     "inmsom_bt_range_check": ([C.c_void_p, C.POINTER(C.c_longlong)], C.c_int),
     "inmsom_bt_volume": ([C.c_void_p, _dp], C.c_int),
     "inmsom_bt_synchronize": ([C.c_void_p], C.c_int),
+    "inmsom_bt_ens_create": ([C.c_int, C.POINTER(Grid), C.POINTER(Forcing), C.c_int, C.c_int,
+                              C.POINTER(C.c_void_p)], C.c_int),
+    "inmsom_bt_ens_destroy": ([C.c_void_p], C.c_int),
+    "inmsom_bt_ens_size": ([C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong),
+                            C.POINTER(C.c_longlong)], C.c_int),
+    "inmsom_bt_ens_set_state": ([C.c_void_p, C.c_int] + [_dp] * 6, C.c_int),
+    "inmsom_bt_ens_get_state": ([C.c_void_p, C.c_int] + [_dp] * 6, C.c_int),
+    "inmsom_bt_ens_set_launch": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
+    "inmsom_bt_ens_run": ([C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int], C.c_int),
+    "inmsom_bt_ens_run_timed": ([C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int, _dp], C.c_int),
+    "inmsom_bt_ens_save": ([C.c_void_p], C.c_int),
+    "inmsom_bt_ens_restore": ([C.c_void_p], C.c_int),
+    "inmsom_bt_ens_range_check": ([C.c_void_p, C.POINTER(C.c_longlong)], C.c_int),
+    "inmsom_bt_ens_volume": ([C.c_void_p, C.c_int, _dp], C.c_int),
 }
 
 

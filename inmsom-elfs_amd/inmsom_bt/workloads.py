@@ -14,6 +14,8 @@ oracle) diverges within ~40 steps at nstep_barotrop = 1.
 """
 from __future__ import annotations
 
+import numpy as np
+
 from .cases import Basin, make_basin
 
 DT = 10.0
@@ -23,6 +25,27 @@ SINGLE = {
     "cfg0": dict(nxi=256, nyi=256, steps=1000),
     "cfg1": dict(nxi=2048, nyi=1536, steps=10000),
 }
+
+
+def ensemble(nmembers: int = 64, seed: int = 7, size_lo=(256, 256), size_hi=(1024, 768),
+             sigma: float = 8.0) -> list:
+    """Independent basins of uneven size: interior sizes uniform in
+    [size_lo, size_hi] (both axes), each with its own seeded bathymetry,
+    wind amplitude 0.1 N/m^2 x U(0.8, 1.2) and 1-3 surface bumps of
+    0.2-0.5 m (radius 20 km) at random interior positions."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(nmembers):
+        nxi = int(rng.integers(size_lo[0], size_hi[0] + 1))
+        nyi = int(rng.integers(size_lo[1], size_hi[1] + 1))
+        tau0 = 0.1 * float(rng.uniform(0.8, 1.2))
+        nb = int(rng.integers(1, 4))
+        bumps = [(float(rng.uniform(0.2, 0.5)), float(rng.uniform(0.2, 0.8)) * nxi,
+                  float(rng.uniform(0.2, 0.8)) * nyi, 20.0e3) for _ in range(nb)]
+        out.append(make_basin(nxi, nyi, seed=int(rng.integers(1 << 30)), sigma=sigma, hmax=200.0,
+                              hland=10.0, f0=1.0e-4, beta=2.0e-11, tau0=tau0, nu=10.0, g=9.81,
+                              dt=DT, bumps=bumps))
+    return out
 
 
 def single_basin(name: str = "cfg1", seed: int = 2024) -> Basin:

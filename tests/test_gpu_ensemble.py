@@ -1,0 +1,84 @@
+"""Batched ensembles on the GPU: every member equals the single-basin
+fused step bit for bit (same arithmetic, different work mapping), the
+FP64_EXACT members equal the oracle, graph replays equal direct launches,
+the launch shape never changes results, and the autotuner restores the
+state it times from."""
+import numpy as np
+import pytest
+
+import oracle
+from _util import STATE, assert_bitwise
+from inmsom_bt import BarotropicModel
+from inmsom_bt.ensemble import Ensemble
+from inmsom_bt.workloads import ensemble
+
+pytestmark = pytest.mark.gpu
+
+
+def _members(n=5, seed=3):
+    return ensemble(n, seed=seed, size_lo=(40, 30), size_hi=(150, 120), sigma=5.0)
+
+
+@pytest.mark.parametrize("mode", ["fp64_exact", "fp64", "fp32"])
+def test_members_equal_single_basin(mode):
+    basins = _members()
+    steps = 8
+    print(f"ensemble {mode}: " + ", ".join(f"{b.nx}x{b.ny}" for b in basins), flush=True)
+    with Ensemble(basins, mode=mode) as ens:
+        ens.run(steps, nstep=2)
+        got = [ens.get_state(k) for k in range(len(basins))]
+    for k, b in enumerate(basins):
+        with BarotropicModel(b) as m:
+            m.run(steps, mode, nstep=2)
+            want = m.get_state()
+        assert_bitwise(got[k], want, STATE, f"member {k} ({b.nx}x{b.ny}) {mode}")
+    if mode == "fp64_exact":
+        ref = oracle.run(basins[1], steps, 2, out_steps=[steps])[steps]
+        assert_bitwise(got[1], ref, STATE, "member 1 vs oracle")
+
+
+def test_graph_replays_match_direct_launches():
+    basins = _members(4, seed=5)
+    with Ensemble(basins, mode="fp32") as g, Ensemble(basins, mode="fp32") as d:
+        for rep in range(3):
+            g.run(1, nstep=2, graph=True)
+            d.run(1, nstep=2, graph=False)
+            # the program's own eager work between replays
+            assert g.range_check() == 0 and d.range_check() == 0
+            g.volume(0)
+            for k in range(len(basins)):
+                assert_bitwise(g.get_state(k), d.get_state(k), STATE, f"replay {rep} member {k}")
+
+
+def test_launch_shape_does_not_change_results():
+    basins = _members(3, seed=9)
+    outs = []
+    for rows, waves in ((0, 0), (7, 37), (48, 512)):
+        with Ensemble(basins, mode="fp64") as ens:
+            ens.set_launch(rows, waves)
+            ens.run(5, nstep=2)
+            outs.append([ens.get_state(k) for k in range(len(basins))])
+    for o in outs[1:]:
+        for k in range(len(basins)):
+            assert_bitwise(o[k], outs[0][k], STATE, f"member {k}")
+
+
+def test_autotune_restores_state_and_conserves_volume(tmp_path):
+    basins = _members(4, seed=11)
+    with Ensemble(basins, mode="fp32") as ens:
+        v0 = [ens.volume(k) for k in range(len(basins))]
+        res = ens.autotune(nstep=2, trial_steps=3, rows=(12, 24), waves=(128, 384),
+                           cache=str(tmp_path / "tune.json"))
+        assert (res["rows_per_wave"], res["waves_per_xcd"]) in {(12, 128), (12, 384), (24, 128),
+                                                                (24, 384)}
+        assert len(res["trials"]) == 4
+        for k, b in enumerate(basins):  # state is the initial one again
+            st = ens.get_state(k)
+            assert np.array_equal(st["ssh"], b.ssh.astype(np.float32).astype(np.float64))
+        ens.run(30, nstep=2)
+        assert ens.range_check() == 0
+        for k in range(len(basins)):
+            assert abs(ens.volume(k) - v0[k]) <= 1e-5 * abs(v0[k])
+        again = ens.autotune(nstep=2, trial_steps=3, rows=(12, 24), waves=(128, 384),
+                             cache=str(tmp_path / "tune.json"))
+        assert again["cached"]
