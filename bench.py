@@ -112,6 +112,43 @@ def cpu_baseline(b, steps):
                       f"with OpenMP"}
 
 
+def ensemble_leg(steps, warmup, nmembers):
+    """cfg2: uneven basins advanced together in FP32 (one persistent,
+    graph-replayed launch per substep), launch shape from the autotuner."""
+    from inmsom_bt.ensemble import Ensemble
+    from inmsom_bt.workloads import ensemble
+
+    t0 = time.perf_counter()
+    basins = ensemble(nmembers)
+    log(f"ensemble: {nmembers} members generated in {time.perf_counter() - t0:.1f} s")
+    with Ensemble(basins, mode="fp32") as ens:
+        nmem, wet, cells = ens.size()
+        tune = ens.autotune(nstep=NSTEP_BAROTROP, trial_steps=10)
+        log(f"ensemble autotune: rows/wave {tune['rows_per_wave']}, waves/XCD {tune['waves_per_xcd']}")
+        ens.reset()
+        v0 = [ens.volume(k) for k in range(nmem)]
+        if warmup:
+            ens.run(warmup, NSTEP_BAROTROP)
+        ms = ens.run_timed(steps, NSTEP_BAROTROP)
+        bad = ens.range_check()
+        drift = max(abs(ens.volume(k) - v0[k]) / abs(v0[k]) for k in range(nmem))
+    launches = steps * NSTEP_BAROTROP
+    avg = ms * 1e-3 / launches
+    bpc = BYTES_PER_CELL["fp32"]
+    achieved = bpc * cells / avg / 1e9
+    res = {"value": wet * launches / (ms * 1e-3), "unit": "wet-cell steps/s",
+           "members": nmem, "wet_cells": wet, "array_cells": cells, "steps": steps,
+           "ms_per_step": ms / steps, "avg_launch_us": avg * 1e6,
+           "launch": {"rows_per_wave": tune["rows_per_wave"], "waves_per_xcd": tune["waves_per_xcd"]},
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_from_profiles("ensemble_fp32"),
+                        "bytes_per_cell": bpc, "cells_per_launch": cells},
+           "range_check_bad_cells": bad, "max_volume_rel_drift": drift}
+    log(f"ensemble fp32: {ms:.1f} ms for {steps} steps ({avg * 1e6:.1f} us/launch), "
+        f"{res['value']:.3e} wet-cell steps/s, {achieved:.0f} GB/s")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +158,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=16, help="outer steps timed on the CPU oracle")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--modes", default="fp64,fp32,fp64_exact")
+    ap.add_argument("--ensemble-steps", type=int, default=5000,
+                    help="outer steps of the 64-member FP32 ensemble (0 = skip)")
+    ap.add_argument("--ensemble-members", type=int, default=64)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -177,6 +217,9 @@ def main():
             line[mode] = results[mode]
     line["fp64_details"] = {k: head[k] for k in ("avg_launch_us", "range_check_bad_cells",
                                                  "volume_rel_drift", "wall_s")}
+    if args.ensemble_steps > 0:
+        line["ensemble_fp32"] = ensemble_leg(args.ensemble_steps, min(args.warmup, 20),
+                                             args.ensemble_members)
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         line["cpu_baseline"] = cpu_baseline(b, args.cpu_steps)
