@@ -16,6 +16,7 @@
 #define INMSOM_BT_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -203,6 +204,82 @@ int inmsom_bt_ens_restore(inmsom_bt_ens *ens);
 int inmsom_bt_ens_range_check(inmsom_bt_ens *ens, long long *bad);
 /* FP64 volume anomaly of one member (fixed-order sum) */
 int inmsom_bt_ens_volume(inmsom_bt_ens *ens, int member, double *volume);
+
+
+/* Block average of the current state onto a grid `factor` times coarser
+ * (what a coarse model of the same ocean sees; used to build the learned
+ * closure's training targets and to judge coarse runs).  Interior blocks of
+ * factor x factor cells; outputs are ((ny-4)/factor+4, (nx-4)/factor+4)
+ * arrays with zero halos: ssh = mean over the block's wet cells, u = mean
+ * of the fine U faces on the block's east edge, v = mean of the fine V
+ * faces on its north edge.  No counterpart in the reference. */
+int inmsom_bt_coarsen_state(inmsom_bt_ctx *ctx, int factor, double *u, double *v, double *ssh);
+
+/* ------------------------------------------------------------------------
+ * Learned lateral-momentum closure for coarse grids.  No counterpart in the
+ * reference (its lateral momentum closure is the constant/Smagorinsky
+ * viscosity of Function/mixing.f90:1-146); the plain viscosity stays the
+ * fallback.  A 4-layer 3x3 CNN, 4 -> 32 -> 32 -> 32 -> 2 channels with ReLU
+ * after the first three layers, BF16 weights and activations, FP32
+ * accumulation on the matrix cores.  Inputs per grid point: u, v, ssh and
+ * hhq_rest of the basin's (ny, nx) arrays, shifted and scaled; outputs: the
+ * u and v tendencies [m/s^2] on U and V points.
+ *
+ * Parameter vector (float): per layer l = 0..3 the weights W[32][kpad_l]
+ * then the bias b[32], with cin_0 = 8 (channels 4..7 zero), cin_l = 32,
+ * kpad_l = 9 cin_l rounded up to 16, and k = t cin_l + ci where tap
+ * t = 3 (dy + 1) + (dx + 1) reads input (y + dy, x + dx); zero padding.
+ * Layer 3 uses output channels 0 and 1 only.
+ *
+ * The object also trains itself on the device: Adam (beta1 0.9, beta2 0.999,
+ * eps 1e-8) on FP32 master weights, on the masked mean-square error of the
+ * two tendencies over a batch of `batch` images of ny x nx. */
+typedef struct inmsom_bt_cnn inmsom_bt_cnn;
+
+int inmsom_bt_cnn_create(int batch, int ny, int nx, int device, inmsom_bt_cnn **out);
+int inmsom_bt_cnn_destroy(inmsom_bt_cnn *net);
+int inmsom_bt_cnn_num_params(inmsom_bt_cnn *net, long long *n);
+/* x_c = (field_c - in_shift[c]) * in_scale[c] for (u, v, ssh, depth);
+ * tendency = output * out_scale */
+int inmsom_bt_cnn_set_norm(inmsom_bt_cnn *net, const float *in_shift, const float *in_scale,
+                           const float *out_scale);
+/* set (and reset the Adam moments) / read the FP32 master parameters */
+int inmsom_bt_cnn_set_params(inmsom_bt_cnn *net, const float *params);
+int inmsom_bt_cnn_get_params(inmsom_bt_cnn *net, float *params);
+/* gradient of the last inmsom_bt_cnn_loss_grad / _train_step */
+int inmsom_bt_cnn_get_grad(inmsom_bt_cnn *net, float *grad);
+/* one batch: x [batch][ny][nx][8] network inputs (rounded to BF16),
+ * target [batch][ny][nx][2] (may be NULL), mask [batch][ny][nx] with bit 0
+ * = U point counted in the loss, bit 1 = V point (may be NULL) */
+int inmsom_bt_cnn_set_batch(inmsom_bt_cnn *net, const float *x, const float *target,
+                            const uint8_t *mask);
+/* forward of the batch: out [batch][ny][nx][2] (may be NULL) */
+int inmsom_bt_cnn_forward(inmsom_bt_cnn *net, float *out);
+/* hidden activation of the last forward (layer 0..2): [batch][ny][nx][32] */
+int inmsom_bt_cnn_get_activation(inmsom_bt_cnn *net, int layer, float *out);
+/* loss and gradient of the batch without an update */
+int inmsom_bt_cnn_loss_grad(inmsom_bt_cnn *net, double *loss);
+/* device-resident training set of nsamples images and one Adam step on the
+ * samples idx[0..batch-1]; *loss is the batch loss before the step */
+int inmsom_bt_cnn_dataset_alloc(inmsom_bt_cnn *net, int nsamples);
+int inmsom_bt_cnn_dataset_set(inmsom_bt_cnn *net, int i, const float *x, const float *target,
+                              const uint8_t *mask);
+int inmsom_bt_cnn_train_step(inmsom_bt_cnn *net, const int *idx, float lr, double *loss);
+/* loss of the samples idx[0..batch-1] without a gradient or an update */
+int inmsom_bt_cnn_eval(inmsom_bt_cnn *net, const int *idx, double *loss);
+int inmsom_bt_cnn_synchronize(inmsom_bt_cnn *net);
+
+/* Attach a closure to an ensemble (NULL detaches): after every outer step
+ * the network runs on every member (batched BF16-MFMA inference over the
+ * uneven basins, inside the per-step graph) and u += tau * tendency on U/V
+ * points, except for members whose closure-tendency L2 norm exceeds
+ * `ratio` times that of their plain viscous tendency nu * Laplacian(u, v):
+ * those keep the plain step (per step).  The network must outlive the
+ * attachment; its weights are read at every step. */
+int inmsom_bt_ens_set_closure(inmsom_bt_ens *ens, inmsom_bt_cnn *net, double ratio);
+/* per member: steps that fell back to plain viscosity, and the closure and
+ * viscous tendency norms of the last step (either pointer may be NULL) */
+int inmsom_bt_ens_closure_stats(inmsom_bt_ens *ens, int *nfallback, double *norms);
 
 #ifdef __cplusplus
 }

@@ -519,6 +519,14 @@ int inmsom_bt_volume(inmsom_bt_ctx *c, double *vol)
     return INMSOM_BT_OK;
 }
 
+int inmsom_bt_coarsen_state(inmsom_bt_ctx *c, int factor, double *u, double *v, double *ssh)
+{
+    REQUIRE(c && u && v && ssh, "null pointer");
+    HIPCHECK(hipSetDevice(c->device));
+    TRY(fused_coarsen(c, factor, u, v, ssh));
+    return INMSOM_BT_OK;
+}
+
 int inmsom_bt_synchronize(inmsom_bt_ctx *c)
 {
     REQUIRE(c, "null context");

@@ -1,0 +1,76 @@
+// Internal interface of the CNN closure shared by the training object
+// (bt_cnn.hip) and the batched ensemble step (bt_ensemble.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "inmsom_bt.h"
+
+namespace ibt {
+namespace cnn {
+
+constexpr int NL = 4;    // layers
+constexpr int CH = 32;   // hidden (and padded output) channels
+constexpr int CIN0 = 8;  // input channels, 4 used (u, v, eta, depth)
+constexpr int NIN = 4;
+constexpr int NOUT = 2;  // output channels used (u and v tendencies)
+
+// One image of a (possibly ragged) batch: rows H, columns W, first pixel.
+struct Img {
+    int H, W;
+    long long pix;
+};
+
+// Input normalisation x_c = (field_c - shift_c) * scale_c for (u, v, eta,
+// depth); the network's outputs times oscale are the u and v tendencies [m/s^2].
+struct Norm {
+    float shift[NIN], scale[NIN], oscale[NOUT];
+};
+
+// One basin the closure acts on.  u, v, s, H are the basin's (ny, nx)
+// arrays (float or double as the owning step stores them); the closure
+// image of the basin is the whole array.
+struct ClosureMember {
+    int nx, ny;
+    long long pix;  // first pixel in the packed activations
+    long long row0; // first row in the per-row partial sums
+    void *u, *v;
+    const void *s, *H;
+    const uint8_t *mb;  // mask bits (bit 1: U point, bit 2: V point)
+    double d, nu;       // grid step [m], lateral viscosity [m^2/s]
+};
+
+// Device buffers of batched inference over a set of basins.
+struct Infer {
+    int nmem = 0;
+    long long npix = 0;
+    int nrows = 0, ntiles = 0;
+    Img *imgs = nullptr;
+    int *tile_img = nullptr, *tile_local = nullptr;
+    int *row_member = nullptr, *row_y = nullptr;
+    __bf16 *x = nullptr, *a0 = nullptr, *a1 = nullptr;
+    float *tend = nullptr;
+    double *part = nullptr;   // [nrows][2]: closure and viscous squared norms
+    double *norms = nullptr;  // [nmem][2]: closure and viscous tendency norms of the last step
+    int *active = nullptr;    // [nmem]: closure applied in the last step
+    int *nfall = nullptr;     // [nmem]: steps that fell back to plain viscosity
+};
+
+int infer_create(int device, int nmem, const int *nx, const int *ny, Infer *inf);
+void infer_destroy(Infer *inf);
+
+// One application of the closure after an outer step of length dt: the
+// network's tendency is added to u and v (dt * tendency on U/V points) of
+// every member whose closure-tendency norm is at most ratio times the norm of
+// its plain viscous tendency nu * Laplacian(u, v); a member above the limit
+// keeps the plain step for this step (counted in nfall).  members is a device
+// table of inf.nmem entries.
+void closure_launch(const inmsom_bt_cnn *net, hipStream_t s, bool f32, ClosureMember *members, Infer &inf,
+                    double dt, double ratio);
+int net_device(const inmsom_bt_cnn *net);
+
+}  // namespace cnn
+}  // namespace ibt

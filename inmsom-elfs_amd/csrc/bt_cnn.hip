@@ -1,0 +1,962 @@
+// Learned lateral-momentum closure: a 4-layer 3x3 CNN (4 -> 32 -> 32 -> 32 -> 2
+// channels, ReLU) on the coarse C-grid, with BF16 weights and activations and
+// FP32 accumulation on the gfx950 matrix cores (v_mfma_f32_32x32x16_bf16),
+// trained on the GPU with hand-written forward, backward and Adam kernels.
+//
+// Convolutions are implicit GEMMs: one wave computes 32 consecutive pixels of
+// one image row x 32 output channels; the reduction runs over K = 9 taps x
+// input channels in steps of 16.  Activations are NHWC (channels innermost)
+// so every lane's 8-element operand slice is one 16-byte load.
+//   forward            y[p, co]  = sum_{t, ci} W[co][t][ci] x[p + o_t, ci]
+//   backward (data)    dx[q, ci] = sum_{t, co} W[co][t][ci] dy[q - o_t, co]
+//                      (the forward kernel with negated tap offsets and the
+//                      weights transposed to [ci][t][co])
+//   backward (weights) dW[co][t][ci] = sum_p dy[p, co] x[p + o_t, ci]
+//                      (pixels are the reduction axis: per-chunk partial sums,
+//                      then a fixed-order reduction -> deterministic)
+// MFMA operand layout (checked on the device): A[i][k] with i = lane % 32,
+// k = 8 (lane / 32) + e; B[k][j] with j = lane % 32; D[i][j] in acc[r] with
+// i = 8 (r / 4) + 4 (lane / 32) + r % 4, j = lane % 32.
+#include <hip/hip_bf16.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bt_cnn.h"
+#include "bt_internal.h"
+
+typedef __bf16 bf16;
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+namespace ibt {
+namespace cnn {
+
+struct Layer {
+    int cin;        // padded input channels (8 or 32)
+    int kpad;       // 9*cin rounded up to 16
+    size_t woff;    // offset of W[32][kpad] in the parameter vector
+    size_t boff;    // offset of b[32]
+};
+
+enum { EPI_RELU_BF16 = 0, EPI_OUT_F32 = 1, EPI_MASK_BF16 = 2 };
+
+template <int CIN, bool NEG, int EPI>
+__global__ void __launch_bounds__(256)
+    k_conv(const Img *__restrict__ imgs, const int *__restrict__ tile_img,
+           const int *__restrict__ tile_local, int ntiles, const bf16 *__restrict__ in,
+           const bf16 *__restrict__ w, int kpad, const float *__restrict__ bias,
+           const bf16 *__restrict__ act_prev, void *__restrict__ out)
+{
+    const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tile >= ntiles) return;  // uniform per wave
+    const int lane = threadIdx.x & 63, i = lane & 31, half = lane >> 5;
+    const Img im = imgs[tile_img[tile]];
+    const int tl = tile_local[tile];
+    const int tx = (im.W + 31) >> 5;
+    const int y = tl / tx, x0 = (tl - y * tx) * 32;
+    const int x = x0 + i;
+    f32x16 acc = {};
+    for (int s = 0; s < kpad / 16; ++s) {
+        const int k0 = 16 * s + 8 * half;
+        const int t = k0 / CIN, ci0 = k0 - t * CIN;
+        bf16x8 a = {};
+        if (t < 9) {
+            int oy = t / 3 - 1, ox = t % 3 - 1;
+            if (NEG) {
+                oy = -oy;
+                ox = -ox;
+            }
+            const int yy = y + oy, xx = x + ox;
+            if (x < im.W && yy >= 0 && yy < im.H && xx >= 0 && xx < im.W)
+                a = *reinterpret_cast<const bf16x8 *>(in + ((im.pix + (size_t)yy * im.W + xx) * CIN + ci0));
+        }
+        const bf16x8 b = *reinterpret_cast<const bf16x8 *>(w + (size_t)i * kpad + k0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    const int co = i;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int px = x0 + 8 * (r >> 2) + 4 * half + (r & 3);
+        if (px >= im.W) continue;
+        const size_t p = im.pix + (size_t)y * im.W + px;
+        float v = acc[r];
+        if (EPI == EPI_RELU_BF16) {
+            v = fmaxf(v + bias[co], 0.0f);
+            static_cast<bf16 *>(out)[p * CH + co] = (bf16)v;
+        } else if (EPI == EPI_OUT_F32) {
+            if (co < NOUT) static_cast<float *>(out)[p * NOUT + co] = v + bias[co];
+        } else {
+            const float g = (float)act_prev[p * CH + co] > 0.0f ? v : 0.0f;
+            static_cast<bf16 *>(out)[p * CH + co] = (bf16)g;
+        }
+    }
+}
+
+// Partial weight and bias gradients of one chunk of pixels (uniform batch:
+// B images of H x W).  NCB column blocks of 32 cover the 9*CIN columns.
+template <int CIN, int NCB>
+__global__ void __launch_bounds__(64)
+    k_wgrad(int H, int W, long long npix, int chunk, const bf16 *__restrict__ dy,
+            const bf16 *__restrict__ x, float *__restrict__ part_w, float *__restrict__ part_b)
+{
+    const int lane = threadIdx.x, j = lane & 31, half = lane >> 5;
+    const long long p0 = (long long)blockIdx.x * chunk;
+    const long long p1 = p0 + chunk < npix ? p0 + chunk : npix;
+    f32x16 acc[NCB];
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) acc[c] = f32x16{};
+    float bsum = 0.0f;
+    const long long hw = (long long)H * W;
+    for (long long p = p0; p < p1; p += 16) {
+        // the lane's 8 consecutive pixels, decoded once and stepped along x
+        bf16x8 a;
+        int yb[8], xb[8];
+        long long bb[8];
+        bool ok[8];
+        long long q0 = p + 8 * half;
+        if (q0 >= p1) q0 = p0;
+        long long b0 = q0 / hw;
+        long long r0 = q0 - b0 * hw;
+        int y0 = (int)(r0 / W), x0 = (int)(r0 - (long long)y0 * W);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            ok[e] = p + 8 * half + e < p1;
+            bb[e] = b0;
+            yb[e] = y0;
+            xb[e] = x0;
+            const long long q = (b0 * H + y0) * W + x0;
+            const bf16 v = ok[e] ? dy[q * CH + j] : (bf16)0.0f;
+            a[e] = v;
+            bsum += (float)v;
+            if (++x0 == W) {
+                x0 = 0;
+                if (++y0 == H) {
+                    y0 = 0;
+                    ++b0;
+                }
+            }
+        }
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) {
+            const int c = cb * 32 + j;
+            const int t = c / CIN, ci = c - t * CIN;
+            bf16x8 b;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                bf16 v = (bf16)0.0f;
+                if (ok[e] && t < 9) {
+                    const int yy = yb[e] + t / 3 - 1, xx = xb[e] + t % 3 - 1;
+                    if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                        v = x[((bb[e] * H + yy) * W + xx) * CIN + ci];
+                }
+                b[e] = v;
+            }
+            acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[cb], 0, 0, 0);
+        }
+    }
+    const int ncol = NCB * 32;
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int co = 8 * (r >> 2) + 4 * half + (r & 3);
+            part_w[((size_t)blockIdx.x * 32 + co) * ncol + cb * 32 + j] = acc[cb][r];
+        }
+    bsum += __shfl_xor(bsum, 32);
+    if (half == 0) part_b[(size_t)blockIdx.x * 32 + j] = bsum;
+}
+
+// Fixed-order sum of the chunk partials into the gradient (layout [32][kpad]).
+__global__ void k_reduce_wgrad(int nchunks, int ncol, int kvalid, int kpad, const float *__restrict__ part_w,
+                               const float *__restrict__ part_b, float *__restrict__ gw,
+                               float *__restrict__ gb)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < 32 * kpad) {
+        const int co = idx / kpad, k = idx - co * kpad;
+        float s = 0.0f;
+        if (k < kvalid)
+            for (int c = 0; c < nchunks; ++c) s += part_w[((size_t)c * 32 + co) * ncol + k];
+        gw[idx] = s;
+    }
+    if (idx < 32) {
+        float s = 0.0f;
+        for (int c = 0; c < nchunks; ++c) s += part_b[(size_t)c * 32 + idx];
+        gb[idx] = s;
+    }
+}
+
+// Masked mean-square loss over U (channel 0) and V (channel 1) points:
+// writes dL/dpred into channels 0..1 of dpred (channels 2..31 stay zero) and
+// per-block partial sums of the squared error.
+__global__ void k_loss(long long n, const float *__restrict__ pred, const float *__restrict__ tgt,
+                       const uint8_t *__restrict__ mask, float inv_count, bf16 *__restrict__ dpred,
+                       float *__restrict__ part)
+{
+    __shared__ float red[256];
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    float e2 = 0.0f;
+    if (p < n) {
+        const uint8_t mk = mask[p];
+#pragma unroll
+        for (int c = 0; c < NOUT; ++c) {
+            const float on = (mk >> c) & 1 ? 1.0f : 0.0f;
+            const float d = (pred[p * NOUT + c] - tgt[p * NOUT + c]) * on;
+            e2 += d * d;
+            dpred[p * CH + c] = (bf16)(2.0f * d * inv_count);
+        }
+        for (int c = NOUT; c < CH; ++c) dpred[p * CH + c] = (bf16)0.0f;
+    }
+    red[threadIdx.x] = e2;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Adam (Kingma & Ba 2015, Algorithm 1) in FP32; omb1 = 1 - b1, omb2 = 1 - b2
+// and the bias corrections c1, c2 come from the host, computed in double.
+__global__ void k_adam(int n, float *__restrict__ w, float *__restrict__ m, float *__restrict__ v,
+                       const float *__restrict__ g, float lr, float b1, float omb1, float b2, float omb2,
+                       float eps, float c1, float c2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + omb1 * gi;
+    const float vi = b2 * v[i] + omb2 * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// FP32 master weights -> BF16 forward layout [co][kpad] and, for layers with
+// 32 input channels, the transposed backward layout [ci][t*32 + co].
+__global__ void k_pack(int kpad, int cin, const float *__restrict__ wm, bf16 *__restrict__ wf,
+                       bf16 *__restrict__ wt)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 32 * kpad) return;
+    const int co = idx / kpad, k = idx - co * kpad;
+    const bf16 v = (bf16)wm[idx];
+    wf[idx] = v;
+    if (wt && k < 9 * cin) {
+        const int t = k / cin, ci = k - t * cin;
+        wt[(size_t)ci * 288 + t * 32 + co] = v;
+    }
+}
+
+
+// ---- closure inference over a set of basins (one block per basin row)
+
+template <class T>
+__device__ inline float ld(const void *p, long long i)
+{
+    return (float)static_cast<const T *>(p)[i];
+}
+
+// inputs of the network: (u, v, eta, depth) shifted and scaled, in BF16
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_features(const ClosureMember *__restrict__ mem, const int *__restrict__ row_member,
+               const int *__restrict__ row_y, Norm nm, bf16 *__restrict__ x)
+{
+    const ClosureMember m = mem[row_member[blockIdx.x]];
+    const int y = row_y[blockIdx.x];
+    for (int i = threadIdx.x; i < m.nx; i += blockDim.x) {
+        const long long a = (long long)y * m.nx + i;
+        const long long p = m.pix + a;
+        bf16x8 f;
+        f[0] = (bf16)((ld<T>(m.u, a) - nm.shift[0]) * nm.scale[0]);
+        f[1] = (bf16)((ld<T>(m.v, a) - nm.shift[1]) * nm.scale[1]);
+        f[2] = (bf16)((ld<T>(m.s, a) - nm.shift[2]) * nm.scale[2]);
+        f[3] = (bf16)((ld<T>(m.H, a) - nm.shift[3]) * nm.scale[3]);
+        f[4] = f[5] = f[6] = f[7] = (bf16)0.0f;
+        *reinterpret_cast<bf16x8 *>(x + p * CIN0) = f;
+    }
+}
+
+// per-row squared norms of the closure tendency and of the plain viscous
+// tendency nu * Laplacian(u, v) on U and V points
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_cl_norm(const ClosureMember *__restrict__ mem, const int *__restrict__ row_member,
+              const int *__restrict__ row_y, Norm nm, const float *__restrict__ tend,
+              double *__restrict__ part)
+{
+    __shared__ double red[2][256];
+    const ClosureMember m = mem[row_member[blockIdx.x]];
+    const int y = row_y[blockIdx.x];
+    double sc = 0.0, sv = 0.0;
+    if (y >= 1 && y < m.ny - 1) {
+        const double k = m.nu / (m.d * m.d);
+        for (int i = 1 + threadIdx.x; i < m.nx - 1; i += blockDim.x) {
+            const long long a = (long long)y * m.nx + i;
+            const uint8_t b = m.mb[a];
+            if (b & MB_LCU) {
+                const double c = (double)tend[(m.pix + a) * NOUT + 0] * nm.oscale[0];
+                const double l = k * ((double)ld<T>(m.u, a - 1) + ld<T>(m.u, a + 1) + ld<T>(m.u, a - m.nx) +
+                                      ld<T>(m.u, a + m.nx) - 4.0 * ld<T>(m.u, a));
+                sc += c * c;
+                sv += l * l;
+            }
+            if (b & MB_LCV) {
+                const double c = (double)tend[(m.pix + a) * NOUT + 1] * nm.oscale[1];
+                const double l = k * ((double)ld<T>(m.v, a - 1) + ld<T>(m.v, a + 1) + ld<T>(m.v, a - m.nx) +
+                                      ld<T>(m.v, a + m.nx) - 4.0 * ld<T>(m.v, a));
+                sc += c * c;
+                sv += l * l;
+            }
+        }
+    }
+    red[0][threadIdx.x] = sc;
+    red[1][threadIdx.x] = sv;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + st];
+            red[1][threadIdx.x] += red[1][threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * (m.row0 + y) + 0] = red[0][0];
+        part[2 * (m.row0 + y) + 1] = red[1][0];
+    }
+}
+
+// per basin: norms and the fallback decision of this step
+__global__ void __launch_bounds__(256)
+    k_cl_decide(const ClosureMember *__restrict__ mem, const double *__restrict__ part, double ratio,
+                double *__restrict__ norms, int *__restrict__ active, int *__restrict__ nfall)
+{
+    __shared__ double red[2][256];
+    const ClosureMember m = mem[blockIdx.x];
+    double sc = 0.0, sv = 0.0;
+    for (int r = threadIdx.x; r < m.ny; r += blockDim.x) {
+        sc += part[2 * (m.row0 + r) + 0];
+        sv += part[2 * (m.row0 + r) + 1];
+    }
+    red[0][threadIdx.x] = sc;
+    red[1][threadIdx.x] = sv;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + st];
+            red[1][threadIdx.x] += red[1][threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double nc = sqrt(red[0][0]), nv = sqrt(red[1][0]);
+        norms[2 * blockIdx.x + 0] = nc;
+        norms[2 * blockIdx.x + 1] = nv;
+        const int ok = nc <= ratio * nv ? 1 : 0;
+        active[blockIdx.x] = ok;
+        nfall[blockIdx.x] += 1 - ok;
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_cl_apply(ClosureMember *__restrict__ mem, const int *__restrict__ row_member,
+               const int *__restrict__ row_y, Norm nm, const float *__restrict__ tend,
+               const int *__restrict__ active, double dt)
+{
+    const int mi = row_member[blockIdx.x];
+    if (!active[mi]) return;
+    const ClosureMember m = mem[mi];
+    const int y = row_y[blockIdx.x];
+    T *u = static_cast<T *>(m.u), *v = static_cast<T *>(m.v);
+    for (int i = threadIdx.x; i < m.nx; i += blockDim.x) {
+        const long long a = (long long)y * m.nx + i;
+        const uint8_t b = m.mb[a];
+        if (b & MB_LCU) u[a] = (T)((double)u[a] + dt * ((double)tend[(m.pix + a) * NOUT + 0] * nm.oscale[0]));
+        if (b & MB_LCV) v[a] = (T)((double)v[a] + dt * ((double)tend[(m.pix + a) * NOUT + 1] * nm.oscale[1]));
+    }
+}
+
+}  // namespace cnn
+}  // namespace ibt
+
+using namespace ibt;
+using namespace ibt::cnn;
+
+struct inmsom_bt_cnn {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int B = 0, H = 0, W = 0;  // training batch shape
+    Layer L[NL];
+    size_t nparam = 0;
+    Norm norm;
+    float *wm = nullptr, *am = nullptr, *av = nullptr, *grad = nullptr;  // master, Adam m/v, gradient
+    bf16 *wf = nullptr, *wt = nullptr;                                   // BF16 forward / transposed
+    size_t wf_off[NL] = {}, wt_off[NL] = {};
+    int adam_t = 0;
+    // training batch buffers
+    bf16 *x = nullptr, *act[3] = {}, *dy = nullptr, *dx = nullptr;
+    float *pred = nullptr, *tgt = nullptr, *loss_part = nullptr, *part_w = nullptr, *part_b = nullptr;
+    uint8_t *mask = nullptr;
+    Img *imgs = nullptr;
+    int *tile_img = nullptr, *tile_local = nullptr;
+    int ntiles = 0;
+    int wg_chunk = 8192, wg_nchunks = 0;
+    // device-resident training set
+    int nsamples = 0;
+    bf16 *ds_x = nullptr;
+    float *ds_t = nullptr;
+    uint8_t *ds_m = nullptr;
+    std::vector<long long> ds_count;
+};
+
+namespace {
+
+int cfail(const std::string &m, int code)
+{
+    set_error(m);
+    return code;
+}
+
+#define CHC(expr)                                                                             \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return cfail(std::string(#expr) + ": " + hipGetErrorString(e_), \
+                                           INMSOM_BT_EDEVICE);                                \
+    } while (0)
+
+#define CTRY(expr)                         \
+    do {                                   \
+        int r_ = (expr);                   \
+        if (r_ != INMSOM_BT_OK) return r_; \
+    } while (0)
+
+template <class T>
+int dmalloc(T **p, size_t count)
+{
+    CHC(hipMalloc(p, count * sizeof(T) + 64));
+    CHC(hipMemset(*p, 0, count * sizeof(T) + 64));
+    return INMSOM_BT_OK;
+}
+
+void layer_table(Layer *L, size_t *nparam)
+{
+    size_t off = 0;
+    for (int l = 0; l < NL; ++l) {
+        L[l].cin = l == 0 ? CIN0 : CH;
+        L[l].kpad = (9 * L[l].cin + 15) / 16 * 16;
+        L[l].woff = off;
+        off += (size_t)32 * L[l].kpad;
+        L[l].boff = off;
+        off += 32;
+    }
+    *nparam = off;
+}
+
+// tile tables (one tile = 32 pixels of one row) of a batch of images
+int build_tiles(const std::vector<Img> &imgs, Img **d_imgs, int **d_ti, int **d_tl, int *ntiles)
+{
+    std::vector<int> ti, tl;
+    for (size_t k = 0; k < imgs.size(); ++k) {
+        const int n = imgs[k].H * ((imgs[k].W + 31) / 32);
+        for (int t = 0; t < n; ++t) {
+            ti.push_back((int)k);
+            tl.push_back(t);
+        }
+    }
+    CTRY(dmalloc(d_imgs, imgs.size()));
+    CTRY(dmalloc(d_ti, ti.size()));
+    CTRY(dmalloc(d_tl, tl.size()));
+    CHC(hipMemcpy(*d_imgs, imgs.data(), imgs.size() * sizeof(Img), hipMemcpyHostToDevice));
+    CHC(hipMemcpy(*d_ti, ti.data(), ti.size() * 4, hipMemcpyHostToDevice));
+    CHC(hipMemcpy(*d_tl, tl.data(), tl.size() * 4, hipMemcpyHostToDevice));
+    *ntiles = (int)ti.size();
+    return INMSOM_BT_OK;
+}
+
+void conv_launch(const inmsom_bt_cnn *c, hipStream_t st, int l, bool bwd, const Img *imgs, const int *ti,
+                 const int *tl, int ntiles, const bf16 *in, const bf16 *act_prev, void *out)
+{
+    const dim3 grid((unsigned)((ntiles + 3) / 4)), blk(256);
+    const Layer &Ly = c->L[l];
+    const float *bias = c->wm + Ly.boff;
+    if (!bwd) {
+        const bf16 *w = c->wf + c->wf_off[l];
+        if (l == 0)
+            hipLaunchKernelGGL((k_conv<CIN0, false, EPI_RELU_BF16>), grid, blk, 0, st, imgs, ti, tl, ntiles, in,
+                               w, Ly.kpad, bias, nullptr, out);
+        else if (l < NL - 1)
+            hipLaunchKernelGGL((k_conv<CH, false, EPI_RELU_BF16>), grid, blk, 0, st, imgs, ti, tl, ntiles, in,
+                               w, Ly.kpad, bias, nullptr, out);
+        else
+            hipLaunchKernelGGL((k_conv<CH, false, EPI_OUT_F32>), grid, blk, 0, st, imgs, ti, tl, ntiles, in, w,
+                               Ly.kpad, bias, nullptr, out);
+    } else {
+        // data gradient through layer l (l >= 1): weights as [ci][t*32 + co]
+        const bf16 *w = c->wt + c->wt_off[l];
+        hipLaunchKernelGGL((k_conv<CH, true, EPI_MASK_BF16>), grid, blk, 0, st, imgs, ti, tl, ntiles, in, w,
+                           9 * CH, nullptr, act_prev, out);
+    }
+}
+
+void wgrad_launch(inmsom_bt_cnn *c, int l, const bf16 *dy, const bf16 *x)
+{
+    const long long npix = (long long)c->B * c->H * c->W;
+    const int nch = (int)((npix + c->wg_chunk - 1) / c->wg_chunk);
+    const Layer &Ly = c->L[l];
+    if (l == 0)
+        hipLaunchKernelGGL((k_wgrad<CIN0, 3>), dim3(nch), dim3(64), 0, c->stream, c->H, c->W, npix,
+                           c->wg_chunk, dy, x, c->part_w, c->part_b);
+    else
+        hipLaunchKernelGGL((k_wgrad<CH, 9>), dim3(nch), dim3(64), 0, c->stream, c->H, c->W, npix, c->wg_chunk,
+                           dy, x, c->part_w, c->part_b);
+    const int ncol = l == 0 ? 96 : 288;
+    const int n = 32 * Ly.kpad;
+    hipLaunchKernelGGL(k_reduce_wgrad, dim3((n + 255) / 256), dim3(256), 0, c->stream, nch, ncol, 9 * Ly.cin,
+                       Ly.kpad, c->part_w, c->part_b, c->grad + Ly.woff, c->grad + Ly.boff);
+}
+
+void pack_all(inmsom_bt_cnn *c)
+{
+    for (int l = 0; l < NL; ++l) {
+        const Layer &Ly = c->L[l];
+        const int n = 32 * Ly.kpad;
+        hipLaunchKernelGGL(k_pack, dim3((n + 255) / 256), dim3(256), 0, c->stream, Ly.kpad, Ly.cin,
+                           c->wm + Ly.woff, c->wf + c->wf_off[l], l >= 1 ? c->wt + c->wt_off[l] : nullptr);
+    }
+}
+
+// forward over a set of images: a0 <- L0(x), a1 <- L1(a0), a2 <- L2(a1), out <- L3(a2)
+void forward_images(const inmsom_bt_cnn *c, hipStream_t st, const Img *imgs, const int *ti, const int *tl,
+                    int ntiles, const bf16 *x, bf16 *a0, bf16 *a1, bf16 *a2, float *out)
+{
+    conv_launch(c, st, 0, false, imgs, ti, tl, ntiles, x, nullptr, a0);
+    conv_launch(c, st, 1, false, imgs, ti, tl, ntiles, a0, nullptr, a1);
+    conv_launch(c, st, 2, false, imgs, ti, tl, ntiles, a1, nullptr, a2);
+    conv_launch(c, st, 3, false, imgs, ti, tl, ntiles, a2, nullptr, out);
+}
+
+void forward_batch(inmsom_bt_cnn *c)
+{
+    forward_images(c, c->stream, c->imgs, c->tile_img, c->tile_local, c->ntiles, c->x, c->act[0], c->act[1],
+                   c->act[2], c->pred);
+}
+
+// loss + full backward of the current batch; gradient into c->grad
+int backward(inmsom_bt_cnn *c, long long count, double *loss)
+{
+    const long long np = (long long)c->B * c->H * c->W;
+    const float inv = count > 0 ? 1.0f / (float)count : 0.0f;
+    const int nb = (int)((np + 255) / 256);
+    hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, c->stream, np, c->pred, c->tgt, c->mask, inv, c->dy,
+                       c->loss_part);
+    wgrad_launch(c, 3, c->dy, c->act[2]);
+    conv_launch(c, c->stream, 3, true, c->imgs, c->tile_img, c->tile_local, c->ntiles, c->dy, c->act[2], c->dx);
+    wgrad_launch(c, 2, c->dx, c->act[1]);
+    conv_launch(c, c->stream, 2, true, c->imgs, c->tile_img, c->tile_local, c->ntiles, c->dx, c->act[1], c->dy);
+    wgrad_launch(c, 1, c->dy, c->act[0]);
+    conv_launch(c, c->stream, 1, true, c->imgs, c->tile_img, c->tile_local, c->ntiles, c->dy, c->act[0], c->dx);
+    wgrad_launch(c, 0, c->dx, c->x);
+    CHC(hipGetLastError());
+    if (loss) {
+        std::vector<float> part(nb);
+        CHC(hipMemcpyAsync(part.data(), c->loss_part, (size_t)nb * 4, hipMemcpyDeviceToHost, c->stream));
+        CHC(hipStreamSynchronize(c->stream));
+        double s = 0.0;
+        for (float v : part) s += v;
+        *loss = count > 0 ? s / (double)count : 0.0;
+    }
+    return INMSOM_BT_OK;
+}
+
+std::vector<bf16> to_bf16(const float *x, size_t n)
+{
+    std::vector<bf16> out(n);
+    for (size_t i = 0; i < n; ++i) out[i] = (bf16)x[i];
+    return out;
+}
+
+}  // namespace
+
+// ---- batched closure inference (used by the ensemble step)
+namespace ibt {
+namespace cnn {
+
+int infer_create(int device, int nmem, const int *nx, const int *ny, Infer *inf)
+{
+    CHC(hipSetDevice(device));
+    *inf = Infer();
+    inf->nmem = nmem;
+    std::vector<Img> imgs(nmem);
+    std::vector<int> rm, ry;
+    long long pix = 0;
+    for (int k = 0; k < nmem; ++k) {
+        imgs[k] = Img{ny[k], nx[k], pix};
+        pix += (long long)nx[k] * ny[k];
+        for (int y = 0; y < ny[k]; ++y) {
+            rm.push_back(k);
+            ry.push_back(y);
+        }
+    }
+    inf->npix = pix;
+    inf->nrows = (int)rm.size();
+    CTRY(build_tiles(imgs, &inf->imgs, &inf->tile_img, &inf->tile_local, &inf->ntiles));
+    CTRY(dmalloc(&inf->row_member, rm.size()));
+    CTRY(dmalloc(&inf->row_y, ry.size()));
+    CHC(hipMemcpy(inf->row_member, rm.data(), rm.size() * 4, hipMemcpyHostToDevice));
+    CHC(hipMemcpy(inf->row_y, ry.data(), ry.size() * 4, hipMemcpyHostToDevice));
+    CTRY(dmalloc(&inf->x, (size_t)pix * CIN0));
+    CTRY(dmalloc(&inf->a0, (size_t)pix * CH));
+    CTRY(dmalloc(&inf->a1, (size_t)pix * CH));
+    CTRY(dmalloc(&inf->tend, (size_t)pix * NOUT));
+    CTRY(dmalloc(&inf->part, (size_t)inf->nrows * 2));
+    CTRY(dmalloc(&inf->norms, (size_t)nmem * 2));
+    CTRY(dmalloc(&inf->active, (size_t)nmem));
+    CTRY(dmalloc(&inf->nfall, (size_t)nmem));
+    return INMSOM_BT_OK;
+}
+
+void infer_destroy(Infer *inf)
+{
+    for (void *p : {(void *)inf->imgs, (void *)inf->tile_img, (void *)inf->tile_local, (void *)inf->row_member,
+                    (void *)inf->row_y, (void *)inf->x, (void *)inf->a0, (void *)inf->a1, (void *)inf->tend,
+                    (void *)inf->part, (void *)inf->norms, (void *)inf->active, (void *)inf->nfall})
+        if (p) hipFree(p);
+    *inf = Infer();
+}
+
+void closure_launch(const inmsom_bt_cnn *net, hipStream_t st, bool f32, ClosureMember *members, Infer &inf,
+                    double dt, double ratio)
+{
+    const dim3 rows((unsigned)inf.nrows), blk(256);
+    if (f32)
+        hipLaunchKernelGGL(k_features<float>, rows, blk, 0, st, members, inf.row_member, inf.row_y, net->norm,
+                           inf.x);
+    else
+        hipLaunchKernelGGL(k_features<double>, rows, blk, 0, st, members, inf.row_member, inf.row_y, net->norm,
+                           inf.x);
+    // activations ping-pong through a0/a1 (layer 2 writes a0 again)
+    forward_images(net, st, inf.imgs, inf.tile_img, inf.tile_local, inf.ntiles, inf.x, inf.a0, inf.a1, inf.a0,
+                   inf.tend);
+    if (f32)
+        hipLaunchKernelGGL(k_cl_norm<float>, rows, blk, 0, st, members, inf.row_member, inf.row_y, net->norm,
+                           inf.tend, inf.part);
+    else
+        hipLaunchKernelGGL(k_cl_norm<double>, rows, blk, 0, st, members, inf.row_member, inf.row_y, net->norm,
+                           inf.tend, inf.part);
+    hipLaunchKernelGGL(k_cl_decide, dim3((unsigned)inf.nmem), blk, 0, st, members, inf.part, ratio, inf.norms,
+                       inf.active, inf.nfall);
+    if (f32)
+        hipLaunchKernelGGL(k_cl_apply<float>, rows, blk, 0, st, members, inf.row_member, inf.row_y, net->norm,
+                           inf.tend, inf.active, dt);
+    else
+        hipLaunchKernelGGL(k_cl_apply<double>, rows, blk, 0, st, members, inf.row_member, inf.row_y, net->norm,
+                           inf.tend, inf.active, dt);
+}
+
+int net_device(const inmsom_bt_cnn *net) { return net->device; }
+
+}  // namespace cnn
+}  // namespace ibt
+
+extern "C" {
+
+int inmsom_bt_cnn_create(int batch, int ny, int nx, int device, inmsom_bt_cnn **out)
+{
+    if (!out || batch < 1 || ny < 3 || nx < 3) return cfail("bad arguments", INMSOM_BT_EINVAL);
+    *out = nullptr;
+    CHC(hipSetDevice(device));
+    auto *c = new inmsom_bt_cnn();
+    c->device = device;
+    c->B = batch;
+    c->H = ny;
+    c->W = nx;
+    layer_table(c->L, &c->nparam);
+    for (int k = 0; k < NIN; ++k) {
+        c->norm.shift[k] = 0.0f;
+        c->norm.scale[k] = 1.0f;
+    }
+    c->norm.oscale[0] = c->norm.oscale[1] = 1.0f;
+    int rc = [&]() -> int {
+        CHC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        CTRY(dmalloc(&c->wm, c->nparam));
+        CTRY(dmalloc(&c->am, c->nparam));
+        CTRY(dmalloc(&c->av, c->nparam));
+        CTRY(dmalloc(&c->grad, c->nparam));
+        size_t fo = 0, to = 0;
+        for (int l = 0; l < NL; ++l) {
+            c->wf_off[l] = fo;
+            fo += (size_t)32 * c->L[l].kpad;
+            c->wt_off[l] = to;
+            if (l >= 1) to += (size_t)32 * 9 * CH;
+        }
+        CTRY(dmalloc(&c->wf, fo));
+        CTRY(dmalloc(&c->wt, to));
+        const size_t np = (size_t)batch * ny * nx;
+        CTRY(dmalloc(&c->x, np * CIN0));
+        for (int k = 0; k < 3; ++k) CTRY(dmalloc(&c->act[k], np * CH));
+        CTRY(dmalloc(&c->dy, np * CH));
+        CTRY(dmalloc(&c->dx, np * CH));
+        CTRY(dmalloc(&c->pred, np * NOUT));
+        CTRY(dmalloc(&c->tgt, np * NOUT));
+        CTRY(dmalloc(&c->mask, np));
+        CTRY(dmalloc(&c->loss_part, (np + 255) / 256));
+        c->wg_nchunks = (int)((np + c->wg_chunk - 1) / c->wg_chunk);
+        CTRY(dmalloc(&c->part_w, (size_t)c->wg_nchunks * 32 * 9 * CH));
+        CTRY(dmalloc(&c->part_b, (size_t)c->wg_nchunks * 32));
+        std::vector<Img> imgs(batch);
+        for (int b = 0; b < batch; ++b) imgs[b] = Img{ny, nx, (long long)b * ny * nx};
+        CTRY(build_tiles(imgs, &c->imgs, &c->tile_img, &c->tile_local, &c->ntiles));
+        return INMSOM_BT_OK;
+    }();
+    if (rc) {
+        inmsom_bt_cnn_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_cnn_destroy(inmsom_bt_cnn *c)
+{
+    if (!c) return INMSOM_BT_OK;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void *p : {(void *)c->wm, (void *)c->am, (void *)c->av, (void *)c->grad, (void *)c->wf,
+                    (void *)c->wt, (void *)c->x, (void *)c->act[0], (void *)c->act[1], (void *)c->act[2],
+                    (void *)c->dy, (void *)c->dx, (void *)c->pred, (void *)c->tgt, (void *)c->mask,
+                    (void *)c->loss_part, (void *)c->part_w, (void *)c->part_b, (void *)c->imgs,
+                    (void *)c->tile_img, (void *)c->tile_local, (void *)c->ds_x, (void *)c->ds_t,
+                    (void *)c->ds_m})
+        if (p) hipFree(p);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_cnn_num_params(inmsom_bt_cnn *c, long long *n)
+{
+    if (!c || !n) return cfail("null pointer", INMSOM_BT_EINVAL);
+    *n = (long long)c->nparam;
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_cnn_set_norm(inmsom_bt_cnn *c, const float *in_shift, const float *in_scale,
+                           const float *out_scale)
+{
+    if (!c || !in_shift || !in_scale || !out_scale) return cfail("null pointer", INMSOM_BT_EINVAL);
+    for (int k = 0; k < NIN; ++k) {
+        c->norm.shift[k] = in_shift[k];
+        c->norm.scale[k] = in_scale[k];
+    }
+    c->norm.oscale[0] = out_scale[0];
+    c->norm.oscale[1] = out_scale[1];
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_cnn_set_params(inmsom_bt_cnn *c, const float *params)
+{
+    if (!c || !params) return cfail("null pointer", INMSOM_BT_EINVAL);
+    CHC(hipSetDevice(c->device));
+    CHC(hipStreamSynchronize(c->stream));
+    // padded entries of W (k >= 9 cin) must stay zero
+    std::vector<float> p(params, params + c->nparam);
+    for (int l = 0; l < NL; ++l)
+        for (int co = 0; co < 32; ++co)
+            for (int k = 9 * c->L[l].cin; k < c->L[l].kpad; ++k) p[c->L[l].woff + (size_t)co * c->L[l].kpad + k] = 0.f;
+    CHC(hipMemcpy(c->wm, p.data(), c->nparam * 4, hipMemcpyHostToDevice));
+    CHC(hipMemset(c->am, 0, c->nparam * 4));
+    CHC(hipMemset(c->av, 0, c->nparam * 4));
+    c->adam_t = 0;
+    pack_all(c);
+    CHC(hipGetLastError());
+    CHC(hipStreamSynchronize(c->stream));
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_cnn_get_params(inmsom_bt_cnn *c, float *params)
+{
+    if (!c || !params) return cfail("null pointer", INMSOM_BT_EINVAL);
+    CHC(hipSetDevice(c->device));
+    CHC(hipStreamSynchronize(c->stream));
+    CHC(hipMemcpy(params, c->wm, c->nparam * 4, hipMemcpyDeviceToHost));
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_cnn_get_grad(inmsom_bt_cnn *c, float *grad)
+{
+    if (!c || !grad) return cfail("null pointer", INMSOM_BT_EINVAL);
+    CHC(hipSetDevice(c->device));
+    CHC(hipStreamSynchronize(c->stream));
+    CHC(hipMemcpy(grad, c->grad, c->nparam * 4, hipMemcpyDeviceToHost));
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_cnn_set_batch(inmsom_bt_cnn *c, const float *x, const float *target, const uint8_t *mask)
+{
+    if (!c || !x) return cfail("null pointer", INMSOM_BT_EINVAL);
+    CHC(hipSetDevice(c->device));
+    CHC(hipStreamSynchronize(c->stream));
+    const size_t np = (size_t)c->B * c->H * c->W;
+    const std::vector<bf16> xb = to_bf16(x, np * CIN0);
+    CHC(hipMemcpy(c->x, xb.data(), xb.size() * 2, hipMemcpyHostToDevice));
+    if (target) CHC(hipMemcpy(c->tgt, target, np * NOUT * 4, hipMemcpyHostToDevice));
+    if (mask) CHC(hipMemcpy(c->mask, mask, np, hipMemcpyHostToDevice));
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_cnn_forward(inmsom_bt_cnn *c, float *out)
+{
+    if (!c) return cfail("null pointer", INMSOM_BT_EINVAL);
+    CHC(hipSetDevice(c->device));
+    forward_batch(c);
+    CHC(hipGetLastError());
+    if (out)
+        CHC(hipMemcpyAsync(out, c->pred, (size_t)c->B * c->H * c->W * NOUT * 4, hipMemcpyDeviceToHost,
+                           c->stream));
+    CHC(hipStreamSynchronize(c->stream));
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_cnn_get_activation(inmsom_bt_cnn *c, int layer, float *out)
+{
+    if (!c || !out || layer < 0 || layer > 2) return cfail("bad arguments", INMSOM_BT_EINVAL);
+    CHC(hipSetDevice(c->device));
+    CHC(hipStreamSynchronize(c->stream));
+    const size_t n = (size_t)c->B * c->H * c->W * CH;
+    std::vector<bf16> a(n);
+    CHC(hipMemcpy(a.data(), c->act[layer], n * 2, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) out[i] = (float)a[i];
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_cnn_loss_grad(inmsom_bt_cnn *c, double *loss)
+{
+    if (!c) return cfail("null pointer", INMSOM_BT_EINVAL);
+    CHC(hipSetDevice(c->device));
+    CHC(hipStreamSynchronize(c->stream));
+    const size_t np = (size_t)c->B * c->H * c->W;
+    std::vector<uint8_t> mk(np);
+    CHC(hipMemcpy(mk.data(), c->mask, np, hipMemcpyDeviceToHost));
+    long long count = 0;
+    for (uint8_t m : mk) count += (m & 1) + ((m >> 1) & 1);
+    forward_batch(c);
+    CTRY(backward(c, count, loss));
+    CHC(hipStreamSynchronize(c->stream));
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_cnn_dataset_alloc(inmsom_bt_cnn *c, int nsamples)
+{
+    if (!c || nsamples < 1) return cfail("bad arguments", INMSOM_BT_EINVAL);
+    CHC(hipSetDevice(c->device));
+    CHC(hipStreamSynchronize(c->stream));
+    for (void **p : {(void **)&c->ds_x, (void **)&c->ds_t, (void **)&c->ds_m})
+        if (*p) {
+            CHC(hipFree(*p));
+            *p = nullptr;
+        }
+    c->nsamples = 0;
+    const size_t hw = (size_t)c->H * c->W;
+    CTRY(dmalloc(&c->ds_x, (size_t)nsamples * hw * CIN0));
+    CTRY(dmalloc(&c->ds_t, (size_t)nsamples * hw * NOUT));
+    CTRY(dmalloc(&c->ds_m, (size_t)nsamples * hw));
+    c->nsamples = nsamples;
+    c->ds_count.assign(nsamples, 0);
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_cnn_dataset_set(inmsom_bt_cnn *c, int i, const float *x, const float *target, const uint8_t *mask)
+{
+    if (!c || i < 0 || i >= c->nsamples || !x || !target || !mask) return cfail("bad arguments", INMSOM_BT_EINVAL);
+    CHC(hipSetDevice(c->device));
+    const size_t hw = (size_t)c->H * c->W;
+    const std::vector<bf16> xb = to_bf16(x, hw * CIN0);
+    CHC(hipMemcpy(c->ds_x + (size_t)i * hw * CIN0, xb.data(), xb.size() * 2, hipMemcpyHostToDevice));
+    CHC(hipMemcpy(c->ds_t + (size_t)i * hw * NOUT, target, hw * NOUT * 4, hipMemcpyHostToDevice));
+    CHC(hipMemcpy(c->ds_m + (size_t)i * hw, mask, hw, hipMemcpyHostToDevice));
+    long long cnt = 0;
+    for (size_t k = 0; k < hw; ++k) cnt += (mask[k] & 1) + ((mask[k] >> 1) & 1);
+    c->ds_count[i] = cnt;
+    return INMSOM_BT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// gathers samples idx[0..B-1] of the training set into the batch buffers
+int gather(inmsom_bt_cnn *c, const int *idx, long long *count)
+{
+    if (!c || !idx) return cfail("null pointer", INMSOM_BT_EINVAL);
+    if (!c->ds_x) return cfail("no training set", INMSOM_BT_ESTATE);
+    for (int b = 0; b < c->B; ++b)
+        if (idx[b] < 0 || idx[b] >= c->nsamples) return cfail("sample index out of range", INMSOM_BT_EINVAL);
+    CHC(hipSetDevice(c->device));
+    const size_t hw = (size_t)c->H * c->W;
+    *count = 0;
+    for (int b = 0; b < c->B; ++b) {
+        const size_t i = (size_t)idx[b];
+        CHC(hipMemcpyAsync(c->x + b * hw * CIN0, c->ds_x + i * hw * CIN0, hw * CIN0 * 2, hipMemcpyDeviceToDevice,
+                           c->stream));
+        CHC(hipMemcpyAsync(c->tgt + b * hw * NOUT, c->ds_t + i * hw * NOUT, hw * NOUT * 4,
+                           hipMemcpyDeviceToDevice, c->stream));
+        CHC(hipMemcpyAsync(c->mask + b * hw, c->ds_m + i * hw, hw, hipMemcpyDeviceToDevice, c->stream));
+        *count += c->ds_count[i];
+    }
+    return INMSOM_BT_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int inmsom_bt_cnn_eval(inmsom_bt_cnn *c, const int *idx, double *loss)
+{
+    long long count = 0;
+    CTRY(gather(c, idx, &count));
+    if (!loss) return cfail("null pointer", INMSOM_BT_EINVAL);
+    forward_batch(c);
+    const long long np = (long long)c->B * c->H * c->W;
+    const int nb = (int)((np + 255) / 256);
+    hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, c->stream, np, c->pred, c->tgt, c->mask,
+                       count > 0 ? 1.0f / (float)count : 0.0f, c->dy, c->loss_part);
+    CHC(hipGetLastError());
+    std::vector<float> part(nb);
+    CHC(hipMemcpyAsync(part.data(), c->loss_part, (size_t)nb * 4, hipMemcpyDeviceToHost, c->stream));
+    CHC(hipStreamSynchronize(c->stream));
+    double s = 0.0;
+    for (float v : part) s += v;
+    *loss = count > 0 ? s / (double)count : 0.0;
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_cnn_train_step(inmsom_bt_cnn *c, const int *idx, float lr, double *loss)
+{
+    long long count = 0;
+    CTRY(gather(c, idx, &count));
+    forward_batch(c);
+    CTRY(backward(c, count, loss));
+    c->adam_t += 1;
+    const double b1 = 0.9, b2 = 0.999;
+    const float c1 = (float)(1.0 - pow(b1, c->adam_t)), c2 = (float)(1.0 - pow(b2, c->adam_t));
+    const int n = (int)c->nparam;
+    hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->wm, c->am, c->av, c->grad, lr,
+                       (float)b1, (float)(1.0 - b1), (float)b2, (float)(1.0 - b2), 1e-8f, c1, c2);
+    pack_all(c);
+    CHC(hipGetLastError());
+    CHC(hipStreamSynchronize(c->stream));
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_cnn_synchronize(inmsom_bt_cnn *c)
+{
+    if (!c) return cfail("null pointer", INMSOM_BT_EINVAL);
+    CHC(hipSetDevice(c->device));
+    CHC(hipStreamSynchronize(c->stream));
+    return INMSOM_BT_OK;
+}
+
+}  // extern "C"

@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "bt_cnn.h"
 #include "bt_fused_body.h"
 #include "bt_internal.h"
 
@@ -116,6 +117,11 @@ struct inmsom_bt_ens {
     unsigned long long *cnt = nullptr;
     // snapshot of every member's state (inmsom_bt_ens_save / _restore)
     std::vector<void *> snap;
+    // learned closure applied after every outer step
+    inmsom_bt_cnn *closure = nullptr;
+    double closure_ratio = 3.0;
+    ibt::cnn::Infer inf;
+    ibt::cnn::ClosureMember *cl_mem[2] = {};  // device tables per buffer set
 };
 
 namespace {
@@ -350,6 +356,20 @@ void outer_step(inmsom_bt_ens *e, int phase, double tau, int nstep, int wpx)
         launch_batch(e, phase, tau / (double)nstep, sub, nstep, wpx);
         phase = 1 - phase;
     }
+    if (e->closure)
+        ibt::cnn::closure_launch(e->closure, e->stream, e->mode == INMSOM_BT_FP32, e->cl_mem[phase], e->inf, tau,
+                                 e->closure_ratio);
+}
+
+void drop_closure(inmsom_bt_ens *e)
+{
+    ibt::cnn::infer_destroy(&e->inf);
+    for (auto &t : e->cl_mem)
+        if (t) {
+            hipFree(t);
+            t = nullptr;
+        }
+    e->closure = nullptr;
 }
 
 void drop_graphs(inmsom_bt_ens *e)
@@ -472,6 +492,7 @@ int inmsom_bt_ens_destroy(inmsom_bt_ens *e)
     }
     for (void *q : e->snap)
         if (q) hipFree(q);
+    drop_closure(e);
     for (void *q : {(void *)e->table[0], (void *)e->table[1], (void *)e->item_member,
                     (void *)e->item_local, (void *)e->rows, (void *)e->cnt})
         if (q) hipFree(q);
@@ -686,6 +707,63 @@ int inmsom_bt_ens_volume(inmsom_bt_ens *e, int k, double *vol)
     double t = 0.0;
     for (double x : h) t += x;
     *vol = t;
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_ens_set_closure(inmsom_bt_ens *e, inmsom_bt_cnn *net, double ratio)
+{
+    if (!e) return fail("null ensemble", INMSOM_BT_EINVAL);
+    if (net && !(ratio > 0.0)) return fail("ratio must be positive", INMSOM_BT_EINVAL);
+    EHC(hipSetDevice(e->device));
+    EHC(hipStreamSynchronize(e->stream));
+    drop_graphs(e);
+    drop_closure(e);
+    if (!net) return INMSOM_BT_OK;
+    if (ibt::cnn::net_device(net) != e->device) return fail("network and ensemble on different devices",
+                                                            INMSOM_BT_EINVAL);
+    if (e->mode == INMSOM_BT_FP64_EXACT)
+        return fail("the closure runs with the FP64 and FP32 modes", INMSOM_BT_EINVAL);
+    const int nm = (int)e->mem.size();
+    std::vector<int> nx(nm), ny(nm);
+    for (int k = 0; k < nm; ++k) {
+        nx[k] = e->mem[k].nx;
+        ny[k] = e->mem[k].ny;
+    }
+    int rc = ibt::cnn::infer_create(e->device, nm, nx.data(), ny.data(), &e->inf);
+    if (rc) {
+        drop_closure(e);
+        return rc;
+    }
+    long long pix = 0, row = 0;
+    for (int b = 0; b < 2; ++b) {
+        std::vector<ibt::cnn::ClosureMember> t(nm);
+        pix = row = 0;
+        for (int k = 0; k < nm; ++k) {
+            const EnsMember &mb = e->mem[k];
+            t[k] = ibt::cnn::ClosureMember{mb.nx, mb.ny, pix, row, mb.lvl[b][0], mb.lvl[b][1], mb.lvl[b][2],
+                                           mb.H, mb.mb, mb.d, mb.nu};
+            pix += (long long)mb.nx * mb.ny;
+            row += mb.ny;
+        }
+        if ((rc = upload_vec(e, &e->cl_mem[b], t))) {
+            drop_closure(e);
+            return rc;
+        }
+    }
+    e->closure = net;
+    e->closure_ratio = ratio;
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_ens_closure_stats(inmsom_bt_ens *e, int *nfallback, double *norms)
+{
+    if (!e) return fail("null ensemble", INMSOM_BT_EINVAL);
+    if (!e->closure) return fail("no closure attached", INMSOM_BT_ESTATE);
+    EHC(hipSetDevice(e->device));
+    EHC(hipStreamSynchronize(e->stream));
+    const size_t nm = e->mem.size();
+    if (nfallback) EHC(hipMemcpy(nfallback, e->inf.nfall, nm * sizeof(int), hipMemcpyDeviceToHost));
+    if (norms) EHC(hipMemcpy(norms, e->inf.norms, nm * 2 * sizeof(double), hipMemcpyDeviceToHost));
     return INMSOM_BT_OK;
 }
 

@@ -18,6 +18,7 @@ int fused_flush(inmsom_bt_ctx *c);
 void fused_release(inmsom_bt_ctx *c);
 int fused_range_check(inmsom_bt_ctx *c, long long *bad);
 int fused_volume(inmsom_bt_ctx *c, double *vol);
+int fused_coarsen(inmsom_bt_ctx *c, int factor, double *u, double *v, double *ssh);
 int fused_set_shape(inmsom_bt_ctx *c, int chunk);
 
 }  // namespace ibt

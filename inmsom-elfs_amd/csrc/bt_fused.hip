@@ -500,6 +500,73 @@ int fused_range_check(inmsom_bt_ctx *c, long long *bad)
     return INMSOM_BT_OK;
 }
 
+// Block average of the state onto a grid `f` times coarser (interior
+// blocks of f x f cells; two halo cells per side on both grids):
+//   ssh: mean over the block's wet cells,
+//   u:   mean of the f fine U faces on the block's east edge,
+//   v:   mean of the f fine V faces on the block's north edge.
+template <class T>
+__global__ void k_coarsen(int nx, int ny, int f, int nxc, int nyc, const T *__restrict__ u,
+                          const T *__restrict__ v, const T *__restrict__ s, const float *__restrict__ lu,
+                          double *__restrict__ uc, double *__restrict__ vc, double *__restrict__ sc)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nxc * nyc) return;
+    const int mc = i % nxc, nc = i / nxc;
+    double us = 0.0, vs = 0.0, ss = 0.0, ws = 0.0;
+    if (mc >= 2 && mc < nxc - 2 && nc >= 2 && nc < nyc - 2) {
+        const int m0 = 2 + (mc - 2) * f, n0 = 2 + (nc - 2) * f;
+        for (int b = 0; b < f; ++b)
+            for (int a = 0; a < f; ++a) {
+                const size_t k = (size_t)(n0 + b) * nx + (m0 + a);
+                ss += (double)s[k] * lu[k];
+                ws += lu[k];
+            }
+        for (int b = 0; b < f; ++b) us += (double)u[(size_t)(n0 + b) * nx + (m0 + f - 1)];
+        for (int a = 0; a < f; ++a) vs += (double)v[(size_t)(n0 + f - 1) * nx + (m0 + a)];
+        us /= f;
+        vs /= f;
+        ss = ws > 0.0 ? ss / ws : 0.0;
+    }
+    uc[i] = us;
+    vc[i] = vs;
+    sc[i] = ss;
+}
+
+int fused_coarsen(inmsom_bt_ctx *c, int f, double *u, double *v, double *ssh)
+{
+    if (f < 1 || (c->nx - 4) % f || (c->ny - 4) % f) {
+        set_error("the interior must divide into factor x factor blocks");
+        return INMSOM_BT_EINVAL;
+    }
+    const int nxc = (c->nx - 4) / f + 4, nyc = (c->ny - 4) / f + 4;
+    const size_t nc = (size_t)nxc * nyc;
+    double *out = c->w[6];  // working arrays of barotropic_dynamics: free between calls
+    double *uo = out, *vo = c->w[7], *so = c->w[8];
+    if (c->state_on_fused && falloc(c)->mode == INMSOM_BT_FP32) {
+        FusedAlloc *a = falloc(c);
+        hipLaunchKernelGGL(k_coarsen<float>, dim3(nblk(nc)), dim3(256), 0, c->stream, c->nx, c->ny, f, nxc, nyc,
+                           (const float *)a->lvl[c->fz_cur][0], (const float *)a->lvl[c->fz_cur][1],
+                           (const float *)a->lvl[c->fz_cur][2], c->mask[0], uo, vo, so);
+    } else {
+        const double *su = c->st[0], *sv = c->st[2], *ss = c->st[4];
+        if (c->state_on_fused) {
+            FusedAlloc *a = falloc(c);
+            su = (const double *)a->lvl[c->fz_cur][0];
+            sv = (const double *)a->lvl[c->fz_cur][1];
+            ss = (const double *)a->lvl[c->fz_cur][2];
+        }
+        hipLaunchKernelGGL(k_coarsen<double>, dim3(nblk(nc)), dim3(256), 0, c->stream, c->nx, c->ny, f, nxc, nyc,
+                           su, sv, ss, c->mask[0], uo, vo, so);
+    }
+    HC(hipGetLastError());
+    HC(hipMemcpyAsync(u, uo, nc * 8, hipMemcpyDeviceToHost, c->stream));
+    HC(hipMemcpyAsync(v, vo, nc * 8, hipMemcpyDeviceToHost, c->stream));
+    HC(hipMemcpyAsync(ssh, so, nc * 8, hipMemcpyDeviceToHost, c->stream));
+    HC(hipStreamSynchronize(c->stream));
+    return INMSOM_BT_OK;
+}
+
 int fused_volume(inmsom_bt_ctx *c, double *vol)
 {
     const int rows = c->ny - 4;

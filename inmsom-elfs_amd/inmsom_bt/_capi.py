@@ -18,6 +18,7 @@ MODES = {"fp64_exact": FP64_EXACT, "fp64": FP64, "fp32": FP32}
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int)
+_u8p = C.POINTER(C.c_uint8)
 
 
 class Grid(C.Structure):
@@ -58,6 +59,7 @@ SIGNATURES = {
     "inmsom_bt_range_check": ([C.c_void_p, C.POINTER(C.c_longlong)], C.c_int),
     "inmsom_bt_volume": ([C.c_void_p, _dp], C.c_int),
     "inmsom_bt_synchronize": ([C.c_void_p], C.c_int),
+    "inmsom_bt_coarsen_state": ([C.c_void_p, C.c_int, _dp, _dp, _dp], C.c_int),
     "inmsom_bt_ens_create": ([C.c_int, C.POINTER(Grid), C.POINTER(Forcing), C.c_int, C.c_int,
                               C.POINTER(C.c_void_p)], C.c_int),
     "inmsom_bt_ens_destroy": ([C.c_void_p], C.c_int),
@@ -72,6 +74,24 @@ SIGNATURES = {
     "inmsom_bt_ens_restore": ([C.c_void_p], C.c_int),
     "inmsom_bt_ens_range_check": ([C.c_void_p, C.POINTER(C.c_longlong)], C.c_int),
     "inmsom_bt_ens_volume": ([C.c_void_p, C.c_int, _dp], C.c_int),
+    "inmsom_bt_cnn_create": ([C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
+    "inmsom_bt_cnn_destroy": ([C.c_void_p], C.c_int),
+    "inmsom_bt_cnn_num_params": ([C.c_void_p, C.POINTER(C.c_longlong)], C.c_int),
+    "inmsom_bt_cnn_set_norm": ([C.c_void_p, _fp, _fp, _fp], C.c_int),
+    "inmsom_bt_cnn_set_params": ([C.c_void_p, _fp], C.c_int),
+    "inmsom_bt_cnn_get_params": ([C.c_void_p, _fp], C.c_int),
+    "inmsom_bt_cnn_get_grad": ([C.c_void_p, _fp], C.c_int),
+    "inmsom_bt_cnn_set_batch": ([C.c_void_p, _fp, _fp, _u8p], C.c_int),
+    "inmsom_bt_cnn_forward": ([C.c_void_p, _fp], C.c_int),
+    "inmsom_bt_cnn_get_activation": ([C.c_void_p, C.c_int, _fp], C.c_int),
+    "inmsom_bt_cnn_loss_grad": ([C.c_void_p, _dp], C.c_int),
+    "inmsom_bt_cnn_dataset_alloc": ([C.c_void_p, C.c_int], C.c_int),
+    "inmsom_bt_cnn_dataset_set": ([C.c_void_p, C.c_int, _fp, _fp, _u8p], C.c_int),
+    "inmsom_bt_cnn_train_step": ([C.c_void_p, _ip, C.c_float, _dp], C.c_int),
+    "inmsom_bt_cnn_eval": ([C.c_void_p, _ip, _dp], C.c_int),
+    "inmsom_bt_cnn_synchronize": ([C.c_void_p], C.c_int),
+    "inmsom_bt_ens_set_closure": ([C.c_void_p, C.c_void_p, C.c_double], C.c_int),
+    "inmsom_bt_ens_closure_stats": ([C.c_void_p, _ip, _dp], C.c_int),
 }
 
 

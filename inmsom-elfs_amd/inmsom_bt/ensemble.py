@@ -57,6 +57,7 @@ class Ensemble:
         if getattr(self, "_h", None) is not None and self._h.value:
             lib().inmsom_bt_ens_destroy(self._h)
             self._h = None
+        self.closure = None
 
     def __del__(self):
         try:
@@ -124,6 +125,26 @@ class Ensemble:
         v = C.c_double()
         check(lib().inmsom_bt_ens_volume(self._h, k, C.byref(v)), "inmsom_bt_ens_volume")
         return v.value
+
+    # ------------------------------------------------------------ closure
+    def set_closure(self, net, ratio: float = 3.0):
+        """Attach a learned closure (a ``ClosureNet``; None detaches): after
+        every outer step the network's tendency is added to each member's u
+        and v, except for members whose closure-tendency norm exceeds
+        ``ratio`` times their plain viscous tendency (plain step instead)."""
+        h = None if net is None else net._h
+        check(lib().inmsom_bt_ens_set_closure(self._h, h, float(ratio)), "inmsom_bt_ens_set_closure")
+        self.closure = net  # the network must outlive the attachment
+
+    def closure_stats(self):
+        """(fallback step counts [M], last-step norms [M, 2] = closure, viscous)."""
+        M = len(self.basins)
+        nf = np.zeros(M, dtype=np.int32)
+        nr = np.zeros((M, 2), dtype=np.float64)
+        check(lib().inmsom_bt_ens_closure_stats(self._h, nf.ctypes.data_as(_capi._ip),
+                                                 nr.ctypes.data_as(_capi._dp)),
+              "inmsom_bt_ens_closure_stats")
+        return nf, nr
 
     # ------------------------------------------------------------ autotuner
     def signature(self) -> str:

@@ -200,6 +200,15 @@ class BarotropicModel:
         check(lib().inmsom_bt_synchronize(self._h), "inmsom_bt_synchronize")
 
     # ------------------------------------------------------------ diagnostics
+    def coarsen(self, factor: int = 4) -> dict:
+        """Block average of the current state onto a grid ``factor`` times
+        coarser: {"u", "v", "ssh"} arrays of ((ny-4)/f+4, (nx-4)/f+4)."""
+        nxc, nyc = (self.nx - 4) // factor + 4, (self.ny - 4) // factor + 4
+        out = {k: np.zeros((nyc, nxc)) for k in ("u", "v", "ssh")}
+        check(lib().inmsom_bt_coarsen_state(self._h, factor, _d(out["u"]), _d(out["v"]), _d(out["ssh"])),
+              "inmsom_bt_coarsen_state")
+        return out
+
     def volume(self) -> float:
         v = C.c_double()
         check(lib().inmsom_bt_volume(self._h, C.byref(v)), "inmsom_bt_volume")

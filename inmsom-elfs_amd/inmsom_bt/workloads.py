@@ -28,11 +28,12 @@ SINGLE = {
 
 
 def ensemble(nmembers: int = 64, seed: int = 7, size_lo=(256, 256), size_hi=(1024, 768),
-             sigma: float = 8.0) -> list:
+             sigma: float = 8.0, d: float = 1000.0, dt: float = DT) -> list:
     """Independent basins of uneven size: interior sizes uniform in
     [size_lo, size_hi] (both axes), each with its own seeded bathymetry,
     wind amplitude 0.1 N/m^2 x U(0.8, 1.2) and 1-3 surface bumps of
-    0.2-0.5 m (radius 20 km) at random interior positions."""
+    0.2-0.5 m (radius 20 km) at random interior positions; grid step d,
+    outer step dt."""
     rng = np.random.default_rng(seed)
     out = []
     for k in range(nmembers):
@@ -42,10 +43,21 @@ def ensemble(nmembers: int = 64, seed: int = 7, size_lo=(256, 256), size_hi=(102
         nb = int(rng.integers(1, 4))
         bumps = [(float(rng.uniform(0.2, 0.5)), float(rng.uniform(0.2, 0.8)) * nxi,
                   float(rng.uniform(0.2, 0.8)) * nyi, 20.0e3) for _ in range(nb)]
-        out.append(make_basin(nxi, nyi, seed=int(rng.integers(1 << 30)), sigma=sigma, hmax=200.0,
+        out.append(make_basin(nxi, nyi, seed=int(rng.integers(1 << 30)), d=d, sigma=sigma, hmax=200.0,
                               hland=10.0, f0=1.0e-4, beta=2.0e-11, tau0=tau0, nu=10.0, g=9.81,
-                              dt=DT, bumps=bumps))
+                              dt=dt, bumps=bumps))
     return out
+
+
+# 4 km members for the learned closure's ensemble: the 1 km generator's
+# 8 km bathymetry smoothing is 2 cells here, outer step 40 s.
+COARSE_FACTOR = 4
+COARSE_DT = DT * COARSE_FACTOR
+
+
+def coarse_ensemble(nmembers: int = 128, seed: int = 11, size_lo=(128, 128), size_hi=(512, 384)) -> list:
+    return ensemble(nmembers, seed=seed, size_lo=size_lo, size_hi=size_hi, sigma=8.0 / COARSE_FACTOR,
+                    d=1000.0 * COARSE_FACTOR, dt=COARSE_DT)
 
 
 def single_basin(name: str = "cfg1", seed: int = 2024) -> Basin:

@@ -1,0 +1,220 @@
+"""The closure network's GPU kernels against the CPU oracle.
+
+Tolerances: the device and the oracle round to BF16 at the same points
+(inputs, weights, hidden activations, back-propagated tendencies) but
+accumulate in different orders (FP32 matrix cores vs float64), so single
+BF16 roundings can flip.  Forward outputs agree to relative L2 <= 1e-2,
+each layer's gradient to <= 2e-2, the Adam update to float32 rounding."""
+import numpy as np
+import pytest
+
+import cnn_oracle as co
+from _util import rel_l2
+from inmsom_bt.closure import ClosureNet, features
+from inmsom_bt.ensemble import Ensemble
+from inmsom_bt.cases import randomize_state
+from inmsom_bt.workloads import ensemble
+
+pytestmark = pytest.mark.gpu
+
+FWD_TOL = 1e-2
+GRAD_TOL = 2e-2
+
+
+def _batch(seed, b, h, w):
+    rng = np.random.default_rng(seed)
+    x = np.zeros((b, h, w, 8), dtype=np.float32)
+    x[..., :4] = rng.standard_normal((b, h, w, 4))
+    tgt = rng.standard_normal((b, h, w, 2)).astype(np.float32)
+    mask = rng.integers(0, 4, size=(b, h, w)).astype(np.uint8)
+    return x, tgt, mask
+
+
+def _params(seed):
+    p = co.init_params(seed, out_scale=0.5)
+    rng = np.random.default_rng(seed + 100)
+    for woff, boff, kp in co.layout():
+        p[boff:boff + 32] = rng.standard_normal(32).astype(np.float32) * 0.1
+    return p
+
+
+# shapes: one tile, ragged tiles (width not a multiple of 32), several images
+SHAPES = [(1, 8, 32), (2, 13, 45), (3, 20, 70)]
+
+
+@pytest.mark.parametrize("b,h,w", SHAPES)
+def test_forward_matches_oracle(b, h, w):
+    print(f"cnn forward {b}x{h}x{w}", flush=True)
+    p = _params(1)
+    x, _, _ = _batch(2, b, h, w)
+    with ClosureNet(b, h, w) as net:
+        net.set_params(p)
+        net.set_batch(x)
+        got = net.forward()
+        acts = [net.activation(l) for l in range(3)]
+    want, wacts = co.forward(p, x, quant=True)
+    assert rel_l2(got, want) <= FWD_TOL, rel_l2(got, want)
+    for l in range(3):
+        assert rel_l2(acts[l], wacts[l + 1]) <= FWD_TOL, (l, rel_l2(acts[l], wacts[l + 1]))
+
+
+@pytest.mark.parametrize("b,h,w", SHAPES)
+def test_gradient_matches_oracle(b, h, w):
+    print(f"cnn gradient {b}x{h}x{w}", flush=True)
+    p = _params(3)
+    x, tgt, mask = _batch(4, b, h, w)
+    with ClosureNet(b, h, w) as net:
+        net.set_params(p)
+        net.set_batch(x, tgt, mask)
+        loss = net.loss_grad()
+        g = net.grad()
+    wl, wg = co.backward(p, x, tgt, mask, quant=True)
+    assert abs(loss - wl) <= 1e-2 * abs(wl), (loss, wl)
+    for l, (woff, boff, kp) in enumerate(co.layout()):
+        gw, ww = g[woff:woff + 32 * kp], wg[woff:woff + 32 * kp]
+        gb, wb = g[boff:boff + 32], wg[boff:boff + 32]
+        assert rel_l2(gw, ww) <= GRAD_TOL, (l, "W", rel_l2(gw, ww))
+        assert rel_l2(gb, wb) <= GRAD_TOL, (l, "b", rel_l2(gb, wb))
+        # padding and unused outputs get exactly zero
+        full = gw.reshape(32, kp)
+        assert not full[:, 9 * co.CIN[l]:].any()
+        if l == 3:
+            assert not full[2:].any() and not gb[2:].any()
+
+
+def test_adam_step_matches_oracle():
+    b, h, w = 2, 16, 40
+    p = _params(5)
+    x, tgt, mask = _batch(6, 3, h, w)
+    with ClosureNet(b, h, w) as net:
+        net.set_params(p)
+        net.dataset_alloc(3)
+        for i in range(3):
+            net.dataset_set(i, x[i], tgt[i], mask[i])
+        pk = p.copy()
+        m = np.zeros_like(p)
+        v = np.zeros_like(p)
+        for t, idx in enumerate(([0, 2], [1, 0], [2, 1]), start=1):
+            loss = net.train_step(idx, lr=1e-3)
+            g = net.grad()
+            got = net.params()
+            want, m, v = co.adam(pk, g, m, v, t, lr=1e-3)
+            np.testing.assert_allclose(got, want, rtol=2e-6, atol=1e-9)
+            wl, wg = co.backward(pk, x[idx], tgt[idx], mask[idx], quant=True)
+            assert abs(loss - wl) <= 1e-2 * abs(wl)
+            assert rel_l2(g, wg) <= GRAD_TOL
+            pk = got
+
+
+def test_training_reduces_loss():
+    """A student network learns a teacher network's outputs: 80 Adam steps
+    at lr 1e-3 reduce the loss at least fourfold."""
+    b, h, w, n = 4, 24, 40, 16
+    rng = np.random.default_rng(8)
+    x = np.zeros((n, h, w, 8), dtype=np.float32)
+    x[..., :4] = rng.standard_normal((n, h, w, 4))
+    teacher = co.init_params(9, out_scale=1.0)
+    tgt = co.forward(teacher, x, quant=False)[0].astype(np.float32)
+    mask = np.full((n, h, w), 3, dtype=np.uint8)
+    with ClosureNet(b, h, w) as net:
+        net.set_params(co.init_params(10, out_scale=0.1))
+        net.dataset_alloc(n)
+        for i in range(n):
+            net.dataset_set(i, x[i], tgt[i], mask[i])
+        losses = []
+        for it in range(80):
+            idx = rng.choice(n, size=b, replace=False)
+            losses.append(net.train_step(idx, lr=1e-3))
+    print(f"loss {np.mean(losses[:4]):.4f} -> {np.mean(losses[-4:]):.4f}", flush=True)
+    assert np.mean(losses[-4:]) * 4 <= np.mean(losses[:4])
+
+
+def _lcu_lcv(bn):
+    wet = bn.wet
+    lcu = np.zeros_like(wet)
+    lcv = np.zeros_like(wet)
+    lcu[1:-2, 1:-2] = wet[1:-2, 1:-2] & wet[1:-2, 2:-1]
+    lcv[1:-2, 1:-2] = wet[1:-2, 1:-2] & wet[2:-1, 1:-2]
+    return lcu, lcv
+
+
+def _closure_members():
+    basins = ensemble(3, seed=11, size_lo=(40, 30), size_hi=(90, 70), sigma=5.0)
+    return [randomize_state(bn, 20 + k) for k, bn in enumerate(basins)]
+
+
+NORM = dict(shift=[0.0, 0.0, 0.0, 100.0], scale=[5.0, 5.0, 3.0, 0.01], out_scale=[1e-5, 1e-5])
+
+
+def test_ensemble_closure_matches_oracle():
+    """One outer step with the closure equals the plain step plus dt times
+    the oracle network's tendency on U/V points; ssh is untouched."""
+    basins = _closure_members()
+    p = co.init_params(12, out_scale=0.3)
+    dt = basins[0].dt
+    with Ensemble(basins, mode="fp64") as plain, Ensemble(basins, mode="fp64") as cl, \
+            ClosureNet(1, 8, 32) as net:
+        net.set_params(p)
+        net.set_norm(**NORM)
+        cl.set_closure(net, ratio=1e30)
+        plain.run(1, nstep=2)
+        cl.run(1, nstep=2)
+        nfall, norms = cl.closure_stats()
+        for k, bn in enumerate(basins):
+            a, b = cl.get_state(k), plain.get_state(k)
+            for f in ("ssh", "sshp", "up", "vp"):
+                assert np.array_equal(a[f], b[f]), f
+            x = features(b["u"], b["v"], b["ssh"], bn.hhq_rest, NORM["shift"], NORM["scale"])
+            pred = co.forward(p, x[None], quant=True)[0][0]
+            lcu, lcv = _lcu_lcv(bn)
+            du = dt * NORM["out_scale"][0] * pred[..., 0] * lcu
+            dv = dt * NORM["out_scale"][1] * pred[..., 1] * lcv
+            assert rel_l2(a["u"] - b["u"], du) <= FWD_TOL, rel_l2(a["u"] - b["u"], du)
+            assert rel_l2(a["v"] - b["v"], dv) <= FWD_TOL, rel_l2(a["v"] - b["v"], dv)
+            assert not (a["u"] - b["u"])[~lcu].any() and not (a["v"] - b["v"])[~lcv].any()
+            # reported norms: closure tendency and nu * Laplacian of the plain step
+            cn = np.sqrt((du[lcu] ** 2).sum() + (dv[lcv] ** 2).sum()) / dt
+            nu = float(bn.mu.max())
+
+            def lap(f):
+                out = np.zeros_like(f)
+                out[1:-1, 1:-1] = (f[1:-1, 2:] + f[1:-1, :-2] + f[2:, 1:-1] + f[:-2, 1:-1]
+                                   - 4 * f[1:-1, 1:-1])
+                return nu * out / (bn.dx[2, 2] ** 2)
+            vn = np.sqrt((lap(b["u"])[lcu] ** 2).sum() + (lap(b["v"])[lcv] ** 2).sum())
+            assert abs(norms[k, 0] - cn) <= FWD_TOL * cn
+            assert abs(norms[k, 1] - vn) <= 1e-10 * vn
+        assert not nfall.any()
+
+
+def test_ensemble_closure_fallback_and_graph_replays():
+    """ratio ~ 0 makes every member fall back (bitwise the plain step); graph
+    replays with the closure equal direct launches, replay after replay."""
+    basins = _closure_members()
+    p = co.init_params(13, out_scale=0.3)
+    with Ensemble(basins, mode="fp32") as plain, Ensemble(basins, mode="fp32") as cl, \
+            ClosureNet(1, 8, 32) as net:
+        net.set_params(p)
+        net.set_norm(**NORM)
+        cl.set_closure(net, ratio=1e-30)
+        plain.run(3, nstep=2)
+        cl.run(3, nstep=2)
+        nfall, _ = cl.closure_stats()
+        assert (nfall == 3).all()
+        for k in range(len(basins)):
+            a, b = cl.get_state(k), plain.get_state(k)
+            for f in a:
+                assert np.array_equal(a[f], b[f]), (k, f)
+    with Ensemble(basins, mode="fp32") as g, Ensemble(basins, mode="fp32") as d, \
+            ClosureNet(1, 8, 32) as net:
+        net.set_params(p)
+        net.set_norm(**NORM)
+        g.set_closure(net, ratio=1e30)
+        d.set_closure(net, ratio=1e30)
+        for rep in range(3):
+            g.run(1, nstep=2, graph=True)
+            d.run(1, nstep=2, graph=False)
+            for k in range(len(basins)):
+                a, b = g.get_state(k), d.get_state(k)
+                for f in a:
+                    assert np.array_equal(a[f], b[f]), (rep, k, f)
