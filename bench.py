@@ -149,6 +149,91 @@ def ensemble_leg(steps, warmup, nmembers):
     return res
 
 
+CLOSURE_PARAMS = os.path.join(ROOT, "inmsom-elfs_amd", "inmsom_bt", "data", "closure_cfg1.npz")
+
+
+def learned_leg(fine, rollout_steps, members, member_steps, ratio=3.0):
+    """cfg3/cfg4: the trained closure (parameters trained on the device by
+    inmsom_bt.closure_experiment, committed) on the 4x-coarsened basin:
+    throughput in simulated time vs the 1 km FP64 run, SSH error vs the
+    coarsened 1 km run with and without the closure over a held-out
+    period; then the closure fused into a 128-member FP32 coarse ensemble."""
+    from inmsom_bt.closure import ClosureNet
+    from inmsom_bt.closure_experiment import load_params
+    from inmsom_bt.ensemble import Ensemble
+    from inmsom_bt.learned import FineTruth, rel, rollout, snapshots
+    from inmsom_bt.workloads import coarse_ensemble
+
+    if not os.path.exists(CLOSURE_PARAMS):
+        log(f"learned leg skipped: {CLOSURE_PARAMS} missing")
+        return None
+    params, nm, meta = load_params(CLOSURE_PARAMS)
+    truth = FineTruth(fine)
+    cb = truth.coarse
+    net = ClosureNet(1, cb.ny, cb.nx)
+    net.set_params(params)
+    net.set_norm(nm.shift, nm.scale, nm.out_scale)
+    out = {"params": os.path.relpath(CLOSURE_PARAMS, ROOT), "trained": meta}
+    # held-out start: after the training snapshots
+    t_start = int(meta["t_start"])
+    t0 = time.perf_counter()
+    start = snapshots(truth, [t_start])[0]
+    truth.advance_to(t_start + truth.per_step * rollout_steps)
+    tr = {rollout_steps: truth.coarse_state()}
+    log(f"learned: fine truth to step {truth.step} in {time.perf_counter() - t0:.1f} s")
+    runs = {}
+    for name, mode, use in (("plain_fp64", "fp64", False), ("learned_fp64", "fp64", True),
+                            ("plain_fp32", "fp32", False), ("learned_fp32", "fp32", True)):
+        r = rollout(cb, start, rollout_steps, tr, net if use else None, ratio, mode)
+        runs[name] = {"ssh_rel_l2": r["ssh"][-1], "u_rel_l2": r["u"][-1], "v_rel_l2": r["v"][-1],
+                      "range_check_bad_cells": r["bad"]}
+        if use:
+            runs[name]["fallback_steps"] = r["fallback_steps"]
+    tm = {}
+    with truth.model as m:
+        ms = m.run_timed(1000, "fp64", nstep=NSTEP_BAROTROP)
+    fine_sim = fine.dt * 1000 / (ms * 1e-3)
+    tm["fine_fp64"] = {"us_per_step": ms, "sim_s_per_s": fine_sim}
+    for name, mode, use in (("plain_fp32", "fp32", False), ("learned_fp32", "fp32", True),
+                            ("learned_fp64", "fp64", True)):
+        r = rollout(cb, start, 1000, None, net if use else None, ratio, mode, timed=True)
+        sim = cb.dt * 1000 / (r["ms"] * 1e-3)
+        tm[name] = {"us_per_step": r["ms"], "sim_s_per_s": sim, "speedup_vs_fine_fp64": sim / fine_sim,
+                    "wet_cell_steps_per_s": cb.n_wet * 1000 * NSTEP_BAROTROP / (r["ms"] * 1e-3)}
+    out.update({"workload": f"{cb.nx - 4}x{cb.ny - 4} at 4 km (4x block average of the 1 km basin), "
+                            f"dt=40 s, nstep_barotrop=2; {rollout_steps} coarse steps from fine step "
+                            f"{t_start} scored against the coarsened FP64 1 km run",
+                "error_vs_coarsened_truth": runs, "throughput": tm})
+    for k, v in runs.items():
+        log(f"learned {k}: ssh rel err {v['ssh_rel_l2']:.4f}")
+    log(f"learned fp32: {tm['learned_fp32']['us_per_step']:.1f} us/step, "
+        f"{tm['learned_fp32']['speedup_vs_fine_fp64']:.1f}x the 1 km FP64 run in simulated time")
+    truth.close()
+    if members > 0:
+        mem = coarse_ensemble(members)
+        with Ensemble(mem, mode="fp32") as ens, Ensemble(mem, mode="fp64") as ref:
+            nmem, wet, cells = ens.size()
+            ens.set_closure(net, ratio)
+            ens.run(5, NSTEP_BAROTROP)
+            ens.reset()
+            ms = ens.run_timed(member_steps, NSTEP_BAROTROP)
+            ref.run(member_steps, NSTEP_BAROTROP)
+            nf, _ = ens.closure_stats()
+            errs = [rel(ens.get_state(k)["ssh"], ref.get_state(k)["ssh"], b.wet) for k, b in enumerate(mem)]
+            bad = ens.range_check()
+        out["ensemble_fp32_closure"] = {
+            "members": nmem, "wet_cells": wet, "array_cells": cells, "steps": member_steps,
+            "ms_per_step": ms / member_steps, "value": wet * member_steps * NSTEP_BAROTROP / (ms * 1e-3),
+            "unit": "wet-cell steps/s", "ssh_rel_l2_vs_plain_fp64": {"median": float(np.median(errs)),
+                                                                    "max": float(np.max(errs))},
+            "fallback": {"members": int((nf > 0).sum()), "member_steps": int(nf.sum())},
+            "range_check_bad_cells": bad}
+        log(f"closure ensemble: {nmem} members, {ms / member_steps:.2f} ms/step, "
+            f"{out['ensemble_fp32_closure']['value']:.3e} wet-cell steps/s")
+    net.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +246,10 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=5000,
                     help="outer steps of the 64-member FP32 ensemble (0 = skip)")
     ap.add_argument("--ensemble-members", type=int, default=64)
+    ap.add_argument("--learned-steps", type=int, default=2000,
+                    help="coarse 40 s steps of the learned-closure rollout (0 = skip)")
+    ap.add_argument("--closure-members", type=int, default=128)
+    ap.add_argument("--closure-member-steps", type=int, default=1000)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -220,6 +309,8 @@ def main():
     if args.ensemble_steps > 0:
         line["ensemble_fp32"] = ensemble_leg(args.ensemble_steps, min(args.warmup, 20),
                                              args.ensemble_members)
+    if args.learned_steps > 0 and args.workload == "cfg1":
+        line["learned"] = learned_leg(b, args.learned_steps, args.closure_members, args.closure_member_steps)
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         line["cpu_baseline"] = cpu_baseline(b, args.cpu_steps)

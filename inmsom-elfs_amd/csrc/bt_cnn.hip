@@ -43,54 +43,102 @@ struct Layer {
 
 enum { EPI_RELU_BF16 = 0, EPI_OUT_F32 = 1, EPI_MASK_BF16 = 2 };
 
+// Rows of one block tile (RW rows per wave, 4 waves) and its column width.
+constexpr int RW = 2;
+constexpr int TR = 4 * RW;
+constexpr int TW = 32;
+
+// LDS pixel stride (bf16 elements) of a staged input tile: 32-channel
+// pixels are padded to 80 bytes so that the 16-byte reads of 8 consecutive
+// lanes (8 consecutive pixels) fall on distinct banks.
+template <int CIN>
+constexpr int lds_stride() { return CIN == 8 ? 8 : CIN + 8; }
+
+// One 3x3 convolution layer over a batch of images, NHWC BF16 in.  A block
+// (4 waves) owns a tile of TR rows x 32 columns of one image: it stages the
+// (TR + 2) x 34 input halo tile in LDS, and each wave multiplies RW rows of
+// 32 pixels by the 32 output channels with its weight fragments held in
+// registers for the whole (persistent) launch.
+//   forward:   out = relu(z + b) in BF16 (EPI_RELU_BF16) or z + b, channels
+//              0..1 in FP32 (EPI_OUT_F32); w = W[co][kpad]
+//   backward:  NEG taps, w = W^T[ci][t*32 + co] (kpad 288), out = dx masked
+//              by act_prev > 0 in BF16 (EPI_MASK_BF16)
 template <int CIN, bool NEG, int EPI>
 __global__ void __launch_bounds__(256)
     k_conv(const Img *__restrict__ imgs, const int *__restrict__ tile_img,
            const int *__restrict__ tile_local, int ntiles, const bf16 *__restrict__ in,
-           const bf16 *__restrict__ w, int kpad, const float *__restrict__ bias,
+           const bf16 *__restrict__ w, const float *__restrict__ bias,
            const bf16 *__restrict__ act_prev, void *__restrict__ out)
 {
-    const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (tile >= ntiles) return;  // uniform per wave
-    const int lane = threadIdx.x & 63, i = lane & 31, half = lane >> 5;
-    const Img im = imgs[tile_img[tile]];
-    const int tl = tile_local[tile];
-    const int tx = (im.W + 31) >> 5;
-    const int y = tl / tx, x0 = (tl - y * tx) * 32;
-    const int x = x0 + i;
-    f32x16 acc = {};
-    for (int s = 0; s < kpad / 16; ++s) {
-        const int k0 = 16 * s + 8 * half;
-        const int t = k0 / CIN, ci0 = k0 - t * CIN;
-        bf16x8 a = {};
-        if (t < 9) {
+    constexpr int KPAD = NEG ? 9 * CH : (9 * CIN + 15) / 16 * 16;
+    constexpr int KS = KPAD / 16;
+    constexpr int LS = lds_stride<CIN>();
+    constexpr int CPP = CIN / 8;                 // 16-byte chunks per pixel
+    constexpr int NCHUNK = (TR + 2) * (TW + 2) * CPP;
+    __shared__ __attribute__((aligned(16))) bf16 tile_in[(TR + 2) * (TW + 2) * LS];
+
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, i = lane & 31, half = lane >> 5;
+    bf16x8 b[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) b[s] = *reinterpret_cast<const bf16x8 *>(w + (size_t)i * KPAD + 16 * s + 8 * half);
+    const float bco = EPI == EPI_MASK_BF16 ? 0.0f : bias[i];
+
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const Img im = imgs[tile_img[tile]];
+        const int tl = tile_local[tile];
+        const int tx = (im.W + TW - 1) / TW;
+        const int ty = tl / tx;
+        const int y0 = ty * TR, x0 = (tl - ty * tx) * TW;
+        __syncthreads();  // the previous tile's reads are done
+        for (int c = threadIdx.x; c < NCHUNK; c += 256) {
+            const int px = c / CPP, part = c - px * CPP;
+            const int ry = px / (TW + 2), rx = px - ry * (TW + 2);
+            const int y = y0 - 1 + ry, x = x0 - 1 + rx;
+            bf16x8 v = {};
+            if (y >= 0 && y < im.H && x >= 0 && x < im.W)
+                v = *reinterpret_cast<const bf16x8 *>(in + ((im.pix + (long long)y * im.W + x) * CIN + part * 8));
+            *reinterpret_cast<bf16x8 *>(tile_in + px * LS + part * 8) = v;
+        }
+        __syncthreads();
+        f32x16 acc[RW];
+#pragma unroll
+        for (int q = 0; q < RW; ++q) acc[q] = f32x16{};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int k0 = 16 * s + 8 * half;
+            const int t = k0 / CIN, ci0 = k0 - t * CIN;
             int oy = t / 3 - 1, ox = t % 3 - 1;
             if (NEG) {
                 oy = -oy;
                 ox = -ox;
             }
-            const int yy = y + oy, xx = x + ox;
-            if (x < im.W && yy >= 0 && yy < im.H && xx >= 0 && xx < im.W)
-                a = *reinterpret_cast<const bf16x8 *>(in + ((im.pix + (size_t)yy * im.W + xx) * CIN + ci0));
-        }
-        const bf16x8 b = *reinterpret_cast<const bf16x8 *>(w + (size_t)i * kpad + k0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-    }
-    const int co = i;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int px = x0 + 8 * (r >> 2) + 4 * half + (r & 3);
-        if (px >= im.W) continue;
-        const size_t p = im.pix + (size_t)y * im.W + px;
-        float v = acc[r];
-        if (EPI == EPI_RELU_BF16) {
-            v = fmaxf(v + bias[co], 0.0f);
-            static_cast<bf16 *>(out)[p * CH + co] = (bf16)v;
-        } else if (EPI == EPI_OUT_F32) {
-            if (co < NOUT) static_cast<float *>(out)[p * NOUT + co] = v + bias[co];
-        } else {
-            const float g = (float)act_prev[p * CH + co] > 0.0f ? v : 0.0f;
-            static_cast<bf16 *>(out)[p * CH + co] = (bf16)g;
+            for (int q = 0; q < RW; ++q) {
+                const int r = wv * RW + q;
+                bf16x8 a = {};
+                if (t < 9) a = *reinterpret_cast<const bf16x8 *>(tile_in + ((r + 1 + oy) * (TW + 2) + (i + 1 + ox)) * LS + ci0);
+                acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[s], acc[q], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < RW; ++q) {
+            const int y = y0 + wv * RW + q;
+            if (y >= im.H) continue;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int px = x0 + 8 * (r >> 2) + 4 * half + (r & 3);
+                if (px >= im.W) continue;
+                const long long p = im.pix + (long long)y * im.W + px;
+                const float v = acc[q][r];
+                if (EPI == EPI_RELU_BF16) {
+                    static_cast<bf16 *>(out)[p * CH + i] = (bf16)fmaxf(v + bco, 0.0f);
+                } else if (EPI == EPI_OUT_F32) {
+                    if (i < NOUT) static_cast<float *>(out)[p * NOUT + i] = v + bco;
+                } else {
+                    const float g = (float)act_prev[p * CH + i] > 0.0f ? v : 0.0f;
+                    static_cast<bf16 *>(out)[p * CH + i] = (bf16)g;
+                }
+            }
         }
     }
 }
@@ -405,7 +453,7 @@ struct inmsom_bt_cnn {
     Img *imgs = nullptr;
     int *tile_img = nullptr, *tile_local = nullptr;
     int ntiles = 0;
-    int wg_chunk = 8192, wg_nchunks = 0;
+    int wg_chunk = 2048, wg_nchunks = 0;  // pixels per weight-gradient wave
     // device-resident training set
     int nsamples = 0;
     bf16 *ds_x = nullptr;
@@ -457,12 +505,12 @@ void layer_table(Layer *L, size_t *nparam)
     *nparam = off;
 }
 
-// tile tables (one tile = 32 pixels of one row) of a batch of images
+// tile tables (one tile = TR rows x TW columns of one image) of a batch of images
 int build_tiles(const std::vector<Img> &imgs, Img **d_imgs, int **d_ti, int **d_tl, int *ntiles)
 {
     std::vector<int> ti, tl;
     for (size_t k = 0; k < imgs.size(); ++k) {
-        const int n = imgs[k].H * ((imgs[k].W + 31) / 32);
+        const int n = ((imgs[k].H + TR - 1) / TR) * ((imgs[k].W + TW - 1) / TW);
         for (int t = 0; t < n; ++t) {
             ti.push_back((int)k);
             tl.push_back(t);
@@ -481,25 +529,26 @@ int build_tiles(const std::vector<Img> &imgs, Img **d_imgs, int **d_ti, int **d_
 void conv_launch(const inmsom_bt_cnn *c, hipStream_t st, int l, bool bwd, const Img *imgs, const int *ti,
                  const int *tl, int ntiles, const bf16 *in, const bf16 *act_prev, void *out)
 {
-    const dim3 grid((unsigned)((ntiles + 3) / 4)), blk(256);
+    // persistent blocks: a few per CU, each looping over tiles
+    const dim3 grid((unsigned)(ntiles < 2048 ? ntiles : 2048)), blk(256);
     const Layer &Ly = c->L[l];
     const float *bias = c->wm + Ly.boff;
     if (!bwd) {
         const bf16 *w = c->wf + c->wf_off[l];
         if (l == 0)
             hipLaunchKernelGGL((k_conv<CIN0, false, EPI_RELU_BF16>), grid, blk, 0, st, imgs, ti, tl, ntiles, in,
-                               w, Ly.kpad, bias, nullptr, out);
+                               w, bias, nullptr, out);
         else if (l < NL - 1)
             hipLaunchKernelGGL((k_conv<CH, false, EPI_RELU_BF16>), grid, blk, 0, st, imgs, ti, tl, ntiles, in,
-                               w, Ly.kpad, bias, nullptr, out);
+                               w, bias, nullptr, out);
         else
             hipLaunchKernelGGL((k_conv<CH, false, EPI_OUT_F32>), grid, blk, 0, st, imgs, ti, tl, ntiles, in, w,
-                               Ly.kpad, bias, nullptr, out);
+                               bias, nullptr, out);
     } else {
         // data gradient through layer l (l >= 1): weights as [ci][t*32 + co]
         const bf16 *w = c->wt + c->wt_off[l];
         hipLaunchKernelGGL((k_conv<CH, true, EPI_MASK_BF16>), grid, blk, 0, st, imgs, ti, tl, ntiles, in, w,
-                           9 * CH, nullptr, act_prev, out);
+                           nullptr, act_prev, out);
     }
 }
 

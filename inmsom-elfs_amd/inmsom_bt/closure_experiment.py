@@ -157,12 +157,14 @@ def main(argv=None):
                                                      for i in range(0, len(losses), 50)]}
     log(f"training: {args.train_steps} steps in {train_s:.1f} s, loss {res['training']['loss_first']:.4f} -> "
         f"{res['training']['loss_last']:.4f}, validation {vl:.4f} (1.0 = no skill)")
+    # the held-out rollout starts after the last training snapshot
+    t_start = steps[-1] + args.every
     save_params(os.path.join(args.out, "closure_params.npz"), net, nm,
                 {"workload": args.workload, "snapshots": n, "every": args.every, "train": ntrain,
-                 "train_steps": args.train_steps, "lr": args.lr, "batch": args.batch})
+                 "train_steps": args.train_steps, "lr": args.lr, "batch": args.batch, "t_start": t_start,
+                 "val_loss": vl, "seed": args.seed})
 
     # ---- 4: rollout against the coarsened truth
-    t_start = steps[-1] + args.every
     start = snapshots(truth, [t_start])[0]
     marks = list(range(args.score_every, args.rollout + 1, args.score_every))
     tr = {}
