@@ -10,6 +10,9 @@ namespace ibt {
 constexpr int WAVE = 64;
 constexpr int SW = 60;  // output columns per wave
 constexpr int NXCD = 8;
+#ifndef PREFETCH_DEPTH_OVERRIDE
+#define PREFETCH_DEPTH_OVERRIDE 0
+#endif
 
 struct FusedParams {
     int nx, ny, strips, chunk, nchunks, total, per_xcd;
@@ -103,7 +106,12 @@ struct RowAddr {
 
 // One work item: `chunk` output rows of one 60-column strip of one basin.
 // Called by every lane of a wave with wave-uniform arguments.
+// Prefetch depth (rows of loads in flight per wave) per mode, chosen so
+// that the extra registers keep the mode's occupancy.
 template <typename R, bool EXACT>
+constexpr int prefetch_depth() { return PREFETCH_DEPTH_OVERRIDE > 0 ? PREFETCH_DEPTH_OVERRIDE : (sizeof(R) == 4 && !EXACT ? 2 : 1); }
+
+template <typename R, bool EXACT, int PD = prefetch_depth<R, EXACT>()>
 __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, int strip)
 {
     const int lane = threadIdx.x;
@@ -135,8 +143,7 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
     R hun_0 = 0, hun_p1 = 0, stt_0 = 0, stt_p1 = 0, bf_0 = 0, bf_p1 = 0;
     R sts_m1 = 0, sts_0 = 0, vo_m1 = 0, vo_0 = 0;
 
-    // Loads run one iteration ahead of their use (software prefetch): the
-    // nx_* registers hold what the next iteration consumes.
+    // Loads run PD iterations ahead of their use (software prefetch).
     const RowAddr ra{p.nx, p.ny, min(max(m, 1), p.nx)};
     const R *S = static_cast<const R *>(p.s), *Hh = static_cast<const R *>(p.H);
     const R *U = static_cast<const R *>(p.u), *V = static_cast<const R *>(p.v);
@@ -146,37 +153,45 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
     auto rowok = [&](int row) { return colok && row >= 1 && row <= p.ny; };
     auto tab = [&](const double *t, int row) { return t[min(max(row, 1), p.ny) - 1]; };
 
-    R nx_s, nx_H, nx_u, nx_v, nx_rd, nx_fx, nx_fy;
-    double nx_up, nx_vp, nx_sp, nx_hupc, nx_hvpc, nx_f, nx_sx, nx_sy, nx_wf;
-    int nx_mb;
-    auto issue = [&](int n) {  // the loads iteration n consumes
+    // one iteration's prefetched loads
+    struct Pre {
+        R s, H, u, v, rd, fx, fy;
+        double up, vp, sp, hupc, hvpc, f, sx, sy, wf;
+        int mb;
+    };
+    auto issue = [&](int n, Pre &q) {  // the loads iteration n consumes
         const size_t i2 = ra.at(n + 2), i1 = ra.at(n + 1), i0 = ra.at(n);
-        nx_s = S[i2];
-        nx_H = Hh[i2];
-        nx_mb = p.mb[i2];
-        nx_u = U[i1];
-        nx_v = V[i1];
-        nx_up = p.up[i1];
-        nx_vp = p.vp[i1];
-        nx_sp = p.sp[i1];
-        nx_rd = RD[i1];
+        q.s = S[i2];
+        q.H = Hh[i2];
+        q.mb = p.mb[i2];
+        q.u = U[i1];
+        q.v = V[i1];
+        q.up = p.up[i1];
+        q.vp = p.vp[i1];
+        q.sp = p.sp[i1];
+        q.rd = RD[i1];
         // per-cell frozen forcing and carried depths of the output row n:
         // element 0 (one cache line per wave) when they are not used
         const size_t iff = frozen ? i0 : 0;
-        nx_fx = FX[iff];
-        nx_fy = FY[iff];
+        q.fx = FX[iff];
+        q.fy = FY[iff];
         if constexpr (EXACT) {
-            nx_hupc = p.hup_c[iff];
-            nx_hvpc = p.hvp_c[iff];
+            q.hupc = p.hup_c[iff];
+            q.hvpc = p.hvp_c[iff];
         } else {
-            nx_hupc = nx_hvpc = 0.0;
+            q.hupc = q.hvpc = 0.0;
         }
-        nx_f = tab(p.row_f, n);
-        nx_sx = tab(p.row_sx, n);
-        nx_sy = tab(p.row_sy, n);
-        nx_wf = tab(p.row_wf, n + 1);
+        q.f = tab(p.row_f, n);
+        q.sx = tab(p.row_sx, n);
+        q.sy = tab(p.row_sy, n);
+        q.wf = tab(p.row_wf, n + 1);
     };
-    issue(r0 - 3);
+    // PD iterations of prefetch: qa is consumed by the current iteration,
+    // qb (PD = 2) by the next one.  Addresses are clamped, so the loads
+    // issued past the chunk's last row stay inside the arrays.
+    Pre qa, qb;
+    issue(r0 - 3, qa);
+    if constexpr (PD == 2) issue(r0 - 2, qb);
     R f_m1 = 0, f_0 = 0, fx_0 = 0, fy_0 = 0, sx_0 = 0, sy_0 = 0, wf_p1 = 0;
     double hupc_0 = 0, hvpc_0 = 0;
 
@@ -184,25 +199,30 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
         const int k1 = n + 1;
         // ---- take the prefetched rows: surface height, depth and masks two
         //      rows ahead, the rest one row ahead
-        s_p2 = rowok(n + 2) ? nx_s : R(0);
-        H_p2 = rowok(n + 2) ? nx_H : R(0);
-        mb_p2 = rowok(n + 2) ? nx_mb : 0;
-        u_p1 = rowok(k1) ? nx_u : R(0);
-        v_p1 = rowok(k1) ? nx_v : R(0);
-        up_p1 = rowok(k1) ? nx_up : 0.0;
-        vp_p1 = rowok(k1) ? nx_vp : 0.0;
-        sp_p1 = rowok(k1) ? nx_sp : 0.0;
-        rd_p1 = rowok(k1) ? nx_rd : R(0);
-        fx_0 = nx_fx;
-        fy_0 = nx_fy;
-        hupc_0 = nx_hupc;
-        hvpc_0 = nx_hvpc;
+        s_p2 = rowok(n + 2) ? qa.s : R(0);
+        H_p2 = rowok(n + 2) ? qa.H : R(0);
+        mb_p2 = rowok(n + 2) ? qa.mb : 0;
+        u_p1 = rowok(k1) ? qa.u : R(0);
+        v_p1 = rowok(k1) ? qa.v : R(0);
+        up_p1 = rowok(k1) ? qa.up : 0.0;
+        vp_p1 = rowok(k1) ? qa.vp : 0.0;
+        sp_p1 = rowok(k1) ? qa.sp : 0.0;
+        rd_p1 = rowok(k1) ? qa.rd : R(0);
+        fx_0 = qa.fx;
+        fy_0 = qa.fy;
+        hupc_0 = qa.hupc;
+        hvpc_0 = qa.hvpc;
         f_m1 = f_0;
-        f_0 = (n >= 1 && n <= p.ny) ? R(nx_f) : R(0);
-        sx_0 = R(nx_sx);
-        sy_0 = R(nx_sy);
-        wf_p1 = (k1 >= 1 && k1 <= p.ny) ? R(nx_wf) : R(0);
-        if (n + 1 < r1) issue(n + 1);
+        f_0 = (n >= 1 && n <= p.ny) ? R(qa.f) : R(0);
+        sx_0 = R(qa.sx);
+        sy_0 = R(qa.sy);
+        wf_p1 = (k1 >= 1 && k1 <= p.ny) ? R(qa.wf) : R(0);
+        if constexpr (PD == 2) {
+            qa = qb;
+            issue(n + 2, qb);
+        } else {
+            if (n + 1 < r1) issue(n + 1, qa);
+        }
 
         const R lu_0 = bitf<R>(mb_0, MB_LU), lu_p1 = bitf<R>(mb_p1, MB_LU), lu_p2 = bitf<R>(mb_p2, MB_LU);
         const R lu_0r = xr(lu_0), lu_p1r = xr(lu_p1);
