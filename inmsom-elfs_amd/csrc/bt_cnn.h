@@ -51,7 +51,7 @@ struct Infer {
     Img *imgs = nullptr;
     int *tile_img = nullptr, *tile_local = nullptr;
     int *row_member = nullptr, *row_y = nullptr;
-    __bf16 *x = nullptr, *a0 = nullptr, *a1 = nullptr;
+    __bf16 *a0 = nullptr, *a1 = nullptr;
     float *tend = nullptr;
     double *part = nullptr;   // [nrows][2]: closure and viscous squared norms
     double *norms = nullptr;  // [nmem][2]: closure and viscous tendency norms of the last step

@@ -20,6 +20,7 @@
 #include <hip/hip_bf16.h>
 
 #include <cmath>
+#include <type_traits>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -63,13 +64,24 @@ constexpr int lds_stride() { return CIN == 8 ? 8 : CIN + 8; }
 //              0..1 in FP32 (EPI_OUT_F32); w = W[co][kpad]
 //   backward:  NEG taps, w = W^T[ci][t*32 + co] (kpad 288), out = dx masked
 //              by act_prev > 0 in BF16 (EPI_MASK_BF16)
-template <int CIN, bool NEG, int EPI>
+//   FT = 1 / 2 (layer 0 of the closure only): the input tile is formed from
+//              the members' float / double (u, v, eta, depth) arrays and the
+//              normalisation instead of being read from `in`
+template <class T>
+__device__ inline float ldf(const void *p, long long i)
+{
+    return (float)static_cast<const T *>(p)[i];
+}
+
+template <int CIN, bool NEG, int EPI, int FT = 0>
 __global__ void __launch_bounds__(256)
     k_conv(const Img *__restrict__ imgs, const int *__restrict__ tile_img,
            const int *__restrict__ tile_local, int ntiles, const bf16 *__restrict__ in,
            const bf16 *__restrict__ w, const float *__restrict__ bias,
-           const bf16 *__restrict__ act_prev, void *__restrict__ out)
+           const bf16 *__restrict__ act_prev, void *__restrict__ out,
+           const ClosureMember *__restrict__ mem = nullptr, Norm nm = Norm{})
 {
+    static_assert(FT == 0 || CIN == CIN0, "features feed the first layer only");
     constexpr int KPAD = NEG ? 9 * CH : (9 * CIN + 15) / 16 * 16;
     constexpr int KS = KPAD / 16;
     constexpr int LS = lds_stride<CIN>();
@@ -95,8 +107,19 @@ __global__ void __launch_bounds__(256)
             const int ry = px / (TW + 2), rx = px - ry * (TW + 2);
             const int y = y0 - 1 + ry, x = x0 - 1 + rx;
             bf16x8 v = {};
-            if (y >= 0 && y < im.H && x >= 0 && x < im.W)
-                v = *reinterpret_cast<const bf16x8 *>(in + ((im.pix + (long long)y * im.W + x) * CIN + part * 8));
+            if (y >= 0 && y < im.H && x >= 0 && x < im.W) {
+                if constexpr (FT == 0) {
+                    v = *reinterpret_cast<const bf16x8 *>(in + ((im.pix + (long long)y * im.W + x) * CIN + part * 8));
+                } else {
+                    using T = typename std::conditional<FT == 1, float, double>::type;
+                    const ClosureMember &m = mem[tile_img[tile]];
+                    const long long a = (long long)y * m.nx + x;
+                    v[0] = (bf16)((ldf<T>(m.u, a) - nm.shift[0]) * nm.scale[0]);
+                    v[1] = (bf16)((ldf<T>(m.v, a) - nm.shift[1]) * nm.scale[1]);
+                    v[2] = (bf16)((ldf<T>(m.s, a) - nm.shift[2]) * nm.scale[2]);
+                    v[3] = (bf16)((ldf<T>(m.H, a) - nm.shift[3]) * nm.scale[3]);
+                }
+            }
             *reinterpret_cast<bf16x8 *>(tile_in + px * LS + part * 8) = v;
         }
         __syncthreads();
@@ -303,30 +326,9 @@ __global__ void k_pack(int kpad, int cin, const float *__restrict__ wm, bf16 *__
 // ---- closure inference over a set of basins (one block per basin row)
 
 template <class T>
-__device__ inline float ld(const void *p, long long i)
+__device__ inline double ld(const void *p, long long i)
 {
-    return (float)static_cast<const T *>(p)[i];
-}
-
-// inputs of the network: (u, v, eta, depth) shifted and scaled, in BF16
-template <class T>
-__global__ void __launch_bounds__(256)
-    k_features(const ClosureMember *__restrict__ mem, const int *__restrict__ row_member,
-               const int *__restrict__ row_y, Norm nm, bf16 *__restrict__ x)
-{
-    const ClosureMember m = mem[row_member[blockIdx.x]];
-    const int y = row_y[blockIdx.x];
-    for (int i = threadIdx.x; i < m.nx; i += blockDim.x) {
-        const long long a = (long long)y * m.nx + i;
-        const long long p = m.pix + a;
-        bf16x8 f;
-        f[0] = (bf16)((ld<T>(m.u, a) - nm.shift[0]) * nm.scale[0]);
-        f[1] = (bf16)((ld<T>(m.v, a) - nm.shift[1]) * nm.scale[1]);
-        f[2] = (bf16)((ld<T>(m.s, a) - nm.shift[2]) * nm.scale[2]);
-        f[3] = (bf16)((ld<T>(m.H, a) - nm.shift[3]) * nm.scale[3]);
-        f[4] = f[5] = f[6] = f[7] = (bf16)0.0f;
-        *reinterpret_cast<bf16x8 *>(x + p * CIN0) = f;
-    }
+    return (double)static_cast<const T *>(p)[i];
 }
 
 // per-row squared norms of the closure tendency and of the plain viscous
@@ -348,14 +350,14 @@ __global__ void __launch_bounds__(256)
             const uint8_t b = m.mb[a];
             if (b & MB_LCU) {
                 const double c = (double)tend[(m.pix + a) * NOUT + 0] * nm.oscale[0];
-                const double l = k * ((double)ld<T>(m.u, a - 1) + ld<T>(m.u, a + 1) + ld<T>(m.u, a - m.nx) +
+                const double l = k * (ld<T>(m.u, a - 1) + ld<T>(m.u, a + 1) + ld<T>(m.u, a - m.nx) +
                                       ld<T>(m.u, a + m.nx) - 4.0 * ld<T>(m.u, a));
                 sc += c * c;
                 sv += l * l;
             }
             if (b & MB_LCV) {
                 const double c = (double)tend[(m.pix + a) * NOUT + 1] * nm.oscale[1];
-                const double l = k * ((double)ld<T>(m.v, a - 1) + ld<T>(m.v, a + 1) + ld<T>(m.v, a - m.nx) +
+                const double l = k * (ld<T>(m.v, a - 1) + ld<T>(m.v, a + 1) + ld<T>(m.v, a - m.nx) +
                                       ld<T>(m.v, a + m.nx) - 4.0 * ld<T>(m.v, a));
                 sc += c * c;
                 sv += l * l;
@@ -378,13 +380,22 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// per basin: norms and the fallback decision of this step
+// The fallback decision of this step and its application.  Every block of a
+// member reduces the member's row partials in the same fixed order (so all
+// reach the same decision); the block of row 0 records the norms and counts
+// a fallback step.  Applied: u += dt * tendency on U points, v likewise.
+template <class T>
 __global__ void __launch_bounds__(256)
-    k_cl_decide(const ClosureMember *__restrict__ mem, const double *__restrict__ part, double ratio,
-                double *__restrict__ norms, int *__restrict__ active, int *__restrict__ nfall)
+    k_cl_apply(ClosureMember *__restrict__ mem, const int *__restrict__ row_member,
+               const int *__restrict__ row_y, Norm nm, const float *__restrict__ tend,
+               const double *__restrict__ part, double ratio, double dt, double *__restrict__ norms,
+               int *__restrict__ active, int *__restrict__ nfall)
 {
     __shared__ double red[2][256];
-    const ClosureMember m = mem[blockIdx.x];
+    __shared__ int ok_s;
+    const int mi = row_member[blockIdx.x];
+    const ClosureMember m = mem[mi];
+    const int y = row_y[blockIdx.x];
     double sc = 0.0, sv = 0.0;
     for (int r = threadIdx.x; r < m.ny; r += blockDim.x) {
         sc += part[2 * (m.row0 + r) + 0];
@@ -402,24 +413,17 @@ __global__ void __launch_bounds__(256)
     }
     if (threadIdx.x == 0) {
         const double nc = sqrt(red[0][0]), nv = sqrt(red[1][0]);
-        norms[2 * blockIdx.x + 0] = nc;
-        norms[2 * blockIdx.x + 1] = nv;
         const int ok = nc <= ratio * nv ? 1 : 0;
-        active[blockIdx.x] = ok;
-        nfall[blockIdx.x] += 1 - ok;
+        ok_s = ok;
+        if (y == 0) {
+            norms[2 * mi + 0] = nc;
+            norms[2 * mi + 1] = nv;
+            active[mi] = ok;
+            nfall[mi] += 1 - ok;
+        }
     }
-}
-
-template <class T>
-__global__ void __launch_bounds__(256)
-    k_cl_apply(ClosureMember *__restrict__ mem, const int *__restrict__ row_member,
-               const int *__restrict__ row_y, Norm nm, const float *__restrict__ tend,
-               const int *__restrict__ active, double dt)
-{
-    const int mi = row_member[blockIdx.x];
-    if (!active[mi]) return;
-    const ClosureMember m = mem[mi];
-    const int y = row_y[blockIdx.x];
+    __syncthreads();
+    if (!ok_s) return;
     T *u = static_cast<T *>(m.u), *v = static_cast<T *>(m.v);
     for (int i = threadIdx.x; i < m.nx; i += blockDim.x) {
         const long long a = (long long)y * m.nx + i;
@@ -658,7 +662,6 @@ int infer_create(int device, int nmem, const int *nx, const int *ny, Infer *inf)
     CTRY(dmalloc(&inf->row_y, ry.size()));
     CHC(hipMemcpy(inf->row_member, rm.data(), rm.size() * 4, hipMemcpyHostToDevice));
     CHC(hipMemcpy(inf->row_y, ry.data(), ry.size() * 4, hipMemcpyHostToDevice));
-    CTRY(dmalloc(&inf->x, (size_t)pix * CIN0));
     CTRY(dmalloc(&inf->a0, (size_t)pix * CH));
     CTRY(dmalloc(&inf->a1, (size_t)pix * CH));
     CTRY(dmalloc(&inf->tend, (size_t)pix * NOUT));
@@ -672,7 +675,7 @@ int infer_create(int device, int nmem, const int *nx, const int *ny, Infer *inf)
 void infer_destroy(Infer *inf)
 {
     for (void *p : {(void *)inf->imgs, (void *)inf->tile_img, (void *)inf->tile_local, (void *)inf->row_member,
-                    (void *)inf->row_y, (void *)inf->x, (void *)inf->a0, (void *)inf->a1, (void *)inf->tend,
+                    (void *)inf->row_y, (void *)inf->a0, (void *)inf->a1, (void *)inf->tend,
                     (void *)inf->part, (void *)inf->norms, (void *)inf->active, (void *)inf->nfall})
         if (p) hipFree(p);
     *inf = Infer();
@@ -682,29 +685,31 @@ void closure_launch(const inmsom_bt_cnn *net, hipStream_t st, bool f32, ClosureM
                     double dt, double ratio)
 {
     const dim3 rows((unsigned)inf.nrows), blk(256);
+    const dim3 grid((unsigned)(inf.ntiles < 2048 ? inf.ntiles : 2048));
+    const Layer &L0 = net->L[0];
+    // layer 0 forms its inputs from the members' state; a0 -> a1 -> a0 -> tend
     if (f32)
-        hipLaunchKernelGGL(k_features<float>, rows, blk, 0, st, members, inf.row_member, inf.row_y, net->norm,
-                           inf.x);
+        hipLaunchKernelGGL((k_conv<CIN0, false, EPI_RELU_BF16, 1>), grid, blk, 0, st, inf.imgs, inf.tile_img,
+                           inf.tile_local, inf.ntiles, nullptr, net->wf + net->wf_off[0], net->wm + L0.boff,
+                           nullptr, inf.a0, members, net->norm);
     else
-        hipLaunchKernelGGL(k_features<double>, rows, blk, 0, st, members, inf.row_member, inf.row_y, net->norm,
-                           inf.x);
-    // activations ping-pong through a0/a1 (layer 2 writes a0 again)
-    forward_images(net, st, inf.imgs, inf.tile_img, inf.tile_local, inf.ntiles, inf.x, inf.a0, inf.a1, inf.a0,
-                   inf.tend);
-    if (f32)
+        hipLaunchKernelGGL((k_conv<CIN0, false, EPI_RELU_BF16, 2>), grid, blk, 0, st, inf.imgs, inf.tile_img,
+                           inf.tile_local, inf.ntiles, nullptr, net->wf + net->wf_off[0], net->wm + L0.boff,
+                           nullptr, inf.a0, members, net->norm);
+    conv_launch(net, st, 1, false, inf.imgs, inf.tile_img, inf.tile_local, inf.ntiles, inf.a0, nullptr, inf.a1);
+    conv_launch(net, st, 2, false, inf.imgs, inf.tile_img, inf.tile_local, inf.ntiles, inf.a1, nullptr, inf.a0);
+    conv_launch(net, st, 3, false, inf.imgs, inf.tile_img, inf.tile_local, inf.ntiles, inf.a0, nullptr, inf.tend);
+    if (f32) {
         hipLaunchKernelGGL(k_cl_norm<float>, rows, blk, 0, st, members, inf.row_member, inf.row_y, net->norm,
                            inf.tend, inf.part);
-    else
+        hipLaunchKernelGGL(k_cl_apply<float>, rows, blk, 0, st, members, inf.row_member, inf.row_y, net->norm,
+                           inf.tend, inf.part, ratio, dt, inf.norms, inf.active, inf.nfall);
+    } else {
         hipLaunchKernelGGL(k_cl_norm<double>, rows, blk, 0, st, members, inf.row_member, inf.row_y, net->norm,
                            inf.tend, inf.part);
-    hipLaunchKernelGGL(k_cl_decide, dim3((unsigned)inf.nmem), blk, 0, st, members, inf.part, ratio, inf.norms,
-                       inf.active, inf.nfall);
-    if (f32)
-        hipLaunchKernelGGL(k_cl_apply<float>, rows, blk, 0, st, members, inf.row_member, inf.row_y, net->norm,
-                           inf.tend, inf.active, dt);
-    else
         hipLaunchKernelGGL(k_cl_apply<double>, rows, blk, 0, st, members, inf.row_member, inf.row_y, net->norm,
-                           inf.tend, inf.active, dt);
+                           inf.tend, inf.part, ratio, dt, inf.norms, inf.active, inf.nfall);
+    }
 }
 
 int net_device(const inmsom_bt_cnn *net) { return net->device; }

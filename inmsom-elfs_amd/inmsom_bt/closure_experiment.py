@@ -32,8 +32,8 @@ import numpy as np
 
 from .closure import ClosureNet
 from .ensemble import Ensemble
-from .learned import (FineTruth, Norms, coarse_steps, fit_norms, load_dataset, mask_coarse, rel, rollout,
-                      samples, snapshots, train, uv_masks)
+from .learned import (CoarseStepper, FineTruth, Norms, fit_norms, load_dataset, rel, rollout, samples, snapshots,
+                      train)
 from .workloads import NSTEP_BAROTROP, coarse_ensemble, single_basin
 
 
@@ -77,14 +77,17 @@ def build_dataset(truth: FineTruth, steps, chunk: int = 64):
     cb = truth.coarse
     raws, tgts = [], []
     mask = None
-    for i0 in range(0, len(steps), chunk):
-        snaps = snapshots(truth, steps[i0:i0 + chunk])
-        plain = coarse_steps(cb, snaps, width=chunk)
-        raw, tgt, mask = samples(cb, snaps, plain)
-        raws.append(raw.astype(np.float32))
-        tgts.append(tgt)
-        if (i0 // chunk) % 8 == 0:
-            log(f"  snapshots {i0 + len(snaps)}/{len(steps)} (fine step {truth.step})")
+    stepper = CoarseStepper(cb, min(chunk, len(steps)))
+    try:
+        for i0 in range(0, len(steps), chunk):
+            snaps = snapshots(truth, steps[i0:i0 + chunk])
+            raw, tgt, mask = samples(cb, snaps, stepper(snaps))
+            raws.append(raw.astype(np.float32))
+            tgts.append(tgt)
+            if (i0 // chunk) % 8 == 0:
+                log(f"  snapshots {i0 + len(snaps)}/{len(steps)} (fine step {truth.step})")
+    finally:
+        stepper.close()
     return np.concatenate(raws), np.concatenate(tgts), mask
 
 

@@ -110,19 +110,37 @@ class Norms:
     out_scale: np.ndarray
 
 
+class CoarseStepper:
+    """An FP64 ensemble of ``width`` copies of the coarse basin that takes one
+    plain outer step from each of a batch of snapshots."""
+
+    def __init__(self, cb: Basin, width: int = 64, nstep: int = 2):
+        self.cb, self.width, self.nstep = cb, width, nstep
+        self.ens = Ensemble([cb] * width, mode="fp64")
+
+    def __call__(self, snaps: Sequence[Snapshot]) -> List[dict]:
+        out: List[dict] = []
+        for i0 in range(0, len(snaps), self.width):
+            chunk = snaps[i0:i0 + self.width]
+            for k, s in enumerate(chunk):
+                self.ens.set_state(k, s.now["u"], s.prev["u"], s.now["v"], s.prev["v"], s.now["ssh"],
+                                   s.prev["ssh"])
+            self.ens.run(1, nstep=self.nstep, tau=self.cb.dt)
+            out.extend(self.ens.get_state(k) for k in range(len(chunk)))
+        return out
+
+    def close(self):
+        self.ens.close()
+
+
 def coarse_steps(cb: Basin, snaps: Sequence[Snapshot], width: int = 64, nstep: int = 2) -> List[dict]:
     """One plain FP64 outer step of the coarse basin from each snapshot's
     (C(t), C(t - dt_c / nstep)), ``width`` snapshots per batched launch."""
-    out: List[dict] = []
-    for i0 in range(0, len(snaps), width):
-        chunk = snaps[i0:i0 + width]
-        with Ensemble([cb] * len(chunk), mode="fp64") as ens:
-            for k, s in enumerate(chunk):
-                ens.set_state(k, s.now["u"], s.prev["u"], s.now["v"], s.prev["v"], s.now["ssh"],
-                              s.prev["ssh"])
-            ens.run(1, nstep=nstep, tau=cb.dt)
-            out.extend(ens.get_state(k) for k in range(len(chunk)))
-    return out
+    st = CoarseStepper(cb, min(width, len(snaps)), nstep)
+    try:
+        return st(snaps)
+    finally:
+        st.close()
 
 
 def samples(cb: Basin, snaps: Sequence[Snapshot], plain: Sequence[dict]):
