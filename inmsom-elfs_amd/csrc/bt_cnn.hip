@@ -261,7 +261,7 @@ __global__ void k_reduce_wgrad(int nchunks, int ncol, int kvalid, int kpad, cons
 }
 
 // Masked mean-square loss over U (channel 0) and V (channel 1) points:
-// writes dL/dpred into channels 0..1 of dpred (channels 2..31 stay zero) and
+// writes dL/dpred into channels 0..1 of dpred (channels 2..31 are zeroed) and
 // per-block partial sums of the squared error.
 __global__ void k_loss(long long n, const float *__restrict__ pred, const float *__restrict__ tgt,
                        const uint8_t *__restrict__ mask, float inv_count, bf16 *__restrict__ dpred,

@@ -72,11 +72,13 @@ def load_params(path: str):
     return z["params"], Norms(z["shift"], z["scale"], z["out_scale"]), json.loads(str(z["meta"]))
 
 
-def build_dataset(truth: FineTruth, steps, chunk: int = 64):
-    """float32 raw inputs [n, 4, ny, nx], targets [n, ny, nx, 2], mask."""
+def build_dataset(truth: FineTruth, steps, chunk: int = 64, keep=()):
+    """float32 raw inputs [n, 4, ny, nx], targets [n, ny, nx, 2], mask, and
+    the snapshots whose indices are in ``keep``."""
     cb = truth.coarse
     raws, tgts = [], []
     mask = None
+    kept = {}
     stepper = CoarseStepper(cb, min(chunk, len(steps)))
     try:
         for i0 in range(0, len(steps), chunk):
@@ -84,11 +86,32 @@ def build_dataset(truth: FineTruth, steps, chunk: int = 64):
             raw, tgt, mask = samples(cb, snaps, stepper(snaps))
             raws.append(raw.astype(np.float32))
             tgts.append(tgt)
+            for k, sn in enumerate(snaps):
+                if i0 + k in keep:
+                    kept[i0 + k] = sn
             if (i0 // chunk) % 8 == 0:
                 log(f"  snapshots {i0 + len(snaps)}/{len(steps)} (fine step {truth.step})")
     finally:
         stepper.close()
-    return np.concatenate(raws), np.concatenate(tgts), mask
+    return np.concatenate(raws), np.concatenate(tgts), mask, kept
+
+
+def choose_strength(cb, net: ClosureNet, nm: Norms, start, truth_state, steps: int, ratio: float,
+                    alphas=(0.25, 0.5, 1.0)):
+    """Closure strength alpha (tendency x alpha) with the lowest SSH error of
+    a ``steps``-step rollout inside the training period; 1.0 when no
+    rollout is available.  Returns (alpha, {alpha: ssh error}, plain error)."""
+    if start is None or steps <= 0:
+        return 1.0, {}, None
+    tr = {steps: truth_state}
+    plain = rollout(cb, start, steps, tr, None, ratio, "fp64")["ssh"][-1]
+    errs = {}
+    for a in alphas:
+        net.set_norm(nm.shift, nm.scale, nm.out_scale * np.float32(a))
+        errs[a] = rollout(cb, start, steps, tr, net, ratio, "fp64")["ssh"][-1]
+    net.set_norm(nm.shift, nm.scale, nm.out_scale)
+    best = min(errs, key=errs.get)
+    return best, errs, plain
 
 
 def val_loss(net: ClosureNet, first: int, n: int) -> float:
@@ -111,6 +134,7 @@ def main(argv=None):
     ap.add_argument("--rollout", type=int, default=10000, help="coarse steps of 40 s")
     ap.add_argument("--score-every", type=int, default=500)
     ap.add_argument("--ratio", type=float, default=3.0)
+    ap.add_argument("--tune-steps", type=int, default=1000, help="coarse steps of the strength-selection rollout")
     ap.add_argument("--members", type=int, default=128)
     ap.add_argument("--member-steps", type=int, default=2000)
     ap.add_argument("--workload", default="cfg1")
@@ -133,9 +157,14 @@ def main(argv=None):
     # ---- 1-2: snapshots and targets
     t0 = T()
     steps = [f + k * args.every for k in range(args.snapshots)]
-    raw, tgt, mask = build_dataset(truth, steps)
+    ntrain = max(args.batch, int(round(args.train_frac * len(steps))))
+    # strength selection: a rollout from the middle of the training period,
+    # scored at a later training snapshot (snapshots are every `every` fine steps)
+    v0 = ntrain // 2
+    dv = min(args.tune_steps * f // args.every, ntrain - 1 - v0)
+    tune_steps = dv * args.every // f if dv > 0 else 0
+    raw, tgt, mask, kept = build_dataset(truth, steps, keep={v0, v0 + dv} if tune_steps else ())
     n = raw.shape[0]
-    ntrain = max(args.batch, int(round(args.train_frac * n)))
     nm = fit_norms(cb, raw[:ntrain], tgt[:ntrain])
     res["dataset"] = {"samples": n, "train": ntrain, "validation": n - ntrain, "seconds": T() - t0,
                       "norm_shift": nm.shift.tolist(), "norm_scale": nm.scale.tolist(),
@@ -154,6 +183,13 @@ def main(argv=None):
     net.synchronize()
     train_s = T() - t0
     vl = val_loss(net, ntrain, n - ntrain)
+    alpha, aerr, perr = choose_strength(cb, net, nm, kept.get(v0), kept[v0 + dv].now if tune_steps else None,
+                                        tune_steps, args.ratio)
+    res["strength"] = {"alpha": alpha, "tune_steps": tune_steps, "ssh_err_by_alpha": {str(k): v for k, v in aerr.items()},
+                       "ssh_err_plain": perr, "start_step": steps[v0] if tune_steps else None}
+    log(f"strength: alpha {alpha} (ssh err {aerr}, plain {perr}) over {tune_steps} coarse steps")
+    nm = Norms(nm.shift, nm.scale, (nm.out_scale * np.float32(alpha)).astype(np.float32))
+    net.set_norm(nm.shift, nm.scale, nm.out_scale)
     res["training"] = {"steps": args.train_steps, "seconds": train_s, "ms_per_step": 1e3 * train_s / args.train_steps,
                        "loss_first": float(np.mean(losses[:10])), "loss_last": float(np.mean(losses[-50:])),
                        "val_loss": vl, "loss_curve": [float(np.mean(losses[i:i + 50]))
@@ -165,7 +201,7 @@ def main(argv=None):
     save_params(os.path.join(args.out, "closure_params.npz"), net, nm,
                 {"workload": args.workload, "snapshots": n, "every": args.every, "train": ntrain,
                  "train_steps": args.train_steps, "lr": args.lr, "batch": args.batch, "t_start": t_start,
-                 "val_loss": vl, "seed": args.seed})
+                 "val_loss": vl, "seed": args.seed, "alpha": alpha})
 
     # ---- 4: rollout against the coarsened truth
     start = snapshots(truth, [t_start])[0]
