@@ -25,7 +25,7 @@ def _coarsen_np(st, b, f):
     u = st["u"][2:-2, 2:-2]
     v = st["v"][2:-2, 2:-2]
     out["u"][2:-2, 2:-2] = u[:, f - 1::f].reshape(cy, f, cx).mean(axis=1)
-    out["v"][2:-2, 2:-2] = v[f - 1::f, :].reshape(cy, f, cx).mean(axis=1)
+    out["v"][2:-2, 2:-2] = v[f - 1::f, :].reshape(cy, cx, f).mean(axis=2)
     return out
 
 
