@@ -51,10 +51,56 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def traffic_from_profiles(mode):
-    path = os.path.join(ROOT, "profiles", "fused_step_cfg1_traffic.json")
-    try:
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "fused_step_cfg1_traffic.json")
+TRAFFIC_KERNELS = {"fp64": "k_fused_stepIdLb0", "fp32": "k_fused_stepIfLb0",
+                   "fp64_exact": "k_fused_stepIdLb1", "ensemble_fp32": "k_fused_batchIfLb0"}
+
+
+def write_traffic(fetch_csv, write_csv, out=TRAFFIC_JSON):
+    """HBM bytes per launch of the fused kernels from two rocprofv3 --pmc
+    passes of this script (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch),
+    each scaled by what the counter reads for k_copy_d in the same run: the
+    FP64 set-up copies 2052 x 1540 doubles (read once, written once)."""
+    import collections
+    import csv
+
+    def load(path, counter):
+        agg = collections.defaultdict(list)
         with open(path) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] == counter:
+                    agg[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+        return {k: (sum(v) / len(v), len(v)) for k, v in agg.items()}
+
+    def find(d, key):
+        ks = [k for k in d if key in k]
+        if len(ks) != 1:
+            raise KeyError(f"{key}: {ks}")
+        return d[ks[0]], ks[0]
+
+    F, W = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
+    kib = 2052 * 1540 * 8 / 1024
+    (fc, _), _ = find(F, "k_copy_d")
+    (wc, _), _ = find(W, "k_copy_d")
+    res = {"calibration": {"kernel": "k_copy_d", "bytes_each_way": 2052 * 1540 * 8, "fetch_kib": fc,
+                           "write_kib": wc, "fetch_factor": kib / fc, "write_factor": kib / wc},
+           "fetch_csv": os.path.relpath(fetch_csv, ROOT), "write_csv": os.path.relpath(write_csv, ROOT)}
+    for mode, key in TRAFFIC_KERNELS.items():
+        try:
+            (f, nf), name = find(F, key)
+            (w, _), _ = find(W, key)
+        except KeyError:
+            continue
+        res[mode] = {"kernel": name, "dispatches": nf, "fetch_kib": f, "write_kib": w,
+                     "fetch_bytes_per_launch": f * kib / fc * 1024, "write_bytes_per_launch": w * kib / wc * 1024}
+    with open(out, "w") as fo:
+        json.dump(res, fo, indent=1)
+    return res
+
+
+def traffic_from_profiles(mode):
+    try:
+        with open(TRAFFIC_JSON) as f:
             t = json.load(f)
         e = t[mode]
         return float(e["fetch_bytes_per_launch"] + e["write_bytes_per_launch"])
@@ -250,7 +296,13 @@ def main():
                     help="coarse 40 s steps of the learned-closure rollout (0 = skip)")
     ap.add_argument("--closure-members", type=int, default=128)
     ap.add_argument("--closure-member-steps", type=int, default=1000)
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write the per-launch HBM traffic of the fused kernels from two rocprofv3 "
+                         "--pmc counter_collection.csv files (FETCH_SIZE, WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.traffic_from:
+        print(json.dumps(write_traffic(*args.traffic_from), indent=1))
+        return
 
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
