@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(256)
 template <int CIN, int NCB>
 __global__ void __launch_bounds__(64)
     k_wgrad(int H, int W, long long npix, int chunk, const bf16 *__restrict__ dy,
-            const bf16 *__restrict__ x, float *__restrict__ part_w, float *__restrict__ part_b)
+            const bf16 *__restrict__ x, float *__restrict__ part)
 {
     const int lane = threadIdx.x, j = lane & 31, half = lane >> 5;
     const long long p0 = (long long)blockIdx.x * chunk;
@@ -228,34 +228,50 @@ __global__ void __launch_bounds__(64)
             acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[cb], 0, 0, 0);
         }
     }
+    // record of this chunk: W partials [32][ncol], then the 32 bias partials
     const int ncol = NCB * 32;
+    float *rec = part + (size_t)blockIdx.x * (32 * ncol + 32);
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int co = 8 * (r >> 2) + 4 * half + (r & 3);
-            part_w[((size_t)blockIdx.x * 32 + co) * ncol + cb * 32 + j] = acc[cb][r];
+            rec[co * ncol + cb * 32 + j] = acc[cb][r];
         }
     bsum += __shfl_xor(bsum, 32);
-    if (half == 0) part_b[(size_t)blockIdx.x * 32 + j] = bsum;
+    if (half == 0) rec[32 * ncol + j] = bsum;
 }
 
-// Fixed-order sum of the chunk partials into the gradient (layout [32][kpad]).
-__global__ void k_reduce_wgrad(int nchunks, int ncol, int kvalid, int kpad, const float *__restrict__ part_w,
-                               const float *__restrict__ part_b, float *__restrict__ gw,
-                               float *__restrict__ gb)
+// Fixed-order reduction of the chunk records (rec floats each) in two
+// passes: groups of RG consecutive chunks in parallel, then the groups.
+constexpr int RG = 64;
+
+__global__ void k_reduce_groups(int nchunks, int rec, const float *__restrict__ part, float *__restrict__ part2)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rec) return;
+    const int c0 = blockIdx.y * RG, c1 = min(c0 + RG, nchunks);
+    float s = 0.0f;
+    for (int c = c0; c < c1; ++c) s += part[(size_t)c * rec + e];
+    part2[(size_t)blockIdx.y * rec + e] = s;
+}
+
+// groups -> gradient: W in [32][kpad] (zero past kvalid), bias [32]
+__global__ void k_reduce_wgrad(int ngroups, int ncol, int kvalid, int kpad, const float *__restrict__ part2,
+                               float *__restrict__ gw, float *__restrict__ gb)
 {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t rec = (size_t)32 * ncol + 32;
     if (idx < 32 * kpad) {
         const int co = idx / kpad, k = idx - co * kpad;
         float s = 0.0f;
         if (k < kvalid)
-            for (int c = 0; c < nchunks; ++c) s += part_w[((size_t)c * 32 + co) * ncol + k];
+            for (int g = 0; g < ngroups; ++g) s += part2[g * rec + (size_t)co * ncol + k];
         gw[idx] = s;
     }
     if (idx < 32) {
         float s = 0.0f;
-        for (int c = 0; c < nchunks; ++c) s += part_b[(size_t)c * 32 + idx];
+        for (int g = 0; g < ngroups; ++g) s += part2[g * rec + (size_t)32 * ncol + idx];
         gb[idx] = s;
     }
 }
@@ -452,7 +468,8 @@ struct inmsom_bt_cnn {
     int adam_t = 0;
     // training batch buffers
     bf16 *x = nullptr, *act[3] = {}, *dy = nullptr, *dx = nullptr;
-    float *pred = nullptr, *tgt = nullptr, *loss_part = nullptr, *part_w = nullptr, *part_b = nullptr;
+    float *pred = nullptr, *tgt = nullptr, *loss_part = nullptr;
+    float *part_w = nullptr, *part_b = nullptr;  // weight-gradient chunk records, group records
     uint8_t *mask = nullptr;
     Img *imgs = nullptr;
     int *tile_img = nullptr, *tile_local = nullptr;
@@ -563,14 +580,18 @@ void wgrad_launch(inmsom_bt_cnn *c, int l, const bf16 *dy, const bf16 *x)
     const Layer &Ly = c->L[l];
     if (l == 0)
         hipLaunchKernelGGL((k_wgrad<CIN0, 3>), dim3(nch), dim3(64), 0, c->stream, c->H, c->W, npix,
-                           c->wg_chunk, dy, x, c->part_w, c->part_b);
+                           c->wg_chunk, dy, x, c->part_w);
     else
         hipLaunchKernelGGL((k_wgrad<CH, 9>), dim3(nch), dim3(64), 0, c->stream, c->H, c->W, npix, c->wg_chunk,
-                           dy, x, c->part_w, c->part_b);
+                           dy, x, c->part_w);
     const int ncol = l == 0 ? 96 : 288;
+    const int rec = 32 * ncol + 32;
+    const int ngroups = (nch + RG - 1) / RG;
+    hipLaunchKernelGGL(k_reduce_groups, dim3((rec + 255) / 256, ngroups), dim3(256), 0, c->stream, nch, rec,
+                       c->part_w, c->part_b);
     const int n = 32 * Ly.kpad;
-    hipLaunchKernelGGL(k_reduce_wgrad, dim3((n + 255) / 256), dim3(256), 0, c->stream, nch, ncol, 9 * Ly.cin,
-                       Ly.kpad, c->part_w, c->part_b, c->grad + Ly.woff, c->grad + Ly.boff);
+    hipLaunchKernelGGL(k_reduce_wgrad, dim3((n + 255) / 256), dim3(256), 0, c->stream, ngroups, ncol,
+                       9 * Ly.cin, Ly.kpad, c->part_b, c->grad + Ly.woff, c->grad + Ly.boff);
 }
 
 void pack_all(inmsom_bt_cnn *c)
@@ -760,8 +781,9 @@ int inmsom_bt_cnn_create(int batch, int ny, int nx, int device, inmsom_bt_cnn **
         CTRY(dmalloc(&c->mask, np));
         CTRY(dmalloc(&c->loss_part, (np + 255) / 256));
         c->wg_nchunks = (int)((np + c->wg_chunk - 1) / c->wg_chunk);
-        CTRY(dmalloc(&c->part_w, (size_t)c->wg_nchunks * 32 * 9 * CH));
-        CTRY(dmalloc(&c->part_b, (size_t)c->wg_nchunks * 32));
+        // chunk records, then group records (32 x 288 weights + 32 biases each)
+        CTRY(dmalloc(&c->part_w, (size_t)c->wg_nchunks * (32 * 9 * CH + 32)));
+        CTRY(dmalloc(&c->part_b, (size_t)((c->wg_nchunks + RG - 1) / RG) * (32 * 9 * CH + 32)));
         std::vector<Img> imgs(batch);
         for (int b = 0; b < batch; ++b) imgs[b] = Img{ny, nx, (long long)b * ny * nx};
         CTRY(build_tiles(imgs, &c->imgs, &c->tile_img, &c->tile_local, &c->ntiles));

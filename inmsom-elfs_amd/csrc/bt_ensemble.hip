@@ -141,7 +141,12 @@ int fail(const std::string &m, int code)
 
 size_t esz(int mode) { return mode == INMSOM_BT_FP32 ? 4 : 8; }
 
-int default_chunk(int mode) { return mode == INMSOM_BT_FP32 ? 20 : 32; }
+int default_chunk(const inmsom_bt_ens *e)
+{
+    long long sr = 0;
+    for (const EnsMember &m : e->mem) sr += (long long)((m.nx + 2 + SW - 1) / SW) * m.ny;
+    return auto_chunk(e->mode == INMSOM_BT_FP32, sr);
+}
 
 int default_waves(int mode)
 {
@@ -408,7 +413,7 @@ int ensure_graphs(inmsom_bt_ens *e, double tau, int nstep, int chunk, int wpx)
 
 int run_steps(inmsom_bt_ens *e, double tau, int nstep, int nsteps, int use_graph)
 {
-    const int chunk = e->chunk > 0 ? e->chunk : default_chunk(e->mode);
+    const int chunk = e->chunk > 0 ? e->chunk : default_chunk(e);
     const int wpx = e->waves_per_xcd > 0 ? e->waves_per_xcd : default_waves(e->mode);
     int rc = build_tables(e, chunk);
     if (rc) return rc;
@@ -598,7 +603,7 @@ int inmsom_bt_ens_run_timed(inmsom_bt_ens *e, double tau, int nstep, int nsteps,
         return fail("bad tau / nstep / nsteps", INMSOM_BT_EINVAL);
     EHC(hipSetDevice(e->device));
     // build tables and graphs before the timed region
-    const int chunk = e->chunk > 0 ? e->chunk : default_chunk(e->mode);
+    const int chunk = e->chunk > 0 ? e->chunk : default_chunk(e);
     const int wpx = e->waves_per_xcd > 0 ? e->waves_per_xcd : default_waves(e->mode);
     int rc = build_tables(e, chunk);
     if (rc) return rc;

@@ -360,8 +360,8 @@ static FusedParams make_params(inmsom_bt_ctx *c, FusedAlloc *a, int b, double ta
     p.nx = c->nx;
     p.ny = c->ny;
     p.strips = (c->nx + 2 + SW - 1) / SW;  // columns 1..nx need strips covering m <= nx
-    // measured on MI355X at 2048x1536: FP64 best at 32 rows, FP32 at 20
-    p.chunk = c->chunk > 0 ? c->chunk : (c->fa && c->fa->mode == INMSOM_BT_FP32 ? 20 : 32);
+    p.chunk = c->chunk > 0 ? c->chunk
+                           : auto_chunk(c->fa && c->fa->mode == INMSOM_BT_FP32, (long long)p.strips * c->ny);
     p.nchunks = (c->ny + p.chunk - 1) / p.chunk;
     p.total = p.strips * p.nchunks;
     p.per_xcd = (p.total + NXCD - 1) / NXCD;

@@ -133,7 +133,10 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--rollout", type=int, default=10000, help="coarse steps of 40 s")
     ap.add_argument("--score-every", type=int, default=500)
-    ap.add_argument("--ratio", type=float, default=3.0)
+    ap.add_argument("--ratio", type=float, default=3.0,
+                    help="ensemble fallback: closure norm above ratio x plain viscous tendency norm")
+    ap.add_argument("--rollout-ratio", type=float, default=0.0,
+                    help="fallback ratio of the single-basin rollouts (0 = closure always applied)")
     ap.add_argument("--tune-steps", type=int, default=1000, help="coarse steps of the strength-selection rollout")
     ap.add_argument("--members", type=int, default=128)
     ap.add_argument("--member-steps", type=int, default=2000)
@@ -183,8 +186,9 @@ def main(argv=None):
     net.synchronize()
     train_s = T() - t0
     vl = val_loss(net, ntrain, n - ntrain)
+    rr = args.rollout_ratio if args.rollout_ratio > 0 else 1e300
     alpha, aerr, perr = choose_strength(cb, net, nm, kept.get(v0), kept[v0 + dv].now if tune_steps else None,
-                                        tune_steps, args.ratio)
+                                        tune_steps, rr)
     res["strength"] = {"alpha": alpha, "tune_steps": tune_steps, "ssh_err_by_alpha": {str(k): v for k, v in aerr.items()},
                        "ssh_err_plain": perr, "start_step": steps[v0] if tune_steps else None}
     log(f"strength: alpha {alpha} (ssh err {aerr}, plain {perr}) over {tune_steps} coarse steps")
@@ -215,11 +219,12 @@ def main(argv=None):
     runs = {}
     for name, mode, use in (("plain_fp64", "fp64", False), ("learned_fp64", "fp64", True),
                             ("plain_fp32", "fp32", False), ("learned_fp32", "fp32", True)):
-        r = rollout(cb, start, args.rollout, tr, net if use else None, args.ratio, mode)
+        r = rollout(cb, start, args.rollout, tr, net if use else None, rr, mode)
         r.pop("final")
         runs[name] = r
         log(f"rollout {name}: ssh rel err @end {r['ssh'][-1]:.4f}, mean {np.mean(r['ssh']):.4f}; "
-            f"u {r['u'][-1]:.4f}, v {r['v'][-1]:.4f}" + (f", fallback steps {r['fallback_steps']}" if use else ""))
+            f"u {r['u'][-1]:.4f}, v {r['v'][-1]:.4f}" +
+            (f", fallback steps {r['fallback_steps']}, last-step norms closure/viscous {r['norms']}" if use else ""))
     res["rollout"] = runs
 
     # ---- throughput: fine FP64 vs coarse with/without closure (simulated seconds per wall second)
@@ -231,7 +236,7 @@ def main(argv=None):
                            "wet_cell_steps_per_s": fine.n_wet * 2000 * NSTEP_BAROTROP / (ms * 1e-3)}
     for name, mode, use in (("coarse_plain_fp64", "fp64", False), ("coarse_learned_fp64", "fp64", True),
                             ("coarse_plain_fp32", "fp32", False), ("coarse_learned_fp32", "fp32", True)):
-        r = rollout(cb, start, 2000, None, net if use else None, args.ratio, mode, timed=True)
+        r = rollout(cb, start, 2000, None, net if use else None, rr, mode, timed=True)
         ms = r["ms"]
         tm[name] = {"ms_per_step": ms / 2000, "sim_s_per_s": cb.dt * 2000 / (ms * 1e-3),
                     "wet_cell_steps_per_s": cb.n_wet * 2000 * NSTEP_BAROTROP / (ms * 1e-3)}
