@@ -198,7 +198,7 @@ def ensemble_leg(steps, warmup, nmembers):
 CLOSURE_PARAMS = os.path.join(ROOT, "inmsom-elfs_amd", "inmsom_bt", "data", "closure_cfg1.npz")
 
 
-def learned_leg(fine, rollout_steps, members, member_steps, ratio=3.0):
+def learned_leg(fine, rollout_steps, members, member_steps, ratio=3.0, rollout_ratio=1e300):
     """cfg3/cfg4: the trained closure (parameters trained on the device by
     inmsom_bt.closure_experiment, committed) on the 4x-coarsened basin:
     throughput in simulated time vs the 1 km FP64 run, SSH error vs the
@@ -230,7 +230,7 @@ def learned_leg(fine, rollout_steps, members, member_steps, ratio=3.0):
     runs = {}
     for name, mode, use in (("plain_fp64", "fp64", False), ("learned_fp64", "fp64", True),
                             ("plain_fp32", "fp32", False), ("learned_fp32", "fp32", True)):
-        r = rollout(cb, start, rollout_steps, tr, net if use else None, ratio, mode)
+        r = rollout(cb, start, rollout_steps, tr, net if use else None, rollout_ratio, mode)
         runs[name] = {"ssh_rel_l2": r["ssh"][-1], "u_rel_l2": r["u"][-1], "v_rel_l2": r["v"][-1],
                       "range_check_bad_cells": r["bad"]}
         if use:
@@ -242,7 +242,7 @@ def learned_leg(fine, rollout_steps, members, member_steps, ratio=3.0):
     tm["fine_fp64"] = {"us_per_step": ms, "sim_s_per_s": fine_sim}
     for name, mode, use in (("plain_fp32", "fp32", False), ("learned_fp32", "fp32", True),
                             ("learned_fp64", "fp64", True)):
-        r = rollout(cb, start, 1000, None, net if use else None, ratio, mode, timed=True)
+        r = rollout(cb, start, 1000, None, net if use else None, rollout_ratio, mode, timed=True)
         sim = cb.dt * 1000 / (r["ms"] * 1e-3)
         tm[name] = {"us_per_step": r["ms"], "sim_s_per_s": sim, "speedup_vs_fine_fp64": sim / fine_sim,
                     "wet_cell_steps_per_s": cb.n_wet * 1000 * NSTEP_BAROTROP / (r["ms"] * 1e-3)}
@@ -257,25 +257,33 @@ def learned_leg(fine, rollout_steps, members, member_steps, ratio=3.0):
     truth.close()
     if members > 0:
         mem = coarse_ensemble(members)
-        with Ensemble(mem, mode="fp32") as ens, Ensemble(mem, mode="fp64") as ref:
-            nmem, wet, cells = ens.size()
-            ens.set_closure(net, ratio)
-            ens.run(5, NSTEP_BAROTROP)
-            ens.reset()
-            ms = ens.run_timed(member_steps, NSTEP_BAROTROP)
+        ens_out = {}
+        with Ensemble(mem, mode="fp64") as ref:
             ref.run(member_steps, NSTEP_BAROTROP)
-            nf, _ = ens.closure_stats()
-            errs = [rel(ens.get_state(k)["ssh"], ref.get_state(k)["ssh"], b.wet) for k, b in enumerate(mem)]
-            bad = ens.range_check()
-        out["ensemble_fp32_closure"] = {
-            "members": nmem, "wet_cells": wet, "array_cells": cells, "steps": member_steps,
-            "ms_per_step": ms / member_steps, "value": wet * member_steps * NSTEP_BAROTROP / (ms * 1e-3),
-            "unit": "wet-cell steps/s", "ssh_rel_l2_vs_plain_fp64": {"median": float(np.median(errs)),
-                                                                    "max": float(np.max(errs))},
-            "fallback": {"members": int((nf > 0).sum()), "member_steps": int(nf.sum())},
-            "range_check_bad_cells": bad}
-        log(f"closure ensemble: {nmem} members, {ms / member_steps:.2f} ms/step, "
-            f"{out['ensemble_fp32_closure']['value']:.3e} wet-cell steps/s")
+            ref_ssh = [ref.get_state(k)["ssh"] for k in range(len(mem))]
+        # the fallback rule as stated (ratio x the plain viscous tendency), then the
+        # closure applied at every step
+        for label, rt in (("fallback_rule", ratio), ("always_on", 1e300)):
+            with Ensemble(mem, mode="fp32") as ens:
+                nmem, wet, cells = ens.size()
+                ens.set_closure(net, rt)
+                ens.run(5, NSTEP_BAROTROP)
+                ens.reset()
+                ms = ens.run_timed(member_steps, NSTEP_BAROTROP)
+                nf, nr = ens.closure_stats()
+                errs = [rel(ens.get_state(k)["ssh"], ref_ssh[k], b.wet) for k, b in enumerate(mem)]
+                bad = ens.range_check()
+            ens_out[label] = {
+                "ratio": rt, "ms_per_step": ms / member_steps,
+                "value": wet * member_steps * NSTEP_BAROTROP / (ms * 1e-3), "unit": "wet-cell steps/s",
+                "ssh_rel_l2_vs_plain_fp64": {"median": float(np.median(errs)), "max": float(np.max(errs))},
+                "fallback": {"members": int((nf > 0).sum()), "member_steps": int(nf.sum())},
+                "last_step_norm_ratio_median": float(np.median(nr[:, 0] / np.maximum(nr[:, 1], 1e-300))),
+                "range_check_bad_cells": bad}
+            log(f"closure ensemble ({label}): {nmem} members, {ms / member_steps:.2f} ms/step, "
+                f"{ens_out[label]['value']:.3e} wet-cell steps/s, fallback members {(nf > 0).sum()}")
+        out["ensemble_fp32_closure"] = {"members": nmem, "wet_cells": wet, "array_cells": cells,
+                                        "steps": member_steps, **ens_out}
     net.close()
     return out
 
