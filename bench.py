@@ -52,8 +52,8 @@ def rel_l2(a, b):
 
 
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "fused_step_cfg1_traffic.json")
-TRAFFIC_KERNELS = {"fp64": "k_fused_stepIdLb0", "fp32": "k_fused_stepIfLb0",
-                   "fp64_exact": "k_fused_stepIdLb1", "ensemble_fp32": "k_fused_batchIfLb0"}
+TRAFFIC_KERNELS = {"fp64": "k_fused_step<double, false>", "fp32": "k_fused_step<float, false>",
+                   "fp64_exact": "k_fused_step<double, true>", "ensemble_fp32": "k_fused_batch<float, false>"}
 
 
 def write_traffic(fetch_csv, write_csv, out=TRAFFIC_JSON):
