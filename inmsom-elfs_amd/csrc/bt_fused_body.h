@@ -53,6 +53,20 @@ __device__ __forceinline__ double xl(double x)  // value of column m-1
 __device__ __forceinline__ float xr(float x) { return __int_as_float(dpp_from_right(__float_as_int(x))); }
 __device__ __forceinline__ float xl(float x) { return __int_as_float(dpp_from_left(__float_as_int(x))); }
 
+// Stores of the step's outputs (read again only by the next launch).
+// Non-temporal in the FP64 modes: measured on MI355X at 2048 x 1536 they
+// take the FP64 step from 100.0 to 95.7 us per launch, leave FP32 unchanged
+// (71.5 / 71.1 us) and slow the FP32 ensemble by 8 %.
+template <bool NT, class T>
+__device__ __forceinline__ void st_out(void *base, size_t i, T v)
+{
+    T *q = static_cast<T *>(base) + i;
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, q);
+    else
+        *q = v;
+}
+
 template <class R>
 __device__ __forceinline__ R bitf(int mb, int bit)
 {
@@ -136,6 +150,7 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
     const bool inx = (m >= 3 && m <= p.nx - 2);
     const bool inx1 = (m >= 2 && m <= p.nx - 1);
 
+    constexpr bool NTS = sizeof(R) == 8;
     Ops<R, EXACT> op(p);
     const R d = op.d;
     const R tau = R(p.tau);
@@ -359,8 +374,8 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
                 rhs_u = fx_0;
                 rhs_v = fy_0;
             } else if (p.nsub > 1 && own) {
-                static_cast<R *>(p.frx)[io] = rhs_u;
-                static_cast<R *>(p.fry)[io] = rhs_v;
+                st_out<NTS, R>(p.frx, io, rhs_u);
+                st_out<NTS, R>(p.fry, io, rhs_v);
             }
             // level n-1 U/V depths: hh_init values at the first substep,
             // the hh_shift-filtered carry afterwards (depth.f90:195-203)
@@ -490,18 +505,18 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
                         if (mb_0 & MB_LLU) hu_new = hu_0 + ts * (hun_0 - 2.0 * hu_0 + hupc) / 2.0 / p.dn;
                         if (mb_0 & MB_LLV) hv_new = hv_0 + ts * (hvn_0 - 2.0 * hv_0 + hvpc) / 2.0 / p.dn;
                     }
-                    p.hup_c[io] = hu_new;
-                    p.hvp_c[io] = hv_new;
+                    st_out<NTS, double>(p.hup_c, io, hu_new);
+                    st_out<NTS, double>(p.hvp_c, io, hv_new);
                 }
             }
             if (own) {
                 const size_t i = io;
-                static_cast<R *>(p.so)[i] = s_new;
-                static_cast<R *>(p.uo)[i] = u_new;
-                static_cast<R *>(p.vo)[i] = v_new;
-                p.spo[i] = sp_new;
-                p.upo[i] = up_new;
-                p.vpo[i] = vp_new;
+                st_out<NTS, R>(p.so, i, s_new);
+                st_out<NTS, R>(p.uo, i, u_new);
+                st_out<NTS, R>(p.vo, i, v_new);
+                st_out<NTS, double>(p.spo, i, sp_new);
+                st_out<NTS, double>(p.upo, i, up_new);
+                st_out<NTS, double>(p.vpo, i, vp_new);
             }
         }
 
