@@ -67,12 +67,6 @@ constexpr int lds_stride() { return CIN == 8 ? 8 : CIN + 8; }
 //   FT = 1 / 2 (layer 0 of the closure only): the input tile is formed from
 //              the members' float / double (u, v, eta, depth) arrays and the
 //              normalisation instead of being read from `in`
-template <class T>
-__device__ inline float ldf(const void *p, long long i)
-{
-    return (float)static_cast<const T *>(p)[i];
-}
-
 template <int CIN, bool NEG, int EPI, int FT = 0>
 __global__ void __launch_bounds__(256)
     k_conv(const Img *__restrict__ imgs, const int *__restrict__ tile_img,
@@ -95,34 +89,71 @@ __global__ void __launch_bounds__(256)
     for (int s = 0; s < KS; ++s) b[s] = *reinterpret_cast<const bf16x8 *>(w + (size_t)i * KPAD + 16 * s + 8 * half);
     const float bco = EPI == EPI_MASK_BF16 ? 0.0f : bias[i];
 
+    // Staging is software-pipelined: the global loads of the block's next
+    // tile are issued into registers before the current tile's matrix
+    // products and committed to LDS after them.
+    constexpr int NPT = (NCHUNK + 255) / 256;  // staged chunks per thread
+    using FT_T = typename std::conditional<FT == 2, double, float>::type;
+    bf16x8 raw[NPT];
+    FT_T rawf[FT ? NPT : 1][4];
+    unsigned valid = 0;
+    auto fetch = [&](int t) {
+        const Img im = imgs[tile_img[t]];
+        const int tl = tile_local[t];
+        const int tx = (im.W + TW - 1) / TW;
+        const int ty = tl / tx;
+        const int y0 = ty * TR, x0 = (tl - ty * tx) * TW;
+        valid = 0;
+#pragma unroll
+        for (int j = 0; j < NPT; ++j) {
+            const int c = threadIdx.x + 256 * j;
+            const int px = c / CPP, part = c - px * CPP;
+            const int ry = px / (TW + 2), rx = px - ry * (TW + 2);
+            const int y = y0 - 1 + ry, x = x0 - 1 + rx;
+            if (c < NCHUNK && y >= 0 && y < im.H && x >= 0 && x < im.W) {
+                valid |= 1u << j;
+                if constexpr (FT == 0) {
+                    raw[j] = *reinterpret_cast<const bf16x8 *>(in + ((im.pix + (long long)y * im.W + x) * CIN + part * 8));
+                } else {
+                    const ClosureMember &m = mem[tile_img[t]];
+                    const long long a = (long long)y * m.nx + x;
+                    rawf[j][0] = static_cast<const FT_T *>(m.u)[a];
+                    rawf[j][1] = static_cast<const FT_T *>(m.v)[a];
+                    rawf[j][2] = static_cast<const FT_T *>(m.s)[a];
+                    rawf[j][3] = static_cast<const FT_T *>(m.H)[a];
+                }
+            }
+        }
+    };
+    auto commit = [&]() {
+#pragma unroll
+        for (int j = 0; j < NPT; ++j) {
+            const int c = threadIdx.x + 256 * j;
+            if (c >= NCHUNK) continue;
+            const int px = c / CPP, part = c - px * CPP;
+            bf16x8 v = {};
+            if (valid & (1u << j)) {
+                if constexpr (FT == 0) {
+                    v = raw[j];
+                } else {
+#pragma unroll
+                    for (int f = 0; f < NIN; ++f) v[f] = (bf16)(((float)rawf[j][f] - nm.shift[f]) * nm.scale[f]);
+                }
+            }
+            *reinterpret_cast<bf16x8 *>(tile_in + px * LS + part * 8) = v;
+        }
+    };
+    if (blockIdx.x < (unsigned)ntiles) fetch(blockIdx.x);
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const Img im = imgs[tile_img[tile]];
         const int tl = tile_local[tile];
         const int tx = (im.W + TW - 1) / TW;
         const int ty = tl / tx;
         const int y0 = ty * TR, x0 = (tl - ty * tx) * TW;
-        __syncthreads();  // the previous tile's reads are done
-        for (int c = threadIdx.x; c < NCHUNK; c += 256) {
-            const int px = c / CPP, part = c - px * CPP;
-            const int ry = px / (TW + 2), rx = px - ry * (TW + 2);
-            const int y = y0 - 1 + ry, x = x0 - 1 + rx;
-            bf16x8 v = {};
-            if (y >= 0 && y < im.H && x >= 0 && x < im.W) {
-                if constexpr (FT == 0) {
-                    v = *reinterpret_cast<const bf16x8 *>(in + ((im.pix + (long long)y * im.W + x) * CIN + part * 8));
-                } else {
-                    using T = typename std::conditional<FT == 1, float, double>::type;
-                    const ClosureMember &m = mem[tile_img[tile]];
-                    const long long a = (long long)y * m.nx + x;
-                    v[0] = (bf16)((ldf<T>(m.u, a) - nm.shift[0]) * nm.scale[0]);
-                    v[1] = (bf16)((ldf<T>(m.v, a) - nm.shift[1]) * nm.scale[1]);
-                    v[2] = (bf16)((ldf<T>(m.s, a) - nm.shift[2]) * nm.scale[2]);
-                    v[3] = (bf16)((ldf<T>(m.H, a) - nm.shift[3]) * nm.scale[3]);
-                }
-            }
-            *reinterpret_cast<bf16x8 *>(tile_in + px * LS + part * 8) = v;
-        }
+        __syncthreads();  // the previous tile's reads of the LDS tile are done
+        commit();
         __syncthreads();
+        if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
         f32x16 acc[RW];
 #pragma unroll
         for (int q = 0; q < RW; ++q) acc[q] = f32x16{};
