@@ -31,6 +31,8 @@
 typedef __bf16 bf16;
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+typedef __attribute__((__vector_size__(2 * sizeof(float)))) float f32x2;
 
 namespace ibt {
 namespace cnn {
@@ -87,7 +89,10 @@ __global__ void __launch_bounds__(256)
     bf16x8 b[KS];
 #pragma unroll
     for (int s = 0; s < KS; ++s) b[s] = *reinterpret_cast<const bf16x8 *>(w + (size_t)i * KPAD + 16 * s + 8 * half);
-    const float bco = EPI == EPI_MASK_BF16 ? 0.0f : bias[i];
+    // bias of the lane's 16 output channels (8 (r / 4) + 4 half + r % 4)
+    float bco[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) bco[r] = EPI == EPI_MASK_BF16 ? 0.0f : bias[8 * (r >> 2) + 4 * half + (r & 3)];
 
     // Staging is software-pipelined: the global loads of the block's next
     // tile are issued into registers before the current tile's matrix
@@ -171,26 +176,37 @@ __global__ void __launch_bounds__(256)
                 const int r = wv * RW + q;
                 bf16x8 a = {};
                 if (t < 9) a = *reinterpret_cast<const bf16x8 *>(tile_in + ((r + 1 + oy) * (TW + 2) + (i + 1 + ox)) * LS + ci0);
-                acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[s], acc[q], 0, 0, 0);
+                // A = weights (rows = output channels), B = pixels: D[co][px], so that
+                // each lane ends with 16 channels of one pixel (4 runs of 4)
+                acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[s], a, acc[q], 0, 0, 0);
             }
         }
+        // lane: pixel x0 + i, channels 8g + 4 half + 0..3 in acc[.][4g .. 4g+3]
+        const int px = x0 + i;
 #pragma unroll
         for (int q = 0; q < RW; ++q) {
             const int y = y0 + wv * RW + q;
-            if (y >= im.H) continue;
+            if (y >= im.H || px >= im.W) continue;
+            const long long p = im.pix + (long long)y * im.W + px;
+            if constexpr (EPI == EPI_OUT_F32) {
+                if (half == 0) {
+                    f32x2 o = {acc[q][0] + bco[0], acc[q][1] + bco[1]};
+                    *reinterpret_cast<f32x2 *>(static_cast<float *>(out) + p * NOUT) = o;
+                }
+            } else {
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int px = x0 + 8 * (r >> 2) + 4 * half + (r & 3);
-                if (px >= im.W) continue;
-                const long long p = im.pix + (long long)y * im.W + px;
-                const float v = acc[q][r];
-                if (EPI == EPI_RELU_BF16) {
-                    static_cast<bf16 *>(out)[p * CH + i] = (bf16)fmaxf(v + bco, 0.0f);
-                } else if (EPI == EPI_OUT_F32) {
-                    if (i < NOUT) static_cast<float *>(out)[p * NOUT + i] = v + bco;
-                } else {
-                    const float g = (float)act_prev[p * CH + i] > 0.0f ? v : 0.0f;
-                    static_cast<bf16 *>(out)[p * CH + i] = (bf16)g;
+                for (int g = 0; g < 4; ++g) {
+                    const int co0 = 8 * g + 4 * half;
+                    bf16x4 o;
+                    if constexpr (EPI == EPI_RELU_BF16) {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) o[e] = (bf16)fmaxf(acc[q][4 * g + e] + bco[4 * g + e], 0.0f);
+                    } else {
+                        const bf16x4 ap = *reinterpret_cast<const bf16x4 *>(act_prev + p * CH + co0);
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) o[e] = (bf16)((float)ap[e] > 0.0f ? acc[q][4 * g + e] : 0.0f);
+                    }
+                    *reinterpret_cast<bf16x4 *>(static_cast<bf16 *>(out) + p * CH + co0) = o;
                 }
             }
         }
