@@ -96,31 +96,28 @@ def build_dataset(truth: FineTruth, steps, chunk: int = 64, keep=()):
     return np.concatenate(raws), np.concatenate(tgts), mask, kept
 
 
-def choose_strength(cb, net: ClosureNet, nm: Norms, start, truth_state, steps: int, ratio: float,
-                    alphas=(0.25, 0.5, 1.0)):
-    """Closure strength alpha (tendency x alpha) with the lowest SSH error of
-    a ``steps``-step rollout inside the training period; 1.0 when no
-    rollout is available.  Returns (alpha, {alpha: ssh error}, plain error)."""
-    if start is None or steps <= 0:
+def choose_strength(cb, net: ClosureNet, nm: Norms, start, truth: dict, ratio: float,
+                    alphas=(0.0625, 0.125, 0.25, 0.5)):
+    """Closure strength alpha (tendency x alpha) from rollouts inside the
+    training period: the largest alpha whose SSH error, averaged over the
+    checkpoints in ``truth`` (coarse step -> coarsened state), is no larger
+    than the plain model's and whose run stays in range; the smallest alpha
+    when none qualifies.  Returns (alpha, per-alpha scores, plain score)."""
+    if start is None or not truth:
         return 1.0, {}, None
-    tr = {steps: truth_state}
-    plain = rollout(cb, start, steps, tr, None, ratio, "fp64")["ssh"][-1]
+    steps = max(truth)
+
+    def score(r):
+        return float(np.mean(r["ssh"])) if r["bad"] == 0 and np.all(np.isfinite(r["ssh"])) else float("inf")
+    plain = score(rollout(cb, start, steps, truth, None, ratio, "fp64"))
     errs = {}
     for a in alphas:
         net.set_norm(nm.shift, nm.scale, nm.out_scale * np.float32(a))
-        errs[a] = rollout(cb, start, steps, tr, net, ratio, "fp64")["ssh"][-1]
+        r = rollout(cb, start, steps, truth, net, ratio, "fp64")
+        errs[a] = {"ssh_mean": score(r), "u_mean": float(np.mean(r["u"])), "v_mean": float(np.mean(r["v"]))}
     net.set_norm(nm.shift, nm.scale, nm.out_scale)
-    best = min(errs, key=errs.get)
-    return best, errs, plain
-
-
-def val_loss(net: ClosureNet, first: int, n: int) -> float:
-    """Mean loss over held-out device samples first..first+n-1 (batches of net.batch)."""
-    losses = []
-    for i0 in range(first, first + n - net.batch + 1, net.batch):
-        idx = np.arange(i0, i0 + net.batch)
-        losses.append(net.eval(idx))
-    return float(np.mean(losses)) if losses else float("nan")
+    ok = [a for a in alphas if errs[a]["ssh_mean"] <= plain]
+    return (max(ok) if ok else min(alphas)), errs, plain
 
 
 def main(argv=None):
@@ -137,7 +134,7 @@ def main(argv=None):
                     help="ensemble fallback: closure norm above ratio x plain viscous tendency norm")
     ap.add_argument("--rollout-ratio", type=float, default=0.0,
                     help="fallback ratio of the single-basin rollouts (0 = closure always applied)")
-    ap.add_argument("--tune-steps", type=int, default=1000, help="coarse steps of the strength-selection rollout")
+    ap.add_argument("--tune-steps", type=int, default=4000, help="coarse steps of the strength-selection rollout")
     ap.add_argument("--members", type=int, default=128)
     ap.add_argument("--member-steps", type=int, default=2000)
     ap.add_argument("--workload", default="cfg1")
@@ -162,11 +159,13 @@ def main(argv=None):
     steps = [f + k * args.every for k in range(args.snapshots)]
     ntrain = max(args.batch, int(round(args.train_frac * len(steps))))
     # strength selection: a rollout from the middle of the training period,
-    # scored at a later training snapshot (snapshots are every `every` fine steps)
+    # scored at later training snapshots every `score_every` coarse steps
+    # (snapshots are every `every` fine steps)
     v0 = ntrain // 2
-    dv = min(args.tune_steps * f // args.every, ntrain - 1 - v0)
-    tune_steps = dv * args.every // f if dv > 0 else 0
-    raw, tgt, mask, kept = build_dataset(truth, steps, keep={v0, v0 + dv} if tune_steps else ())
+    per_mark = args.score_every * f // args.every if (args.score_every * f) % args.every == 0 else 0
+    nmarks = min(args.tune_steps // args.score_every, (ntrain - 1 - v0) // per_mark) if per_mark else 0
+    tune_idx = [v0 + per_mark * j for j in range(1, nmarks + 1)]
+    raw, tgt, mask, kept = build_dataset(truth, steps, keep={v0, *tune_idx} if tune_idx else ())
     n = raw.shape[0]
     nm = fit_norms(cb, raw[:ntrain], tgt[:ntrain])
     res["dataset"] = {"samples": n, "train": ntrain, "validation": n - ntrain, "seconds": T() - t0,
@@ -187,11 +186,13 @@ def main(argv=None):
     train_s = T() - t0
     vl = val_loss(net, ntrain, n - ntrain)
     rr = args.rollout_ratio if args.rollout_ratio > 0 else 1e300
-    alpha, aerr, perr = choose_strength(cb, net, nm, kept.get(v0), kept[v0 + dv].now if tune_steps else None,
-                                        tune_steps, rr)
-    res["strength"] = {"alpha": alpha, "tune_steps": tune_steps, "ssh_err_by_alpha": {str(k): v for k, v in aerr.items()},
-                       "ssh_err_plain": perr, "start_step": steps[v0] if tune_steps else None}
-    log(f"strength: alpha {alpha} (ssh err {aerr}, plain {perr}) over {tune_steps} coarse steps")
+    tune_truth = {args.score_every * (j + 1): kept[ix].now for j, ix in enumerate(tune_idx)}
+    alpha, aerr, perr = choose_strength(cb, net, nm, kept.get(v0) if tune_idx else None, tune_truth, rr)
+    res["strength"] = {"alpha": alpha, "tune_steps": max(tune_truth) if tune_truth else 0,
+                       "by_alpha": {str(k): v for k, v in aerr.items()}, "ssh_mean_plain": perr,
+                       "start_step": steps[v0] if tune_idx else None,
+                       "rule": "largest alpha whose mean SSH error over the window is <= plain's"}
+    log(f"strength: alpha {alpha} (scores {aerr}, plain ssh mean {perr}) over {res['strength']['tune_steps']} coarse steps")
     nm = Norms(nm.shift, nm.scale, (nm.out_scale * np.float32(alpha)).astype(np.float32))
     net.set_norm(nm.shift, nm.scale, nm.out_scale)
     res["training"] = {"steps": args.train_steps, "seconds": train_s, "ms_per_step": 1e3 * train_s / args.train_steps,
