@@ -120,6 +120,14 @@ def choose_strength(cb, net: ClosureNet, nm: Norms, start, truth: dict, ratio: f
     return (max(ok) if ok else min(alphas)), errs, plain
 
 
+def val_loss(net: ClosureNet, first: int, n: int) -> float:
+    """Mean loss over held-out device samples first..first+n-1 (batches of net.batch)."""
+    losses = []
+    for i0 in range(first, first + n - net.batch + 1, net.batch):
+        losses.append(net.eval(np.arange(i0, i0 + net.batch)))
+    return float(np.mean(losses)) if losses else float("nan")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
     ap.add_argument("--snapshots", type=int, default=2000)
