@@ -217,6 +217,7 @@ def learned_leg(fine, rollout_steps, members, member_steps, ratio=3.0, rollout_r
     truth = FineTruth(fine)
     cb = truth.coarse
     net = ClosureNet(1, cb.ny, cb.nx)
+    net.set_output(meta.get("output", "uv"))
     net.set_params(params)
     net.set_norm(nm.shift, nm.scale, nm.out_scale)
     out = {"params": os.path.relpath(CLOSURE_PARAMS, ROOT), "trained": meta}

@@ -243,6 +243,16 @@ int inmsom_bt_cnn_num_params(inmsom_bt_cnn *net, long long *n);
  * tendency = output * out_scale */
 int inmsom_bt_cnn_set_norm(inmsom_bt_cnn *net, const float *in_shift, const float *in_scale,
                            const float *out_scale);
+/* what the network's outputs mean: INMSOM_BT_CNN_UV (default) the u and v
+ * tendencies (outputs 0, 1 times out_scale[0], out_scale[1]);
+ * INMSOM_BT_CNN_PSI a stream-function tendency psi on H points (output 0
+ * times out_scale[0]) whose differences are the tendencies,
+ * du(m, n) = -(psi(m, n) - psi(m, n-1)), dv(m, n) = psi(m, n) - psi(m-1, n):
+ * divergence-free on the C-grid (the grid step is folded into out_scale).
+ * Applies to training (loss) and to the closure in the step. */
+#define INMSOM_BT_CNN_UV 0
+#define INMSOM_BT_CNN_PSI 1
+int inmsom_bt_cnn_set_output(inmsom_bt_cnn *net, int kind);
 /* set (and reset the Adam moments) / read the FP32 master parameters */
 int inmsom_bt_cnn_set_params(inmsom_bt_cnn *net, const float *params);
 int inmsom_bt_cnn_get_params(inmsom_bt_cnn *net, float *params);

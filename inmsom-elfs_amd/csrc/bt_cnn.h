@@ -25,10 +25,28 @@ struct Img {
 };
 
 // Input normalisation x_c = (field_c - shift_c) * scale_c for (u, v, eta,
-// depth); the network's outputs times oscale are the u and v tendencies [m/s^2].
+// depth); the network's outputs times oscale are the u and v tendencies
+// [m/s^2] (kind OUT_UV), or output 0 times oscale[0] is a stream-function
+// tendency psi on H points whose differences give them (kind OUT_PSI):
+//   du(m, n) = -(psi(m, n) - psi(m, n-1)),  dv(m, n) = psi(m, n) - psi(m-1, n)
+// (divergence-free; the grid step is folded into oscale).
+enum { OUT_UV = 0, OUT_PSI = 1 };
 struct Norm {
     float shift[NIN], scale[NIN], oscale[NOUT];
+    int kind;
 };
+
+// u and v tendencies (normalised units) at pixel (x, y) of an H x W image
+// whose channel-0 output is psi at stride `st` floats per pixel.
+__host__ __device__ inline void psi_uv(const float *psi, int st, int H, int W, int x, int y, float *du, float *dv)
+{
+    (void)H;
+    const float c = psi[((long long)y * W + x) * st];
+    const float s = y > 0 ? psi[((long long)(y - 1) * W + x) * st] : 0.0f;
+    const float w = x > 0 ? psi[((long long)y * W + x - 1) * st] : 0.0f;
+    *du = -(c - s);
+    *dv = c - w;
+}
 
 // One basin the closure acts on.  u, v, s, H are the basin's (ny, nx)
 // arrays (float or double as the owning step stores them); the closure

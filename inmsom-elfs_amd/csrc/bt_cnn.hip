@@ -323,24 +323,50 @@ __global__ void k_reduce_wgrad(int ngroups, int ncol, int kvalid, int kpad, cons
     }
 }
 
-// Masked mean-square loss over U (channel 0) and V (channel 1) points:
-// writes dL/dpred into channels 0..1 of dpred (channels 2..31 are zeroed) and
-// per-block partial sums of the squared error.
-__global__ void k_loss(long long n, const float *__restrict__ pred, const float *__restrict__ tgt,
-                       const uint8_t *__restrict__ mask, float inv_count, bf16 *__restrict__ dpred,
-                       float *__restrict__ part)
+// Masked mean-square loss over U (channel 0) and V (channel 1) points of
+// a batch of H x W images: writes dL/dpred into dpred (32 channels, the
+// unused ones zeroed) and per-block partial sums of the squared error.
+// OUT_PSI: the tendencies are the stream-function differences (psi_uv) and
+// dL/dpsi is their adjoint: psi(x, y) enters du(x, y), du(x, y+1),
+// dv(x, y) and dv(x+1, y).
+__global__ void k_loss(long long n, int H, int W, int kind, const float *__restrict__ pred,
+                       const float *__restrict__ tgt, const uint8_t *__restrict__ mask, float inv_count,
+                       bf16 *__restrict__ dpred, float *__restrict__ part)
 {
     __shared__ float red[256];
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     float e2 = 0.0f;
     if (p < n) {
         const uint8_t mk = mask[p];
+        if (kind == OUT_UV) {
 #pragma unroll
-        for (int c = 0; c < NOUT; ++c) {
-            const float on = (mk >> c) & 1 ? 1.0f : 0.0f;
-            const float d = (pred[p * NOUT + c] - tgt[p * NOUT + c]) * on;
-            e2 += d * d;
-            dpred[p * CH + c] = (bf16)(2.0f * d * inv_count);
+            for (int c = 0; c < NOUT; ++c) {
+                const float on = (mk >> c) & 1 ? 1.0f : 0.0f;
+                const float d = (pred[p * NOUT + c] - tgt[p * NOUT + c]) * on;
+                e2 += d * d;
+                dpred[p * CH + c] = (bf16)(2.0f * d * inv_count);
+            }
+        } else {
+            const long long hw = (long long)H * W;
+            const long long img = p / hw;
+            const int r = (int)(p - img * hw), y = r / W, x = r - y * W;
+            const float *ps = pred + img * hw * NOUT;
+            const float *tg = tgt + img * hw * NOUT;
+            const uint8_t *mi = mask + img * hw;
+            // residual on U / V points of pixel (xx, yy) of this image
+            auto res = [&](int xx, int yy, int c) -> float {
+                const long long q = (long long)yy * W + xx;
+                if (!((mi[q] >> c) & 1)) return 0.0f;
+                float du, dv;
+                psi_uv(ps, NOUT, H, W, xx, yy, &du, &dv);
+                return (c == 0 ? du : dv) - tg[q * NOUT + c];
+            };
+            const float ru = res(x, y, 0), rv = res(x, y, 1);
+            e2 = ru * ru + rv * rv;
+            const float ru_n = y + 1 < H ? res(x, y + 1, 0) : 0.0f;
+            const float rv_e = x + 1 < W ? res(x + 1, y, 1) : 0.0f;
+            dpred[p * CH + 0] = (bf16)(2.0f * inv_count * (-ru + ru_n + rv - rv_e));
+            dpred[p * CH + 1] = (bf16)0.0f;
         }
         for (int c = NOUT; c < CH; ++c) dpred[p * CH + c] = (bf16)0.0f;
     }
@@ -394,6 +420,22 @@ __device__ inline double ld(const void *p, long long i)
     return (double)static_cast<const T *>(p)[i];
 }
 
+// closure tendency [m/s^2] on the U and V points of member pixel (x, y)
+__device__ inline void member_tend(const ClosureMember &m, const Norm &nm, const float *__restrict__ tend, int x,
+                                   int y, double *cu, double *cv)
+{
+    if (nm.kind == OUT_UV) {
+        const long long q = (m.pix + (long long)y * m.nx + x) * NOUT;
+        *cu = (double)tend[q] * nm.oscale[0];
+        *cv = (double)tend[q + 1] * nm.oscale[1];
+    } else {
+        float du, dv;
+        psi_uv(tend + m.pix * NOUT, NOUT, m.ny, m.nx, x, y, &du, &dv);
+        *cu = (double)du * nm.oscale[0];
+        *cv = (double)dv * nm.oscale[0];
+    }
+}
+
 // per-row squared norms of the closure tendency and of the plain viscous
 // tendency nu * Laplacian(u, v) on U and V points
 template <class T>
@@ -411,15 +453,17 @@ __global__ void __launch_bounds__(256)
         for (int i = 1 + threadIdx.x; i < m.nx - 1; i += blockDim.x) {
             const long long a = (long long)y * m.nx + i;
             const uint8_t b = m.mb[a];
+            double tu, tv;
+            member_tend(m, nm, tend, i, y, &tu, &tv);
             if (b & MB_LCU) {
-                const double c = (double)tend[(m.pix + a) * NOUT + 0] * nm.oscale[0];
+                const double c = tu;
                 const double l = k * (ld<T>(m.u, a - 1) + ld<T>(m.u, a + 1) + ld<T>(m.u, a - m.nx) +
                                       ld<T>(m.u, a + m.nx) - 4.0 * ld<T>(m.u, a));
                 sc += c * c;
                 sv += l * l;
             }
             if (b & MB_LCV) {
-                const double c = (double)tend[(m.pix + a) * NOUT + 1] * nm.oscale[1];
+                const double c = tv;
                 const double l = k * (ld<T>(m.v, a - 1) + ld<T>(m.v, a + 1) + ld<T>(m.v, a - m.nx) +
                                       ld<T>(m.v, a + m.nx) - 4.0 * ld<T>(m.v, a));
                 sc += c * c;
@@ -491,8 +535,10 @@ __global__ void __launch_bounds__(256)
     for (int i = threadIdx.x; i < m.nx; i += blockDim.x) {
         const long long a = (long long)y * m.nx + i;
         const uint8_t b = m.mb[a];
-        if (b & MB_LCU) u[a] = (T)((double)u[a] + dt * ((double)tend[(m.pix + a) * NOUT + 0] * nm.oscale[0]));
-        if (b & MB_LCV) v[a] = (T)((double)v[a] + dt * ((double)tend[(m.pix + a) * NOUT + 1] * nm.oscale[1]));
+        double tu, tv;
+        member_tend(m, nm, tend, i, y, &tu, &tv);
+        if (b & MB_LCU) u[a] = (T)((double)u[a] + dt * tu);
+        if (b & MB_LCV) v[a] = (T)((double)v[a] + dt * tv);
     }
 }
 
@@ -673,7 +719,8 @@ int backward(inmsom_bt_cnn *c, long long count, double *loss)
     const long long np = (long long)c->B * c->H * c->W;
     const float inv = count > 0 ? 1.0f / (float)count : 0.0f;
     const int nb = (int)((np + 255) / 256);
-    hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, c->stream, np, c->pred, c->tgt, c->mask, inv, c->dy,
+    hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, c->stream, np, c->H, c->W, c->norm.kind, c->pred, c->tgt,
+                       c->mask, inv, c->dy,
                        c->loss_part);
     wgrad_launch(c, 3, c->dy, c->act[2]);
     conv_launch(c, c->stream, 3, true, c->imgs, c->tile_img, c->tile_local, c->ntiles, c->dy, c->act[2], c->dx);
@@ -803,6 +850,7 @@ int inmsom_bt_cnn_create(int batch, int ny, int nx, int device, inmsom_bt_cnn **
         c->norm.scale[k] = 1.0f;
     }
     c->norm.oscale[0] = c->norm.oscale[1] = 1.0f;
+    c->norm.kind = OUT_UV;
     int rc = [&]() -> int {
         CHC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         CTRY(dmalloc(&c->wm, c->nparam));
@@ -878,6 +926,14 @@ int inmsom_bt_cnn_set_norm(inmsom_bt_cnn *c, const float *in_shift, const float 
     }
     c->norm.oscale[0] = out_scale[0];
     c->norm.oscale[1] = out_scale[1];
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_cnn_set_output(inmsom_bt_cnn *c, int kind)
+{
+    if (!c) return cfail("null pointer", INMSOM_BT_EINVAL);
+    if (kind != OUT_UV && kind != OUT_PSI) return cfail("unknown output kind", INMSOM_BT_EINVAL);
+    c->norm.kind = kind;
     return INMSOM_BT_OK;
 }
 
@@ -1044,7 +1100,8 @@ int inmsom_bt_cnn_eval(inmsom_bt_cnn *c, const int *idx, double *loss)
     forward_batch(c);
     const long long np = (long long)c->B * c->H * c->W;
     const int nb = (int)((np + 255) / 256);
-    hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, c->stream, np, c->pred, c->tgt, c->mask,
+    hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, c->stream, np, c->H, c->W, c->norm.kind, c->pred, c->tgt,
+                       c->mask,
                        count > 0 ? 1.0f / (float)count : 0.0f, c->dy, c->loss_part);
     CHC(hipGetLastError());
     std::vector<float> part(nb);

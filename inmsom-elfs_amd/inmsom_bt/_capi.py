@@ -78,6 +78,7 @@ SIGNATURES = {
     "inmsom_bt_cnn_destroy": ([C.c_void_p], C.c_int),
     "inmsom_bt_cnn_num_params": ([C.c_void_p, C.POINTER(C.c_longlong)], C.c_int),
     "inmsom_bt_cnn_set_norm": ([C.c_void_p, _fp, _fp, _fp], C.c_int),
+    "inmsom_bt_cnn_set_output": ([C.c_void_p, C.c_int], C.c_int),
     "inmsom_bt_cnn_set_params": ([C.c_void_p, _fp], C.c_int),
     "inmsom_bt_cnn_get_params": ([C.c_void_p, _fp], C.c_int),
     "inmsom_bt_cnn_get_grad": ([C.c_void_p, _fp], C.c_int),

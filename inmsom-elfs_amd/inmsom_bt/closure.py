@@ -22,6 +22,7 @@ from . import _capi
 from ._capi import check, lib
 
 CIN0, CH, NIN, NOUT = 8, 32, 4, 2
+OUTPUT_KINDS = {"uv": 0, "psi": 1}
 
 
 def _f(a: np.ndarray):
@@ -51,6 +52,7 @@ class ClosureNet:
         self.scale = np.ones(NIN, dtype=np.float32)
         self.out_scale = np.ones(NOUT, dtype=np.float32)
         self.nsamples = 0
+        self.kind = "uv"
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -76,6 +78,13 @@ class ClosureNet:
         self.out_scale = np.ascontiguousarray(out_scale, dtype=np.float32)
         check(lib().inmsom_bt_cnn_set_norm(self._h, _f(self.shift), _f(self.scale), _f(self.out_scale)),
               "inmsom_bt_cnn_set_norm")
+
+    def set_output(self, kind: str = "uv"):
+        """'uv': outputs are the u, v tendencies; 'psi': output 0 is a
+        stream-function tendency on H points, differenced into
+        divergence-free u, v tendencies (inmsom_bt_cnn_set_output)."""
+        check(lib().inmsom_bt_cnn_set_output(self._h, OUTPUT_KINDS[kind]), "inmsom_bt_cnn_set_output")
+        self.kind = kind
 
     def set_params(self, p: np.ndarray):
         p = np.ascontiguousarray(p, dtype=np.float32)

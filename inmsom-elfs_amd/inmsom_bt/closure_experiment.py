@@ -148,6 +148,8 @@ def main(argv=None):
     ap.add_argument("--workload", default="cfg1")
     ap.add_argument("--out", default="gpurun_out/closure")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--output", default="uv", choices=["uv", "psi"],
+                    help="network outputs: u, v tendencies, or a stream-function tendency (divergence-free)")
     args = ap.parse_args(argv)
     os.makedirs(args.out, exist_ok=True)
     res: dict = {"args": vars(args)}
@@ -176,6 +178,9 @@ def main(argv=None):
     raw, tgt, mask, kept = build_dataset(truth, steps, keep={v0, *tune_idx} if tune_idx else ())
     n = raw.shape[0]
     nm = fit_norms(cb, raw[:ntrain], tgt[:ntrain])
+    if args.output == "psi":
+        # one scale for psi: both tendencies are its differences
+        nm.out_scale[:] = np.float32(np.sqrt(np.mean(nm.out_scale.astype(np.float64) ** 2)))
     res["dataset"] = {"samples": n, "train": ntrain, "validation": n - ntrain, "seconds": T() - t0,
                       "norm_shift": nm.shift.tolist(), "norm_scale": nm.scale.tolist(),
                       "target_rms": nm.out_scale.tolist()}
@@ -183,6 +188,7 @@ def main(argv=None):
 
     # ---- 3: training
     net = ClosureNet(args.batch, cb.ny, cb.nx)
+    net.set_output(args.output)
     net.set_norm(nm.shift, nm.scale, nm.out_scale)
     net.set_params(init_params(args.seed, net.nparam, out_scale=0.1))
     net.dataset_alloc(n)
@@ -214,7 +220,7 @@ def main(argv=None):
     save_params(os.path.join(args.out, "closure_params.npz"), net, nm,
                 {"workload": args.workload, "snapshots": n, "every": args.every, "train": ntrain,
                  "train_steps": args.train_steps, "lr": args.lr, "batch": args.batch, "t_start": t_start,
-                 "val_loss": vl, "seed": args.seed, "alpha": alpha})
+                 "val_loss": vl, "seed": args.seed, "alpha": alpha, "output": args.output})
 
     # ---- 4: rollout against the coarsened truth
     start = snapshots(truth, [t_start])[0]

@@ -129,6 +129,25 @@ def forward(params: np.ndarray, x: np.ndarray, quant: bool = True):
     return pred, acts
 
 
+def psi_uv(psi: np.ndarray):
+    """Stream-function output (channel 0, H points) -> u, v tendencies:
+    du(x, y) = -(psi(x, y) - psi(x, y-1)), dv(x, y) = psi(x, y) - psi(x-1, y),
+    zero psi outside the image; psi [B, H, W]."""
+    s = np.zeros_like(psi)
+    s[:, 1:, :] = psi[:, :-1, :]
+    w = np.zeros_like(psi)
+    w[:, :, 1:] = psi[:, :, :-1]
+    return -(psi - s), psi - w
+
+
+def tendencies(pred: np.ndarray, kind: str = "uv") -> np.ndarray:
+    """Network outputs [B, H, W, 2] -> u, v tendencies [B, H, W, 2] (normalised)."""
+    if kind == "uv":
+        return pred
+    du, dv = psi_uv(pred[..., 0])
+    return np.stack([du, dv], axis=-1)
+
+
 def loss(pred: np.ndarray, target: np.ndarray, mask: np.ndarray) -> Tuple[float, np.ndarray, int]:
     """(loss, d loss / d pred, count) of the masked mean-square error."""
     m = np.stack([(mask & 1) != 0, (mask & 2) != 0], axis=-1).astype(np.float64)
@@ -138,17 +157,27 @@ def loss(pred: np.ndarray, target: np.ndarray, mask: np.ndarray) -> Tuple[float,
 
 
 def backward(params: np.ndarray, x: np.ndarray, target: np.ndarray, mask: np.ndarray,
-             quant: bool = True):
-    """(loss, gradient vector in the parameter layout)."""
+             quant: bool = True, kind: str = "uv"):
+    """(loss, gradient vector in the parameter layout).  kind 'psi': the loss
+    is on tendencies(pred, 'psi') and dL/dpsi is the adjoint of psi_uv."""
     q = bf16 if quant else (lambda a: np.asarray(a, dtype=np.float64))
     pred, acts = forward(params, x, quant)
     m = np.stack([(mask & 1) != 0, (mask & 2) != 0], axis=-1)
     count = int(m.sum())
-    d = (pred - target) * m
+    d = (tendencies(pred, kind) - target) * m
     lval = float((d * d).sum() / max(count, 1))
+    if kind == "psi":
+        # psi(x, y) enters du(x, y) (-1), du(x, y+1) (+1), dv(x, y) (+1), dv(x+1, y) (-1)
+        ru, rv = d[..., 0], d[..., 1]
+        ru_n = np.zeros_like(ru)
+        ru_n[:, :-1, :] = ru[:, 1:, :]
+        rv_e = np.zeros_like(rv)
+        rv_e[:, :, :-1] = rv[:, :, 1:]
+        adj = -ru + ru_n + rv - rv_e
+        d = np.stack([adj, np.zeros_like(adj)], axis=-1)
     if quant:
         inv = np.float32(1.0) / np.float32(max(count, 1))
-        g = bf16((np.float32(2.0) * d.astype(np.float32) * inv).astype(np.float32))
+        g = bf16((np.float32(2.0) * inv * d.astype(np.float32)).astype(np.float32))
     else:
         g = 2.0 * d / max(count, 1)
     gz = np.zeros(pred.shape[:3] + (CH,))

@@ -51,11 +51,11 @@ def test_bf16_rounding_matches_torch():
     np.testing.assert_array_equal(co.bf16(a), want)
 
 
-@pytest.mark.parametrize("seed", [0, 1])
-def test_backward_matches_finite_differences(seed):
+@pytest.mark.parametrize("seed,kind", [(0, "uv"), (1, "uv"), (2, "psi")])
+def test_backward_matches_finite_differences(seed, kind):
     p, x, tgt, mask = _case(seed)
     p64 = p.astype(np.float64)
-    _, g = co.backward(p64, x.astype(np.float64), tgt, mask, quant=False)
+    _, g = co.backward(p64, x.astype(np.float64), tgt, mask, quant=False, kind=kind)
     rng = np.random.default_rng(seed + 10)
     lay = co.layout()
     idx = []
@@ -72,9 +72,9 @@ def test_backward_matches_finite_differences(seed):
     for i in idx:
         pp = p64.copy()
         pp[i] += eps
-        lp = co.loss(co.forward(pp, x, quant=False)[0], tgt, mask)[0]
+        lp = co.loss(co.tendencies(co.forward(pp, x, quant=False)[0], kind), tgt, mask)[0]
         pp[i] -= 2 * eps
-        lm = co.loss(co.forward(pp, x, quant=False)[0], tgt, mask)[0]
+        lm = co.loss(co.tendencies(co.forward(pp, x, quant=False)[0], kind), tgt, mask)[0]
         fd = (lp - lm) / (2 * eps)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
 
@@ -111,3 +111,17 @@ def test_adam_first_step_closed_form():
     np.testing.assert_allclose(p1, p - 1e-3 * g / (np.abs(g) + 1e-8), rtol=1e-6, atol=1e-9)
     np.testing.assert_allclose(m, 0.1 * g, rtol=1e-6)
     np.testing.assert_allclose(v, 0.001 * g * g, rtol=1e-5)
+
+
+def test_psi_tendencies_are_divergence_free():
+    """Interior T cells: du(x, y) - du(x-1, y) + dv(x, y) - dv(x, y-1) = 0 for
+    any psi (the C-grid divergence of the stream-function tendencies)."""
+    rng = np.random.default_rng(7)
+    psi = rng.standard_normal((2, 9, 11))
+    du, dv = co.psi_uv(psi)
+    div = du[:, 1:, 1:] - du[:, 1:, :-1] + dv[:, 1:, 1:] - dv[:, :-1, 1:]
+    assert np.abs(div).max() < 1e-12
+    # and psi = y (a uniform zonal flow) gives du = -1, dv = 0 inside
+    yy = np.broadcast_to(np.arange(9.0)[None, :, None], (1, 9, 11))
+    du, dv = co.psi_uv(yy)
+    assert np.all(du[:, 1:, :] == -1.0) and np.all(dv[:, :, 1:] == 0.0)

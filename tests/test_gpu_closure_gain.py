@@ -38,6 +38,7 @@ def test_trained_closure_beats_plain_viscosity():
     tr = {STEPS: truth.coarse_state()}
     truth.close()
     with ClosureNet(1, cb.ny, cb.nx) as net:
+        net.set_output(meta.get("output", "uv"))
         net.set_params(params)
         net.set_norm(nm.shift, nm.scale, nm.out_scale)
         plain = rollout(cb, start, STEPS, tr, None, mode="fp64")

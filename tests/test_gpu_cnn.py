@@ -228,3 +228,47 @@ def test_ensemble_closure_fallback_and_graph_replays():
                 a, b = g.get_state(k), d.get_state(k)
                 for f in a:
                     assert np.array_equal(a[f], b[f]), (rep, k, f)
+
+
+@pytest.mark.parametrize("b,h,w", [(1, 8, 32), (2, 13, 45)])
+def test_gradient_psi_output_matches_oracle(b, h, w):
+    """Stream-function outputs: loss on the differenced tendencies, gradient
+    through their adjoint."""
+    p = _params(21)
+    x, tgt, mask = _batch(22, b, h, w)
+    with ClosureNet(b, h, w) as net:
+        net.set_output("psi")
+        net.set_params(p)
+        net.set_batch(x, tgt, mask)
+        loss = net.loss_grad()
+        g = net.grad()
+    wl, wg = co.backward(p, x, tgt, mask, quant=True, kind="psi")
+    assert abs(loss - wl) <= 1e-2 * abs(wl), (loss, wl)
+    for l, (woff, boff, kp) in enumerate(co.layout()):
+        assert rel_l2(g[woff:woff + 32 * kp], wg[woff:woff + 32 * kp]) <= GRAD_TOL, l
+        assert rel_l2(g[boff:boff + 32], wg[boff:boff + 32]) <= GRAD_TOL, l
+
+
+def test_ensemble_closure_psi_matches_oracle():
+    """Stream-function closure in the step: u, v change by dt times the
+    differenced psi on U/V points."""
+    basins = _closure_members()
+    p = co.init_params(14, out_scale=0.3)
+    dt = basins[0].dt
+    with Ensemble(basins, mode="fp64") as plain, Ensemble(basins, mode="fp64") as cl, \
+            ClosureNet(1, 8, 32) as net:
+        net.set_output("psi")
+        net.set_params(p)
+        net.set_norm(**NORM)
+        cl.set_closure(net, ratio=1e30)
+        plain.run(1, nstep=2)
+        cl.run(1, nstep=2)
+        for k, bn in enumerate(basins):
+            a, b = cl.get_state(k), plain.get_state(k)
+            x = features(b["u"], b["v"], b["ssh"], bn.hhq_rest, NORM["shift"], NORM["scale"])
+            t = co.tendencies(co.forward(p, x[None], quant=True)[0], "psi")[0]
+            lcu, lcv = _lcu_lcv(bn)
+            du = dt * NORM["out_scale"][0] * t[..., 0] * lcu
+            dv = dt * NORM["out_scale"][0] * t[..., 1] * lcv
+            assert rel_l2(a["u"] - b["u"], du) <= FWD_TOL, rel_l2(a["u"] - b["u"], du)
+            assert rel_l2(a["v"] - b["v"], dv) <= FWD_TOL, rel_l2(a["v"] - b["v"], dv)
