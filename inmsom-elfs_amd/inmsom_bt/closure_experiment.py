@@ -97,7 +97,7 @@ def build_dataset(truth: FineTruth, steps, chunk: int = 64, keep=()):
 
 
 def choose_strength(cb, net: ClosureNet, nm: Norms, start, truth: dict, ratio: float,
-                    alphas=(0.0625, 0.125, 0.25, 0.5)):
+                    alphas=(0.0625, 0.125, 0.25, 0.5, 1.0)):
     """Closure strength alpha (tendency x alpha) from rollouts inside the
     training period: the largest alpha whose SSH error, averaged over the
     checkpoints in ``truth`` (coarse step -> coarsened state), is no larger
