@@ -295,7 +295,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10000, help="timed outer steps (10 s each)")
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1"])
-    ap.add_argument("--cpu-steps", type=int, default=16, help="outer steps timed on the CPU oracle")
+    ap.add_argument("--cpu-steps", type=int, default=150,
+                    help="outer steps timed on the CPU oracle (about 15 s on 16 threads)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--modes", default="fp64,fp32,fp64_exact")
     ap.add_argument("--ensemble-steps", type=int, default=5000,

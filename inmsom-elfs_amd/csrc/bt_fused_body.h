@@ -53,10 +53,12 @@ __device__ __forceinline__ double xl(double x)  // value of column m-1
 __device__ __forceinline__ float xr(float x) { return __int_as_float(dpp_from_right(__float_as_int(x))); }
 __device__ __forceinline__ float xl(float x) { return __int_as_float(dpp_from_left(__float_as_int(x))); }
 
-// Stores of the step's outputs (read again only by the next launch).
-// Non-temporal in the FP64 modes: measured on MI355X at 2048 x 1536 they
-// take the FP64 step from 100.0 to 95.7 us per launch, leave FP32 unchanged
-// (71.5 / 71.1 us) and slow the FP32 ensemble by 8 %.
+// Stores of the step's outputs.  Measured on MI355X at 2048 x 1536 (us per
+// launch, FP64 / FP32 / 64-member FP32 ensemble): all cached 100.0 / 71.5 /
+// 577; non-temporal level-n and filtered-level stores in FP64, filtered-level
+// stores in FP32 (FP64 arrays), frozen forcing cached: 94.4 / 66.5 / 572.
+// Non-temporal FP32 level-n stores (69.0 / 612 FP32 / ensemble) and
+// non-temporal loads of the filtered level (74.3 / 603) are slower.
 template <bool NT, class T>
 __device__ __forceinline__ void st_out(void *base, size_t i, T v)
 {
@@ -150,7 +152,11 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
     const bool inx = (m >= 3 && m <= p.nx - 2);
     const bool inx1 = (m >= 2 && m <= p.nx - 1);
 
+    // store policy (measured, see st_out): level-n outputs non-temporal in
+    // the FP64 modes, the FP64 filtered level non-temporal in every mode,
+    // the frozen forcing (re-read by the next substep) cached
     constexpr bool NTS = sizeof(R) == 8;
+    constexpr bool NTS_D = true;
     Ops<R, EXACT> op(p);
     const R d = op.d;
     const R tau = R(p.tau);
@@ -374,8 +380,8 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
                 rhs_u = fx_0;
                 rhs_v = fy_0;
             } else if (p.nsub > 1 && own) {
-                st_out<NTS, R>(p.frx, io, rhs_u);
-                st_out<NTS, R>(p.fry, io, rhs_v);
+                st_out<false, R>(p.frx, io, rhs_u);
+                st_out<false, R>(p.fry, io, rhs_v);
             }
             // level n-1 U/V depths: hh_init values at the first substep,
             // the hh_shift-filtered carry afterwards (depth.f90:195-203)
@@ -514,9 +520,9 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
                 st_out<NTS, R>(p.so, i, s_new);
                 st_out<NTS, R>(p.uo, i, u_new);
                 st_out<NTS, R>(p.vo, i, v_new);
-                st_out<NTS, double>(p.spo, i, sp_new);
-                st_out<NTS, double>(p.upo, i, up_new);
-                st_out<NTS, double>(p.vpo, i, vp_new);
+                st_out<NTS_D, double>(p.spo, i, sp_new);
+                st_out<NTS_D, double>(p.upo, i, up_new);
+                st_out<NTS_D, double>(p.vpo, i, vp_new);
             }
         }
 
