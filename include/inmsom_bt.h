@@ -206,6 +206,32 @@ int inmsom_bt_ens_range_check(inmsom_bt_ens *ens, long long *bad);
 int inmsom_bt_ens_volume(inmsom_bt_ens *ens, int member, double *volume);
 
 
+/* The 2-D -> 3-D coupling that follows barotropic_dynamics in the model
+ * step, Control/dyncall.f90:185-231 (host arrays in and out, like
+ * inmsom_bt_barotropic_dynamics; 3-D arrays are (nz, ny, nx), i.e. the
+ * reference's column-major (m, n, k)).  The caller runs vertical_velocity
+ * between the two calls, as the reference does.
+ *
+ * inmsom_bt_remove_barotropic: uu(:,:,k) -= uu2d on lcu and
+ * vv(:,:,k) -= vv2d on lcv, k = 1..nz, m = 2..nx-1, n = 2..ny-1
+ * (dyncall.f90:185-199). */
+int inmsom_bt_remove_barotropic(inmsom_bt_ctx *ctx, int nz, double *uu, double *vv,
+                                const double *uu2d, const double *vv2d);
+/* inmsom_bt_internal_update (dyncall.f90:201-231): uu2d = ubrtr_i / hhu and
+ * pgrx -= (slpr(m+1) - slpr(m)) * hhu * dyh / RefDen on lcu (v likewise);
+ * ssh_internal (Function/vel_ssh.f90:682-751): ssh_new from the continuity
+ * equation with the time-mean transports ubrtr_i, vbrtr_i and wflux,
+ * hh_update of hhn = {hhqn, hhun, hhvn, hhhn} when full_free_surface > 0,
+ * and the Robert-Asselin step of (ssh_i, sshp_i) with time_smooth / 2;
+ * then uu(:,:,k) += uu2d, vv(:,:,k) += vv2d.  uu2d, vv2d, pgrx, pgry,
+ * ssh_i, sshp_i, hhn and uu, vv are read and written (cells the
+ * reference does not write keep their values). */
+int inmsom_bt_internal_update(inmsom_bt_ctx *ctx, double tau, int nz, const double *ubrtr_i,
+                              const double *vbrtr_i, const double *hhu, const double *hhv,
+                              const double *slpr, const double *wflux, double *uu2d, double *vv2d,
+                              double *pgrx, double *pgry, double *ssh_i, double *sshp_i,
+                              double *const hhn[4], double *uu, double *vv);
+
 /* Block average of the current state onto a grid `factor` times coarser
  * (what a coarse model of the same ocean sees; used to build the learned
  * closure's training targets and to judge coarse runs).  Interior blocks of

@@ -85,5 +85,7 @@ struct inmsom_bt_ctx {
     int fz_cur = 0;
     int chunk = 0;                 // output rows per wave of the fused step (0 = per-mode default)
     double *scal = nullptr;        // device scalars (reductions)
+    double *cpl = nullptr;         // staging of the 2-D -> 3-D coupling (bt_coupling.hip)
+    size_t cpl_n = 0;              // doubles in cpl
     long long *cnt = nullptr;
 };

@@ -15,5 +15,9 @@ void lit_barotropic_dynamics(inmsom_bt_ctx *c, double tau, int nstep, int ksw4, 
                              const double *RHSy, const double *wflux, const double *mu,
                              const double *mu4, const double *rdis);
 void lit_forcing(inmsom_bt_ctx *c);
+// periodic wrap of a field (no-op without periodicity) and hh_update
+// (Function/depth.f90:108-170) of (hqn, hun, hvn, hhn) from the surface sh
+void lit_cyclize(inmsom_bt_ctx *c, double *f);
+void lit_hh_update(inmsom_bt_ctx *c, const double *sh, double *hqn, double *hun, double *hvn, double *hhn);
 
 }  // namespace ibt

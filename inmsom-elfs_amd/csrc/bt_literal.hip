@@ -478,6 +478,16 @@ static void cyclize(inmsom_bt_ctx *c, double *f)
         hipLaunchKernelGGL(k_cyclize_y, dim3((c->nx + 255) / 256), dim3(256), 0, c->stream, c->g, f);
 }
 
+void lit_cyclize(inmsom_bt_ctx *c, double *f) { cyclize(c, f); }
+
+void lit_hh_update(inmsom_bt_ctx *c, const double *sh, double *hqn, double *hun, double *hvn, double *hhn)
+{
+    hipLaunchKernelGGL(k_hh_update, grid2(c->g), BLK2, 0, c->stream, c->g, sh, hqn, hun, hvn, hhn);
+    cyclize(c, hun);
+    cyclize(c, hvn);
+    cyclize(c, hhn);
+}
+
 // Device array of the 12 depth pointers (k_hh_init and k_shift index
 // it); written once when the context is created.
 static inline double *const *dep_table(inmsom_bt_ctx *c)

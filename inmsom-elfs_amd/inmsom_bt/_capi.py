@@ -60,6 +60,9 @@ SIGNATURES = {
     "inmsom_bt_volume": ([C.c_void_p, _dp], C.c_int),
     "inmsom_bt_synchronize": ([C.c_void_p], C.c_int),
     "inmsom_bt_coarsen_state": ([C.c_void_p, C.c_int, _dp, _dp, _dp], C.c_int),
+    "inmsom_bt_remove_barotropic": ([C.c_void_p, C.c_int, _dp, _dp, _dp, _dp], C.c_int),
+    "inmsom_bt_internal_update": ([C.c_void_p, C.c_double, C.c_int] + [_dp] * 12 + [C.POINTER(_dp), _dp, _dp],
+                                  C.c_int),
     "inmsom_bt_ens_create": ([C.c_int, C.POINTER(Grid), C.POINTER(Forcing), C.c_int, C.c_int,
                               C.POINTER(C.c_void_p)], C.c_int),
     "inmsom_bt_ens_destroy": ([C.c_void_p], C.c_int),

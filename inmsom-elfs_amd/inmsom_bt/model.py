@@ -199,6 +199,36 @@ class BarotropicModel:
     def synchronize(self):
         check(lib().inmsom_bt_synchronize(self._h), "inmsom_bt_synchronize")
 
+    # ------------------------------------------------------------ 2-D -> 3-D coupling
+    def remove_barotropic(self, uu: np.ndarray, vv: np.ndarray, uu2d: np.ndarray, vv2d: np.ndarray):
+        """Control/dyncall.f90:185-199 on (nz, ny, nx) arrays, in place."""
+        for a in (uu, vv):
+            if a.dtype != np.float64 or not a.flags.c_contiguous or a.shape[1:] != (self.ny, self.nx):
+                raise ValueError("uu, vv must be C-contiguous float64 (nz, ny, nx)")
+        u2, v2 = (np.ascontiguousarray(a, dtype=np.float64) for a in (uu2d, vv2d))
+        check(lib().inmsom_bt_remove_barotropic(self._h, uu.shape[0], _d(uu), _d(vv), _d(u2), _d(v2)),
+              "inmsom_bt_remove_barotropic")
+
+    def internal_update(self, tau: float, fields: dict, uu: np.ndarray, vv: np.ndarray) -> dict:
+        """Control/dyncall.f90:201-231 with ssh_internal.  ``fields`` holds
+        ubrtr_i, vbrtr_i, hhu, hhv, slpr, wflux (inputs) and uu2d, vv2d,
+        pgrx, pgry, ssh_i, sshp_i, hhqn, hhun, hhvn, hhhn (updated; copies
+        are returned); uu, vv (nz, ny, nx) are updated in place."""
+        ins = {k: np.ascontiguousarray(fields[k], dtype=np.float64)
+               for k in ("ubrtr_i", "vbrtr_i", "hhu", "hhv", "slpr", "wflux")}
+        io = {k: np.array(fields[k], dtype=np.float64, order="C")
+              for k in ("uu2d", "vv2d", "pgrx", "pgry", "ssh_i", "sshp_i", "hhqn", "hhun", "hhvn", "hhhn")}
+        hh = (_capi._dp * 4)(*[_d(io[k]) for k in ("hhqn", "hhun", "hhvn", "hhhn")])
+        for a in (uu, vv):
+            if a.dtype != np.float64 or not a.flags.c_contiguous or a.shape[1:] != (self.ny, self.nx):
+                raise ValueError("uu, vv must be C-contiguous float64 (nz, ny, nx)")
+        check(lib().inmsom_bt_internal_update(
+            self._h, float(tau), uu.shape[0], *[_d(ins[k]) for k in ("ubrtr_i", "vbrtr_i", "hhu", "hhv", "slpr",
+                                                                      "wflux")],
+            *[_d(io[k]) for k in ("uu2d", "vv2d", "pgrx", "pgry", "ssh_i", "sshp_i")], hh, _d(uu), _d(vv)),
+            "inmsom_bt_internal_update")
+        return io
+
     # ------------------------------------------------------------ diagnostics
     def coarsen(self, factor: int = 4) -> dict:
         """Block average of the current state onto a grid ``factor`` times

@@ -643,6 +643,86 @@ void bto_bottom_stress(const bto_grid *g, double cb_nl, double ebottom,
         }
 }
 
+/* Control/dyncall.f90:185-199 */
+void bto_remove_barotropic(const bto_grid *g, int nz, double *uu, double *vv,
+                           const double *uu2d, const double *vv2d)
+{
+    const size_t plane = (size_t)g->nx * g->ny;
+#pragma omp parallel for schedule(static)
+    for (int n = NYS - 1; n <= NYE + 1; ++n)
+        for (int m = NXS - 1; m <= NXE + 1; ++m) {
+            if (g->lcu[IX(m, n)] > 0.5f)
+                for (int k = 0; k < nz; ++k) uu[k * plane + IX(m, n)] = uu[k * plane + IX(m, n)] - uu2d[IX(m, n)];
+            if (g->lcv[IX(m, n)] > 0.5f)
+                for (int k = 0; k < nz; ++k) vv[k * plane + IX(m, n)] = vv[k * plane + IX(m, n)] - vv2d[IX(m, n)];
+        }
+}
+
+/* Control/dyncall.f90:201-231 (vertical_velocity excluded) with
+ * ssh_internal, Function/vel_ssh.f90:682-751 */
+void bto_internal_update(const bto_grid *g, double tau, int nz,
+                         const double *ubrtr_i, const double *vbrtr_i,
+                         const double *hhu, const double *hhv, const double *slpr,
+                         const double *wflux, double *uu2d, double *vv2d,
+                         double *pgrx, double *pgry, double *ssh_i, double *sshp_i,
+                         double *hhqn, double *hhun, double *hhvn, double *hhhn,
+                         double *uu, double *vv)
+{
+    const size_t plane = (size_t)g->nx * g->ny;
+    const double rho0 = g->ref_den;
+    /* dyncall.f90:203-217 */
+#pragma omp parallel for schedule(static)
+    for (int n = NYS - 1; n <= NYE + 1; ++n)
+        for (int m = NXS - 1; m <= NXE + 1; ++m) {
+            if (g->lcu[IX(m, n)] > 0.5f) {
+                uu2d[IX(m, n)] = ubrtr_i[IX(m, n)] / hhu[IX(m, n)];
+                pgrx[IX(m, n)] = pgrx[IX(m, n)] -
+                                 (slpr[IX(m + 1, n)] - slpr[IX(m, n)]) * hhu[IX(m, n)] * g->dyh[IX(m, n)] / rho0;
+            }
+            if (g->lcv[IX(m, n)] > 0.5f) {
+                vv2d[IX(m, n)] = vbrtr_i[IX(m, n)] / hhv[IX(m, n)];
+                pgry[IX(m, n)] = pgry[IX(m, n)] -
+                                 (slpr[IX(m, n + 1)] - slpr[IX(m, n)]) * hhv[IX(m, n)] * g->dxh[IX(m, n)] / rho0;
+            }
+        }
+    /* ssh_internal: vel_ssh.f90:705-720 */
+    double *ssh_new = (double *)calloc(plane, sizeof(double));
+    const double ffs = (double)g->full_free_surface;
+#pragma omp parallel for schedule(static)
+    for (int n = NYS; n <= NYE; ++n)
+        for (int m = NXS; m <= NXE; ++m)
+            if (g->lu[IX(m, n)] > 0.5f)
+                ssh_new[IX(m, n)] =
+                    sshp_i[IX(m, n)] +
+                    2.0 * tau *
+                        (wflux[IX(m, n)] / rho0 * ffs -
+                         (ubrtr_i[IX(m, n)] * g->dyh[IX(m, n)] - ubrtr_i[IX(m - 1, n)] * g->dyh[IX(m - 1, n)] +
+                          vbrtr_i[IX(m, n)] * g->dxh[IX(m, n)] - vbrtr_i[IX(m, n - 1)] * g->dxh[IX(m, n - 1)]) /
+                             (g->dx[IX(m, n)] * g->dy[IX(m, n)]));
+    /* :722-735 */
+    cyclize(g, ssh_new);
+    if (g->full_free_surface > 0) bto_hh_update(g, hhqn, hhun, hhvn, hhhn, ssh_new);
+    /* :737-748 */
+    const double ts = g->time_smooth;
+#pragma omp parallel for schedule(static)
+    for (int n = NYS - 1; n <= NYE + 1; ++n)
+        for (int m = NXS - 1; m <= NXE + 1; ++m)
+            if (g->lu[IX(m, n)] > 0.5f) {
+                sshp_i[IX(m, n)] = ssh_i[IX(m, n)] + ts * (ssh_new[IX(m, n)] - 2.0 * ssh_i[IX(m, n)] + sshp_i[IX(m, n)]) / 2.0;
+                ssh_i[IX(m, n)] = ssh_new[IX(m, n)];
+            }
+    free(ssh_new);
+    /* dyncall.f90:217-231 */
+#pragma omp parallel for schedule(static)
+    for (int n = NYS - 1; n <= NYE + 1; ++n)
+        for (int m = NXS - 1; m <= NXE + 1; ++m) {
+            if (g->lcu[IX(m, n)] > 0.5f)
+                for (int k = 0; k < nz; ++k) uu[k * plane + IX(m, n)] = uu[k * plane + IX(m, n)] + uu2d[IX(m, n)];
+            if (g->lcv[IX(m, n)] > 0.5f)
+                for (int k = 0; k < nz; ++k) vv[k * plane + IX(m, n)] = vv[k * plane + IX(m, n)] + vv2d[IX(m, n)];
+        }
+}
+
 /* Control/dyncall.f90:110-130 for a 2-D flow (all 3-D terms zero). */
 void bto_rhs_assembly(const bto_grid *g, const double *surf_x,
                       const double *surf_y, const double *bot_x,

@@ -132,6 +132,26 @@ void bto_model_step(const bto_grid *g, bto_depths *d, double tau, int nstep,
                     const double *wflux, const double *mu, const double *mu4,
                     const double *rdis, double *work);
 
+/* Control/dyncall.f90:185-199: the barotropic part removed from the 3-D
+ * velocity, uu(:,:,k) -= uu2d on lcu and vv(:,:,k) -= vv2d on lcv for
+ * k = 1..nz over m = 2..nx-1, n = 2..ny-1; uu, vv are (nz, ny, nx). */
+void bto_remove_barotropic(const bto_grid *g, int nz, double *uu, double *vv,
+                           const double *uu2d, const double *vv2d);
+
+/* Control/dyncall.f90:201-231 without vertical_velocity: uu2d = ubrtr_i/hhu
+ * and pgrx -= (slpr(m+1)-slpr(m))*hhu*dyh/RefDen on lcu (v likewise on
+ * lcv); ssh_internal (Function/vel_ssh.f90:682-751): continuity with the
+ * time-mean transports into ssh_new, hh_update of (hhqn, hhun, hhvn, hhhn)
+ * from it when full_free_surface > 0, and the Asselin step of
+ * (ssh_i, sshp_i) with time_smooth/2; then uu(:,:,k) += uu2d, vv += vv2d. */
+void bto_internal_update(const bto_grid *g, double tau, int nz,
+                         const double *ubrtr_i, const double *vbrtr_i,
+                         const double *hhu, const double *hhv, const double *slpr,
+                         const double *wflux, double *uu2d, double *vv2d,
+                         double *pgrx, double *pgry, double *ssh_i, double *sshp_i,
+                         double *hhqn, double *hhun, double *hhvn, double *hhhn,
+                         double *uu, double *vv);
+
 /* Wet-cell volume anomaly sum(eta * dx * dy) over lu interior cells,
  * summed row by row in a fixed order. */
 double bto_volume(const bto_grid *g, const double *ssh);

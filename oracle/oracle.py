@@ -226,6 +226,29 @@ UNIT_LAYOUT = ([("init_" + k) for k in DEPTHS]
                   "surf_x", "surf_y", "bottom_friction", "bot_x", "bot_y"])
 
 
+# 2-D -> 3-D coupling (Control/dyncall.f90:185-231, ssh_internal)
+COUPLING_IN = ("ubrtr_i", "vbrtr_i", "hhu", "hhv", "slpr", "wflux", "uu2d", "vv2d", "pgrx", "pgry",
+               "ssh_i", "sshp_i")
+COUPLING_OUT = ("uu2d", "vv2d", "pgrx", "pgry", "ssh_i", "sshp_i", "hhqn", "hhun", "hhvn", "hhhn")
+
+
+def coupling(og: "OracleGrid", tau: float, inp: Dict[str, np.ndarray]) -> Dict[str, np.ndarray]:
+    """bto_remove_barotropic then bto_internal_update on copies of ``inp``
+    (the fields of COUPLING_IN plus 3-D "uu", "vv" of shape (nz, ny, nx))."""
+    L = lib()
+    w = {k: np.ascontiguousarray(inp[k], dtype=np.float64).copy() for k in inp}
+    nz = w["uu"].shape[0]
+    hn = {k: np.zeros_like(w["ssh_i"]) for k in ("hhqn", "hhun", "hhvn", "hhhn")}
+    L.bto_remove_barotropic(C.byref(og.g), nz, _d(w["uu"]), _d(w["vv"]), _d(w["uu2d"]), _d(w["vv2d"]))
+    L.bto_internal_update(C.byref(og.g), C.c_double(tau), nz, _d(w["ubrtr_i"]), _d(w["vbrtr_i"]), _d(w["hhu"]),
+                          _d(w["hhv"]), _d(w["slpr"]), _d(w["wflux"]), _d(w["uu2d"]), _d(w["vv2d"]),
+                          _d(w["pgrx"]), _d(w["pgry"]), _d(w["ssh_i"]), _d(w["sshp_i"]), _d(hn["hhqn"]),
+                          _d(hn["hhun"]), _d(hn["hhvn"]), _d(hn["hhhn"]), _d(w["uu"]), _d(w["vv"]))
+    out = {k: w[k] for k in ("uu", "vv", "uu2d", "vv2d", "pgrx", "pgry", "ssh_i", "sshp_i")}
+    out.update(hn)
+    return out
+
+
 def have_reference() -> bool:
     return os.path.exists(REF_HARNESS)
 
@@ -241,7 +264,16 @@ def run_reference(case_path: str, out_path: str, nx: int, ny: int, mode: int,
     arrs = [raw[i * per:(i + 1) * per].reshape(ny, nx) for i in range(raw.size // per)]
     res = {"masks": dict(zip(MASKS, arrs[:7]))}
     arrs = arrs[7:]
-    if mode == 0:
+    if mode == 2:
+        nz = (len(arrs) - len(COUPLING_IN) - len(COUPLING_OUT)) // 4
+        k = len(COUPLING_IN)
+        inp = dict(zip(COUPLING_IN, arrs[:k]))
+        inp["uu"] = np.stack(arrs[k:k + nz]); inp["vv"] = np.stack(arrs[k + nz:k + 2 * nz])
+        k += 2 * nz
+        out = {"uu": np.stack(arrs[k:k + nz]), "vv": np.stack(arrs[k + nz:k + 2 * nz])}
+        out.update(zip(COUPLING_OUT, arrs[k + 2 * nz:]))
+        res["coupling_in"], res["coupling_out"] = inp, out
+    elif mode == 0:
         nf = len(OUT_FIELDS) + len(DEPTHS)
         for k, step in enumerate(out_steps):
             blk = arrs[k * nf:(k + 1) * nf]

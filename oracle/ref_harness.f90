@@ -23,6 +23,15 @@
 !   ubrtr_i, vbrtr_i, ssh4gradx, ssh4grady and the 12 hh*_e arrays.
 ! mode 1: one call of each helper routine on the case's state arrays
 !   (see the write sequence at the end of this file).
+! mode 2: the 2-D -> 3-D coupling after one barotropic step.  One step as
+!   in mode 0 gives ubrtr_i, vbrtr_i and pgrx/pgry (the ssh4grad outputs);
+!   synthetic 3-D velocities and internal-mode fields are set (below), all
+!   inputs are written, then the sequence of Control/dyncall.f90:185-231
+!   without vertical_velocity runs -- the removal and re-add loops and the
+!   uu2d / pgrx update restated from dyncall (they sit inside
+!   ocean_model_step), ssh_internal called as the reference has it -- and
+!   uu, vv, uu2d, vv2d, pgrx, pgry, ssh_i, sshp_i, hhqn, hhun, hhvn, hhhn
+!   are written.
 program ref_harness
 use main_basin_pars
 use mpi_parallel_tools
@@ -130,6 +139,80 @@ if (mode == 0) then
       endif
     endif
   enddo
+else if (mode == 2) then
+  ksw_ssbc = 1
+  ksw_lat4 = ksw4
+  call hh_init(hhq_e, hhqp_e, hhqn_e, hhu_e, hhup_e, hhun_e, hhv_e, hhvp_e, hhvn_e, &
+               hhh_e, hhhp_e, hhhn_e, ssh_e, sshp_e, hhq_rest)
+  r_diss = rdis
+  uu(:,:,nz)  = ubrtr_e
+  vv(:,:,nz)  = vbrtr_e
+  uup(:,:,nz) = ubrtrp_e
+  vvp(:,:,nz) = vbrtrp_e
+  hhu = hhu_e
+  hhv = hhv_e
+  hhq = hhq_e
+  call sea_surface_fluxes
+  call sea_bottom_fluxes
+  RHSx2d = 0.0d0
+  RHSy2d = 0.0d0
+  do n = ny_start, ny_end
+    do m = nx_start, nx_end
+      if (lcu(m,n) > 0.5) RHSx2d(m,n) = (surf_stress_x(m,n) + bot_stress_x(m,n))*dxt(m,n)*dyh(m,n)
+      if (lcv(m,n) > 0.5) RHSy2d(m,n) = (surf_stress_y(m,n) + bot_stress_y(m,n))*dyt(m,n)*dxh(m,n)
+    enddo
+  enddo
+  call barotropic_dynamics(tau, nstep, ksw4, ubrtr_e, ubrtrp_e, vbrtr_e, vbrtrp_e, &
+                           ssh_e, sshp_e, ubrtr_i, vbrtr_i, ssh4gx, ssh4gy,       &
+                           RHSx2d, RHSy2d, wflux, mu, mu4, vort, strt, strs,      &
+                           fxa, fya, r_diss, radv, rady, rdfx, rdfy)
+  ! synthetic coupling inputs
+  do k = 1, nz
+    uu(:,:,k) = ubrtrp_e*(1.0d0 + 0.05d0*k) + 0.01d0*k
+    vv(:,:,k) = vbrtrp_e*(1.0d0 - 0.03d0*k) - 0.02d0*k
+  enddo
+  uu2d = ubrtrp_e
+  vv2d = vbrtrp_e
+  hhu = hhu_e
+  hhv = hhv_e
+  slpr = 1.0d4*sshp_e + 50.0d0*taux
+  pgrx = ssh4gx
+  pgry = ssh4gy
+  ssh_i = sshp_e
+  sshp_i = ssh_e
+  hhqn = 0.0d0; hhun = 0.0d0; hhvn = 0.0d0; hhhn = 0.0d0
+  write(23) ubrtr_i, vbrtr_i, hhu, hhv, slpr, wflux, uu2d, vv2d, pgrx, pgry, ssh_i, sshp_i
+  write(23) uu, vv
+  ! Control/dyncall.f90:185-199
+  do n = ny_start-1, ny_end+1
+    do m = nx_start-1, nx_end+1
+      if (lcu(m,n) > 0.5) uu(m,n,1:nz) = uu(m,n,1:nz) - uu2d(m,n)
+      if (lcv(m,n) > 0.5) vv(m,n,1:nz) = vv(m,n,1:nz) - vv2d(m,n)
+    enddo
+  enddo
+  ! Control/dyncall.f90:201-217
+  do n = ny_start-1, ny_end+1
+    do m = nx_start-1, nx_end+1
+      if (lcu(m,n) > 0.5) then
+        uu2d(m,n) = ubrtr_i(m,n)/hhu(m,n)
+        pgrx(m,n) = pgrx(m,n) - (slpr(m+1,n) - slpr(m,n))*hhu(m,n)*dyh(m,n)/RefDen
+      endif
+      if (lcv(m,n) > 0.5) then
+        vv2d(m,n) = vbrtr_i(m,n)/hhv(m,n)
+        pgry(m,n) = pgry(m,n) - (slpr(m,n+1) - slpr(m,n))*hhv(m,n)*dxh(m,n)/RefDen
+      endif
+    enddo
+  enddo
+  call ssh_internal(tau, ssh_i, sshp_i, ubrtr_i, vbrtr_i, wflux)
+  ! Control/dyncall.f90:217-231
+  do n = ny_start-1, ny_end+1
+    do m = nx_start-1, nx_end+1
+      if (lcu(m,n) > 0.5) uu(m,n,1:nz) = uu(m,n,1:nz) + uu2d(m,n)
+      if (lcv(m,n) > 0.5) vv(m,n,1:nz) = vv(m,n,1:nz) + vv2d(m,n)
+    enddo
+  enddo
+  write(23) uu, vv
+  write(23) uu2d, vv2d, pgrx, pgry, ssh_i, sshp_i, hhqn, hhun, hhvn, hhhn
 else
   ! helper routines, one call each, on the case's state arrays
   ! hh_init from (ssh_e, sshp_e)

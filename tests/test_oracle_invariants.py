@@ -4,6 +4,7 @@ the depth averages, stress tensor and Rayleigh coefficient, hand-derived
 continuity increments, the C-grid mask definitions, gravity-wave speed,
 mirror symmetry and (for the periodic wrap, which the shipped reference
 build does not exercise) translation invariance."""
+import ctypes as C
 import math
 
 import numpy as np
@@ -220,3 +221,57 @@ def test_periodic_x_translation_invariance(shift):
     for k in ("ssh", "u", "v"):
         want = np.roll(o1[k][:, 2:-2], shift, axis=1)
         assert np.allclose(o2[k][:, 2:-2], want, rtol=0, atol=1e-12 * max(1.0, np.max(np.abs(want))))
+
+
+def _coupling_inputs(b, nz=3, seed=0, wflux=True):
+    rng = np.random.default_rng(seed)
+    shape = (b.ny, b.nx)
+
+    def f(scale):
+        return rng.standard_normal(shape) * scale
+    wet = b.wet.astype(np.float64)
+    inp = {"ubrtr_i": f(5.0), "vbrtr_i": f(5.0), "hhu": 50.0 + np.abs(f(10.0)), "hhv": 50.0 + np.abs(f(10.0)),
+           "slpr": 1.0e5 + f(100.0), "wflux": f(1e-6) * wet if wflux else np.zeros(shape),
+           "uu2d": f(0.1), "vv2d": f(0.1), "pgrx": f(1.0), "pgry": f(1.0), "ssh_i": f(0.2) * wet,
+           "sshp_i": f(0.2) * wet}
+    inp["uu"] = rng.standard_normal((nz,) + shape)
+    inp["vv"] = rng.standard_normal((nz,) + shape)
+    return inp
+
+
+def test_coupling_removal_closed_form():
+    b = _flat(40, 30)
+    og = oracle.OracleGrid(b)
+    inp = _coupling_inputs(b, nz=4)
+    uu, vv = inp["uu"].copy(), inp["vv"].copy()
+    oracle.lib().bto_remove_barotropic(C.byref(og.g), 4, oracle._d(uu), oracle._d(vv),
+                                       oracle._d(np.ascontiguousarray(inp["uu2d"])),
+                                       oracle._d(np.ascontiguousarray(inp["vv2d"])))
+    lcu = og.masks["lcu"] > 0.5
+    lcv = og.masks["lcv"] > 0.5
+    for k in range(4):
+        np.testing.assert_array_equal(uu[k], np.where(lcu, inp["uu"][k] - inp["uu2d"], inp["uu"][k]))
+        np.testing.assert_array_equal(vv[k], np.where(lcv, inp["vv"][k] - inp["vv2d"], inp["vv"][k]))
+
+
+def test_ssh_internal_conserves_volume():
+    """With no surface water flux the continuity step with the time-mean
+    transports telescopes in a closed basin: sum(ssh_new dx dy) over wet
+    cells equals sum(sshp_i dx dy); ssh_i takes ssh_new."""
+    b = _flat(48, 36)
+    og = oracle.OracleGrid(b)
+    inp = _coupling_inputs(b, nz=2, seed=3, wflux=False)
+    lcu = og.masks["lcu"] > 0.5
+    lcv = og.masks["lcv"] > 0.5
+    inp["ubrtr_i"] = inp["ubrtr_i"] * lcu          # transports live on open faces
+    inp["vbrtr_i"] = inp["vbrtr_i"] * lcv
+    out = oracle.coupling(og, b.dt, inp)
+    v0 = og.volume(inp["sshp_i"])
+    v1 = og.volume(out["ssh_i"])
+    assert abs(v1 - v0) <= 1e-12 * np.abs(inp["sshp_i"]).sum() * 1e6
+    # and the surface moved: the transports are not divergence-free
+    assert np.abs(out["ssh_i"] - inp["sshp_i"]).max() > 1e-6
+    # uu, vv: removal then re-add of the new 2-D velocity ubrtr_i / hhu
+    k = 1
+    want = np.where(lcu, inp["uu"][k] - inp["uu2d"] + inp["ubrtr_i"] / inp["hhu"], inp["uu"][k])
+    np.testing.assert_allclose(out["uu"][k], want, rtol=1e-15, atol=1e-15)

@@ -32,3 +32,20 @@ def test_random_case_bitwise(seed, nstep, ksw4, metrics):
     got = oracle.run(b, 5, nstep, ksw4, out_steps=steps, rdis=rd)
     for s in steps:
         assert_bitwise(got[s], ref[s], FIELDS + oracle.DEPTHS, f"seed {seed} step {s}")
+
+
+@pytest.mark.parametrize("seed,metrics", [(211, "uniform"), (212, "stretched")])
+def test_coupling_bitwise(seed, metrics):
+    """2-D -> 3-D coupling (Control/dyncall.f90:185-231 with the reference's
+    own ssh_internal) against the oracle on the inputs the harness wrote."""
+    b = make_basin(285, 159, seed=seed, sigma=4.0, g=9.8, hmax=120.0, metrics=metrics)
+    b = randomize_state(b, seed + 1, amp_uv=0.05, wflux_amp=1e-6)
+    rd = oracle.OracleGrid(b).rayleigh()
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "case.bin")
+        write_case(path, b, rd, 1, 2, 0, 2)
+        ref = oracle.run_reference(path, path + ".out", b.nx, b.ny, 2)
+    inp, want = ref["coupling_in"], ref["coupling_out"]
+    assert inp["uu"].shape[0] == 20 and np.abs(inp["slpr"]).max() > 0
+    got = oracle.coupling(oracle.OracleGrid(b), b.dt, inp)
+    assert_bitwise(got, want, ("uu", "vv") + oracle.COUPLING_OUT, f"coupling seed {seed}")
