@@ -26,6 +26,7 @@ sys.path.insert(0, os.path.join(ROOT, "inmsom-elfs_amd"))
 import numpy as np  # noqa: E402
 
 from inmsom_bt import BarotropicModel  # noqa: E402
+from inmsom_bt.model import gridcon  # noqa: E402
 from inmsom_bt.workloads import NSTEP_BAROTROP, single_basin  # noqa: E402
 
 METRIC = "wet-cell steps/s on 1 GPU: plain FP64 vs FP32/batched/learned at fixed SSH err"
@@ -56,20 +57,36 @@ TRAFFIC_KERNELS = {"fp64": "k_fused_step<double, false>", "fp32": "k_fused_step<
                    "fp64_exact": "k_fused_step<double, true>", "ensemble_fp32": "k_fused_batch<float, false>"}
 
 
-def write_traffic(fetch_csv, write_csv, out=TRAFFIC_JSON):
-    """HBM bytes per launch of the fused kernels from two rocprofv3 --pmc
-    passes of this script (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch),
-    each scaled by what the counter reads for k_copy_d in the same run: the
-    FP64 set-up copies 2052 x 1540 doubles (read once, written once)."""
+# kernels of one 2-D -> 3-D coupling and their launches per coupling
+COUPLING_SEQUENCE = {"k_barotropic_part(": 2, "k_internal_uv(": 1, "k_ssh_internal(ibt::DevGrid, double": 1,
+                     "k_ssh_internal_shift(": 1, "k_hh_update(": 1}
+
+
+def write_traffic(fetch_path, write_path, out=TRAFFIC_JSON):
+    """HBM bytes per launch of the fused kernels (and per coupling) from two
+    rocprofv3 --pmc passes of this script (FETCH_SIZE, then WRITE_SIZE; KiB
+    per dispatch; counter_collection.csv or the rocpd results.db), each
+    scaled by what the counter reads for k_copy_d in the same run: the FP64
+    set-up copies 2052 x 1540 doubles (read once, written once).  Entries
+    found in these files replace the same entries of ``out``; the others
+    are kept."""
     import collections
     import csv
+    import sqlite3
 
     def load(path, counter):
         agg = collections.defaultdict(list)
-        with open(path) as f:
-            for r in csv.DictReader(f):
-                if r["Counter_Name"] == counter:
-                    agg[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+        if path.endswith(".db"):
+            con = sqlite3.connect(path)
+            rows = con.execute("select name, counter_value from pmc_events where counter_name = ?", (counter,))
+            for name, v in rows:
+                agg[name].append(float(v))
+            con.close()
+        else:
+            with open(path) as f:
+                for r in csv.DictReader(f):
+                    if r["Counter_Name"] == counter:
+                        agg[r["Kernel_Name"]].append(float(r["Counter_Value"]))
         return {k: (sum(v) / len(v), len(v)) for k, v in agg.items()}
 
     def find(d, key):
@@ -78,21 +95,40 @@ def write_traffic(fetch_csv, write_csv, out=TRAFFIC_JSON):
             raise KeyError(f"{key}: {ks}")
         return d[ks[0]], ks[0]
 
-    F, W = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
+    F, W = load(fetch_path, "FETCH_SIZE"), load(write_path, "WRITE_SIZE")
     kib = 2052 * 1540 * 8 / 1024
     (fc, _), _ = find(F, "k_copy_d")
     (wc, _), _ = find(W, "k_copy_d")
-    res = {"calibration": {"kernel": "k_copy_d", "bytes_each_way": 2052 * 1540 * 8, "fetch_kib": fc,
-                           "write_kib": wc, "fetch_factor": kib / fc, "write_factor": kib / wc},
-           "fetch_csv": os.path.relpath(fetch_csv, ROOT), "write_csv": os.path.relpath(write_csv, ROOT)}
+    ff, wf = kib / fc, kib / wc
+    src = {"fetch_file": os.path.relpath(fetch_path, ROOT), "write_file": os.path.relpath(write_path, ROOT),
+           "calibration": {"kernel": "k_copy_d", "bytes_each_way": 2052 * 1540 * 8, "fetch_kib": fc,
+                           "write_kib": wc, "fetch_factor": ff, "write_factor": wf}}
+    try:
+        with open(out) as f:
+            res = json.load(f)
+    except (OSError, ValueError):
+        res = {}
     for mode, key in TRAFFIC_KERNELS.items():
         try:
             (f, nf), name = find(F, key)
             (w, _), _ = find(W, key)
         except KeyError:
             continue
-        res[mode] = {"kernel": name, "dispatches": nf, "fetch_kib": f, "write_kib": w,
-                     "fetch_bytes_per_launch": f * kib / fc * 1024, "write_bytes_per_launch": w * kib / wc * 1024}
+        res[mode] = dict(src, kernel=name, dispatches=nf, fetch_kib=f, write_kib=w,
+                         fetch_bytes_per_launch=f * ff * 1024, write_bytes_per_launch=w * wf * 1024)
+    try:
+        parts = {}
+        for key, times in COUPLING_SEQUENCE.items():
+            (f, nf), name = find(F, key)
+            (w, _), _ = find(W, key)
+            parts[name] = {"per_coupling": times, "dispatches": nf, "fetch_kib": f, "write_kib": w}
+        res["coupling"] = dict(src, kernels=parts,
+                               fetch_bytes_per_launch=sum(p["per_coupling"] * p["fetch_kib"] for p in parts.values())
+                               * ff * 1024,
+                               write_bytes_per_launch=sum(p["per_coupling"] * p["write_kib"] for p in parts.values())
+                               * wf * 1024)
+    except KeyError:
+        pass
     with open(out, "w") as fo:
         json.dump(res, fo, indent=1)
     return res
@@ -193,6 +229,91 @@ def ensemble_leg(steps, warmup, nmembers):
     log(f"ensemble fp32: {ms:.1f} ms for {steps} steps ({avg * 1e6:.1f} us/launch), "
         f"{res['value']:.3e} wet-cell steps/s, {achieved:.0f} GB/s")
     return res
+
+
+# nz of the reference's configuration (Inc/basinpar.fi:4-6)
+COUPLING_NZ = 20
+# Algorithmic bytes of one coupling (remove + update) per array cell for
+# the 2-D fields, each plane touched once per kernel that uses it (float
+# masks 4 B, fields 8 B): removal 24 (lcu, lcv, uu2d, vv2d), 2-D velocity
+# and pgrx/pgry 112, continuity 76, hh_update 128 (depth at rest, ssh_new,
+# eight metrics, four masks read; four depths written), Asselin shift 44,
+# re-add 24.  The 3-D part is 2 passes x (uu, vv) x (read + write) x 8 B
+# x nz per open U/V face.
+COUPLING_2D_BYTES = 24 + 112 + 76 + 128 + 44 + 24
+
+
+def coupling_leg(m, b, nz, nrep):
+    """The 2-D -> 3-D coupling after the step (Control/dyncall.f90:169-231)
+    on the same basin with nz levels: device kernels timed with HIP events
+    on the library's stream, and the whole host-array call (PCIe
+    included), next to the oracle's C on one host core."""
+    rng = np.random.default_rng(7)
+    shape = (b.ny, b.nx)
+    wet = b.wet.astype(np.float64)
+    f = {"ubrtr_i": 5.0 * rng.standard_normal(shape), "vbrtr_i": 5.0 * rng.standard_normal(shape),
+         "hhu": 50.0 + np.abs(rng.standard_normal(shape)), "hhv": 50.0 + np.abs(rng.standard_normal(shape)),
+         "slpr": 1.0e5 + 100.0 * rng.standard_normal(shape), "wflux": 1e-6 * rng.standard_normal(shape) * wet,
+         "uu2d": 0.1 * rng.standard_normal(shape), "vv2d": 0.1 * rng.standard_normal(shape),
+         "pgrx": rng.standard_normal(shape), "pgry": rng.standard_normal(shape),
+         "ssh_i": 0.2 * rng.standard_normal(shape) * wet, "sshp_i": 0.2 * rng.standard_normal(shape) * wet}
+    z = np.zeros(shape)
+    f.update(hhqn=z, hhun=z, hhvn=z, hhhn=z)
+    uu = rng.standard_normal((nz,) + shape)
+    vv = rng.standard_normal((nz,) + shape)
+    inp = dict(f, uu=uu.copy(), vv=vv.copy())
+    m.remove_barotropic(uu, vv, f["uu2d"], f["vv2d"])          # first call allocates the staging
+    m.internal_update(b.dt, f, uu, vv)
+    uu[:], vv[:] = inp["uu"], inp["vv"]
+    m.synchronize()
+    t0 = time.perf_counter()
+    m.remove_barotropic(uu, vv, f["uu2d"], f["vv2d"])
+    m.internal_update(b.dt, f, uu, vv)
+    host_s = time.perf_counter() - t0
+    # the device-address entry points on HBM-resident arrays (what a caller
+    # whose 3-D fields live on the GPU pays per step, launch and sync included)
+    from inmsom_bt.hipmem import DeviceArray
+    dev = {k: DeviceArray.from_host(a, m.device) for k, a in f.items()}
+    duu, dvv = DeviceArray.from_host(uu, m.device), DeviceArray.from_host(vv, m.device)
+    m.remove_barotropic_dev(duu, dvv, dev["uu2d"], dev["vv2d"])
+    m.internal_update_dev(b.dt, dev, duu, dvv)
+    t0 = time.perf_counter()
+    m.remove_barotropic_dev(duu, dvv, dev["uu2d"], dev["vv2d"])
+    m.internal_update_dev(b.dt, dev, duu, dvv)
+    dev_s = time.perf_counter() - t0
+    for a in list(dev.values()) + [duu, dvv]:
+        a.free()
+    m.coupling_timed(b.dt, nz, 2)                                # warm-up
+    ms = m.coupling_timed(b.dt, nz, nrep)
+    avg_s = ms * 1e-3 / nrep
+    masks = gridcon(b.lbasins)
+    faces = int((masks["lcu"] > 0.5).sum() + (masks["lcv"] > 0.5).sum())
+    alg = 2 * 2 * 8 * nz * faces + COUPLING_2D_BYTES * b.nx * b.ny
+    achieved = alg / avg_s / 1e9
+    res = {"levels": nz, "array_cells": b.nx * b.ny, "open_faces": faces,
+           "value": b.n_wet * nz / avg_s, "unit": "wet-cell levels coupled/s (device-resident)",
+           "device_ms": avg_s * 1e3, "device_array_call_ms": dev_s * 1e3, "host_call_ms": host_s * 1e3,
+           "host_call_rate": b.n_wet * nz / host_s,
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_from_profiles("coupling"),
+                        "bytes_per_launch_sequence": alg}}
+    log(f"coupling nz={nz}: device {avg_s * 1e3:.3f} ms ({achieved:.0f} GB/s), device-array calls "
+        f"{dev_s * 1e3:.2f} ms, host-array calls {host_s * 1e3:.1f} ms")
+    return res, inp
+
+
+def coupling_cpu(b, inp):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    og = oracle.OracleGrid(b)
+    t0 = time.perf_counter()
+    oracle.coupling(og, b.dt, inp)
+    dt = time.perf_counter() - t0
+    nz = inp["uu"].shape[0]
+    return {"value": b.n_wet * nz / dt, "unit": "wet-cell levels coupled/s", "cores": 1, "kind": "port",
+            "sample": f"one coupling of the same basin with {nz} levels, {dt:.2f} s, oracle/bt_oracle.c "
+                      "(bto_remove_barotropic + bto_internal_update, one thread)"}
 
 
 CLOSURE_PARAMS = os.path.join(ROOT, "inmsom-elfs_amd", "inmsom_bt", "data", "closure_cfg1.npz")
@@ -306,9 +427,12 @@ def main():
                     help="coarse 40 s steps of the learned-closure rollout (0 = skip)")
     ap.add_argument("--closure-members", type=int, default=128)
     ap.add_argument("--closure-member-steps", type=int, default=1000)
-    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+    ap.add_argument("--coupling-levels", type=int, default=COUPLING_NZ,
+                    help="levels of the 2-D -> 3-D coupling leg (0: skip it)")
+    ap.add_argument("--coupling-reps", type=int, default=50)
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH", "WRITE"),
                     help="write the per-launch HBM traffic of the fused kernels from two rocprofv3 "
-                         "--pmc counter_collection.csv files (FETCH_SIZE, WRITE_SIZE) and exit")
+                         "--pmc outputs (counter_collection.csv or results.db; FETCH_SIZE, WRITE_SIZE) and exit")
     args = ap.parse_args()
     if args.traffic_from:
         print(json.dumps(write_traffic(*args.traffic_from), indent=1))
@@ -329,6 +453,9 @@ def main():
     with BarotropicModel(b, device=0) as m:
         for mode in modes:
             results[mode], states[mode] = timed_mode(m, b, mode, args.steps, args.warmup, ncell)
+        coupling = None
+        if args.coupling_levels > 0:
+            coupling, coupling_inp = coupling_leg(m, b, args.coupling_levels, args.coupling_reps)
 
     errs = {}
     if "fp64" in states:
@@ -368,6 +495,11 @@ def main():
             line[mode] = results[mode]
     line["fp64_details"] = {k: head[k] for k in ("avg_launch_us", "range_check_bad_cells",
                                                  "volume_rel_drift", "wall_s")}
+    if coupling is not None:
+        if not args.no_cpu_baseline:
+            coupling["cpu_baseline"] = coupling_cpu(b, coupling_inp)
+        del coupling_inp
+        line["coupling"] = coupling
     if args.ensemble_steps > 0:
         line["ensemble_fp32"] = ensemble_leg(args.ensemble_steps, min(args.warmup, 20),
                                              args.ensemble_members)

@@ -207,17 +207,17 @@ int inmsom_bt_ens_volume(inmsom_bt_ens *ens, int member, double *volume);
 
 
 /* The 2-D -> 3-D coupling that follows barotropic_dynamics in the model
- * step, Control/dyncall.f90:185-231 (host arrays in and out, like
+ * step, Control/dyncall.f90:169-231 (host arrays in and out, like
  * inmsom_bt_barotropic_dynamics; 3-D arrays are (nz, ny, nx), i.e. the
  * reference's column-major (m, n, k)).  The caller runs vertical_velocity
  * between the two calls, as the reference does.
  *
  * inmsom_bt_remove_barotropic: uu(:,:,k) -= uu2d on lcu and
  * vv(:,:,k) -= vv2d on lcv, k = 1..nz, m = 2..nx-1, n = 2..ny-1
- * (dyncall.f90:185-199). */
+ * (dyncall.f90:169-184). */
 int inmsom_bt_remove_barotropic(inmsom_bt_ctx *ctx, int nz, double *uu, double *vv,
                                 const double *uu2d, const double *vv2d);
-/* inmsom_bt_internal_update (dyncall.f90:201-231): uu2d = ubrtr_i / hhu and
+/* inmsom_bt_internal_update (dyncall.f90:188-231): uu2d = ubrtr_i / hhu and
  * pgrx -= (slpr(m+1) - slpr(m)) * hhu * dyh / RefDen on lcu (v likewise);
  * ssh_internal (Function/vel_ssh.f90:682-751): ssh_new from the continuity
  * equation with the time-mean transports ubrtr_i, vbrtr_i and wflux,
@@ -231,6 +231,26 @@ int inmsom_bt_internal_update(inmsom_bt_ctx *ctx, double tau, int nz, const doub
                               const double *slpr, const double *wflux, double *uu2d, double *vv2d,
                               double *pgrx, double *pgry, double *ssh_i, double *sshp_i,
                               double *const hhn[4], double *uu, double *vv);
+/* The same two steps on arrays already in device memory (device addresses
+ * on the context's GPU, same shapes and meanings): no host copies, for a
+ * caller whose 3-D fields live in HBM.  Work runs on the library's stream,
+ * which does not wait for other streams: the arrays must be ready when
+ * the call is made.  Both calls return after the work has finished. */
+int inmsom_bt_remove_barotropic_dev(inmsom_bt_ctx *ctx, int nz, double *uu, double *vv,
+                                    const double *uu2d, const double *vv2d);
+int inmsom_bt_internal_update_dev(inmsom_bt_ctx *ctx, double tau, int nz, const double *ubrtr_i,
+                                  const double *vbrtr_i, const double *hhu, const double *hhv,
+                                  const double *slpr, const double *wflux, double *uu2d, double *vv2d,
+                                  double *pgrx, double *pgry, double *ssh_i, double *sshp_i,
+                                  double *const hhn[4], double *uu, double *vv);
+/* Timing of the coupling's device work: nrep repetitions of the kernels of
+ * inmsom_bt_remove_barotropic then inmsom_bt_internal_update on the fields
+ * the last inmsom_bt_internal_update call left on the device (it must have
+ * had at least nz levels; INMSOM_BT_ESTATE otherwise), without the host
+ * copies; *ms = HIP-event time of all repetitions.  Each repetition
+ * advances those device fields; nothing is copied back.  No counterpart in
+ * the reference (a measurement entry beside inmsom_bt_run_timed). */
+int inmsom_bt_coupling_timed(inmsom_bt_ctx *ctx, double tau, int nz, int nrep, double *ms);
 
 /* Block average of the current state onto a grid `factor` times coarser
  * (what a coarse model of the same ocean sees; used to build the learned

@@ -87,5 +87,6 @@ struct inmsom_bt_ctx {
     double *scal = nullptr;        // device scalars (reductions)
     double *cpl = nullptr;         // staging of the 2-D -> 3-D coupling (bt_coupling.hip)
     size_t cpl_n = 0;              // doubles in cpl
+    double *cpl_new = nullptr;     // one plane: ssh_new of the device-pointer coupling
     long long *cnt = nullptr;
 };

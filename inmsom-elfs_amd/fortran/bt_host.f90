@@ -5,7 +5,7 @@
 ! with the range check after every step and reports the same timer the
 ! reference prints (time_barotrop).
 !
-! usage: inmsom_bt_host <par dir> <case.bin> <out.bin> [literal|fp64_exact|fp64|fp32] [g]
+! usage: inmsom_bt_host <par dir> <case.bin> <out.bin> [literal|fp64_exact|fp64|fp32] [g] [coupling.bin]
 !
 !   ocean_run.par  line 2 time_step [s], line 3 run_duration [days],
 !                  line 9 nstep_barotrop                 (inmsom_head.f90:47-55)
@@ -14,6 +14,13 @@
 !                  inmsom_bt.cases.write_case (layout in oracle/ref_harness.f90);
 !                  its nsteps field, when positive, overrides run_duration.
 !   out.bin        final ssh_e, sshp_e, ubrtr_e, ubrtrp_e, vbrtr_e, vbrtrp_e (real8)
+!   coupling.bin   optional 3-D side of the last step: int32 nz, then real8
+!                  hhu, hhv, slpr, uu2d, vv2d, ssh_i, sshp_i (nx x ny) and
+!                  uu, vv (nx x ny x nz).  After the last step the host runs
+!                  the 2-D -> 3-D coupling (Control/dyncall.f90:169-231) with
+!                  the step's ubrtr_i, vbrtr_i, pgrx, pgry and wflux, and
+!                  appends uu, vv, uu2d, vv2d, pgrx, pgry, ssh_i, sshp_i,
+!                  hhqn, hhun, hhvn, hhhn to out.bin.
 !
 ! Mode 'literal' runs the stage-by-stage device path (any metrics, the
 ! biharmonic term); the fused modes need uniform metrics.
@@ -22,16 +29,18 @@ use iso_c_binding
 use inmsom_bt
 implicit none
 
-character(512) :: pardir, case_path, out_path, arg
+character(512) :: pardir, case_path, out_path, arg, cpl_path
 character(256) :: lines(64)
 character(16) :: mode_name
 integer :: nl, ierr, magic, version, nx, ny, nsteps_case, nstep_case, ksw4_case, cmode, nout
-integer :: nstep_barotrop, ksw_lat4, nsteps, step, mode
+integer :: nstep_barotrop, ksw_lat4, nsteps, step, mode, nz
 integer, allocatable, target :: lbasins(:,:), out_steps(:)
 real(8) :: time_step, run_duration, tau_case, cb_nl, ebottom, g
 real(8), allocatable, target :: f(:,:,:)
 real(8), allocatable, target :: st(:,:,:)
 real(8), allocatable, target :: rdis(:,:)
+! coupling: c2 = ubrtr_i vbrtr_i pgrx pgry hhu hhv slpr uu2d vv2d ssh_i sshp_i hhqn hhun hhvn hhhn
+real(8), allocatable, target :: c2(:,:,:), uu(:,:,:), vv(:,:,:)
 integer(c_long_long) :: bad
 integer(8) :: c0, c1, crate
 real(8) :: time_barotrop, vol0, vol1
@@ -144,12 +153,43 @@ call bt_check(inmsom_bt_get_state(bt_ctx, c_loc(st(1,1,3)), c_loc(st(1,1,4)), c_
 call bt_check(inmsom_bt_volume(bt_ctx, vol1), 'inmsom_bt_volume')
 open(23, file=trim(out_path), access='stream', form='unformatted', status='replace')
 write(23) st
+if (command_argument_count() >= 6) then
+  call get_command_argument(6, cpl_path)
+  call coupling_step(trim(cpl_path))
+  write(23) uu, vv, c2(:,:,8:9), c2(:,:,3:4), c2(:,:,10:15)
+endif
 close(23)
 write(*,'(a,es12.5)') ' Time barotropic: ', time_barotrop
 write(*,'(a,es12.5)') ' volume drift (relative): ', abs(vol1 - vol0) / max(abs(vol0), 1.0d-300)
 call bt_check(inmsom_bt_finalize(bt_ctx), 'inmsom_bt_finalize')
 
 contains
+
+! The 2-D -> 3-D coupling of Control/dyncall.f90:169-231 after the last
+! step, on the time means that step left (inmsom_bt_get_means).
+subroutine coupling_step(path)
+  character(*), intent(in) :: path
+  integer :: io
+  open(24, file=path, access='stream', form='unformatted', status='old', iostat=io)
+  if (io /= 0) then
+    write(*,*) 'cannot open ', path
+    error stop 5
+  endif
+  read(24) nz
+  allocate(c2(nx, ny, 15), uu(nx, ny, nz), vv(nx, ny, nz))
+  c2 = 0.0d0
+  read(24) c2(:,:,5:11)
+  read(24) uu, vv
+  close(24)
+  call bt_check(inmsom_bt_get_means(bt_ctx, c_loc(c2(1,1,1)), c_loc(c2(1,1,2)), c_loc(c2(1,1,3)), &
+                c_loc(c2(1,1,4))), 'inmsom_bt_get_means')
+  call remove_barotropic_gpu(nz, uu, vv, c2(:,:,8), c2(:,:,9))
+  ! (the 3-D model's vertical_velocity runs here)
+  call internal_update_gpu(time_step, nz, c2(:,:,1), c2(:,:,2), c2(:,:,5), c2(:,:,6), c2(:,:,7), f(:,:,13), &
+                           c2(:,:,8), c2(:,:,9), c2(:,:,3), c2(:,:,4), c2(:,:,10), c2(:,:,11), c2(:,:,12), &
+                           c2(:,:,13), c2(:,:,14), c2(:,:,15), uu, vv)
+  write(*,'(a,i0,a)') ' coupling step done (', nz, ' levels)'
+end subroutine coupling_step
 
 ! Service/read_write_parameters.f90:2-41: the file's lines, one per entry.
 subroutine read_lines(path, buf, n)

@@ -7,6 +7,9 @@
 !   * barotropic_dynamics_gpu(...) has the argument list of the reference's
 !                                 barotropic_dynamics (Function/vel_ssh.f90:379-406)
 !                                 and runs it on the GPU;
+!   * remove_barotropic_gpu(...) and internal_update_gpu(...) replace the
+!                                 2-D -> 3-D coupling loops around vertical_velocity
+!                                 (Control/dyncall.f90:169-231, with ssh_internal);
 !   * the inmsom_bt_* interfaces expose the whole C ABI.
 module inmsom_bt
 use iso_c_binding
@@ -138,6 +141,48 @@ interface
     import :: c_int, c_ptr
     type(c_ptr), value :: ctx
   end function
+  integer(c_int) function inmsom_bt_remove_barotropic(ctx, nz, uu, vv, uu2d, vv2d) &
+      bind(C, name="inmsom_bt_remove_barotropic")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: nz
+    type(c_ptr), value :: uu, vv, uu2d, vv2d
+  end function
+  integer(c_int) function inmsom_bt_internal_update(ctx, tau, nz, ubrtr_i, vbrtr_i, hhu, hhv, slpr, wflux, &
+      uu2d, vv2d, pgrx, pgry, ssh_i, sshp_i, hhn, uu, vv) bind(C, name="inmsom_bt_internal_update")
+    import :: c_int, c_double, c_ptr
+    type(c_ptr), value :: ctx
+    real(c_double), value :: tau
+    integer(c_int), value :: nz
+    type(c_ptr), value :: ubrtr_i, vbrtr_i, hhu, hhv, slpr, wflux, uu2d, vv2d, pgrx, pgry, ssh_i, sshp_i
+    type(c_ptr), intent(in) :: hhn(4)
+    type(c_ptr), value :: uu, vv
+  end function
+  ! the same on device addresses (e.g. from hipMalloc or OpenMP use_device_addr)
+  integer(c_int) function inmsom_bt_remove_barotropic_dev(ctx, nz, uu, vv, uu2d, vv2d) &
+      bind(C, name="inmsom_bt_remove_barotropic_dev")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: nz
+    type(c_ptr), value :: uu, vv, uu2d, vv2d
+  end function
+  integer(c_int) function inmsom_bt_internal_update_dev(ctx, tau, nz, ubrtr_i, vbrtr_i, hhu, hhv, slpr, &
+      wflux, uu2d, vv2d, pgrx, pgry, ssh_i, sshp_i, hhn, uu, vv) bind(C, name="inmsom_bt_internal_update_dev")
+    import :: c_int, c_double, c_ptr
+    type(c_ptr), value :: ctx
+    real(c_double), value :: tau
+    integer(c_int), value :: nz
+    type(c_ptr), value :: ubrtr_i, vbrtr_i, hhu, hhv, slpr, wflux, uu2d, vv2d, pgrx, pgry, ssh_i, sshp_i
+    type(c_ptr), intent(in) :: hhn(4)
+    type(c_ptr), value :: uu, vv
+  end function
+  integer(c_int) function inmsom_bt_coupling_timed(ctx, tau, nz, nrep, ms) bind(C, name="inmsom_bt_coupling_timed")
+    import :: c_int, c_double, c_ptr
+    type(c_ptr), value :: ctx
+    real(c_double), value :: tau
+    integer(c_int), value :: nz, nrep
+    real(c_double), intent(out) :: ms
+  end function
   function c_strlen(s) bind(C, name="strlen")
     import :: c_ptr, c_size_t
     type(c_ptr), value :: s
@@ -184,5 +229,36 @@ subroutine barotropic_dynamics_gpu(tau, nstep, ksw4, ubrtr_e, ubrtrp_e, vbrtr_e,
                 c_loc(ssh4grady), c_loc(RHSx), c_loc(RHSy), c_loc(wflux), c_loc(mu), c_loc(mu4), &
                 c_loc(rdis)), 'barotropic_dynamics_gpu')
 end subroutine barotropic_dynamics_gpu
+
+! Control/dyncall.f90:169-184 on the context bt_ctx: the barotropic part
+! uu2d, vv2d taken out of the 3-D velocity uu, vv (nz levels) before
+! vertical_velocity.
+subroutine remove_barotropic_gpu(nz, uu, vv, uu2d, vv2d)
+  integer, intent(in) :: nz
+  real(8), target, intent(inout) :: uu(*), vv(*)
+  real(8), target, intent(in) :: uu2d(*), vv2d(*)
+  call bt_check(inmsom_bt_remove_barotropic(bt_ctx, int(nz, c_int), c_loc(uu), c_loc(vv), c_loc(uu2d), &
+                c_loc(vv2d)), 'remove_barotropic_gpu')
+end subroutine remove_barotropic_gpu
+
+! Control/dyncall.f90:188-231 on the context bt_ctx, after vertical_velocity:
+! the time-mean 2-D velocity and the sea-level-pressure term of pgrx, pgry,
+! ssh_internal (Function/vel_ssh.f90:682-751, which also updates the 3-D
+! model's depths hhqn, hhun, hhvn, hhhn when full_free_surface > 0) and the
+! barotropic part put back into uu, vv.
+subroutine internal_update_gpu(tau, nz, ubrtr_i, vbrtr_i, hhu, hhv, slpr, wf_tot, uu2d, vv2d, pgrx, pgry, &
+                               ssh_i, sshp_i, hhqn, hhun, hhvn, hhhn, uu, vv)
+  real(8), intent(in) :: tau
+  integer, intent(in) :: nz
+  real(8), target, intent(in) :: ubrtr_i(*), vbrtr_i(*), hhu(*), hhv(*), slpr(*), wf_tot(*)
+  real(8), target, intent(inout) :: uu2d(*), vv2d(*), pgrx(*), pgry(*), ssh_i(*), sshp_i(*)
+  real(8), target, intent(inout) :: hhqn(*), hhun(*), hhvn(*), hhhn(*), uu(*), vv(*)
+  type(c_ptr) :: hhn(4)
+  hhn = [c_loc(hhqn), c_loc(hhun), c_loc(hhvn), c_loc(hhhn)]
+  call bt_check(inmsom_bt_internal_update(bt_ctx, real(tau, c_double), int(nz, c_int), c_loc(ubrtr_i), &
+                c_loc(vbrtr_i), c_loc(hhu), c_loc(hhv), c_loc(slpr), c_loc(wf_tot), c_loc(uu2d), c_loc(vv2d), &
+                c_loc(pgrx), c_loc(pgry), c_loc(ssh_i), c_loc(sshp_i), hhn, c_loc(uu), c_loc(vv)), &
+                'internal_update_gpu')
+end subroutine internal_update_gpu
 
 end module inmsom_bt

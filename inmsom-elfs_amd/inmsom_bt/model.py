@@ -93,6 +93,7 @@ class BarotropicModel:
                  periodicity=(0, 0), full_free_surface: int = 1,
                  masks: Optional[Dict[str, np.ndarray]] = None):
         self.b = b
+        self.device = device
         self.nx, self.ny = b.nx, b.ny
         self._keep: list = []
         g = _grid_struct(b, self._keep, periodicity, full_free_surface, masks)
@@ -201,7 +202,7 @@ class BarotropicModel:
 
     # ------------------------------------------------------------ 2-D -> 3-D coupling
     def remove_barotropic(self, uu: np.ndarray, vv: np.ndarray, uu2d: np.ndarray, vv2d: np.ndarray):
-        """Control/dyncall.f90:185-199 on (nz, ny, nx) arrays, in place."""
+        """Control/dyncall.f90:169-184 on (nz, ny, nx) arrays, in place."""
         for a in (uu, vv):
             if a.dtype != np.float64 or not a.flags.c_contiguous or a.shape[1:] != (self.ny, self.nx):
                 raise ValueError("uu, vv must be C-contiguous float64 (nz, ny, nx)")
@@ -210,7 +211,7 @@ class BarotropicModel:
               "inmsom_bt_remove_barotropic")
 
     def internal_update(self, tau: float, fields: dict, uu: np.ndarray, vv: np.ndarray) -> dict:
-        """Control/dyncall.f90:201-231 with ssh_internal.  ``fields`` holds
+        """Control/dyncall.f90:188-231 with ssh_internal.  ``fields`` holds
         ubrtr_i, vbrtr_i, hhu, hhv, slpr, wflux (inputs) and uu2d, vv2d,
         pgrx, pgry, ssh_i, sshp_i, hhqn, hhun, hhvn, hhhn (updated; copies
         are returned); uu, vv (nz, ny, nx) are updated in place."""
@@ -228,6 +229,45 @@ class BarotropicModel:
             *[_d(io[k]) for k in ("uu2d", "vv2d", "pgrx", "pgry", "ssh_i", "sshp_i")], hh, _d(uu), _d(vv)),
             "inmsom_bt_internal_update")
         return io
+
+    def _dev(self, t, shape):
+        """Device address of a DeviceArray (inmsom_bt.hipmem) of the given
+        shape on this model's GPU."""
+        from .hipmem import DeviceArray
+        if not isinstance(t, DeviceArray) or t.ptr is None:
+            raise ValueError("device arrays must be inmsom_bt.hipmem.DeviceArray objects")
+        if t.shape != tuple(shape) or t.device != self.device:
+            raise ValueError(f"device array must be on GPU {self.device} with shape {tuple(shape)}, "
+                             f"not GPU {t.device}, {t.shape}")
+        return C.cast(C.c_void_p(t.ptr), _capi._dp)
+
+    def remove_barotropic_dev(self, uu, vv, uu2d, vv2d):
+        """As remove_barotropic on DeviceArray arguments (no host copies);
+        the arrays must be ready (synchronise the producing stream first)."""
+        nz = uu.shape[0]
+        s3, s2 = (nz, self.ny, self.nx), (self.ny, self.nx)
+        check(lib().inmsom_bt_remove_barotropic_dev(self._h, nz, self._dev(uu, s3), self._dev(vv, s3),
+                                                    self._dev(uu2d, s2), self._dev(vv2d, s2)),
+              "inmsom_bt_remove_barotropic_dev")
+
+    def internal_update_dev(self, tau: float, fields: dict, uu, vv):
+        """As internal_update on device arrays, all updated in place."""
+        nz = uu.shape[0]
+        s3, s2 = (nz, self.ny, self.nx), (self.ny, self.nx)
+        hh = (_capi._dp * 4)(*[self._dev(fields[k], s2) for k in ("hhqn", "hhun", "hhvn", "hhhn")])
+        check(lib().inmsom_bt_internal_update_dev(
+            self._h, float(tau), nz,
+            *[self._dev(fields[k], s2) for k in ("ubrtr_i", "vbrtr_i", "hhu", "hhv", "slpr", "wflux", "uu2d",
+                                                 "vv2d", "pgrx", "pgry", "ssh_i", "sshp_i")],
+            hh, self._dev(uu, s3), self._dev(vv, s3)), "inmsom_bt_internal_update_dev")
+
+    def coupling_timed(self, tau: float, nz: int, nrep: int) -> float:
+        """HIP-event time (ms) of nrep repetitions of the coupling's device
+        kernels on the fields the last internal_update left on the device."""
+        ms = C.c_double()
+        check(lib().inmsom_bt_coupling_timed(self._h, float(tau), int(nz), int(nrep), C.byref(ms)),
+              "inmsom_bt_coupling_timed")
+        return ms.value
 
     # ------------------------------------------------------------ diagnostics
     def coarsen(self, factor: int = 4) -> dict:

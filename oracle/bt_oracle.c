@@ -643,7 +643,7 @@ void bto_bottom_stress(const bto_grid *g, double cb_nl, double ebottom,
         }
 }
 
-/* Control/dyncall.f90:185-199 */
+/* Control/dyncall.f90:169-184 */
 void bto_remove_barotropic(const bto_grid *g, int nz, double *uu, double *vv,
                            const double *uu2d, const double *vv2d)
 {
@@ -658,7 +658,7 @@ void bto_remove_barotropic(const bto_grid *g, int nz, double *uu, double *vv,
         }
 }
 
-/* Control/dyncall.f90:201-231 (vertical_velocity excluded) with
+/* Control/dyncall.f90:188-231 (vertical_velocity excluded) with
  * ssh_internal, Function/vel_ssh.f90:682-751 */
 void bto_internal_update(const bto_grid *g, double tau, int nz,
                          const double *ubrtr_i, const double *vbrtr_i,
@@ -670,7 +670,7 @@ void bto_internal_update(const bto_grid *g, double tau, int nz,
 {
     const size_t plane = (size_t)g->nx * g->ny;
     const double rho0 = g->ref_den;
-    /* dyncall.f90:203-217 */
+    /* dyncall.f90:188-206 */
 #pragma omp parallel for schedule(static)
     for (int n = NYS - 1; n <= NYE + 1; ++n)
         for (int m = NXS - 1; m <= NXE + 1; ++m) {
@@ -712,7 +712,7 @@ void bto_internal_update(const bto_grid *g, double tau, int nz,
                 ssh_i[IX(m, n)] = ssh_new[IX(m, n)];
             }
     free(ssh_new);
-    /* dyncall.f90:217-231 */
+    /* dyncall.f90:216-231 */
 #pragma omp parallel for schedule(static)
     for (int n = NYS - 1; n <= NYE + 1; ++n)
         for (int m = NXS - 1; m <= NXE + 1; ++m) {

@@ -132,13 +132,13 @@ void bto_model_step(const bto_grid *g, bto_depths *d, double tau, int nstep,
                     const double *wflux, const double *mu, const double *mu4,
                     const double *rdis, double *work);
 
-/* Control/dyncall.f90:185-199: the barotropic part removed from the 3-D
+/* Control/dyncall.f90:169-184: the barotropic part removed from the 3-D
  * velocity, uu(:,:,k) -= uu2d on lcu and vv(:,:,k) -= vv2d on lcv for
  * k = 1..nz over m = 2..nx-1, n = 2..ny-1; uu, vv are (nz, ny, nx). */
 void bto_remove_barotropic(const bto_grid *g, int nz, double *uu, double *vv,
                            const double *uu2d, const double *vv2d);
 
-/* Control/dyncall.f90:201-231 without vertical_velocity: uu2d = ubrtr_i/hhu
+/* Control/dyncall.f90:188-231 without vertical_velocity: uu2d = ubrtr_i/hhu
  * and pgrx -= (slpr(m+1)-slpr(m))*hhu*dyh/RefDen on lcu (v likewise on
  * lcv); ssh_internal (Function/vel_ssh.f90:682-751): continuity with the
  * time-mean transports into ssh_new, hh_update of (hhqn, hhun, hhvn, hhhn)

@@ -226,7 +226,7 @@ UNIT_LAYOUT = ([("init_" + k) for k in DEPTHS]
                   "surf_x", "surf_y", "bottom_friction", "bot_x", "bot_y"])
 
 
-# 2-D -> 3-D coupling (Control/dyncall.f90:185-231, ssh_internal)
+# 2-D -> 3-D coupling (Control/dyncall.f90:169-231, ssh_internal)
 COUPLING_IN = ("ubrtr_i", "vbrtr_i", "hhu", "hhv", "slpr", "wflux", "uu2d", "vv2d", "pgrx", "pgry",
                "ssh_i", "sshp_i")
 COUPLING_OUT = ("uu2d", "vv2d", "pgrx", "pgry", "ssh_i", "sshp_i", "hhqn", "hhun", "hhvn", "hhhn")

@@ -26,7 +26,7 @@
 ! mode 2: the 2-D -> 3-D coupling after one barotropic step.  One step as
 !   in mode 0 gives ubrtr_i, vbrtr_i and pgrx/pgry (the ssh4grad outputs);
 !   synthetic 3-D velocities and internal-mode fields are set (below), all
-!   inputs are written, then the sequence of Control/dyncall.f90:185-231
+!   inputs are written, then the sequence of Control/dyncall.f90:169-231
 !   without vertical_velocity runs -- the removal and re-add loops and the
 !   uu2d / pgrx update restated from dyncall (they sit inside
 !   ocean_model_step), ssh_internal called as the reference has it -- and
@@ -183,14 +183,14 @@ else if (mode == 2) then
   hhqn = 0.0d0; hhun = 0.0d0; hhvn = 0.0d0; hhhn = 0.0d0
   write(23) ubrtr_i, vbrtr_i, hhu, hhv, slpr, wflux, uu2d, vv2d, pgrx, pgry, ssh_i, sshp_i
   write(23) uu, vv
-  ! Control/dyncall.f90:185-199
+  ! Control/dyncall.f90:169-184
   do n = ny_start-1, ny_end+1
     do m = nx_start-1, nx_end+1
       if (lcu(m,n) > 0.5) uu(m,n,1:nz) = uu(m,n,1:nz) - uu2d(m,n)
       if (lcv(m,n) > 0.5) vv(m,n,1:nz) = vv(m,n,1:nz) - vv2d(m,n)
     enddo
   enddo
-  ! Control/dyncall.f90:201-217
+  ! Control/dyncall.f90:188-206
   do n = ny_start-1, ny_end+1
     do m = nx_start-1, nx_end+1
       if (lcu(m,n) > 0.5) then
@@ -204,7 +204,7 @@ else if (mode == 2) then
     enddo
   enddo
   call ssh_internal(tau, ssh_i, sshp_i, ubrtr_i, vbrtr_i, wflux)
-  ! Control/dyncall.f90:217-231
+  ! Control/dyncall.f90:216-231
   do n = ny_start-1, ny_end+1
     do m = nx_start-1, nx_end+1
       if (lcu(m,n) > 0.5) uu(m,n,1:nz) = uu(m,n,1:nz) + uu2d(m,n)

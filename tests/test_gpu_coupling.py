@@ -1,4 +1,4 @@
-"""The 2-D -> 3-D coupling on the GPU (Control/dyncall.f90:185-231 with
+"""The 2-D -> 3-D coupling on the GPU (Control/dyncall.f90:169-231 with
 ssh_internal, Function/vel_ssh.f90:682-751) against the oracle, bit for
 bit; the oracle itself is pinned to the compiled reference harness."""
 import numpy as np
@@ -87,3 +87,42 @@ def test_coupling_after_dropin_step():
                     pgry=acc_g["ssh4grady"])
         got = _device(m, b.dt, ginp)
     assert_bitwise(got, want, OUT, "coupling after the drop-in step")
+
+
+def test_coupling_timed_needs_staged_fields():
+    from inmsom_bt._capi import BtError
+    b = make_basin(40, 30, seed=44, sigma=4.0)
+    with BarotropicModel(b) as m:
+        with pytest.raises(BtError, match="inmsom_bt_internal_update first"):
+            m.coupling_timed(b.dt, 3, 2)
+        inp = _inputs(b, 3, seed=1)
+        got = _device(m, b.dt, inp)
+        assert np.isfinite(got["ssh_i"]).all()
+        assert m.coupling_timed(b.dt, 3, 2) > 0.0
+        assert m.coupling_timed(b.dt, 2, 1) > 0.0     # fewer levels fit the staging
+        with pytest.raises(BtError, match="staged"):
+            m.coupling_timed(b.dt, 4, 1)
+
+
+@pytest.mark.parametrize("periodicity", [(0, 0), (1, 1)])
+def test_coupling_device_pointers_match_oracle(periodicity):
+    """The device-address entry points on arrays already in HBM give the
+    oracle's result bit for bit, like the host-array ones."""
+    from inmsom_bt.hipmem import DeviceArray
+    nz = 6
+    print(f"coupling on device arrays, periodicity={periodicity}", flush=True)
+    b = randomize_state(make_basin(66, 52, seed=45, sigma=4.0, metrics="stretched"), 8)
+    inp = _inputs(b, nz, seed=12)
+    want = oracle.coupling(oracle.OracleGrid(b, periodicity=periodicity), b.dt, inp)
+    z = np.zeros_like(inp["ssh_i"])
+    host = dict(inp, hhqn=z, hhun=z, hhvn=z, hhhn=z)
+    with BarotropicModel(b, periodicity=periodicity) as m:
+        dev = {k: DeviceArray.from_host(a) for k, a in host.items()}
+        m.remove_barotropic_dev(dev["uu"], dev["vv"], dev["uu2d"], dev["vv2d"])
+        m.internal_update_dev(b.dt, dev, dev["uu"], dev["vv"])
+        with pytest.raises(ValueError):
+            m.remove_barotropic_dev(dev["uu"], dev["vv"], dev["uu"], dev["vv2d"])
+        got = {k: dev[k].to_host() for k in OUT}
+        for a in dev.values():
+            a.free()
+    assert_bitwise(got, want, OUT, f"coupling on device arrays {periodicity}")

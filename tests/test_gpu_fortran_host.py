@@ -56,3 +56,51 @@ def test_fortran_host_range_check_stops_run():
                            capture_output=True, text=True, timeout=120)
     assert p.returncode != 0
     assert "range check failed at step 1" in p.stdout
+
+
+def test_fortran_host_coupling_bitwise():
+    """The host's coupling step (remove_barotropic_gpu, internal_update_gpu
+    through the ISO_C_BINDING module) after the last barotropic step, fed
+    with that step's time means, against the oracle's coupling on the
+    oracle's means (Control/dyncall.f90:137-231 in order)."""
+    b = randomize_state(make_basin(80, 56, seed=53, sigma=5.0), 4, wflux_amp=1e-5)
+    rd = oracle.OracleGrid(b).rayleigh()
+    steps, nz = 6, 7
+    want_run = oracle.run(b, steps, 2, out_steps=[steps], rdis=rd)[steps]
+    rng = np.random.default_rng(17)
+    shape = (b.ny, b.nx)
+    wet = b.wet.astype(np.float64)
+    side = {"hhu": 50.0 + np.abs(rng.standard_normal(shape)), "hhv": 50.0 + np.abs(rng.standard_normal(shape)),
+            "slpr": 1.0e5 + 100.0 * rng.standard_normal(shape), "uu2d": 0.1 * rng.standard_normal(shape),
+            "vv2d": 0.1 * rng.standard_normal(shape), "ssh_i": 0.2 * rng.standard_normal(shape) * wet,
+            "sshp_i": 0.2 * rng.standard_normal(shape) * wet}
+    uu = rng.standard_normal((nz,) + shape)
+    vv = rng.standard_normal((nz,) + shape)
+    inp = dict(side, uu=uu, vv=vv, ubrtr_i=want_run["ubrtr_i"], vbrtr_i=want_run["vbrtr_i"],
+               pgrx=want_run["ssh4gradx"], pgry=want_run["ssh4grady"], wflux=np.ascontiguousarray(b.wflux))
+    want = oracle.coupling(oracle.OracleGrid(b), b.dt, inp)
+    with tempfile.TemporaryDirectory() as tmp:
+        case = os.path.join(tmp, "case.bin")
+        cpl = os.path.join(tmp, "coupling.bin")
+        out = os.path.join(tmp, "out.bin")
+        write_case(case, b, rd, steps, 2, 0, 0, [])
+        with open(cpl, "wb") as fh:
+            np.array([nz], dtype="<i4").tofile(fh)
+            for k in ("hhu", "hhv", "slpr", "uu2d", "vv2d", "ssh_i", "sshp_i"):
+                np.ascontiguousarray(side[k], dtype="<f8").tofile(fh)
+            uu.astype("<f8").tofile(fh)
+            vv.astype("<f8").tofile(fh)
+        p = subprocess.run([HOST, PARDIR, case, out, "literal", repr(b.g), cpl], capture_output=True, text=True,
+                           timeout=120)
+        print(p.stdout, p.stderr)
+        assert p.returncode == 0, p.stdout + p.stderr
+        raw = np.fromfile(out, dtype="<f8")
+    per = b.nx * b.ny
+    st = dict(zip(("ssh", "sshp", "u", "up", "v", "vp"), raw[:6 * per].reshape(6, b.ny, b.nx)))
+    assert_bitwise(st, want_run, STATE, "fortran host state before the coupling")
+    rest = raw[6 * per:]
+    got = {"uu": rest[:nz * per].reshape(nz, b.ny, b.nx), "vv": rest[nz * per:2 * nz * per].reshape(nz, b.ny, b.nx)}
+    planes = rest[2 * nz * per:].reshape(10, b.ny, b.nx)
+    got.update(zip(("uu2d", "vv2d", "pgrx", "pgry", "ssh_i", "sshp_i", "hhqn", "hhun", "hhvn", "hhhn"), planes))
+    assert "coupling step done (7 levels)" in p.stdout
+    assert_bitwise(got, want, tuple(got), "fortran host coupling")

@@ -275,3 +275,35 @@ def test_ssh_internal_conserves_volume():
     k = 1
     want = np.where(lcu, inp["uu"][k] - inp["uu2d"] + inp["ubrtr_i"] / inp["hhu"], inp["uu"][k])
     np.testing.assert_allclose(out["uu"][k], want, rtol=1e-15, atol=1e-15)
+
+
+def test_coupling_velocity_and_pressure_closed_form():
+    """A sea-level pressure linear in x and y changes pgrx by the constant
+    -a hhu dyh / RefDen and pgry by -c hhv dxh / RefDen on open faces of
+    the wide range, and sets uu2d = ubrtr_i / hhu there; the re-added 3-D
+    velocity is uu - uu2d_old + uu2d_new on those faces; everything else
+    keeps its value."""
+    b = make_basin(30, 24, seed=4, sigma=3.0, metrics="stretched")
+    og = oracle.OracleGrid(b)
+    inp = _coupling_inputs(b, nz=2, seed=5)
+    a, c = 3.0, -2.0
+    mm = np.arange(1, b.nx + 1)[None, :]
+    nn = np.arange(1, b.ny + 1)[:, None]
+    inp["slpr"] = 1.0e5 + a * mm + c * nn + 0.0 * inp["slpr"]
+    out = oracle.coupling(og, b.dt, inp)
+    wide = np.zeros((b.ny, b.nx), dtype=bool)
+    wide[1:-1, 1:-1] = True
+    lcu = (og.masks["lcu"] > 0.5) & wide
+    lcv = (og.masks["lcv"] > 0.5) & wide
+    rho = b.rho0
+    np.testing.assert_allclose(out["pgrx"], np.where(lcu, inp["pgrx"] - a * inp["hhu"] * b.dyh / rho, inp["pgrx"]),
+                               rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(out["pgry"], np.where(lcv, inp["pgry"] - c * inp["hhv"] * b.dxh / rho, inp["pgry"]),
+                               rtol=1e-13, atol=1e-15)
+    np.testing.assert_array_equal(out["uu2d"], np.where(lcu, inp["ubrtr_i"] / inp["hhu"], inp["uu2d"]))
+    np.testing.assert_array_equal(out["vv2d"], np.where(lcv, inp["vbrtr_i"] / inp["hhv"], inp["vv2d"]))
+    for k in range(2):
+        np.testing.assert_allclose(out["uu"][k], np.where(lcu, inp["uu"][k] - inp["uu2d"] + out["uu2d"],
+                                                          inp["uu"][k]), rtol=1e-14, atol=1e-14)
+        np.testing.assert_allclose(out["vv"][k], np.where(lcv, inp["vv"][k] - inp["vv2d"] + out["vv2d"],
+                                                          inp["vv"][k]), rtol=1e-14, atol=1e-14)

@@ -36,7 +36,7 @@ def test_random_case_bitwise(seed, nstep, ksw4, metrics):
 
 @pytest.mark.parametrize("seed,metrics", [(211, "uniform"), (212, "stretched")])
 def test_coupling_bitwise(seed, metrics):
-    """2-D -> 3-D coupling (Control/dyncall.f90:185-231 with the reference's
+    """2-D -> 3-D coupling (Control/dyncall.f90:169-231 with the reference's
     own ssh_internal) against the oracle on the inputs the harness wrote."""
     b = make_basin(285, 159, seed=seed, sigma=4.0, g=9.8, hmax=120.0, metrics=metrics)
     b = randomize_state(b, seed + 1, amp_uv=0.05, wflux_amp=1e-6)
