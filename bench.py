@@ -3,8 +3,10 @@
 
 Workload (inmsom_bt.workloads.single_basin('cfg1')): a 2048 x 1536 closed
 basin at 1 km, 10 s outer steps, barotropic mode sub-cycled twice
-(5 s leap-frog substeps).  A bench "step" is one outer step = two fused
-launches; the reported value counts state-advancing leap-frog substeps:
+(5 s leap-frog substeps).  A bench "step" is one outer step = two
+leap-frog substeps, which FP64 and FP32 advance in one fused launch (the
+second substep takes the first's rows from registers) and FP64_EXACT in
+two; the reported value counts state-advancing leap-frog substeps:
 
     value = wet cells * steps * nstep_barotrop / timed seconds
 
@@ -32,11 +34,13 @@ from inmsom_bt.workloads import NSTEP_BAROTROP, single_basin  # noqa: E402
 METRIC = "wet-cell steps/s on 1 GPU: plain FP64 vs FP32/batched/learned at fixed SSH err"
 PEAK_HBM_GBS = 8000.0
 
-# Algorithmic bytes per streamed cell per launch (substep), nstep_barotrop = 2:
-# level n and filtered level n-1 of (u, v, ssh) read and written, the depth at
-# rest and the Rayleigh coefficient read, and the two RHS components frozen
-# over the call (written by the first substep, read by the second: one
-# 8-byte (FP32: 4-byte) array access per component per launch on average).
+# Algorithmic bytes per streamed cell per substep, nstep_barotrop = 2: level
+# n and filtered level n-1 of (u, v, ssh) read and written, the depth at rest
+# and the Rayleigh coefficient read, and the two RHS components frozen over
+# the call (written by the first substep, read by the second: one 8-byte
+# (FP32: 4-byte) array access per component per substep on average).  A
+# launch of two substeps is credited twice this; it moves far less, since
+# the intermediate level never leaves the registers (roofline.traffic).
 BYTES_PER_CELL = {
     "fp64": 12 * 8 + 8 + 8 + 2 * 8,          # 128
     "fp64_exact": 12 * 8 + 8 + 8 + 2 * 8,    # 128
@@ -53,7 +57,8 @@ def rel_l2(a, b):
 
 
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "fused_step_cfg1_traffic.json")
-TRAFFIC_KERNELS = {"fp64": "k_fused_step<double, false>", "fp32": "k_fused_step<float, false>",
+# the kernel each leg launches (two substeps per launch in FP64 and FP32)
+TRAFFIC_KERNELS = {"fp64": "k_fused_step2<double, false>", "fp32": "k_fused_step2<float, false>",
                    "fp64_exact": "k_fused_step<double, true>", "ensemble_fp32": "k_fused_batch<float, false>"}
 
 
@@ -156,23 +161,26 @@ def timed_mode(m, b, mode, steps, warmup, ncell):
     bad = m.range_check()
     vol1 = m.volume()
     st = m.get_state()
-    launches = steps * NSTEP_BAROTROP
+    subs = m.substeps_per_launch(mode)
+    substeps = steps * NSTEP_BAROTROP
+    launches = substeps // subs
     avg_launch_s = ms * 1e-3 / launches
     bpc = BYTES_PER_CELL[mode]
-    achieved = bpc * ncell / avg_launch_s / 1e9
+    achieved = bpc * subs * ncell / avg_launch_s / 1e9
     traffic = traffic_from_profiles(mode)
     res = {
-        "value": b.n_wet * launches / (ms * 1e-3),
+        "value": b.n_wet * substeps / (ms * 1e-3),
         "ms_per_step": ms / steps,
         "wall_s": wall,
         "avg_launch_us": avg_launch_s * 1e6,
+        "substeps_per_launch": subs,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "bytes_per_cell": bpc, "cells_per_launch": ncell},
+                     "bytes_per_cell_substep": bpc, "cells_per_launch": ncell, "substeps_per_launch": subs},
         "range_check_bad_cells": bad,
         "volume_rel_drift": abs(vol1 - vol0) / max(abs(vol0), 1e-300),
     }
-    log(f"{mode}: {ms:.1f} ms for {steps} steps ({avg_launch_s*1e6:.1f} us/launch), "
+    log(f"{mode}: {ms:.1f} ms for {steps} steps ({avg_launch_s*1e6:.1f} us/launch of {subs} substeps), "
         f"{res['value']:.3e} wet-cell steps/s, {achieved:.0f} GB/s")
     return res, st
 

@@ -154,6 +154,15 @@ int inmsom_bt_run_timed(inmsom_bt_ctx *ctx, double tau, int nstep, int nsteps, i
 int inmsom_bt_set_launch(inmsom_bt_ctx *ctx, int rows_per_wave);
 int inmsom_bt_get_launch(inmsom_bt_ctx *ctx, int *rows_per_wave);
 
+/* Substeps of one call that the fused step advances per launch: 2 (with
+ * nstep >= 2 each pair of substeps is one march over the rows, the second
+ * substep taking the first's rows from registers; an odd last substep runs
+ * alone), 1 (one launch per substep), or 0 (default: 2 in FP64 and FP32, 1
+ * in FP64_EXACT).  Results do not depend on it. */
+int inmsom_bt_set_substeps_per_launch(inmsom_bt_ctx *ctx, int substeps);
+/* the substeps per launch the fused step uses in `mode` (1 or 2) */
+int inmsom_bt_get_substeps_per_launch(inmsom_bt_ctx *ctx, int mode, int *substeps);
+
 /* Range check of Control/dyncall.f90:452-516 on the device state:
  * *bad receives the number of wet cells with NaN, |ssh| >= 10 or
  * |u|,|v| >= 100.  Returns INMSOM_BT_ERANGE when *bad > 0. */

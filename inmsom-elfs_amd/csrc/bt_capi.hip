@@ -496,6 +496,21 @@ int inmsom_bt_set_launch(inmsom_bt_ctx *c, int rows_per_wave)
     return INMSOM_BT_OK;
 }
 
+int inmsom_bt_set_substeps_per_launch(inmsom_bt_ctx *c, int substeps)
+{
+    REQUIRE(c, "null context");
+    TRY(fused_set_substeps(c, substeps));
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_get_substeps_per_launch(inmsom_bt_ctx *c, int mode, int *substeps)
+{
+    REQUIRE(c && substeps, "null pointer");
+    REQUIRE(mode == INMSOM_BT_FP64_EXACT || mode == INMSOM_BT_FP64 || mode == INMSOM_BT_FP32, "unknown mode");
+    *substeps = subs_per_launch(c, mode);
+    return INMSOM_BT_OK;
+}
+
 int inmsom_bt_get_launch(inmsom_bt_ctx *c, int *rows_per_wave)
 {
     REQUIRE(c && rows_per_wave, "null pointer");

@@ -20,5 +20,7 @@ int fused_range_check(inmsom_bt_ctx *c, long long *bad);
 int fused_volume(inmsom_bt_ctx *c, double *vol);
 int fused_coarsen(inmsom_bt_ctx *c, int factor, double *u, double *v, double *ssh);
 int fused_set_shape(inmsom_bt_ctx *c, int chunk);
+int fused_set_substeps(inmsom_bt_ctx *c, int k);
+int subs_per_launch(const inmsom_bt_ctx *c, int mode);
 
 }  // namespace ibt

@@ -48,6 +48,17 @@ __global__ void __launch_bounds__(WAVE) k_fused_step(FusedParams p)
     fused_chunk<R, EXACT>(p, chunk_id, strip);
 }
 
+// Substeps p.sub and p.sub + 1 in one launch (58-column strips).
+template <typename R, bool EXACT>
+__global__ void __launch_bounds__(WAVE) k_fused_step2(FusedParams p)
+{
+    const int w = blockIdx.x;
+    const int t = (w % NXCD) * p.per_xcd + (w / NXCD);
+    if (t >= p.total) return;
+    const int chunk_id = t / p.strips, strip = t - chunk_id * p.strips;
+    fused_chunk2<R, EXACT>(p, chunk_id, strip);
+}
+
 // -------------------------------------------------------------- helpers
 __global__ void k_pack_masks(int n, const float *lu, const float *luu, const float *luh,
                              const float *lcu, const float *lcv, const float *llu, const float *llv,
@@ -144,6 +155,16 @@ static int hipfail(hipError_t e, const char *what)
     } while (0)
 
 static inline unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
+
+int fused_set_substeps(inmsom_bt_ctx *c, int k)
+{
+    if (k < 0 || k > 2) {
+        set_error("substeps per launch must be 0 (automatic), 1 or 2");
+        return INMSOM_BT_EINVAL;
+    }
+    c->subs_per_launch = k;
+    return INMSOM_BT_OK;
+}
 
 int fused_set_shape(inmsom_bt_ctx *c, int chunk)
 {
@@ -353,15 +374,45 @@ int fused_flush(inmsom_bt_ctx *c)
     return INMSOM_BT_OK;
 }
 
+// Parameters of a launch that reads buffer set b and writes 1 - b: substep
+// `sub`, or substeps sub and sub + 1 when `two`.
+// Substeps per launch: as set, or automatically two in FP64 and FP32 and
+// one in FP64_EXACT, whose IEEE divides make it compute-bound (measured on
+// MI355X at 2048 x 1536, us per substep, one / two per launch: FP64 91.3 /
+// 81.5, FP32 66.1 / 53.9, FP64_EXACT 153.8 / 183.1).
+int subs_per_launch(const inmsom_bt_ctx *c, int mode)
+{
+    if (c->subs_per_launch > 0) return c->subs_per_launch;
+    return mode == INMSOM_BT_FP64_EXACT ? 1 : 2;
+}
+
+// Waves of the fused kernel for (mode, two) that the device holds at once:
+// CUs x waves per CU from the occupancy calculator, once per context.
+static long long wave_slots(inmsom_bt_ctx *c, int mode, bool two)
+{
+    const int idx = (mode == INMSOM_BT_FP64_EXACT ? 0 : mode == INMSOM_BT_FP64 ? 1 : 2) + (two ? 3 : 0);
+    if (c->slots[idx] > 0) return c->slots[idx];
+    const void *k[6] = {(const void *)&k_fused_step<double, true>, (const void *)&k_fused_step<double, false>,
+                        (const void *)&k_fused_step<float, false>, (const void *)&k_fused_step2<double, true>,
+                        (const void *)&k_fused_step2<double, false>, (const void *)&k_fused_step2<float, false>};
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k[idx], WAVE, 0) != hipSuccess || per_cu < 1)
+        per_cu = 4;
+    c->slots[idx] = (long long)cus * per_cu;
+    return c->slots[idx];
+}
+
 static FusedParams make_params(inmsom_bt_ctx *c, FusedAlloc *a, int b, double tau, int nstep,
-                               int sub)
+                               int sub, int mode, bool two)
 {
     FusedParams p{};
     p.nx = c->nx;
     p.ny = c->ny;
-    p.strips = (c->nx + 2 + SW - 1) / SW;  // columns 1..nx need strips covering m <= nx
-    p.chunk = c->chunk > 0 ? c->chunk
-                           : auto_chunk(c->fa && c->fa->mode == INMSOM_BT_FP32, (long long)p.strips * c->ny);
+    const int sw = two ? SW2 : SW;
+    p.strips = (c->nx + 2 + sw - 1) / sw;  // columns 1..nx need strips covering m <= nx
+    p.chunk = c->chunk > 0 ? c->chunk : fit_chunk(c->ny, p.strips, wave_slots(c, mode, two), two ? 7 : 3, 4);
     p.nchunks = (c->ny + p.chunk - 1) / p.chunk;
     p.total = p.strips * p.nchunks;
     p.per_xcd = (p.total + NXCD - 1) / NXCD;
@@ -403,9 +454,18 @@ static FusedParams make_params(inmsom_bt_ctx *c, FusedAlloc *a, int b, double ta
     return p;
 }
 
-static void launch(inmsom_bt_ctx *c, const FusedParams &p, int mode)
+static void launch(inmsom_bt_ctx *c, const FusedParams &p, int mode, bool two)
 {
     dim3 grid((unsigned)(p.per_xcd * NXCD)), blk(WAVE);
+    if (two) {
+        if (mode == INMSOM_BT_FP64_EXACT)
+            hipLaunchKernelGGL((k_fused_step2<double, true>), grid, blk, 0, c->stream, p);
+        else if (mode == INMSOM_BT_FP64)
+            hipLaunchKernelGGL((k_fused_step2<double, false>), grid, blk, 0, c->stream, p);
+        else
+            hipLaunchKernelGGL((k_fused_step2<float, false>), grid, blk, 0, c->stream, p);
+        return;
+    }
     if (mode == INMSOM_BT_FP64_EXACT)
         hipLaunchKernelGGL((k_fused_step<double, true>), grid, blk, 0, c->stream, p);
     else if (mode == INMSOM_BT_FP64)
@@ -450,10 +510,12 @@ int fused_run(inmsom_bt_ctx *c, double tau, int nstep, int nsteps, int mode, dou
         HC(hipEventRecord(e0, c->stream));
     }
     for (int s = 0; s < nsteps; ++s)
-        for (int sub = 1; sub <= nstep; ++sub) {
-            FusedParams p = make_params(c, a, c->fz_cur, tau, nstep, sub);
-            launch(c, p, mode);
+        for (int sub = 1; sub <= nstep;) {
+            const bool two = subs_per_launch(c, mode) == 2 && sub + 1 <= nstep;
+            FusedParams p = make_params(c, a, c->fz_cur, tau, nstep, sub, mode, two);
+            launch(c, p, mode, two);
             c->fz_cur = 1 - c->fz_cur;
+            sub += two ? 2 : 1;
         }
     HC(hipGetLastError());
     if (ms) {
@@ -465,7 +527,6 @@ int fused_run(inmsom_bt_ctx *c, double tau, int nstep, int nsteps, int mode, dou
         hipEventDestroy(e0);
         hipEventDestroy(e1);
     } else {
-#pragma clang fp contract(fast)
         HC(hipStreamSynchronize(c->stream));
     }
     return INMSOM_BT_OK;

@@ -8,11 +8,9 @@
 namespace ibt {
 
 constexpr int WAVE = 64;
-constexpr int SW = 60;  // output columns per wave
+constexpr int SW = 60;   // output columns per wave, one substep per launch
+constexpr int SW2 = 58;  // output columns per wave, two substeps per launch
 constexpr int NXCD = 8;
-#ifndef PREFETCH_DEPTH_OVERRIDE
-#define PREFETCH_DEPTH_OVERRIDE 0
-#endif
 
 struct FusedParams {
     int nx, ny, strips, chunk, nchunks, total, per_xcd;
@@ -120,12 +118,39 @@ struct RowAddr {
     }
 };
 
-// One work item: `chunk` output rows of one 60-column strip of one basin.
-// Called by every lane of a wave with wave-uniform arguments.
-// Rows per wave when the caller does not choose one: the measured best for
-// large grids (FP64 32, FP32 20 at 2048 x 1536), fewer when the work items
-// (strips x row chunks) would not give the chip about 4 per CU: an item
-// marches its rows one after another, so small grids are latency-bound.
+// Rows per wave when the caller does not choose one, for a grid of `rows`
+// rows cut into `strips` strips on a chip that holds `slots` waves of the
+// kernel at once.  An item marches its chunk plus `warm` warm-up rows, and
+// the launch takes as many rounds as it has items per slot, so the time
+// goes as rounds x (chunk + warm): pick the number of rounds k (1..8) that
+// minimises it, with the chunk that fits k rounds exactly.  Measured on
+// MI355X at 2048 x 1536 (1024 SIMDs): one item more than a round costs up to
+// 40 % (two substeps per launch in FP32: 28 rows, 1980 items in 2048 slots,
+// 54 us per substep; 27 rows, 2088 items, 77 us).  Small grids keep at
+// least `lo` rows per item.
+inline int fit_chunk(int rows, int strips, long long slots, int warm, int lo)
+{
+    int best = rows;
+    long long best_cost = -1;
+    for (int k = 1; k <= 8; ++k) {
+        const long long per = (long long)k * slots / strips;  // chunks per strip that fit k rounds
+        if (per < 1) continue;
+        int chunk = (int)((rows + per - 1) / per);
+        if (chunk < lo) chunk = lo;
+        const long long items = (long long)strips * ((rows + chunk - 1) / chunk);
+        const long long rounds = (items + slots - 1) / slots;
+        const long long cost = rounds * (chunk + warm);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = chunk;
+        }
+    }
+    return best;
+}
+
+// The older rule, kept for the ensemble's persistent launch: the measured
+// best for large grids (FP64 32, FP32 20 rows), fewer when the work items
+// (strips x row chunks) would not give the chip about 4 per CU.
 inline int auto_chunk(bool fp32, long long strip_rows)
 {
     const long long big = fp32 ? 20 : 32;
@@ -133,37 +158,45 @@ inline int auto_chunk(bool fp32, long long strip_rows)
     return (int)(c < 4 ? 4 : c > big ? big : c);
 }
 
-// Prefetch depth (iterations of loads in flight per wave).  Measured on
-// MI355X at 2048 x 1536: depth 2 keeps FP32 at 3 waves/SIMD but runs 5 %
-// slower (73.8 vs 70.4 us per launch), and costs FP64 half its occupancy,
-// so every mode uses depth 1.
+// What one substep consumes at iteration n of its march (already zeroed
+// outside the array): surface height, depth at rest and mask bits of row
+// n+2; level-n velocity, filtered level and Rayleigh coefficient of row
+// n+1; the frozen forcing and carried U/V depths of row n; the row tables.
+template <typename R>
+struct StageIn {
+    R s2, H2;
+    int mb2;
+    R u1, v1, rd1;
+    double up1, vp1, sp1;
+    R fx0, fy0;
+    double hupc0, hvpc0;
+    R f0, sx0, sy0, wf1;
+};
+
+// What one substep produces for output row n: the new level-n fields and
+// filtered level after the Robert-Asselin shift, the forcing it used
+// (formed at the call's first substep, frozen afterwards) and, in
+// FP64_EXACT, the hh_shift-filtered U/V depths for the next substep.
+template <typename R>
+struct StageOut {
+    R s, u, v;
+    double sp, up, vp;
+    R rhs_u, rhs_v;
+    double hu, hv;
+};
+
+// One leap-frog substep marching down the rows of one strip: the rolling
+// register window (suffix m1 = row n-1, 0 = row n, p1 = n+1, p2 = n+2) of
+// input rows and of the T- and H-point intermediates.  row(n) takes the
+// inputs of iteration n and, when asked, computes the outputs of row n.
 template <typename R, bool EXACT>
-constexpr int prefetch_depth() { return PREFETCH_DEPTH_OVERRIDE > 0 ? PREFETCH_DEPTH_OVERRIDE : 1; }
+struct Stage {
+    const FusedParams &p;
+    const Ops<R, EXACT> op;
+    const int sub;
+    const bool frozen;
+    const bool inx, inx1;
 
-template <typename R, bool EXACT, int PD = prefetch_depth<R, EXACT>()>
-__device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, int strip)
-{
-    const int lane = threadIdx.x;
-    const int m = strip * SW + lane - 1;
-    const bool colok = (m >= 1 && m <= p.nx);
-    const bool writer = colok && lane >= 2 && lane <= SW + 1;
-    const int r0 = chunk_id * p.chunk + 1;
-    const int r1 = min(r0 + p.chunk, p.ny + 1);
-    const bool inx = (m >= 3 && m <= p.nx - 2);
-    const bool inx1 = (m >= 2 && m <= p.nx - 1);
-
-    // store policy (measured, see st_out): level-n outputs non-temporal in
-    // the FP64 modes, the FP64 filtered level non-temporal in every mode,
-    // the frozen forcing (re-read by the next substep) cached
-    constexpr bool NTS = sizeof(R) == 8;
-    constexpr bool NTS_D = true;
-    Ops<R, EXACT> op(p);
-    const R d = op.d;
-    const R tau = R(p.tau);
-    const R G = R(p.g);
-    const R nu = R(p.nu);
-
-    // rolling window (suffix m1 = row n-1, 0 = row n, p1 = n+1, p2 = n+2)
     R u_m1 = 0, u_0 = 0, u_p1 = 0, v_m1 = 0, v_0 = 0, v_p1 = 0;
     R s_0 = 0, s_p1 = 0, s_p2 = 0, H_p1 = 0, H_p2 = 0;
     double up_0 = 0, up_p1 = 0, vp_0 = 0, vp_p1 = 0, sp_0 = 0, sp_p1 = 0;
@@ -176,87 +209,33 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
     double sn_0 = 0, sn_p1 = 0;
     R hun_0 = 0, hun_p1 = 0, stt_0 = 0, stt_p1 = 0, bf_0 = 0, bf_p1 = 0;
     R sts_m1 = 0, sts_0 = 0, vo_m1 = 0, vo_0 = 0;
+    R f_m1 = 0, f_0 = 0;
 
-    // Loads run PD iterations ahead of their use (software prefetch).
-    const RowAddr ra{p.nx, p.ny, min(max(m, 1), p.nx)};
-    const R *S = static_cast<const R *>(p.s), *Hh = static_cast<const R *>(p.H);
-    const R *U = static_cast<const R *>(p.u), *V = static_cast<const R *>(p.v);
-    const R *RD = static_cast<const R *>(p.rd);
-    const R *FX = static_cast<const R *>(p.frx), *FY = static_cast<const R *>(p.fry);
-    const bool frozen = p.sub > 1;
-    auto rowok = [&](int row) { return colok && row >= 1 && row <= p.ny; };
-    auto tab = [&](const double *t, int row) { return t[min(max(row, 1), p.ny) - 1]; };
+    __device__ Stage(const FusedParams &pp, int sub_, int m)
+        : p(pp), op(pp), sub(sub_), frozen(sub_ > 1), inx(m >= 3 && m <= pp.nx - 2),
+          inx1(m >= 2 && m <= pp.nx - 1)
+    {
+    }
 
-    // one iteration's prefetched loads
-    struct Pre {
-        R s, H, u, v, rd, fx, fy;
-        double up, vp, sp, hupc, hvpc, f, sx, sy, wf;
-        int mb;
-    };
-    auto issue = [&](int n, Pre &q) {  // the loads iteration n consumes
-        const size_t i2 = ra.at(n + 2), i1 = ra.at(n + 1), i0 = ra.at(n);
-        q.s = S[i2];
-        q.H = Hh[i2];
-        q.mb = p.mb[i2];
-        q.u = U[i1];
-        q.v = V[i1];
-        q.up = p.up[i1];
-        q.vp = p.vp[i1];
-        q.sp = p.sp[i1];
-        q.rd = RD[i1];
-        // per-cell frozen forcing and carried depths of the output row n:
-        // element 0 (one cache line per wave) when they are not used
-        const size_t iff = frozen ? i0 : 0;
-        q.fx = FX[iff];
-        q.fy = FY[iff];
-        if constexpr (EXACT) {
-            q.hupc = p.hup_c[iff];
-            q.hvpc = p.hvp_c[iff];
-        } else {
-            q.hupc = q.hvpc = 0.0;
-        }
-        q.f = tab(p.row_f, n);
-        q.sx = tab(p.row_sx, n);
-        q.sy = tab(p.row_sy, n);
-        q.wf = tab(p.row_wf, n + 1);
-    };
-    // PD iterations of prefetch: qa is consumed by the current iteration,
-    // qb (PD = 2) by the next one.  Addresses are clamped, so the loads
-    // issued past the chunk's last row stay inside the arrays.
-    Pre qa, qb;
-    issue(r0 - 3, qa);
-    if constexpr (PD == 2) issue(r0 - 2, qb);
-    R f_m1 = 0, f_0 = 0, fx_0 = 0, fy_0 = 0, sx_0 = 0, sy_0 = 0, wf_p1 = 0;
-    double hupc_0 = 0, hvpc_0 = 0;
-
-    for (int n = r0 - 3; n < r1; ++n) {
+    __device__ __forceinline__ void row(int n, const StageIn<R> &q, bool produce, StageOut<R> &o)
+    {
+        const R d = op.d;
+        const R tau = R(p.tau);
+        const R G = R(p.g);
+        const R nu = R(p.nu);
         const int k1 = n + 1;
-        // ---- take the prefetched rows: surface height, depth and masks two
-        //      rows ahead, the rest one row ahead
-        s_p2 = rowok(n + 2) ? qa.s : R(0);
-        H_p2 = rowok(n + 2) ? qa.H : R(0);
-        mb_p2 = rowok(n + 2) ? qa.mb : 0;
-        u_p1 = rowok(k1) ? qa.u : R(0);
-        v_p1 = rowok(k1) ? qa.v : R(0);
-        up_p1 = rowok(k1) ? qa.up : 0.0;
-        vp_p1 = rowok(k1) ? qa.vp : 0.0;
-        sp_p1 = rowok(k1) ? qa.sp : 0.0;
-        rd_p1 = rowok(k1) ? qa.rd : R(0);
-        fx_0 = qa.fx;
-        fy_0 = qa.fy;
-        hupc_0 = qa.hupc;
-        hvpc_0 = qa.hvpc;
+        s_p2 = q.s2;
+        H_p2 = q.H2;
+        mb_p2 = q.mb2;
+        u_p1 = q.u1;
+        v_p1 = q.v1;
+        up_p1 = q.up1;
+        vp_p1 = q.vp1;
+        sp_p1 = q.sp1;
+        rd_p1 = q.rd1;
         f_m1 = f_0;
-        f_0 = (n >= 1 && n <= p.ny) ? R(qa.f) : R(0);
-        sx_0 = R(qa.sx);
-        sy_0 = R(qa.sy);
-        wf_p1 = (k1 >= 1 && k1 <= p.ny) ? R(qa.wf) : R(0);
-        if constexpr (PD == 2) {
-            qa = qb;
-            issue(n + 2, qb);
-        } else {
-            if (n + 1 < r1) issue(n + 1, qa);
-        }
+        f_0 = q.f0;
+        const R sx_0 = q.sx0, sy_0 = q.sy0, wf_p1 = q.wf1;
 
         const R lu_0 = bitf<R>(mb_0, MB_LU), lu_p1 = bitf<R>(mb_p1, MB_LU), lu_p2 = bitf<R>(mb_p2, MB_LU);
         const R lu_0r = xr(lu_0), lu_p1r = xr(lu_p1);
@@ -309,7 +288,7 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
             bf = R(0);
             if constexpr (EXACT) {
                 st = d / d * (upr / d - up1l / d) - d / d * (vpr / d - vp0r / d);
-                if (p.sub == 1) {
+                if (sub == 1) {
                     R ut = (u_p1 * d * d * hu_p1 + u1l * d * d * hu1l) / 2.0 / d / d / hq_p1;
                     R vt = (v_p1 * d * d * hv_p1 + v_0 * d * d * hv_0) / 2.0 / d / d / hq_p1;
                     bf = R(p.cb) * sqrt(ut * ut + vt * vt + R(p.eb));
@@ -317,7 +296,7 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
             } else {
 #pragma clang fp contract(fast)
                 st = op.inv_d * ((upr - up1l) - (vpr - vp0r));
-                if (p.sub == 1) {
+                if (sub == 1) {
                     const R rh = R(0.5) / hq_p1;
                     R ut = (u_p1 * hu_p1 + u1l * hu1l) * rh;
                     R vt = (v_p1 * hv_p1 + v_0 * hv_0) * rh;
@@ -325,7 +304,7 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
                 }
             }
             stt_p1 = tw ? st : R(0);
-            bf_p1 = (tw && p.sub == 1) ? bf : R(0);
+            bf_p1 = (tw && sub == 1) ? bf : R(0);
         }
 
         // ---- H-point stage of row n: shear and metric vorticity
@@ -347,7 +326,7 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
         }
 
         // ---- outputs of row n
-        if (n >= r0) {
+        if (produce) {
             const bool iny = (n >= 3 && n <= p.ny - 2);
             const R lu_m1 = bitf<R>(mb_m1, MB_LU);
             const R mu_m1 = lu_m1 * nu, mu_0 = lu_0 * nu, mu_p1 = lu_p1 * nu;
@@ -363,8 +342,6 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
             const R up0 = R(up_0), vp0 = R(vp_0);
             const R luu0 = bitf<R>(mb_0, MB_LUU), luum1 = bitf<R>(mb_m1, MB_LUU);
             const R sx = sx_0, sy = sy_0;
-            const size_t io = (size_t)(n - 1) * p.nx + (m - 1);
-            const bool own = writer && n <= p.ny;
             // the RHS of Control/dyncall.f90:114-126 is formed once per call
             // (first substep) and frozen over the call's substeps
             R rhs_u, rhs_v;
@@ -377,19 +354,18 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
                 rhs_v = sy - R(0.5) * (bf_0 + bf_p1) * vp0;
             }
             if (frozen) {
-                rhs_u = fx_0;
-                rhs_v = fy_0;
-            } else if (p.nsub > 1 && own) {
-                st_out<false, R>(p.frx, io, rhs_u);
-                st_out<false, R>(p.fry, io, rhs_v);
+                rhs_u = q.fx0;
+                rhs_v = q.fy0;
             }
+            o.rhs_u = rhs_u;
+            o.rhs_v = rhs_v;
             // level n-1 U/V depths: hh_init values at the first substep,
             // the hh_shift-filtered carry afterwards (depth.f90:195-203)
             R hupc = hup_0, hvpc = hvp_0;
             if constexpr (EXACT) {
                 if (frozen) {
-                    hupc = R(hupc_0);
-                    hvpc = R(hvpc_0);
+                    hupc = R(q.hupc0);
+                    hvpc = R(q.hvpc0);
                 }
             }
             R un, vn;
@@ -469,7 +445,7 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
             un = uw ? un : R(0);
             vn = vw ? vn : R(0);
 
-            // time-level shift with the Robert-Asselin filter (vel_ssh.f90:588-612), nstep = 1
+            // time-level shift with the Robert-Asselin filter (vel_ssh.f90:588-612)
             R s_new = s_0, u_new = u_0, v_new = v_0;
             double sp_new = sp_0, up_new = up_0, vp_new = vp_0;
             if (inx1 && n >= 2 && n <= p.ny - 1) {
@@ -502,27 +478,22 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
                     v_new = vn;
                 }
             }
+            o.s = s_new;
+            o.u = u_new;
+            o.v = v_new;
+            o.sp = sp_new;
+            o.up = up_new;
+            o.vp = vp_new;
             if constexpr (EXACT) {
                 // hh_shift of the U/V depths for the next substep (depth.f90:195-203)
-                if (p.nsub > 1 && p.sub < p.nsub && own) {
-                    const double ts = p.ts;
-                    double hu_new = hupc, hv_new = hvpc;
-                    if (inx1 && n >= 2 && n <= p.ny - 1) {
-                        if (mb_0 & MB_LLU) hu_new = hu_0 + ts * (hun_0 - 2.0 * hu_0 + hupc) / 2.0 / p.dn;
-                        if (mb_0 & MB_LLV) hv_new = hv_0 + ts * (hvn_0 - 2.0 * hv_0 + hvpc) / 2.0 / p.dn;
-                    }
-                    st_out<NTS, double>(p.hup_c, io, hu_new);
-                    st_out<NTS, double>(p.hvp_c, io, hv_new);
+                const double ts = p.ts;
+                double hu_new = hupc, hv_new = hvpc;
+                if (inx1 && n >= 2 && n <= p.ny - 1) {
+                    if (mb_0 & MB_LLU) hu_new = hu_0 + ts * (hun_0 - 2.0 * hu_0 + hupc) / 2.0 / p.dn;
+                    if (mb_0 & MB_LLV) hv_new = hv_0 + ts * (hvn_0 - 2.0 * hv_0 + hvpc) / 2.0 / p.dn;
                 }
-            }
-            if (own) {
-                const size_t i = io;
-                st_out<NTS, R>(p.so, i, s_new);
-                st_out<NTS, R>(p.uo, i, u_new);
-                st_out<NTS, R>(p.vo, i, v_new);
-                st_out<NTS_D, double>(p.spo, i, sp_new);
-                st_out<NTS_D, double>(p.upo, i, up_new);
-                st_out<NTS_D, double>(p.vpo, i, vp_new);
+                o.hu = hu_new;
+                o.hv = hv_new;
             }
         }
 
@@ -542,7 +513,233 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
         stt_0 = stt_p1; bf_0 = bf_p1;
         sts_m1 = sts_0; vo_m1 = vo_0;
     }
+};
+
+// Loads of one iteration n of a strip (issued one iteration ahead of their
+// use: measured on MI355X at 2048 x 1536, two iterations ahead runs 5 %
+// slower in FP32 and halves FP64's occupancy).
+template <typename R, bool EXACT>
+struct Loader {
+    const FusedParams &p;
+    const RowAddr ra;
+    const bool colok, frozen;
+    struct Pre {
+        R s, H, u, v, rd, fx, fy;
+        double up, vp, sp, hupc, hvpc, f, sx, sy, wf;
+        int mb;
+    } q;
+
+    __device__ Loader(const FusedParams &pp, int m, bool frozen_)
+        : p(pp), ra{pp.nx, pp.ny, min(max(m, 1), pp.nx)}, colok(m >= 1 && m <= pp.nx), frozen(frozen_)
+    {
+    }
+    __device__ __forceinline__ bool rowok(int row) const { return colok && row >= 1 && row <= p.ny; }
+    __device__ __forceinline__ double tab(const double *t, int row) const
+    {
+        return t[min(max(row, 1), p.ny) - 1];
+    }
+    __device__ __forceinline__ void issue(int n)  // the loads iteration n consumes
+    {
+        const R *S = static_cast<const R *>(p.s), *Hh = static_cast<const R *>(p.H);
+        const R *U = static_cast<const R *>(p.u), *V = static_cast<const R *>(p.v);
+        const R *RD = static_cast<const R *>(p.rd);
+        const R *FX = static_cast<const R *>(p.frx), *FY = static_cast<const R *>(p.fry);
+        const size_t i2 = ra.at(n + 2), i1 = ra.at(n + 1), i0 = ra.at(n);
+        q.s = S[i2];
+        q.H = Hh[i2];
+        q.mb = p.mb[i2];
+        q.u = U[i1];
+        q.v = V[i1];
+        q.up = p.up[i1];
+        q.vp = p.vp[i1];
+        q.sp = p.sp[i1];
+        q.rd = RD[i1];
+        // per-cell frozen forcing and carried depths of the output row n:
+        // element 0 (one cache line per wave) when they are not used
+        const size_t iff = frozen ? i0 : 0;
+        q.fx = FX[iff];
+        q.fy = FY[iff];
+        if constexpr (EXACT) {
+            q.hupc = p.hup_c[iff];
+            q.hvpc = p.hvp_c[iff];
+        } else {
+            q.hupc = q.hvpc = 0.0;
+        }
+        q.f = tab(p.row_f, n);
+        q.sx = tab(p.row_sx, n);
+        q.sy = tab(p.row_sy, n);
+        q.wf = tab(p.row_wf, n + 1);
+    }
+    // the prefetched values of iteration n, zeroed outside the array
+    __device__ __forceinline__ void take(int n, StageIn<R> &in) const
+    {
+        const int k1 = n + 1;
+        in.s2 = rowok(n + 2) ? q.s : R(0);
+        in.H2 = rowok(n + 2) ? q.H : R(0);
+        in.mb2 = rowok(n + 2) ? q.mb : 0;
+        in.u1 = rowok(k1) ? q.u : R(0);
+        in.v1 = rowok(k1) ? q.v : R(0);
+        in.up1 = rowok(k1) ? q.up : 0.0;
+        in.vp1 = rowok(k1) ? q.vp : 0.0;
+        in.sp1 = rowok(k1) ? q.sp : 0.0;
+        in.rd1 = rowok(k1) ? q.rd : R(0);
+        in.fx0 = q.fx;
+        in.fy0 = q.fy;
+        in.hupc0 = q.hupc;
+        in.hvpc0 = q.hvpc;
+        in.f0 = (n >= 1 && n <= p.ny) ? R(q.f) : R(0);
+        in.sx0 = R(q.sx);
+        in.sy0 = R(q.sy);
+        in.wf1 = (k1 >= 1 && k1 <= p.ny) ? R(q.wf) : R(0);
+    }
+};
+
+// Store policy (measured, see st_out): level-n outputs non-temporal in the
+// FP64 modes, the FP64 filtered level non-temporal in every mode.
+template <typename R>
+__device__ __forceinline__ void store_state(const FusedParams &p, size_t i, const StageOut<R> &o)
+{
+    constexpr bool NTS = sizeof(R) == 8;
+    st_out<NTS, R>(p.so, i, o.s);
+    st_out<NTS, R>(p.uo, i, o.u);
+    st_out<NTS, R>(p.vo, i, o.v);
+    st_out<true, double>(p.spo, i, o.sp);
+    st_out<true, double>(p.upo, i, o.up);
+    st_out<true, double>(p.vpo, i, o.vp);
 }
 
+// One work item: `chunk` output rows of one 60-column strip of one basin,
+// substep p.sub.  Called by every lane of a wave with wave-uniform
+// arguments.
+template <typename R, bool EXACT>
+__device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, int strip)
+{
+    const int lane = threadIdx.x;
+    const int m = strip * SW + lane - 1;
+    const bool colok = (m >= 1 && m <= p.nx);
+    const bool writer = colok && lane >= 2 && lane <= SW + 1;
+    const int r0 = chunk_id * p.chunk + 1;
+    const int r1 = min(r0 + p.chunk, p.ny + 1);
+
+    Stage<R, EXACT> st(p, p.sub, m);
+    Loader<R, EXACT> ld(p, m, st.frozen);
+    StageIn<R> in;
+    StageOut<R> o;
+    ld.issue(r0 - 3);
+    for (int n = r0 - 3; n < r1; ++n) {
+        ld.take(n, in);
+        if (n + 1 < r1) ld.issue(n + 1);
+        st.row(n, in, n >= r0, o);
+        if (n >= r0 && writer && n <= p.ny) {
+            const size_t io = (size_t)(n - 1) * p.nx + (m - 1);
+            // the forcing formed by the first substep, for the later ones
+            if (!st.frozen && p.nsub > 1) {
+                st_out<false, R>(p.frx, io, o.rhs_u);
+                st_out<false, R>(p.fry, io, o.rhs_v);
+            }
+            if constexpr (EXACT) {
+                if (p.nsub > 1 && p.sub < p.nsub) {
+                    st_out<sizeof(R) == 8, double>(p.hup_c, io, o.hu);
+                    st_out<sizeof(R) == 8, double>(p.hvp_c, io, o.hv);
+                }
+            }
+            store_state<R>(p, io, o);
+        }
+    }
+}
+
+// Two substeps, p.sub and p.sub + 1, in one march: stage A reads the
+// arrays; stage B runs two rows behind A on A's outputs, which never leave
+// the registers.  A strip is 58 output columns (B's output at lane L
+// needs A's lanes L-1..L+2, which need the loads of L-2..L+4); A starts
+// two rows above the chunk so that B's window sees A's rows from r0-2 on.
+// The results are those of two launches of fused_chunk.
+template <typename R, bool EXACT>
+__device__ __forceinline__ void fused_chunk2(const FusedParams &p, int chunk_id, int strip)
+{
+    const int lane = threadIdx.x;
+    const int m = strip * SW2 + lane - 1;
+    const bool colok = (m >= 1 && m <= p.nx);
+    const bool writer = colok && lane >= 2 && lane <= SW2 + 1;
+    const int r0 = chunk_id * p.chunk + 1;
+    const int r1 = min(r0 + p.chunk, p.ny + 1);
+    const int subB = p.sub + 1;
+
+    Stage<R, EXACT> A(p, p.sub, m), B(p, subB, m);
+    Loader<R, EXACT> ld(p, m, A.frozen);
+    StageIn<R> ia, ib;
+    StageOut<R> oa, ob;
+    // delay lines from A's iteration to B's (B's iteration n - 2 runs in
+    // A's iteration n): the static rows A loaded, A's outputs of row n - 1
+    // (B's level-n input one row ahead) and A's forcing / carried depths
+    R H_d1 = 0, H_d2 = 0, rd_d1 = 0, rd_d2 = 0;
+    int mb_d1 = 0, mb_d2 = 0;
+    R fx_d1 = 0, fx_d2 = 0, fy_d1 = 0, fy_d2 = 0;
+    double hu_d1 = 0, hu_d2 = 0, hv_d1 = 0, hv_d2 = 0;
+    R u_prev = 0, v_prev = 0;
+    double up_prev = 0, vp_prev = 0, sp_prev = 0;
+    ld.issue(r0 - 5);
+    for (int n = r0 - 5; n < r1 + 2; ++n) {
+        ld.take(n, ia);
+        if (n + 1 < r1 + 2) ld.issue(n + 1);
+        const bool keepA = n >= r0 - 2;
+        A.row(n, ia, keepA, oa);
+        // A's row n as the 2-launch sequence would have stored it: zero
+        // outside the array, as B's loads would read it
+        const bool okn = ld.rowok(n);
+        const R sA = okn && keepA ? oa.s : R(0);
+        const R fxA = keepA ? oa.rhs_u : R(0), fyA = keepA ? oa.rhs_v : R(0);
+        const double huA = keepA ? oa.hu : 0.0, hvA = keepA ? oa.hv : 0.0;
+        const int nb = n - 2;
+        if (nb >= r0 - 3) {
+            ib.s2 = sA;
+            ib.H2 = H_d2;
+            ib.mb2 = mb_d2;
+            ib.u1 = u_prev;
+            ib.v1 = v_prev;
+            ib.up1 = up_prev;
+            ib.vp1 = vp_prev;
+            ib.sp1 = sp_prev;
+            ib.rd1 = rd_d2;
+            ib.fx0 = fx_d2;
+            ib.fy0 = fy_d2;
+            ib.hupc0 = hu_d2;
+            ib.hvpc0 = hv_d2;
+            ib.f0 = (nb >= 1 && nb <= p.ny) ? R(ld.tab(p.row_f, nb)) : R(0);
+            ib.sx0 = R(ld.tab(p.row_sx, nb));
+            ib.sy0 = R(ld.tab(p.row_sy, nb));
+            ib.wf1 = (nb + 1 >= 1 && nb + 1 <= p.ny) ? R(ld.tab(p.row_wf, nb + 1)) : R(0);
+            const bool produce = nb >= r0;
+            B.row(nb, ib, produce, ob);
+            if (produce && writer && nb <= p.ny) {
+                const size_t io = (size_t)(nb - 1) * p.nx + (m - 1);
+                if (!A.frozen && p.nsub > subB) {
+                    st_out<false, R>(p.frx, io, ob.rhs_u);
+                    st_out<false, R>(p.fry, io, ob.rhs_v);
+                }
+                if constexpr (EXACT) {
+                    if (subB < p.nsub) {
+                        st_out<sizeof(R) == 8, double>(p.hup_c, io, ob.hu);
+                        st_out<sizeof(R) == 8, double>(p.hvp_c, io, ob.hv);
+                    }
+                }
+                store_state<R>(p, io, ob);
+            }
+        }
+        // shift the delay lines
+        H_d2 = H_d1; H_d1 = ia.H2;
+        mb_d2 = mb_d1; mb_d1 = ia.mb2;
+        rd_d2 = rd_d1; rd_d1 = ia.rd1;
+        fx_d2 = fx_d1; fx_d1 = fxA;
+        fy_d2 = fy_d1; fy_d1 = fyA;
+        hu_d2 = hu_d1; hu_d1 = huA;
+        hv_d2 = hv_d1; hv_d1 = hvA;
+        u_prev = okn && keepA ? oa.u : R(0);
+        v_prev = okn && keepA ? oa.v : R(0);
+        up_prev = okn && keepA ? oa.up : 0.0;
+        vp_prev = okn && keepA ? oa.vp : 0.0;
+        sp_prev = okn && keepA ? oa.sp : 0.0;
+    }
+}
 
 }  // namespace ibt

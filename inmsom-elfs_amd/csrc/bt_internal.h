@@ -84,6 +84,8 @@ struct inmsom_bt_ctx {
     ibt::FusedAlloc *fa = nullptr;  // ping-pong state buffers of the fused step
     int fz_cur = 0;
     int chunk = 0;                 // output rows per wave of the fused step (0 = per-mode default)
+    int subs_per_launch = 0;       // fused substeps per launch (0 = by mode, 1, 2 = two in one march)
+    long long slots[6] = {};       // waves the device holds at once, per fused kernel (0 = not queried)
     double *scal = nullptr;        // device scalars (reductions)
     double *cpl = nullptr;         // staging of the 2-D -> 3-D coupling (bt_coupling.hip)
     size_t cpl_n = 0;              // doubles in cpl

@@ -56,6 +56,8 @@ SIGNATURES = {
     "inmsom_bt_run_timed": ([C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int, _dp], C.c_int),
     "inmsom_bt_set_launch": ([C.c_void_p, C.c_int], C.c_int),
     "inmsom_bt_get_launch": ([C.c_void_p, C.POINTER(C.c_int)], C.c_int),
+    "inmsom_bt_set_substeps_per_launch": ([C.c_void_p, C.c_int], C.c_int),
+    "inmsom_bt_get_substeps_per_launch": ([C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
     "inmsom_bt_range_check": ([C.c_void_p, C.POINTER(C.c_longlong)], C.c_int),
     "inmsom_bt_volume": ([C.c_void_p, _dp], C.c_int),
     "inmsom_bt_synchronize": ([C.c_void_p], C.c_int),

@@ -189,6 +189,17 @@ class BarotropicModel:
               "inmsom_bt_run_timed")
         return ms.value
 
+    def set_substeps_per_launch(self, k: int):
+        """Fused step: 2 = substep pairs in one launch, 1 = one launch per
+        substep, 0 = by mode (default); results do not depend on it."""
+        check(lib().inmsom_bt_set_substeps_per_launch(self._h, int(k)), "inmsom_bt_set_substeps_per_launch")
+
+    def substeps_per_launch(self, mode: str) -> int:
+        k = C.c_int()
+        check(lib().inmsom_bt_get_substeps_per_launch(self._h, _capi.MODES[mode], C.byref(k)),
+              "inmsom_bt_get_substeps_per_launch")
+        return k.value
+
     def set_launch(self, rows_per_wave: int):
         check(lib().inmsom_bt_set_launch(self._h, int(rows_per_wave)), "inmsom_bt_set_launch")
 

@@ -84,6 +84,25 @@ def test_fused_exact_bitwise(shape, nstep):
     assert_bitwise(got, want, STATE, f"fused fp64_exact {shape} nstep={nstep}")
 
 
+@pytest.mark.parametrize("mode", ["fp64_exact", "fp64", "fp32"])
+@pytest.mark.parametrize("shape,nstep,chunk", [((150, 110), 2, 0), ((150, 110), 3, 5), ((61, 7), 4, 0),
+                                               ((300, 64), 2, 3), ((57, 40), 2, 1), ((116, 91), 4, 7)])
+def test_two_substeps_per_launch_bitwise(mode, shape, nstep, chunk):
+    """Two substeps in one march give what two launches give, bit for bit,
+    in every mode, over strip edges, chunk edges and odd nstep."""
+    b = randomize_state(make_basin(*shape, seed=33, sigma=5.0), 9)
+    steps = 9
+    print(f"two substeps/launch {mode} {shape} nstep={nstep} chunk={chunk}", flush=True)
+    out = {}
+    for k in (1, 2):
+        with BarotropicModel(b) as m:
+            m.set_launch(chunk)
+            m.set_substeps_per_launch(k)
+            m.run(steps, mode, nstep=nstep)
+            out[k] = m.get_state()
+    assert_bitwise(out[2], out[1], STATE, f"two vs one substep per launch, {mode} {shape} nstep={nstep}")
+
+
 def test_fused_fp64_within_1e12():
     """Reassociated FP64 step: relative L2 <= 1e-12 after 200 outer steps."""
     b = make_basin(200, 150, seed=41, sigma=8.0)
