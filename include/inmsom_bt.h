@@ -200,6 +200,8 @@ int inmsom_bt_ens_get_state(inmsom_bt_ens *ens, int member, double *u, double *u
 /* launch shape: output rows per wave (0 = default) and waves per XCD of
  * the persistent grid (0 = default); results do not depend on it */
 int inmsom_bt_ens_set_launch(inmsom_bt_ens *ens, int rows_per_wave, int waves_per_xcd);
+/* substeps per launch, as inmsom_bt_set_substeps_per_launch (0 = by mode) */
+int inmsom_bt_ens_set_substeps_per_launch(inmsom_bt_ens *ens, int substeps);
 /* nsteps outer steps of tau, nstep substeps each; use_graph = 1 replays a
  * captured graph per outer step, 0 launches directly */
 int inmsom_bt_ens_run(inmsom_bt_ens *ens, double tau, int nstep, int nsteps, int use_graph);

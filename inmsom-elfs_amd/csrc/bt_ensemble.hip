@@ -41,6 +41,28 @@ __global__ void __launch_bounds__(WAVE)
     }
 }
 
+// Substeps sub and sub + 1 in one march per item (58-column strips).
+template <typename R, bool EXACT>
+__global__ void __launch_bounds__(WAVE)
+    k_fused_batch2(const FusedParams *__restrict__ table, const int *__restrict__ item_member,
+                   const int *__restrict__ item_local, int total, int per_xcd, int waves_per_xcd,
+                   double tau_inner, int sub, int nsub)
+{
+    const int w = blockIdx.x;
+    const int xcd = w % NXCD, j = w / NXCD;
+    const int t0 = xcd * per_xcd, t1 = min(t0 + per_xcd, total);
+    for (int t = t0 + j; t < t1; t += waves_per_xcd) {
+        FusedParams p = table[item_member[t]];
+        p.tau = tau_inner;
+        p.sub = sub;
+        p.nsub = nsub;
+        p.dn = (double)nsub;
+        const int local = item_local[t];
+        const int chunk_id = local / p.strips, strip = local - chunk_id * p.strips;
+        fused_chunk2<R, EXACT>(p, chunk_id, strip);
+    }
+}
+
 template <class R>
 __global__ void k_member_range(int nx, int ny, const uint8_t *mb, const R *ssh, const R *u, const R *v,
                                unsigned long long *bad)
@@ -107,12 +129,15 @@ struct inmsom_bt_ens {
     int total = 0;
     int chunk = 0;          // rows per wave, 0 = per-mode default
     int waves_per_xcd = 0;  // persistent grid, 0 = default
+    int subs = 0;           // substeps per launch: 0 = by mode, 1, 2
     int built_chunk = -1;
+    bool built_two = false;
     int phase = 0;          // buffer set holding the current state
     // graph of one outer step for (tau, nstep, start phase)
     hipGraphExec_t gexec[2] = {};
     double g_tau = -1.0;
     int g_nstep = -1, g_chunk = -1, g_wpx = -1;
+    bool g_two = false;
     double *rows = nullptr;
     unsigned long long *cnt = nullptr;
     // snapshot of every member's state (inmsom_bt_ens_save / _restore)
@@ -141,17 +166,50 @@ int fail(const std::string &m, int code)
 
 size_t esz(int mode) { return mode == INMSOM_BT_FP32 ? 4 : 8; }
 
-int default_chunk(const inmsom_bt_ens *e)
+// two substeps per launch: as set, or by mode (not in FP64_EXACT, whose
+// IEEE divides make it compute-bound)
+bool two_per_launch(const inmsom_bt_ens *e)
 {
-    long long sr = 0;
-    for (const EnsMember &m : e->mem) sr += (long long)((m.nx + 2 + SW - 1) / SW) * m.ny;
-    return auto_chunk(e->mode == INMSOM_BT_FP32, sr);
+    return e->subs > 0 ? e->subs == 2 : e->mode != INMSOM_BT_FP64_EXACT;
 }
 
-int default_waves(int mode)
+// Waves of the batch kernel one XCD holds at once: its CUs x the kernel's
+// waves per CU from the occupancy calculator.
+int default_waves(const inmsom_bt_ens *e, bool two)
 {
-    // 32 CUs per XCD x 4 SIMDs x the fused kernel's occupancy (2 FP64, 3 FP32)
-    return mode == INMSOM_BT_FP32 ? 384 : 256;
+    const void *k;
+    if (e->mode == INMSOM_BT_FP64_EXACT)
+        k = two ? (const void *)&k_fused_batch2<double, true> : (const void *)&k_fused_batch<double, true>;
+    else if (e->mode == INMSOM_BT_FP64)
+        k = two ? (const void *)&k_fused_batch2<double, false> : (const void *)&k_fused_batch<double, false>;
+    else
+        k = two ? (const void *)&k_fused_batch2<float, false> : (const void *)&k_fused_batch<float, false>;
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device) != hipSuccess || cus < NXCD)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, WAVE, 0) != hipSuccess || per_cu < 1)
+        per_cu = 4;
+    return cus / NXCD * per_cu;
+}
+
+// Rows per wave: the chunk (4..64) that minimises rounds x (chunk + warm-up
+// rows) over all members' items for the persistent grid's wave slots.
+int default_chunk(const inmsom_bt_ens *e, bool two, int wpx)
+{
+    const int sw = two ? SW2 : SW, warm = two ? 7 : 3;
+    const long long slots = (long long)wpx * NXCD;
+    int best = 16;
+    long long best_cost = -1;
+    for (int c = 4; c <= 64; ++c) {
+        long long items = 0;
+        for (const EnsMember &m : e->mem) items += (long long)((m.nx + 2 + sw - 1) / sw) * ((m.ny + c - 1) / c);
+        const long long cost = (items + slots - 1) / slots * (c + warm);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = c;
+        }
+    }
+    return best;
 }
 
 template <class T>
@@ -262,12 +320,13 @@ int build_member(inmsom_bt_ens *e, const inmsom_bt_grid *gr, const inmsom_bt_for
     return INMSOM_BT_OK;
 }
 
-FusedParams member_params(const inmsom_bt_ens *e, const EnsMember &mb, int phase, int chunk)
+FusedParams member_params(const inmsom_bt_ens *e, const EnsMember &mb, int phase, int chunk, bool two)
 {
     FusedParams p{};
     p.nx = mb.nx;
     p.ny = mb.ny;
-    p.strips = (mb.nx + 2 + SW - 1) / SW;
+    const int sw = two ? SW2 : SW;
+    p.strips = (mb.nx + 2 + sw - 1) / sw;
     p.chunk = chunk;
     p.nchunks = (mb.ny + chunk - 1) / chunk;
     p.total = p.strips * p.nchunks;
@@ -308,13 +367,13 @@ FusedParams member_params(const inmsom_bt_ens *e, const EnsMember &mb, int phase
 }
 
 // (Re)builds the descriptor tables and the work list for a launch shape.
-int build_tables(inmsom_bt_ens *e, int chunk)
+int build_tables(inmsom_bt_ens *e, int chunk, bool two)
 {
-    if (e->built_chunk == chunk) return INMSOM_BT_OK;
+    if (e->built_chunk == chunk && e->built_two == two) return INMSOM_BT_OK;
     std::vector<FusedParams> t[2];
     std::vector<int> im, il;
     for (size_t k = 0; k < e->mem.size(); ++k) {
-        for (int ph = 0; ph < 2; ++ph) t[ph].push_back(member_params(e, e->mem[k], ph, chunk));
+        for (int ph = 0; ph < 2; ++ph) t[ph].push_back(member_params(e, e->mem[k], ph, chunk, two));
         const int items = t[0].back().total;
         e->mem[k].items = items;
         for (int i = 0; i < items; ++i) {
@@ -335,14 +394,27 @@ int build_tables(inmsom_bt_ens *e, int chunk)
     if ((rc = upload_vec(e, &e->item_member, im)) || (rc = upload_vec(e, &e->item_local, il))) return rc;
     e->total = (int)im.size();
     e->built_chunk = chunk;
+    e->built_two = two;
     return INMSOM_BT_OK;
 }
 
-void launch_batch(inmsom_bt_ens *e, int phase, double tau_inner, int sub, int nsub, int wpx)
+void launch_batch(inmsom_bt_ens *e, int phase, double tau_inner, int sub, int nsub, int wpx, bool two)
 {
     const int per_xcd = (e->total + NXCD - 1) / NXCD;
     dim3 grid((unsigned)(wpx * NXCD)), blk(WAVE);
     const FusedParams *tb = e->table[phase];
+    if (two) {
+        if (e->mode == INMSOM_BT_FP64_EXACT)
+            hipLaunchKernelGGL((k_fused_batch2<double, true>), grid, blk, 0, e->stream, tb, e->item_member,
+                               e->item_local, e->total, per_xcd, wpx, tau_inner, sub, nsub);
+        else if (e->mode == INMSOM_BT_FP64)
+            hipLaunchKernelGGL((k_fused_batch2<double, false>), grid, blk, 0, e->stream, tb, e->item_member,
+                               e->item_local, e->total, per_xcd, wpx, tau_inner, sub, nsub);
+        else
+            hipLaunchKernelGGL((k_fused_batch2<float, false>), grid, blk, 0, e->stream, tb, e->item_member,
+                               e->item_local, e->total, per_xcd, wpx, tau_inner, sub, nsub);
+        return;
+    }
     if (e->mode == INMSOM_BT_FP64_EXACT)
         hipLaunchKernelGGL((k_fused_batch<double, true>), grid, blk, 0, e->stream, tb, e->item_member,
                            e->item_local, e->total, per_xcd, wpx, tau_inner, sub, nsub);
@@ -354,12 +426,17 @@ void launch_batch(inmsom_bt_ens *e, int phase, double tau_inner, int sub, int ns
                            e->item_local, e->total, per_xcd, wpx, tau_inner, sub, nsub);
 }
 
+// launches per outer step: substep pairs, and an odd last substep alone
+int launches_per_step(int nstep, bool two) { return two ? nstep / 2 + nstep % 2 : nstep; }
+
 // launches of one outer step starting from buffer set `phase`
-void outer_step(inmsom_bt_ens *e, int phase, double tau, int nstep, int wpx)
+void outer_step(inmsom_bt_ens *e, int phase, double tau, int nstep, int wpx, bool two)
 {
-    for (int sub = 1; sub <= nstep; ++sub) {
-        launch_batch(e, phase, tau / (double)nstep, sub, nstep, wpx);
+    for (int sub = 1; sub <= nstep;) {
+        const bool pair = two && sub + 1 <= nstep;
+        launch_batch(e, phase, tau / (double)nstep, sub, nstep, wpx, pair);
         phase = 1 - phase;
+        sub += pair ? 2 : 1;
     }
     if (e->closure)
         ibt::cnn::closure_launch(e->closure, e->stream, e->mode == INMSOM_BT_FP32, e->cl_mem[phase], e->inf, tau,
@@ -387,17 +464,17 @@ void drop_graphs(inmsom_bt_ens *e)
     e->g_nstep = -1;
 }
 
-int ensure_graphs(inmsom_bt_ens *e, double tau, int nstep, int chunk, int wpx)
+int ensure_graphs(inmsom_bt_ens *e, double tau, int nstep, int chunk, int wpx, bool two)
 {
     if (e->gexec[0] && e->g_tau == tau && e->g_nstep == nstep && e->g_chunk == chunk &&
-        e->g_wpx == wpx)
+        e->g_wpx == wpx && e->g_two == two)
         return INMSOM_BT_OK;
     drop_graphs(e);
-    const int nph = (nstep % 2) ? 2 : 1;
+    const int nph = (launches_per_step(nstep, two) % 2) ? 2 : 1;
     for (int ph = 0; ph < nph; ++ph) {
         hipGraph_t graph = nullptr;
         EHC(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-        outer_step(e, ph, tau, nstep, wpx);
+        outer_step(e, ph, tau, nstep, wpx, two);
         EHC(hipStreamEndCapture(e->stream, &graph));
         hipError_t r = hipGraphInstantiate(&e->gexec[ph], graph, nullptr, nullptr, 0);
         hipGraphDestroy(graph);
@@ -408,26 +485,38 @@ int ensure_graphs(inmsom_bt_ens *e, double tau, int nstep, int chunk, int wpx)
     e->g_nstep = nstep;
     e->g_chunk = chunk;
     e->g_wpx = wpx;
+    e->g_two = two;
     return INMSOM_BT_OK;
+}
+
+// the launch shape of the next run: (two substeps per launch, waves per
+// XCD, rows per wave), each as set or by default
+void launch_shape(const inmsom_bt_ens *e, bool *two, int *wpx, int *chunk)
+{
+    *two = two_per_launch(e);
+    *wpx = e->waves_per_xcd > 0 ? e->waves_per_xcd : default_waves(e, *two);
+    *chunk = e->chunk > 0 ? e->chunk : default_chunk(e, *two, *wpx);
 }
 
 int run_steps(inmsom_bt_ens *e, double tau, int nstep, int nsteps, int use_graph)
 {
-    const int chunk = e->chunk > 0 ? e->chunk : default_chunk(e);
-    const int wpx = e->waves_per_xcd > 0 ? e->waves_per_xcd : default_waves(e->mode);
-    int rc = build_tables(e, chunk);
+    bool two;
+    int wpx, chunk;
+    launch_shape(e, &two, &wpx, &chunk);
+    int rc = build_tables(e, chunk, two);
     if (rc) return rc;
+    const bool flip = launches_per_step(nstep, two) % 2;
     if (use_graph) {
-        rc = ensure_graphs(e, tau, nstep, chunk, wpx);
+        rc = ensure_graphs(e, tau, nstep, chunk, wpx, two);
         if (rc) return rc;
         for (int s = 0; s < nsteps; ++s) {
-            EHC(hipGraphLaunch(e->gexec[(nstep % 2) ? e->phase : 0], e->stream));
-            if (nstep % 2) e->phase = 1 - e->phase;
+            EHC(hipGraphLaunch(e->gexec[flip ? e->phase : 0], e->stream));
+            if (flip) e->phase = 1 - e->phase;
         }
     } else {
         for (int s = 0; s < nsteps; ++s) {
-            outer_step(e, e->phase, tau, nstep, wpx);
-            if (nstep % 2) e->phase = 1 - e->phase;
+            outer_step(e, e->phase, tau, nstep, wpx, two);
+            if (flip) e->phase = 1 - e->phase;
         }
     }
     EHC(hipGetLastError());
@@ -583,6 +672,14 @@ int inmsom_bt_ens_set_launch(inmsom_bt_ens *e, int rows_per_wave, int waves_per_
     return INMSOM_BT_OK;
 }
 
+int inmsom_bt_ens_set_substeps_per_launch(inmsom_bt_ens *e, int substeps)
+{
+    if (!e) return fail("null ensemble", INMSOM_BT_EINVAL);
+    if (substeps < 0 || substeps > 2) return fail("substeps per launch must be 0 (by mode), 1 or 2", INMSOM_BT_EINVAL);
+    e->subs = substeps;
+    return INMSOM_BT_OK;
+}
+
 int inmsom_bt_ens_run(inmsom_bt_ens *e, double tau, int nstep, int nsteps, int use_graph)
 {
     if (!e) return fail("null ensemble", INMSOM_BT_EINVAL);
@@ -603,11 +700,12 @@ int inmsom_bt_ens_run_timed(inmsom_bt_ens *e, double tau, int nstep, int nsteps,
         return fail("bad tau / nstep / nsteps", INMSOM_BT_EINVAL);
     EHC(hipSetDevice(e->device));
     // build tables and graphs before the timed region
-    const int chunk = e->chunk > 0 ? e->chunk : default_chunk(e);
-    const int wpx = e->waves_per_xcd > 0 ? e->waves_per_xcd : default_waves(e->mode);
-    int rc = build_tables(e, chunk);
+    bool two;
+    int wpx, chunk;
+    launch_shape(e, &two, &wpx, &chunk);
+    int rc = build_tables(e, chunk, two);
     if (rc) return rc;
-    if (use_graph && (rc = ensure_graphs(e, tau, nstep, chunk, wpx))) return rc;
+    if (use_graph && (rc = ensure_graphs(e, tau, nstep, chunk, wpx, two))) return rc;
     hipEvent_t e0, e1;
     EHC(hipEventCreate(&e0));
     EHC(hipEventCreate(&e1));

@@ -21,8 +21,8 @@ from ._capi import check, lib
 from .cases import Basin
 from .model import STATE, _d, _grid_struct, rayleigh
 
-DEFAULT_ROWS = (8, 12, 16, 20, 24, 32, 48)
-DEFAULT_WAVES = (128, 256, 384, 512)
+DEFAULT_ROWS = (0, 12, 16, 24, 32, 40, 48, 64)
+DEFAULT_WAVES = (0, 256, 384, 512)
 
 
 class Ensemble:
@@ -95,6 +95,12 @@ class Ensemble:
         check(lib().inmsom_bt_ens_set_launch(self._h, rows_per_wave, waves_per_xcd),
               "inmsom_bt_ens_set_launch")
         self.launch = (rows_per_wave, waves_per_xcd)
+
+    def set_substeps_per_launch(self, k: int):
+        """2 = substep pairs in one launch, 1 = one launch per substep,
+        0 = by mode (default); results do not depend on it."""
+        check(lib().inmsom_bt_ens_set_substeps_per_launch(self._h, int(k)),
+              "inmsom_bt_ens_set_substeps_per_launch")
 
     def run(self, nsteps: int, nstep: int = 2, tau: Optional[float] = None, graph: bool = True):
         tau = self.basins[0].dt if tau is None else tau

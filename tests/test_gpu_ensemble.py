@@ -53,7 +53,7 @@ def test_graph_replays_match_direct_launches():
 def test_launch_shape_does_not_change_results():
     basins = _members(3, seed=9)
     outs = []
-    for rows, waves in ((0, 0), (7, 37), (48, 512)):
+    for rows, waves in ((0, 0), (7, 37), (48, 512), (3, 130)):
         with Ensemble(basins, mode="fp64") as ens:
             ens.set_launch(rows, waves)
             ens.run(5, nstep=2)
@@ -61,6 +61,22 @@ def test_launch_shape_does_not_change_results():
     for o in outs[1:]:
         for k in range(len(basins)):
             assert_bitwise(o[k], outs[0][k], STATE, f"member {k}")
+
+
+@pytest.mark.parametrize("mode,nstep", [("fp32", 2), ("fp64", 3), ("fp64_exact", 2), ("fp32", 4)])
+def test_two_substeps_per_launch_equal_one(mode, nstep):
+    """Substep pairs in one persistent launch give the same members, bit for
+    bit, as one launch per substep, with and without graphs."""
+    basins = _members(4, seed=11)
+    outs = []
+    for k, graph in ((1, True), (2, True), (2, False)):
+        with Ensemble(basins, mode=mode) as ens:
+            ens.set_substeps_per_launch(k)
+            ens.run(5, nstep=nstep, graph=graph)
+            outs.append([ens.get_state(j) for j in range(len(basins))])
+    for o in outs[1:]:
+        for j in range(len(basins)):
+            assert_bitwise(o[j], outs[0][j], STATE, f"member {j} {mode} nstep={nstep}")
 
 
 def test_autotune_restores_state_and_conserves_volume(tmp_path):
