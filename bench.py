@@ -59,7 +59,7 @@ def rel_l2(a, b):
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "fused_step_cfg1_traffic.json")
 # the kernel each leg launches (two substeps per launch in FP64 and FP32)
 TRAFFIC_KERNELS = {"fp64": "k_fused_step2<double, false>", "fp32": "k_fused_step2<float, false>",
-                   "fp64_exact": "k_fused_step<double, true>", "ensemble_fp32": "k_fused_batch<float, false>"}
+                   "fp64_exact": "k_fused_step<double, true>", "ensemble_fp32": "k_fused_batch2<float, false>"}
 
 
 # kernels of one 2-D -> 3-D coupling and their launches per coupling
@@ -222,17 +222,19 @@ def ensemble_leg(steps, warmup, nmembers):
         ms = ens.run_timed(steps, NSTEP_BAROTROP)
         bad = ens.range_check()
         drift = max(abs(ens.volume(k) - v0[k]) / abs(v0[k]) for k in range(nmem))
-    launches = steps * NSTEP_BAROTROP
+        subs = ens.substeps_per_launch()
+    substeps = steps * NSTEP_BAROTROP
+    launches = substeps // subs
     avg = ms * 1e-3 / launches
     bpc = BYTES_PER_CELL["fp32"]
-    achieved = bpc * cells / avg / 1e9
-    res = {"value": wet * launches / (ms * 1e-3), "unit": "wet-cell steps/s",
+    achieved = bpc * subs * cells / avg / 1e9
+    res = {"value": wet * substeps / (ms * 1e-3), "unit": "wet-cell steps/s",
            "members": nmem, "wet_cells": wet, "array_cells": cells, "steps": steps,
-           "ms_per_step": ms / steps, "avg_launch_us": avg * 1e6,
+           "ms_per_step": ms / steps, "avg_launch_us": avg * 1e6, "substeps_per_launch": subs,
            "launch": {"rows_per_wave": tune["rows_per_wave"], "waves_per_xcd": tune["waves_per_xcd"]},
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_from_profiles("ensemble_fp32"),
-                        "bytes_per_cell": bpc, "cells_per_launch": cells},
+                        "bytes_per_cell_substep": bpc, "cells_per_launch": cells, "substeps_per_launch": subs},
            "range_check_bad_cells": bad, "max_volume_rel_drift": drift}
     log(f"ensemble fp32: {ms:.1f} ms for {steps} steps ({avg * 1e6:.1f} us/launch), "
         f"{res['value']:.3e} wet-cell steps/s, {achieved:.0f} GB/s")

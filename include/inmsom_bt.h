@@ -157,8 +157,9 @@ int inmsom_bt_get_launch(inmsom_bt_ctx *ctx, int *rows_per_wave);
 /* Substeps of one call that the fused step advances per launch: 2 (with
  * nstep >= 2 each pair of substeps is one march over the rows, the second
  * substep taking the first's rows from registers; an odd last substep runs
- * alone), 1 (one launch per substep), or 0 (default: 2 in FP64 and FP32, 1
- * in FP64_EXACT).  Results do not depend on it. */
+ * alone), 1 (one launch per substep), or 0 (default: 2 in FP64 and FP32
+ * from 2^20 cells on, 1 in FP64_EXACT and on smaller grids).  Results do
+ * not depend on it. */
 int inmsom_bt_set_substeps_per_launch(inmsom_bt_ctx *ctx, int substeps);
 /* the substeps per launch the fused step uses in `mode` (1 or 2) */
 int inmsom_bt_get_substeps_per_launch(inmsom_bt_ctx *ctx, int mode, int *substeps);
@@ -202,6 +203,7 @@ int inmsom_bt_ens_get_state(inmsom_bt_ens *ens, int member, double *u, double *u
 int inmsom_bt_ens_set_launch(inmsom_bt_ens *ens, int rows_per_wave, int waves_per_xcd);
 /* substeps per launch, as inmsom_bt_set_substeps_per_launch (0 = by mode) */
 int inmsom_bt_ens_set_substeps_per_launch(inmsom_bt_ens *ens, int substeps);
+int inmsom_bt_ens_get_substeps_per_launch(inmsom_bt_ens *ens, int *substeps);
 /* nsteps outer steps of tau, nstep substeps each; use_graph = 1 replays a
  * captured graph per outer step, 0 launches directly */
 int inmsom_bt_ens_run(inmsom_bt_ens *ens, double tau, int nstep, int nsteps, int use_graph);

@@ -20,7 +20,7 @@
 
 namespace ibt {
 
-template <typename R, bool EXACT>
+template <typename R, bool EXACT, bool FORM>
 __global__ void __launch_bounds__(WAVE)
     k_fused_batch(const FusedParams *__restrict__ table, const int *__restrict__ item_member,
                   const int *__restrict__ item_local, int total, int per_xcd, int waves_per_xcd,
@@ -37,12 +37,12 @@ __global__ void __launch_bounds__(WAVE)
         p.dn = (double)nsub;
         const int local = item_local[t];
         const int chunk_id = local / p.strips, strip = local - chunk_id * p.strips;
-        fused_chunk<R, EXACT>(p, chunk_id, strip);
+        fused_chunk<R, EXACT, FORM>(p, chunk_id, strip);
     }
 }
 
 // Substeps sub and sub + 1 in one march per item (58-column strips).
-template <typename R, bool EXACT>
+template <typename R, bool EXACT, bool AFORM>
 __global__ void __launch_bounds__(WAVE)
     k_fused_batch2(const FusedParams *__restrict__ table, const int *__restrict__ item_member,
                    const int *__restrict__ item_local, int total, int per_xcd, int waves_per_xcd,
@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(WAVE)
         p.dn = (double)nsub;
         const int local = item_local[t];
         const int chunk_id = local / p.strips, strip = local - chunk_id * p.strips;
-        fused_chunk2<R, EXACT>(p, chunk_id, strip);
+        fused_chunk2<R, EXACT, AFORM>(p, chunk_id, strip);
     }
 }
 
@@ -166,33 +166,46 @@ int fail(const std::string &m, int code)
 
 size_t esz(int mode) { return mode == INMSOM_BT_FP32 ? 4 : 8; }
 
-// two substeps per launch: as set, or by mode (not in FP64_EXACT, whose
-// IEEE divides make it compute-bound)
+// two substeps per launch: as set, or by mode and size as for one basin
+// (not in FP64_EXACT, whose IEEE divides make it compute-bound, nor below
+// TWO_SUBSTEP_CELLS cells in all, where the launches are latency-bound)
 bool two_per_launch(const inmsom_bt_ens *e)
 {
-    return e->subs > 0 ? e->subs == 2 : e->mode != INMSOM_BT_FP64_EXACT;
+    if (e->subs > 0) return e->subs == 2;
+    long long cells = 0;
+    for (const EnsMember &m : e->mem) cells += (long long)m.n;
+    return e->mode != INMSOM_BT_FP64_EXACT && cells >= TWO_SUBSTEP_CELLS;
 }
 
 // Waves of the batch kernel one XCD holds at once: its CUs x the kernel's
 // waves per CU from the occupancy calculator.
+// the batch kernel for (mode, two substeps, forming substep)
+template <bool FORM>
+const void *batch_kernel(int mode, bool two)
+{
+    if (two) {
+        if (mode == INMSOM_BT_FP64_EXACT) return (const void *)&k_fused_batch2<double, true, FORM>;
+        if (mode == INMSOM_BT_FP64) return (const void *)&k_fused_batch2<double, false, FORM>;
+        return (const void *)&k_fused_batch2<float, false, FORM>;
+    }
+    if (mode == INMSOM_BT_FP64_EXACT) return (const void *)&k_fused_batch<double, true, FORM>;
+    if (mode == INMSOM_BT_FP64) return (const void *)&k_fused_batch<double, false, FORM>;
+    return (const void *)&k_fused_batch<float, false, FORM>;
+}
+
 int default_waves(const inmsom_bt_ens *e, bool two)
 {
-    const void *k;
-    if (e->mode == INMSOM_BT_FP64_EXACT)
-        k = two ? (const void *)&k_fused_batch2<double, true> : (const void *)&k_fused_batch<double, true>;
-    else if (e->mode == INMSOM_BT_FP64)
-        k = two ? (const void *)&k_fused_batch2<double, false> : (const void *)&k_fused_batch<double, false>;
-    else
-        k = two ? (const void *)&k_fused_batch2<float, false> : (const void *)&k_fused_batch<float, false>;
     int cus = 0, per_cu = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device) != hipSuccess || cus < NXCD)
         cus = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, WAVE, 0) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<true>(e->mode, two), WAVE, 0) !=
+            hipSuccess ||
+        per_cu < 1)
         per_cu = 4;
     return cus / NXCD * per_cu;
 }
 
-// Rows per wave: the chunk (4..64) that minimises rounds x (chunk + warm-up
+// Rows per wave: the chunk (2..64) that minimises rounds x (chunk + warm-up
 // rows) over all members' items for the persistent grid's wave slots.
 int default_chunk(const inmsom_bt_ens *e, bool two, int wpx)
 {
@@ -200,7 +213,7 @@ int default_chunk(const inmsom_bt_ens *e, bool two, int wpx)
     const long long slots = (long long)wpx * NXCD;
     int best = 16;
     long long best_cost = -1;
-    for (int c = 4; c <= 64; ++c) {
+    for (int c = 2; c <= 64; ++c) {
         long long items = 0;
         for (const EnsMember &m : e->mem) items += (long long)((m.nx + 2 + sw - 1) / sw) * ((m.ny + c - 1) / c);
         const long long cost = (items + slots - 1) / slots * (c + warm);
@@ -400,30 +413,14 @@ int build_tables(inmsom_bt_ens *e, int chunk, bool two)
 
 void launch_batch(inmsom_bt_ens *e, int phase, double tau_inner, int sub, int nsub, int wpx, bool two)
 {
-    const int per_xcd = (e->total + NXCD - 1) / NXCD;
+    int per_xcd = (e->total + NXCD - 1) / NXCD;
     dim3 grid((unsigned)(wpx * NXCD)), blk(WAVE);
     const FusedParams *tb = e->table[phase];
-    if (two) {
-        if (e->mode == INMSOM_BT_FP64_EXACT)
-            hipLaunchKernelGGL((k_fused_batch2<double, true>), grid, blk, 0, e->stream, tb, e->item_member,
-                               e->item_local, e->total, per_xcd, wpx, tau_inner, sub, nsub);
-        else if (e->mode == INMSOM_BT_FP64)
-            hipLaunchKernelGGL((k_fused_batch2<double, false>), grid, blk, 0, e->stream, tb, e->item_member,
-                               e->item_local, e->total, per_xcd, wpx, tau_inner, sub, nsub);
-        else
-            hipLaunchKernelGGL((k_fused_batch2<float, false>), grid, blk, 0, e->stream, tb, e->item_member,
-                               e->item_local, e->total, per_xcd, wpx, tau_inner, sub, nsub);
-        return;
-    }
-    if (e->mode == INMSOM_BT_FP64_EXACT)
-        hipLaunchKernelGGL((k_fused_batch<double, true>), grid, blk, 0, e->stream, tb, e->item_member,
-                           e->item_local, e->total, per_xcd, wpx, tau_inner, sub, nsub);
-    else if (e->mode == INMSOM_BT_FP64)
-        hipLaunchKernelGGL((k_fused_batch<double, false>), grid, blk, 0, e->stream, tb, e->item_member,
-                           e->item_local, e->total, per_xcd, wpx, tau_inner, sub, nsub);
-    else
-        hipLaunchKernelGGL((k_fused_batch<float, false>), grid, blk, 0, e->stream, tb, e->item_member,
-                           e->item_local, e->total, per_xcd, wpx, tau_inner, sub, nsub);
+    const int *im = e->item_member, *il = e->item_local;
+    int total = e->total;
+    void *args[] = {&tb, &im, &il, &total, &per_xcd, &wpx, &tau_inner, &sub, &nsub};
+    const void *k = sub == 1 ? batch_kernel<true>(e->mode, two) : batch_kernel<false>(e->mode, two);
+    hipLaunchKernel(k, grid, blk, args, 0, e->stream);
 }
 
 // launches per outer step: substep pairs, and an odd last substep alone
@@ -677,6 +674,13 @@ int inmsom_bt_ens_set_substeps_per_launch(inmsom_bt_ens *e, int substeps)
     if (!e) return fail("null ensemble", INMSOM_BT_EINVAL);
     if (substeps < 0 || substeps > 2) return fail("substeps per launch must be 0 (by mode), 1 or 2", INMSOM_BT_EINVAL);
     e->subs = substeps;
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_ens_get_substeps_per_launch(inmsom_bt_ens *e, int *substeps)
+{
+    if (!e || !substeps) return fail("null pointer", INMSOM_BT_EINVAL);
+    *substeps = two_per_launch(e) ? 2 : 1;
     return INMSOM_BT_OK;
 }
 

@@ -36,7 +36,7 @@
 
 namespace ibt {
 
-template <typename R, bool EXACT>
+template <typename R, bool EXACT, bool FORM>
 __global__ void __launch_bounds__(WAVE) k_fused_step(FusedParams p)
 {
     // XCD-aware work mapping: workgroup w runs on XCD w % 8; XCD k takes
@@ -45,18 +45,18 @@ __global__ void __launch_bounds__(WAVE) k_fused_step(FusedParams p)
     const int t = (w % NXCD) * p.per_xcd + (w / NXCD);
     if (t >= p.total) return;
     const int chunk_id = t / p.strips, strip = t - chunk_id * p.strips;
-    fused_chunk<R, EXACT>(p, chunk_id, strip);
+    fused_chunk<R, EXACT, FORM>(p, chunk_id, strip);
 }
 
 // Substeps p.sub and p.sub + 1 in one launch (58-column strips).
-template <typename R, bool EXACT>
+template <typename R, bool EXACT, bool AFORM>
 __global__ void __launch_bounds__(WAVE) k_fused_step2(FusedParams p)
 {
     const int w = blockIdx.x;
     const int t = (w % NXCD) * p.per_xcd + (w / NXCD);
     if (t >= p.total) return;
     const int chunk_id = t / p.strips, strip = t - chunk_id * p.strips;
-    fused_chunk2<R, EXACT>(p, chunk_id, strip);
+    fused_chunk2<R, EXACT, AFORM>(p, chunk_id, strip);
 }
 
 // -------------------------------------------------------------- helpers
@@ -376,14 +376,30 @@ int fused_flush(inmsom_bt_ctx *c)
 
 // Parameters of a launch that reads buffer set b and writes 1 - b: substep
 // `sub`, or substeps sub and sub + 1 when `two`.
-// Substeps per launch: as set, or automatically two in FP64 and FP32 and
-// one in FP64_EXACT, whose IEEE divides make it compute-bound (measured on
-// MI355X at 2048 x 1536, us per substep, one / two per launch: FP64 91.3 /
-// 81.5, FP32 66.1 / 53.9, FP64_EXACT 153.8 / 183.1).
+// Substeps per launch: as set, or automatically two in FP64 and FP32 on
+// grids large enough to be bandwidth-bound and one otherwise.  Measured on
+// MI355X (us per substep, one / two per launch): 2048 x 1536 FP64 91.3 /
+// 81.5, FP32 66.1 / 53.9, FP64_EXACT 153.8 / 183.1 (its IEEE divides make
+// it compute-bound); 512 x 384 (latency-bound, rows per wave 2) FP32
+// 8.7 / 9.5 per substep.
 int subs_per_launch(const inmsom_bt_ctx *c, int mode)
 {
     if (c->subs_per_launch > 0) return c->subs_per_launch;
-    return mode == INMSOM_BT_FP64_EXACT ? 1 : 2;
+    return mode != INMSOM_BT_FP64_EXACT && c->n >= TWO_SUBSTEP_CELLS ? 2 : 1;
+}
+
+// the kernel for (mode, two substeps, forming substep)
+template <bool FORM>
+static const void *fused_kernel(int mode, bool two)
+{
+    if (two) {
+        if (mode == INMSOM_BT_FP64_EXACT) return (const void *)&k_fused_step2<double, true, FORM>;
+        if (mode == INMSOM_BT_FP64) return (const void *)&k_fused_step2<double, false, FORM>;
+        return (const void *)&k_fused_step2<float, false, FORM>;
+    }
+    if (mode == INMSOM_BT_FP64_EXACT) return (const void *)&k_fused_step<double, true, FORM>;
+    if (mode == INMSOM_BT_FP64) return (const void *)&k_fused_step<double, false, FORM>;
+    return (const void *)&k_fused_step<float, false, FORM>;
 }
 
 // Waves of the fused kernel for (mode, two) that the device holds at once:
@@ -392,13 +408,12 @@ static long long wave_slots(inmsom_bt_ctx *c, int mode, bool two)
 {
     const int idx = (mode == INMSOM_BT_FP64_EXACT ? 0 : mode == INMSOM_BT_FP64 ? 1 : 2) + (two ? 3 : 0);
     if (c->slots[idx] > 0) return c->slots[idx];
-    const void *k[6] = {(const void *)&k_fused_step<double, true>, (const void *)&k_fused_step<double, false>,
-                        (const void *)&k_fused_step<float, false>, (const void *)&k_fused_step2<double, true>,
-                        (const void *)&k_fused_step2<double, false>, (const void *)&k_fused_step2<float, false>};
     int cus = 0, per_cu = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1)
         cus = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k[idx], WAVE, 0) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fused_kernel<true>(mode, two), WAVE, 0) !=
+            hipSuccess ||
+        per_cu < 1)
         per_cu = 4;
     c->slots[idx] = (long long)cus * per_cu;
     return c->slots[idx];
@@ -412,7 +427,7 @@ static FusedParams make_params(inmsom_bt_ctx *c, FusedAlloc *a, int b, double ta
     p.ny = c->ny;
     const int sw = two ? SW2 : SW;
     p.strips = (c->nx + 2 + sw - 1) / sw;  // columns 1..nx need strips covering m <= nx
-    p.chunk = c->chunk > 0 ? c->chunk : fit_chunk(c->ny, p.strips, wave_slots(c, mode, two), two ? 7 : 3, 4);
+    p.chunk = c->chunk > 0 ? c->chunk : fit_chunk(c->ny, p.strips, wave_slots(c, mode, two), two ? 7 : 3, 2);
     p.nchunks = (c->ny + p.chunk - 1) / p.chunk;
     p.total = p.strips * p.nchunks;
     p.per_xcd = (p.total + NXCD - 1) / NXCD;
@@ -457,21 +472,9 @@ static FusedParams make_params(inmsom_bt_ctx *c, FusedAlloc *a, int b, double ta
 static void launch(inmsom_bt_ctx *c, const FusedParams &p, int mode, bool two)
 {
     dim3 grid((unsigned)(p.per_xcd * NXCD)), blk(WAVE);
-    if (two) {
-        if (mode == INMSOM_BT_FP64_EXACT)
-            hipLaunchKernelGGL((k_fused_step2<double, true>), grid, blk, 0, c->stream, p);
-        else if (mode == INMSOM_BT_FP64)
-            hipLaunchKernelGGL((k_fused_step2<double, false>), grid, blk, 0, c->stream, p);
-        else
-            hipLaunchKernelGGL((k_fused_step2<float, false>), grid, blk, 0, c->stream, p);
-        return;
-    }
-    if (mode == INMSOM_BT_FP64_EXACT)
-        hipLaunchKernelGGL((k_fused_step<double, true>), grid, blk, 0, c->stream, p);
-    else if (mode == INMSOM_BT_FP64)
-        hipLaunchKernelGGL((k_fused_step<double, false>), grid, blk, 0, c->stream, p);
-    else
-        hipLaunchKernelGGL((k_fused_step<float, false>), grid, blk, 0, c->stream, p);
+    const void *k = p.sub == 1 ? fused_kernel<true>(mode, two) : fused_kernel<false>(mode, two);
+    void *args[] = {const_cast<FusedParams *>(&p)};
+    hipLaunchKernel(k, grid, blk, args, 0, c->stream);
 }
 
 int fused_run(inmsom_bt_ctx *c, double tau, int nstep, int nsteps, int mode, double *ms)

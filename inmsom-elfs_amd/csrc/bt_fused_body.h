@@ -11,6 +11,9 @@ constexpr int WAVE = 64;
 constexpr int SW = 60;   // output columns per wave, one substep per launch
 constexpr int SW2 = 58;  // output columns per wave, two substeps per launch
 constexpr int NXCD = 8;
+// cells (summed over an ensemble's members) from which the fused step runs
+// two substeps per launch by default
+constexpr long long TWO_SUBSTEP_CELLS = 1LL << 20;
 
 struct FusedParams {
     int nx, ny, strips, chunk, nchunks, total, per_xcd;
@@ -126,8 +129,8 @@ struct RowAddr {
 // minimises it, with the chunk that fits k rounds exactly.  Measured on
 // MI355X at 2048 x 1536 (1024 SIMDs): one item more than a round costs up to
 // 40 % (two substeps per launch in FP32: 28 rows, 1980 items in 2048 slots,
-// 54 us per substep; 27 rows, 2088 items, 77 us).  Small grids keep at
-// least `lo` rows per item.
+// 54 us per substep; 27 rows, 2088 items, 77 us).  Items keep at least `lo`
+// rows (512 x 384, one substep per launch: 2 rows 8.7 us, 1 row 10.5 us).
 inline int fit_chunk(int rows, int strips, long long slots, int warm, int lo)
 {
     int best = rows;
@@ -189,12 +192,13 @@ struct StageOut {
 // register window (suffix m1 = row n-1, 0 = row n, p1 = n+1, p2 = n+2) of
 // input rows and of the T- and H-point intermediates.  row(n) takes the
 // inputs of iteration n and, when asked, computes the outputs of row n.
-template <typename R, bool EXACT>
+// FORM: the call's first substep, which forms the wind + drag forcing;
+// the later substeps take it frozen (StageIn::fx0, fy0).
+template <typename R, bool EXACT, bool FORM>
 struct Stage {
+    static constexpr bool frozen = !FORM;
     const FusedParams &p;
     const Ops<R, EXACT> op;
-    const int sub;
-    const bool frozen;
     const bool inx, inx1;
 
     R u_m1 = 0, u_0 = 0, u_p1 = 0, v_m1 = 0, v_0 = 0, v_p1 = 0;
@@ -211,9 +215,8 @@ struct Stage {
     R sts_m1 = 0, sts_0 = 0, vo_m1 = 0, vo_0 = 0;
     R f_m1 = 0, f_0 = 0;
 
-    __device__ Stage(const FusedParams &pp, int sub_, int m)
-        : p(pp), op(pp), sub(sub_), frozen(sub_ > 1), inx(m >= 3 && m <= pp.nx - 2),
-          inx1(m >= 2 && m <= pp.nx - 1)
+    __device__ Stage(const FusedParams &pp, int m)
+        : p(pp), op(pp), inx(m >= 3 && m <= pp.nx - 2), inx1(m >= 2 && m <= pp.nx - 1)
     {
     }
 
@@ -288,7 +291,7 @@ struct Stage {
             bf = R(0);
             if constexpr (EXACT) {
                 st = d / d * (upr / d - up1l / d) - d / d * (vpr / d - vp0r / d);
-                if (sub == 1) {
+                if constexpr (FORM) {
                     R ut = (u_p1 * d * d * hu_p1 + u1l * d * d * hu1l) / 2.0 / d / d / hq_p1;
                     R vt = (v_p1 * d * d * hv_p1 + v_0 * d * d * hv_0) / 2.0 / d / d / hq_p1;
                     bf = R(p.cb) * sqrt(ut * ut + vt * vt + R(p.eb));
@@ -296,7 +299,7 @@ struct Stage {
             } else {
 #pragma clang fp contract(fast)
                 st = op.inv_d * ((upr - up1l) - (vpr - vp0r));
-                if (sub == 1) {
+                if constexpr (FORM) {
                     const R rh = R(0.5) / hq_p1;
                     R ut = (u_p1 * hu_p1 + u1l * hu1l) * rh;
                     R vt = (v_p1 * hv_p1 + v_0 * hv_0) * rh;
@@ -304,7 +307,7 @@ struct Stage {
                 }
             }
             stt_p1 = tw ? st : R(0);
-            bf_p1 = (tw && sub == 1) ? bf : R(0);
+            bf_p1 = (tw && FORM) ? bf : R(0);
         }
 
         // ---- H-point stage of row n: shear and metric vorticity
@@ -344,29 +347,25 @@ struct Stage {
             const R sx = sx_0, sy = sy_0;
             // the RHS of Control/dyncall.f90:114-126 is formed once per call
             // (first substep) and frozen over the call's substeps
-            R rhs_u, rhs_v;
-            if constexpr (EXACT) {
-                rhs_u = 0.0 + (sx + (-(bf_0 + bf0r) / 2.0 * up0)) * d * d;
-                rhs_v = 0.0 + (sy + (-(bf_0 + bf_p1) / 2.0 * vp0)) * d * d;
-            } else {
+            R rhs_u = q.fx0, rhs_v = q.fy0;
+            if constexpr (FORM) {
+                if constexpr (EXACT) {
+                    rhs_u = 0.0 + (sx + (-(bf_0 + bf0r) / 2.0 * up0)) * d * d;
+                    rhs_v = 0.0 + (sy + (-(bf_0 + bf_p1) / 2.0 * vp0)) * d * d;
+                } else {
 #pragma clang fp contract(fast)
-                rhs_u = sx - R(0.5) * (bf_0 + bf0r) * up0;
-                rhs_v = sy - R(0.5) * (bf_0 + bf_p1) * vp0;
-            }
-            if (frozen) {
-                rhs_u = q.fx0;
-                rhs_v = q.fy0;
+                    rhs_u = sx - R(0.5) * (bf_0 + bf0r) * up0;
+                    rhs_v = sy - R(0.5) * (bf_0 + bf_p1) * vp0;
+                }
             }
             o.rhs_u = rhs_u;
             o.rhs_v = rhs_v;
             // level n-1 U/V depths: hh_init values at the first substep,
             // the hh_shift-filtered carry afterwards (depth.f90:195-203)
             R hupc = hup_0, hvpc = hvp_0;
-            if constexpr (EXACT) {
-                if (frozen) {
-                    hupc = R(q.hupc0);
-                    hvpc = R(q.hvpc0);
-                }
+            if constexpr (EXACT && frozen) {
+                hupc = R(q.hupc0);
+                hvpc = R(q.hvpc0);
             }
             R un, vn;
             if constexpr (EXACT) {
@@ -518,19 +517,19 @@ struct Stage {
 // Loads of one iteration n of a strip (issued one iteration ahead of their
 // use: measured on MI355X at 2048 x 1536, two iterations ahead runs 5 %
 // slower in FP32 and halves FP64's occupancy).
-template <typename R, bool EXACT>
+template <typename R, bool EXACT, bool FROZEN>
 struct Loader {
     const FusedParams &p;
     const RowAddr ra;
-    const bool colok, frozen;
+    const bool colok;
     struct Pre {
         R s, H, u, v, rd, fx, fy;
         double up, vp, sp, hupc, hvpc, f, sx, sy, wf;
         int mb;
     } q;
 
-    __device__ Loader(const FusedParams &pp, int m, bool frozen_)
-        : p(pp), ra{pp.nx, pp.ny, min(max(m, 1), pp.nx)}, colok(m >= 1 && m <= pp.nx), frozen(frozen_)
+    __device__ Loader(const FusedParams &pp, int m)
+        : p(pp), ra{pp.nx, pp.ny, min(max(m, 1), pp.nx)}, colok(m >= 1 && m <= pp.nx)
     {
     }
     __device__ __forceinline__ bool rowok(int row) const { return colok && row >= 1 && row <= p.ny; }
@@ -554,14 +553,16 @@ struct Loader {
         q.vp = p.vp[i1];
         q.sp = p.sp[i1];
         q.rd = RD[i1];
-        // per-cell frozen forcing and carried depths of the output row n:
-        // element 0 (one cache line per wave) when they are not used
-        const size_t iff = frozen ? i0 : 0;
-        q.fx = FX[iff];
-        q.fy = FY[iff];
-        if constexpr (EXACT) {
-            q.hupc = p.hup_c[iff];
-            q.hvpc = p.hvp_c[iff];
+        // per-cell frozen forcing and carried depths of the output row n
+        if constexpr (FROZEN) {
+            q.fx = FX[i0];
+            q.fy = FY[i0];
+        } else {
+            q.fx = q.fy = R(0);
+        }
+        if constexpr (EXACT && FROZEN) {
+            q.hupc = p.hup_c[i0];
+            q.hvpc = p.hvp_c[i0];
         } else {
             q.hupc = q.hvpc = 0.0;
         }
@@ -609,9 +610,9 @@ __device__ __forceinline__ void store_state(const FusedParams &p, size_t i, cons
 }
 
 // One work item: `chunk` output rows of one 60-column strip of one basin,
-// substep p.sub.  Called by every lane of a wave with wave-uniform
-// arguments.
-template <typename R, bool EXACT>
+// substep p.sub (FORM: p.sub == 1).  Called by every lane of a wave with
+// wave-uniform arguments.
+template <typename R, bool EXACT, bool FORM>
 __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, int strip)
 {
     const int lane = threadIdx.x;
@@ -621,8 +622,8 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
     const int r0 = chunk_id * p.chunk + 1;
     const int r1 = min(r0 + p.chunk, p.ny + 1);
 
-    Stage<R, EXACT> st(p, p.sub, m);
-    Loader<R, EXACT> ld(p, m, st.frozen);
+    Stage<R, EXACT, FORM> st(p, m);
+    Loader<R, EXACT, !FORM> ld(p, m);
     StageIn<R> in;
     StageOut<R> o;
     ld.issue(r0 - 3);
@@ -633,7 +634,7 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
         if (n >= r0 && writer && n <= p.ny) {
             const size_t io = (size_t)(n - 1) * p.nx + (m - 1);
             // the forcing formed by the first substep, for the later ones
-            if (!st.frozen && p.nsub > 1) {
+            if (FORM && p.nsub > 1) {
                 st_out<false, R>(p.frx, io, o.rhs_u);
                 st_out<false, R>(p.fry, io, o.rhs_v);
             }
@@ -648,13 +649,16 @@ __device__ __forceinline__ void fused_chunk(const FusedParams &p, int chunk_id, 
     }
 }
 
-// Two substeps, p.sub and p.sub + 1, in one march: stage A reads the
-// arrays; stage B runs two rows behind A on A's outputs, which never leave
-// the registers.  A strip is 58 output columns (B's output at lane L
-// needs A's lanes L-1..L+2, which need the loads of L-2..L+4); A starts
-// two rows above the chunk so that B's window sees A's rows from r0-2 on.
-// The results are those of two launches of fused_chunk.
-template <typename R, bool EXACT>
+// Two substeps, p.sub and p.sub + 1, in one march (AFORM: p.sub == 1):
+// stage A reads the arrays; stage B runs two rows behind A on A's outputs
+// and on the static rows A loaded two iterations earlier (depth at rest,
+// mask bits, Rayleigh coefficient), which never leave the registers
+// (re-reading them from cache instead was measured 5 % slower).  A strip
+// is 58 output columns (B's output at lane L needs
+// A's lanes L-1..L+2, which need the loads of L-2..L+4); A starts two rows
+// above the chunk so that B's window sees A's rows from r0-2 on.  The
+// results are those of two launches of fused_chunk.
+template <typename R, bool EXACT, bool AFORM>
 __device__ __forceinline__ void fused_chunk2(const FusedParams &p, int chunk_id, int strip)
 {
     const int lane = threadIdx.x;
@@ -665,13 +669,14 @@ __device__ __forceinline__ void fused_chunk2(const FusedParams &p, int chunk_id,
     const int r1 = min(r0 + p.chunk, p.ny + 1);
     const int subB = p.sub + 1;
 
-    Stage<R, EXACT> A(p, p.sub, m), B(p, subB, m);
-    Loader<R, EXACT> ld(p, m, A.frozen);
+    Stage<R, EXACT, AFORM> A(p, m);
+    Stage<R, EXACT, false> B(p, m);
+    Loader<R, EXACT, !AFORM> ld(p, m);
     StageIn<R> ia, ib;
     StageOut<R> oa, ob;
     // delay lines from A's iteration to B's (B's iteration n - 2 runs in
-    // A's iteration n): the static rows A loaded, A's outputs of row n - 1
-    // (B's level-n input one row ahead) and A's forcing / carried depths
+    // A's iteration n): the static rows A loaded, A's forcing / carried
+    // depths, and A's outputs of row n - 1 (B's level-n input one row ahead)
     R H_d1 = 0, H_d2 = 0, rd_d1 = 0, rd_d2 = 0;
     int mb_d1 = 0, mb_d2 = 0;
     R fx_d1 = 0, fx_d2 = 0, fy_d1 = 0, fy_d2 = 0;
@@ -680,19 +685,17 @@ __device__ __forceinline__ void fused_chunk2(const FusedParams &p, int chunk_id,
     double up_prev = 0, vp_prev = 0, sp_prev = 0;
     ld.issue(r0 - 5);
     for (int n = r0 - 5; n < r1 + 2; ++n) {
+        const int nb = n - 2;
+        const bool runB = nb >= r0 - 3;
         ld.take(n, ia);
         if (n + 1 < r1 + 2) ld.issue(n + 1);
         const bool keepA = n >= r0 - 2;
         A.row(n, ia, keepA, oa);
         // A's row n as the 2-launch sequence would have stored it: zero
         // outside the array, as B's loads would read it
-        const bool okn = ld.rowok(n);
-        const R sA = okn && keepA ? oa.s : R(0);
-        const R fxA = keepA ? oa.rhs_u : R(0), fyA = keepA ? oa.rhs_v : R(0);
-        const double huA = keepA ? oa.hu : 0.0, hvA = keepA ? oa.hv : 0.0;
-        const int nb = n - 2;
-        if (nb >= r0 - 3) {
-            ib.s2 = sA;
+        const bool okn = ld.rowok(n) && keepA;
+        if (runB) {
+            ib.s2 = okn ? oa.s : R(0);
             ib.H2 = H_d2;
             ib.mb2 = mb_d2;
             ib.u1 = u_prev;
@@ -713,7 +716,7 @@ __device__ __forceinline__ void fused_chunk2(const FusedParams &p, int chunk_id,
             B.row(nb, ib, produce, ob);
             if (produce && writer && nb <= p.ny) {
                 const size_t io = (size_t)(nb - 1) * p.nx + (m - 1);
-                if (!A.frozen && p.nsub > subB) {
+                if (AFORM && p.nsub > subB) {
                     st_out<false, R>(p.frx, io, ob.rhs_u);
                     st_out<false, R>(p.fry, io, ob.rhs_v);
                 }
@@ -730,15 +733,17 @@ __device__ __forceinline__ void fused_chunk2(const FusedParams &p, int chunk_id,
         H_d2 = H_d1; H_d1 = ia.H2;
         mb_d2 = mb_d1; mb_d1 = ia.mb2;
         rd_d2 = rd_d1; rd_d1 = ia.rd1;
-        fx_d2 = fx_d1; fx_d1 = fxA;
-        fy_d2 = fy_d1; fy_d1 = fyA;
-        hu_d2 = hu_d1; hu_d1 = huA;
-        hv_d2 = hv_d1; hv_d1 = hvA;
-        u_prev = okn && keepA ? oa.u : R(0);
-        v_prev = okn && keepA ? oa.v : R(0);
-        up_prev = okn && keepA ? oa.up : 0.0;
-        vp_prev = okn && keepA ? oa.vp : 0.0;
-        sp_prev = okn && keepA ? oa.sp : 0.0;
+        fx_d2 = fx_d1; fx_d1 = keepA ? oa.rhs_u : R(0);
+        fy_d2 = fy_d1; fy_d1 = keepA ? oa.rhs_v : R(0);
+        if constexpr (EXACT) {
+            hu_d2 = hu_d1; hu_d1 = keepA ? oa.hu : 0.0;
+            hv_d2 = hv_d1; hv_d1 = keepA ? oa.hv : 0.0;
+        }
+        u_prev = okn ? oa.u : R(0);
+        v_prev = okn ? oa.v : R(0);
+        up_prev = okn ? oa.up : 0.0;
+        vp_prev = okn ? oa.vp : 0.0;
+        sp_prev = okn ? oa.sp : 0.0;
     }
 }
 

@@ -102,6 +102,12 @@ class Ensemble:
         check(lib().inmsom_bt_ens_set_substeps_per_launch(self._h, int(k)),
               "inmsom_bt_ens_set_substeps_per_launch")
 
+    def substeps_per_launch(self) -> int:
+        k = C.c_int()
+        check(lib().inmsom_bt_ens_get_substeps_per_launch(self._h, C.byref(k)),
+              "inmsom_bt_ens_get_substeps_per_launch")
+        return k.value
+
     def run(self, nsteps: int, nstep: int = 2, tau: Optional[float] = None, graph: bool = True):
         tau = self.basins[0].dt if tau is None else tau
         check(lib().inmsom_bt_ens_run(self._h, float(tau), nstep, nsteps, int(graph)), "inmsom_bt_ens_run")
