@@ -66,18 +66,13 @@ constexpr int lds_stride() { return CIN == 8 ? 8 : CIN + 8; }
 //              0..1 in FP32 (EPI_OUT_F32); w = W[co][kpad]
 //   backward:  NEG taps, w = W^T[ci][t*32 + co] (kpad 288), out = dx masked
 //              by act_prev > 0 in BF16 (EPI_MASK_BF16)
-//   FT = 1 / 2 (layer 0 of the closure only): the input tile is formed from
-//              the members' float / double (u, v, eta, depth) arrays and the
-//              normalisation instead of being read from `in`
-template <int CIN, bool NEG, int EPI, int FT = 0>
+template <int CIN, bool NEG, int EPI>
 __global__ void __launch_bounds__(256)
     k_conv(const Img *__restrict__ imgs, const int *__restrict__ tile_img,
            const int *__restrict__ tile_local, int ntiles, const bf16 *__restrict__ in,
            const bf16 *__restrict__ w, const float *__restrict__ bias,
-           const bf16 *__restrict__ act_prev, void *__restrict__ out,
-           const ClosureMember *__restrict__ mem = nullptr, Norm nm = Norm{})
+           const bf16 *__restrict__ act_prev, void *__restrict__ out)
 {
-    static_assert(FT == 0 || CIN == CIN0, "features feed the first layer only");
     constexpr int KPAD = NEG ? 9 * CH : (9 * CIN + 15) / 16 * 16;
     constexpr int KS = KPAD / 16;
     constexpr int LS = lds_stride<CIN>();
@@ -98,9 +93,7 @@ __global__ void __launch_bounds__(256)
     // tile are issued into registers before the current tile's matrix
     // products and committed to LDS after them.
     constexpr int NPT = (NCHUNK + 255) / 256;  // staged chunks per thread
-    using FT_T = typename std::conditional<FT == 2, double, float>::type;
     bf16x8 raw[NPT];
-    FT_T rawf[FT ? NPT : 1][4];
     unsigned valid = 0;
     auto fetch = [&](int t) {
         const Img im = imgs[tile_img[t]];
@@ -117,16 +110,7 @@ __global__ void __launch_bounds__(256)
             const int y = y0 - 1 + ry, x = x0 - 1 + rx;
             if (c < NCHUNK && y >= 0 && y < im.H && x >= 0 && x < im.W) {
                 valid |= 1u << j;
-                if constexpr (FT == 0) {
-                    raw[j] = *reinterpret_cast<const bf16x8 *>(in + ((im.pix + (long long)y * im.W + x) * CIN + part * 8));
-                } else {
-                    const ClosureMember &m = mem[tile_img[t]];
-                    const long long a = (long long)y * m.nx + x;
-                    rawf[j][0] = static_cast<const FT_T *>(m.u)[a];
-                    rawf[j][1] = static_cast<const FT_T *>(m.v)[a];
-                    rawf[j][2] = static_cast<const FT_T *>(m.s)[a];
-                    rawf[j][3] = static_cast<const FT_T *>(m.H)[a];
-                }
+                raw[j] = *reinterpret_cast<const bf16x8 *>(in + ((im.pix + (long long)y * im.W + x) * CIN + part * 8));
             }
         }
     };
@@ -137,14 +121,7 @@ __global__ void __launch_bounds__(256)
             if (c >= NCHUNK) continue;
             const int px = c / CPP, part = c - px * CPP;
             bf16x8 v = {};
-            if (valid & (1u << j)) {
-                if constexpr (FT == 0) {
-                    v = raw[j];
-                } else {
-#pragma unroll
-                    for (int f = 0; f < NIN; ++f) v[f] = (bf16)(((float)rawf[j][f] - nm.shift[f]) * nm.scale[f]);
-                }
-            }
+            if (valid & (1u << j)) v = raw[j];
             *reinterpret_cast<bf16x8 *>(tile_in + px * LS + part * 8) = v;
         }
     };
@@ -208,6 +185,135 @@ __global__ void __launch_bounds__(256)
                     }
                     *reinterpret_cast<bf16x4 *>(static_cast<bf16 *>(out) + p * CH + co0) = o;
                 }
+            }
+        }
+    }
+}
+
+// Layers 0 and 1 of the closure in one pass (inference).  For each tile of
+// layer 1 (TR x 32 outputs) the block stages the members' normalised
+// (u, v, eta, depth) on the (TR + 4) x 36 pixels the two layers reach,
+// computes layer 0's activations on layer 1's (TR + 2) x 34 input pixels
+// into LDS (zero outside the image, as layer 1's padding), then layer 1
+// from LDS.  Same products in the same order and the same rounding as two
+// separate launches, so the activations are bit-identical, without layer
+// 0's launch and its activation round trip through memory.  (Fusing layers
+// 2 and 3 the same way was measured slower: layer 2's 1.33x recompute on
+// the halo costs more than the launch it saves.)
+template <int FT>
+__global__ void __launch_bounds__(256)
+    k_conv01(const Img *__restrict__ imgs, const int *__restrict__ tile_img, const int *__restrict__ tile_local,
+             int ntiles, const bf16 *__restrict__ w0, const float *__restrict__ bias0, const bf16 *__restrict__ w1,
+             const float *__restrict__ bias1, bf16 *__restrict__ out, const ClosureMember *__restrict__ mem, Norm nm)
+{
+    using FT_T = typename std::conditional<FT == 2, double, float>::type;
+    constexpr int LSS = lds_stride<CIN0>();
+    constexpr int KS0 = (9 * CIN0 + 15) / 16, KS1 = 9 * CH / 16;
+    constexpr int LS = lds_stride<CH>();
+    constexpr int FH = TR + 4, FW = TW + 4;            // staged feature pixels
+    constexpr int AH = TR + 2, AW = TW + 2;            // layer-0 outputs = layer-1 inputs
+    constexpr int NA = AH * AW, NBLK0 = (NA + 31) / 32;
+    __shared__ __attribute__((aligned(16))) bf16 feat[FH * FW * LSS];
+    __shared__ __attribute__((aligned(16))) bf16 tile_in[AH * AW * LS];
+
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, i = lane & 31, half = lane >> 5;
+    bf16x8 b0[KS0], b1[KS1];
+#pragma unroll
+    for (int s = 0; s < KS0; ++s)
+        b0[s] = *reinterpret_cast<const bf16x8 *>(w0 + (size_t)i * (16 * KS0) + 16 * s + 8 * half);
+#pragma unroll
+    for (int s = 0; s < KS1; ++s)
+        b1[s] = *reinterpret_cast<const bf16x8 *>(w1 + (size_t)i * (16 * KS1) + 16 * s + 8 * half);
+    float bc0[16], bc1[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        bc0[r] = bias0[8 * (r >> 2) + 4 * half + (r & 3)];
+        bc1[r] = bias1[8 * (r >> 2) + 4 * half + (r & 3)];
+    }
+
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const Img im = imgs[tile_img[tile]];
+        const int tl = tile_local[tile];
+        const int tx = (im.W + TW - 1) / TW;
+        const int ty = tl / tx;
+        const int y0 = ty * TR, x0 = (tl - ty * tx) * TW;
+        __syncthreads();  // the previous tile's reads of both LDS tiles are done
+        // features of pixels (y0 - 2 + fy, x0 - 2 + fx)
+        const ClosureMember &m = mem[tile_img[tile]];
+        for (int q = threadIdx.x; q < FH * FW; q += 256) {
+            const int fy = q / FW, fx = q - fy * FW;
+            const int y = y0 - 2 + fy, x = x0 - 2 + fx;
+            bf16x8 v = {};
+            if (y >= 0 && y < im.H && x >= 0 && x < im.W) {
+                const long long a = (long long)y * m.nx + x;
+                const FT_T f[NIN] = {static_cast<const FT_T *>(m.u)[a], static_cast<const FT_T *>(m.v)[a],
+                                     static_cast<const FT_T *>(m.s)[a], static_cast<const FT_T *>(m.H)[a]};
+#pragma unroll
+                for (int c = 0; c < NIN; ++c) v[c] = (bf16)(((float)f[c] - nm.shift[c]) * nm.scale[c]);
+            }
+            *reinterpret_cast<bf16x8 *>(feat + q * LSS) = v;
+        }
+        __syncthreads();
+        // layer 0 on the AH x AW pixels (y0 - 1 + ry, x0 - 1 + rx), 32 per block of products
+        for (int j = wv; j < NBLK0; j += 4) {
+            const int p = 32 * j + i;
+            const int ry = p / AW, rx = p - ry * AW;
+            f32x16 acc = f32x16{};
+#pragma unroll
+            for (int s = 0; s < KS0; ++s) {
+                const int k0 = 16 * s + 8 * half;
+                const int t = k0 / CIN0;
+                bf16x8 a = {};
+                if (t < 9 && p < NA) {
+                    const int oy = t / 3 - 1, ox = t % 3 - 1;
+                    a = *reinterpret_cast<const bf16x8 *>(feat + ((ry + 1 + oy) * FW + (rx + 1 + ox)) * LSS);
+                }
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b0[s], a, acc, 0, 0, 0);
+            }
+            if (p < NA) {
+                const int y = y0 - 1 + ry, x = x0 - 1 + rx;
+                const bool in_img = y >= 0 && y < im.H && x >= 0 && x < im.W;
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    bf16x4 o;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        o[e] = in_img ? (bf16)fmaxf(acc[4 * g + e] + bc0[4 * g + e], 0.0f) : (bf16)0.0f;
+                    *reinterpret_cast<bf16x4 *>(tile_in + p * LS + 8 * g + 4 * half) = o;
+                }
+            }
+        }
+        __syncthreads();
+        // layer 1 from LDS, as k_conv
+        f32x16 acc[RW];
+#pragma unroll
+        for (int q = 0; q < RW; ++q) acc[q] = f32x16{};
+#pragma unroll
+        for (int s = 0; s < KS1; ++s) {
+            const int k0 = 16 * s + 8 * half;
+            const int t = k0 / CH, ci0 = k0 - t * CH;
+            const int oy = t / 3 - 1, ox = t % 3 - 1;
+#pragma unroll
+            for (int q = 0; q < RW; ++q) {
+                const int r = wv * RW + q;
+                const bf16x8 a =
+                    *reinterpret_cast<const bf16x8 *>(tile_in + ((r + 1 + oy) * AW + (i + 1 + ox)) * LS + ci0);
+                acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b1[s], a, acc[q], 0, 0, 0);
+            }
+        }
+        const int px = x0 + i;
+#pragma unroll
+        for (int q = 0; q < RW; ++q) {
+            const int y = y0 + wv * RW + q;
+            if (y >= im.H || px >= im.W) continue;
+            const long long pp = im.pix + (long long)y * im.W + px;
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int co0 = 8 * g + 4 * half;
+                bf16x4 o;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) o[e] = (bf16)fmaxf(acc[q][4 * g + e] + bc1[4 * g + e], 0.0f);
+                *reinterpret_cast<bf16x4 *>(out + pp * CH + co0) = o;
             }
         }
     }
@@ -412,7 +518,28 @@ __global__ void k_pack(int kpad, int cin, const float *__restrict__ wm, bf16 *__
 }
 
 
-// ---- closure inference over a set of basins (one block per basin row)
+// ---- closure inference over a set of basins (one block of 4 waves per basin row)
+
+// Sums of (a, b) over a block of 4 waves in a fixed order: an xor butterfly
+// in each wave, then the waves in order.  Every thread ends with the same
+// bits (IEEE addition commutes).
+__device__ inline void block_sum2(double *a, double *b)
+{
+    __shared__ double red[2][4];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        *a += __shfl_xor(*a, off);
+        *b += __shfl_xor(*b, off);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][w] = *a;
+        red[1][w] = *b;
+    }
+    __syncthreads();
+    *a = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
+    *b = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
+}
 
 template <class T>
 __device__ inline double ld(const void *p, long long i)
@@ -444,7 +571,6 @@ __global__ void __launch_bounds__(256)
               const int *__restrict__ row_y, Norm nm, const float *__restrict__ tend,
               double *__restrict__ part)
 {
-    __shared__ double red[2][256];
     const ClosureMember m = mem[row_member[blockIdx.x]];
     const int y = row_y[blockIdx.x];
     double sc = 0.0, sv = 0.0;
@@ -471,19 +597,10 @@ __global__ void __launch_bounds__(256)
             }
         }
     }
-    red[0][threadIdx.x] = sc;
-    red[1][threadIdx.x] = sv;
-    __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + st];
-            red[1][threadIdx.x] += red[1][threadIdx.x + st];
-        }
-        __syncthreads();
-    }
+    block_sum2(&sc, &sv);
     if (threadIdx.x == 0) {
-        part[2 * (m.row0 + y) + 0] = red[0][0];
-        part[2 * (m.row0 + y) + 1] = red[1][0];
+        part[2 * (m.row0 + y) + 0] = sc;
+        part[2 * (m.row0 + y) + 1] = sv;
     }
 }
 
@@ -498,8 +615,6 @@ __global__ void __launch_bounds__(256)
                const double *__restrict__ part, double ratio, double dt, double *__restrict__ norms,
                int *__restrict__ active, int *__restrict__ nfall)
 {
-    __shared__ double red[2][256];
-    __shared__ int ok_s;
     const int mi = row_member[blockIdx.x];
     const ClosureMember m = mem[mi];
     const int y = row_y[blockIdx.x];
@@ -508,28 +623,15 @@ __global__ void __launch_bounds__(256)
         sc += part[2 * (m.row0 + r) + 0];
         sv += part[2 * (m.row0 + r) + 1];
     }
-    red[0][threadIdx.x] = sc;
-    red[1][threadIdx.x] = sv;
-    __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + st];
-            red[1][threadIdx.x] += red[1][threadIdx.x + st];
-        }
-        __syncthreads();
+    block_sum2(&sc, &sv);
+    const double nc = sqrt(sc), nv = sqrt(sv);
+    const int ok_s = nc <= ratio * nv ? 1 : 0;
+    if (threadIdx.x == 0 && y == 0) {
+        norms[2 * mi + 0] = nc;
+        norms[2 * mi + 1] = nv;
+        active[mi] = ok_s;
+        nfall[mi] += 1 - ok_s;
     }
-    if (threadIdx.x == 0) {
-        const double nc = sqrt(red[0][0]), nv = sqrt(red[1][0]);
-        const int ok = nc <= ratio * nv ? 1 : 0;
-        ok_s = ok;
-        if (y == 0) {
-            norms[2 * mi + 0] = nc;
-            norms[2 * mi + 1] = nv;
-            active[mi] = ok;
-            nfall[mi] += 1 - ok;
-        }
-    }
-    __syncthreads();
     if (!ok_s) return;
     T *u = static_cast<T *>(m.u), *v = static_cast<T *>(m.v);
     for (int i = threadIdx.x; i < m.nx; i += blockDim.x) {
@@ -802,16 +904,15 @@ void closure_launch(const inmsom_bt_cnn *net, hipStream_t st, bool f32, ClosureM
     const dim3 rows((unsigned)inf.nrows), blk(256);
     const dim3 grid((unsigned)(inf.ntiles < 2048 ? inf.ntiles : 2048));
     const Layer &L0 = net->L[0];
-    // layer 0 forms its inputs from the members' state; a0 -> a1 -> a0 -> tend
+    // layers 0 and 1 from the members' state in one pass (-> a1), then a1 -> a0 -> tend
+    const Layer &L1 = net->L[1];
+    const bf16 *w0 = net->wf + net->wf_off[0], *w1 = net->wf + net->wf_off[1];
     if (f32)
-        hipLaunchKernelGGL((k_conv<CIN0, false, EPI_RELU_BF16, 1>), grid, blk, 0, st, inf.imgs, inf.tile_img,
-                           inf.tile_local, inf.ntiles, nullptr, net->wf + net->wf_off[0], net->wm + L0.boff,
-                           nullptr, inf.a0, members, net->norm);
+        hipLaunchKernelGGL((k_conv01<1>), grid, blk, 0, st, inf.imgs, inf.tile_img, inf.tile_local, inf.ntiles, w0,
+                           net->wm + L0.boff, w1, net->wm + L1.boff, inf.a1, members, net->norm);
     else
-        hipLaunchKernelGGL((k_conv<CIN0, false, EPI_RELU_BF16, 2>), grid, blk, 0, st, inf.imgs, inf.tile_img,
-                           inf.tile_local, inf.ntiles, nullptr, net->wf + net->wf_off[0], net->wm + L0.boff,
-                           nullptr, inf.a0, members, net->norm);
-    conv_launch(net, st, 1, false, inf.imgs, inf.tile_img, inf.tile_local, inf.ntiles, inf.a0, nullptr, inf.a1);
+        hipLaunchKernelGGL((k_conv01<2>), grid, blk, 0, st, inf.imgs, inf.tile_img, inf.tile_local, inf.ntiles, w0,
+                           net->wm + L0.boff, w1, net->wm + L1.boff, inf.a1, members, net->norm);
     conv_launch(net, st, 2, false, inf.imgs, inf.tile_img, inf.tile_local, inf.ntiles, inf.a1, nullptr, inf.a0);
     conv_launch(net, st, 3, false, inf.imgs, inf.tile_img, inf.tile_local, inf.ntiles, inf.a0, nullptr, inf.tend);
     if (f32) {
