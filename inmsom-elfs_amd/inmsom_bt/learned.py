@@ -208,7 +208,9 @@ def rollout(cb: Basin, start: Snapshot, nsteps: int, truth: Optional[Dict[int, D
     """Coarse run of ``nsteps`` outer steps from ``start`` (C(t), C(t - dt_c / nstep)).
     ``truth`` maps coarse step -> coarsened fine state; errors are scored at
     those steps.  Returns errors, the final state, fallback counts and the
-    run's wall time (ms) when ``timed``."""
+    run's wall time (ms) when ``timed`` (after the ensemble's autotuner has
+    picked the launch shape from the same state; results do not depend on
+    the shape)."""
     lcu, lcv = uv_masks(cb)
     wet = cb.wet
     res: dict = {"steps": [], "ssh": [], "u": [], "v": []}
@@ -217,6 +219,8 @@ def rollout(cb: Basin, start: Snapshot, nsteps: int, truth: Optional[Dict[int, D
                       start.prev["ssh"])
         if net is not None:
             ens.set_closure(net, ratio)
+        if timed:
+            ens.autotune(nstep=nstep, trial_steps=20, rows=(0, 2, 3, 4, 6, 8), waves=(0, 256, 384, 512))
         marks = sorted(truth) if truth else []
         marks = [k for k in marks if 0 < k <= nsteps]
         if not marks or marks[-1] != nsteps:
