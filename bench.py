@@ -57,9 +57,12 @@ def rel_l2(a, b):
 
 
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "fused_step_cfg1_traffic.json")
-# the kernel each leg launches (two substeps per launch in FP64 and FP32)
-TRAFFIC_KERNELS = {"fp64": "k_fused_step2<double, false>", "fp32": "k_fused_step2<float, false>",
-                   "fp64_exact": "k_fused_step<double, true>", "ensemble_fp32": "k_fused_batch2<float, false>"}
+# the kernels each leg launches (two substeps per launch in FP64 and FP32; the
+# last template argument marks the call's first, forcing-forming substep)
+TRAFFIC_KERNELS = {"fp64": ("k_fused_step2<double, false, true>",),
+                   "fp32": ("k_fused_step2<float, false, true>",),
+                   "fp64_exact": ("k_fused_step<double, true, true>", "k_fused_step<double, true, false>"),
+                   "ensemble_fp32": ("k_fused_batch2<float, false, true>",)}
 
 
 # kernels of one 2-D -> 3-D coupling and their launches per coupling
@@ -113,14 +116,17 @@ def write_traffic(fetch_path, write_path, out=TRAFFIC_JSON):
             res = json.load(f)
     except (OSError, ValueError):
         res = {}
-    for mode, key in TRAFFIC_KERNELS.items():
+    for mode, keys in TRAFFIC_KERNELS.items():
         try:
-            (f, nf), name = find(F, key)
-            (w, _), _ = find(W, key)
+            parts = [(find(F, k), find(W, k)) for k in keys]
         except KeyError:
             continue
-        res[mode] = dict(src, kernel=name, dispatches=nf, fetch_kib=f, write_kib=w,
-                         fetch_bytes_per_launch=f * ff * 1024, write_bytes_per_launch=w * wf * 1024)
+        # the mean launch over the mode's kernels, weighted by their dispatches
+        nf = sum(pf[0][1] for pf, _ in parts)
+        f = sum(pf[0][0] * pf[0][1] for pf, _ in parts) / nf
+        w = sum(pw[0][0] * pf[0][1] for pf, pw in parts) / nf
+        res[mode] = dict(src, kernel=" + ".join(pf[1] for pf, _ in parts), dispatches=nf, fetch_kib=f,
+                         write_kib=w, fetch_bytes_per_launch=f * ff * 1024, write_bytes_per_launch=w * wf * 1024)
     try:
         parts = {}
         for key, times in COUPLING_SEQUENCE.items():

@@ -127,6 +127,19 @@ interface
     type(c_ptr), value :: ctx
     integer(c_int), intent(out) :: rows
   end function
+  integer(c_int) function inmsom_bt_set_substeps_per_launch(ctx, substeps) &
+      bind(C, name="inmsom_bt_set_substeps_per_launch")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: substeps
+  end function
+  integer(c_int) function inmsom_bt_get_substeps_per_launch(ctx, mode, substeps) &
+      bind(C, name="inmsom_bt_get_substeps_per_launch")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: mode
+    integer(c_int), intent(out) :: substeps
+  end function
   integer(c_int) function inmsom_bt_range_check(ctx, bad) bind(C, name="inmsom_bt_range_check")
     import :: c_int, c_long_long, c_ptr
     type(c_ptr), value :: ctx
