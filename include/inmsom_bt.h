@@ -256,6 +256,32 @@ int inmsom_bt_internal_update_dev(inmsom_bt_ctx *ctx, double tau, int nz, const 
                                   const double *slpr, const double *wflux, double *uu2d, double *vv2d,
                                   double *pgrx, double *pgry, double *ssh_i, double *sshp_i,
                                   double *const hhn[4], double *uu, double *vv);
+/* The forcing of the barotropic step from the 3-D model, Control/
+ * dyncall.f90:37-130 with ksw_vert = ksw_lat = ksw_lat4 = 0 (host arrays
+ * in and out, like inmsom_bt_barotropic_dynamics): stress_components of
+ * the level n-1 velocities uup, vvp (3-D) and uup2d, vvp2d (2-D),
+ * sea_surface_fluxes from taux, tauy on T points, sea_bottom_fluxes from
+ * the bottom level k = nz of uu, vv, uup, vvp (drag parameters of
+ * inmsom_bt_set_forcing), uv_trans of (uu, vv) and (uu2d, vv2d) and
+ * uv_diff2 with amuv and amuv2d, on the 3-D model's depths hhq, hhu, hhv,
+ * hhh (Function/vel_ssh.f90:2-184); pgx, pgy (3-D, or both NULL) are what
+ * pressure_gradients adds to the 3-D RHS; depth_ave with the layer
+ * thicknesses dz[nz] (Function/depth.f90:223-254); and the assembly of
+ * :109-126 into rhsx2d, rhsy2d, which are read and written (cells the
+ * reference does not write keep their values).  3-D arrays are
+ * (nz, ny, nx), i.e. the reference's column-major (m, n, k). */
+int inmsom_bt_forcing_3d(inmsom_bt_ctx *ctx, int nz, const double *dz, const double *uu, const double *vv,
+                         const double *uup, const double *vvp, const double *amuv, const double *uu2d,
+                         const double *vv2d, const double *uup2d, const double *vvp2d, const double *amuv2d,
+                         const double *hhq, const double *hhu, const double *hhv, const double *hhh,
+                         const double *taux, const double *tauy, const double *slpr, const double *pgx,
+                         const double *pgy, double *rhsx2d, double *rhsy2d);
+/* Timing of its device work: nrep repetitions on the fields the last
+ * inmsom_bt_forcing_3d call with nz levels left on the device (no host
+ * copies); *ms = HIP-event time of all repetitions.  No counterpart in the
+ * reference (a measurement entry beside inmsom_bt_run_timed). */
+int inmsom_bt_forcing_3d_timed(inmsom_bt_ctx *ctx, int nz, int nrep, double *ms);
+
 /* Timing of the coupling's device work: nrep repetitions of the kernels of
  * inmsom_bt_remove_barotropic then inmsom_bt_internal_update on the fields
  * the last inmsom_bt_internal_update call left on the device (it must have

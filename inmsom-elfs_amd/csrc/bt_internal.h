@@ -90,5 +90,9 @@ struct inmsom_bt_ctx {
     double *cpl = nullptr;         // staging of the 2-D -> 3-D coupling (bt_coupling.hip)
     size_t cpl_n = 0;              // doubles in cpl
     double *cpl_new = nullptr;     // one plane: ssh_new of the device-pointer coupling
+    double *f3 = nullptr;          // staging of the 3-D forcing producer (bt_forcing3d.hip)
+    size_t f3_n = 0;               // doubles in f3
+    int f3_nz = 0;                 // levels of the staged fields (0 = none)
+    bool f3_pg = false;            // staged pressure-gradient terms
     long long *cnt = nullptr;
 };

@@ -272,6 +272,44 @@ class BarotropicModel:
                                                  "vv2d", "pgrx", "pgry", "ssh_i", "sshp_i")],
             hh, self._dev(uu, s3), self._dev(vv, s3)), "inmsom_bt_internal_update_dev")
 
+    FORCING_3D = ("uu", "vv", "uup", "vvp", "amuv")
+    FORCING_2D = ("uu2d", "vv2d", "uup2d", "vvp2d", "amuv2d", "hhq", "hhu", "hhv", "hhh", "taux", "tauy",
+                  "slpr")
+
+    def forcing_3d(self, fields: dict, dz, rhsx2d, rhsy2d, pgx=None, pgy=None) -> dict:
+        """Control/dyncall.f90:37-130 (ksw_vert = ksw_lat = ksw_lat4 = 0):
+        ``fields`` holds the 3-D (nz, ny, nx) uu, vv, uup, vvp, amuv and the
+        2-D uu2d, vv2d, uup2d, vvp2d, amuv2d, hhq, hhu, hhv, hhh, taux, tauy,
+        slpr; pgx, pgy are optional 3-D pressure-gradient terms.  Returns
+        the updated copies of rhsx2d, rhsy2d."""
+        a3 = {k: np.ascontiguousarray(fields[k], dtype=np.float64) for k in self.FORCING_3D}
+        nz = a3["uu"].shape[0]
+        for k, a in a3.items():
+            if a.shape != (nz, self.ny, self.nx):
+                raise ValueError(f"{k} must be (nz, ny, nx)")
+        a2 = {k: np.ascontiguousarray(fields[k], dtype=np.float64) for k in self.FORCING_2D}
+        dzc = np.ascontiguousarray(dz, dtype=np.float64)
+        if dzc.shape != (nz,):
+            raise ValueError("dz must have nz values")
+        pg = [np.ascontiguousarray(p, dtype=np.float64) if p is not None else None for p in (pgx, pgy)]
+        out = {"rhsx2d": np.array(rhsx2d, dtype=np.float64, order="C"),
+               "rhsy2d": np.array(rhsy2d, dtype=np.float64, order="C")}
+        check(lib().inmsom_bt_forcing_3d(
+            self._h, nz, _d(dzc), *[_d(a3[k]) for k in self.FORCING_3D],
+            *[_d(a2[k]) for k in ("uu2d", "vv2d", "uup2d", "vvp2d", "amuv2d", "hhq", "hhu", "hhv", "hhh", "taux",
+                                  "tauy", "slpr")],
+            *[(_d(p) if p is not None else None) for p in pg], _d(out["rhsx2d"]), _d(out["rhsy2d"])),
+            "inmsom_bt_forcing_3d")
+        return out
+
+    def forcing_3d_timed(self, nz: int, nrep: int) -> float:
+        """HIP-event time (ms) of nrep repetitions of the forcing's device
+        work on the fields the last forcing_3d call left on the device."""
+        ms = C.c_double()
+        check(lib().inmsom_bt_forcing_3d_timed(self._h, int(nz), int(nrep), C.byref(ms)),
+              "inmsom_bt_forcing_3d_timed")
+        return ms.value
+
     def coupling_timed(self, tau: float, nz: int, nrep: int) -> float:
         """HIP-event time (ms) of nrep repetitions of the coupling's device
         kernels on the fields the last internal_update left on the device."""

@@ -745,6 +745,88 @@ void bto_rhs_assembly(const bto_grid *g, const double *surf_x,
         }
 }
 
+/* Control/dyncall.f90:37-130 without the vertical mixing, Smagorinsky,
+ * isopycnal slopes and biharmonic branches (ksw_vert = ksw_lat = 0,
+ * ksw_lat4 = 0): stress_components on the level n-1 3-D and 2-D
+ * velocities, sea_surface_fluxes and sea_bottom_fluxes (the bottom level
+ * k = nz of the 3-D velocities), uv_trans and uv_diff2 at nz levels and on
+ * the 2-D velocities, the caller's 3-D pressure-gradient terms (pressure_
+ * gradients, NULL = none), depth_ave with dz and the assembly of
+ * :109-126.  3-D arrays are (nz, ny, nx); rhsx2d, rhsy2d are read and
+ * written (cells the reference does not write keep their values). */
+void bto_forcing_3d(const bto_grid *g, int nz, const double *dz, double cb_nl, double ebottom,
+                    const double *uu, const double *vv, const double *uup, const double *vvp,
+                    const double *amuv, const double *uu2d, const double *vv2d,
+                    const double *uup2d, const double *vvp2d, const double *amuv2d,
+                    const double *hhq, const double *hhu, const double *hhv, const double *hhh,
+                    const double *taux, const double *tauy, const double *slpr,
+                    const double *pgx, const double *pgy, double *rhsx2d, double *rhsy2d)
+{
+    const size_t sz = (size_t)g->nx * g->ny, s3 = sz * (size_t)nz;
+    double *st3 = calloc(s3, sizeof(double)), *ss3 = calloc(s3, sizeof(double));
+    double *vort3 = calloc(s3, sizeof(double));
+    double *tx3 = calloc(s3, sizeof(double)), *ty3 = calloc(s3, sizeof(double));
+    double *rx3 = calloc(s3, sizeof(double)), *ry3 = calloc(s3, sizeof(double));
+    double *w2[12];
+    for (int k = 0; k < 12; ++k) w2[k] = calloc(sz, sizeof(double));
+    double *st2 = w2[0], *ss2 = w2[1], *vort2 = w2[2], *txd = w2[3], *tyd = w2[4];
+    double *dfx = w2[5], *dfy = w2[6], *sfx = w2[7], *sfy = w2[8], *bfx = w2[9], *bfy = w2[10];
+    double *bf = w2[11];
+    double *tx2 = calloc(sz, sizeof(double)), *ty2 = calloc(sz, sizeof(double));
+
+    /* dyncall.f90:38-39 */
+    for (int k = 0; k < nz; ++k)
+        bto_stress_components(g, uup + k * sz, vvp + k * sz, st3 + k * sz, ss3 + k * sz);
+    bto_stress_components(g, uup2d, vvp2d, st2, ss2);
+    /* :53-59 */
+    bto_surface_stress(g, taux, tauy, sfx, sfy);
+    bto_bottom_stress(g, cb_nl, ebottom, uu + (size_t)(nz - 1) * sz, vv + (size_t)(nz - 1) * sz,
+                      uup + (size_t)(nz - 1) * sz, vvp + (size_t)(nz - 1) * sz, hhu, hhv, hhq, bf, bfx, bfy);
+    /* :65-81 (RHSx3d and RHSx2d_diff_disp start at zero, :65-66) */
+    for (int k = 0; k < nz; ++k) {
+        bto_uv_trans(g, uu + k * sz, vv + k * sz, vort3 + k * sz, hhq, hhu, hhv, hhh, tx3 + k * sz, ty3 + k * sz);
+        bto_uv_diff2(g, amuv + k * sz, st3 + k * sz, ss3 + k * sz, hhq, hhu, hhv, hhh, rx3 + k * sz,
+                     ry3 + k * sz);
+    }
+    bto_uv_trans(g, uu2d, vv2d, vort2, hhq, hhu, hhv, hhh, txd, tyd);
+    bto_uv_diff2(g, amuv2d, st2, ss2, hhq, hhu, hhv, hhh, dfx, dfy);
+    /* :92-98, the caller's pressure_gradients terms */
+    if (pgx && pgy)
+        for (size_t i = 0; i < s3; ++i) {
+            rx3[i] = rx3[i] + pgx[i];
+            ry3[i] = ry3[i] + pgy[i];
+        }
+    /* :100-104, depth_ave (Function/depth.f90:223-254) with kbcl = 0 */
+    const double *src[4] = {rx3, ry3, tx3, ty3};
+    double *dst[4] = {rhsx2d, rhsy2d, tx2, ty2};
+    const float *msk[4] = {g->llu, g->llv, g->llu, g->llv};
+    for (int q = 0; q < 4; ++q)
+        for (int n = 2; n <= g->ny - 1; ++n)
+            for (int m = 2; m <= g->nx - 1; ++m) {
+                const size_t i = IX(m, n);
+                if (msk[q][i] > 0.5f) {
+                    double f = 0.0;
+                    for (int k = 0; k < nz; ++k) f = f + src[q][k * sz + i] * dz[k];
+                    dst[q][i] = f;
+                }
+            }
+    /* :109-126 */
+    const double rho0 = g->ref_den;
+    for (int n = NYS; n <= NYE; ++n)
+        for (int m = NXS; m <= NXE; ++m) {
+            const size_t i = IX(m, n);
+            if (g->lcu[i] > 0.5f)
+                rhsx2d[i] = rhsx2d[i] - dfx[i] + tx2[i] - txd[i] + (sfx[i] + bfx[i]) * g->dxt[i] * g->dyh[i] -
+                            (slpr[IX(m + 1, n)] - slpr[i]) * hhu[i] * g->dyh[i] / rho0;
+            if (g->lcv[i] > 0.5f)
+                rhsy2d[i] = rhsy2d[i] - dfy[i] + ty2[i] - tyd[i] + (sfy[i] + bfy[i]) * g->dyt[i] * g->dxh[i] -
+                            (slpr[IX(m, n + 1)] - slpr[i]) * hhv[i] * g->dxh[i] / rho0;
+        }
+    free(st3); free(ss3); free(vort3); free(tx3); free(ty3); free(rx3); free(ry3);
+    for (int k = 0; k < 12; ++k) free(w2[k]);
+    free(tx2); free(ty2);
+}
+
 void bto_model_step(const bto_grid *g, bto_depths *d, double tau, int nstep,
                     int ksw4, double cb_nl, double ebottom, double *ubrtr_e,
                     double *ubrtrp_e, double *vbrtr_e, double *vbrtrp_e,

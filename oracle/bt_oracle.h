@@ -132,6 +132,20 @@ void bto_model_step(const bto_grid *g, bto_depths *d, double tau, int nstep,
                     const double *wflux, const double *mu, const double *mu4,
                     const double *rdis, double *work);
 
+/* Control/dyncall.f90:37-130 (ksw_vert = ksw_lat = ksw_lat4 = 0): the 2-D
+ * forcing RHSx2d, RHSy2d of the barotropic step from the 3-D and 2-D
+ * velocities (stress components, surface and bottom stresses, uv_trans
+ * and uv_diff2 at nz levels and in 2-D, the caller's 3-D pressure-gradient
+ * terms or NULL, depth_ave with dz, assembly of :109-126).  3-D arrays are
+ * (nz, ny, nx); rhsx2d, rhsy2d are read and written. */
+void bto_forcing_3d(const bto_grid *g, int nz, const double *dz, double cb_nl, double ebottom,
+                    const double *uu, const double *vv, const double *uup, const double *vvp,
+                    const double *amuv, const double *uu2d, const double *vv2d,
+                    const double *uup2d, const double *vvp2d, const double *amuv2d,
+                    const double *hhq, const double *hhu, const double *hhv, const double *hhh,
+                    const double *taux, const double *tauy, const double *slpr,
+                    const double *pgx, const double *pgy, double *rhsx2d, double *rhsy2d);
+
 /* Control/dyncall.f90:169-184: the barotropic part removed from the 3-D
  * velocity, uu(:,:,k) -= uu2d on lcu and vv(:,:,k) -= vv2d on lcv for
  * k = 1..nz over m = 2..nx-1, n = 2..ny-1; uu, vv are (nz, ny, nx). */

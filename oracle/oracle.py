@@ -249,6 +249,30 @@ def coupling(og: "OracleGrid", tau: float, inp: Dict[str, np.ndarray]) -> Dict[s
     return out
 
 
+# forcing of the barotropic step (Control/dyncall.f90:37-130)
+FORCING_IN_3D = ("uu", "vv", "uup", "vvp", "amuv", "pgx", "pgy")
+FORCING_IN_2D = ("uu2d", "vv2d", "uup2d", "vvp2d", "amuv2d", "hhq", "hhu", "hhv", "hhh", "slpr", "rhsx2d",
+                 "rhsy2d")
+
+
+def forcing_3d(og: "OracleGrid", inp: Dict[str, np.ndarray], cb_nl: float, ebottom: float,
+               pressure: bool = True) -> Dict[str, np.ndarray]:
+    """bto_forcing_3d on copies of ``inp`` (dz, FORCING_IN_3D as (nz, ny, nx),
+    FORCING_IN_2D, and the basin's taux/tauy); returns rhsx2d, rhsy2d."""
+    L = lib()
+    b = og.b
+    w = {k: np.ascontiguousarray(inp[k], dtype=np.float64).copy() for k in inp}
+    nz = w["uu"].shape[0]
+    taux = np.ascontiguousarray(b.taux, dtype=np.float64)
+    tauy = np.ascontiguousarray(b.tauy, dtype=np.float64)
+    pg = (_d(w["pgx"]), _d(w["pgy"])) if pressure else (None, None)
+    L.bto_forcing_3d(C.byref(og.g), nz, _d(w["dz"]), C.c_double(cb_nl), C.c_double(ebottom),
+                     *[_d(w[k]) for k in ("uu", "vv", "uup", "vvp", "amuv", "uu2d", "vv2d", "uup2d", "vvp2d",
+                                          "amuv2d", "hhq", "hhu", "hhv", "hhh")],
+                     _d(taux), _d(tauy), _d(w["slpr"]), *pg, _d(w["rhsx2d"]), _d(w["rhsy2d"]))
+    return {"rhsx2d": w["rhsx2d"], "rhsy2d": w["rhsy2d"]}
+
+
 def have_reference() -> bool:
     return os.path.exists(REF_HARNESS)
 
@@ -261,6 +285,22 @@ def run_reference(case_path: str, out_path: str, nx: int, ny: int, mode: int,
                    stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, cwd=os.path.dirname(out_path))
     raw = np.fromfile(out_path, dtype="<f8")
     per = nx * ny
+    if mode == 3:
+        # masks, dz (nz values), 7 3-D fields, 12 planes of inputs, 2 of outputs
+        nz = (raw.size - (7 + len(FORCING_IN_2D) + 2) * per) // (1 + len(FORCING_IN_3D) * per)
+        res = {"masks": {k: raw[i * per:(i + 1) * per].reshape(ny, nx) for i, k in enumerate(MASKS)}}
+        o = 7 * per
+        inp = {"dz": raw[o:o + nz].copy()}
+        o += nz
+        for k in FORCING_IN_3D:
+            inp[k] = raw[o:o + nz * per].reshape(nz, ny, nx)
+            o += nz * per
+        for k in FORCING_IN_2D:
+            inp[k] = raw[o:o + per].reshape(ny, nx)
+            o += per
+        res["forcing_in"] = inp
+        res["forcing_out"] = {"rhsx2d": raw[o:o + per].reshape(ny, nx), "rhsy2d": raw[o + per:o + 2 * per].reshape(ny, nx)}
+        return res
     arrs = [raw[i * per:(i + 1) * per].reshape(ny, nx) for i in range(raw.size // per)]
     res = {"masks": dict(zip(MASKS, arrs[:7]))}
     arrs = arrs[7:]

@@ -32,6 +32,15 @@
 !   ocean_model_step), ssh_internal called as the reference has it -- and
 !   uu, vv, uu2d, vv2d, pgrx, pgry, ssh_i, sshp_i, hhqn, hhun, hhvn, hhhn
 !   are written.
+! mode 3: the forcing of the barotropic step, Control/dyncall.f90:37-130
+!   with ksw_vert = ksw_lat = ksw_lat4 = 0, on synthetic 3-D fields at the
+!   compiled nz: the inputs are written (dz, uu, vv, uup, vvp, amuv, the
+!   2-D fields, the 3-D model depths, slpr, 3-D pressure-gradient terms
+!   that stand for what pressure_gradients adds to RHSx3d, and the
+!   initial RHSx2d, RHSy2d), the reference's stress_components,
+!   sea_surface_fluxes, sea_bottom_fluxes, uv_trans, uv_diff2 and depth_ave
+!   run in dyncall's order, the assembly loop of :109-126 is restated
+!   (it sits inside ocean_model_step), and RHSx2d, RHSy2d are written.
 program ref_harness
 use main_basin_pars
 use mpi_parallel_tools
@@ -48,6 +57,7 @@ real(8), allocatable :: wflux(:,:), mu(:,:), mu4(:,:), rdis(:,:)
 real(8), allocatable :: ssh4gx(:,:), ssh4gy(:,:), vort(:,:), strt(:,:), strs(:,:)
 real(8), allocatable :: fxa(:,:), fya(:,:), radv(:,:), rady(:,:), rdfx(:,:), rdfy(:,:)
 real(8), allocatable :: ax(:,:), ay(:,:)
+real(8), allocatable :: pgx3(:,:,:), pgy3(:,:,:)
 
 call get_command_argument(1, case_path)
 call get_command_argument(2, out_path)
@@ -213,6 +223,67 @@ else if (mode == 2) then
   enddo
   write(23) uu, vv
   write(23) uu2d, vv2d, pgrx, pgry, ssh_i, sshp_i, hhqn, hhun, hhvn, hhhn
+else if (mode == 3) then
+  ksw_ssbc = 1
+  call hh_init(hhq_e, hhqp_e, hhqn_e, hhu_e, hhup_e, hhun_e, hhv_e, hhvp_e, hhvn_e, &
+               hhh_e, hhhp_e, hhhn_e, ssh_e, sshp_e, hhq_rest)
+  hhq = hhq_e; hhu = hhu_e; hhv = hhv_e; hhh = hhh_e
+  allocate(pgx3(nx,ny,nz), pgy3(nx,ny,nz))
+  do k = 1, nz
+    dz(k) = (0.5d0 + k)/(0.5d0*nz + 0.5d0*nz*(nz + 1))
+    uu(:,:,k)  = (ubrtr_e*(1.0d0 + 0.05d0*k) + 0.01d0*k)*lcu
+    vv(:,:,k)  = (vbrtr_e*(1.0d0 - 0.03d0*k) - 0.02d0*k)*lcv
+    uup(:,:,k) = (ubrtrp_e*(1.0d0 + 0.04d0*k) + 0.005d0*k)*lcu
+    vvp(:,:,k) = (vbrtrp_e*(1.0d0 - 0.02d0*k) - 0.01d0*k)*lcv
+    amuv(:,:,k) = mu*(1.0d0 + 0.1d0*k)
+    pgx3(:,:,k) = 1.0d-2*(sshp_e - ssh_e)*k
+    pgy3(:,:,k) = -2.0d-2*(ssh_e + 0.1d0)*k
+  enddo
+  uu2d = ubrtr_e; vv2d = vbrtr_e; uup2d = ubrtrp_e; vvp2d = vbrtrp_e
+  amuv2d = mu
+  slpr = 1.0d4*sshp_e + 50.0d0*taux
+  RHSx2d = 7.0d0; RHSy2d = -3.0d0
+  write(23) dz
+  write(23) uu, vv, uup, vvp, amuv, pgx3, pgy3
+  write(23) uu2d, vv2d, uup2d, vvp2d, amuv2d, hhq, hhu, hhv, hhh, slpr, RHSx2d, RHSy2d
+  ! Control/dyncall.f90:38-39
+  call stress_components(uup, vvp, stress_t, stress_s, nz)
+  call stress_components(uup2d, vvp2d, stress_t2d, stress_s2d, 1)
+  ! :53-59
+  call sea_surface_fluxes
+  call sea_bottom_fluxes
+  ! :65-81
+  RHSx3d = 0.0d0; RHSy3d = 0.0d0
+  RHSx3d_tran = 0.0d0; RHSy3d_tran = 0.0d0
+  RHSx2d_diff_disp = 0.0d0; RHSy2d_diff_disp = 0.0d0
+  call uv_trans(uu, vv, r_vort, hhq, hhu, hhv, hhh, RHSx3d_tran, RHSy3d_tran, nz)
+  call uv_trans(uu2d, vv2d, r_vort2d, hhq, hhu, hhv, hhh, RHSx2d_tran_disp, RHSy2d_tran_disp, 1)
+  call uv_diff2(amuv, stress_t, stress_s, hhq, hhu, hhv, hhh, RHSx3d, RHSy3d, nz)
+  call uv_diff2(amuv2d, stress_t2d, stress_s2d, hhq, hhu, hhv, hhh, RHSx2d_diff_disp, RHSy2d_diff_disp, 1)
+  ! :92-98, what pressure_gradients adds
+  RHSx3d = RHSx3d + pgx3
+  RHSy3d = RHSy3d + pgy3
+  ! :100-104
+  call depth_ave(RHSx3d, RHSx2d, llu, 0)
+  call depth_ave(RHSy3d, RHSy2d, llv, 0)
+  call depth_ave(RHSx3d_tran, RHSx2d_tran, llu, 0)
+  call depth_ave(RHSy3d_tran, RHSy2d_tran, llv, 0)
+  ! :107-126
+  do n = ny_start, ny_end
+    do m = nx_start, nx_end
+      if (lcu(m,n) > 0.5) then
+        RHSx2d(m,n) = RHSx2d(m,n) - RHSx2d_diff_disp(m,n) + RHSx2d_tran(m,n) - RHSx2d_tran_disp(m,n) &
+                    + (surf_stress_x(m,n) + bot_stress_x(m,n))*dxt(m,n)*dyh(m,n)                   &
+                    - (slpr(m+1,n) - slpr(m,n))*hhu(m,n)*dyh(m,n)/RefDen
+      endif
+      if (lcv(m,n) > 0.5) then
+        RHSy2d(m,n) = RHSy2d(m,n) - RHSy2d_diff_disp(m,n) + RHSy2d_tran(m,n) - RHSy2d_tran_disp(m,n) &
+                    + (surf_stress_y(m,n) + bot_stress_y(m,n))*dyt(m,n)*dxh(m,n)                   &
+                    - (slpr(m,n+1) - slpr(m,n))*hhv(m,n)*dxh(m,n)/RefDen
+      endif
+    enddo
+  enddo
+  write(23) RHSx2d, RHSy2d
 else
   ! helper routines, one call each, on the case's state arrays
   ! hh_init from (ssh_e, sshp_e)

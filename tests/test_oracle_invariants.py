@@ -307,3 +307,37 @@ def test_coupling_velocity_and_pressure_closed_form():
                                                           inp["uu"][k]), rtol=1e-14, atol=1e-14)
         np.testing.assert_allclose(out["vv"][k], np.where(lcv, inp["vv"][k] - inp["vv2d"] + out["vv2d"],
                                                           inp["vv"][k]), rtol=1e-14, atol=1e-14)
+
+
+def test_forcing_3d_closed_form_at_rest():
+    """With every velocity zero, RHSx2d on open interior U faces is the depth
+    mean of the pressure-gradient terms plus the wind stress times the face
+    area minus the sea-level-pressure gradient (Control/dyncall.f90:
+    92-126): the advective, viscous and bottom terms vanish."""
+    b = make_basin(28, 22, seed=6, sigma=3.0, metrics="stretched")
+    og = oracle.OracleGrid(b)
+    nz = 4
+    sh = (b.ny, b.nx)
+    rng = np.random.default_rng(2)
+    z3, z2 = np.zeros((nz,) + sh), np.zeros(sh)
+    a = np.array([1.0, -2.0, 0.5, 3.0]) * 1e-3
+    dz = np.array([0.1, 0.2, 0.3, 0.4])
+    inp = {"uu": z3, "vv": z3, "uup": z3, "vvp": z3, "amuv": 10.0 + z3,
+           "pgx": a[:, None, None] + z3, "pgy": -a[:, None, None] + z3,
+           "uu2d": z2, "vv2d": z2, "uup2d": z2, "vvp2d": z2, "amuv2d": 10.0 + z2,
+           "hhq": 40.0 + np.abs(rng.standard_normal(sh)), "hhu": 40.0 + np.abs(rng.standard_normal(sh)),
+           "hhv": 40.0 + np.abs(rng.standard_normal(sh)), "hhh": 40.0 + z2,
+           "slpr": 1.0e5 + 3.0 * np.arange(b.nx)[None, :] + 0.0 * z2, "rhsx2d": z2 + 5.0, "rhsy2d": z2 + 5.0, "dz": dz}
+    out = oracle.forcing_3d(og, inp, b.cb_nl, b.ebottom)
+    inner = np.zeros(sh, dtype=bool)
+    inner[2:-2, 2:-2] = True
+    lcu = (og.masks["lcu"] > 0.5) & inner
+    taux = b.taux
+    wind = (taux * b.dx * b.dy + np.roll(taux * b.dx * b.dy, -1, axis=1)) / b.rho0 / 2.0
+    want = (a * dz).sum() + wind - 3.0 * inp["hhu"] * b.dyh / b.rho0
+    np.testing.assert_allclose(out["rhsx2d"][lcu], want[lcu], rtol=1e-12, atol=1e-14)
+    # on llu cells that are not open interior faces only the depth mean is written
+    llu_only = (og.masks["llu"] > 0.5) & ~lcu
+    llu_only[[0, -1], :] = False
+    llu_only[:, [0, -1]] = False
+    assert np.allclose(out["rhsx2d"][llu_only], (a * dz).sum(), rtol=1e-13)

@@ -49,3 +49,25 @@ def test_coupling_bitwise(seed, metrics):
     assert inp["uu"].shape[0] == 20 and np.abs(inp["slpr"]).max() > 0
     got = oracle.coupling(oracle.OracleGrid(b), b.dt, inp)
     assert_bitwise(got, want, ("uu", "vv") + oracle.COUPLING_OUT, f"coupling seed {seed}")
+
+
+@pytest.mark.parametrize("seed,metrics", [(7, "uniform"), (8, "stretched")])
+def test_forcing_3d_bitwise(seed, metrics):
+    """The forcing of the barotropic step (Control/dyncall.f90:37-130: the
+    reference's stress_components, surface and bottom fluxes, uv_trans,
+    uv_diff2 at nz levels and depth_ave, then the assembly) against the
+    oracle on the inputs the harness wrote."""
+    b = make_basin(285, 159, seed=seed, sigma=4.0, g=9.8, hmax=120.0, metrics=metrics)
+    b = randomize_state(b, seed + 1, amp_uv=0.05)
+    rd = oracle.OracleGrid(b).rayleigh()
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "case.bin")
+        write_case(path, b, rd, 1, 2, 0, 3)
+        ref = oracle.run_reference(path, path + ".out", b.nx, b.ny, 3)
+    inp, want = ref["forcing_in"], ref["forcing_out"]
+    assert inp["uu"].shape[0] == 20 and abs(inp["dz"].sum() - 1.0) < 1e-12
+    got = oracle.forcing_3d(oracle.OracleGrid(b), inp, b.cb_nl, b.ebottom)
+    assert_bitwise(got, want, ("rhsx2d", "rhsy2d"), f"forcing 3-D seed {seed} {metrics}")
+    # cells the reference leaves alone keep the caller's values
+    untouched = (want["rhsx2d"] == 7.0)
+    assert untouched.any() and (got["rhsx2d"][untouched] == 7.0).all()
