@@ -332,6 +332,85 @@ def coupling_cpu(b, inp):
                       "(bto_remove_barotropic + bto_internal_update, one thread)"}
 
 
+# Algorithmic bytes of the forcing per array cell: each 3-D input (uu, vv,
+# uup, vvp, amuv, pgx, pgy) read once per level (56 B x nz); the 2-D inputs
+# read once (uu2d, vv2d, uup2d, vvp2d, amuv2d, hhq, hhu, hhv, hhh, taux,
+# tauy, slpr: 96 B; eight metrics: 64 B; masks lu, luu, lcu, lcv, llu, llv
+# as float: 24 B; dz is negligible); RHSx2d and RHSy2d read and written
+# (32 B).
+FORCING_3D_BYTES = 56
+FORCING_2D_BYTES = 96 + 64 + 24 + 32
+
+
+def forcing_fields(b, nz, seed=11):
+    """Synthetic level-(n, n-1) 3-D and 2-D fields of the forcing on basin b
+    (velocities zero on closed faces, positive viscosities and depths)."""
+    rng = np.random.default_rng(seed)
+    sh = (b.ny, b.nx)
+    masks = gridcon(b.lbasins)
+    lcu, lcv = (masks["lcu"] > 0.5).astype(np.float64), (masks["lcv"] > 0.5).astype(np.float64)
+    f = {}
+    for k, msk in (("uu", lcu), ("vv", lcv), ("uup", lcu), ("vvp", lcv)):
+        f[k] = 0.1 * rng.standard_normal((nz,) + sh) * msk
+    f["amuv"] = 10.0 + np.abs(rng.standard_normal((nz,) + sh))
+    f["pgx"] = 1e-3 * rng.standard_normal((nz,) + sh)
+    f["pgy"] = 1e-3 * rng.standard_normal((nz,) + sh)
+    for k, msk in (("uu2d", lcu), ("vv2d", lcv), ("uup2d", lcu), ("vvp2d", lcv)):
+        f[k] = 0.1 * rng.standard_normal(sh) * msk
+    f["amuv2d"] = 10.0 + np.abs(rng.standard_normal(sh))
+    for k in ("hhq", "hhu", "hhv", "hhh"):
+        f[k] = 50.0 + np.abs(rng.standard_normal(sh))
+    f["slpr"] = 1.0e5 + 100.0 * rng.standard_normal(sh)
+    f["taux"], f["tauy"] = b.taux, b.tauy
+    f["rhsx2d"], f["rhsy2d"] = np.zeros(sh), np.zeros(sh)
+    w = np.arange(1, nz + 1) + 0.5
+    f["dz"] = w / w.sum()
+    return f
+
+
+def forcing_leg(m, b, nz, nrep):
+    """The forcing of the barotropic step from the 3-D model (Control/
+    dyncall.f90:37-130) on the same basin with nz levels: the device work
+    timed with HIP events on the library's stream, and the whole host-array
+    call (PCIe included)."""
+    f = forcing_fields(b, nz)
+    pg = (f["pgx"], f["pgy"])
+    m.forcing_3d(f, f["dz"], f["rhsx2d"], f["rhsy2d"], *pg)          # first call allocates the staging
+    m.synchronize()
+    t0 = time.perf_counter()
+    m.forcing_3d(f, f["dz"], f["rhsx2d"], f["rhsy2d"], *pg)
+    host_s = time.perf_counter() - t0
+    m.forcing_3d_timed(nz, 2)                                         # warm-up
+    ms = m.forcing_3d_timed(nz, nrep)
+    avg_s = ms * 1e-3 / nrep
+    ncell = b.nx * b.ny
+    alg = (FORCING_3D_BYTES * nz + FORCING_2D_BYTES) * ncell
+    achieved = alg / avg_s / 1e9
+    res = {"levels": nz, "array_cells": ncell,
+           "value": b.n_wet * nz / avg_s, "unit": "wet-cell levels of forcing/s (device-resident)",
+           "device_ms": avg_s * 1e3, "host_call_ms": host_s * 1e3, "host_call_rate": b.n_wet * nz / host_s,
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_from_profiles("forcing"),
+                        "bytes_per_call": alg}}
+    log(f"forcing nz={nz}: device {avg_s * 1e3:.3f} ms ({achieved:.0f} GB/s), host-array call "
+        f"{host_s * 1e3:.1f} ms")
+    return res, f
+
+
+def forcing_cpu(b, f):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    og = oracle.OracleGrid(b)
+    t0 = time.perf_counter()
+    oracle.forcing_3d(og, f, b.cb_nl, b.ebottom)
+    dt = time.perf_counter() - t0
+    nz = f["uu"].shape[0]
+    return {"value": b.n_wet * nz / dt, "unit": "wet-cell levels of forcing/s", "cores": 1, "kind": "port",
+            "sample": f"one forcing of the same basin with {nz} levels, {dt:.2f} s, oracle/bt_oracle.c "
+                      "(bto_forcing_3d, one thread)"}
+
+
 CLOSURE_PARAMS = os.path.join(ROOT, "inmsom-elfs_amd", "inmsom_bt", "data", "closure_cfg1.npz")
 
 
@@ -446,6 +525,9 @@ def main():
     ap.add_argument("--coupling-levels", type=int, default=COUPLING_NZ,
                     help="levels of the 2-D -> 3-D coupling leg (0: skip it)")
     ap.add_argument("--coupling-reps", type=int, default=50)
+    ap.add_argument("--forcing-levels", type=int, default=COUPLING_NZ,
+                    help="levels of the forcing leg (3-D model -> barotropic RHS; 0: skip it)")
+    ap.add_argument("--forcing-reps", type=int, default=20)
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH", "WRITE"),
                     help="write the per-launch HBM traffic of the fused kernels from two rocprofv3 "
                          "--pmc outputs (counter_collection.csv or results.db; FETCH_SIZE, WRITE_SIZE) and exit")
@@ -472,6 +554,9 @@ def main():
         coupling = None
         if args.coupling_levels > 0:
             coupling, coupling_inp = coupling_leg(m, b, args.coupling_levels, args.coupling_reps)
+        forcing = None
+        if args.forcing_levels > 0:
+            forcing, forcing_inp = forcing_leg(m, b, args.forcing_levels, args.forcing_reps)
 
     errs = {}
     if "fp64" in states:
@@ -516,6 +601,11 @@ def main():
             coupling["cpu_baseline"] = coupling_cpu(b, coupling_inp)
         del coupling_inp
         line["coupling"] = coupling
+    if forcing is not None:
+        if not args.no_cpu_baseline:
+            forcing["cpu_baseline"] = forcing_cpu(b, forcing_inp)
+        del forcing_inp
+        line["forcing"] = forcing
     if args.ensemble_steps > 0:
         line["ensemble_fp32"] = ensemble_leg(args.ensemble_steps, min(args.warmup, 20),
                                              args.ensemble_members)

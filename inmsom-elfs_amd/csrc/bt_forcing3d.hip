@@ -1,87 +1,496 @@
 // The forcing of the barotropic step from the 3-D model (Control/
-// dyncall.f90:37-130 with ksw_vert = ksw_lat = ksw_lat4 = 0): stress
-// components of the level n-1 3-D and 2-D velocities, the surface and
-// bottom stresses (the bottom level k = nz of the 3-D velocities), uv_trans
-// and uv_diff2 at every level and on the 2-D velocities, the caller's 3-D
-// pressure-gradient terms, depth_ave with dz, and the assembly of
-// RHSx2d / RHSy2d (:109-126).  Reference index ranges, masks and
-// operation order (-ffp-contract=off), one level at a time through the
-// stage-by-stage path's kernels.
+// dyncall.f90:37-130 with ksw_vert = ksw_lat = ksw_lat4 = 0) in one kernel:
+// stress components of the level n-1 velocities (:38-39), uv_trans and
+// uv_diff2 at every level and on the 2-D velocities (:65-81), the caller's
+// pressure-gradient terms (:92-98), depth_ave with dz (:100-104), the
+// surface and bottom stresses (:53-59, the bottom level k = nz of the 3-D
+// velocities) and the assembly of RHSx2d / RHSy2d (:107-126).
+//
+// The reference materialises stress_t/stress_s, r_vort and four 3-D RHS
+// arrays per level and re-reads them; here a block of F3_NW waves owns a
+// 64-column tile of F3_NW * F3_R rows (62 output columns and all rows but
+// the first and last: a one-point halo), keeps each owned point's metrics
+// and depths in registers for the whole call and marches the levels: per
+// level it reads uu, vv, uup, vvp, amuv (and pgx, pgy) once, exchanges x
+// neighbours by DPP lane shifts and y neighbours through the thread's own
+// rows or a small LDS exchange between waves, and adds the level's terms
+// times dz to depth sums held in registers.  Every value is formed with the reference's expression and
+// operation order (this file is compiled with -ffp-contract=off), so the
+// result is the reference's bit for bit.  On periodic boundaries the
+// wrapped halo values of the stress, vorticity and bottom-friction arrays
+// (Service/oc_algorithms.f90:50-95, :118-135) come from a small table a
+// pre-pass fills from their source points.
 #include <string>
 
+#include "bt_fused_body.h"
 #include "bt_internal.h"
-#include "bt_literal.h"
 
 namespace ibt {
 
 namespace {
 
-#define KERNEL_MN                                            \
-    const int m = blockIdx.x * blockDim.x + threadIdx.x + 1; \
-    const int n = blockIdx.y * blockDim.y + threadIdx.y + 1;
-#define I(mm, nn) IDX(g, (mm), (nn))
-#define IN_INTERIOR(g) (m >= 3 && m <= (g).nx - 2 && n >= 3 && n <= (g).ny - 2)
+constexpr int F3_R = 1;             // rows per thread
+constexpr int F3_NW = 12;           // waves per block
+constexpr int F3_ROWS = F3_R * F3_NW;  // tile rows including the two halo rows
+constexpr int F3_COLS = 62;         // output columns per block (lanes 1..62)
 
-__global__ void k_fill3(size_t count, double *__restrict__ a, double v)
+struct F3Params {
+    DevGrid g;
+    int nz, with_pg;
+    const double *dz;                                         // nz values
+    const double *uu, *vv, *uup, *vvp, *amuv, *pgx, *pgy;     // (nz, ny, nx)
+    const double *uu2d, *vv2d, *uup2d, *vvp2d, *amuv2d;
+    const double *hhq, *hhu, *hhv, *hhh, *taux, *tauy, *slpr;
+    const double *rhsx_in, *rhsy_in;
+    double *rhsx, *rhsy;                                      // may alias rhs*_in
+    const uint8_t *mb;
+    double cb_nl, ebottom;
+    // periodic halo table: (st, ss, vort) per wrapped point and level
+    // (level nz: the 2-D velocities; level nz + 1: bf in slot 0)
+    double *halo;
+    int nslots;
+};
+
+__device__ __forceinline__ bool interior(const DevGrid &g, int m, int n)
 {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) a[i] = v;
+    return m >= 3 && m <= g.nx - 2 && n >= 3 && n <= g.ny - 2;
 }
 
-// pressure_gradients adds its terms to the 3-D RHS (dyncall.f90:92-98)
-__global__ void k_add3(size_t count, double *__restrict__ a, const double *__restrict__ b)
+// Values of the stress, vorticity and bottom-friction arrays at (m, n)
+// read from global memory: used for the periodic halo points, whose values
+// are those of the wrapped source point.  Same expressions as the owned-
+// point path below and as Function/mixing.f90:115-168, vel_ssh.f90:30-42,
+// fluxes.f90:231-243.
+__device__ double st_at(const DevGrid &g, const uint8_t *mb, const double *u, const double *v, int m, int n)
 {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) a[i] = a[i] + b[i];
+    if (!interior(g, m, n)) return 0.0;
+    const size_t i = IDX(g, m, n), il = IDX(g, m - 1, n), is = IDX(g, m, n - 1);
+    if (!(mb[i] & MB_LU)) return 0.0;
+    return g.dy[i] / g.dx[i] * (u[i] / g.dyh[i] - u[il] / g.dyh[il]) -
+           g.dx[i] / g.dy[i] * (v[i] / g.dxh[i] - v[is] / g.dxh[is]);
 }
 
-// bottom stress, Function/fluxes.f90:261-270
-__global__ void k_bot_stress(DevGrid g, const double *__restrict__ bf, const double *__restrict__ uup,
-                             const double *__restrict__ vvp, double *__restrict__ bx, double *__restrict__ by)
+__device__ double ss_at(const DevGrid &g, const uint8_t *mb, const double *u, const double *v, int m, int n)
 {
-    KERNEL_MN
-    if (!IN_INTERIOR(g)) return;
-    const size_t i = I(m, n);
-    if (g.lcu[i] > 0.5f) bx[i] = -(bf[i] + bf[I(m + 1, n)]) / 2.0 * uup[i];
-    if (g.lcv[i] > 0.5f) by[i] = -(bf[i] + bf[I(m, n + 1)]) / 2.0 * vvp[i];
+    if (!interior(g, m, n)) return 0.0;
+    const size_t i = IDX(g, m, n), iN = IDX(g, m, n + 1), iE = IDX(g, m + 1, n);
+    if (!(mb[i] & MB_LUU)) return 0.0;
+    return g.dxb[i] / g.dyb[i] * (u[iN] / g.dxt[iN] - u[i] / g.dxt[i]) +
+           g.dyb[i] / g.dxb[i] * (v[iE] / g.dyt[iE] - v[i] / g.dyt[i]);
 }
 
-// depth_ave (Function/depth.f90:223-254) with kbcl = 0 over
-// bnd_x1+1..bnd_x2-1 x bnd_y1+1..bnd_y2-1, on mask llu (v = 0) or llv (v = 1)
-__global__ void k_depth_ave(DevGrid g, int nz, const double *__restrict__ dz, const double *__restrict__ u,
-                            int v, double *__restrict__ utr)
+__device__ double vort_at(const DevGrid &g, const uint8_t *mb, const double *u, const double *v, int m, int n)
 {
-    KERNEL_MN
-    if (m < 2 || m > g.nx - 1 || n < 2 || n > g.ny - 1) return;
-    const size_t i = I(m, n), plane = (size_t)g.nx * g.ny;
-    if ((v ? g.llv[i] : g.llu[i]) > 0.5f) {
-        double f = 0.0;
-        for (int k = 0; k < nz; ++k) f = f + u[k * plane + i] * dz[k];
-        utr[i] = f;
+    if (!interior(g, m, n)) return 0.0;
+    const size_t i = IDX(g, m, n), iN = IDX(g, m, n + 1), iE = IDX(g, m + 1, n);
+    if (!(mb[i] & MB_LUU)) return 0.0;
+    return (v[iE] * g.dyt[iE] - v[i] * g.dyt[i]) - (u[iN] * g.dxt[iN] - u[i] * g.dxt[i]) -
+           ((v[iE] - v[i]) * g.dyb[i] - (u[iN] - u[i]) * g.dxb[i]);
+}
+
+__device__ double bf_at(const F3Params &p, const double *uu, const double *vv, int m, int n)
+{
+    const DevGrid &g = p.g;
+    if (!interior(g, m, n)) return 0.0;
+    const size_t i = IDX(g, m, n), il = IDX(g, m - 1, n), is = IDX(g, m, n - 1);
+    if (!(p.mb[i] & MB_LU)) return 0.0;
+    const double ut = (uu[i] * g.dxt[i] * g.dyh[i] * p.hhu[i] + uu[il] * g.dxt[il] * g.dyh[il] * p.hhu[il]) /
+                      2.0 / g.dx[i] / g.dy[i] / p.hhq[i];
+    const double vt = (vv[i] * g.dxh[i] * g.dyt[i] * p.hhv[i] + vv[is] * g.dxh[is] * g.dyt[is] * p.hhv[is]) /
+                      2.0 / g.dx[i] / g.dy[i] / p.hhq[i];
+    return p.cb_nl * sqrt(ut * ut + vt * vt + p.ebottom);
+}
+
+// Row of the periodic halo table of (m, n), -1 when (m, n) is not a
+// wrapped point: columns 2 and nx - 1 with periodicity in x, rows 2 and
+// ny - 1 with periodicity in y.
+__device__ __forceinline__ int halo_slot(const DevGrid &g, int m, int n)
+{
+    if (g.px && (m == 2 || m == g.nx - 1)) return (m == 2 ? 0 : g.ny) + n - 1;
+    if (g.py && (n == 2 || n == g.ny - 1)) return 2 * g.ny + (n == 2 ? 0 : g.nx) + m - 1;
+    return -1;
+}
+
+// The periodic wrap (Service/oc_algorithms.f90:50-95 then :118-135) as a
+// map from a halo point to the point whose value it receives.
+__device__ __forceinline__ void wrap_src(const DevGrid &g, int m, int n, int &mm, int &nn)
+{
+    mm = (g.px && m == 2) ? g.nx - 2 : (g.px && m == g.nx - 1) ? 3 : m;
+    nn = (g.py && n == 2) ? g.ny - 2 : (g.py && n == g.ny - 1) ? 3 : n;
+}
+
+// One thread per (wrapped point, level): the stress, vorticity (levels
+// 0 .. nz) and bottom-friction (level nz + 1) values the wrap gives it.
+__global__ void k_wrap_halo(F3Params p)
+{
+    const DevGrid &g = p.g;
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x, lvl = blockIdx.y;
+    if (slot >= p.nslots) return;
+    int m, n;
+    if (slot < 2 * g.ny) {
+        if (!g.px) return;
+        m = slot < g.ny ? 2 : g.nx - 1;
+        n = slot % g.ny + 1;
+    } else {
+        if (!g.py) return;
+        const int r = slot - 2 * g.ny;
+        n = r < g.nx ? 2 : g.ny - 1;
+        m = r % g.nx + 1;
+    }
+    if (halo_slot(g, m, n) != slot) return;  // a corner listed under x
+    int mm, nn;
+    wrap_src(g, m, n, mm, nn);
+    double *h = p.halo + ((size_t)lvl * p.nslots + slot) * 3;
+    const size_t pl = (size_t)g.nx * g.ny;
+    if (lvl == p.nz + 1) {
+        const size_t bot = (size_t)(p.nz - 1) * pl;
+        h[0] = bf_at(p, p.uu + bot, p.vv + bot, mm, nn);
+        return;
+    }
+    const bool l2 = lvl == p.nz;
+    const size_t off = l2 ? 0 : (size_t)lvl * pl;
+    const double *U = l2 ? p.uu2d : p.uu + off, *V = l2 ? p.vv2d : p.vv + off;
+    const double *UP = l2 ? p.uup2d : p.uup + off, *VP = l2 ? p.vvp2d : p.vvp + off;
+    h[0] = st_at(g, p.mb, UP, VP, mm, nn);
+    h[1] = ss_at(g, p.mb, UP, VP, mm, nn);
+    h[2] = vort_at(g, p.mb, U, V, mm, nn);
+}
+
+// Level-independent values of one owned point.
+struct Pt {
+    double cdydx, cdxdy, cbxy, cbyx;  // dy/dx, dx/dy, dxb/dyb, dyb/dxb
+    double dy2, dx2, dxb, dyb, dxb2, dyb2, dxh, dyh, dxt, dyt;
+    double hu, hv, hh, hq;
+};
+
+// What a thread holds for its R owned points (rows n[0] .. n[R-1] of
+// column m) for the whole call.
+template <int R>
+struct Own {
+    int lane, w, m;
+    int n[R], slot[R];
+    size_t idx[R];
+    bool ok[R], out[R], in[R];
+    uint8_t mbit[R];
+    Pt s[R];
+    double dxb2_b, hh_b, luu_b;  // of the point below row 0
+};
+
+template <int NW>
+struct Lds {
+    double e0[NW][4][64];   // before the stress step
+    double e1[NW][12][64];  // before the transport/viscosity step
+};
+
+// uv_trans and uv_diff2 of one level (Function/vel_ssh.f90:30-179 with the
+// stress components of mixing.f90:115-168) at the owned points: radv/rady
+// the transport terms, rdfx/rdfy the viscous terms, zero off lcu/lcv or
+// outside the interior.  U, V: level-n velocities; UP, VP: level n-1; MU
+// the viscosity.  Ends after the second exchange barrier; the caller must
+// not rewrite e0 before every wave has passed it (the next call's first
+// writes come after that barrier in program order).
+template <int R, int NW>
+__device__ __forceinline__ void level_terms(const F3Params &p, const Own<R> &o, Lds<NW> &L, int lvl, const double *U,
+                                            const double *V, const double *UP, const double *VP, const double *MU,
+                                            double (&radv)[R], double (&rdfx)[R], double (&rady)[R],
+                                            double (&rdfy)[R])
+{
+    const int lane = o.lane, w = o.w;
+    double u[R], v[R], mu[R], a[R], b[R], c[R], d[R], ev[R], fu[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const size_t i = o.idx[j];
+        u[j] = o.ok[j] ? U[i] : 0.0;
+        v[j] = o.ok[j] ? V[i] : 0.0;
+        const double up = o.ok[j] ? UP[i] : 0.0, vp = o.ok[j] ? VP[i] : 0.0;
+        mu[j] = o.ok[j] ? MU[i] : 0.0;
+        a[j] = up / o.s[j].dyh;
+        b[j] = vp / o.s[j].dxh;
+        c[j] = up / o.s[j].dxt;
+        d[j] = vp / o.s[j].dyt;
+        ev[j] = v[j] * o.s[j].dyt;
+        fu[j] = u[j] * o.s[j].dxt;
+    }
+    L.e0[w][0][lane] = c[0];
+    L.e0[w][1][lane] = fu[0];
+    L.e0[w][2][lane] = u[0];
+    L.e0[w][3][lane] = b[R - 1];
+    __syncthreads();
+    double ss[R], A[R], B[R], Cv[R], D[R], E[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const double b_s = j > 0 ? b[j - 1] : (w > 0 ? L.e0[w - 1][3][lane] : 0.0);
+        const bool up_ok = j < R - 1 || w < NW - 1;
+        const double c_n = j < R - 1 ? c[j + 1] : (up_ok ? L.e0[w + 1][0][lane] : 0.0);
+        const double f_n = j < R - 1 ? fu[j + 1] : (up_ok ? L.e0[w + 1][1][lane] : 0.0);
+        const double u_n = j < R - 1 ? u[j + 1] : (up_ok ? L.e0[w + 1][2][lane] : 0.0);
+        const double a_w = xl(a[j]);
+        const double d_e = xr(d[j]), e_e = xr(ev[j]), v_e = xr(v[j]);
+        const Pt &s = o.s[j];
+        const bool inj = o.in[j];
+        double st = (inj && (o.mbit[j] & MB_LU)) ? s.cdydx * (a[j] - a_w) - s.cdxdy * (b[j] - b_s) : 0.0;
+        ss[j] = (inj && (o.mbit[j] & MB_LUU)) ? s.cbxy * (c_n - c[j]) + s.cbyx * (d_e - d[j]) : 0.0;
+        double vo = (inj && (o.mbit[j] & MB_LUU))
+                        ? (e_e - ev[j]) - (f_n - fu[j]) - ((v_e - v[j]) * s.dyb - (u_n - u[j]) * s.dxb)
+                        : 0.0;
+        if (o.slot[j] >= 0) {
+            const double *h = p.halo + ((size_t)lvl * p.nslots + o.slot[j]) * 3;
+            st = h[0];
+            ss[j] = h[1];
+            vo = h[2];
+        }
+        // products shared by neighbouring stencils: A = u dyh hu, B = v dxh hv,
+        // C = vort hh, D = dy^2 mu hq st, E = dx^2 mu hq st
+        A[j] = u[j] * s.dyh * s.hu;
+        B[j] = v[j] * s.dxh * s.hv;
+        Cv[j] = vo * s.hh;
+        D[j] = s.dy2 * mu[j] * s.hq * st;
+        E[j] = s.dx2 * mu[j] * s.hq * st;
+    }
+    // first row: what the wave above needs at n + 1; last row: what the
+    // wave below needs at n - 1
+    L.e1[w][0][lane] = u[0];
+    L.e1[w][1][lane] = mu[0];
+    L.e1[w][2][lane] = A[0];
+    L.e1[w][3][lane] = E[0];
+    L.e1[w][4][lane] = B[0];
+    L.e1[w][5][lane] = v[0];
+    L.e1[w][6][lane] = B[R - 1];
+    L.e1[w][7][lane] = Cv[R - 1];
+    L.e1[w][8][lane] = v[R - 1];
+    L.e1[w][9][lane] = mu[R - 1];
+    L.e1[w][10][lane] = ss[R - 1];
+    L.e1[w][11][lane] = u[R - 1];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const Pt &s = o.s[j];
+        const bool up_ok = j < R - 1 || w < NW - 1, dn_ok = j > 0 || w > 0;
+        const bool cu = o.in[j] && (o.mbit[j] & MB_LCU), cv = o.in[j] && (o.mbit[j] & MB_LCV);
+        const double u_n = j < R - 1 ? u[j + 1] : (up_ok ? L.e1[w + 1][0][lane] : 0.0);
+        const double mu_n = j < R - 1 ? mu[j + 1] : (up_ok ? L.e1[w + 1][1][lane] : 0.0);
+        const double mu_e = xr(mu[j]), mu_ne = xr(mu_n);
+        const double muh_p = (mu[j] + mu_e + mu_n + mu_ne) / 4.0;
+        {  // U face
+            const double B_s = j > 0 ? B[j - 1] : (dn_ok ? L.e1[w - 1][6][lane] : 0.0);
+            const double C_s = j > 0 ? Cv[j - 1] : (dn_ok ? L.e1[w - 1][7][lane] : 0.0);
+            const double v_s = j > 0 ? v[j - 1] : (dn_ok ? L.e1[w - 1][8][lane] : 0.0);
+            const double mu_s = j > 0 ? mu[j - 1] : (dn_ok ? L.e1[w - 1][9][lane] : 0.0);
+            const double ss_s = j > 0 ? ss[j - 1] : (dn_ok ? L.e1[w - 1][10][lane] : 0.0);
+            const double u_s = j > 0 ? u[j - 1] : (dn_ok ? L.e1[w - 1][11][lane] : 0.0);
+            const double dxb2_s = j > 0 ? o.s[j - 1].dxb2 : o.dxb2_b;
+            const double hh_s = j > 0 ? o.s[j - 1].hh : o.hh_b;
+            const double luu_s = j > 0 ? ((o.mbit[j - 1] & MB_LUU) ? 1.0 : 0.0) : o.luu_b;
+            const double luu = (o.mbit[j] & MB_LUU) ? 1.0 : 0.0;
+            const double A_e = xr(A[j]), A_w = xl(A[j]), u_e = xr(u[j]), u_w = xl(u[j]), B_e = xr(B[j]);
+            const double B_se = xr(B_s), v_e = xr(v[j]), v_se = xr(v_s), mu_se = xr(mu_s), D_e = xr(D[j]);
+            const double fx_p = (A[j] + A_e) / 2.0 * (u[j] + u_e) / 2.0;
+            const double fx_m = (A[j] + A_w) / 2.0 * (u[j] + u_w) / 2.0;
+            const double fy_p = (B[j] + B_e) / 2.0 * (u_n + u[j]) / 2.0 * luu;
+            const double fy_m = (B_s + B_se) / 2.0 * (u_s + u[j]) / 2.0 * luu_s;
+            const double tr = -(fx_p - fx_m + fy_p - fy_m) + (Cv[j] * (v_e + v[j]) + C_s * (v_se + v_s)) / 4.0;
+            const double muh_m = (mu[j] + mu_e + mu_s + mu_se) / 4.0;
+            const double pp = (D_e - D[j]) / s.dyh;
+            const double qq = (s.dxb2 * muh_p * s.hh * ss[j] - dxb2_s * muh_m * hh_s * ss_s) / s.dxt;
+            radv[j] = cu ? tr : 0.0;
+            rdfx[j] = cu ? pp + qq : 0.0;
+        }
+        {  // V face
+            const double A_n = j < R - 1 ? A[j + 1] : (up_ok ? L.e1[w + 1][2][lane] : 0.0);
+            const double E_n = j < R - 1 ? E[j + 1] : (up_ok ? L.e1[w + 1][3][lane] : 0.0);
+            const double B_n = j < R - 1 ? B[j + 1] : (up_ok ? L.e1[w + 1][4][lane] : 0.0);
+            const double v_n = j < R - 1 ? v[j + 1] : (up_ok ? L.e1[w + 1][5][lane] : 0.0);
+            const double B_s = j > 0 ? B[j - 1] : (dn_ok ? L.e1[w - 1][6][lane] : 0.0);
+            const double v_s = j > 0 ? v[j - 1] : (dn_ok ? L.e1[w - 1][8][lane] : 0.0);
+            const double A_w = xl(A[j]), v_e = xr(v[j]), v_w = xl(v[j]), A_nw = xl(A_n), C_w = xl(Cv[j]);
+            const double u_nw = xl(u_n), u_w = xl(u[j]), mu_w = xl(mu[j]), mu_nw = xl(mu_n);
+            const double dyb2_w = xl(s.dyb2), hh_w = xl(s.hh), ss_w = xl(ss[j]);
+            const double fy_p = (B[j] + B_n) / 2.0 * (v[j] + v_n) / 2.0;
+            const double fy_m = (B[j] + B_s) / 2.0 * (v[j] + v_s) / 2.0;
+            const double fx_p = (A[j] + A_n) / 2.0 * (v_e + v[j]) / 2.0;
+            const double fx_m = (A_w + A_nw) / 2.0 * (v_w + v[j]) / 2.0;
+            const double tr = -(fx_p - fx_m + fy_p - fy_m) - (Cv[j] * (u_n + u[j]) + C_w * (u_nw + u_w)) / 4.0;
+            const double muh_m = (mu[j] + mu_w + mu_n + mu_nw) / 4.0;
+            const double pp = (E_n - E[j]) / s.dxh;
+            const double qq = (s.dyb2 * muh_p * s.hh * ss[j] - dyb2_w * muh_m * hh_w * ss_w) / s.dyt;
+            rady[j] = cv ? tr : 0.0;
+            rdfy[j] = cv ? -pp + qq : 0.0;
+        }
     }
 }
 
-struct AsmArgs {
-    const double *dfx, *dfy, *tx2, *ty2, *txd, *tyd, *sfx, *sfy, *bfx, *bfy, *hhu, *hhv, *slpr;
-    double *rhsx, *rhsy;
-};
-
-// Control/dyncall.f90:107-126
-__global__ void k_forcing_assembly(DevGrid g, AsmArgs a)
+template <int R, int NW>
+__global__ __launch_bounds__(64 * NW) void k_forcing(F3Params p)
 {
-    KERNEL_MN
-    if (!IN_INTERIOR(g)) return;
-    const size_t i = I(m, n);
-    if (g.lcu[i] > 0.5f)
-        a.rhsx[i] = a.rhsx[i] - a.dfx[i] + a.tx2[i] - a.txd[i] + (a.sfx[i] + a.bfx[i]) * g.dxt[i] * g.dyh[i] -
-                    (a.slpr[I(m + 1, n)] - a.slpr[i]) * a.hhu[i] * g.dyh[i] / g.rho0;
-    if (g.lcv[i] > 0.5f)
-        a.rhsy[i] = a.rhsy[i] - a.dfy[i] + a.ty2[i] - a.tyd[i] + (a.sfy[i] + a.bfy[i]) * g.dyt[i] * g.dxh[i] -
-                    (a.slpr[I(m, n + 1)] - a.slpr[i]) * a.hhv[i] * g.dxh[i] / g.rho0;
+    const DevGrid &g = p.g;
+    const int nx = g.nx, ny = g.ny;
+    const size_t pl = (size_t)nx * ny;
+    __shared__ Lds<NW> L;
+    Own<R> o;
+    o.lane = threadIdx.x & 63;
+    o.w = threadIdx.x >> 6;
+    const int lane = o.lane, w = o.w;
+    // tile: columns 1 + 62 bx .. 64 + 62 bx, rows 1 + (R NW - 2) by .. ;
+    // owned rows of this thread: n0 .. n0 + R - 1
+    const int m = 1 + (int)blockIdx.x * F3_COLS + lane;
+    const int n0 = 1 + (int)blockIdx.y * (R * NW - 2) + w * R;
+    o.m = m;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int n = n0 + j, e = w * R + j;  // e: tile row
+        o.n[j] = n;
+        o.ok[j] = m <= nx && n <= ny;
+        o.out[j] = o.ok[j] && lane >= 1 && lane <= 62 && e >= 1 && e <= R * NW - 2 && m >= 2 && m <= nx - 1 &&
+                   n >= 2 && n <= ny - 1;
+        o.in[j] = interior(g, m, n);
+        o.slot[j] = o.ok[j] ? halo_slot(g, m, n) : -1;
+        o.idx[j] = o.ok[j] ? IDX(g, m, n) : 0;
+        const size_t i = o.idx[j];
+        Pt &s = o.s[j];
+        if (o.ok[j]) {
+            const double dx = g.dx[i], dy = g.dy[i], dxb = g.dxb[i], dyb = g.dyb[i];
+            s.cdydx = dy / dx;
+            s.cdxdy = dx / dy;
+            s.cbxy = dxb / dyb;
+            s.cbyx = dyb / dxb;
+            s.dy2 = dy * dy;
+            s.dx2 = dx * dx;
+            s.dxb = dxb;
+            s.dyb = dyb;
+            s.dxb2 = dxb * dxb;
+            s.dyb2 = dyb * dyb;
+            s.dxh = g.dxh[i];
+            s.dyh = g.dyh[i];
+            s.dxt = g.dxt[i];
+            s.dyt = g.dyt[i];
+            s.hu = p.hhu[i];
+            s.hv = p.hhv[i];
+            s.hh = p.hhh[i];
+            s.hq = p.hhq[i];
+            o.mbit[j] = p.mb[i];
+        } else {
+            s = Pt{1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0};
+            o.mbit[j] = 0;
+        }
+    }
+    // dxb^2, hh and the luu mask of the point below row 0 (n0 - 1)
+    L.e1[w][0][lane] = o.s[R - 1].dxb2;
+    L.e1[w][1][lane] = o.s[R - 1].hh;
+    L.e1[w][2][lane] = (o.mbit[R - 1] & MB_LUU) ? 1.0 : 0.0;
+    __syncthreads();
+    o.dxb2_b = w > 0 ? L.e1[w - 1][0][lane] : 1.0;
+    o.hh_b = w > 0 ? L.e1[w - 1][1][lane] : 1.0;
+    o.luu_b = w > 0 ? L.e1[w - 1][2][lane] : 0.0;
+
+    // depth sums (depth_ave, Function/depth.f90:223-254) of RHS3d = uv_diff2
+    // + pressure gradients and of the transport terms, in level order
+    double ax[R], atx[R], ay[R], aty[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) ax[j] = atx[j] = ay[j] = aty[j] = 0.0;
+    for (int k = 0; k < p.nz; ++k) {
+        const size_t off = (size_t)k * pl;
+        double radv[R], rdfx[R], rady[R], rdfy[R];
+        level_terms<R, NW>(p, o, L, k, p.uu + off, p.vv + off, p.uup + off, p.vvp + off, p.amuv + off, radv, rdfx,
+                           rady, rdfy);
+        const double dzk = p.dz[k];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            double rx = rdfx[j], ry = rdfy[j];
+            if (p.with_pg && o.out[j]) {
+                rx = rx + p.pgx[off + o.idx[j]];
+                ry = ry + p.pgy[off + o.idx[j]];
+            }
+            ax[j] = ax[j] + rx * dzk;
+            ay[j] = ay[j] + ry * dzk;
+            atx[j] = atx[j] + radv[j] * dzk;
+            aty[j] = aty[j] + rady[j] * dzk;
+        }
+    }
+    // the 2-D velocities (RHSx2d_tran_disp, RHSx2d_diff_disp)
+    double txd[R], dfx[R], tyd[R], dfy[R];
+    level_terms<R, NW>(p, o, L, p.nz, p.uu2d, p.vv2d, p.uup2d, p.vvp2d, p.amuv2d, txd, dfx, tyd, dfy);
+
+    // surface and bottom stresses (Function/fluxes.f90:29-37, :231-272) on
+    // the bottom level, then the assembly (Control/dyncall.f90:107-126)
+    const size_t bot = (size_t)(p.nz - 1) * pl;
+    const double *UB = p.uu + bot, *VB = p.vv + bot;
+    double G[R], H[R], wx[R], wy[R], sl[R], dxr[R], dyr[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const size_t i = o.idx[j];
+        const bool ok = o.ok[j];
+        const double dx = ok ? g.dx[i] : 1.0, dy = ok ? g.dy[i] : 1.0;
+        dxr[j] = dx;
+        dyr[j] = dy;
+        G[j] = (ok ? UB[i] : 0.0) * o.s[j].dxt * o.s[j].dyh * o.s[j].hu;
+        H[j] = (ok ? VB[i] : 0.0) * o.s[j].dxh * o.s[j].dyt * o.s[j].hv;
+        wx[j] = (ok ? p.taux[i] : 0.0) * dx * dy;
+        wy[j] = (ok ? p.tauy[i] : 0.0) * dx * dy;
+        sl[j] = ok ? p.slpr[i] : 0.0;
+    }
+    __syncthreads();  // every wave is done with e0/e1 of the last level
+    L.e0[w][0][lane] = H[R - 1];
+    L.e0[w][1][lane] = wy[0];
+    L.e0[w][2][lane] = sl[0];
+    __syncthreads();
+    double bf[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const double H_s = j > 0 ? H[j - 1] : (w > 0 ? L.e0[w - 1][0][lane] : 0.0);
+        const double G_w = xl(G[j]);
+        const double hq = o.s[j].hq;
+        const double ut = (G[j] + G_w) / 2.0 / dxr[j] / dyr[j] / hq;
+        const double vt = (H[j] + H_s) / 2.0 / dxr[j] / dyr[j] / hq;
+        bf[j] = (o.in[j] && (o.mbit[j] & MB_LU)) ? p.cb_nl * sqrt(ut * ut + vt * vt + p.ebottom) : 0.0;
+        if (o.slot[j] >= 0) bf[j] = p.halo[((size_t)(p.nz + 1) * p.nslots + o.slot[j]) * 3];
+    }
+    L.e1[w][0][lane] = bf[0];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const bool up_ok = j < R - 1 || w < NW - 1;
+        const double bf_n = j < R - 1 ? bf[j + 1] : (up_ok ? L.e1[w + 1][0][lane] : 0.0);
+        const double wy_n = j < R - 1 ? wy[j + 1] : (up_ok ? L.e0[w + 1][1][lane] : 0.0);
+        const double sl_n = j < R - 1 ? sl[j + 1] : (up_ok ? L.e0[w + 1][2][lane] : 0.0);
+        const double bf_e = xr(bf[j]), wx_e = xr(wx[j]), sl_e = xr(sl[j]);
+        if (!o.out[j]) continue;
+        const Pt &s = o.s[j];
+        const size_t i = o.idx[j];
+        const uint8_t mbj = o.mbit[j];
+        double X = (mbj & MB_LLU) ? ax[j] : p.rhsx_in[i];
+        double Y = (mbj & MB_LLV) ? ay[j] : p.rhsy_in[i];
+        const double tx2 = (mbj & MB_LLU) ? atx[j] : 0.0;
+        const double ty2 = (mbj & MB_LLV) ? aty[j] : 0.0;
+        const double rho0 = g.rho0;
+        if (o.in[j] && (mbj & MB_LCU)) {
+            const double sfx = (wx[j] + wx_e) / rho0 / 2.0 / s.dxt / s.dyh;
+            const double bfx = -(bf[j] + bf_e) / 2.0 * p.uup[bot + i];
+            X = X - dfx[j] + tx2 - txd[j] + (sfx + bfx) * s.dxt * s.dyh - (sl_e - sl[j]) * s.hu * s.dyh / rho0;
+        }
+        if (o.in[j] && (mbj & MB_LCV)) {
+            const double sfy = (wy[j] + wy_n) / rho0 / 2.0 / s.dxh / s.dyt;
+            const double bfy = -(bf[j] + bf_n) / 2.0 * p.vvp[bot + i];
+            Y = Y - dfy[j] + ty2 - tyd[j] + (sfy + bfy) * s.dyt * s.dxh - (sl_n - sl[j]) * s.hv * s.dxh / rho0;
+        }
+        p.rhsx[i] = X;
+        p.rhsy[i] = Y;
+    }
 }
 
-inline dim3 grid2(const DevGrid &g) { return dim3((g.nx + 63) / 64, (g.ny + 3) / 4); }
-const dim3 BLK2(64, 4);
+__global__ void k_pack_forcing_masks(size_t n, const float *lu, const float *luu, const float *lcu,
+                                     const float *lcv, const float *llu, const float *llv, uint8_t *mb)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint8_t b = 0;
+    if (lu[i] > 0.5f) b |= MB_LU;
+    if (luu[i] > 0.5f) b |= MB_LUU;
+    if (lcu[i] > 0.5f) b |= MB_LCU;
+    if (lcv[i] > 0.5f) b |= MB_LCV;
+    if (llu[i] > 0.5f) b |= MB_LLU;
+    if (llv[i] > 0.5f) b |= MB_LLV;
+    mb[i] = b;
+}
 
 int ffail(const std::string &m, int code)
 {
@@ -101,81 +510,101 @@ int ffail(const std::string &m, int code)
         if (r_ != INMSOM_BT_OK) return r_; \
     } while (0)
 
-// staging: 3-D inputs and work arrays (nz planes each), 2-D planes, dz
-enum { UU, VV, UUP, VVP, AMUV, PGX, PGY, ST3, SS3, VORT3, TX3, TY3, RX3, RY3, N3 };
-enum { UU2D, VV2D, UUP2D, VVP2D, AMUV2D, HHQ, HHU, HHV, HHH, SLPR, TAUX, TAUY, RHSX, RHSY,
-       ST2, SS2, VORT2, TXD, TYD, DFX, DFY, SFX, SFY, BFX, BFY, BF, TX2, TY2, DZ, N2 };
+// staging of the host-array entry: 3-D inputs (nz planes each), 2-D
+// planes, dz; the mask byte per cell is kept for every call
+enum { UU, VV, UUP, VVP, AMUV, PGX, PGY, N3 };
+enum { UU2D, VV2D, UUP2D, VVP2D, AMUV2D, HHQ, HHU, HHV, HHH, TAUX, TAUY, SLPR, RHSX, RHSY, DZ, N2 };
 
-size_t need_doubles(const inmsom_bt_ctx *c, int nz) { return ((size_t)N3 * nz + N2) * c->n; }
+int ensure_masks(inmsom_bt_ctx *c)
+{
+    if (c->f3_mb) return INMSOM_BT_OK;
+    CHF(hipMalloc(&c->f3_mb, c->n));
+    hipLaunchKernelGGL(k_pack_forcing_masks, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0, c->stream, c->n,
+                       c->mask[0], c->mask[1], c->mask[3], c->mask[4], c->mask[5], c->mask[6], c->f3_mb);
+    CHF(hipGetLastError());
+    return INMSOM_BT_OK;
+}
 
 int ensure_staging(inmsom_bt_ctx *c, int nz)
 {
-    const size_t need = need_doubles(c, nz);
+    const size_t need = ((size_t)N3 * nz + N2) * c->n;
     if (c->f3 && c->f3_n >= need) return INMSOM_BT_OK;
     if (c->f3) CHF(hipFree(c->f3));
     c->f3 = nullptr;
     c->f3_n = 0;
+    c->f3_nz = 0;
     CHF(hipMalloc(&c->f3, need * sizeof(double)));
     c->f3_n = need;
-    c->f3_nz = 0;
+    return INMSOM_BT_OK;
+}
+
+int ensure_halo(inmsom_bt_ctx *c, int nz)
+{
+    if (!c->g.px && !c->g.py) return INMSOM_BT_OK;
+    const size_t need = (size_t)(nz + 2) * 2 * (c->nx + c->ny) * 3;
+    if (c->f3_halo && c->f3_halo_n >= need) return INMSOM_BT_OK;
+    if (c->f3_halo) CHF(hipFree(c->f3_halo));
+    c->f3_halo = nullptr;
+    c->f3_halo_n = 0;
+    CHF(hipMalloc(&c->f3_halo, need * sizeof(double)));
+    c->f3_halo_n = need;
     return INMSOM_BT_OK;
 }
 
 inline double *p3(inmsom_bt_ctx *c, int nz, int k) { return c->f3 + (size_t)k * nz * c->n; }
 inline double *p2(inmsom_bt_ctx *c, int nz, int k) { return c->f3 + ((size_t)N3 * nz + k) * c->n; }
 
-// the device work of one call on the staged fields (with_pg: add the
-// pressure-gradient terms)
-void launch_forcing(inmsom_bt_ctx *c, int nz, bool with_pg)
+F3Params staged_params(inmsom_bt_ctx *c, int nz, bool with_pg)
 {
-    const DevGrid &g = c->g;
-    const size_t pl = c->n, n3 = (size_t)nz * pl;
-    auto fill = [&](double *a, size_t count) {
-        hipLaunchKernelGGL(k_fill3, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, c->stream, count, a, 0.0);
-    };
-    // work arrays start at zero (the reference's RHSx3d = 0 and calloc'd scratch)
-    for (int k = ST3; k <= RY3; ++k) fill(p3(c, nz, k), n3);
-    for (int k = ST2; k <= TY2; ++k) fill(p2(c, nz, k), pl);
-    // dyncall.f90:38-39
-    for (int k = 0; k < nz; ++k)
-        lit_stress(c, p3(c, nz, UUP) + k * pl, p3(c, nz, VVP) + k * pl, p3(c, nz, ST3) + k * pl,
-                   p3(c, nz, SS3) + k * pl);
-    lit_stress(c, p2(c, nz, UUP2D), p2(c, nz, VVP2D), p2(c, nz, ST2), p2(c, nz, SS2));
-    // :53-59
-    lit_surface_stress(c, p2(c, nz, TAUX), p2(c, nz, TAUY), p2(c, nz, SFX), p2(c, nz, SFY));
-    const size_t bot = (size_t)(nz - 1) * pl;
-    lit_bottom_friction(c, p3(c, nz, UU) + bot, p3(c, nz, VV) + bot, p2(c, nz, HHU), p2(c, nz, HHV),
-                        p2(c, nz, HHQ), p2(c, nz, BF));
-    hipLaunchKernelGGL(k_bot_stress, grid2(g), BLK2, 0, c->stream, g, p2(c, nz, BF), p3(c, nz, UUP) + bot,
-                       p3(c, nz, VVP) + bot, p2(c, nz, BFX), p2(c, nz, BFY));
-    // :65-81, level by level and on the 2-D velocities
-    for (int k = 0; k < nz; ++k) {
-        const size_t o = k * pl;
-        lit_trans_diff2(c, p3(c, nz, UU) + o, p3(c, nz, VV) + o, p3(c, nz, VORT3) + o, p2(c, nz, HHQ),
-                        p2(c, nz, HHU), p2(c, nz, HHV), p2(c, nz, HHH), p3(c, nz, AMUV) + o, p3(c, nz, ST3) + o,
-                        p3(c, nz, SS3) + o, p3(c, nz, TX3) + o, p3(c, nz, TY3) + o, p3(c, nz, RX3) + o,
-                        p3(c, nz, RY3) + o);
-    }
-    lit_trans_diff2(c, p2(c, nz, UU2D), p2(c, nz, VV2D), p2(c, nz, VORT2), p2(c, nz, HHQ), p2(c, nz, HHU),
-                    p2(c, nz, HHV), p2(c, nz, HHH), p2(c, nz, AMUV2D), p2(c, nz, ST2), p2(c, nz, SS2),
-                    p2(c, nz, TXD), p2(c, nz, TYD), p2(c, nz, DFX), p2(c, nz, DFY));
-    // :92-98
-    if (with_pg) {
-        const dim3 gr((unsigned)((n3 + 255) / 256));
-        hipLaunchKernelGGL(k_add3, gr, dim3(256), 0, c->stream, n3, p3(c, nz, RX3), p3(c, nz, PGX));
-        hipLaunchKernelGGL(k_add3, gr, dim3(256), 0, c->stream, n3, p3(c, nz, RY3), p3(c, nz, PGY));
-    }
-    // :100-104
-    const double *dz = p2(c, nz, DZ);
-    hipLaunchKernelGGL(k_depth_ave, grid2(g), BLK2, 0, c->stream, g, nz, dz, p3(c, nz, RX3), 0, p2(c, nz, RHSX));
-    hipLaunchKernelGGL(k_depth_ave, grid2(g), BLK2, 0, c->stream, g, nz, dz, p3(c, nz, RY3), 1, p2(c, nz, RHSY));
-    hipLaunchKernelGGL(k_depth_ave, grid2(g), BLK2, 0, c->stream, g, nz, dz, p3(c, nz, TX3), 0, p2(c, nz, TX2));
-    hipLaunchKernelGGL(k_depth_ave, grid2(g), BLK2, 0, c->stream, g, nz, dz, p3(c, nz, TY3), 1, p2(c, nz, TY2));
-    // :107-126
-    AsmArgs a{p2(c, nz, DFX), p2(c, nz, DFY), p2(c, nz, TX2), p2(c, nz, TY2), p2(c, nz, TXD), p2(c, nz, TYD),
-              p2(c, nz, SFX), p2(c, nz, SFY), p2(c, nz, BFX), p2(c, nz, BFY), p2(c, nz, HHU), p2(c, nz, HHV),
-              p2(c, nz, SLPR), p2(c, nz, RHSX), p2(c, nz, RHSY)};
-    hipLaunchKernelGGL(k_forcing_assembly, grid2(g), BLK2, 0, c->stream, g, a);
+    F3Params p{};
+    p.g = c->g;
+    p.nz = nz;
+    p.with_pg = with_pg;
+    p.dz = p2(c, nz, DZ);
+    p.uu = p3(c, nz, UU);
+    p.vv = p3(c, nz, VV);
+    p.uup = p3(c, nz, UUP);
+    p.vvp = p3(c, nz, VVP);
+    p.amuv = p3(c, nz, AMUV);
+    p.pgx = p3(c, nz, PGX);
+    p.pgy = p3(c, nz, PGY);
+    p.uu2d = p2(c, nz, UU2D);
+    p.vv2d = p2(c, nz, VV2D);
+    p.uup2d = p2(c, nz, UUP2D);
+    p.vvp2d = p2(c, nz, VVP2D);
+    p.amuv2d = p2(c, nz, AMUV2D);
+    p.hhq = p2(c, nz, HHQ);
+    p.hhu = p2(c, nz, HHU);
+    p.hhv = p2(c, nz, HHV);
+    p.hhh = p2(c, nz, HHH);
+    p.taux = p2(c, nz, TAUX);
+    p.tauy = p2(c, nz, TAUY);
+    p.slpr = p2(c, nz, SLPR);
+    p.rhsx_in = p.rhsx = p2(c, nz, RHSX);
+    p.rhsy_in = p.rhsy = p2(c, nz, RHSY);
+    p.mb = c->f3_mb;
+    p.cb_nl = c->cb_nl;
+    p.ebottom = c->ebottom;
+    p.halo = c->f3_halo;
+    p.nslots = 2 * (c->nx + c->ny);
+    return p;
+}
+
+void launch_forcing(inmsom_bt_ctx *c, const F3Params &p)
+{
+    if (c->g.px || c->g.py)
+        hipLaunchKernelGGL(k_wrap_halo, dim3((unsigned)((p.nslots + 255) / 256), (unsigned)(p.nz + 2)), dim3(256), 0,
+                           c->stream, p);
+    const dim3 grid((unsigned)((c->nx - 2 + F3_COLS - 1) / F3_COLS),
+                    (unsigned)((c->ny - 2 + F3_ROWS - 3) / (F3_ROWS - 2)));
+    hipLaunchKernelGGL((k_forcing<F3_R, F3_NW>), grid, dim3(64 * F3_NW), 0, c->stream, p);
+}
+
+int check_args(inmsom_bt_ctx *c, int nz)
+{
+    if (nz < 1 || nz > 1000) return ffail("nz must be in 1..1000", INMSOM_BT_EINVAL);
+    if (c->nx < 5 || c->ny < 5) return ffail("the forcing needs nx, ny >= 5", INMSOM_BT_EINVAL);
+    return INMSOM_BT_OK;
 }
 
 }  // namespace
@@ -197,21 +626,22 @@ int inmsom_bt_forcing_3d(inmsom_bt_ctx *c, int nz, const double *dz, const doubl
         !hhq || !hhu || !hhv || !hhh || !taux || !tauy || !slpr || !rhsx2d || !rhsy2d)
         return ffail("null pointer", INMSOM_BT_EINVAL);
     if ((pgx == nullptr) != (pgy == nullptr)) return ffail("pgx and pgy: both or neither", INMSOM_BT_EINVAL);
-    if (nz < 1 || nz > 1000) return ffail("nz must be in 1..1000", INMSOM_BT_EINVAL);
-    if ((size_t)nz > c->n) return ffail("nz exceeds the plane size", INMSOM_BT_EINVAL);
+    TRYF(check_args(c, nz));
     CHF(hipSetDevice(c->device));
+    TRYF(ensure_masks(c));
     TRYF(ensure_staging(c, nz));
+    TRYF(ensure_halo(c, nz));
     const size_t pl = c->n, n3 = (size_t)nz * pl;
     auto put = [&](double *dst, const double *src, size_t count) {
         return hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyHostToDevice, c->stream);
     };
-    const double *in3[7] = {uu, vv, uup, vvp, amuv, pgx, pgy};
-    for (int k = 0; k < 7; ++k)
+    const double *in3[N3] = {uu, vv, uup, vvp, amuv, pgx, pgy};
+    for (int k = 0; k < N3; ++k)
         if (in3[k]) CHF(put(p3(c, nz, k), in3[k], n3));
-    const double *in2[14] = {uu2d, vv2d, uup2d, vvp2d, amuv2d, hhq, hhu, hhv, hhh, slpr, taux, tauy, rhsx2d, rhsy2d};
-    for (int k = 0; k < 14; ++k) CHF(put(p2(c, nz, k), in2[k], pl));
+    const double *in2[DZ] = {uu2d, vv2d, uup2d, vvp2d, amuv2d, hhq, hhu, hhv, hhh, taux, tauy, slpr, rhsx2d, rhsy2d};
+    for (int k = 0; k < DZ; ++k) CHF(put(p2(c, nz, k), in2[k], pl));
     CHF(put(p2(c, nz, DZ), dz, nz));
-    launch_forcing(c, nz, pgx != nullptr);
+    launch_forcing(c, staged_params(c, nz, pgx != nullptr));
     CHF(hipGetLastError());
     CHF(hipMemcpyAsync(rhsx2d, p2(c, nz, RHSX), pl * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CHF(hipMemcpyAsync(rhsy2d, p2(c, nz, RHSY), pl * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -228,11 +658,12 @@ int inmsom_bt_forcing_3d_timed(inmsom_bt_ctx *c, int nz, int nrep, double *ms)
     if (!c->f3 || c->f3_nz != nz)
         return ffail("no forcing fields staged for nz levels: call inmsom_bt_forcing_3d first", INMSOM_BT_ESTATE);
     CHF(hipSetDevice(c->device));
+    const F3Params p = staged_params(c, nz, c->f3_pg);
     hipEvent_t e0, e1;
     CHF(hipEventCreate(&e0));
     CHF(hipEventCreate(&e1));
     CHF(hipEventRecord(e0, c->stream));
-    for (int r = 0; r < nrep; ++r) launch_forcing(c, nz, c->f3_pg);
+    for (int r = 0; r < nrep; ++r) launch_forcing(c, p);
     CHF(hipGetLastError());
     CHF(hipEventRecord(e1, c->stream));
     CHF(hipEventSynchronize(e1));

@@ -94,5 +94,8 @@ struct inmsom_bt_ctx {
     size_t f3_n = 0;               // doubles in f3
     int f3_nz = 0;                 // levels of the staged fields (0 = none)
     bool f3_pg = false;            // staged pressure-gradient terms
+    uint8_t *f3_mb = nullptr;      // mask bits of the forcing (lu, luu, lcu, lcv, llu, llv)
+    double *f3_halo = nullptr;     // periodic halo table of the forcing
+    size_t f3_halo_n = 0;          // doubles in f3_halo
     long long *cnt = nullptr;
 };
