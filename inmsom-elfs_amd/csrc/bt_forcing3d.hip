@@ -156,166 +156,188 @@ __global__ void k_wrap_halo(F3Params p)
     h[2] = vort_at(g, p.mb, U, V, mm, nn);
 }
 
-// Level-independent values of one owned point.
+// Level-independent values of one owned point held in registers; the
+// others (KS_*) live in LDS for the whole call.
 struct Pt {
-    double cdydx, cdxdy, cbxy, cbyx;  // dy/dx, dx/dy, dxb/dyb, dyb/dxb
-    double dy2, dx2, dxb, dyb, dxb2, dyb2, dxh, dyh, dxt, dyt;
-    double hu, hv, hh, hq;
+    double dxh, dyh, dxt, dyt, hu, hv, hh, hq;
 };
+enum { KS_DYDX, KS_DXDY, KS_BXY, KS_BYX, KS_DY2, KS_DX2, KS_DXB, KS_DYB, KS_DXB2, KS_DYB2, KS_N };
 
 // What a thread holds for its R owned points (rows n[0] .. n[R-1] of
 // column m) for the whole call.
 template <int R>
 struct Own {
-    int lane, w, m;
+    int lane, w, m, t;  // t: thread index in the block
     int n[R], slot[R];
     size_t idx[R];
     bool ok[R], out[R], in[R];
     uint8_t mbit[R];
     Pt s[R];
-    double dxb2_b, hh_b, luu_b;  // of the point below row 0
+    double dxb2_b, hh_b;  // of the point below row 0
 };
 
-template <int NW>
+template <int R, int NW>
 struct Lds {
-    double e0[NW][4][64];   // before the stress step
-    double e1[NW][12][64];  // before the transport/viscosity step
+    double e0[NW][9][64];  // before the stress step
+    double e1[NW][5][64];  // before the last step of the level
+    double ks[KS_N][NW * R * 64];
 };
+
+// One level's inputs at the owned points.
+template <int R>
+struct LevelIn {
+    double u[R], v[R], up[R], vp[R], mu[R];
+};
+
+template <int R>
+__device__ __forceinline__ void load_level(const Own<R> &o, const double *U, const double *V, const double *UP,
+                                           const double *VP, const double *MU, LevelIn<R> &in)
+{
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const size_t i = o.idx[j];
+        const bool ok = o.ok[j];
+        in.u[j] = ok ? U[i] : 0.0;
+        in.v[j] = ok ? V[i] : 0.0;
+        in.up[j] = ok ? UP[i] : 0.0;
+        in.vp[j] = ok ? VP[i] : 0.0;
+        in.mu[j] = ok ? MU[i] : 0.0;
+    }
+}
 
 // uv_trans and uv_diff2 of one level (Function/vel_ssh.f90:30-179 with the
 // stress components of mixing.f90:115-168) at the owned points: radv/rady
 // the transport terms, rdfx/rdfy the viscous terms, zero off lcu/lcv or
-// outside the interior.  U, V: level-n velocities; UP, VP: level n-1; MU
-// the viscosity.  Ends after the second exchange barrier; the caller must
-// not rewrite e0 before every wave has passed it (the next call's first
-// writes come after that barrier in program order).
+// outside the interior.  `in`: level-n velocities u, v, level n-1 up, vp
+// and the viscosity mu; `lvl` selects the level's rows of the periodic
+// halo table.
+//
+// The reference's backward fluxes are its forward fluxes of the
+// neighbouring point with the operands of each sum swapped, and IEEE
+// addition is commutative: fx_m(m, n) = fx_p(m-1, n), fy_m(m, n) =
+// fy_p(m, n-1), and the second vorticity term of the U (V) face is the
+// first one of the point below (to the left).  They are taken from the
+// neighbour, bit for bit the values the reference forms.
+//
+// Two barriers; LDS e0 and e1 are rewritten by the next call only after
+// every wave has passed the next call's first barrier or this call's
+// second one respectively.
 template <int R, int NW>
-__device__ __forceinline__ void level_terms(const F3Params &p, const Own<R> &o, Lds<NW> &L, int lvl, const double *U,
-                                            const double *V, const double *UP, const double *VP, const double *MU,
-                                            double (&radv)[R], double (&rdfx)[R], double (&rady)[R],
-                                            double (&rdfy)[R])
+__device__ __forceinline__ void level_terms(const F3Params &p, const Own<R> &o, Lds<R, NW> &L, int lvl,
+                                            const LevelIn<R> &in, double (&radv)[R], double (&rdfx)[R],
+                                            double (&rady)[R], double (&rdfy)[R])
 {
     const int lane = o.lane, w = o.w;
-    double u[R], v[R], mu[R], a[R], b[R], c[R], d[R], ev[R], fu[R];
+    const double *u = in.u, *v = in.v, *mu = in.mu;
+    double a[R], b[R], c[R], d[R], ev[R], fu[R], A[R], B[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-        const size_t i = o.idx[j];
-        u[j] = o.ok[j] ? U[i] : 0.0;
-        v[j] = o.ok[j] ? V[i] : 0.0;
-        const double up = o.ok[j] ? UP[i] : 0.0, vp = o.ok[j] ? VP[i] : 0.0;
-        mu[j] = o.ok[j] ? MU[i] : 0.0;
-        a[j] = up / o.s[j].dyh;
-        b[j] = vp / o.s[j].dxh;
-        c[j] = up / o.s[j].dxt;
-        d[j] = vp / o.s[j].dyt;
-        ev[j] = v[j] * o.s[j].dyt;
-        fu[j] = u[j] * o.s[j].dxt;
+        const Pt &s = o.s[j];
+        a[j] = in.up[j] / s.dyh;
+        b[j] = in.vp[j] / s.dxh;
+        c[j] = in.up[j] / s.dxt;
+        d[j] = in.vp[j] / s.dyt;
+        ev[j] = v[j] * s.dyt;
+        fu[j] = u[j] * s.dxt;
+        A[j] = u[j] * s.dyh * s.hu;  // the transport products u dyh hu, v dxh hv
+        B[j] = v[j] * s.dxh * s.hv;
     }
+    // first row: what the wave above needs at n + 1; last row: what the
+    // wave below needs at n - 1
     L.e0[w][0][lane] = c[0];
     L.e0[w][1][lane] = fu[0];
     L.e0[w][2][lane] = u[0];
-    L.e0[w][3][lane] = b[R - 1];
+    L.e0[w][3][lane] = A[0];
+    L.e0[w][4][lane] = B[0];
+    L.e0[w][5][lane] = v[0];
+    L.e0[w][6][lane] = mu[0];
+    L.e0[w][7][lane] = b[R - 1];
+    L.e0[w][8][lane] = mu[R - 1];
     __syncthreads();
-    double ss[R], A[R], B[R], Cv[R], D[R], E[R];
+    // per point after this step: the U-face sums fx_p - fx_m + fy_p (XU),
+    // fy_p (FYU), the vorticity term (T1), pp (PU), the first q product
+    // (QU) and muh_m (MU_M); the V-face XV, FYV, TV (both vorticity terms),
+    // qq (QV); E and ss
+    double XU[R], FYU[R], T1[R], PU[R], QU[R], MUM[R], XV[R], FYV[R], TV[R], QV[R], E[R], ss[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-        const double b_s = j > 0 ? b[j - 1] : (w > 0 ? L.e0[w - 1][3][lane] : 0.0);
-        const bool up_ok = j < R - 1 || w < NW - 1;
+        const Pt &s = o.s[j];
+        const int t = o.t * R + j;
+        const bool up_ok = j < R - 1 || w < NW - 1, dn_ok = j > 0 || w > 0;
+        const double b_s = j > 0 ? b[j - 1] : (dn_ok ? L.e0[w - 1][7][lane] : 0.0);
+        const double mu_s = j > 0 ? mu[j - 1] : (dn_ok ? L.e0[w - 1][8][lane] : 0.0);
         const double c_n = j < R - 1 ? c[j + 1] : (up_ok ? L.e0[w + 1][0][lane] : 0.0);
         const double f_n = j < R - 1 ? fu[j + 1] : (up_ok ? L.e0[w + 1][1][lane] : 0.0);
         const double u_n = j < R - 1 ? u[j + 1] : (up_ok ? L.e0[w + 1][2][lane] : 0.0);
-        const double a_w = xl(a[j]);
-        const double d_e = xr(d[j]), e_e = xr(ev[j]), v_e = xr(v[j]);
-        const Pt &s = o.s[j];
+        const double A_n = j < R - 1 ? A[j + 1] : (up_ok ? L.e0[w + 1][3][lane] : 0.0);
+        const double B_n = j < R - 1 ? B[j + 1] : (up_ok ? L.e0[w + 1][4][lane] : 0.0);
+        const double v_n = j < R - 1 ? v[j + 1] : (up_ok ? L.e0[w + 1][5][lane] : 0.0);
+        const double mu_n = j < R - 1 ? mu[j + 1] : (up_ok ? L.e0[w + 1][6][lane] : 0.0);
+        const double a_w = xl(a[j]), d_e = xr(d[j]), e_e = xr(ev[j]), v_e = xr(v[j]);
+        const double A_e = xr(A[j]), u_e = xr(u[j]), B_e = xr(B[j]);
+        const double mu_e = xr(mu[j]), mu_ne = xr(mu_n), mu_se = xr(mu_s), mu_w = xl(mu[j]), mu_nw = xl(mu_n);
         const bool inj = o.in[j];
-        double st = (inj && (o.mbit[j] & MB_LU)) ? s.cdydx * (a[j] - a_w) - s.cdxdy * (b[j] - b_s) : 0.0;
-        ss[j] = (inj && (o.mbit[j] & MB_LUU)) ? s.cbxy * (c_n - c[j]) + s.cbyx * (d_e - d[j]) : 0.0;
-        double vo = (inj && (o.mbit[j] & MB_LUU))
-                        ? (e_e - ev[j]) - (f_n - fu[j]) - ((v_e - v[j]) * s.dyb - (u_n - u[j]) * s.dxb)
+        const uint8_t mb = o.mbit[j];
+        double st = (inj && (mb & MB_LU))
+                        ? L.ks[KS_DYDX][t] * (a[j] - a_w) - L.ks[KS_DXDY][t] * (b[j] - b_s)
                         : 0.0;
+        ss[j] = (inj && (mb & MB_LUU)) ? L.ks[KS_BXY][t] * (c_n - c[j]) + L.ks[KS_BYX][t] * (d_e - d[j]) : 0.0;
+        double vo = (inj && (mb & MB_LUU)) ? (e_e - ev[j]) - (f_n - fu[j]) -
+                                                 ((v_e - v[j]) * L.ks[KS_DYB][t] - (u_n - u[j]) * L.ks[KS_DXB][t])
+                                           : 0.0;
         if (o.slot[j] >= 0) {
             const double *h = p.halo + ((size_t)lvl * p.nslots + o.slot[j]) * 3;
             st = h[0];
             ss[j] = h[1];
             vo = h[2];
         }
-        // products shared by neighbouring stencils: A = u dyh hu, B = v dxh hv,
-        // C = vort hh, D = dy^2 mu hq st, E = dx^2 mu hq st
-        A[j] = u[j] * s.dyh * s.hu;
-        B[j] = v[j] * s.dxh * s.hv;
-        Cv[j] = vo * s.hh;
-        D[j] = s.dy2 * mu[j] * s.hq * st;
-        E[j] = s.dx2 * mu[j] * s.hq * st;
+        const double Cv = vo * s.hh;
+        const double D = L.ks[KS_DY2][t] * mu[j] * s.hq * st;
+        E[j] = L.ks[KS_DX2][t] * mu[j] * s.hq * st;
+        const double luu = (mb & MB_LUU) ? 1.0 : 0.0;
+        const double fxU = (A[j] + A_e) / 2.0 * (u[j] + u_e) / 2.0;
+        FYU[j] = (B[j] + B_e) / 2.0 * (u_n + u[j]) / 2.0 * luu;
+        XU[j] = fxU - xl(fxU) + FYU[j];
+        T1[j] = Cv * (v_e + v[j]);
+        const double fxV = (A[j] + A_n) / 2.0 * (v_e + v[j]) / 2.0;
+        FYV[j] = (B[j] + B_n) / 2.0 * (v[j] + v_n) / 2.0;
+        XV[j] = fxV - xl(fxV) + FYV[j];
+        const double T2 = Cv * (u_n + u[j]);
+        TV[j] = T2 + xl(T2);
+        const double muh_p = (mu[j] + mu_e + mu_n + mu_ne) / 4.0;
+        MUM[j] = (mu[j] + mu_e + mu_s + mu_se) / 4.0;
+        PU[j] = (xr(D) - D) / s.dyh;
+        QU[j] = L.ks[KS_DXB2][t] * muh_p * s.hh * ss[j];
+        const double muh_mv = (mu[j] + mu_w + mu_n + mu_nw) / 4.0;
+        const double dyb2 = L.ks[KS_DYB2][t];
+        QV[j] = (dyb2 * muh_p * s.hh * ss[j] - xl(dyb2) * muh_mv * xl(s.hh) * xl(ss[j])) / s.dyt;
     }
-    // first row: what the wave above needs at n + 1; last row: what the
-    // wave below needs at n - 1
-    L.e1[w][0][lane] = u[0];
-    L.e1[w][1][lane] = mu[0];
-    L.e1[w][2][lane] = A[0];
-    L.e1[w][3][lane] = E[0];
-    L.e1[w][4][lane] = B[0];
-    L.e1[w][5][lane] = v[0];
-    L.e1[w][6][lane] = B[R - 1];
-    L.e1[w][7][lane] = Cv[R - 1];
-    L.e1[w][8][lane] = v[R - 1];
-    L.e1[w][9][lane] = mu[R - 1];
-    L.e1[w][10][lane] = ss[R - 1];
-    L.e1[w][11][lane] = u[R - 1];
+    L.e1[w][0][lane] = E[0];
+    L.e1[w][1][lane] = FYU[R - 1];
+    L.e1[w][2][lane] = FYV[R - 1];
+    L.e1[w][3][lane] = T1[R - 1];
+    L.e1[w][4][lane] = ss[R - 1];
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < R; ++j) {
         const Pt &s = o.s[j];
         const bool up_ok = j < R - 1 || w < NW - 1, dn_ok = j > 0 || w > 0;
+        const double E_n = j < R - 1 ? E[j + 1] : (up_ok ? L.e1[w + 1][0][lane] : 0.0);
+        const double fyU_s = j > 0 ? FYU[j - 1] : (dn_ok ? L.e1[w - 1][1][lane] : 0.0);
+        const double fyV_s = j > 0 ? FYV[j - 1] : (dn_ok ? L.e1[w - 1][2][lane] : 0.0);
+        const double T1_s = j > 0 ? T1[j - 1] : (dn_ok ? L.e1[w - 1][3][lane] : 0.0);
+        const double ss_s = j > 0 ? ss[j - 1] : (dn_ok ? L.e1[w - 1][4][lane] : 0.0);
+        const double dxb2_s = j > 0 ? L.ks[KS_DXB2][o.t * R + j - 1] : o.dxb2_b;
+        const double hh_s = j > 0 ? o.s[j - 1].hh : o.hh_b;
         const bool cu = o.in[j] && (o.mbit[j] & MB_LCU), cv = o.in[j] && (o.mbit[j] & MB_LCV);
-        const double u_n = j < R - 1 ? u[j + 1] : (up_ok ? L.e1[w + 1][0][lane] : 0.0);
-        const double mu_n = j < R - 1 ? mu[j + 1] : (up_ok ? L.e1[w + 1][1][lane] : 0.0);
-        const double mu_e = xr(mu[j]), mu_ne = xr(mu_n);
-        const double muh_p = (mu[j] + mu_e + mu_n + mu_ne) / 4.0;
-        {  // U face
-            const double B_s = j > 0 ? B[j - 1] : (dn_ok ? L.e1[w - 1][6][lane] : 0.0);
-            const double C_s = j > 0 ? Cv[j - 1] : (dn_ok ? L.e1[w - 1][7][lane] : 0.0);
-            const double v_s = j > 0 ? v[j - 1] : (dn_ok ? L.e1[w - 1][8][lane] : 0.0);
-            const double mu_s = j > 0 ? mu[j - 1] : (dn_ok ? L.e1[w - 1][9][lane] : 0.0);
-            const double ss_s = j > 0 ? ss[j - 1] : (dn_ok ? L.e1[w - 1][10][lane] : 0.0);
-            const double u_s = j > 0 ? u[j - 1] : (dn_ok ? L.e1[w - 1][11][lane] : 0.0);
-            const double dxb2_s = j > 0 ? o.s[j - 1].dxb2 : o.dxb2_b;
-            const double hh_s = j > 0 ? o.s[j - 1].hh : o.hh_b;
-            const double luu_s = j > 0 ? ((o.mbit[j - 1] & MB_LUU) ? 1.0 : 0.0) : o.luu_b;
-            const double luu = (o.mbit[j] & MB_LUU) ? 1.0 : 0.0;
-            const double A_e = xr(A[j]), A_w = xl(A[j]), u_e = xr(u[j]), u_w = xl(u[j]), B_e = xr(B[j]);
-            const double B_se = xr(B_s), v_e = xr(v[j]), v_se = xr(v_s), mu_se = xr(mu_s), D_e = xr(D[j]);
-            const double fx_p = (A[j] + A_e) / 2.0 * (u[j] + u_e) / 2.0;
-            const double fx_m = (A[j] + A_w) / 2.0 * (u[j] + u_w) / 2.0;
-            const double fy_p = (B[j] + B_e) / 2.0 * (u_n + u[j]) / 2.0 * luu;
-            const double fy_m = (B_s + B_se) / 2.0 * (u_s + u[j]) / 2.0 * luu_s;
-            const double tr = -(fx_p - fx_m + fy_p - fy_m) + (Cv[j] * (v_e + v[j]) + C_s * (v_se + v_s)) / 4.0;
-            const double muh_m = (mu[j] + mu_e + mu_s + mu_se) / 4.0;
-            const double pp = (D_e - D[j]) / s.dyh;
-            const double qq = (s.dxb2 * muh_p * s.hh * ss[j] - dxb2_s * muh_m * hh_s * ss_s) / s.dxt;
-            radv[j] = cu ? tr : 0.0;
-            rdfx[j] = cu ? pp + qq : 0.0;
-        }
-        {  // V face
-            const double A_n = j < R - 1 ? A[j + 1] : (up_ok ? L.e1[w + 1][2][lane] : 0.0);
-            const double E_n = j < R - 1 ? E[j + 1] : (up_ok ? L.e1[w + 1][3][lane] : 0.0);
-            const double B_n = j < R - 1 ? B[j + 1] : (up_ok ? L.e1[w + 1][4][lane] : 0.0);
-            const double v_n = j < R - 1 ? v[j + 1] : (up_ok ? L.e1[w + 1][5][lane] : 0.0);
-            const double B_s = j > 0 ? B[j - 1] : (dn_ok ? L.e1[w - 1][6][lane] : 0.0);
-            const double v_s = j > 0 ? v[j - 1] : (dn_ok ? L.e1[w - 1][8][lane] : 0.0);
-            const double A_w = xl(A[j]), v_e = xr(v[j]), v_w = xl(v[j]), A_nw = xl(A_n), C_w = xl(Cv[j]);
-            const double u_nw = xl(u_n), u_w = xl(u[j]), mu_w = xl(mu[j]), mu_nw = xl(mu_n);
-            const double dyb2_w = xl(s.dyb2), hh_w = xl(s.hh), ss_w = xl(ss[j]);
-            const double fy_p = (B[j] + B_n) / 2.0 * (v[j] + v_n) / 2.0;
-            const double fy_m = (B[j] + B_s) / 2.0 * (v[j] + v_s) / 2.0;
-            const double fx_p = (A[j] + A_n) / 2.0 * (v_e + v[j]) / 2.0;
-            const double fx_m = (A_w + A_nw) / 2.0 * (v_w + v[j]) / 2.0;
-            const double tr = -(fx_p - fx_m + fy_p - fy_m) - (Cv[j] * (u_n + u[j]) + C_w * (u_nw + u_w)) / 4.0;
-            const double muh_m = (mu[j] + mu_w + mu_n + mu_nw) / 4.0;
-            const double pp = (E_n - E[j]) / s.dxh;
-            const double qq = (s.dyb2 * muh_p * s.hh * ss[j] - dyb2_w * muh_m * hh_w * ss_w) / s.dyt;
-            rady[j] = cv ? tr : 0.0;
-            rdfy[j] = cv ? -pp + qq : 0.0;
-        }
+        const double trU = -(XU[j] - fyU_s) + (T1[j] + T1_s) / 4.0;
+        const double qU = (QU[j] - dxb2_s * MUM[j] * hh_s * ss_s) / s.dxt;
+        radv[j] = cu ? trU : 0.0;
+        rdfx[j] = cu ? PU[j] + qU : 0.0;
+        const double trV = -(XV[j] - fyV_s) - TV[j] / 4.0;
+        const double pV = (E_n - E[j]) / s.dxh;
+        rady[j] = cv ? trV : 0.0;
+        rdfy[j] = cv ? -pV + QV[j] : 0.0;
     }
 }
 
@@ -325,8 +347,9 @@ __global__ __launch_bounds__(64 * NW) void k_forcing(F3Params p)
     const DevGrid &g = p.g;
     const int nx = g.nx, ny = g.ny;
     const size_t pl = (size_t)nx * ny;
-    __shared__ Lds<NW> L;
+    __shared__ Lds<R, NW> L;
     Own<R> o;
+    o.t = threadIdx.x;
     o.lane = threadIdx.x & 63;
     o.w = threadIdx.x >> 6;
     const int lane = o.lane, w = o.w;
@@ -338,6 +361,7 @@ __global__ __launch_bounds__(64 * NW) void k_forcing(F3Params p)
 #pragma unroll
     for (int j = 0; j < R; ++j) {
         const int n = n0 + j, e = w * R + j;  // e: tile row
+        const int t = o.t * R + j;
         o.n[j] = n;
         o.ok[j] = m <= nx && n <= ny;
         o.out[j] = o.ok[j] && lane >= 1 && lane <= 62 && e >= 1 && e <= R * NW - 2 && m >= 2 && m <= nx - 1 &&
@@ -349,16 +373,16 @@ __global__ __launch_bounds__(64 * NW) void k_forcing(F3Params p)
         Pt &s = o.s[j];
         if (o.ok[j]) {
             const double dx = g.dx[i], dy = g.dy[i], dxb = g.dxb[i], dyb = g.dyb[i];
-            s.cdydx = dy / dx;
-            s.cdxdy = dx / dy;
-            s.cbxy = dxb / dyb;
-            s.cbyx = dyb / dxb;
-            s.dy2 = dy * dy;
-            s.dx2 = dx * dx;
-            s.dxb = dxb;
-            s.dyb = dyb;
-            s.dxb2 = dxb * dxb;
-            s.dyb2 = dyb * dyb;
+            L.ks[KS_DYDX][t] = dy / dx;
+            L.ks[KS_DXDY][t] = dx / dy;
+            L.ks[KS_BXY][t] = dxb / dyb;
+            L.ks[KS_BYX][t] = dyb / dxb;
+            L.ks[KS_DY2][t] = dy * dy;
+            L.ks[KS_DX2][t] = dx * dx;
+            L.ks[KS_DXB][t] = dxb;
+            L.ks[KS_DYB][t] = dyb;
+            L.ks[KS_DXB2][t] = dxb * dxb;
+            L.ks[KS_DYB2][t] = dyb * dyb;
             s.dxh = g.dxh[i];
             s.dyh = g.dyh[i];
             s.dxt = g.dxt[i];
@@ -369,36 +393,52 @@ __global__ __launch_bounds__(64 * NW) void k_forcing(F3Params p)
             s.hq = p.hhq[i];
             o.mbit[j] = p.mb[i];
         } else {
-            s = Pt{1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0};
+#pragma unroll
+            for (int q = 0; q < KS_N; ++q) L.ks[q][t] = 1.0;
+            s = Pt{1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0};
             o.mbit[j] = 0;
         }
     }
-    // dxb^2, hh and the luu mask of the point below row 0 (n0 - 1)
-    L.e1[w][0][lane] = o.s[R - 1].dxb2;
+    // dxb^2 and hh of the point below row 0 (n0 - 1)
+    L.e1[w][0][lane] = L.ks[KS_DXB2][o.t * R + R - 1];
     L.e1[w][1][lane] = o.s[R - 1].hh;
-    L.e1[w][2][lane] = (o.mbit[R - 1] & MB_LUU) ? 1.0 : 0.0;
     __syncthreads();
     o.dxb2_b = w > 0 ? L.e1[w - 1][0][lane] : 1.0;
     o.hh_b = w > 0 ? L.e1[w - 1][1][lane] : 1.0;
-    o.luu_b = w > 0 ? L.e1[w - 1][2][lane] : 0.0;
 
     // depth sums (depth_ave, Function/depth.f90:223-254) of RHS3d = uv_diff2
-    // + pressure gradients and of the transport terms, in level order
+    // + pressure gradients and of the transport terms, in level order; the
+    // next level's inputs are loaded while the current one is formed
     double ax[R], atx[R], ay[R], aty[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) ax[j] = atx[j] = ay[j] = aty[j] = 0.0;
+    LevelIn<R> nxt;
+    load_level(o, p.uu, p.vv, p.uup, p.vvp, p.amuv, nxt);
     for (int k = 0; k < p.nz; ++k) {
         const size_t off = (size_t)k * pl;
+        const LevelIn<R> cur = nxt;
+        if (k + 1 < p.nz) {
+            const size_t o1 = off + pl;
+            load_level(o, p.uu + o1, p.vv + o1, p.uup + o1, p.vvp + o1, p.amuv + o1, nxt);
+        } else {
+            load_level(o, p.uu2d, p.vv2d, p.uup2d, p.vvp2d, p.amuv2d, nxt);
+        }
+        double pgx[R], pgy[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const bool pg = p.with_pg && o.out[j];
+            pgx[j] = pg ? p.pgx[off + o.idx[j]] : 0.0;
+            pgy[j] = pg ? p.pgy[off + o.idx[j]] : 0.0;
+        }
         double radv[R], rdfx[R], rady[R], rdfy[R];
-        level_terms<R, NW>(p, o, L, k, p.uu + off, p.vv + off, p.uup + off, p.vvp + off, p.amuv + off, radv, rdfx,
-                           rady, rdfy);
+        level_terms<R, NW>(p, o, L, k, cur, radv, rdfx, rady, rdfy);
         const double dzk = p.dz[k];
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             double rx = rdfx[j], ry = rdfy[j];
-            if (p.with_pg && o.out[j]) {
-                rx = rx + p.pgx[off + o.idx[j]];
-                ry = ry + p.pgy[off + o.idx[j]];
+            if (p.with_pg) {
+                rx = rx + pgx[j];
+                ry = ry + pgy[j];
             }
             ax[j] = ax[j] + rx * dzk;
             ay[j] = ay[j] + ry * dzk;
@@ -408,7 +448,7 @@ __global__ __launch_bounds__(64 * NW) void k_forcing(F3Params p)
     }
     // the 2-D velocities (RHSx2d_tran_disp, RHSx2d_diff_disp)
     double txd[R], dfx[R], tyd[R], dfy[R];
-    level_terms<R, NW>(p, o, L, p.nz, p.uu2d, p.vv2d, p.uup2d, p.vvp2d, p.amuv2d, txd, dfx, tyd, dfy);
+    level_terms<R, NW>(p, o, L, p.nz, nxt, txd, dfx, tyd, dfy);
 
     // surface and bottom stresses (Function/fluxes.f90:29-37, :231-272) on
     // the bottom level, then the assembly (Control/dyncall.f90:107-126)
