@@ -62,7 +62,8 @@ TRAFFIC_JSON = os.path.join(ROOT, "profiles", "fused_step_cfg1_traffic.json")
 TRAFFIC_KERNELS = {"fp64": ("k_fused_step2<double, false, true>",),
                    "fp32": ("k_fused_step2<float, false, true>",),
                    "fp64_exact": ("k_fused_step<double, true, true>", "k_fused_step<double, true, false>"),
-                   "ensemble_fp32": ("k_fused_batch2<float, false, true>",)}
+                   "ensemble_fp32": ("k_fused_batch2<float, false, true>",),
+                   "forcing": ("k_forcing<",)}
 
 
 # kernels of one 2-D -> 3-D coupling and their launches per coupling

@@ -276,6 +276,19 @@ int inmsom_bt_forcing_3d(inmsom_bt_ctx *ctx, int nz, const double *dz, const dou
                          const double *hhq, const double *hhu, const double *hhv, const double *hhh,
                          const double *taux, const double *tauy, const double *slpr, const double *pgx,
                          const double *pgy, double *rhsx2d, double *rhsy2d);
+/* The same forcing on arrays already in device memory (device addresses on
+ * the context's GPU, same shapes and meanings; dz stays a host array of
+ * nz values): no host copies, for a caller whose 3-D fields live in HBM.
+ * rhsx2d, rhsy2d are updated in place.  Work runs on the library's
+ * stream, which does not wait for other streams: the arrays must be ready
+ * when the call is made; the call returns after the work has finished.
+ * Control/dyncall.f90:37-130 as above. */
+int inmsom_bt_forcing_3d_dev(inmsom_bt_ctx *ctx, int nz, const double *dz, const double *uu, const double *vv,
+                             const double *uup, const double *vvp, const double *amuv, const double *uu2d,
+                             const double *vv2d, const double *uup2d, const double *vvp2d, const double *amuv2d,
+                             const double *hhq, const double *hhu, const double *hhv, const double *hhh,
+                             const double *taux, const double *tauy, const double *slpr, const double *pgx,
+                             const double *pgy, double *rhsx2d, double *rhsy2d);
 /* Timing of its device work: nrep repetitions on the fields the last
  * inmsom_bt_forcing_3d call with nz levels left on the device (no host
  * copies); *ms = HIP-event time of all repetitions.  No counterpart in the

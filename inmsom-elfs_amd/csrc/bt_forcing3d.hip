@@ -49,6 +49,7 @@ struct F3Params {
     // (level nz: the 2-D velocities; level nz + 1: bf in slot 0)
     double *halo;
     int nslots;
+    int nbx, nby;  // tiles along x and y
 };
 
 __device__ __forceinline__ bool interior(const DevGrid &g, int m, int n)
@@ -169,7 +170,7 @@ template <int R>
 struct Own {
     int lane, w, m, t;  // t: thread index in the block
     int n[R], slot[R];
-    size_t idx[R];
+    unsigned idx[R];  // cell index m - 1 + (n - 1) nx (the array holds fewer than 2^29 cells)
     bool ok[R], out[R], in[R];
     uint8_t mbit[R];
     Pt s[R];
@@ -189,19 +190,27 @@ struct LevelIn {
     double u[R], v[R], up[R], vp[R], mu[R];
 };
 
+// a[i] as a scalar base plus a 32-bit byte offset (one address form for
+// every load of a level)
+__device__ __forceinline__ double ld(const double *a, unsigned i)
+{
+    return *(const double *)((const char *)a + i * 8u);
+}
+
 template <int R>
 __device__ __forceinline__ void load_level(const Own<R> &o, const double *U, const double *V, const double *UP,
                                            const double *VP, const double *MU, LevelIn<R> &in)
 {
+    // points outside the array (o.idx = 0) feed only halo points: their
+    // values are never used and need no mask
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-        const size_t i = o.idx[j];
-        const bool ok = o.ok[j];
-        in.u[j] = ok ? U[i] : 0.0;
-        in.v[j] = ok ? V[i] : 0.0;
-        in.up[j] = ok ? UP[i] : 0.0;
-        in.vp[j] = ok ? VP[i] : 0.0;
-        in.mu[j] = ok ? MU[i] : 0.0;
+        const unsigned i = o.idx[j];
+        in.u[j] = ld(U, i);
+        in.v[j] = ld(V, i);
+        in.up[j] = ld(UP, i);
+        in.vp[j] = ld(VP, i);
+        in.mu[j] = ld(MU, i);
     }
 }
 
@@ -228,6 +237,9 @@ __device__ __forceinline__ void level_terms(const F3Params &p, const Own<R> &o, 
                                             double (&rady)[R], double (&rdfy)[R])
 {
     const int lane = o.lane, w = o.w;
+    // neighbouring waves; the first and last wave hold halo rows, whose
+    // values from outside the tile are never used: they read their own
+    const int wd = w > 0 ? w - 1 : 0, wu = w < NW - 1 ? w + 1 : NW - 1;
     const double *u = in.u, *v = in.v, *mu = in.mu;
     double a[R], b[R], c[R], d[R], ev[R], fu[R], A[R], B[R];
 #pragma unroll
@@ -263,16 +275,15 @@ __device__ __forceinline__ void level_terms(const F3Params &p, const Own<R> &o, 
     for (int j = 0; j < R; ++j) {
         const Pt &s = o.s[j];
         const int t = o.t * R + j;
-        const bool up_ok = j < R - 1 || w < NW - 1, dn_ok = j > 0 || w > 0;
-        const double b_s = j > 0 ? b[j - 1] : (dn_ok ? L.e0[w - 1][7][lane] : 0.0);
-        const double mu_s = j > 0 ? mu[j - 1] : (dn_ok ? L.e0[w - 1][8][lane] : 0.0);
-        const double c_n = j < R - 1 ? c[j + 1] : (up_ok ? L.e0[w + 1][0][lane] : 0.0);
-        const double f_n = j < R - 1 ? fu[j + 1] : (up_ok ? L.e0[w + 1][1][lane] : 0.0);
-        const double u_n = j < R - 1 ? u[j + 1] : (up_ok ? L.e0[w + 1][2][lane] : 0.0);
-        const double A_n = j < R - 1 ? A[j + 1] : (up_ok ? L.e0[w + 1][3][lane] : 0.0);
-        const double B_n = j < R - 1 ? B[j + 1] : (up_ok ? L.e0[w + 1][4][lane] : 0.0);
-        const double v_n = j < R - 1 ? v[j + 1] : (up_ok ? L.e0[w + 1][5][lane] : 0.0);
-        const double mu_n = j < R - 1 ? mu[j + 1] : (up_ok ? L.e0[w + 1][6][lane] : 0.0);
+        const double b_s = j > 0 ? b[j - 1] : L.e0[wd][7][lane];
+        const double mu_s = j > 0 ? mu[j - 1] : L.e0[wd][8][lane];
+        const double c_n = j < R - 1 ? c[j + 1] : L.e0[wu][0][lane];
+        const double f_n = j < R - 1 ? fu[j + 1] : L.e0[wu][1][lane];
+        const double u_n = j < R - 1 ? u[j + 1] : L.e0[wu][2][lane];
+        const double A_n = j < R - 1 ? A[j + 1] : L.e0[wu][3][lane];
+        const double B_n = j < R - 1 ? B[j + 1] : L.e0[wu][4][lane];
+        const double v_n = j < R - 1 ? v[j + 1] : L.e0[wu][5][lane];
+        const double mu_n = j < R - 1 ? mu[j + 1] : L.e0[wu][6][lane];
         const double a_w = xl(a[j]), d_e = xr(d[j]), e_e = xr(ev[j]), v_e = xr(v[j]);
         const double A_e = xr(A[j]), u_e = xr(u[j]), B_e = xr(B[j]);
         const double mu_e = xr(mu[j]), mu_ne = xr(mu_n), mu_se = xr(mu_s), mu_w = xl(mu[j]), mu_nw = xl(mu_n);
@@ -321,12 +332,11 @@ __device__ __forceinline__ void level_terms(const F3Params &p, const Own<R> &o, 
 #pragma unroll
     for (int j = 0; j < R; ++j) {
         const Pt &s = o.s[j];
-        const bool up_ok = j < R - 1 || w < NW - 1, dn_ok = j > 0 || w > 0;
-        const double E_n = j < R - 1 ? E[j + 1] : (up_ok ? L.e1[w + 1][0][lane] : 0.0);
-        const double fyU_s = j > 0 ? FYU[j - 1] : (dn_ok ? L.e1[w - 1][1][lane] : 0.0);
-        const double fyV_s = j > 0 ? FYV[j - 1] : (dn_ok ? L.e1[w - 1][2][lane] : 0.0);
-        const double T1_s = j > 0 ? T1[j - 1] : (dn_ok ? L.e1[w - 1][3][lane] : 0.0);
-        const double ss_s = j > 0 ? ss[j - 1] : (dn_ok ? L.e1[w - 1][4][lane] : 0.0);
+        const double E_n = j < R - 1 ? E[j + 1] : L.e1[wu][0][lane];
+        const double fyU_s = j > 0 ? FYU[j - 1] : L.e1[wd][1][lane];
+        const double fyV_s = j > 0 ? FYV[j - 1] : L.e1[wd][2][lane];
+        const double T1_s = j > 0 ? T1[j - 1] : L.e1[wd][3][lane];
+        const double ss_s = j > 0 ? ss[j - 1] : L.e1[wd][4][lane];
         const double dxb2_s = j > 0 ? L.ks[KS_DXB2][o.t * R + j - 1] : o.dxb2_b;
         const double hh_s = j > 0 ? o.s[j - 1].hh : o.hh_b;
         const bool cu = o.in[j] && (o.mbit[j] & MB_LCU), cv = o.in[j] && (o.mbit[j] & MB_LCV);
@@ -341,6 +351,24 @@ __device__ __forceinline__ void level_terms(const F3Params &p, const Own<R> &o, 
     }
 }
 
+// Tile of a block.  Blocks are dealt to the eight XCDs in turn (block b to
+// XCD b % 8); each XCD takes a contiguous run of the tiles listed band by
+// band (bands TILE_BAND tiles wide, rows bottom to top within a band), so
+// the blocks one XCD runs at once cover a compact patch and the halo rows
+// and columns they share are read from HBM once into that XCD's L2.
+constexpr int TILE_BAND = 4;
+__device__ __forceinline__ void tile_of_block(int nbx, int nby, int b, int &bx, int &by)
+{
+    const int T = nbx * nby, x = b % NXCD, q = b / NXCD;
+    const int per = T / NXCD, extra = T % NXCD;  // XCD x runs per + (x < extra) blocks
+    const int pos = x * per + (x < extra ? x : extra) + q;
+    const int band = pos / (TILE_BAND * nby);
+    const int bw = min(TILE_BAND, nbx - band * TILE_BAND);
+    const int r = pos - band * TILE_BAND * nby;
+    by = r / bw;
+    bx = band * TILE_BAND + r % bw;
+}
+
 template <int R, int NW>
 __global__ __launch_bounds__(64 * NW) void k_forcing(F3Params p)
 {
@@ -353,10 +381,12 @@ __global__ __launch_bounds__(64 * NW) void k_forcing(F3Params p)
     o.lane = threadIdx.x & 63;
     o.w = threadIdx.x >> 6;
     const int lane = o.lane, w = o.w;
+    int bx, by;
+    tile_of_block(p.nbx, p.nby, (int)blockIdx.x, bx, by);
     // tile: columns 1 + 62 bx .. 64 + 62 bx, rows 1 + (R NW - 2) by .. ;
     // owned rows of this thread: n0 .. n0 + R - 1
-    const int m = 1 + (int)blockIdx.x * F3_COLS + lane;
-    const int n0 = 1 + (int)blockIdx.y * (R * NW - 2) + w * R;
+    const int m = 1 + bx * F3_COLS + lane;
+    const int n0 = 1 + by * (R * NW - 2) + w * R;
     o.m = m;
 #pragma unroll
     for (int j = 0; j < R; ++j) {
@@ -368,10 +398,10 @@ __global__ __launch_bounds__(64 * NW) void k_forcing(F3Params p)
                    n >= 2 && n <= ny - 1;
         o.in[j] = interior(g, m, n);
         o.slot[j] = o.ok[j] ? halo_slot(g, m, n) : -1;
-        o.idx[j] = o.ok[j] ? IDX(g, m, n) : 0;
-        const size_t i = o.idx[j];
+        o.idx[j] = o.ok[j] ? (unsigned)IDX(g, m, n) : 0u;
+        const unsigned i = o.idx[j];
         Pt &s = o.s[j];
-        if (o.ok[j]) {
+        {
             const double dx = g.dx[i], dy = g.dy[i], dxb = g.dxb[i], dyb = g.dyb[i];
             L.ks[KS_DYDX][t] = dy / dx;
             L.ks[KS_DXDY][t] = dx / dy;
@@ -391,12 +421,7 @@ __global__ __launch_bounds__(64 * NW) void k_forcing(F3Params p)
             s.hv = p.hhv[i];
             s.hh = p.hhh[i];
             s.hq = p.hhq[i];
-            o.mbit[j] = p.mb[i];
-        } else {
-#pragma unroll
-            for (int q = 0; q < KS_N; ++q) L.ks[q][t] = 1.0;
-            s = Pt{1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0};
-            o.mbit[j] = 0;
+            o.mbit[j] = o.ok[j] ? p.mb[i] : 0;
         }
     }
     // dxb^2 and hh of the point below row 0 (n0 - 1)
@@ -407,28 +432,20 @@ __global__ __launch_bounds__(64 * NW) void k_forcing(F3Params p)
     o.hh_b = w > 0 ? L.e1[w - 1][1][lane] : 1.0;
 
     // depth sums (depth_ave, Function/depth.f90:223-254) of RHS3d = uv_diff2
-    // + pressure gradients and of the transport terms, in level order; the
-    // next level's inputs are loaded while the current one is formed
+    // + pressure gradients and of the transport terms, in level order
     double ax[R], atx[R], ay[R], aty[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) ax[j] = atx[j] = ay[j] = aty[j] = 0.0;
-    LevelIn<R> nxt;
-    load_level(o, p.uu, p.vv, p.uup, p.vvp, p.amuv, nxt);
-    for (int k = 0; k < p.nz; ++k) {
+    // one level: its terms at the owned points added to the depth sums
+    auto level = [&](int k, const LevelIn<R> &cur) {
         const size_t off = (size_t)k * pl;
-        const LevelIn<R> cur = nxt;
-        if (k + 1 < p.nz) {
-            const size_t o1 = off + pl;
-            load_level(o, p.uu + o1, p.vv + o1, p.uup + o1, p.vvp + o1, p.amuv + o1, nxt);
-        } else {
-            load_level(o, p.uu2d, p.vv2d, p.uup2d, p.vvp2d, p.amuv2d, nxt);
-        }
         double pgx[R], pgy[R];
+        if (p.with_pg) {
 #pragma unroll
-        for (int j = 0; j < R; ++j) {
-            const bool pg = p.with_pg && o.out[j];
-            pgx[j] = pg ? p.pgx[off + o.idx[j]] : 0.0;
-            pgy[j] = pg ? p.pgy[off + o.idx[j]] : 0.0;
+            for (int j = 0; j < R; ++j) {
+                pgx[j] = ld(p.pgx + off, o.idx[j]);
+                pgy[j] = ld(p.pgy + off, o.idx[j]);
+            }
         }
         double radv[R], rdfx[R], rady[R], rdfy[R];
         level_terms<R, NW>(p, o, L, k, cur, radv, rdfx, rady, rdfy);
@@ -445,6 +462,23 @@ __global__ __launch_bounds__(64 * NW) void k_forcing(F3Params p)
             atx[j] = atx[j] + radv[j] * dzk;
             aty[j] = aty[j] + rady[j] * dzk;
         }
+    };
+    // loads of level k (k = nz: the 2-D velocities) into `in`
+    auto fetch = [&](int k, LevelIn<R> &in) {
+        if (k < p.nz) {
+            const size_t off = (size_t)k * pl;
+            load_level(o, p.uu + off, p.vv + off, p.uup + off, p.vvp + off, p.amuv + off, in);
+        } else {
+            load_level(o, p.uu2d, p.vv2d, p.uup2d, p.vvp2d, p.amuv2d, in);
+        }
+    };
+    // level k + 1 is loaded while level k is formed
+    LevelIn<R> nxt;
+    fetch(0, nxt);
+    for (int k = 0; k < p.nz; ++k) {
+        const LevelIn<R> cur = nxt;
+        fetch(k + 1, nxt);
+        level(k, cur);
     }
     // the 2-D velocities (RHSx2d_tran_disp, RHSx2d_diff_disp)
     double txd[R], dfx[R], tyd[R], dfy[R];
@@ -458,16 +492,16 @@ __global__ __launch_bounds__(64 * NW) void k_forcing(F3Params p)
 #pragma unroll
     for (int j = 0; j < R; ++j) {
         const size_t i = o.idx[j];
-        const bool ok = o.ok[j];
-        const double dx = ok ? g.dx[i] : 1.0, dy = ok ? g.dy[i] : 1.0;
+        const double dx = g.dx[i], dy = g.dy[i];
         dxr[j] = dx;
         dyr[j] = dy;
-        G[j] = (ok ? UB[i] : 0.0) * o.s[j].dxt * o.s[j].dyh * o.s[j].hu;
-        H[j] = (ok ? VB[i] : 0.0) * o.s[j].dxh * o.s[j].dyt * o.s[j].hv;
-        wx[j] = (ok ? p.taux[i] : 0.0) * dx * dy;
-        wy[j] = (ok ? p.tauy[i] : 0.0) * dx * dy;
-        sl[j] = ok ? p.slpr[i] : 0.0;
+        G[j] = UB[i] * o.s[j].dxt * o.s[j].dyh * o.s[j].hu;
+        H[j] = VB[i] * o.s[j].dxh * o.s[j].dyt * o.s[j].hv;
+        wx[j] = p.taux[i] * dx * dy;
+        wy[j] = p.tauy[i] * dx * dy;
+        sl[j] = p.slpr[i];
     }
+    const int wd = w > 0 ? w - 1 : 0, wu = w < NW - 1 ? w + 1 : NW - 1;
     __syncthreads();  // every wave is done with e0/e1 of the last level
     L.e0[w][0][lane] = H[R - 1];
     L.e0[w][1][lane] = wy[0];
@@ -476,7 +510,7 @@ __global__ __launch_bounds__(64 * NW) void k_forcing(F3Params p)
     double bf[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-        const double H_s = j > 0 ? H[j - 1] : (w > 0 ? L.e0[w - 1][0][lane] : 0.0);
+        const double H_s = j > 0 ? H[j - 1] : L.e0[wd][0][lane];
         const double G_w = xl(G[j]);
         const double hq = o.s[j].hq;
         const double ut = (G[j] + G_w) / 2.0 / dxr[j] / dyr[j] / hq;
@@ -488,10 +522,9 @@ __global__ __launch_bounds__(64 * NW) void k_forcing(F3Params p)
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-        const bool up_ok = j < R - 1 || w < NW - 1;
-        const double bf_n = j < R - 1 ? bf[j + 1] : (up_ok ? L.e1[w + 1][0][lane] : 0.0);
-        const double wy_n = j < R - 1 ? wy[j + 1] : (up_ok ? L.e0[w + 1][1][lane] : 0.0);
-        const double sl_n = j < R - 1 ? sl[j + 1] : (up_ok ? L.e0[w + 1][2][lane] : 0.0);
+        const double bf_n = j < R - 1 ? bf[j + 1] : L.e1[wu][0][lane];
+        const double wy_n = j < R - 1 ? wy[j + 1] : L.e0[wu][1][lane];
+        const double sl_n = j < R - 1 ? sl[j + 1] : L.e0[wu][2][lane];
         const double bf_e = xr(bf[j]), wx_e = xr(wx[j]), sl_e = xr(sl[j]);
         if (!o.out[j]) continue;
         const Pt &s = o.s[j];
@@ -627,6 +660,8 @@ F3Params staged_params(inmsom_bt_ctx *c, int nz, bool with_pg)
     p.ebottom = c->ebottom;
     p.halo = c->f3_halo;
     p.nslots = 2 * (c->nx + c->ny);
+    p.nbx = (c->nx - 2 + F3_COLS - 1) / F3_COLS;
+    p.nby = (c->ny - 2 + F3_ROWS - 3) / (F3_ROWS - 2);
     return p;
 }
 
@@ -635,15 +670,15 @@ void launch_forcing(inmsom_bt_ctx *c, const F3Params &p)
     if (c->g.px || c->g.py)
         hipLaunchKernelGGL(k_wrap_halo, dim3((unsigned)((p.nslots + 255) / 256), (unsigned)(p.nz + 2)), dim3(256), 0,
                            c->stream, p);
-    const dim3 grid((unsigned)((c->nx - 2 + F3_COLS - 1) / F3_COLS),
-                    (unsigned)((c->ny - 2 + F3_ROWS - 3) / (F3_ROWS - 2)));
-    hipLaunchKernelGGL((k_forcing<F3_R, F3_NW>), grid, dim3(64 * F3_NW), 0, c->stream, p);
+    hipLaunchKernelGGL((k_forcing<F3_R, F3_NW>), dim3((unsigned)(p.nbx * p.nby)), dim3(64 * F3_NW), 0, c->stream,
+                       p);
 }
 
 int check_args(inmsom_bt_ctx *c, int nz)
 {
     if (nz < 1 || nz > 1000) return ffail("nz must be in 1..1000", INMSOM_BT_EINVAL);
     if (c->nx < 5 || c->ny < 5) return ffail("the forcing needs nx, ny >= 5", INMSOM_BT_EINVAL);
+    if (c->n >= ((size_t)1 << 29)) return ffail("the forcing needs nx * ny < 2^29", INMSOM_BT_EINVAL);
     return INMSOM_BT_OK;
 }
 
@@ -688,6 +723,62 @@ int inmsom_bt_forcing_3d(inmsom_bt_ctx *c, int nz, const double *dz, const doubl
     CHF(hipStreamSynchronize(c->stream));
     c->f3_nz = nz;
     c->f3_pg = pgx != nullptr;
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_forcing_3d_dev(inmsom_bt_ctx *c, int nz, const double *dz, const double *uu, const double *vv,
+                             const double *uup, const double *vvp, const double *amuv, const double *uu2d,
+                             const double *vv2d, const double *uup2d, const double *vvp2d, const double *amuv2d,
+                             const double *hhq, const double *hhu, const double *hhv, const double *hhh,
+                             const double *taux, const double *tauy, const double *slpr, const double *pgx,
+                             const double *pgy, double *rhsx2d, double *rhsy2d)
+{
+    if (!c || !dz || !uu || !vv || !uup || !vvp || !amuv || !uu2d || !vv2d || !uup2d || !vvp2d || !amuv2d ||
+        !hhq || !hhu || !hhv || !hhh || !taux || !tauy || !slpr || !rhsx2d || !rhsy2d)
+        return ffail("null pointer", INMSOM_BT_EINVAL);
+    if ((pgx == nullptr) != (pgy == nullptr)) return ffail("pgx and pgy: both or neither", INMSOM_BT_EINVAL);
+    TRYF(check_args(c, nz));
+    CHF(hipSetDevice(c->device));
+    TRYF(ensure_masks(c));
+    TRYF(ensure_halo(c, nz));
+    if (!c->f3_dz) CHF(hipMalloc(&c->f3_dz, 1000 * sizeof(double)));
+    CHF(hipMemcpyAsync(c->f3_dz, dz, nz * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    F3Params p{};
+    p.g = c->g;
+    p.nz = nz;
+    p.with_pg = pgx != nullptr;
+    p.dz = c->f3_dz;
+    p.uu = uu;
+    p.vv = vv;
+    p.uup = uup;
+    p.vvp = vvp;
+    p.amuv = amuv;
+    p.pgx = pgx;
+    p.pgy = pgy;
+    p.uu2d = uu2d;
+    p.vv2d = vv2d;
+    p.uup2d = uup2d;
+    p.vvp2d = vvp2d;
+    p.amuv2d = amuv2d;
+    p.hhq = hhq;
+    p.hhu = hhu;
+    p.hhv = hhv;
+    p.hhh = hhh;
+    p.taux = taux;
+    p.tauy = tauy;
+    p.slpr = slpr;
+    p.rhsx_in = p.rhsx = rhsx2d;
+    p.rhsy_in = p.rhsy = rhsy2d;
+    p.mb = c->f3_mb;
+    p.cb_nl = c->cb_nl;
+    p.ebottom = c->ebottom;
+    p.halo = c->f3_halo;
+    p.nslots = 2 * (c->nx + c->ny);
+    p.nbx = (c->nx - 2 + F3_COLS - 1) / F3_COLS;
+    p.nby = (c->ny - 2 + F3_ROWS - 3) / (F3_ROWS - 2);
+    launch_forcing(c, p);
+    CHF(hipGetLastError());
+    CHF(hipStreamSynchronize(c->stream));
     return INMSOM_BT_OK;
 }
 

@@ -96,6 +96,7 @@ struct inmsom_bt_ctx {
     bool f3_pg = false;            // staged pressure-gradient terms
     uint8_t *f3_mb = nullptr;      // mask bits of the forcing (lu, luu, lcu, lcv, llu, llv)
     double *f3_halo = nullptr;     // periodic halo table of the forcing
+    double *f3_dz = nullptr;       // dz of the device-address forcing (up to 1000 levels)
     size_t f3_halo_n = 0;          // doubles in f3_halo
     long long *cnt = nullptr;
 };

@@ -5,7 +5,8 @@
 ! with the range check after every step and reports the same timer the
 ! reference prints (time_barotrop).
 !
-! usage: inmsom_bt_host <par dir> <case.bin> <out.bin> [literal|fp64_exact|fp64|fp32] [g] [coupling.bin]
+! usage: inmsom_bt_host <par dir> <case.bin> <out.bin> [literal|fp64_exact|fp64|fp32] [g] [coupling.bin|-]
+!                       [forcing.bin]
 !
 !   ocean_run.par  line 2 time_step [s], line 3 run_duration [days],
 !                  line 9 nstep_barotrop                 (inmsom_head.f90:47-55)
@@ -20,7 +21,13 @@
 !                  the 2-D -> 3-D coupling (Control/dyncall.f90:169-231) with
 !                  the step's ubrtr_i, vbrtr_i, pgrx, pgry and wflux, and
 !                  appends uu, vv, uu2d, vv2d, pgrx, pgry, ssh_i, sshp_i,
-!                  hhqn, hhun, hhvn, hhhn to out.bin.
+!                  hhqn, hhun, hhvn, hhhn to out.bin ('-': no coupling).
+!   forcing.bin    optional 3-D model fields: int32 nz, real8 dz(nz), then
+!                  uu, vv, uup, vvp, amuv, pgx, pgy (nx x ny x nz) and uu2d,
+!                  vv2d, uup2d, vvp2d, amuv2d, hhq, hhu, hhv, hhh, slpr,
+!                  RHSx2d, RHSy2d (nx x ny).  The host forms the forcing of
+!                  the barotropic step (Control/dyncall.f90:37-130) with the
+!                  case's wind stress and appends RHSx2d, RHSy2d to out.bin.
 !
 ! Mode 'literal' runs the stage-by-stage device path (any metrics, the
 ! biharmonic term); the fused modes need uniform metrics.
@@ -29,7 +36,7 @@ use iso_c_binding
 use inmsom_bt
 implicit none
 
-character(512) :: pardir, case_path, out_path, arg, cpl_path
+character(512) :: pardir, case_path, out_path, arg, cpl_path, frc_path
 character(256) :: lines(64)
 character(16) :: mode_name
 integer :: nl, ierr, magic, version, nx, ny, nsteps_case, nstep_case, ksw4_case, cmode, nout
@@ -41,6 +48,8 @@ real(8), allocatable, target :: st(:,:,:)
 real(8), allocatable, target :: rdis(:,:)
 ! coupling: c2 = ubrtr_i vbrtr_i pgrx pgry hhu hhv slpr uu2d vv2d ssh_i sshp_i hhqn hhun hhvn hhhn
 real(8), allocatable, target :: c2(:,:,:), uu(:,:,:), vv(:,:,:)
+! forcing: f3 = uu vv uup vvp amuv pgx pgy; f2 = uu2d vv2d uup2d vvp2d amuv2d hhq hhu hhv hhh slpr RHSx2d RHSy2d
+real(8), allocatable, target :: f3(:,:,:,:), f2(:,:,:), dzv(:)
 integer(c_long_long) :: bad
 integer(8) :: c0, c1, crate
 real(8) :: time_barotrop, vol0, vol1
@@ -155,8 +164,15 @@ open(23, file=trim(out_path), access='stream', form='unformatted', status='repla
 write(23) st
 if (command_argument_count() >= 6) then
   call get_command_argument(6, cpl_path)
-  call coupling_step(trim(cpl_path))
-  write(23) uu, vv, c2(:,:,8:9), c2(:,:,3:4), c2(:,:,10:15)
+  if (trim(cpl_path) /= '-') then
+    call coupling_step(trim(cpl_path))
+    write(23) uu, vv, c2(:,:,8:9), c2(:,:,3:4), c2(:,:,10:15)
+  endif
+endif
+if (command_argument_count() >= 7) then
+  call get_command_argument(7, frc_path)
+  call forcing_step(trim(frc_path))
+  write(23) f2(:,:,11:12)
 endif
 close(23)
 write(*,'(a,es12.5)') ' Time barotropic: ', time_barotrop
@@ -190,6 +206,28 @@ subroutine coupling_step(path)
                            c2(:,:,13), c2(:,:,14), c2(:,:,15), uu, vv)
   write(*,'(a,i0,a)') ' coupling step done (', nz, ' levels)'
 end subroutine coupling_step
+
+! The forcing of the barotropic step from the 3-D model, Control/
+! dyncall.f90:37-130, on the fields of `path` and the case's wind stress.
+subroutine forcing_step(path)
+  character(*), intent(in) :: path
+  integer :: io, nzf
+  open(25, file=path, access='stream', form='unformatted', status='old', iostat=io)
+  if (io /= 0) then
+    write(*,*) 'cannot open ', path
+    error stop 5
+  endif
+  read(25) nzf
+  allocate(dzv(nzf), f3(nx, ny, nzf, 7), f2(nx, ny, 12))
+  read(25) dzv
+  read(25) f3
+  read(25) f2
+  close(25)
+  call forcing_3d_gpu(nzf, dzv, f3(:,:,:,1), f3(:,:,:,2), f3(:,:,:,3), f3(:,:,:,4), f3(:,:,:,5), f2(:,:,1), &
+                      f2(:,:,2), f2(:,:,3), f2(:,:,4), f2(:,:,5), f2(:,:,6), f2(:,:,7), f2(:,:,8), f2(:,:,9), &
+                      f(:,:,11), f(:,:,12), f2(:,:,10), f2(:,:,11), f2(:,:,12), f3(:,:,:,6), f3(:,:,:,7))
+  write(*,'(a,i0,a)') ' forcing step done (', nzf, ' levels)'
+end subroutine forcing_step
 
 ! Service/read_write_parameters.f90:2-41: the file's lines, one per entry.
 subroutine read_lines(path, buf, n)

@@ -196,6 +196,31 @@ interface
     integer(c_int), value :: nz, nrep
     real(c_double), intent(out) :: ms
   end function
+  ! forcing of the barotropic step from the 3-D model (Control/dyncall.f90:37-130)
+  integer(c_int) function inmsom_bt_forcing_3d(ctx, nz, dz, uu, vv, uup, vvp, amuv, uu2d, vv2d, uup2d, &
+      vvp2d, amuv2d, hhq, hhu, hhv, hhh, taux, tauy, slpr, pgx, pgy, rhsx2d, rhsy2d) &
+      bind(C, name="inmsom_bt_forcing_3d")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: nz
+    type(c_ptr), value :: dz, uu, vv, uup, vvp, amuv, uu2d, vv2d, uup2d, vvp2d, amuv2d, hhq, hhu, hhv, hhh
+    type(c_ptr), value :: taux, tauy, slpr, pgx, pgy, rhsx2d, rhsy2d
+  end function
+  integer(c_int) function inmsom_bt_forcing_3d_dev(ctx, nz, dz, uu, vv, uup, vvp, amuv, uu2d, vv2d, uup2d, &
+      vvp2d, amuv2d, hhq, hhu, hhv, hhh, taux, tauy, slpr, pgx, pgy, rhsx2d, rhsy2d) &
+      bind(C, name="inmsom_bt_forcing_3d_dev")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: nz
+    type(c_ptr), value :: dz, uu, vv, uup, vvp, amuv, uu2d, vv2d, uup2d, vvp2d, amuv2d, hhq, hhu, hhv, hhh
+    type(c_ptr), value :: taux, tauy, slpr, pgx, pgy, rhsx2d, rhsy2d
+  end function
+  integer(c_int) function inmsom_bt_forcing_3d_timed(ctx, nz, nrep, ms) bind(C, name="inmsom_bt_forcing_3d_timed")
+    import :: c_int, c_double, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: nz, nrep
+    real(c_double), intent(out) :: ms
+  end function
   function c_strlen(s) bind(C, name="strlen")
     import :: c_ptr, c_size_t
     type(c_ptr), value :: s
@@ -273,5 +298,30 @@ subroutine internal_update_gpu(tau, nz, ubrtr_i, vbrtr_i, hhu, hhv, slpr, wf_tot
                 c_loc(pgrx), c_loc(pgry), c_loc(ssh_i), c_loc(sshp_i), hhn, c_loc(uu), c_loc(vv)), &
                 'internal_update_gpu')
 end subroutine internal_update_gpu
+
+! Control/dyncall.f90:37-130 (ksw_vert = ksw_lat = ksw_lat4 = 0) on the
+! context bt_ctx: RHSx2d, RHSy2d from the 3-D model's level-n velocities
+! uu, vv, level n-1 uup, vvp, viscosity amuv (nz levels of thickness dz),
+! the 2-D velocities and viscosity, the depths hhq .. hhh, wind stress,
+! sea-level pressure and, when given, the terms pgx, pgy that
+! pressure_gradients adds to the 3-D RHS.
+subroutine forcing_3d_gpu(nz, dz, uu, vv, uup, vvp, amuv, uu2d, vv2d, uup2d, vvp2d, amuv2d, hhq, hhu, hhv, &
+                          hhh, taux, tauy, slpr, RHSx2d, RHSy2d, pgx, pgy)
+  integer, intent(in) :: nz
+  real(8), target, intent(in) :: dz(*), uu(*), vv(*), uup(*), vvp(*), amuv(*)
+  real(8), target, intent(in) :: uu2d(*), vv2d(*), uup2d(*), vvp2d(*), amuv2d(*), hhq(*), hhu(*), hhv(*)
+  real(8), target, intent(in) :: hhh(*), taux(*), tauy(*), slpr(*)
+  real(8), target, intent(inout) :: RHSx2d(*), RHSy2d(*)
+  real(8), target, intent(in), optional :: pgx(*), pgy(*)
+  type(c_ptr) :: px, py
+  px = c_null_ptr
+  py = c_null_ptr
+  if (present(pgx)) px = c_loc(pgx)
+  if (present(pgy)) py = c_loc(pgy)
+  call bt_check(inmsom_bt_forcing_3d(bt_ctx, int(nz, c_int), c_loc(dz), c_loc(uu), c_loc(vv), c_loc(uup), &
+                c_loc(vvp), c_loc(amuv), c_loc(uu2d), c_loc(vv2d), c_loc(uup2d), c_loc(vvp2d), c_loc(amuv2d), &
+                c_loc(hhq), c_loc(hhu), c_loc(hhv), c_loc(hhh), c_loc(taux), c_loc(tauy), c_loc(slpr), px, py, &
+                c_loc(RHSx2d), c_loc(RHSy2d)), 'forcing_3d_gpu')
+end subroutine forcing_3d_gpu
 
 end module inmsom_bt

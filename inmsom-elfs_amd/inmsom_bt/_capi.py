@@ -70,6 +70,7 @@ SIGNATURES = {
                                       C.c_int),
     "inmsom_bt_coupling_timed": ([C.c_void_p, C.c_double, C.c_int, C.c_int, _dp], C.c_int),
     "inmsom_bt_forcing_3d": ([C.c_void_p, C.c_int] + [_dp] * 22, C.c_int),
+    "inmsom_bt_forcing_3d_dev": ([C.c_void_p, C.c_int] + [_dp] * 22, C.c_int),
     "inmsom_bt_forcing_3d_timed": ([C.c_void_p, C.c_int, C.c_int, _dp], C.c_int),
     "inmsom_bt_ens_create": ([C.c_int, C.POINTER(Grid), C.POINTER(Forcing), C.c_int, C.c_int,
                               C.POINTER(C.c_void_p)], C.c_int),

@@ -302,6 +302,20 @@ class BarotropicModel:
             "inmsom_bt_forcing_3d")
         return out
 
+    def forcing_3d_dev(self, fields: dict, dz, rhsx2d, rhsy2d, pgx=None, pgy=None):
+        """As forcing_3d on DeviceArray arguments (no host copies; dz a host
+        sequence): rhsx2d, rhsy2d are updated in place."""
+        dzc = np.ascontiguousarray(dz, dtype=np.float64)
+        nz = dzc.shape[0]
+        s3, s2 = (nz, self.ny, self.nx), (self.ny, self.nx)
+        if (pgx is None) != (pgy is None):
+            raise ValueError("pgx and pgy: both or neither")
+        check(lib().inmsom_bt_forcing_3d_dev(
+            self._h, nz, _d(dzc), *[self._dev(fields[k], s3) for k in self.FORCING_3D],
+            *[self._dev(fields[k], s2) for k in self.FORCING_2D],
+            *[(self._dev(q, s3) if q is not None else None) for q in (pgx, pgy)],
+            self._dev(rhsx2d, s2), self._dev(rhsy2d, s2)), "inmsom_bt_forcing_3d_dev")
+
     def forcing_3d_timed(self, nz: int, nrep: int) -> float:
         """HIP-event time (ms) of nrep repetitions of the forcing's device
         work on the fields the last forcing_3d call left on the device."""

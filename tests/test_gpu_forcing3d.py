@@ -53,3 +53,49 @@ def test_forcing_3d_matches_oracle(nz, periodicity, metrics, pg):
     assert_bitwise(got, want, ("rhsx2d", "rhsy2d"), f"forcing 3-D nz={nz} {periodicity} {metrics}")
     assert_bitwise(got1, want1, ("rhsx2d", "rhsy2d"), "forcing 3-D, second call nz=1")
     assert np.abs(got["rhsx2d"] - 7.0).max() > 0.0 and (got["rhsx2d"] == 7.0).any()
+
+
+@pytest.mark.parametrize("periodicity,pg", [((0, 0), True), ((1, 1), False)])
+def test_forcing_3d_device_arrays(periodicity, pg):
+    """The device-address entry on HBM arrays: rhsx2d, rhsy2d updated in place,
+    bit for bit the oracle's."""
+    from inmsom_bt.hipmem import DeviceArray
+    print(f"forcing 3-D device arrays periodicity={periodicity} pg={pg}", flush=True)
+    nz = 7
+    b = randomize_state(make_basin(66, 58, seed=5, sigma=4.0, metrics="stretched"), 4)
+    f = _fields(b, nz, seed=3)
+    og = oracle.OracleGrid(b, periodicity=periodicity)
+    want = oracle.forcing_3d(og, f, b.cb_nl, b.ebottom, pressure=pg)
+    with BarotropicModel(b, periodicity=periodicity) as m:
+        host = dict(f, taux=b.taux, tauy=b.tauy)
+        keys = m.FORCING_3D + m.FORCING_2D
+        dev = {k: DeviceArray.from_host(np.ascontiguousarray(host[k], dtype=np.float64), m.device) for k in keys}
+        rx = DeviceArray.from_host(f["rhsx2d"], m.device)
+        ry = DeviceArray.from_host(f["rhsy2d"], m.device)
+        pgd = [DeviceArray.from_host(f[k], m.device) for k in ("pgx", "pgy")] if pg else [None, None]
+        try:
+            m.forcing_3d_dev(dev, f["dz"], rx, ry, *pgd)
+            got = {"rhsx2d": rx.to_host(), "rhsy2d": ry.to_host()}
+        finally:
+            for a in list(dev.values()) + [rx, ry] + [q for q in pgd if q is not None]:
+                a.free()
+    assert_bitwise(got, want, ("rhsx2d", "rhsy2d"), f"device-array forcing {periodicity}")
+
+
+def test_forcing_3d_argument_errors():
+    from inmsom_bt._capi import BtError, ESTATE, EINVAL
+    b = make_basin(30, 24, seed=2)
+    f = _fields(b, 3, seed=1)
+    with BarotropicModel(b) as m:
+        with pytest.raises(BtError) as e:
+            m.forcing_3d_timed(3, 1)            # nothing staged yet
+        assert e.value.code == ESTATE
+        fld = dict(f, taux=b.taux, tauy=b.tauy)
+        with pytest.raises(BtError) as e:
+            m.forcing_3d(fld, f["dz"], f["rhsx2d"], f["rhsy2d"], f["pgx"], None)
+        assert e.value.code == EINVAL
+        m.forcing_3d(fld, f["dz"], f["rhsx2d"], f["rhsy2d"])
+        with pytest.raises(BtError) as e:
+            m.forcing_3d_timed(4, 1)            # staged with 3 levels
+        assert e.value.code == ESTATE
+        assert m.forcing_3d_timed(3, 1) > 0.0

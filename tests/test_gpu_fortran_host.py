@@ -104,3 +104,53 @@ def test_fortran_host_coupling_bitwise():
     got.update(zip(("uu2d", "vv2d", "pgrx", "pgry", "ssh_i", "sshp_i", "hhqn", "hhun", "hhvn", "hhhn"), planes))
     assert "coupling step done (7 levels)" in p.stdout
     assert_bitwise(got, want, tuple(got), "fortran host coupling")
+
+
+def test_fortran_host_forcing_bitwise():
+    """The host's forcing step (forcing_3d_gpu through the ISO_C_BINDING
+    module, Control/dyncall.f90:37-130) with the case's wind stress and
+    drag, against the oracle, bit for bit."""
+    b = randomize_state(make_basin(72, 50, seed=31, sigma=5.0, metrics="stretched"), 4)
+    rd = oracle.OracleGrid(b).rayleigh()
+    nz = 6
+    rng = np.random.default_rng(23)
+    shape = (b.ny, b.nx)
+    og = oracle.OracleGrid(b)
+    lcu = (og.masks["lcu"] > 0.5).astype(np.float64)
+    lcv = (og.masks["lcv"] > 0.5).astype(np.float64)
+    inp = {"dz": (np.arange(nz) + 1.0) / (nz * (nz + 1) / 2.0)}
+    for k, msk in (("uu", lcu), ("vv", lcv), ("uup", lcu), ("vvp", lcv)):
+        inp[k] = 0.1 * rng.standard_normal((nz,) + shape) * msk
+    inp["amuv"] = 10.0 + np.abs(rng.standard_normal((nz,) + shape))
+    inp["pgx"] = 1e-3 * rng.standard_normal((nz,) + shape)
+    inp["pgy"] = 1e-3 * rng.standard_normal((nz,) + shape)
+    for k, msk in (("uu2d", lcu), ("vv2d", lcv), ("uup2d", lcu), ("vvp2d", lcv)):
+        inp[k] = 0.1 * rng.standard_normal(shape) * msk
+    inp["amuv2d"] = 10.0 + np.abs(rng.standard_normal(shape))
+    for k in ("hhq", "hhu", "hhv", "hhh"):
+        inp[k] = 50.0 + np.abs(rng.standard_normal(shape))
+    inp["slpr"] = 1.0e5 + 100.0 * rng.standard_normal(shape)
+    inp["rhsx2d"], inp["rhsy2d"] = np.full(shape, 2.0), np.full(shape, -1.0)
+    want = oracle.forcing_3d(og, inp, b.cb_nl, b.ebottom)
+    with tempfile.TemporaryDirectory() as tmp:
+        case = os.path.join(tmp, "case.bin")
+        frc = os.path.join(tmp, "forcing.bin")
+        out = os.path.join(tmp, "out.bin")
+        write_case(case, b, rd, 1, 2, 0, 0, [])
+        with open(frc, "wb") as fh:
+            np.array([nz], dtype="<i4").tofile(fh)
+            inp["dz"].astype("<f8").tofile(fh)
+            for k in ("uu", "vv", "uup", "vvp", "amuv", "pgx", "pgy"):
+                np.ascontiguousarray(inp[k], dtype="<f8").tofile(fh)
+            for k in ("uu2d", "vv2d", "uup2d", "vvp2d", "amuv2d", "hhq", "hhu", "hhv", "hhh", "slpr", "rhsx2d",
+                      "rhsy2d"):
+                np.ascontiguousarray(inp[k], dtype="<f8").tofile(fh)
+        p = subprocess.run([HOST, PARDIR, case, out, "literal", repr(b.g), "-", frc], capture_output=True,
+                           text=True, timeout=120)
+        print(p.stdout, p.stderr)
+        assert p.returncode == 0, p.stdout + p.stderr
+        raw = np.fromfile(out, dtype="<f8")
+    per = b.nx * b.ny
+    got = dict(zip(("rhsx2d", "rhsy2d"), raw[6 * per:].reshape(2, b.ny, b.nx)))
+    assert "forcing step done (6 levels)" in p.stdout
+    assert_bitwise(got, want, ("rhsx2d", "rhsy2d"), "fortran host forcing")
