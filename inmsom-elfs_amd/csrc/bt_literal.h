@@ -19,14 +19,5 @@ void lit_forcing(inmsom_bt_ctx *c);
 // (Function/depth.f90:108-170) of (hqn, hun, hvn, hhn) from the surface sh
 void lit_cyclize(inmsom_bt_ctx *c, double *f);
 void lit_hh_update(inmsom_bt_ctx *c, const double *sh, double *hqn, double *hun, double *hvn, double *hhn);
-// one level of stress_components, uv_trans + uv_diff2 (vorticity into
-// vort), wind stress on U/V faces and the quadratic bottom friction
-void lit_stress(inmsom_bt_ctx *c, const double *u, const double *v, double *str_t, double *str_s);
-void lit_trans_diff2(inmsom_bt_ctx *c, const double *u, const double *v, double *vort, const double *hq,
-                     const double *hu, const double *hv, const double *hh, const double *mu, const double *str_t,
-                     const double *str_s, double *radv, double *rady, double *rdfx, double *rdfy);
-void lit_surface_stress(inmsom_bt_ctx *c, const double *taux, const double *tauy, double *sx, double *sy);
-void lit_bottom_friction(inmsom_bt_ctx *c, const double *uu, const double *vv, const double *hhu,
-                         const double *hhv, const double *hhq, double *bf);
 
 }  // namespace ibt

@@ -488,40 +488,6 @@ void lit_hh_update(inmsom_bt_ctx *c, const double *sh, double *hqn, double *hun,
     cyclize(c, hhn);
 }
 
-// One level of the helper routines, for callers that loop over levels:
-// stress_components (Function/mixing.f90:115-168), uv_trans with uv_diff2
-// (Function/vel_ssh.f90:2-184, vorticity into `vort`), and the surface
-// and bottom fluxes (Function/fluxes.f90:29-37, 226-272).
-void lit_stress(inmsom_bt_ctx *c, const double *u, const double *v, double *str_t, double *str_s)
-{
-    hipLaunchKernelGGL(k_stress, grid2(c->g), BLK2, 0, c->stream, c->g, u, v, str_t, str_s);
-    cyclize(c, str_t);
-    cyclize(c, str_s);
-}
-
-void lit_trans_diff2(inmsom_bt_ctx *c, const double *u, const double *v, double *vort, const double *hq,
-                     const double *hu, const double *hv, const double *hh, const double *mu, const double *str_t,
-                     const double *str_s, double *radv, double *rady, double *rdfx, double *rdfy)
-{
-    hipLaunchKernelGGL(k_vort, grid2(c->g), BLK2, 0, c->stream, c->g, u, v, vort);
-    cyclize(c, vort);
-    hipLaunchKernelGGL(k_trans_diff2, grid2(c->g), BLK2, 0, c->stream, c->g, u, v, vort, hq, hu, hv, hh, mu,
-                       str_t, str_s, radv, rady, rdfx, rdfy);
-}
-
-void lit_surface_stress(inmsom_bt_ctx *c, const double *taux, const double *tauy, double *sx, double *sy)
-{
-    hipLaunchKernelGGL(k_surface_stress, grid2(c->g), BLK2, 0, c->stream, c->g, taux, tauy, sx, sy);
-}
-
-void lit_bottom_friction(inmsom_bt_ctx *c, const double *uu, const double *vv, const double *hhu,
-                         const double *hhv, const double *hhq, double *bf)
-{
-    hipLaunchKernelGGL(k_bottom_friction, grid2(c->g), BLK2, 0, c->stream, c->g, c->cb_nl, c->ebottom, uu, vv,
-                       hhu, hhv, hhq, bf);
-    cyclize(c, bf);
-}
-
 // Device array of the 12 depth pointers (k_hh_init and k_shift index
 // it); written once when the context is created.
 static inline double *const *dep_table(inmsom_bt_ctx *c)
