@@ -295,6 +295,51 @@ int inmsom_bt_forcing_3d_dev(inmsom_bt_ctx *ctx, int nz, const double *dz, const
  * reference (a measurement entry beside inmsom_bt_run_timed). */
 int inmsom_bt_forcing_3d_timed(inmsom_bt_ctx *ctx, int nz, int nrep, double *ms);
 
+/* DIOGIN direct-access records (Service/input_output_data.f90): field
+ * (nx, ny, nz) column-major (first index fastest, as the reference's
+ * arrays), lu (nx, ny) the ocean mask; the subdomain [nxb, nxe] x [nyb, nye]
+ * of levels nzb..nze goes to records (nfield-1)*(nze-nzb+1)+1 .. of the
+ * file path/fname (joined as fulfname does, :4-114).  Records hold float32
+ * (wdstd/rdstd, :117-327) or float64 (wdstd8/rdstd8, :329-533) values with
+ * no record markers.  The writer stores land points (|lu| < 0.5) as
+ * undef = -1e32 and leaves them 0 in field; the reader sets them to 0.
+ * The file is created if missing and never truncated (status='unknown').
+ * Errors return INMSOM_BT_EINVAL (the reference stops the run). */
+int inmsom_bt_wdstd(const char *path, const char *fname, int nfield, float *field, const float *lu, int nx, int ny,
+                    int nz, int nxb, int nxe, int nyb, int nye, int nzb, int nze);
+int inmsom_bt_rdstd(const char *path, const char *fname, int nfield, float *field, const float *lu, int nx, int ny,
+                    int nz, int nxb, int nxe, int nyb, int nye, int nzb, int nze);
+int inmsom_bt_wdstd8(const char *path, const char *fname, int nfield, double *field, const float *lu, int nx, int ny,
+                     int nz, int nxb, int nxe, int nyb, int nye, int nzb, int nze);
+int inmsom_bt_rdstd8(const char *path, const char *fname, int nfield, double *field, const float *lu, int nx, int ny,
+                     int nz, int nxb, int nxe, int nyb, int nye, int nzb, int nze);
+/* GrADS descriptor of a DIOGIN file, ctl_file_write
+ * (Service/rw_ctl_file.f90:1-138): writes fname with its extension
+ * replaced by .ctl, naming fname's .dat in DSET; same arguments and text
+ * as the reference's writer (x0/y0/z0: first value and step hx/hy/hz when
+ * the type is 0, all values when 1; ht the time step in seconds, >= 60;
+ * yr_type 0 adds OPTIONS 365_DAY_CALENDAR). */
+int inmsom_bt_ctl_write(const char *fname, float undef, int nx, int ny, int nz, int nt, int xtype, const double *x0,
+                        double hx, int ytype, const double *y0, double hy, int ztype, const double *z0, double hz,
+                        int yr_type, int year0, int month0, int day0, int hour0, int minute0, float ht,
+                        const char *title, const char *varname);
+
+/* Snapshot of the external-mode state in the reference's output formats
+ * (Control/output.f90:115-154, 155-190, 230-260): path/ssh.dat (lu,
+ * T points m = 3..nx-2, n = 3..ny-2), path/ubrtr.dat (llu, m = 2..nx-2)
+ * and path/vbrtr.dat (llv, n = 2..ny-2), float32 DIOGIN records (land =
+ * undef -1e32), record nrec of each, and their GrADS descriptors (.ctl,
+ * as ctl_file_write: t-dimension nrec, linear axes from the first T
+ * point xt0, yt0 with steps dxst, dyst, the velocity axes shifted half a
+ * step, calendar/year/month/day/hour/minute of the first record, tstep
+ * seconds between records).  The fields are packed on the device and
+ * copied once into pinned memory; *ms (if not NULL) = HIP-event time of
+ * the packing and that copy.  No counterpart as one routine in the
+ * reference (local_output writes the 3-D model's fields this way). */
+int inmsom_bt_write_snapshot(inmsom_bt_ctx *ctx, const char *path, int nrec, double xt0, double dxst, double yt0,
+                             double dyst, int calendar, int year, int month, int day, int hour, int minute,
+                             float tstep, double *ms);
+
 /* Timing of the coupling's device work: nrep repetitions of the kernels of
  * inmsom_bt_remove_barotropic then inmsom_bt_internal_update on the fields
  * the last inmsom_bt_internal_update call left on the device (it must have

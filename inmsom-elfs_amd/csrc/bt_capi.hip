@@ -329,8 +329,10 @@ int inmsom_bt_finalize(inmsom_bt_ctx *c)
                     (void *)c->rdfy, (void *)c->rhsx, (void *)c->rhsy, (void *)c->bf,
                     (void *)c->surf_x, (void *)c->surf_y, (void *)c->bot_x, (void *)c->bot_y,
                     (void *)c->scal, (void *)c->cnt, (void *)c->cpl, (void *)c->cpl_new, (void *)c->f3,
-                    (void *)c->f3_mb, (void *)c->f3_halo, (void *)c->f3_dz})
+                    (void *)c->f3_mb, (void *)c->f3_halo, (void *)c->f3_dz,
+                    (void *)c->snap_dev})
         fr(p);
+    if (c->snap_host) hipHostFree(c->snap_host);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return INMSOM_BT_OK;

@@ -97,6 +97,9 @@ struct inmsom_bt_ctx {
     uint8_t *f3_mb = nullptr;      // mask bits of the forcing (lu, luu, lcu, lcv, llu, llv)
     double *f3_halo = nullptr;     // periodic halo table of the forcing
     double *f3_dz = nullptr;       // dz of the device-address forcing (up to 1000 levels)
+    float *snap_dev = nullptr;     // packed snapshot fields (bt_snapshot.hip)
+    float *snap_host = nullptr;    // pinned copy of snap_dev
+    size_t snap_n = 0;             // floats in snap_dev / snap_host
     size_t f3_halo_n = 0;          // doubles in f3_halo
     long long *cnt = nullptr;
 };

@@ -72,6 +72,15 @@ SIGNATURES = {
     "inmsom_bt_forcing_3d": ([C.c_void_p, C.c_int] + [_dp] * 22, C.c_int),
     "inmsom_bt_forcing_3d_dev": ([C.c_void_p, C.c_int] + [_dp] * 22, C.c_int),
     "inmsom_bt_forcing_3d_timed": ([C.c_void_p, C.c_int, C.c_int, _dp], C.c_int),
+    "inmsom_bt_wdstd": ([C.c_char_p, C.c_char_p, C.c_int, _fp, _fp] + [C.c_int] * 9, C.c_int),
+    "inmsom_bt_rdstd": ([C.c_char_p, C.c_char_p, C.c_int, _fp, _fp] + [C.c_int] * 9, C.c_int),
+    "inmsom_bt_wdstd8": ([C.c_char_p, C.c_char_p, C.c_int, _dp, _fp] + [C.c_int] * 9, C.c_int),
+    "inmsom_bt_rdstd8": ([C.c_char_p, C.c_char_p, C.c_int, _dp, _fp] + [C.c_int] * 9, C.c_int),
+    "inmsom_bt_ctl_write": ([C.c_char_p, C.c_float] + [C.c_int] * 5 + [_dp, C.c_double, C.c_int, _dp, C.c_double,
+                                                                      C.c_int, _dp, C.c_double] + [C.c_int] * 6
+                            + [C.c_float, C.c_char_p, C.c_char_p], C.c_int),
+    "inmsom_bt_write_snapshot": ([C.c_void_p, C.c_char_p, C.c_int] + [C.c_double] * 4 + [C.c_int] * 6
+                                 + [C.c_float, _dp], C.c_int),
     "inmsom_bt_ens_create": ([C.c_int, C.POINTER(Grid), C.POINTER(Forcing), C.c_int, C.c_int,
                               C.POINTER(C.c_void_p)], C.c_int),
     "inmsom_bt_ens_destroy": ([C.c_void_p], C.c_int),

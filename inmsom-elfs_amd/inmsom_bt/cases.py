@@ -303,3 +303,76 @@ def write_case(path: str, b: Basin, rdis: np.ndarray, nsteps: int, nstep: int = 
         f.write(np.ascontiguousarray(rdis, dtype="<f8").tobytes())
         for name in FIELDS_STATE:
             f.write(np.ascontiguousarray(getattr(b, name), dtype="<f8").tobytes())
+
+
+# The reference's shipped configuration (Inc/basinpar.fi:4-19): 289 x 163
+# points, first computed T point at 27.525 E, 40.94 N, steps 0.05 x 0.04
+# degrees, Earth radius 6371 km.
+REF_NX, REF_NY = 289, 163
+REF_LON0, REF_LAT0, REF_DLON, REF_DLAT = 27.525, 40.94, 0.05, 0.04
+EARTH_RADIUS = 6371.0e3
+EARTH_OMEGA = 7.292115974546454e-5
+
+
+def read_mask(path: str, nx: int, ny: int) -> np.ndarray:
+    """lbasins from a DIOGIN text mask as gridcon reads it
+    (Service/grid_construction.f90:16-28): a comment line, then ny rows of
+    nx digits from the northern row (n = ny) down; 0 = ocean.  Returns
+    (ny, nx) int32 with row 0 = n = 1."""
+    with open(path, "r", newline="") as f:
+        lines = f.read().splitlines()
+    rows = [ln.rstrip("\r") for ln in lines[1:1 + ny]]
+    if len(rows) != ny or any(len(r) < nx for r in rows):
+        raise ValueError(f"{path}: expected {ny} rows of {nx} digits")
+    lb = np.array([[int(ch) for ch in r[:nx]] for r in rows], dtype=np.int32)
+    return np.ascontiguousarray(lb[::-1])
+
+
+def spherical_metrics(nx: int, ny: int, lon0: float, lat0: float, dlon: float, dlat: float,
+                      r: float = EARTH_RADIUS) -> dict:
+    """Metric coefficients of a regular longitude-latitude grid (rx = R cos(lat),
+    ry = R, Inc/basinpar.fi:31-32): x-distances at the latitude of each
+    staggered row (T and U rows at lat_t, V and H rows half a step north),
+    y-distances R * dlat.  This project's construction for the shipped
+    grid; the reference's basinpar is not on the path."""
+    deg = math.pi / 180.0
+    lat_t = lat0 + (np.arange(ny, dtype=np.float64) - 2.0) * dlat
+    lat_v = lat_t + 0.5 * dlat
+    cx_t = r * np.cos(lat_t * deg) * dlon * deg
+    cx_v = r * np.cos(lat_v * deg) * dlon * deg
+    ry = r * dlat * deg
+    ones = np.ones((ny, nx))
+    return {"dx": cx_t[:, None] * ones, "dy": ry * ones, "dxt": cx_t[:, None] * ones, "dyt": ry * ones,
+            "dxh": cx_v[:, None] * ones, "dyh": ry * ones, "dxb": cx_v[:, None] * ones, "dyb": ry * ones}
+
+
+def reference_basin(mask_path: str, topo_dir: str, topo_name: str = "topo.dat", *, metrics: str = "spherical",
+                    d: float = 4000.0, nu: float = 100.0, tau0: float = 0.05, g: float = 9.8,
+                    dt: float = 16.0) -> Basin:
+    """The reference's shipped basin: its land mask (mask.txt) and its
+    bathymetry on the model grid, read as init_pars does
+    (Control/init_pars.f90:151-152: prdstd of the box mmm..mm x nnn..nn
+    with lu, land 0), on spherical metrics or uniform ones of step d."""
+    from .fileio import rdstd
+
+    nx, ny = REF_NX, REF_NY
+    lb = read_mask(mask_path, nx, ny)
+    lu = (lb == 0).astype(np.float32)
+    topo = np.zeros((ny, nx), dtype=np.float32)
+    rdstd(topo_dir, topo_name, 1, topo, lu, (3, nx - 2, 3, ny - 2, 1, 1))
+    hhq_rest = topo.astype(np.float64)
+    if metrics == "spherical":
+        met = spherical_metrics(nx, ny, REF_LON0, REF_LAT0, REF_DLON, REF_DLAT)
+    else:
+        met = uniform_metrics(nx, ny, d)
+    deg = math.pi / 180.0
+    lat_v = REF_LAT0 + (np.arange(ny, dtype=np.float64) - 2.0 + 0.5) * REF_DLAT
+    lat_t = lat_v - 0.5 * REF_DLAT
+    rlh_s = np.repeat((2.0 * EARTH_OMEGA * np.sin(lat_v * deg))[:, None], nx, axis=1)
+    y = (lat_t - lat_t[2]) / (lat_t[-3] - lat_t[2])
+    taux = np.repeat((tau0 * np.cos(math.pi * y))[:, None], nx, axis=1)
+    zero = np.zeros((ny, nx))
+    mu = nu * lu.astype(np.float64)
+    return Basin(nx=nx, ny=ny, lbasins=lb, hhq_rest=hhq_rest, rlh_s=rlh_s, taux=taux, tauy=zero.copy(),
+                 wflux=zero.copy(), mu=mu, mu4=zero.copy(), ssh=zero.copy(), sshp=zero.copy(), u=zero.copy(),
+                 up=zero.copy(), v=zero.copy(), vp=zero.copy(), g=g, dt=dt, seed=0, **met)

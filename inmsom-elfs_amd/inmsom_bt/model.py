@@ -316,6 +316,19 @@ class BarotropicModel:
             *[(self._dev(q, s3) if q is not None else None) for q in (pgx, pgy)],
             self._dev(rhsx2d, s2), self._dev(rhsy2d, s2)), "inmsom_bt_forcing_3d_dev")
 
+    def write_snapshot(self, path: str, nrec: int, xt0: float = 0.0, dxst: float = 1.0, yt0: float = 0.0,
+                       dyst: float = 1.0, date=(2000, 1, 1, 0, 0), calendar: int = 1,
+                       tstep: float = 86400.0) -> float:
+        """ssh.dat, ubrtr.dat, vbrtr.dat (+ .ctl) in ``path`` with record
+        ``nrec`` = the current state (reference output formats); returns the
+        device time (ms) of packing and copying the fields."""
+        ms = C.c_double()
+        y, mo, d, h, mi = date
+        check(lib().inmsom_bt_write_snapshot(self._h, path.encode(), int(nrec), float(xt0), float(dxst), float(yt0),
+                                             float(dyst), int(calendar), y, mo, d, h, mi, float(tstep),
+                                             C.byref(ms)), "inmsom_bt_write_snapshot")
+        return ms.value
+
     def forcing_3d_timed(self, nz: int, nrep: int) -> float:
         """HIP-event time (ms) of nrep repetitions of the forcing's device
         work on the fields the last forcing_3d call left on the device."""
