@@ -412,6 +412,54 @@ def forcing_cpu(b, f):
                       "(bto_forcing_3d, one thread)"}
 
 
+def snapshot_leg(m, b, count):
+    """The state written in the reference's output formats (ssh, ubrtr,
+    vbrtr as DIOGIN float32 records with GrADS descriptors, one record per
+    snapshot) from the device: packing and the copy to pinned memory timed
+    with HIP events, and the whole call including the file writes into a
+    temporary directory; the host conversion and the host record writer on
+    the same state as the CPU baseline."""
+    import shutil
+    import tempfile
+
+    from inmsom_bt import fileio
+    from inmsom_bt.model import gridcon
+
+    nx, ny = b.nx, b.ny
+    cells = (nx - 4) * (ny - 4) + (nx - 3) * (ny - 4) + (nx - 4) * (ny - 3)
+    out_bytes = 4 * cells
+    tmp = tempfile.mkdtemp(prefix="inmsom_snap_")
+    try:
+        m.write_snapshot(tmp, 1)                                     # allocates the buffers
+        dev_ms, wall = [], []
+        for r in range(count):
+            t0 = time.perf_counter()
+            dev_ms.append(m.write_snapshot(tmp, r + 2))
+            wall.append(time.perf_counter() - t0)
+        st = m.get_state()
+        masks = gridcon(b.lbasins)
+        t0 = time.perf_counter()
+        for name, f, lu, box in (("h_ssh.dat", st["ssh"], masks["lu"], (3, nx - 2, 3, ny - 2)),
+                                 ("h_u.dat", st["u"], masks["llu"], (2, nx - 2, 3, ny - 2)),
+                                 ("h_v.dat", st["v"], masks["llv"], (3, nx - 2, 2, ny - 2))):
+            fileio.wdstd(tmp, name, 1, np.ascontiguousarray(f, dtype=np.float32), lu, box + (1, 1))
+        host_s = time.perf_counter() - t0
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    dev_s = float(np.mean(dev_ms)) * 1e-3
+    wall_s = float(np.mean(wall))
+    res = {"snapshots": count, "bytes_per_snapshot": out_bytes,
+           "value": out_bytes / wall_s / 1e9, "unit": "GB/s of snapshot records written (device state to files)",
+           "call_ms": wall_s * 1e3, "device_pack_and_copy_ms": dev_s * 1e3,
+           "device_pack_and_copy_GBps": out_bytes / dev_s / 1e9,
+           "cpu_baseline": {"value": out_bytes / host_s / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
+                            "sample": f"FP64 state converted on the host and written with the host DIOGIN "
+                                      f"writer (inmsom_bt_wdstd), one snapshot, {host_s * 1e3:.1f} ms"}}
+    log(f"snapshot: {out_bytes / 1e6:.1f} MB per snapshot, call {wall_s * 1e3:.2f} ms, device pack+copy "
+        f"{dev_s * 1e3:.3f} ms, host writer {host_s * 1e3:.1f} ms")
+    return res
+
+
 CLOSURE_PARAMS = os.path.join(ROOT, "inmsom-elfs_amd", "inmsom_bt", "data", "closure_cfg1.npz")
 
 
@@ -529,6 +577,8 @@ def main():
     ap.add_argument("--forcing-levels", type=int, default=COUPLING_NZ,
                     help="levels of the forcing leg (3-D model -> barotropic RHS; 0: skip it)")
     ap.add_argument("--forcing-reps", type=int, default=20)
+    ap.add_argument("--snapshots", type=int, default=3,
+                    help="snapshots of the state written in the reference's output formats (0: skip)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH", "WRITE"),
                     help="write the per-launch HBM traffic of the fused kernels from two rocprofv3 "
                          "--pmc outputs (counter_collection.csv or results.db; FETCH_SIZE, WRITE_SIZE) and exit")
@@ -558,6 +608,7 @@ def main():
         forcing = None
         if args.forcing_levels > 0:
             forcing, forcing_inp = forcing_leg(m, b, args.forcing_levels, args.forcing_reps)
+        snapshot = snapshot_leg(m, b, args.snapshots) if args.snapshots > 0 else None
 
     errs = {}
     if "fp64" in states:
@@ -607,6 +658,8 @@ def main():
             forcing["cpu_baseline"] = forcing_cpu(b, forcing_inp)
         del forcing_inp
         line["forcing"] = forcing
+    if snapshot is not None:
+        line["snapshot"] = snapshot
     if args.ensemble_steps > 0:
         line["ensemble_fp32"] = ensemble_leg(args.ensemble_steps, min(args.warmup, 20),
                                              args.ensemble_members)
