@@ -827,6 +827,149 @@ void bto_forcing_3d(const bto_grid *g, int nz, const double *dz, double cb_nl, d
     free(tx2); free(ty2);
 }
 
+/* Service/oc_algorithms.f90:494-524 (factor8): the sweep for
+ * a(j) f(j-1) + b(j) f(j) + c(j) f(j+1) = eta(j), j = 1..n. */
+static void factor8(int n, const double *a, const double *b, const double *c, const double *eta, double *rksi,
+                    double *x, double *y)
+{
+    x[0] = -(c[0] / b[0]);
+    y[0] = eta[0] / b[0];
+    for (int j = 1; j < n; ++j) {
+        const double scr = 1.0 / (a[j] * x[j - 1] + b[j]);
+        x[j] = -(c[j] * scr);
+        y[j] = (eta[j] - a[j] * y[j - 1]) * scr;
+    }
+    rksi[n - 1] = y[n - 1];
+    for (int j = n - 2; j >= 0; --j) rksi[j] = x[j] * rksi[j + 1] + y[j];
+}
+
+/* Function/vel_ssh.f90:794-1027 (baroclinic_dynamics): per open U (V) face
+ * the implicit vertical viscosity/advection column of nz levels, solved
+ * by factor8; un, vn wrapped on periodic boundaries; then the
+ * Robert-Asselin step of uup, vvp and uu = un, vv = vn over the interior
+ * and its one-point ring.  3-D arrays (nz, ny, nx); ww and nu (nz+1, ny,
+ * nx); dz[nz], hzt[nz+1]. */
+void bto_baroclinic_dynamics(const bto_grid *g, int nz, double tau, const double *dz, const double *hzt,
+                             double *uu, double *uup, double *vv, double *vvp, const double *ww,
+                             const double *nu, const double *rhsx, const double *rhsy,
+                             const double *rhsx_tran, const double *rhsy_tran, const double *pgrx,
+                             const double *pgry, const double *rdis, const double *tx_surf,
+                             const double *ty_surf, const double *tx_bot, const double *ty_bot,
+                             const double *hhu, const double *hhup, const double *hhun, const double *hhv,
+                             const double *hhvp, const double *hhvn, const double *hhh)
+{
+    const size_t sz = (size_t)g->nx * g->ny;
+    double *un = calloc(sz * nz, sizeof(double)), *vn = calloc(sz * nz, sizeof(double));
+    double *a = malloc(6 * nz * sizeof(double));
+    double *b = a + nz, *c = b + nz, *eta = c + nz, *x = eta + nz, *y = x + nz;
+    double *rksi = malloc(nz * sizeof(double));
+#define L3(f, m, n, k) (f)[(size_t)((k) - 1) * sz + IX(m, n)]
+    for (int n = NYS; n <= NYE; ++n)
+        for (int m = NXS; m <= NXE; ++m) {
+            const size_t i = IX(m, n);
+            if (g->lcu[i] > 0.5f) {
+                const double bp = hhun[i] * g->dxt[i] * g->dyh[i] / tau / 2.0;
+                const double bp0 = hhup[i] * g->dxt[i] * g->dyh[i] / tau / 2.0;
+                double dm, dp = 0.0, fz_m, fz_p = 0.0;
+                for (int k = 1; k <= nz; ++k) {
+                    dm = (k == 1) ? 0.0 : dp;
+                    fz_m = (k == 1) ? 0.0 : fz_p;
+                    if (k < nz) {
+                        dp = (L3(nu, m, n, k + 1) + L3(nu, m + 1, n, k + 1)) / 2.0 / hhun[i] / hzt[k] * g->dxt[i] *
+                             g->dyh[i];
+                        fz_p = (L3(ww, m, n, k + 1) * g->dx[i] * g->dy[i] +
+                                L3(ww, m + 1, n, k + 1) * g->dx[IX(m + 1, n)] * g->dy[IX(m + 1, n)]) /
+                               2.0 * (L3(uu, m, n, k) + L3(uu, m, n, k + 1)) / 2.0;
+                    } else {
+                        dp = 0.0;
+                        fz_p = 0.0;
+                    }
+                    const double dzk = dz[k - 1];
+                    c[k - 1] = (k < nz) ? -(dp / dzk) : 0.0;
+                    a[k - 1] = (k == 1) ? 0.0 : -(dm / dzk);
+                    b[k - 1] = (k == 1) ? bp + dp / dzk : (k == nz) ? bp + dm / dzk : bp + (dp + dm) / dzk;
+                    const double up = L3(uup, m, n, k);
+                    double e = bp0 * up + L3(rhsx, m, n, k) + L3(rhsx_tran, m, n, k) + pgrx[i] -
+                               (fz_p - fz_m) / dzk +
+                               (g->rlh_s[i] * hhh[i] * g->dxb[i] * g->dyb[i] *
+                                    (L3(vv, m + 1, n, k) + L3(vv, m, n, k)) +
+                                g->rlh_s[IX(m, n - 1)] * hhh[IX(m, n - 1)] * g->dxb[IX(m, n - 1)] *
+                                    g->dyb[IX(m, n - 1)] * (L3(vv, m + 1, n - 1, k) + L3(vv, m, n - 1, k))) /
+                                   4.0 -
+                               (rdis[i] + rdis[IX(m + 1, n)]) / 2.0 * up * g->dxt[i] * g->dyh[i] * hhu[i];
+                    if (k == 1) e = e + tx_surf[i] / dzk * g->dxt[i] * g->dyh[i];
+                    if (k == nz) e = e + tx_bot[i] / dzk * g->dxt[i] * g->dyh[i];
+                    eta[k - 1] = e;
+                }
+                factor8(nz, a, b, c, eta, rksi, x, y);
+                for (int k = 1; k <= nz; ++k) L3(un, m, n, k) = rksi[k - 1];
+            }
+            if (g->lcv[i] > 0.5f) {
+                const double bp = hhvn[i] * g->dxh[i] * g->dyt[i] / tau / 2.0;
+                const double bp0 = hhvp[i] * g->dxh[i] * g->dyt[i] / tau / 2.0;
+                double dm, dp = 0.0, fz_m, fz_p = 0.0;
+                for (int k = 1; k <= nz; ++k) {
+                    dm = (k == 1) ? 0.0 : dp;
+                    fz_m = (k == 1) ? 0.0 : fz_p;
+                    if (k < nz) {
+                        dp = (L3(nu, m, n, k + 1) + L3(nu, m, n + 1, k + 1)) / 2.0 / hhvn[i] / hzt[k] * g->dxh[i] *
+                             g->dyt[i];
+                        fz_p = (L3(ww, m, n, k + 1) * g->dx[i] * g->dy[i] +
+                                L3(ww, m, n + 1, k + 1) * g->dx[IX(m, n + 1)] * g->dy[IX(m, n + 1)]) /
+                               2.0 * (L3(vv, m, n, k) + L3(vv, m, n, k + 1)) / 2.0;
+                    } else {
+                        dp = 0.0;
+                        fz_p = 0.0;
+                    }
+                    const double dzk = dz[k - 1];
+                    c[k - 1] = (k < nz) ? -(dp / dzk) : 0.0;
+                    a[k - 1] = (k == 1) ? 0.0 : -(dm / dzk);
+                    b[k - 1] = (k == 1) ? bp + dp / dzk : (k == nz) ? bp + dm / dzk : bp + (dp + dm) / dzk;
+                    const double vp = L3(vvp, m, n, k);
+                    double e = bp0 * vp + L3(rhsy, m, n, k) + L3(rhsy_tran, m, n, k) + pgry[i] -
+                               (fz_p - fz_m) / dzk -
+                               (g->rlh_s[i] * hhh[i] * g->dxb[i] * g->dyb[i] *
+                                    (L3(uu, m, n + 1, k) + L3(uu, m, n, k)) +
+                                g->rlh_s[IX(m - 1, n)] * hhh[IX(m - 1, n)] * g->dxb[IX(m - 1, n)] *
+                                    g->dyb[IX(m - 1, n)] * (L3(uu, m - 1, n + 1, k) + L3(uu, m - 1, n, k))) /
+                                   4.0 -
+                               (rdis[i] + rdis[IX(m, n + 1)]) / 2.0 * vp * g->dxh[i] * g->dyt[i] * hhv[i];
+                    if (k == 1) e = e + ty_surf[i] / dzk * g->dxh[i] * g->dyt[i];
+                    if (k == nz) e = e + ty_bot[i] / dzk * g->dxh[i] * g->dyt[i];
+                    eta[k - 1] = e;
+                }
+                factor8(nz, a, b, c, eta, rksi, x, y);
+                for (int k = 1; k <= nz; ++k) L3(vn, m, n, k) = rksi[k - 1];
+            }
+        }
+    for (int k = 0; k < nz; ++k) {
+        cyclize(g, un + (size_t)k * sz);
+        cyclize(g, vn + (size_t)k * sz);
+    }
+    const double ts = g->time_smooth;
+    for (int n = NYS - 1; n <= NYE + 1; ++n)
+        for (int m = NXS - 1; m <= NXE + 1; ++m) {
+            const size_t i = IX(m, n);
+            if (g->lcu[i] > 0.5f)
+                for (int k = 1; k <= nz; ++k) {
+                    L3(uup, m, n, k) = L3(uu, m, n, k) + ts * (L3(un, m, n, k) - 2.0 * L3(uu, m, n, k) +
+                                                               L3(uup, m, n, k)) / 2.0;
+                    L3(uu, m, n, k) = L3(un, m, n, k);
+                }
+            if (g->lcv[i] > 0.5f)
+                for (int k = 1; k <= nz; ++k) {
+                    L3(vvp, m, n, k) = L3(vv, m, n, k) + ts * (L3(vn, m, n, k) - 2.0 * L3(vv, m, n, k) +
+                                                               L3(vvp, m, n, k)) / 2.0;
+                    L3(vv, m, n, k) = L3(vn, m, n, k);
+                }
+        }
+#undef L3
+    free(un);
+    free(vn);
+    free(a);
+    free(rksi);
+}
+
 void bto_model_step(const bto_grid *g, bto_depths *d, double tau, int nstep,
                     int ksw4, double cb_nl, double ebottom, double *ubrtr_e,
                     double *ubrtrp_e, double *vbrtr_e, double *vbrtrp_e,

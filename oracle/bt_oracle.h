@@ -166,6 +166,19 @@ void bto_internal_update(const bto_grid *g, double tau, int nz,
                          double *hhqn, double *hhun, double *hhvn, double *hhhn,
                          double *uu, double *vv);
 
+/* Function/vel_ssh.f90:794-1027 (baroclinic_dynamics) with factor8
+ * (Service/oc_algorithms.f90:494-524): 3-D arrays (nz, ny, nx), ww and nu
+ * (nz+1, ny, nx), dz[nz], hzt[nz+1]; uu, uup, vv, vvp read and written.
+ * nz >= 2. */
+void bto_baroclinic_dynamics(const bto_grid *g, int nz, double tau, const double *dz, const double *hzt,
+                             double *uu, double *uup, double *vv, double *vvp, const double *ww,
+                             const double *nu, const double *rhsx, const double *rhsy,
+                             const double *rhsx_tran, const double *rhsy_tran, const double *pgrx,
+                             const double *pgry, const double *rdis, const double *tx_surf,
+                             const double *ty_surf, const double *tx_bot, const double *ty_bot,
+                             const double *hhu, const double *hhup, const double *hhun, const double *hhv,
+                             const double *hhvp, const double *hhvn, const double *hhh);
+
 /* Wet-cell volume anomaly sum(eta * dx * dy) over lu interior cells,
  * summed row by row in a fixed order. */
 double bto_volume(const bto_grid *g, const double *ssh);

@@ -273,6 +273,26 @@ def forcing_3d(og: "OracleGrid", inp: Dict[str, np.ndarray], cb_nl: float, ebott
     return {"rhsx2d": w["rhsx2d"], "rhsy2d": w["rhsy2d"]}
 
 
+BAROCLINIC_IN_3D = ("uu", "uup", "vv", "vvp", "rhsx", "rhsy", "rhsx_tran", "rhsy_tran")
+BAROCLINIC_IN_W = ("ww", "nu")                      # nz + 1 levels
+BAROCLINIC_IN_2D = ("pgrx", "pgry", "rdis", "tx_surf", "ty_surf", "tx_bot", "ty_bot", "hhu", "hhup", "hhun",
+                    "hhv", "hhvp", "hhvn", "hhh")
+BAROCLINIC_OUT = ("uu", "uup", "vv", "vvp")
+
+
+def baroclinic(og: "OracleGrid", tau: float, inp: Dict[str, np.ndarray]) -> Dict[str, np.ndarray]:
+    """bto_baroclinic_dynamics on copies of ``inp`` (dz, hzt, BAROCLINIC_IN_3D
+    as (nz, ny, nx), ww/nu as (nz+1, ny, nx), BAROCLINIC_IN_2D); returns
+    the updated uu, uup, vv, vvp."""
+    L = lib()
+    w = {k: np.ascontiguousarray(inp[k], dtype=np.float64).copy() for k in inp}
+    nz = w["uu"].shape[0]
+    L.bto_baroclinic_dynamics(C.byref(og.g), nz, C.c_double(tau), _d(w["dz"]), _d(w["hzt"]),
+                              *[_d(w[k]) for k in ("uu", "uup", "vv", "vvp", "ww", "nu", "rhsx", "rhsy", "rhsx_tran",
+                                                   "rhsy_tran") + BAROCLINIC_IN_2D])
+    return {k: w[k] for k in BAROCLINIC_OUT}
+
+
 def have_reference() -> bool:
     return os.path.exists(REF_HARNESS)
 
@@ -285,6 +305,33 @@ def run_reference(case_path: str, out_path: str, nx: int, ny: int, mode: int,
                    stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, cwd=os.path.dirname(out_path))
     raw = np.fromfile(out_path, dtype="<f8")
     per = nx * ny
+    if mode == 4:
+        # masks, dz (nz), hzt (nz+1), 8 3-D inputs, 2 of nz+1 levels, 14 planes, 4 3-D outputs
+        nin2 = len(BAROCLINIC_IN_2D)
+        # raw = 7 per + nz + (nz+1) + 8 nz per + 2 (nz+1) per + nin2 per + 4 nz per
+        nz = (raw.size - (7 + 2 + nin2) * per - 1) // (2 + 14 * per)
+        res = {"masks": {k: raw[i * per:(i + 1) * per].reshape(ny, nx) for i, k in enumerate(MASKS)}}
+        o = 7 * per
+        inp = {"dz": raw[o:o + nz].copy()}
+        o += nz
+        inp["hzt"] = raw[o:o + nz + 1].copy()
+        o += nz + 1
+        for k in BAROCLINIC_IN_3D:
+            inp[k] = raw[o:o + nz * per].reshape(nz, ny, nx).copy()
+            o += nz * per
+        for k in BAROCLINIC_IN_W:
+            inp[k] = raw[o:o + (nz + 1) * per].reshape(nz + 1, ny, nx).copy()
+            o += (nz + 1) * per
+        for k in BAROCLINIC_IN_2D:
+            inp[k] = raw[o:o + per].reshape(ny, nx).copy()
+            o += per
+        out = {}
+        for k in BAROCLINIC_OUT:
+            out[k] = raw[o:o + nz * per].reshape(nz, ny, nx).copy()
+            o += nz * per
+        assert o == raw.size
+        res["baroclinic_in"], res["baroclinic_out"] = inp, out
+        return res
     if mode == 3:
         # masks, dz (nz values), 7 3-D fields, 12 planes of inputs, 2 of outputs
         nz = (raw.size - (7 + len(FORCING_IN_2D) + 2) * per) // (1 + len(FORCING_IN_3D) * per)

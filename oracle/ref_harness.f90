@@ -41,6 +41,10 @@
 !   sea_surface_fluxes, sea_bottom_fluxes, uv_trans, uv_diff2 and depth_ave
 !   run in dyncall's order, the assembly loop of :109-126 is restated
 !   (it sits inside ocean_model_step), and RHSx2d, RHSy2d are written.
+! mode 4: baroclinic_dynamics (Function/vel_ssh.f90:794-1027) on synthetic
+!   3-D fields at the compiled nz (the reference's own module depths from
+!   hh_init, a non-uniform vertical grid hzt, dz): the inputs are written,
+!   the routine is called, and uu, uup, vv, vvp are written.
 program ref_harness
 use main_basin_pars
 use mpi_parallel_tools
@@ -284,6 +288,45 @@ else if (mode == 3) then
     enddo
   enddo
   write(23) RHSx2d, RHSy2d
+else if (mode == 4) then
+  ! baroclinic_dynamics (Function/vel_ssh.f90:794-1027) on synthetic 3-D
+  ! fields at the compiled nz, the depths of hh_init, a non-uniform
+  ! vertical grid; inputs written, then the updated uu, uup, vv, vvp
+  call hh_init(hhq_e, hhqp_e, hhqn_e, hhu_e, hhup_e, hhun_e, hhv_e, hhvp_e, hhvn_e, &
+               hhh_e, hhhp_e, hhhn_e, ssh_e, sshp_e, hhq_rest)
+  hhu = hhu_e; hhup = hhup_e; hhun = hhun_e
+  hhv = hhv_e; hhvp = hhvp_e; hhvn = hhvn_e; hhh = hhh_e
+  do k = 1, nz
+    dz(k) = (0.5d0 + k)/(0.5d0*nz + 0.5d0*nz*(nz + 1))
+    uu(:,:,k)  = (ubrtr_e*(1.0d0 + 0.05d0*k) + 0.01d0*k)*lcu
+    vv(:,:,k)  = (vbrtr_e*(1.0d0 - 0.03d0*k) - 0.02d0*k)*lcv
+    uup(:,:,k) = (ubrtrp_e*(1.0d0 + 0.04d0*k) + 0.005d0*k)*lcu
+    vvp(:,:,k) = (vbrtrp_e*(1.0d0 - 0.02d0*k) - 0.01d0*k)*lcv
+    RHSx3d(:,:,k) = 1.0d2*(ubrtr_e + 0.1d0*k)*dxt*dyh*1.0d-3
+    RHSy3d(:,:,k) = -1.0d2*(vbrtr_e - 0.05d0*k)*dxh*dyt*1.0d-3
+    RHSx3d_tran(:,:,k) = 3.0d1*(ssh_e - 0.02d0*k)*dxt*dyh*1.0d-3
+    RHSy3d_tran(:,:,k) = 2.0d1*(sshp_e + 0.03d0*k)*dxh*dyt*1.0d-3
+  enddo
+  do k = 1, nz + 1
+    hzt(k) = (0.6d0 + 0.05d0*k)/nz
+    ww(:,:,k) = 1.0d-4*(ssh_e - 0.5d0*sshp_e + 0.01d0*k)
+    anzu(:,:,k) = 1.0d-3*(1.0d0 + 0.2d0*k) + 1.0d-2*abs(ubrtr_e)
+  enddo
+  pgrx = 5.0d1*(sshp_e - ssh_e)*dxt*dyh*1.0d-3
+  pgry = -4.0d1*(sshp_e + ssh_e)*dxh*dyt*1.0d-3
+  r_diss = rdis
+  surf_stress_x = 1.0d-4*taux
+  surf_stress_y = 1.0d-4*(taux - 0.3d0)
+  bot_stress_x = -1.0d-5*ubrtr_e
+  bot_stress_y = -1.0d-5*vbrtr_e
+  write(23) dz, hzt
+  write(23) uu, uup, vv, vvp, RHSx3d, RHSy3d, RHSx3d_tran, RHSy3d_tran
+  write(23) ww, anzu
+  write(23) pgrx, pgry, r_diss, surf_stress_x, surf_stress_y, bot_stress_x, bot_stress_y, &
+            hhu, hhup, hhun, hhv, hhvp, hhvn, hhh
+  call baroclinic_dynamics(tau, uu, uup, vv, vvp, ww, anzu, RHSx3d, RHSy3d, RHSx3d_tran, RHSy3d_tran, &
+                           pgrx, pgry, r_diss, surf_stress_x, surf_stress_y, bot_stress_x, bot_stress_y)
+  write(23) uu, uup, vv, vvp
 else
   ! helper routines, one call each, on the case's state arrays
   ! hh_init from (ssh_e, sshp_e)

@@ -341,3 +341,88 @@ def test_forcing_3d_closed_form_at_rest():
     llu_only[[0, -1], :] = False
     llu_only[:, [0, -1]] = False
     assert np.allclose(out["rhsx2d"][llu_only], (a * dz).sum(), rtol=1e-13)
+
+
+def _baroclinic_pure_viscosity(seed, uniform_profile):
+    """Inputs of baroclinic_dynamics with only the implicit vertical
+    viscosity left: no vertical velocity, Coriolis, drag, RHS, pressure or
+    stress terms, equal depths at all time levels."""
+    b = make_basin(40, 30, seed=seed, sigma=3.0, metrics="stretched")
+    b.rlh_s = np.zeros_like(b.rlh_s)
+    og = oracle.OracleGrid(b)
+    rng = np.random.default_rng(seed)
+    nz, sh = 8, (b.ny, b.nx)
+    z3, z2 = np.zeros((nz,) + sh), np.zeros(sh)
+    h = 40.0 + np.abs(rng.standard_normal(sh))
+    if uniform_profile:
+        col = rng.standard_normal(sh)
+        uup = np.broadcast_to(col, (nz,) + sh).copy()
+        vvp = np.broadcast_to(-col, (nz,) + sh).copy()
+    else:
+        uup, vvp = rng.standard_normal((nz,) + sh), rng.standard_normal((nz,) + sh)
+    inp = {"dz": 0.5 + rng.random(nz), "hzt": 0.5 + rng.random(nz + 1), "uu": uup.copy(), "uup": uup,
+           "vv": vvp.copy(), "vvp": vvp, "ww": np.zeros((nz + 1,) + sh),
+           "nu": 1e-2 * (1.0 + rng.random((nz + 1,) + sh)), "rhsx": z3, "rhsy": z3, "rhsx_tran": z3,
+           "rhsy_tran": z3}
+    inp["dz"] /= inp["dz"].sum()
+    for k in ("pgrx", "pgry", "rdis", "tx_surf", "ty_surf", "tx_bot", "ty_bot"):
+        inp[k] = z2
+    for k in ("hhu", "hhup", "hhun", "hhv", "hhvp", "hhvn", "hhh"):
+        inp[k] = h
+    return b, og, inp
+
+
+def test_baroclinic_viscosity_keeps_uniform_profile():
+    """Vertical diffusion leaves a depth-uniform velocity unchanged: every
+    row of the implicit system sums to bp (a + b + c = bp), so un = uup."""
+    b, og, inp = _baroclinic_pure_viscosity(3, True)
+    out = oracle.baroclinic(og, 40.0, inp)
+    cu = oracle.OracleGrid(b).masks["lcu"] > 0.5
+    cu[:2, :] = cu[-2:, :] = False
+    cu[:, :2] = cu[:, -2:] = False
+    for name, src in (("uu", "uup"), ("vv", "vvp")):
+        err = np.abs(out[name] - inp[src])[:, cu].max()
+        assert err <= 1e-12 * np.abs(inp[src]).max(), (name, err)
+
+
+def test_baroclinic_viscosity_conserves_depth_integral_and_matches_dense_solve():
+    """The implicit viscosity only moves momentum between levels: the
+    dz-weighted sum of un equals that of uup (bp = bp0, no stress).  Each
+    column also equals a dense solve of the tridiagonal system built from
+    the reference's coefficients (factor8 is the Thomas algorithm)."""
+    b, og, inp = _baroclinic_pure_viscosity(4, False)
+    tau = 40.0
+    out = oracle.baroclinic(og, tau, inp)
+    dz, hzt, nu = inp["dz"], inp["hzt"], inp["nu"]
+    masks = og.masks
+    nz = dz.size
+    checked = 0
+    for n in range(2, b.ny - 2):
+        for m in range(2, b.nx - 2):
+            if masks["lcu"][n, m] <= 0.5:
+                continue
+            s_in, s_out = (dz * inp["uup"][:, n, m]).sum(), (dz * out["uu"][:, n, m]).sum()
+            assert abs(s_out - s_in) <= 1e-12 * max(1.0, np.abs(inp["uup"][:, n, m]).max())
+            if checked < 25:
+                area = b.dxt[n, m] * b.dyh[n, m]
+                bp = inp["hhun"][n, m] * area / tau / 2.0
+                d = [(nu[k + 1, n, m] + nu[k + 1, n, m + 1]) / 2.0 / inp["hhun"][n, m] / hzt[k + 1] * area
+                     for k in range(nz - 1)]
+                A = np.zeros((nz, nz))
+                for k in range(nz):
+                    A[k, k] = bp
+                    if k > 0:
+                        A[k, k] += d[k - 1] / dz[k]
+                        A[k, k - 1] = -d[k - 1] / dz[k]
+                    if k < nz - 1:
+                        A[k, k] += d[k] / dz[k]
+                        A[k, k + 1] = -d[k] / dz[k]
+                want = np.linalg.solve(A, bp * inp["uup"][:, n, m])
+                assert np.allclose(out["uu"][:, n, m], want, rtol=1e-12, atol=1e-14)
+                checked += 1
+    assert checked == 25
+    # the Robert-Asselin step of the filtered level (vel_ssh.f90:1006-1010)
+    ts = b.time_smooth
+    want_p = inp["uu"] + ts * (out["uu"] - 2.0 * inp["uu"] + inp["uup"]) / 2.0
+    cu = masks["lcu"] > 0.5
+    assert np.allclose(out["uup"][:, cu], want_p[:, cu], rtol=1e-13, atol=1e-15)

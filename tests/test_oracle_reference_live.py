@@ -71,3 +71,22 @@ def test_forcing_3d_bitwise(seed, metrics):
     # cells the reference leaves alone keep the caller's values
     untouched = (want["rhsx2d"] == 7.0)
     assert untouched.any() and (got["rhsx2d"][untouched] == 7.0).all()
+
+
+@pytest.mark.parametrize("seed,metrics", [(9, "uniform"), (10, "stretched")])
+def test_baroclinic_bitwise(seed, metrics):
+    """baroclinic_dynamics (Function/vel_ssh.f90:794-1027 with factor8,
+    Service/oc_algorithms.f90:494-524) called by the harness on its
+    synthetic 3-D fields, against the oracle on the inputs it wrote."""
+    b = make_basin(285, 159, seed=seed, sigma=4.0, g=9.8, hmax=120.0, metrics=metrics)
+    b = randomize_state(b, seed + 1, amp_uv=0.05)
+    rd = oracle.OracleGrid(b).rayleigh()
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "case.bin")
+        write_case(path, b, rd, 1, 2, 0, 4)
+        ref = oracle.run_reference(path, path + ".out", b.nx, b.ny, 4)
+    inp, want = ref["baroclinic_in"], ref["baroclinic_out"]
+    assert inp["uu"].shape[0] == 20 and inp["ww"].shape[0] == 21
+    got = oracle.baroclinic(oracle.OracleGrid(b), b.dt, inp)
+    assert_bitwise(got, want, oracle.BAROCLINIC_OUT, f"baroclinic seed {seed} {metrics}")
+    assert not np.array_equal(got["uu"], inp["uu"])
