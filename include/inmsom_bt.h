@@ -340,6 +340,39 @@ int inmsom_bt_write_snapshot(inmsom_bt_ctx *ctx, const char *path, int nrec, dou
                              double dyst, int calendar, int year, int month, int day, int hour, int minute,
                              float tstep, double *ms);
 
+/* baroclinic_dynamics, Function/vel_ssh.f90:794-1027 (with factor8,
+ * Service/oc_algorithms.f90:494-524), with the reference's arguments and
+ * the module arrays it reads: uu, uup, vv, vvp, RHSx, RHSy, RHSx_tran,
+ * RHSy_tran (nz, ny, nx); ww and nu (the reference's anzu) (nz+1, ny, nx);
+ * pgrx, pgry, rdis (r_diss), tx_surf, ty_surf, tx_bot, ty_bot (ny, nx);
+ * the vertical grid dz[nz], hzt[nz+1]; the 3-D model's depths hhu, hhup,
+ * hhun, hhv, hhvp, hhvn, hhh (ny, nx).  Coriolis, metrics, masks and
+ * time_smooth are the context's.  uu, uup, vv, vvp are updated in place
+ * (host arrays).  nz in 2..128. */
+int inmsom_bt_baroclinic_dynamics(inmsom_bt_ctx *ctx, double tau, int nz, double *uu, double *uup, double *vv,
+                                  double *vvp, const double *ww, const double *nu, const double *rhsx,
+                                  const double *rhsy, const double *rhsx_tran, const double *rhsy_tran,
+                                  const double *pgrx, const double *pgry, const double *rdis, const double *tx_surf,
+                                  const double *ty_surf, const double *tx_bot, const double *ty_bot,
+                                  const double *dz, const double *hzt, const double *hhu, const double *hhup,
+                                  const double *hhun, const double *hhv, const double *hhvp, const double *hhvn,
+                                  const double *hhh);
+/* The same on device addresses (dz, hzt stay host arrays); returns after
+ * the work has finished. */
+int inmsom_bt_baroclinic_dynamics_dev(inmsom_bt_ctx *ctx, double tau, int nz, double *uu, double *uup, double *vv,
+                                      double *vvp, const double *ww, const double *nu, const double *rhsx,
+                                      const double *rhsy, const double *rhsx_tran, const double *rhsy_tran,
+                                      const double *pgrx, const double *pgry, const double *rdis,
+                                      const double *tx_surf, const double *ty_surf, const double *tx_bot,
+                                      const double *ty_bot, const double *dz, const double *hzt, const double *hhu,
+                                      const double *hhup, const double *hhun, const double *hhv,
+                                      const double *hhvp, const double *hhvn, const double *hhh);
+/* Timing: nrep repetitions of the device work on the fields the last
+ * inmsom_bt_baroclinic_dynamics call with nz levels left on the device
+ * (each repetition advances them); *ms = HIP-event time of all.  No
+ * counterpart in the reference. */
+int inmsom_bt_baroclinic_timed(inmsom_bt_ctx *ctx, double tau, int nz, int nrep, double *ms);
+
 /* Timing of the coupling's device work: nrep repetitions of the kernels of
  * inmsom_bt_remove_barotropic then inmsom_bt_internal_update on the fields
  * the last inmsom_bt_internal_update call left on the device (it must have

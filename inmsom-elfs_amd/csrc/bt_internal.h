@@ -97,6 +97,11 @@ struct inmsom_bt_ctx {
     uint8_t *f3_mb = nullptr;      // mask bits of the forcing (lu, luu, lcu, lcv, llu, llv)
     double *f3_halo = nullptr;     // periodic halo table of the forcing
     double *f3_dz = nullptr;       // dz of the device-address forcing (up to 1000 levels)
+    double *bc = nullptr;          // staging of baroclinic_dynamics (bt_baroclinic.hip)
+    size_t bc_n = 0;               // doubles in bc
+    int bc_nz = 0;                 // levels of the staged fields (0 = none)
+    double *bc_dev = nullptr;      // un, vn and the vertical grid of the device-address call
+    size_t bc_dev_n = 0;           // doubles in bc_dev
     float *snap_dev = nullptr;     // packed snapshot fields (bt_snapshot.hip)
     float *snap_host = nullptr;    // pinned copy of snap_dev
     size_t snap_n = 0;             // floats in snap_dev / snap_host

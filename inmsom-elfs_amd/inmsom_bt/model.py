@@ -329,6 +329,63 @@ class BarotropicModel:
                                              C.byref(ms)), "inmsom_bt_write_snapshot")
         return ms.value
 
+    BAROCLINIC_3D = ("rhsx", "rhsy", "rhsx_tran", "rhsy_tran")
+    BAROCLINIC_W = ("ww", "nu")
+    BAROCLINIC_2D = ("pgrx", "pgry", "rdis", "tx_surf", "ty_surf", "tx_bot", "ty_bot")
+    BAROCLINIC_DEPTHS = ("hhu", "hhup", "hhun", "hhv", "hhvp", "hhvn", "hhh")
+
+    def baroclinic_dynamics(self, tau: float, fields: dict) -> dict:
+        """Function/vel_ssh.f90:794-1027 on host arrays: ``fields`` holds uu,
+        uup, vv, vvp and rhsx, rhsy, rhsx_tran, rhsy_tran (nz, ny, nx), ww
+        and nu (nz+1, ny, nx), the 2-D pgrx, pgry, rdis, tx_surf, ty_surf,
+        tx_bot, ty_bot, the depths hhu .. hhh, and dz[nz], hzt[nz+1].
+        Returns the updated uu, uup, vv, vvp."""
+        out = {k: np.array(fields[k], dtype=np.float64, order="C") for k in ("uu", "uup", "vv", "vvp")}
+        nz = out["uu"].shape[0]
+        s3, sw, s2 = (nz, self.ny, self.nx), (nz + 1, self.ny, self.nx), (self.ny, self.nx)
+        arr = {}
+        for keys, shape in ((self.BAROCLINIC_3D, s3), (self.BAROCLINIC_W, sw),
+                            (self.BAROCLINIC_2D + self.BAROCLINIC_DEPTHS, s2)):
+            for k in keys:
+                arr[k] = np.ascontiguousarray(fields[k], dtype=np.float64)
+                if arr[k].shape != shape:
+                    raise ValueError(f"{k} must have shape {shape}")
+        for k in out:
+            if out[k].shape != s3:
+                raise ValueError(f"{k} must have shape {s3}")
+        dz = np.ascontiguousarray(fields["dz"], dtype=np.float64)
+        hzt = np.ascontiguousarray(fields["hzt"], dtype=np.float64)
+        if dz.shape != (nz,) or hzt.shape != (nz + 1,):
+            raise ValueError("dz must have nz values and hzt nz + 1")
+        check(lib().inmsom_bt_baroclinic_dynamics(
+            self._h, float(tau), nz, *[_d(out[k]) for k in ("uu", "uup", "vv", "vvp")],
+            *[_d(arr[k]) for k in self.BAROCLINIC_W + self.BAROCLINIC_3D + self.BAROCLINIC_2D],
+            _d(dz), _d(hzt), *[_d(arr[k]) for k in self.BAROCLINIC_DEPTHS]), "inmsom_bt_baroclinic_dynamics")
+        return out
+
+    def baroclinic_dynamics_dev(self, tau: float, fields: dict):
+        """As baroclinic_dynamics on DeviceArray fields (dz, hzt host
+        sequences); uu, uup, vv, vvp updated in place."""
+        dz = np.ascontiguousarray(fields["dz"], dtype=np.float64)
+        hzt = np.ascontiguousarray(fields["hzt"], dtype=np.float64)
+        nz = dz.shape[0]
+        s3, sw, s2 = (nz, self.ny, self.nx), (nz + 1, self.ny, self.nx), (self.ny, self.nx)
+        check(lib().inmsom_bt_baroclinic_dynamics_dev(
+            self._h, float(tau), nz, *[self._dev(fields[k], s3) for k in ("uu", "uup", "vv", "vvp")],
+            *[self._dev(fields[k], sw) for k in self.BAROCLINIC_W],
+            *[self._dev(fields[k], s3) for k in self.BAROCLINIC_3D],
+            *[self._dev(fields[k], s2) for k in self.BAROCLINIC_2D],
+            _d(dz), _d(hzt), *[self._dev(fields[k], s2) for k in self.BAROCLINIC_DEPTHS]),
+            "inmsom_bt_baroclinic_dynamics_dev")
+
+    def baroclinic_timed(self, tau: float, nz: int, nrep: int) -> float:
+        """HIP-event time (ms) of nrep repetitions of baroclinic_dynamics's
+        device work on the fields the last host-array call left staged."""
+        ms = C.c_double()
+        check(lib().inmsom_bt_baroclinic_timed(self._h, float(tau), int(nz), int(nrep), C.byref(ms)),
+              "inmsom_bt_baroclinic_timed")
+        return ms.value
+
     def forcing_3d_timed(self, nz: int, nrep: int) -> float:
         """HIP-event time (ms) of nrep repetitions of the forcing's device
         work on the fields the last forcing_3d call left on the device."""
