@@ -69,6 +69,9 @@ TRAFFIC_KERNELS = {"fp64": ("k_fused_step2<double, false, true>",),
 # kernels of one 2-D -> 3-D coupling and their launches per coupling
 COUPLING_SEQUENCE = {"k_barotropic_part(": 2, "k_internal_uv(": 1, "k_ssh_internal(ibt::DevGrid, double": 1,
                      "k_ssh_internal_shift(": 1, "k_hh_update(": 1}
+# kernels of one baroclinic_dynamics call (nz = 20)
+BAROCLINIC_SEQUENCE = {"k_bc_solve<20>": 1, "k_bc_update(": 1}
+SEQUENCES = {"coupling": COUPLING_SEQUENCE, "baroclinic": BAROCLINIC_SEQUENCE}
 
 
 def write_traffic(fetch_path, write_path, out=TRAFFIC_JSON):
@@ -128,19 +131,18 @@ def write_traffic(fetch_path, write_path, out=TRAFFIC_JSON):
         w = sum(pw[0][0] * pf[0][1] for pf, pw in parts) / nf
         res[mode] = dict(src, kernel=" + ".join(pf[1] for pf, _ in parts), dispatches=nf, fetch_kib=f,
                          write_kib=w, fetch_bytes_per_launch=f * ff * 1024, write_bytes_per_launch=w * wf * 1024)
-    try:
-        parts = {}
-        for key, times in COUPLING_SEQUENCE.items():
-            (f, nf), name = find(F, key)
-            (w, _), _ = find(W, key)
-            parts[name] = {"per_coupling": times, "dispatches": nf, "fetch_kib": f, "write_kib": w}
-        res["coupling"] = dict(src, kernels=parts,
-                               fetch_bytes_per_launch=sum(p["per_coupling"] * p["fetch_kib"] for p in parts.values())
-                               * ff * 1024,
-                               write_bytes_per_launch=sum(p["per_coupling"] * p["write_kib"] for p in parts.values())
-                               * wf * 1024)
-    except KeyError:
-        pass
+    for leg, seq in SEQUENCES.items():
+        try:
+            parts = {}
+            for key, times in seq.items():
+                (f, nf), name = find(F, key)
+                (w, _), _ = find(W, key)
+                parts[name] = {"per_call": times, "dispatches": nf, "fetch_kib": f, "write_kib": w}
+        except KeyError:
+            continue
+        res[leg] = dict(src, kernels=parts,
+                        fetch_bytes_per_launch=sum(p["per_call"] * p["fetch_kib"] for p in parts.values()) * ff * 1024,
+                        write_bytes_per_launch=sum(p["per_call"] * p["write_kib"] for p in parts.values()) * wf * 1024)
     with open(out, "w") as fo:
         json.dump(res, fo, indent=1)
     return res
@@ -412,6 +414,77 @@ def forcing_cpu(b, f):
                       "(bto_forcing_3d, one thread)"}
 
 
+# Algorithmic bytes of baroclinic_dynamics per array cell: uu, uup, vv,
+# vvp read and written and RHSx, RHSy, RHSx_tran, RHSy_tran read (96 B x
+# nz), ww and nu read (16 B x (nz + 1)); 2-D: pgrx, pgry, rdis, four
+# stresses, seven depths (112 B), eight metrics and rlh_s (72 B), lcu, lcv
+# as float (8 B).
+BAROCLINIC_BYTES_NZ = 96 + 16
+BAROCLINIC_BYTES_2D = 16 + 112 + 72 + 8
+
+
+def baroclinic_fields(b, nz, seed=13):
+    rng = np.random.default_rng(seed)
+    sh = (b.ny, b.nx)
+    f = {k: 0.1 * rng.standard_normal((nz,) + sh) for k in ("uu", "uup", "vv", "vvp")}
+    f["ww"] = 1e-4 * rng.standard_normal((nz + 1,) + sh)
+    f["nu"] = 1e-3 + 1e-2 * rng.random((nz + 1,) + sh)
+    for k in ("rhsx", "rhsy", "rhsx_tran", "rhsy_tran"):
+        f[k] = 1e3 * rng.standard_normal((nz,) + sh)
+    for k in ("pgrx", "pgry"):
+        f[k] = 1e3 * rng.standard_normal(sh)
+    f["rdis"] = 1e-5 * rng.random(sh)
+    for k in ("tx_surf", "ty_surf", "tx_bot", "ty_bot"):
+        f[k] = 1e-4 * rng.standard_normal(sh)
+    for k in ("hhu", "hhup", "hhun", "hhv", "hhvp", "hhvn", "hhh"):
+        f[k] = 40.0 + rng.random(sh)
+    dz = 0.5 + rng.random(nz)
+    f["dz"] = dz / dz.sum()
+    f["hzt"] = (0.5 + rng.random(nz + 1)) / nz
+    return f
+
+
+def baroclinic_leg(m, b, nz, nrep, tau=40.0):
+    """baroclinic_dynamics (Function/vel_ssh.f90:794-1027) on the same basin
+    with nz levels: device time (HIP events on the library's stream) and
+    the whole host-array call (PCIe included)."""
+    f = baroclinic_fields(b, nz)
+    m.baroclinic_dynamics(tau, f)                                   # stages the fields
+    m.synchronize()
+    t0 = time.perf_counter()
+    m.baroclinic_dynamics(tau, f)
+    host_s = time.perf_counter() - t0
+    m.baroclinic_timed(tau, nz, 2)                                  # warm-up
+    ms = m.baroclinic_timed(tau, nz, nrep)
+    avg_s = ms * 1e-3 / nrep
+    ncell = b.nx * b.ny
+    alg = (BAROCLINIC_BYTES_NZ * nz + BAROCLINIC_BYTES_2D) * ncell
+    achieved = alg / avg_s / 1e9
+    res = {"levels": nz, "array_cells": ncell,
+           "value": b.n_wet * nz / avg_s, "unit": "wet-cell levels of baroclinic_dynamics/s (device-resident)",
+           "device_ms": avg_s * 1e3, "host_call_ms": host_s * 1e3,
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_from_profiles("baroclinic"),
+                        "bytes_per_call": alg}}
+    log(f"baroclinic nz={nz}: device {avg_s * 1e3:.3f} ms ({achieved:.0f} GB/s), host-array call "
+        f"{host_s * 1e3:.1f} ms")
+    return res, f
+
+
+def baroclinic_cpu(b, f, tau=40.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    og = oracle.OracleGrid(b)
+    t0 = time.perf_counter()
+    oracle.baroclinic(og, tau, f)
+    dt = time.perf_counter() - t0
+    nz = f["uu"].shape[0]
+    return {"value": b.n_wet * nz / dt, "unit": "wet-cell levels of baroclinic_dynamics/s", "cores": 1,
+            "kind": "port", "sample": f"one call on the same basin with {nz} levels, {dt:.2f} s, "
+                                      "oracle/bt_oracle.c (bto_baroclinic_dynamics, one thread)"}
+
+
 def snapshot_leg(m, b, count):
     """The state written in the reference's output formats (ssh, ubrtr,
     vbrtr as DIOGIN float32 records with GrADS descriptors, one record per
@@ -577,6 +650,9 @@ def main():
     ap.add_argument("--forcing-levels", type=int, default=COUPLING_NZ,
                     help="levels of the forcing leg (3-D model -> barotropic RHS; 0: skip it)")
     ap.add_argument("--forcing-reps", type=int, default=20)
+    ap.add_argument("--baroclinic-levels", type=int, default=COUPLING_NZ,
+                    help="levels of the baroclinic_dynamics leg (0: skip it)")
+    ap.add_argument("--baroclinic-reps", type=int, default=20)
     ap.add_argument("--snapshots", type=int, default=3,
                     help="snapshots of the state written in the reference's output formats (0: skip)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH", "WRITE"),
@@ -608,6 +684,9 @@ def main():
         forcing = None
         if args.forcing_levels > 0:
             forcing, forcing_inp = forcing_leg(m, b, args.forcing_levels, args.forcing_reps)
+        baroclinic = None
+        if args.baroclinic_levels > 0:
+            baroclinic, baroclinic_inp = baroclinic_leg(m, b, args.baroclinic_levels, args.baroclinic_reps)
         snapshot = snapshot_leg(m, b, args.snapshots) if args.snapshots > 0 else None
 
     errs = {}
@@ -658,6 +737,11 @@ def main():
             forcing["cpu_baseline"] = forcing_cpu(b, forcing_inp)
         del forcing_inp
         line["forcing"] = forcing
+    if baroclinic is not None:
+        if not args.no_cpu_baseline:
+            baroclinic["cpu_baseline"] = baroclinic_cpu(b, baroclinic_inp)
+        del baroclinic_inp
+        line["baroclinic"] = baroclinic
     if snapshot is not None:
         line["snapshot"] = snapshot
     if args.ensemble_steps > 0:

@@ -12,7 +12,10 @@
 // buffer because the next column's Coriolis term reads the old uu, vv of
 // its neighbours.  k_bc_update: the reference's update loop over the
 // interior and its one-point ring, un taken at the point the periodic wrap
-// copies it from.  Reference expressions and operation order throughout
+// copies it from.  (Applying the filter of uup, vvp inside the solve, which
+// saves the update a read and a write of them, was 1.5 ms slower at
+// 2048 x 1536 x 20: the old levels held for it cost the sweep its load
+// batching.)  Reference expressions and operation order throughout
 // (-ffp-contract=off): bit-identical to the reference.
 #include <string>
 
@@ -28,6 +31,7 @@ struct BcParams {
     DevGrid g;
     int nz;
     double tau;
+    int tiles_x, tiles_y;  // 64 x 4 tiles of the solve over the interior
     const double *dz, *hzt;                    // nz, nz + 1 (device)
     double *uu, *uup, *vv, *vvp;               // (nz, ny, nx), updated
     const double *ww, *nu;                     // (nz + 1, ny, nx)
@@ -95,6 +99,8 @@ __device__ void solve_u(const BcParams &p, int m, int n)
             y[k - 1] = (e - a * y[k - 2]) * scr;
         }
     }
+    // backward sweep; the filtered level of this point is read by no other
+    // column, so its Robert-Asselin step (vel_ssh.f90:1006-1007) is done here
     double r = y[nz - 1];
     p.un[(size_t)(nz - 1) * pl + i] = r;
 #pragma unroll UR
@@ -169,12 +175,27 @@ __device__ void solve_v(const BcParams &p, int m, int n)
     }
 }
 
+// Tiles in row-major order, dealt to the eight XCDs (block b to XCD b % 8)
+// as contiguous runs, so the row bands one XCD holds at once are
+// neighbours and the rows n - 1, n + 1 the columns read are shared in
+// its L2.
+__device__ __forceinline__ void bc_tile(int tx, int ty, int b, int &bx, int &by)
+{
+    const int T = tx * ty, xcd = b % 8, q = b / 8;
+    const int per = T / 8, extra = T % 8;
+    const int pos = xcd * per + (xcd < extra ? xcd : extra) + q;
+    by = pos / tx;
+    bx = pos % tx;
+}
+
 template <int NZ>
 __global__ __launch_bounds__(256) void k_bc_solve(BcParams p)
 {
     const DevGrid &g = p.g;
-    const int m = blockIdx.x * blockDim.x + threadIdx.x + 3;
-    const int n = blockIdx.y * blockDim.y + threadIdx.y + 3;
+    int bx, by;
+    bc_tile(p.tiles_x, p.tiles_y, (int)blockIdx.x, bx, by);
+    const int m = bx * 64 + (int)(threadIdx.x & 63) + 3;
+    const int n = by * 4 + (int)(threadIdx.x >> 6) + 3;
     if (m > g.nx - 2 || n > g.ny - 2) return;
     const size_t i = I(m, n);
     if (g.lcu[i] > 0.5f) solve_u<NZ>(p, m, n);
@@ -296,14 +317,17 @@ BcParams staged(inmsom_bt_ctx *c, int nz, double tau)
 
 void launch(inmsom_bt_ctx *c, const BcParams &p)
 {
-    const dim3 blk(64, 4);
-    const dim3 gs((unsigned)((c->nx - 4 + 63) / 64), (unsigned)((c->ny - 4 + 3) / 4));
-    if (p.nz == 20)
-        hipLaunchKernelGGL(k_bc_solve<20>, gs, blk, 0, c->stream, p);
+    BcParams q = p;
+    q.tiles_x = (c->nx - 4 + 63) / 64;
+    q.tiles_y = (c->ny - 4 + 3) / 4;
+    const dim3 gs((unsigned)(q.tiles_x * q.tiles_y));
+    if (q.nz == 20)
+        hipLaunchKernelGGL(k_bc_solve<20>, gs, dim3(256), 0, c->stream, q);
     else
-        hipLaunchKernelGGL(k_bc_solve<0>, gs, blk, 0, c->stream, p);
+        hipLaunchKernelGGL(k_bc_solve<0>, gs, dim3(256), 0, c->stream, q);
+    const dim3 blk(64, 4);
     const dim3 gu((unsigned)((c->nx - 2 + 63) / 64), (unsigned)((c->ny - 2 + 3) / 4));
-    hipLaunchKernelGGL(k_bc_update, gu, blk, 0, c->stream, p, c->g.ts);
+    hipLaunchKernelGGL(k_bc_update, gu, blk, 0, c->stream, q, c->g.ts);
 }
 
 int check(inmsom_bt_ctx *c, int nz, double tau)
