@@ -6,7 +6,7 @@
 ! reference prints (time_barotrop).
 !
 ! usage: inmsom_bt_host <par dir> <case.bin> <out.bin> [literal|fp64_exact|fp64|fp32] [g] [coupling.bin|-]
-!                       [forcing.bin]
+!                       [forcing.bin|-] [baroclinic.bin]
 !
 !   ocean_run.par  line 2 time_step [s], line 3 run_duration [days],
 !                  line 9 nstep_barotrop                 (inmsom_head.f90:47-55)
@@ -27,7 +27,15 @@
 !                  vv2d, uup2d, vvp2d, amuv2d, hhq, hhu, hhv, hhh, slpr,
 !                  RHSx2d, RHSy2d (nx x ny).  The host forms the forcing of
 !                  the barotropic step (Control/dyncall.f90:37-130) with the
-!                  case's wind stress and appends RHSx2d, RHSy2d to out.bin.
+!                  case's wind stress and appends RHSx2d, RHSy2d to out.bin
+!                  ('-': none).
+!   baroclinic.bin optional: int32 nz, real8 tau, dz(nz), hzt(nz+1), then
+!                  uu, uup, vv, vvp, RHSx, RHSy, RHSx_tran, RHSy_tran
+!                  (nx x ny x nz), ww, anzu (nx x ny x (nz+1)), pgrx, pgry,
+!                  r_diss, tx_surf, ty_surf, tx_bot, ty_bot, hhu, hhup, hhun,
+!                  hhv, hhvp, hhvn, hhh (nx x ny).  The host runs
+!                  baroclinic_dynamics (Function/vel_ssh.f90:794-1027) and
+!                  appends uu, uup, vv, vvp to out.bin.
 !
 ! Mode 'literal' runs the stage-by-stage device path (any metrics, the
 ! biharmonic term); the fused modes need uniform metrics.
@@ -36,7 +44,7 @@ use iso_c_binding
 use inmsom_bt
 implicit none
 
-character(512) :: pardir, case_path, out_path, arg, cpl_path, frc_path
+character(512) :: pardir, case_path, out_path, arg, cpl_path, frc_path, bcl_path
 character(256) :: lines(64)
 character(16) :: mode_name
 integer :: nl, ierr, magic, version, nx, ny, nsteps_case, nstep_case, ksw4_case, cmode, nout
@@ -50,6 +58,8 @@ real(8), allocatable, target :: rdis(:,:)
 real(8), allocatable, target :: c2(:,:,:), uu(:,:,:), vv(:,:,:)
 ! forcing: f3 = uu vv uup vvp amuv pgx pgy; f2 = uu2d vv2d uup2d vvp2d amuv2d hhq hhu hhv hhh slpr RHSx2d RHSy2d
 real(8), allocatable, target :: f3(:,:,:,:), f2(:,:,:), dzv(:)
+! baroclinic: b3 = uu uup vv vvp RHSx RHSy RHSx_tran RHSy_tran; bw = ww anzu; b2 = 14 planes
+real(8), allocatable, target :: b3(:,:,:,:), bw(:,:,:,:), b2(:,:,:), bdz(:), bhzt(:)
 integer(c_long_long) :: bad
 integer(8) :: c0, c1, crate
 real(8) :: time_barotrop, vol0, vol1
@@ -171,8 +181,15 @@ if (command_argument_count() >= 6) then
 endif
 if (command_argument_count() >= 7) then
   call get_command_argument(7, frc_path)
-  call forcing_step(trim(frc_path))
-  write(23) f2(:,:,11:12)
+  if (trim(frc_path) /= '-') then
+    call forcing_step(trim(frc_path))
+    write(23) f2(:,:,11:12)
+  endif
+endif
+if (command_argument_count() >= 8) then
+  call get_command_argument(8, bcl_path)
+  call baroclinic_step(trim(bcl_path))
+  write(23) b3(:,:,:,1:4)
 endif
 close(23)
 write(*,'(a,es12.5)') ' Time barotropic: ', time_barotrop
@@ -228,6 +245,31 @@ subroutine forcing_step(path)
                       f(:,:,11), f(:,:,12), f2(:,:,10), f2(:,:,11), f2(:,:,12), f3(:,:,:,6), f3(:,:,:,7))
   write(*,'(a,i0,a)') ' forcing step done (', nzf, ' levels)'
 end subroutine forcing_step
+
+! baroclinic_dynamics (Function/vel_ssh.f90:794-1027) on the fields of `path`.
+subroutine baroclinic_step(path)
+  character(*), intent(in) :: path
+  integer :: io, nzb
+  real(8) :: tau_b
+  open(26, file=path, access='stream', form='unformatted', status='old', iostat=io)
+  if (io /= 0) then
+    write(*,*) 'cannot open ', path
+    error stop 5
+  endif
+  read(26) nzb
+  read(26) tau_b
+  allocate(bdz(nzb), bhzt(nzb + 1), b3(nx, ny, nzb, 8), bw(nx, ny, nzb + 1, 2), b2(nx, ny, 14))
+  read(26) bdz, bhzt
+  read(26) b3
+  read(26) bw
+  read(26) b2
+  close(26)
+  call baroclinic_dynamics_gpu(tau_b, nzb, b3(:,:,:,1), b3(:,:,:,2), b3(:,:,:,3), b3(:,:,:,4), bw(:,:,:,1), &
+                               bw(:,:,:,2), b3(:,:,:,5), b3(:,:,:,6), b3(:,:,:,7), b3(:,:,:,8), b2(:,:,1), &
+                               b2(:,:,2), b2(:,:,3), b2(:,:,4), b2(:,:,5), b2(:,:,6), b2(:,:,7), bdz, bhzt, &
+                               b2(:,:,8), b2(:,:,9), b2(:,:,10), b2(:,:,11), b2(:,:,12), b2(:,:,13), b2(:,:,14))
+  write(*,'(a,i0,a)') ' baroclinic step done (', nzb, ' levels)'
+end subroutine baroclinic_step
 
 ! Service/read_write_parameters.f90:2-41: the file's lines, one per entry.
 subroutine read_lines(path, buf, n)

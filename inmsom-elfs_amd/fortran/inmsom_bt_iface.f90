@@ -221,6 +221,27 @@ interface
     integer(c_int), value :: nz, nrep
     real(c_double), intent(out) :: ms
   end function
+  ! baroclinic_dynamics (Function/vel_ssh.f90:794-1027)
+  integer(c_int) function inmsom_bt_baroclinic_dynamics(ctx, tau, nz, uu, uup, vv, vvp, ww, nu, rhsx, rhsy, &
+      rhsx_tran, rhsy_tran, pgrx, pgry, rdis, tx_surf, ty_surf, tx_bot, ty_bot, dz, hzt, hhu, hhup, hhun, &
+      hhv, hhvp, hhvn, hhh) bind(C, name="inmsom_bt_baroclinic_dynamics")
+    import :: c_int, c_double, c_ptr
+    type(c_ptr), value :: ctx
+    real(c_double), value :: tau
+    integer(c_int), value :: nz
+    type(c_ptr), value :: uu, uup, vv, vvp, ww, nu, rhsx, rhsy, rhsx_tran, rhsy_tran, pgrx, pgry, rdis
+    type(c_ptr), value :: tx_surf, ty_surf, tx_bot, ty_bot, dz, hzt, hhu, hhup, hhun, hhv, hhvp, hhvn, hhh
+  end function
+  integer(c_int) function inmsom_bt_baroclinic_dynamics_dev(ctx, tau, nz, uu, uup, vv, vvp, ww, nu, rhsx, &
+      rhsy, rhsx_tran, rhsy_tran, pgrx, pgry, rdis, tx_surf, ty_surf, tx_bot, ty_bot, dz, hzt, hhu, hhup, &
+      hhun, hhv, hhvp, hhvn, hhh) bind(C, name="inmsom_bt_baroclinic_dynamics_dev")
+    import :: c_int, c_double, c_ptr
+    type(c_ptr), value :: ctx
+    real(c_double), value :: tau
+    integer(c_int), value :: nz
+    type(c_ptr), value :: uu, uup, vv, vvp, ww, nu, rhsx, rhsy, rhsx_tran, rhsy_tran, pgrx, pgry, rdis
+    type(c_ptr), value :: tx_surf, ty_surf, tx_bot, ty_bot, dz, hzt, hhu, hhup, hhun, hhv, hhvp, hhvn, hhh
+  end function
   function c_strlen(s) bind(C, name="strlen")
     import :: c_ptr, c_size_t
     type(c_ptr), value :: s
@@ -323,5 +344,25 @@ subroutine forcing_3d_gpu(nz, dz, uu, vv, uup, vvp, amuv, uu2d, vv2d, uup2d, vvp
                 c_loc(hhq), c_loc(hhu), c_loc(hhv), c_loc(hhh), c_loc(taux), c_loc(tauy), c_loc(slpr), px, py, &
                 c_loc(RHSx2d), c_loc(RHSy2d)), 'forcing_3d_gpu')
 end subroutine forcing_3d_gpu
+
+! Function/vel_ssh.f90:794-1027 on the context bt_ctx with the reference's
+! argument list, followed by the module arrays it reads: the vertical grid
+! dz(nz), hzt(nz+1) and the 3-D model's depths hhu .. hhh.  uu, uup, vv,
+! vvp are updated in place.
+subroutine baroclinic_dynamics_gpu(tau, nz, uu, uup, vv, vvp, ww, anzu, RHSx, RHSy, RHSx_tran, RHSy_tran, &
+                                   pgrx, pgry, r_diss, tx_surf, ty_surf, tx_bot, ty_bot, dz, hzt, hhu, hhup, &
+                                   hhun, hhv, hhvp, hhvn, hhh)
+  real(8), intent(in) :: tau
+  integer, intent(in) :: nz
+  real(8), target, intent(inout) :: uu(*), uup(*), vv(*), vvp(*)
+  real(8), target, intent(in) :: ww(*), anzu(*), RHSx(*), RHSy(*), RHSx_tran(*), RHSy_tran(*)
+  real(8), target, intent(in) :: pgrx(*), pgry(*), r_diss(*), tx_surf(*), ty_surf(*), tx_bot(*), ty_bot(*)
+  real(8), target, intent(in) :: dz(*), hzt(*), hhu(*), hhup(*), hhun(*), hhv(*), hhvp(*), hhvn(*), hhh(*)
+  call bt_check(inmsom_bt_baroclinic_dynamics(bt_ctx, real(tau, c_double), int(nz, c_int), c_loc(uu), &
+                c_loc(uup), c_loc(vv), c_loc(vvp), c_loc(ww), c_loc(anzu), c_loc(RHSx), c_loc(RHSy), &
+                c_loc(RHSx_tran), c_loc(RHSy_tran), c_loc(pgrx), c_loc(pgry), c_loc(r_diss), c_loc(tx_surf), &
+                c_loc(ty_surf), c_loc(tx_bot), c_loc(ty_bot), c_loc(dz), c_loc(hzt), c_loc(hhu), c_loc(hhup), &
+                c_loc(hhun), c_loc(hhv), c_loc(hhvp), c_loc(hhvn), c_loc(hhh)), 'baroclinic_dynamics_gpu')
+end subroutine baroclinic_dynamics_gpu
 
 end module inmsom_bt

@@ -154,3 +154,51 @@ def test_fortran_host_forcing_bitwise():
     got = dict(zip(("rhsx2d", "rhsy2d"), raw[6 * per:].reshape(2, b.ny, b.nx)))
     assert "forcing step done (6 levels)" in p.stdout
     assert_bitwise(got, want, ("rhsx2d", "rhsy2d"), "fortran host forcing")
+
+
+def test_fortran_host_baroclinic_bitwise():
+    """The host's baroclinic step (baroclinic_dynamics_gpu through the
+    ISO_C_BINDING module, Function/vel_ssh.f90:794-1027) against the oracle,
+    bit for bit."""
+    b = randomize_state(make_basin(66, 48, seed=37, sigma=5.0, metrics="stretched"), 4)
+    rd = oracle.OracleGrid(b).rayleigh()
+    nz, tau = 20, 40.0
+    rng = np.random.default_rng(29)
+    sh = (b.ny, b.nx)
+    f = {k: 0.1 * rng.standard_normal((nz,) + sh) for k in ("uu", "uup", "vv", "vvp")}
+    for k in ("rhsx", "rhsy", "rhsx_tran", "rhsy_tran"):
+        f[k] = 1e3 * rng.standard_normal((nz,) + sh)
+    f["ww"] = 1e-4 * rng.standard_normal((nz + 1,) + sh)
+    f["nu"] = 1e-3 + 1e-2 * rng.random((nz + 1,) + sh)
+    two = ("pgrx", "pgry", "rdis", "tx_surf", "ty_surf", "tx_bot", "ty_bot", "hhu", "hhup", "hhun", "hhv", "hhvp",
+           "hhvn", "hhh")
+    for k in two[:7]:
+        f[k] = 1e-3 * rng.standard_normal(sh)
+    f["rdis"] = 1e-5 * rng.random(sh)
+    for k in two[7:]:
+        f[k] = 40.0 + rng.random(sh)
+    dz = 0.5 + rng.random(nz)
+    f["dz"] = dz / dz.sum()
+    f["hzt"] = (0.5 + rng.random(nz + 1)) / nz
+    want = oracle.baroclinic(oracle.OracleGrid(b), tau, f)
+    with tempfile.TemporaryDirectory() as tmp:
+        case = os.path.join(tmp, "case.bin")
+        bcl = os.path.join(tmp, "baroclinic.bin")
+        out = os.path.join(tmp, "out.bin")
+        write_case(case, b, rd, 1, 2, 0, 0, [])
+        with open(bcl, "wb") as fh:
+            np.array([nz], dtype="<i4").tofile(fh)
+            np.array([tau], dtype="<f8").tofile(fh)
+            f["dz"].astype("<f8").tofile(fh)
+            f["hzt"].astype("<f8").tofile(fh)
+            for k in ("uu", "uup", "vv", "vvp", "rhsx", "rhsy", "rhsx_tran", "rhsy_tran", "ww", "nu") + two:
+                np.ascontiguousarray(f[k], dtype="<f8").tofile(fh)
+        p = subprocess.run([HOST, PARDIR, case, out, "literal", repr(b.g), "-", "-", bcl], capture_output=True,
+                           text=True, timeout=120)
+        print(p.stdout, p.stderr)
+        assert p.returncode == 0, p.stdout + p.stderr
+        raw = np.fromfile(out, dtype="<f8")
+    per = b.nx * b.ny
+    got = dict(zip(("uu", "uup", "vv", "vvp"), raw[6 * per:].reshape(4, nz, b.ny, b.nx)))
+    assert "baroclinic step done (20 levels)" in p.stdout
+    assert_bitwise(got, want, ("uu", "uup", "vv", "vvp"), "fortran host baroclinic")
