@@ -970,6 +970,33 @@ void bto_baroclinic_dynamics(const bto_grid *g, int nz, double tau, const double
     free(rksi);
 }
 
+/* Function/vel_ssh.f90:1030-1076 (vertical_velocity): w from the
+ * continuity equation, integrated from the bottom (w(nz+1) = 0) on lu
+ * interior columns; then the periodic wrap of all nz+1 levels.  u, v
+ * (nz, ny, nx), w (nz+1, ny, nx) read and written (cells not computed
+ * keep their values). */
+void bto_vertical_velocity(const bto_grid *g, int nz, const double *dz, const double *u, const double *v, double *w,
+                           const double *hu, const double *hv, const double *wflux)
+{
+    const size_t sz = (size_t)g->nx * g->ny;
+    const double ffs = (double)g->full_free_surface;
+    for (int n = NYS; n <= NYE; ++n)
+        for (int m = NXS; m <= NXE; ++m) {
+            const size_t i = IX(m, n), iw = IX(m - 1, n), is = IX(m, n - 1);
+            if (g->lu[i] > 0.5f) {
+                w[(size_t)nz * sz + i] = 0.0;
+                for (int k = nz; k >= 1; --k) {
+                    const size_t o = (size_t)(k - 1) * sz;
+                    w[o + i] = w[o + sz + i] + dz[k - 1] / (g->dx[i] * g->dy[i]) *
+                                                   (u[o + i] * hu[i] * g->dyh[i] - u[o + iw] * hu[iw] * g->dyh[iw] +
+                                                    v[o + i] * hv[i] * g->dxh[i] - v[o + is] * hv[is] * g->dxh[is] +
+                                                    wflux[i] / g->ref_den * ffs);
+                }
+            }
+        }
+    for (int k = 0; k <= nz; ++k) cyclize(g, w + (size_t)k * sz);
+}
+
 void bto_model_step(const bto_grid *g, bto_depths *d, double tau, int nstep,
                     int ksw4, double cb_nl, double ebottom, double *ubrtr_e,
                     double *ubrtrp_e, double *vbrtr_e, double *vbrtrp_e,

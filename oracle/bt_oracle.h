@@ -179,6 +179,11 @@ void bto_baroclinic_dynamics(const bto_grid *g, int nz, double tau, const double
                              const double *hhu, const double *hhup, const double *hhun, const double *hhv,
                              const double *hhvp, const double *hhvn, const double *hhh);
 
+/* Function/vel_ssh.f90:1030-1076 (vertical_velocity): u, v (nz, ny, nx),
+ * w (nz+1, ny, nx) read and written, dz[nz]; hu, hv, wflux (ny, nx). */
+void bto_vertical_velocity(const bto_grid *g, int nz, const double *dz, const double *u, const double *v, double *w,
+                           const double *hu, const double *hv, const double *wflux);
+
 /* Wet-cell volume anomaly sum(eta * dx * dy) over lu interior cells,
  * summed row by row in a fixed order. */
 double bto_volume(const bto_grid *g, const double *ssh);

@@ -293,6 +293,17 @@ def baroclinic(og: "OracleGrid", tau: float, inp: Dict[str, np.ndarray]) -> Dict
     return {k: w[k] for k in BAROCLINIC_OUT}
 
 
+def vertical_velocity(og: "OracleGrid", inp: Dict[str, np.ndarray]) -> np.ndarray:
+    """bto_vertical_velocity on copies of ``inp`` (dz, uu, vv (nz, ny, nx),
+    ww (nz+1, ny, nx), hu, hv, wflux); returns the new ww."""
+    L = lib()
+    w = {k: np.ascontiguousarray(inp[k], dtype=np.float64).copy() for k in inp}
+    nz = w["uu"].shape[0]
+    L.bto_vertical_velocity(C.byref(og.g), nz, _d(w["dz"]), _d(w["uu"]), _d(w["vv"]), _d(w["ww"]), _d(w["hu"]),
+                            _d(w["hv"]), _d(w["wflux"]))
+    return w["ww"]
+
+
 def have_reference() -> bool:
     return os.path.exists(REF_HARNESS)
 
@@ -305,6 +316,23 @@ def run_reference(case_path: str, out_path: str, nx: int, ny: int, mode: int,
                    stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, cwd=os.path.dirname(out_path))
     raw = np.fromfile(out_path, dtype="<f8")
     per = nx * ny
+    if mode == 5:
+        # masks, dz (nz), uu, vv (nz planes), ww (nz+1), hu, hv, wflux, ww out (nz+1)
+        nz = (raw.size - 7 * per - 3 * per - 2 * per) // (1 + 4 * per)
+        res = {"masks": {k: raw[i * per:(i + 1) * per].reshape(ny, nx) for i, k in enumerate(MASKS)}}
+        o = 7 * per
+        inp = {"dz": raw[o:o + nz].copy()}
+        o += nz
+        for k, n_ in (("uu", nz), ("vv", nz), ("ww", nz + 1)):
+            inp[k] = raw[o:o + n_ * per].reshape(n_, ny, nx).copy()
+            o += n_ * per
+        for k in ("hu", "hv", "wflux"):
+            inp[k] = raw[o:o + per].reshape(ny, nx).copy()
+            o += per
+        res["vv_in"] = inp
+        res["ww_out"] = raw[o:o + (nz + 1) * per].reshape(nz + 1, ny, nx).copy()
+        assert o + (nz + 1) * per == raw.size
+        return res
     if mode == 4:
         # masks, dz (nz), hzt (nz+1), 8 3-D inputs, 2 of nz+1 levels, 14 planes, 4 3-D outputs
         nin2 = len(BAROCLINIC_IN_2D)

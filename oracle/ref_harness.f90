@@ -45,6 +45,10 @@
 !   3-D fields at the compiled nz (the reference's own module depths from
 !   hh_init, a non-uniform vertical grid hzt, dz): the inputs are written,
 !   the routine is called, and uu, uup, vv, vvp are written.
+! mode 5: vertical_velocity (Function/vel_ssh.f90:1030-1077) on synthetic
+!   3-D velocities, the depths of hh_init and the case's wflux: the inputs
+!   (dz, uu, vv, the initial ww, hhu, hhv, wflux) and the resulting ww are
+!   written.
 program ref_harness
 use main_basin_pars
 use mpi_parallel_tools
@@ -327,6 +331,26 @@ else if (mode == 4) then
   call baroclinic_dynamics(tau, uu, uup, vv, vvp, ww, anzu, RHSx3d, RHSy3d, RHSx3d_tran, RHSy3d_tran, &
                            pgrx, pgry, r_diss, surf_stress_x, surf_stress_y, bot_stress_x, bot_stress_y)
   write(23) uu, uup, vv, vvp
+else if (mode == 5) then
+  ! vertical_velocity (Function/vel_ssh.f90:1030-1077) on synthetic 3-D
+  ! velocities at the compiled nz, the depths of hh_init and the case's
+  ! wflux; ww starts at values the routine must leave where it does not
+  ! write
+  call hh_init(hhq_e, hhqp_e, hhqn_e, hhu_e, hhup_e, hhun_e, hhv_e, hhvp_e, hhvn_e, &
+               hhh_e, hhhp_e, hhhn_e, ssh_e, sshp_e, hhq_rest)
+  do k = 1, nz
+    dz(k) = (0.5d0 + k)/(0.5d0*nz + 0.5d0*nz*(nz + 1))
+    uu(:,:,k) = (ubrtr_e*(1.0d0 + 0.05d0*k) + 0.01d0*k)*lcu
+    vv(:,:,k) = (vbrtr_e*(1.0d0 - 0.03d0*k) - 0.02d0*k)*lcv
+  enddo
+  do k = 1, nz + 1
+    ww(:,:,k) = 7.0d0 + 0.1d0*k + 1.0d-3*ssh_e
+  enddo
+  write(23) dz
+  write(23) uu, vv, ww
+  write(23) hhu_e, hhv_e, wflux
+  call vertical_velocity(uu, vv, ww, hhu_e, hhv_e, wflux)
+  write(23) ww
 else
   ! helper routines, one call each, on the case's state arrays
   ! hh_init from (ssh_e, sshp_e)

@@ -426,3 +426,35 @@ def test_baroclinic_viscosity_conserves_depth_integral_and_matches_dense_solve()
     want_p = inp["uu"] + ts * (out["uu"] - 2.0 * inp["uu"] + inp["uup"]) / 2.0
     cu = masks["lcu"] > 0.5
     assert np.allclose(out["uup"][:, cu], want_p[:, cu], rtol=1e-13, atol=1e-15)
+
+
+def test_vertical_velocity_closed_basin_balance():
+    """Continuity: in a closed basin the transport divergence sums to zero
+    over the wet cells, so sum(w(k=1) dx dy) over lu interior cells equals
+    sum(wflux / RefDen) * sum(dz) (full free surface); with every velocity
+    zero each column's w grows linearly from the bottom."""
+    b = make_basin(50, 36, seed=8, sigma=3.0, metrics="stretched")
+    og = oracle.OracleGrid(b)
+    rng = np.random.default_rng(8)
+    nz, sh = 6, (b.ny, b.nx)
+    lcu, lcv, lu = (og.masks[k] > 0.5 for k in ("lcu", "lcv", "lu"))
+    dz = 0.5 + rng.random(nz)
+    wflux = 1e-5 * rng.standard_normal(sh) * lu
+    inp = {"dz": dz, "uu": rng.standard_normal((nz,) + sh) * lcu, "vv": rng.standard_normal((nz,) + sh) * lcv,
+           "ww": np.zeros((nz + 1,) + sh), "hu": 30.0 + rng.random(sh), "hv": 30.0 + rng.random(sh),
+           "wflux": wflux}
+    w = oracle.vertical_velocity(og, inp)
+    inner = lu.copy()
+    inner[:2, :] = inner[-2:, :] = False
+    inner[:, :2] = inner[:, -2:] = False
+    total = (w[0] * b.dx * b.dy)[inner].sum()
+    want = (wflux[inner] / b.rho0).sum() * dz.sum()
+    scale = np.abs(inp["uu"]).max() * 30.0 * b.dyh.max() * dz.sum() * inner.sum()
+    assert abs(total - want) <= 1e-12 * scale
+    inp0 = dict(inp, uu=np.zeros_like(inp["uu"]), vv=np.zeros_like(inp["vv"]))
+    w0 = oracle.vertical_velocity(og, inp0)
+    cum = np.concatenate([np.cumsum(dz[::-1])[::-1], [0.0]])
+    rate = wflux / b.rho0 / (b.dx * b.dy)
+    for k in range(nz + 1):
+        assert np.allclose(w0[k][inner], (cum[k] * rate * b.dx * b.dy / (b.dx * b.dy))[inner], rtol=1e-12,
+                           atol=1e-30)

@@ -90,3 +90,22 @@ def test_baroclinic_bitwise(seed, metrics):
     got = oracle.baroclinic(oracle.OracleGrid(b), b.dt, inp)
     assert_bitwise(got, want, oracle.BAROCLINIC_OUT, f"baroclinic seed {seed} {metrics}")
     assert not np.array_equal(got["uu"], inp["uu"])
+
+
+@pytest.mark.parametrize("seed,metrics", [(11, "uniform"), (12, "stretched")])
+def test_vertical_velocity_bitwise(seed, metrics):
+    """vertical_velocity (Function/vel_ssh.f90:1030-1076) called by the
+    harness, against the oracle on the inputs it wrote; cells the routine
+    does not compute keep their initial values."""
+    b = make_basin(285, 159, seed=seed, sigma=4.0, g=9.8, hmax=120.0, metrics=metrics)
+    b = randomize_state(b, seed + 1, amp_uv=0.05, wflux_amp=1e-5)
+    rd = oracle.OracleGrid(b).rayleigh()
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "case.bin")
+        write_case(path, b, rd, 1, 2, 0, 5)
+        ref = oracle.run_reference(path, path + ".out", b.nx, b.ny, 5)
+    inp, want = ref["vv_in"], ref["ww_out"]
+    assert inp["uu"].shape[0] == 20 and want.shape[0] == 21
+    got = oracle.vertical_velocity(oracle.OracleGrid(b), inp)
+    assert_bitwise({"ww": got}, {"ww": want}, ("ww",), f"vertical velocity seed {seed} {metrics}")
+    assert np.any(want != inp["ww"]) and np.any(want == inp["ww"])
