@@ -485,6 +485,35 @@ def baroclinic_cpu(b, f, tau=40.0):
                                       "oracle/bt_oracle.c (bto_baroclinic_dynamics, one thread)"}
 
 
+# Algorithmic bytes of vertical_velocity per array cell: u, v read (16 B x
+# nz), w written (8 B x (nz + 1)); hu, hv, wflux, dx, dy, dxh, dyh read
+# (56 B) and lu as float (4 B).
+def vertical_leg(m, b, nz, nrep):
+    """vertical_velocity (Function/vel_ssh.f90:1030-1076) on the same basin
+    with nz levels: device time (HIP events) and the host-array call."""
+    rng = np.random.default_rng(17)
+    sh = (b.ny, b.nx)
+    dz = 0.5 + rng.random(nz)
+    f = {"dz": dz / dz.sum(), "uu": 0.1 * rng.standard_normal((nz,) + sh),
+         "vv": 0.1 * rng.standard_normal((nz,) + sh), "ww": np.zeros((nz + 1,) + sh),
+         "hu": 40.0 + rng.random(sh), "hv": 40.0 + rng.random(sh), "wflux": 1e-5 * rng.standard_normal(sh)}
+    args = (f["dz"], f["uu"], f["vv"], f["ww"], f["hu"], f["hv"], f["wflux"])
+    m.vertical_velocity(*args)
+    t0 = time.perf_counter()
+    m.vertical_velocity(*args)
+    host_s = time.perf_counter() - t0
+    m.vertical_velocity_timed(nz, 2)
+    avg_s = m.vertical_velocity_timed(nz, nrep) * 1e-3 / nrep
+    alg = (16 * nz + 8 * (nz + 1) + 60) * b.nx * b.ny
+    achieved = alg / avg_s / 1e9
+    log(f"vertical velocity nz={nz}: device {avg_s * 1e3:.3f} ms ({achieved:.0f} GB/s), host-array call "
+        f"{host_s * 1e3:.1f} ms")
+    return {"levels": nz, "value": b.n_wet * nz / avg_s, "unit": "wet-cell levels of vertical_velocity/s",
+            "device_ms": avg_s * 1e3, "host_call_ms": host_s * 1e3,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": achieved / PEAK_HBM_GBS, "traffic": None, "bytes_per_call": alg}}
+
+
 def snapshot_leg(m, b, count):
     """The state written in the reference's output formats (ssh, ubrtr,
     vbrtr as DIOGIN float32 records with GrADS descriptors, one record per
@@ -687,6 +716,8 @@ def main():
         baroclinic = None
         if args.baroclinic_levels > 0:
             baroclinic, baroclinic_inp = baroclinic_leg(m, b, args.baroclinic_levels, args.baroclinic_reps)
+        vertical = vertical_leg(m, b, args.baroclinic_levels, args.baroclinic_reps) \
+            if args.baroclinic_levels > 0 else None
         snapshot = snapshot_leg(m, b, args.snapshots) if args.snapshots > 0 else None
 
     errs = {}
@@ -742,6 +773,8 @@ def main():
             baroclinic["cpu_baseline"] = baroclinic_cpu(b, baroclinic_inp)
         del baroclinic_inp
         line["baroclinic"] = baroclinic
+    if vertical is not None:
+        line["vertical_velocity"] = vertical
     if snapshot is not None:
         line["snapshot"] = snapshot
     if args.ensemble_steps > 0:

@@ -373,6 +373,23 @@ int inmsom_bt_baroclinic_dynamics_dev(inmsom_bt_ctx *ctx, double tau, int nz, do
  * counterpart in the reference. */
 int inmsom_bt_baroclinic_timed(inmsom_bt_ctx *ctx, double tau, int nz, int nrep, double *ms);
 
+/* vertical_velocity, Function/vel_ssh.f90:1030-1076: w (nz+1, ny, nx) of
+ * the 3-D model from the continuity equation with u, v (nz, ny, nx), the
+ * depths hu, hv, the water flux wflux (ny, nx; times full_free_surface)
+ * and dz[nz], integrated from w(nz+1) = 0 on lu interior columns, then
+ * wrapped on periodic boundaries.  w is read and written (cells the
+ * reference does not compute keep their values).  Host arrays; the _dev
+ * variant takes device addresses (dz a host array) and returns after the
+ * work has finished. */
+int inmsom_bt_vertical_velocity(inmsom_bt_ctx *ctx, int nz, const double *dz, const double *u, const double *v,
+                                double *w, const double *hu, const double *hv, const double *wflux);
+int inmsom_bt_vertical_velocity_dev(inmsom_bt_ctx *ctx, int nz, const double *dz, const double *u, const double *v,
+                                    double *w, const double *hu, const double *hv, const double *wflux);
+/* Timing: nrep repetitions of the device work on the fields the last
+ * inmsom_bt_vertical_velocity call with nz levels staged; *ms = HIP-event
+ * time of all.  No counterpart in the reference. */
+int inmsom_bt_vertical_velocity_timed(inmsom_bt_ctx *ctx, int nz, int nrep, double *ms);
+
 /* Timing of the coupling's device work: nrep repetitions of the kernels of
  * inmsom_bt_remove_barotropic then inmsom_bt_internal_update on the fields
  * the last inmsom_bt_internal_update call left on the device (it must have

@@ -102,6 +102,10 @@ struct inmsom_bt_ctx {
     int bc_nz = 0;                 // levels of the staged fields (0 = none)
     double *bc_dev = nullptr;      // un, vn and the vertical grid of the device-address call
     size_t bc_dev_n = 0;           // doubles in bc_dev
+    double *vv = nullptr;          // staging of vertical_velocity (bt_vertical.hip)
+    size_t vv_n = 0;               // doubles in vv
+    int vv_nz = 0;                 // levels of the staged fields (0 = none)
+    double *vv_dz = nullptr;       // dz of the device-address call (up to 1000 levels)
     float *snap_dev = nullptr;     // packed snapshot fields (bt_snapshot.hip)
     float *snap_host = nullptr;    // pinned copy of snap_dev
     size_t snap_n = 0;             // floats in snap_dev / snap_host

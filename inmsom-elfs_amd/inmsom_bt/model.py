@@ -386,6 +386,41 @@ class BarotropicModel:
               "inmsom_bt_baroclinic_timed")
         return ms.value
 
+    def vertical_velocity(self, dz, uu, vv, ww, hu, hv, wflux) -> np.ndarray:
+        """Function/vel_ssh.f90:1030-1076 on host arrays: uu, vv (nz, ny, nx),
+        ww (nz+1, ny, nx) (cells not computed keep their values), hu, hv,
+        wflux (ny, nx), dz[nz].  Returns the new ww."""
+        u = np.ascontiguousarray(uu, dtype=np.float64)
+        v = np.ascontiguousarray(vv, dtype=np.float64)
+        nz = u.shape[0]
+        w = np.array(ww, dtype=np.float64, order="C")
+        if u.shape != (nz, self.ny, self.nx) or v.shape != u.shape or w.shape != (nz + 1, self.ny, self.nx):
+            raise ValueError("uu, vv must be (nz, ny, nx) and ww (nz+1, ny, nx)")
+        two = [np.ascontiguousarray(a, dtype=np.float64) for a in (hu, hv, wflux)]
+        dzc = np.ascontiguousarray(dz, dtype=np.float64)
+        if dzc.shape != (nz,):
+            raise ValueError("dz must have nz values")
+        check(lib().inmsom_bt_vertical_velocity(self._h, nz, _d(dzc), _d(u), _d(v), _d(w), *[_d(a) for a in two]),
+              "inmsom_bt_vertical_velocity")
+        return w
+
+    def vertical_velocity_dev(self, dz, uu, vv, ww, hu, hv, wflux):
+        """As vertical_velocity on DeviceArray arguments; ww updated in place."""
+        dzc = np.ascontiguousarray(dz, dtype=np.float64)
+        nz = dzc.shape[0]
+        s3, sw, s2 = (nz, self.ny, self.nx), (nz + 1, self.ny, self.nx), (self.ny, self.nx)
+        check(lib().inmsom_bt_vertical_velocity_dev(self._h, nz, _d(dzc), self._dev(uu, s3), self._dev(vv, s3),
+                                                    self._dev(ww, sw), self._dev(hu, s2), self._dev(hv, s2),
+                                                    self._dev(wflux, s2)), "inmsom_bt_vertical_velocity_dev")
+
+    def vertical_velocity_timed(self, nz: int, nrep: int) -> float:
+        """HIP-event time (ms) of nrep repetitions of vertical_velocity's
+        device work on the fields the last host-array call staged."""
+        ms = C.c_double()
+        check(lib().inmsom_bt_vertical_velocity_timed(self._h, int(nz), int(nrep), C.byref(ms)),
+              "inmsom_bt_vertical_velocity_timed")
+        return ms.value
+
     def forcing_3d_timed(self, nz: int, nrep: int) -> float:
         """HIP-event time (ms) of nrep repetitions of the forcing's device
         work on the fields the last forcing_3d call left on the device."""
