@@ -242,6 +242,21 @@ interface
     type(c_ptr), value :: uu, uup, vv, vvp, ww, nu, rhsx, rhsy, rhsx_tran, rhsy_tran, pgrx, pgry, rdis
     type(c_ptr), value :: tx_surf, ty_surf, tx_bot, ty_bot, dz, hzt, hhu, hhup, hhun, hhv, hhvp, hhvn, hhh
   end function
+  ! vertical_velocity (Function/vel_ssh.f90:1030-1076)
+  integer(c_int) function inmsom_bt_vertical_velocity(ctx, nz, dz, u, v, w, hu, hv, wflux) &
+      bind(C, name="inmsom_bt_vertical_velocity")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: nz
+    type(c_ptr), value :: dz, u, v, w, hu, hv, wflux
+  end function
+  integer(c_int) function inmsom_bt_vertical_velocity_dev(ctx, nz, dz, u, v, w, hu, hv, wflux) &
+      bind(C, name="inmsom_bt_vertical_velocity_dev")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: nz
+    type(c_ptr), value :: dz, u, v, w, hu, hv, wflux
+  end function
   function c_strlen(s) bind(C, name="strlen")
     import :: c_ptr, c_size_t
     type(c_ptr), value :: s
@@ -364,5 +379,16 @@ subroutine baroclinic_dynamics_gpu(tau, nz, uu, uup, vv, vvp, ww, anzu, RHSx, RH
                 c_loc(ty_surf), c_loc(tx_bot), c_loc(ty_bot), c_loc(dz), c_loc(hzt), c_loc(hhu), c_loc(hhup), &
                 c_loc(hhun), c_loc(hhv), c_loc(hhvp), c_loc(hhvn), c_loc(hhh)), 'baroclinic_dynamics_gpu')
 end subroutine baroclinic_dynamics_gpu
+
+! Function/vel_ssh.f90:1030-1076 on the context bt_ctx, the reference's
+! arguments followed by the vertical grid dz(nz) it reads; w (nz+1
+! levels) is read and written.
+subroutine vertical_velocity_gpu(nz, u, v, w, hu, hv, wflux, dz)
+  integer, intent(in) :: nz
+  real(8), target, intent(in) :: u(*), v(*), hu(*), hv(*), wflux(*), dz(*)
+  real(8), target, intent(inout) :: w(*)
+  call bt_check(inmsom_bt_vertical_velocity(bt_ctx, int(nz, c_int), c_loc(dz), c_loc(u), c_loc(v), c_loc(w), &
+                c_loc(hu), c_loc(hv), c_loc(wflux)), 'vertical_velocity_gpu')
+end subroutine vertical_velocity_gpu
 
 end module inmsom_bt
