@@ -997,6 +997,42 @@ void bto_vertical_velocity(const bto_grid *g, int nz, const double *dz, const do
     for (int k = 0; k <= nz; ++k) cyclize(g, w + (size_t)k * sz);
 }
 
+/* Function/vel_ssh.f90:320-376 (pressure_gradients): the hydrostatic
+ * pressure of density den (nz, ny, nx) integrated down the sigma levels
+ * z[nz] between the two T columns of each lcu (x) and lcv (y) interior
+ * point, the gradient subtracted from rhsx, rhsy (nz, ny, nx) in place;
+ * hhq, hhu, hhv are the 3-D model's depths.  Cells not on lcu (lcv) keep
+ * their values. */
+void bto_pressure_gradients(const bto_grid *g, int nz, const double *z, const double *den, const double *hhq,
+                            const double *hhu, const double *hhv, double *rhsx, double *rhsy)
+{
+    const size_t sz = (size_t)g->nx * g->ny;
+    const double gr = g->free_fall_acc, rho = g->ref_den;
+    for (int n = NYS; n <= NYE; ++n)
+        for (int m = NXS; m <= NXE; ++m) {
+            const size_t i = IX(m, n);
+            for (int dir = 0; dir < 2; ++dir) {
+                const float *mask = dir ? g->lcv : g->lcu;
+                if (!(mask[i] > 0.5f)) continue;
+                /* j: the other T column (east for u, north for v) */
+                const size_t j = dir ? IX(m, n + 1) : IX(m + 1, n);
+                const double h1 = hhq[j], h0 = hhq[i];
+                const double fac = dir ? hhv[i] : hhu[i], met = dir ? g->dxh[i] : g->dyh[i];
+                double *rhs = dir ? rhsy : rhsx;
+                double ph = h1 * z[0] * den[j] - h0 * z[0] * den[i];
+                double pl = h1 * z[0] * den[i] - h0 * z[0] * den[j];
+                rhs[i] = rhs[i] - (ph - pl) * fac * met * gr / rho / 2.0;
+                for (int k = 2; k <= nz; ++k) {
+                    const size_t o = (size_t)(k - 1) * sz, op = (size_t)(k - 2) * sz;
+                    ph = ph + (h1 * z[k - 1] * den[op + j] - h1 * z[k - 2] * den[o + j]) -
+                         (h0 * z[k - 1] * den[op + i] - h0 * z[k - 2] * den[o + i]);
+                    pl = h1 * z[k - 1] * den[o + i] - h0 * z[k - 1] * den[o + j];
+                    rhs[o + i] = rhs[o + i] - (ph - pl) * fac * met * gr / rho / 2.0;
+                }
+            }
+        }
+}
+
 void bto_model_step(const bto_grid *g, bto_depths *d, double tau, int nstep,
                     int ksw4, double cb_nl, double ebottom, double *ubrtr_e,
                     double *ubrtrp_e, double *vbrtr_e, double *vbrtrp_e,

@@ -184,6 +184,12 @@ void bto_baroclinic_dynamics(const bto_grid *g, int nz, double tau, const double
 void bto_vertical_velocity(const bto_grid *g, int nz, const double *dz, const double *u, const double *v, double *w,
                            const double *hu, const double *hv, const double *wflux);
 
+/* Function/vel_ssh.f90:320-376 (pressure_gradients): den, rhsx, rhsy
+ * (nz, ny, nx), rhsx/rhsy updated in place on lcu/lcv; z[nz]; hhq, hhu,
+ * hhv (ny, nx). */
+void bto_pressure_gradients(const bto_grid *g, int nz, const double *z, const double *den, const double *hhq,
+                            const double *hhu, const double *hhv, double *rhsx, double *rhsy);
+
 /* Wet-cell volume anomaly sum(eta * dx * dy) over lu interior cells,
  * summed row by row in a fixed order. */
 double bto_volume(const bto_grid *g, const double *ssh);

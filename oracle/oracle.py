@@ -304,6 +304,17 @@ def vertical_velocity(og: "OracleGrid", inp: Dict[str, np.ndarray]) -> np.ndarra
     return w["ww"]
 
 
+def pressure_gradients(og: "OracleGrid", inp: Dict[str, np.ndarray]) -> Dict[str, np.ndarray]:
+    """bto_pressure_gradients on copies of ``inp`` (z, den, rhsx, rhsy as
+    (nz, ny, nx), hhq, hhu, hhv); returns the updated rhsx, rhsy."""
+    L = lib()
+    w = {k: np.ascontiguousarray(inp[k], dtype=np.float64).copy() for k in inp}
+    nz = w["den"].shape[0]
+    L.bto_pressure_gradients(C.byref(og.g), nz, _d(w["z"]), _d(w["den"]), _d(w["hhq"]), _d(w["hhu"]),
+                             _d(w["hhv"]), _d(w["rhsx"]), _d(w["rhsy"]))
+    return {"rhsx": w["rhsx"], "rhsy": w["rhsy"]}
+
+
 def have_reference() -> bool:
     return os.path.exists(REF_HARNESS)
 
@@ -316,6 +327,26 @@ def run_reference(case_path: str, out_path: str, nx: int, ny: int, mode: int,
                    stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, cwd=os.path.dirname(out_path))
     raw = np.fromfile(out_path, dtype="<f8")
     per = nx * ny
+    if mode == 6:
+        # masks, z (nz), den, rhsx, rhsy (nz planes), hhq, hhu, hhv, rhsx, rhsy out
+        nz = (raw.size - 7 * per - 3 * per) // (1 + 5 * per)
+        res = {"masks": {k: raw[i * per:(i + 1) * per].reshape(ny, nx) for i, k in enumerate(MASKS)}}
+        o = 7 * per
+        inp = {"z": raw[o:o + nz].copy()}
+        o += nz
+        for k in ("den", "rhsx", "rhsy"):
+            inp[k] = raw[o:o + nz * per].reshape(nz, ny, nx).copy()
+            o += nz * per
+        for k in ("hhq", "hhu", "hhv"):
+            inp[k] = raw[o:o + per].reshape(ny, nx).copy()
+            o += per
+        out = {}
+        for k in ("rhsx", "rhsy"):
+            out[k] = raw[o:o + nz * per].reshape(nz, ny, nx).copy()
+            o += nz * per
+        assert o == raw.size
+        res["pg_in"], res["pg_out"] = inp, out
+        return res
     if mode == 5:
         # masks, dz (nz), uu, vv (nz planes), ww (nz+1), hu, hv, wflux, ww out (nz+1)
         nz = (raw.size - 7 * per - 3 * per - 2 * per) // (1 + 4 * per)

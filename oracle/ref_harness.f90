@@ -49,6 +49,10 @@
 !   3-D velocities, the depths of hh_init and the case's wflux: the inputs
 !   (dz, uu, vv, the initial ww, hhu, hhv, wflux) and the resulting ww are
 !   written.
+! mode 6: pressure_gradients (Function/vel_ssh.f90:320-376) on a synthetic
+!   density and sigma grid at the compiled nz, the depths of hh_init and
+!   synthetic 3-D right-hand sides: the inputs (z, den, RHSx3d, RHSy3d,
+!   hhq, hhu, hhv) and the updated RHSx3d, RHSy3d are written.
 program ref_harness
 use main_basin_pars
 use mpi_parallel_tools
@@ -351,6 +355,23 @@ else if (mode == 5) then
   write(23) hhu_e, hhv_e, wflux
   call vertical_velocity(uu, vv, ww, hhu_e, hhv_e, wflux)
   write(23) ww
+else if (mode == 6) then
+  ! pressure_gradients (Function/vel_ssh.f90:320-376); a stratified,
+  ! horizontally varying density and a non-uniform sigma grid
+  call hh_init(hhq_e, hhqp_e, hhqn_e, hhu_e, hhup_e, hhun_e, hhv_e, hhvp_e, hhvn_e, &
+               hhh_e, hhhp_e, hhhn_e, ssh_e, sshp_e, hhq_rest)
+  hhq = hhq_e; hhu = hhu_e; hhv = hhv_e
+  do k = 1, nz
+    z(k) = (dble(k) - 0.5d0)**1.3d0/(dble(nz) - 0.4d0)**1.3d0
+    den(:,:,k) = 1025.0d0 + 0.4d0*k + 2.0d0*ssh_e - 0.5d0*ubrtr_e + 0.3d0*vbrtrp_e*k
+    RHSx3d(:,:,k) = 1.0d2*(ubrtr_e + 0.1d0*k)*dxt*dyh*1.0d-3
+    RHSy3d(:,:,k) = -1.0d2*(vbrtr_e - 0.05d0*k)*dxh*dyt*1.0d-3
+  enddo
+  write(23) z
+  write(23) den, RHSx3d, RHSy3d
+  write(23) hhq, hhu, hhv
+  call pressure_gradients(den, RHSx3d, RHSy3d)
+  write(23) RHSx3d, RHSy3d
 else
   ! helper routines, one call each, on the case's state arrays
   ! hh_init from (ssh_e, sshp_e)

@@ -458,3 +458,42 @@ def test_vertical_velocity_closed_basin_balance():
     for k in range(nz + 1):
         assert np.allclose(w0[k][inner], (cum[k] * rate * b.dx * b.dy / (b.dx * b.dy))[inner], rtol=1e-12,
                            atol=1e-30)
+
+
+def test_pressure_gradients_closed_forms():
+    """Hydrostatics on sigma levels: a density that is the same in every
+    cell leaves the right-hand sides unchanged whatever the depths (the
+    sigma-coordinate pressure-gradient error of a uniform fluid is zero);
+    with equal depths h and a density that differs only between the two
+    columns, rhs(k) drops by g/RefDen * h * z(k) * (den1 - den0) * hhu * dyh
+    (hhv * dxh for y), the hydrostatic pressure difference at depth h z(k)."""
+    b = make_basin(40, 30, seed=9, sigma=3.0, metrics="stretched")
+    og = oracle.OracleGrid(b)
+    rng = np.random.default_rng(9)
+    nz, sh = 7, (b.ny, b.nx)
+    lcu, lcv = (og.masks[k] > 0.5 for k in ("lcu", "lcv"))
+    z = np.sort(rng.random(nz))
+    rhsx, rhsy = rng.standard_normal((nz,) + sh), rng.standard_normal((nz,) + sh)
+    hhu, hhv = 20.0 + rng.random(sh), 20.0 + rng.random(sh)
+    uniform = {"z": z, "den": np.full((nz,) + sh, 1027.5), "rhsx": rhsx, "rhsy": rhsy,
+               "hhq": 10.0 + 40.0 * rng.random(sh), "hhu": hhu, "hhv": hhv}
+    out = oracle.pressure_gradients(og, uniform)
+    scale = 1027.5 * 50.0 * b.g / b.rho0 * 21.0 * max(b.dxh.max(), b.dyh.max())
+    for k in ("rhsx", "rhsy"):
+        assert np.abs(out[k] - uniform[k]).max() <= 1e-13 * scale
+    h = 37.0
+    den_col = 1025.0 + 3.0 * rng.standard_normal(sh)
+    tilt = {"z": z, "den": np.broadcast_to(den_col, (nz,) + sh).copy(), "rhsx": rhsx, "rhsy": rhsy,
+            "hhq": np.full(sh, h), "hhu": hhu, "hhv": hhv}
+    out = oracle.pressure_gradients(og, tilt)
+    dx_den = np.zeros(sh)
+    dx_den[:, :-1] = den_col[:, 1:] - den_col[:, :-1]
+    dy_den = np.zeros(sh)
+    dy_den[:-1, :] = den_col[1:, :] - den_col[:-1, :]
+    for k in range(nz):
+        want_x = rhsx[k] - b.g / b.rho0 * h * z[k] * dx_den * hhu * b.dyh
+        want_y = rhsy[k] - b.g / b.rho0 * h * z[k] * dy_den * hhv * b.dxh
+        assert np.allclose(out["rhsx"][k][lcu], want_x[lcu], rtol=1e-11, atol=1e-11 * scale)
+        assert np.allclose(out["rhsy"][k][lcv], want_y[lcv], rtol=1e-11, atol=1e-11 * scale)
+        assert np.array_equal(out["rhsx"][k][~lcu], rhsx[k][~lcu])
+        assert np.array_equal(out["rhsy"][k][~lcv], rhsy[k][~lcv])

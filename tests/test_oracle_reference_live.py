@@ -109,3 +109,23 @@ def test_vertical_velocity_bitwise(seed, metrics):
     got = oracle.vertical_velocity(oracle.OracleGrid(b), inp)
     assert_bitwise({"ww": got}, {"ww": want}, ("ww",), f"vertical velocity seed {seed} {metrics}")
     assert np.any(want != inp["ww"]) and np.any(want == inp["ww"])
+
+
+@pytest.mark.parametrize("seed,metrics", [(21, "uniform"), (22, "stretched")])
+def test_pressure_gradients_bitwise(seed, metrics):
+    """pressure_gradients (Function/vel_ssh.f90:320-376) called by the
+    harness, against the oracle on the inputs it wrote; right-hand sides
+    off lcu (lcv) keep their values."""
+    b = make_basin(285, 159, seed=seed, sigma=4.0, g=9.8, hmax=120.0, metrics=metrics)
+    b = randomize_state(b, seed + 1, amp_uv=0.05)
+    rd = oracle.OracleGrid(b).rayleigh()
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "case.bin")
+        write_case(path, b, rd, 1, 2, 0, 6)
+        ref = oracle.run_reference(path, path + ".out", b.nx, b.ny, 6)
+    inp, want = ref["pg_in"], ref["pg_out"]
+    assert inp["den"].shape[0] == 20
+    got = oracle.pressure_gradients(oracle.OracleGrid(b), inp)
+    assert_bitwise(got, want, ("rhsx", "rhsy"), f"pressure gradients seed {seed} {metrics}")
+    for k in ("rhsx", "rhsy"):
+        assert np.any(want[k] != inp[k]) and np.any(want[k] == inp[k])
