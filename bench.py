@@ -63,7 +63,9 @@ TRAFFIC_KERNELS = {"fp64": ("k_fused_step2<double, false, true>",),
                    "fp32": ("k_fused_step2<float, false, true>",),
                    "fp64_exact": ("k_fused_step<double, true, true>", "k_fused_step<double, true, false>"),
                    "ensemble_fp32": ("k_fused_batch2<float, false, true>",),
-                   "forcing": ("k_forcing<",)}
+                   "forcing": ("k_forcing<",),
+                   "vertical_velocity": ("k_vertical(",),
+                   "pressure_gradients": ("k_pressure(",)}
 
 
 # kernels of one 2-D -> 3-D coupling and their launches per coupling
@@ -511,7 +513,52 @@ def vertical_leg(m, b, nz, nrep):
     return {"levels": nz, "value": b.n_wet * nz / avg_s, "unit": "wet-cell levels of vertical_velocity/s",
             "device_ms": avg_s * 1e3, "host_call_ms": host_s * 1e3,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                         "frac": achieved / PEAK_HBM_GBS, "traffic": None, "bytes_per_call": alg}}
+                         "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_from_profiles("vertical_velocity"),
+                         "bytes_per_call": alg}}
+
+
+def pressure_cpu(b, f):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    og = oracle.OracleGrid(b)
+    t0 = time.perf_counter()
+    oracle.pressure_gradients(og, f)
+    dt = time.perf_counter() - t0
+    nz = f["den"].shape[0]
+    return {"value": b.n_wet * nz / dt, "unit": "wet-cell levels of pressure_gradients/s", "cores": 1,
+            "kind": "port", "sample": f"one call on the same basin with {nz} levels, {dt:.2f} s, "
+                                      "oracle/bt_oracle.c (bto_pressure_gradients, one thread)"}
+
+
+# Algorithmic bytes of pressure_gradients per array cell: den read (8 B x
+# nz), rhsx, rhsy read and written (32 B x nz); hhq, hhu, hhv, dxh, dyh read
+# (40 B) and lcu, lcv as float (8 B).
+def pressure_leg(m, b, nz, nrep):
+    """pressure_gradients (Function/vel_ssh.f90:320-376) on the same basin
+    with nz levels: device time (HIP events) and the host-array call."""
+    rng = np.random.default_rng(19)
+    sh = (b.ny, b.nx)
+    f = {"z": np.sort(rng.random(nz)),
+         "den": 1025.0 + 0.3 * np.arange(1, nz + 1)[:, None, None] + rng.standard_normal((nz,) + sh),
+         "rhsx": rng.standard_normal((nz,) + sh), "rhsy": rng.standard_normal((nz,) + sh),
+         "hhq": 30.0 + 20.0 * rng.random(sh), "hhu": 30.0 + rng.random(sh), "hhv": 30.0 + rng.random(sh)}
+    args = (f["z"], f["den"], f["rhsx"], f["rhsy"], f["hhq"], f["hhu"], f["hhv"])
+    m.pressure_gradients(*args)
+    t0 = time.perf_counter()
+    m.pressure_gradients(*args)
+    host_s = time.perf_counter() - t0
+    m.pressure_gradients_timed(nz, 2)
+    avg_s = m.pressure_gradients_timed(nz, nrep) * 1e-3 / nrep
+    alg = (40 * nz + 48) * b.nx * b.ny
+    achieved = alg / avg_s / 1e9
+    log(f"pressure gradients nz={nz}: device {avg_s * 1e3:.3f} ms ({achieved:.0f} GB/s), host-array call "
+        f"{host_s * 1e3:.1f} ms")
+    return f, {"levels": nz, "value": b.n_wet * nz / avg_s, "unit": "wet-cell levels of pressure_gradients/s",
+            "device_ms": avg_s * 1e3, "host_call_ms": host_s * 1e3,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_from_profiles("pressure_gradients"),
+                         "bytes_per_call": alg}}
 
 
 def snapshot_leg(m, b, count):
@@ -718,6 +765,9 @@ def main():
             baroclinic, baroclinic_inp = baroclinic_leg(m, b, args.baroclinic_levels, args.baroclinic_reps)
         vertical = vertical_leg(m, b, args.baroclinic_levels, args.baroclinic_reps) \
             if args.baroclinic_levels > 0 else None
+        pressure = None
+        if args.baroclinic_levels > 0:
+            pressure_inp, pressure = pressure_leg(m, b, args.baroclinic_levels, args.baroclinic_reps)
         snapshot = snapshot_leg(m, b, args.snapshots) if args.snapshots > 0 else None
 
     errs = {}
@@ -775,6 +825,11 @@ def main():
         line["baroclinic"] = baroclinic
     if vertical is not None:
         line["vertical_velocity"] = vertical
+    if pressure is not None:
+        if not args.no_cpu_baseline:
+            pressure["cpu_baseline"] = pressure_cpu(b, pressure_inp)
+        del pressure_inp
+        line["pressure_gradients"] = pressure
     if snapshot is not None:
         line["snapshot"] = snapshot
     if args.ensemble_steps > 0:

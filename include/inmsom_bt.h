@@ -390,6 +390,27 @@ int inmsom_bt_vertical_velocity_dev(inmsom_bt_ctx *ctx, int nz, const double *dz
  * time of all.  No counterpart in the reference. */
 int inmsom_bt_vertical_velocity_timed(inmsom_bt_ctx *ctx, int nz, int nrep, double *ms);
 
+/* pressure_gradients, Function/vel_ssh.f90:320-376: the hydrostatic
+ * pressure of the density den (nz, ny, nx) integrated down the sigma
+ * levels z[nz] between the two T columns of each lcu (x) and lcv (y)
+ * interior point, its gradient (times hhu dyh, hhv dxh, FreeFallAcc /
+ * RefDen) subtracted from rhsx, rhsy (nz, ny, nx) in place; hhq, hhu, hhv
+ * (ny, nx) are the 3-D model's depths (the reference reads z and them from
+ * Modules/mod_basin_grid.f90).  Cells off lcu (lcv) keep their values.
+ * Called on zeroed rhsx, rhsy it yields the pgx, pgy that
+ * inmsom_bt_forcing_3d adds.  Host arrays; the _dev variant takes device
+ * addresses (z a host array) and returns after the work has finished. */
+int inmsom_bt_pressure_gradients(inmsom_bt_ctx *ctx, int nz, const double *z, const double *den, const double *hhq,
+                                 const double *hhu, const double *hhv, double *rhsx, double *rhsy);
+int inmsom_bt_pressure_gradients_dev(inmsom_bt_ctx *ctx, int nz, const double *z, const double *den,
+                                     const double *hhq, const double *hhu, const double *hhv, double *rhsx,
+                                     double *rhsy);
+/* Timing: nrep repetitions of the device work on the fields the last
+ * inmsom_bt_pressure_gradients call with nz levels staged (each repetition
+ * subtracts again); *ms = HIP-event time of all.  No counterpart in the
+ * reference. */
+int inmsom_bt_pressure_gradients_timed(inmsom_bt_ctx *ctx, int nz, int nrep, double *ms);
+
 /* Timing of the coupling's device work: nrep repetitions of the kernels of
  * inmsom_bt_remove_barotropic then inmsom_bt_internal_update on the fields
  * the last inmsom_bt_internal_update call left on the device (it must have

@@ -175,25 +175,12 @@ __device__ void solve_v(const BcParams &p, int m, int n)
     }
 }
 
-// Tiles in row-major order, dealt to the eight XCDs (block b to XCD b % 8)
-// as contiguous runs, so the row bands one XCD holds at once are
-// neighbours and the rows n - 1, n + 1 the columns read are shared in
-// its L2.
-__device__ __forceinline__ void bc_tile(int tx, int ty, int b, int &bx, int &by)
-{
-    const int T = tx * ty, xcd = b % 8, q = b / 8;
-    const int per = T / 8, extra = T % 8;
-    const int pos = xcd * per + (xcd < extra ? xcd : extra) + q;
-    by = pos / tx;
-    bx = pos % tx;
-}
-
 template <int NZ>
 __global__ __launch_bounds__(256) void k_bc_solve(BcParams p)
 {
     const DevGrid &g = p.g;
     int bx, by;
-    bc_tile(p.tiles_x, p.tiles_y, (int)blockIdx.x, bx, by);
+    xcd_tile(p.tiles_x, p.tiles_y, (int)blockIdx.x, bx, by);
     const int m = bx * 64 + (int)(threadIdx.x & 63) + 3;
     const int n = by * 4 + (int)(threadIdx.x >> 6) + 3;
     if (m > g.nx - 2 || n > g.ny - 2) return;
@@ -205,11 +192,13 @@ __global__ __launch_bounds__(256) void k_bc_solve(BcParams p)
 // vel_ssh.f90:1000-1023 over m = 2..nx-1, n = 2..ny-1; un, vn outside the
 // interior are zero, or on periodic boundaries the value the wrap of
 // Service/oc_algorithms.f90:50-95, :118-135 copies in
-__global__ void k_bc_update(BcParams p, double ts)
+__global__ __launch_bounds__(256) void k_bc_update(BcParams p, double ts, int tiles_x, int tiles_y)
 {
     const DevGrid &g = p.g;
-    const int m = blockIdx.x * blockDim.x + threadIdx.x + 2;
-    const int n = blockIdx.y * blockDim.y + threadIdx.y + 2;
+    int bx, by;
+    xcd_tile(tiles_x, tiles_y, (int)blockIdx.x, bx, by);
+    const int m = bx * 64 + (int)(threadIdx.x & 63) + 2;
+    const int n = by * 4 + (int)(threadIdx.x >> 6) + 2;
     if (m > g.nx - 1 || n > g.ny - 1) return;
     const size_t pl = (size_t)g.nx * g.ny, i = I(m, n);
     const int mm = (g.px && m == 2) ? g.nx - 2 : (g.px && m == g.nx - 1) ? 3 : m;
@@ -325,9 +314,9 @@ void launch(inmsom_bt_ctx *c, const BcParams &p)
         hipLaunchKernelGGL(k_bc_solve<20>, gs, dim3(256), 0, c->stream, q);
     else
         hipLaunchKernelGGL(k_bc_solve<0>, gs, dim3(256), 0, c->stream, q);
-    const dim3 blk(64, 4);
-    const dim3 gu((unsigned)((c->nx - 2 + 63) / 64), (unsigned)((c->ny - 2 + 3) / 4));
-    hipLaunchKernelGGL(k_bc_update, gu, blk, 0, c->stream, q, c->g.ts);
+    // the update's 64 x 4 tiles cover m = 2..nx-1, n = 2..ny-1
+    const int ux = (c->nx - 2 + 63) / 64, uy = (c->ny - 2 + 3) / 4;
+    hipLaunchKernelGGL(k_bc_update, dim3((unsigned)(ux * uy)), dim3(256), 0, c->stream, q, c->g.ts, ux, uy);
 }
 
 int check(inmsom_bt_ctx *c, int nz, double tau)

@@ -330,7 +330,8 @@ int inmsom_bt_finalize(inmsom_bt_ctx *c)
                     (void *)c->surf_x, (void *)c->surf_y, (void *)c->bot_x, (void *)c->bot_y,
                     (void *)c->scal, (void *)c->cnt, (void *)c->cpl, (void *)c->cpl_new, (void *)c->f3,
                     (void *)c->f3_mb, (void *)c->f3_halo, (void *)c->f3_dz,
-                    (void *)c->snap_dev, (void *)c->bc, (void *)c->bc_dev, (void *)c->vv, (void *)c->vv_dz})
+                    (void *)c->snap_dev, (void *)c->bc, (void *)c->bc_dev, (void *)c->vv, (void *)c->vv_dz,
+                    (void *)c->pg, (void *)c->pg_z})
         fr(p);
     if (c->snap_host) hipHostFree(c->snap_host);
     if (c->stream) hipStreamDestroy(c->stream);

@@ -28,6 +28,21 @@ __host__ __device__ inline size_t IDX(const DevGrid &g, int m, int n)
     return (size_t)(n - 1) * (size_t)g.nx + (size_t)(m - 1);
 }
 
+// Tiles of a tiles_x x tiles_y grid in row-major order, dealt to the eight
+// XCDs (the hardware hands workgroup b to XCD b % 8) as contiguous runs: the
+// tiles one XCD holds at once are neighbours, so the cache lines a tile
+// shares with the tiles beside it (rows are not whole 128-byte lines) and
+// the rows above and below stay in that XCD's L2.  A launch of exactly
+// tiles_x * tiles_y workgroups visits every tile once.
+__device__ __forceinline__ void xcd_tile(int tiles_x, int tiles_y, int b, int &bx, int &by)
+{
+    const int T = tiles_x * tiles_y, xcd = b % 8, q = b / 8;
+    const int per = T / 8, extra = T % 8;
+    const int pos = xcd * per + (xcd < extra ? xcd : extra) + q;
+    by = pos / tiles_x;
+    bx = pos % tiles_x;
+}
+
 // Mask bits of the fused kernels (one byte per cell).
 enum : uint8_t {
     MB_LU = 1, MB_LCU = 2, MB_LCV = 4, MB_LUU = 8, MB_LUH = 16, MB_LLU = 32, MB_LLV = 64
@@ -106,6 +121,10 @@ struct inmsom_bt_ctx {
     size_t vv_n = 0;               // doubles in vv
     int vv_nz = 0;                 // levels of the staged fields (0 = none)
     double *vv_dz = nullptr;       // dz of the device-address call (up to 1000 levels)
+    double *pg = nullptr;          // staging of pressure_gradients (bt_pressure.hip)
+    size_t pg_n = 0;               // doubles in pg
+    int pg_nz = 0;                 // levels of the staged fields (0 = none)
+    double *pg_z = nullptr;        // z of the device-address call (up to 1000 levels)
     float *snap_dev = nullptr;     // packed snapshot fields (bt_snapshot.hip)
     float *snap_host = nullptr;    // pinned copy of snap_dev
     size_t snap_n = 0;             // floats in snap_dev / snap_host

@@ -5,8 +5,9 @@
 // Cells the reference does not compute keep the caller's values.
 //
 // k_vertical: one thread per interior (m, n), the column walked from the
-// bottom in registers; consecutive lanes take consecutive m, so each
-// level's loads and stores are coalesced.  The wrap reads the computed
+// bottom in registers, in 64 x 4 tiles dealt to the XCDs in contiguous
+// runs; consecutive lanes take consecutive m, so each level's loads and
+// stores are coalesced.  The wrap reads the computed
 // values and runs as two launches after it (x first, then y, as the
 // reference orders them, which fixes the corners).
 #include <string>
@@ -20,6 +21,7 @@ namespace {
 struct VvParams {
     DevGrid g;
     int nz;
+    int tiles_x, tiles_y;       // 64 x 4 tiles over the interior
     const double *dz;           // nz (device)
     const double *u, *v;        // (nz, ny, nx)
     double *w;                  // (nz + 1, ny, nx)
@@ -31,8 +33,10 @@ struct VvParams {
 __global__ __launch_bounds__(256) void k_vertical(VvParams p)
 {
     const DevGrid &g = p.g;
-    const int m = blockIdx.x * blockDim.x + threadIdx.x + 3;
-    const int n = blockIdx.y * blockDim.y + threadIdx.y + 3;
+    int bx, by;
+    xcd_tile(p.tiles_x, p.tiles_y, (int)blockIdx.x, bx, by);
+    const int m = bx * 64 + (int)(threadIdx.x & 63) + 3;
+    const int n = by * 4 + (int)(threadIdx.x >> 6) + 3;
     if (m > g.nx - 2 || n > g.ny - 2) return;
     const size_t pl = (size_t)g.nx * g.ny, i = I(m, n), iw = I(m - 1, n), is = I(m, n - 1);
     if (!(g.lu[i] > 0.5f)) return;
@@ -90,10 +94,11 @@ int vfail(const std::string &m, int code)
                                            INMSOM_BT_EDEVICE);                                      \
     } while (0)
 
-void launch(inmsom_bt_ctx *c, const VvParams &p)
+void launch(inmsom_bt_ctx *c, VvParams p)
 {
-    const dim3 blk(64, 4), grid((unsigned)((c->nx - 4 + 63) / 64), (unsigned)((c->ny - 4 + 3) / 4));
-    hipLaunchKernelGGL(k_vertical, grid, blk, 0, c->stream, p);
+    p.tiles_x = (c->nx - 4 + 63) / 64;
+    p.tiles_y = (c->ny - 4 + 3) / 4;
+    hipLaunchKernelGGL(k_vertical, dim3((unsigned)(p.tiles_x * p.tiles_y)), dim3(256), 0, c->stream, p);
     if (c->g.px)
         hipLaunchKernelGGL(k_wrap_x, dim3((unsigned)((c->ny + 255) / 256), (unsigned)(p.nz + 1)), dim3(256), 0,
                            c->stream, c->g, p.nz + 1, p.w);

@@ -257,6 +257,21 @@ interface
     integer(c_int), value :: nz
     type(c_ptr), value :: dz, u, v, w, hu, hv, wflux
   end function
+  ! pressure_gradients (Function/vel_ssh.f90:320-376)
+  integer(c_int) function inmsom_bt_pressure_gradients(ctx, nz, z, den, hhq, hhu, hhv, rhsx, rhsy) &
+      bind(C, name="inmsom_bt_pressure_gradients")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: nz
+    type(c_ptr), value :: z, den, hhq, hhu, hhv, rhsx, rhsy
+  end function
+  integer(c_int) function inmsom_bt_pressure_gradients_dev(ctx, nz, z, den, hhq, hhu, hhv, rhsx, rhsy) &
+      bind(C, name="inmsom_bt_pressure_gradients_dev")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: nz
+    type(c_ptr), value :: z, den, hhq, hhu, hhv, rhsx, rhsy
+  end function
   function c_strlen(s) bind(C, name="strlen")
     import :: c_ptr, c_size_t
     type(c_ptr), value :: s
@@ -390,5 +405,17 @@ subroutine vertical_velocity_gpu(nz, u, v, w, hu, hv, wflux, dz)
   call bt_check(inmsom_bt_vertical_velocity(bt_ctx, int(nz, c_int), c_loc(dz), c_loc(u), c_loc(v), c_loc(w), &
                 c_loc(hu), c_loc(hv), c_loc(wflux)), 'vertical_velocity_gpu')
 end subroutine vertical_velocity_gpu
+
+! Function/vel_ssh.f90:320-376 on the context bt_ctx, the reference's
+! arguments followed by what it reads from Modules/mod_basin_grid.f90: the
+! sigma levels z(nz) and the depths hhq, hhu, hhv.  RHSx, RHSy are read and
+! written.
+subroutine pressure_gradients_gpu(nz, den, RHSx, RHSy, z, hhq, hhu, hhv)
+  integer, intent(in) :: nz
+  real(8), target, intent(in) :: den(*), z(*), hhq(*), hhu(*), hhv(*)
+  real(8), target, intent(inout) :: RHSx(*), RHSy(*)
+  call bt_check(inmsom_bt_pressure_gradients(bt_ctx, int(nz, c_int), c_loc(z), c_loc(den), c_loc(hhq), &
+                c_loc(hhu), c_loc(hhv), c_loc(RHSx), c_loc(RHSy)), 'pressure_gradients_gpu')
+end subroutine pressure_gradients_gpu
 
 end module inmsom_bt

@@ -421,6 +421,44 @@ class BarotropicModel:
               "inmsom_bt_vertical_velocity_timed")
         return ms.value
 
+    def pressure_gradients(self, z, den, rhsx, rhsy, hhq, hhu, hhv):
+        """Function/vel_ssh.f90:320-376 on host arrays: den, rhsx, rhsy
+        (nz, ny, nx), z[nz], hhq, hhu, hhv (ny, nx).  Returns the updated
+        (rhsx, rhsy) as new arrays (cells off lcu, lcv keep their values)."""
+        d = np.ascontiguousarray(den, dtype=np.float64)
+        nz = d.shape[0]
+        rx = np.array(rhsx, dtype=np.float64, order="C")
+        ry = np.array(rhsy, dtype=np.float64, order="C")
+        if d.shape != (nz, self.ny, self.nx) or rx.shape != d.shape or ry.shape != d.shape:
+            raise ValueError("den, rhsx, rhsy must be (nz, ny, nx)")
+        two = [np.ascontiguousarray(a, dtype=np.float64) for a in (hhq, hhu, hhv)]
+        if any(a.shape != (self.ny, self.nx) for a in two):
+            raise ValueError("hhq, hhu, hhv must be (ny, nx)")
+        zc = np.ascontiguousarray(z, dtype=np.float64)
+        if zc.shape != (nz,):
+            raise ValueError("z must have nz values")
+        check(lib().inmsom_bt_pressure_gradients(self._h, nz, _d(zc), _d(d), *[_d(a) for a in two], _d(rx), _d(ry)),
+              "inmsom_bt_pressure_gradients")
+        return rx, ry
+
+    def pressure_gradients_dev(self, z, den, rhsx, rhsy, hhq, hhu, hhv):
+        """As pressure_gradients on DeviceArray arguments; rhsx, rhsy updated
+        in place."""
+        zc = np.ascontiguousarray(z, dtype=np.float64)
+        nz = zc.shape[0]
+        s3, s2 = (nz, self.ny, self.nx), (self.ny, self.nx)
+        check(lib().inmsom_bt_pressure_gradients_dev(self._h, nz, _d(zc), self._dev(den, s3), self._dev(hhq, s2),
+                                                     self._dev(hhu, s2), self._dev(hhv, s2), self._dev(rhsx, s3),
+                                                     self._dev(rhsy, s3)), "inmsom_bt_pressure_gradients_dev")
+
+    def pressure_gradients_timed(self, nz: int, nrep: int) -> float:
+        """HIP-event time (ms) of nrep repetitions of pressure_gradients's
+        device work on the fields the last host-array call staged."""
+        ms = C.c_double()
+        check(lib().inmsom_bt_pressure_gradients_timed(self._h, int(nz), int(nrep), C.byref(ms)),
+              "inmsom_bt_pressure_gradients_timed")
+        return ms.value
+
     def forcing_3d_timed(self, nz: int, nrep: int) -> float:
         """HIP-event time (ms) of nrep repetitions of the forcing's device
         work on the fields the last forcing_3d call left on the device."""
