@@ -15,9 +15,13 @@ namespace ibt {
 
 namespace {
 
-#define KERNEL_MN                                            \
-    const int m = blockIdx.x * blockDim.x + threadIdx.x + 1; \
-    const int n = blockIdx.y * blockDim.y + threadIdx.y + 1;
+// (m, n) of a 2-D launch over the whole array; the blocks go to the XCDs
+// in contiguous row-major runs (xcd_tile)
+#define KERNEL_MN                                                                                   \
+    int bx_, by_;                                                                                   \
+    xcd_tile((int)gridDim.x, (int)gridDim.y, (int)(blockIdx.y * gridDim.x + blockIdx.x), bx_, by_); \
+    const int m = bx_ * (int)blockDim.x + (int)threadIdx.x + 1;                                     \
+    const int n = by_ * (int)blockDim.y + (int)threadIdx.y + 1;
 #define I(mm, nn) IDX(g, (mm), (nn))
 // nx_start-1 .. nx_end+1 x ny_start-1 .. ny_end+1 on one rank
 #define IN_WIDE(g) (m >= 2 && m <= (g).nx - 1 && n >= 2 && n <= (g).ny - 1)

@@ -6,7 +6,7 @@
 ! reference prints (time_barotrop).
 !
 ! usage: inmsom_bt_host <par dir> <case.bin> <out.bin> [literal|fp64_exact|fp64|fp32] [g] [coupling.bin|-]
-!                       [forcing.bin|-] [baroclinic.bin]
+!                       [forcing.bin|-] [baroclinic.bin|-] [pressure.bin]
 !
 !   ocean_run.par  line 2 time_step [s], line 3 run_duration [days],
 !                  line 9 nstep_barotrop                 (inmsom_head.f90:47-55)
@@ -35,7 +35,11 @@
 !                  r_diss, tx_surf, ty_surf, tx_bot, ty_bot, hhu, hhup, hhun,
 !                  hhv, hhvp, hhvn, hhh (nx x ny).  The host runs
 !                  baroclinic_dynamics (Function/vel_ssh.f90:794-1027) and
-!                  appends uu, uup, vv, vvp to out.bin.
+!                  appends uu, uup, vv, vvp to out.bin ('-': none).
+!   pressure.bin   optional: int32 nz, real8 z(nz), then den, RHSx, RHSy
+!                  (nx x ny x nz), hhq, hhu, hhv (nx x ny).  The host runs
+!                  pressure_gradients (Function/vel_ssh.f90:320-376) and
+!                  appends RHSx, RHSy to out.bin.
 !
 ! Mode 'literal' runs the stage-by-stage device path (any metrics, the
 ! biharmonic term); the fused modes need uniform metrics.
@@ -44,7 +48,7 @@ use iso_c_binding
 use inmsom_bt
 implicit none
 
-character(512) :: pardir, case_path, out_path, arg, cpl_path, frc_path, bcl_path
+character(512) :: pardir, case_path, out_path, arg, cpl_path, frc_path, bcl_path, prs_path
 character(256) :: lines(64)
 character(16) :: mode_name
 integer :: nl, ierr, magic, version, nx, ny, nsteps_case, nstep_case, ksw4_case, cmode, nout
@@ -60,6 +64,8 @@ real(8), allocatable, target :: c2(:,:,:), uu(:,:,:), vv(:,:,:)
 real(8), allocatable, target :: f3(:,:,:,:), f2(:,:,:), dzv(:)
 ! baroclinic: b3 = uu uup vv vvp RHSx RHSy RHSx_tran RHSy_tran; bw = ww anzu; b2 = 14 planes
 real(8), allocatable, target :: b3(:,:,:,:), bw(:,:,:,:), b2(:,:,:), bdz(:), bhzt(:)
+! pressure gradients: p3 = den RHSx RHSy; p2 = hhq hhu hhv
+real(8), allocatable, target :: p3(:,:,:,:), p2(:,:,:), pz(:)
 integer(c_long_long) :: bad
 integer(8) :: c0, c1, crate
 real(8) :: time_barotrop, vol0, vol1
@@ -188,8 +194,15 @@ if (command_argument_count() >= 7) then
 endif
 if (command_argument_count() >= 8) then
   call get_command_argument(8, bcl_path)
-  call baroclinic_step(trim(bcl_path))
-  write(23) b3(:,:,:,1:4)
+  if (trim(bcl_path) /= '-') then
+    call baroclinic_step(trim(bcl_path))
+    write(23) b3(:,:,:,1:4)
+  endif
+endif
+if (command_argument_count() >= 9) then
+  call get_command_argument(9, prs_path)
+  call pressure_step(trim(prs_path))
+  write(23) p3(:,:,:,2:3)
 endif
 close(23)
 write(*,'(a,es12.5)') ' Time barotropic: ', time_barotrop
@@ -270,6 +283,25 @@ subroutine baroclinic_step(path)
                                b2(:,:,8), b2(:,:,9), b2(:,:,10), b2(:,:,11), b2(:,:,12), b2(:,:,13), b2(:,:,14))
   write(*,'(a,i0,a)') ' baroclinic step done (', nzb, ' levels)'
 end subroutine baroclinic_step
+
+! pressure_gradients (Function/vel_ssh.f90:320-376) on the fields of `path`.
+subroutine pressure_step(path)
+  character(*), intent(in) :: path
+  integer :: io, nzp
+  open(27, file=path, access='stream', form='unformatted', status='old', iostat=io)
+  if (io /= 0) then
+    write(*,*) 'cannot open ', path
+    error stop 5
+  endif
+  read(27) nzp
+  allocate(pz(nzp), p3(nx, ny, nzp, 3), p2(nx, ny, 3))
+  read(27) pz
+  read(27) p3
+  read(27) p2
+  close(27)
+  call pressure_gradients_gpu(nzp, p3(:,:,:,1), p3(:,:,:,2), p3(:,:,:,3), pz, p2(:,:,1), p2(:,:,2), p2(:,:,3))
+  write(*,'(a,i0,a)') ' pressure gradients done (', nzp, ' levels)'
+end subroutine pressure_step
 
 ! Service/read_write_parameters.f90:2-41: the file's lines, one per entry.
 subroutine read_lines(path, buf, n)

@@ -202,3 +202,38 @@ def test_fortran_host_baroclinic_bitwise():
     got = dict(zip(("uu", "uup", "vv", "vvp"), raw[6 * per:].reshape(4, nz, b.ny, b.nx)))
     assert "baroclinic step done (20 levels)" in p.stdout
     assert_bitwise(got, want, ("uu", "uup", "vv", "vvp"), "fortran host baroclinic")
+
+
+def test_fortran_host_pressure_gradients_bitwise():
+    """The host's pressure step (pressure_gradients_gpu through the
+    ISO_C_BINDING module, Function/vel_ssh.f90:320-376) against the oracle,
+    bit for bit; RHSx, RHSy are updated in place as the reference does."""
+    b = randomize_state(make_basin(66, 48, seed=41, sigma=5.0, metrics="stretched"), 4)
+    rd = oracle.OracleGrid(b).rayleigh()
+    nz = 20
+    rng = np.random.default_rng(31)
+    sh = (b.ny, b.nx)
+    f = {"z": np.sort(rng.random(nz)),
+         "den": 1025.0 + 0.3 * np.arange(1, nz + 1)[:, None, None] + rng.standard_normal((nz,) + sh),
+         "rhsx": rng.standard_normal((nz,) + sh), "rhsy": rng.standard_normal((nz,) + sh),
+         "hhq": 30.0 + 20.0 * rng.random(sh), "hhu": 30.0 + rng.random(sh), "hhv": 30.0 + rng.random(sh)}
+    want = oracle.pressure_gradients(oracle.OracleGrid(b), f)
+    with tempfile.TemporaryDirectory() as tmp:
+        case = os.path.join(tmp, "case.bin")
+        prs = os.path.join(tmp, "pressure.bin")
+        out = os.path.join(tmp, "out.bin")
+        write_case(case, b, rd, 1, 2, 0, 0, [])
+        with open(prs, "wb") as fh:
+            np.array([nz], dtype="<i4").tofile(fh)
+            f["z"].astype("<f8").tofile(fh)
+            for k in ("den", "rhsx", "rhsy", "hhq", "hhu", "hhv"):
+                np.ascontiguousarray(f[k], dtype="<f8").tofile(fh)
+        p = subprocess.run([HOST, PARDIR, case, out, "literal", repr(b.g), "-", "-", "-", prs], capture_output=True,
+                           text=True, timeout=120)
+        print(p.stdout, p.stderr)
+        assert p.returncode == 0, p.stdout + p.stderr
+        raw = np.fromfile(out, dtype="<f8")
+    per = b.nx * b.ny
+    got = dict(zip(("rhsx", "rhsy"), raw[6 * per:].reshape(2, nz, b.ny, b.nx)))
+    assert "pressure gradients done (20 levels)" in p.stdout
+    assert_bitwise(got, want, ("rhsx", "rhsy"), "fortran host pressure gradients")
