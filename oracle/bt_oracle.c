@@ -1033,6 +1033,44 @@ void bto_pressure_gradients(const bto_grid *g, int nz, const double *z, const do
         }
 }
 
+/* Service/oc_algorithms.f90:138-161 (density): the Brydon et al. (1999,
+ * J. Geophys. Res. 104(C1), 1537-1540) polynomial equation of state as the
+ * reference evaluates it, the deviation from 1020 kg/m^3 at potential
+ * temperature t [C], salinity s [psu] and pressure p_pa [Pa] (in MPa
+ * inside; the pure-pressure term is left out there, as here).  The terms
+ * are summed left to right as written in the reference. */
+static double eos_brydon(double t, double s, double p_pa)
+{
+    const double p = p_pa * 1.0e-6;
+    const double den_cgs = -2.00920601e-02 + (5.10768e-05 - 3.69119e-06 * p + 6.54837e-09 * p * p) * t +
+                           (8.05999e-04 - 9.34012e-07 * p + 1.38777e-09 * p * p) * s +
+                           (-7.40849e-06 + 5.33243e-08 * p - 1.01563e-10 * p * p) * t * t +
+                           (-3.01036e-06 + 1.75145e-08 * p - 2.34892e-11 * p * p) * t * s +
+                           (3.32267e-08 - 3.25887e-10 * p + 4.98612e-13 * p * p) * t * t * t +
+                           (3.21931e-08 - 1.65849e-10 * p + 2.17612e-13 * p * p) * t * t * s;
+    return den_cgs * 1000.0;
+}
+
+/* Control/dyncall.f90:437-449 (the density definition): on lu points of
+ * m = 2..nx-1, n = 2..ny-1, every level, den at the pressure
+ * FreeFallAcc * RefDen * hhq * z(k) and den_pot at zero pressure.  tt, ss,
+ * den, den_pot (nz, ny, nx); cells off lu keep their values. */
+void bto_density(const bto_grid *g, int nz, const double *z, const double *tt, const double *ss, const double *hhq,
+                 double *den, double *den_pot)
+{
+    const size_t sz = (size_t)g->nx * g->ny;
+    for (int n = NYS - 1; n <= NYE + 1; ++n)
+        for (int m = NXS - 1; m <= NXE + 1; ++m) {
+            const size_t i = IX(m, n);
+            if (!(g->lu[i] > 0.5f)) continue;
+            for (int k = 0; k < nz; ++k) {
+                const size_t o = (size_t)k * sz + i;
+                den[o] = eos_brydon(tt[o], ss[o], g->free_fall_acc * g->ref_den * hhq[i] * z[k]);
+                den_pot[o] = eos_brydon(tt[o], ss[o], 0.0);
+            }
+        }
+}
+
 void bto_model_step(const bto_grid *g, bto_depths *d, double tau, int nstep,
                     int ksw4, double cb_nl, double ebottom, double *ubrtr_e,
                     double *ubrtrp_e, double *vbrtr_e, double *vbrtrp_e,

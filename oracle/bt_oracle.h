@@ -190,6 +190,12 @@ void bto_vertical_velocity(const bto_grid *g, int nz, const double *dz, const do
 void bto_pressure_gradients(const bto_grid *g, int nz, const double *z, const double *den, const double *hhq,
                             const double *hhu, const double *hhv, double *rhsx, double *rhsy);
 
+/* Control/dyncall.f90:437-449 with Service/oc_algorithms.f90:138-161: den
+ * (at the level's pressure) and den_pot (at zero pressure) from tt, ss on
+ * lu points of m = 2..nx-1, n = 2..ny-1; (nz, ny, nx) arrays, z[nz]. */
+void bto_density(const bto_grid *g, int nz, const double *z, const double *tt, const double *ss, const double *hhq,
+                 double *den, double *den_pot);
+
 /* Wet-cell volume anomaly sum(eta * dx * dy) over lu interior cells,
  * summed row by row in a fixed order. */
 double bto_volume(const bto_grid *g, const double *ssh);

@@ -315,6 +315,17 @@ def pressure_gradients(og: "OracleGrid", inp: Dict[str, np.ndarray]) -> Dict[str
     return {"rhsx": w["rhsx"], "rhsy": w["rhsy"]}
 
 
+def density(og: "OracleGrid", inp: Dict[str, np.ndarray]) -> Dict[str, np.ndarray]:
+    """bto_density on copies of ``inp`` (z, tt, ss, den, den_pot as
+    (nz, ny, nx), hhq); returns the new den, den_pot."""
+    L = lib()
+    w = {k: np.ascontiguousarray(inp[k], dtype=np.float64).copy() for k in inp}
+    nz = w["tt"].shape[0]
+    L.bto_density(C.byref(og.g), nz, _d(w["z"]), _d(w["tt"]), _d(w["ss"]), _d(w["hhq"]), _d(w["den"]),
+                  _d(w["den_pot"]))
+    return {"den": w["den"], "den_pot": w["den_pot"]}
+
+
 def have_reference() -> bool:
     return os.path.exists(REF_HARNESS)
 
@@ -327,6 +338,25 @@ def run_reference(case_path: str, out_path: str, nx: int, ny: int, mode: int,
                    stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, cwd=os.path.dirname(out_path))
     raw = np.fromfile(out_path, dtype="<f8")
     per = nx * ny
+    if mode == 7:
+        # masks, z (nz), tt, ss, den, den_pot (nz planes), hhq, den, den_pot out
+        nz = (raw.size - 8 * per) // (1 + 6 * per)
+        res = {"masks": {k: raw[i * per:(i + 1) * per].reshape(ny, nx) for i, k in enumerate(MASKS)}}
+        o = 7 * per
+        inp = {"z": raw[o:o + nz].copy()}
+        o += nz
+        for k in ("tt", "ss", "den", "den_pot"):
+            inp[k] = raw[o:o + nz * per].reshape(nz, ny, nx).copy()
+            o += nz * per
+        inp["hhq"] = raw[o:o + per].reshape(ny, nx).copy()
+        o += per
+        out = {}
+        for k in ("den", "den_pot"):
+            out[k] = raw[o:o + nz * per].reshape(nz, ny, nx).copy()
+            o += nz * per
+        assert o == raw.size
+        res["eos_in"], res["eos_out"] = inp, out
+        return res
     if mode == 6:
         # masks, z (nz), den, rhsx, rhsy (nz planes), hhq, hhu, hhv, rhsx, rhsy out
         nz = (raw.size - 7 * per - 3 * per) // (1 + 5 * per)

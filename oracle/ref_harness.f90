@@ -53,6 +53,11 @@
 !   density and sigma grid at the compiled nz, the depths of hh_init and
 !   synthetic 3-D right-hand sides: the inputs (z, den, RHSx3d, RHSy3d,
 !   hhq, hhu, hhv) and the updated RHSx3d, RHSy3d are written.
+! mode 7: the density definition (Control/dyncall.f90:437-449, restated
+!   here as it is inline in the reference's caller) with the reference's
+!   own density function (Service/oc_algorithms.f90:138-161) on synthetic
+!   tt, ss, a non-uniform sigma grid and the depths of hh_init: the inputs
+!   (z, tt, ss, the initial den, den_pot, hhq) and den, den_pot are written.
 program ref_harness
 use main_basin_pars
 use mpi_parallel_tools
@@ -70,6 +75,7 @@ real(8), allocatable :: ssh4gx(:,:), ssh4gy(:,:), vort(:,:), strt(:,:), strs(:,:
 real(8), allocatable :: fxa(:,:), fya(:,:), radv(:,:), rady(:,:), rdfx(:,:), rdfy(:,:)
 real(8), allocatable :: ax(:,:), ay(:,:)
 real(8), allocatable :: pgx3(:,:,:), pgy3(:,:,:)
+real(8), external :: density
 
 call get_command_argument(1, case_path)
 call get_command_argument(2, out_path)
@@ -372,6 +378,32 @@ else if (mode == 6) then
   write(23) hhq, hhu, hhv
   call pressure_gradients(den, RHSx3d, RHSy3d)
   write(23) RHSx3d, RHSy3d
+else if (mode == 7) then
+  ! the density definition, Control/dyncall.f90:437-449
+  call hh_init(hhq_e, hhqp_e, hhqn_e, hhu_e, hhup_e, hhun_e, hhv_e, hhvp_e, hhvn_e, &
+               hhh_e, hhhp_e, hhhn_e, ssh_e, sshp_e, hhq_rest)
+  hhq = hhq_e
+  do k = 1, nz
+    z(k) = (dble(k) - 0.5d0)**1.3d0/(dble(nz) - 0.4d0)**1.3d0
+    tt(:,:,k) = 18.0d0 - 0.6d0*k + 40.0d0*ssh_e + 3.0d0*ubrtr_e
+    ss(:,:,k) = 17.5d0 + 0.2d0*k - 20.0d0*sshp_e + 2.0d0*vbrtr_e
+  enddo
+  den = 7.0d0
+  den_pot = -7.0d0
+  write(23) z
+  write(23) tt, ss, den, den_pot
+  write(23) hhq
+  do n = ny_start - 1, ny_end + 1
+    do m = nx_start - 1, nx_end + 1
+      if (lu(m,n) > 0.5) then
+        do k = 1, nz
+          den(m,n,k) = density(tt(m,n,k), ss(m,n,k), FreeFallAcc*RefDen*hhq(m,n)*z(k))
+          den_pot(m,n,k) = density(tt(m,n,k), ss(m,n,k), 0.0d0)
+        enddo
+      endif
+    enddo
+  enddo
+  write(23) den, den_pot
 else
   ! helper routines, one call each, on the case's state arrays
   ! hh_init from (ssh_e, sshp_e)

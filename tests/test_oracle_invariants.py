@@ -497,3 +497,49 @@ def test_pressure_gradients_closed_forms():
         assert np.allclose(out["rhsy"][k][lcv], want_y[lcv], rtol=1e-11, atol=1e-11 * scale)
         assert np.array_equal(out["rhsx"][k][~lcu], rhsx[k][~lcu])
         assert np.array_equal(out["rhsy"][k][~lcv], rhsy[k][~lcv])
+
+
+def test_density_special_cases():
+    """The equation of state where its form fixes the answer: at t = 0 and
+    s = 0 every pressure term vanishes (the pure-pressure term is left out),
+    so den = den_pot = 1000 x -2.00920601e-2 for any depth; at t = 0 it is
+    affine in s at any pressure (equal steps in s give equal steps in den);
+    den_pot is den at zero depth; and with t, s fixed den depends on the
+    depth only through the pressure g RefDen hhq z, so two columns with the
+    same hhq z agree.  Only lu points of m, n = 2..n-1 are written."""
+    b = make_basin(30, 24, seed=5, sigma=3.0)
+    og = oracle.OracleGrid(b)
+    rng = np.random.default_rng(5)
+    nz, sh = 4, (b.ny, b.nx)
+    lu = og.masks["lu"] > 0.5
+    inner = lu.copy()
+    inner[0, :] = inner[-1, :] = False
+    inner[:, 0] = inner[:, -1] = False
+    z = np.array([0.1, 0.35, 0.6, 0.9])
+    hhq = 20.0 + 80.0 * rng.random(sh)
+    zero = np.zeros((nz,) + sh)
+    out = oracle.density(og, {"z": z, "tt": zero, "ss": zero, "hhq": hhq, "den": np.full((nz,) + sh, 5.0),
+                              "den_pot": np.full((nz,) + sh, 5.0)})
+    c0 = -2.00920601e-02 * 1000.0
+    for k in ("den", "den_pot"):
+        assert np.all(out[k][:, inner] == c0)
+        assert np.all(out[k][:, ~inner] == 5.0)
+    s1 = 10.0 + 20.0 * rng.random((nz,) + sh)
+    res = [oracle.density(og, {"z": z, "tt": zero, "ss": a * s1, "hhq": hhq, "den": zero, "den_pot": zero})
+           for a in (0.0, 1.0, 2.0)]
+    step1 = res[1]["den"] - res[0]["den"]
+    step2 = res[2]["den"] - res[1]["den"]
+    assert np.allclose(step1[:, inner], step2[:, inner], rtol=1e-12, atol=1e-12)
+    tt = 25.0 * rng.random((nz,) + sh)
+    ss = 35.0 * rng.random((nz,) + sh)
+    surf = oracle.density(og, {"z": np.zeros(nz), "tt": tt, "ss": ss, "hhq": hhq, "den": zero, "den_pot": zero})
+    assert np.array_equal(surf["den"][:, inner], surf["den_pot"][:, inner])
+    full = oracle.density(og, {"z": z, "tt": tt, "ss": ss, "hhq": hhq, "den": zero, "den_pot": zero})
+    assert np.array_equal(full["den_pot"], surf["den_pot"])
+    assert np.all(np.abs(full["den"] - full["den_pot"])[:, inner] > 0.0)
+    # same pressure in two columns: hhq z equal when hhq doubles and z halves
+    col = {"tt": np.broadcast_to(tt[:, 5:6, 5:6], (nz,) + sh).copy(),
+           "ss": np.broadcast_to(ss[:, 5:6, 5:6], (nz,) + sh).copy()}
+    a = oracle.density(og, dict(col, z=z, hhq=np.full(sh, 40.0), den=zero, den_pot=zero))
+    b2 = oracle.density(og, dict(col, z=z * 0.5, hhq=np.full(sh, 80.0), den=zero, den_pot=zero))
+    assert np.allclose(a["den"][:, inner], b2["den"][:, inner], rtol=0, atol=1e-12)

@@ -129,3 +129,24 @@ def test_pressure_gradients_bitwise(seed, metrics):
     assert_bitwise(got, want, ("rhsx", "rhsy"), f"pressure gradients seed {seed} {metrics}")
     for k in ("rhsx", "rhsy"):
         assert np.any(want[k] != inp[k]) and np.any(want[k] == inp[k])
+
+
+@pytest.mark.parametrize("seed,metrics", [(31, "uniform"), (32, "stretched")])
+def test_density_bitwise(seed, metrics):
+    """The density definition (Control/dyncall.f90:437-449, restated in the
+    harness) with the reference's own density function
+    (Service/oc_algorithms.f90:138-161), against the oracle on the inputs it
+    wrote; cells off lu and outside m, n = 2..n-1 keep their values."""
+    b = make_basin(285, 159, seed=seed, sigma=4.0, g=9.8, hmax=120.0, metrics=metrics)
+    b = randomize_state(b, seed + 1, amp_uv=0.05)
+    rd = oracle.OracleGrid(b).rayleigh()
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "case.bin")
+        write_case(path, b, rd, 1, 2, 0, 7)
+        ref = oracle.run_reference(path, path + ".out", b.nx, b.ny, 7)
+    inp, want = ref["eos_in"], ref["eos_out"]
+    assert inp["tt"].shape[0] == 20
+    got = oracle.density(oracle.OracleGrid(b), inp)
+    assert_bitwise(got, want, ("den", "den_pot"), f"density seed {seed} {metrics}")
+    assert np.any(want["den"] == 7.0) and np.any(want["den"] != 7.0)
+    assert np.any(want["den"] != want["den_pot"])
