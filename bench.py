@@ -65,7 +65,8 @@ TRAFFIC_KERNELS = {"fp64": ("k_fused_step2<double, false, true>",),
                    "ensemble_fp32": ("k_fused_batch2<float, false, true>",),
                    "forcing": ("k_forcing<",),
                    "vertical_velocity": ("k_vertical(",),
-                   "pressure_gradients": ("k_pressure(",)}
+                   "pressure_gradients": ("k_pressure(",),
+                   "density": ("k_density(",)}
 
 
 # kernels of one 2-D -> 3-D coupling and their launches per coupling
@@ -517,6 +518,49 @@ def vertical_leg(m, b, nz, nrep):
                          "bytes_per_call": alg}}
 
 
+# Algorithmic bytes of the density definition per array cell: tt, ss read
+# and den, den_pot written (32 B x nz); hhq read (8 B) and lu as float (4 B).
+def density_leg(m, b, nz, nrep):
+    """The density definition (Control/dyncall.f90:437-449) on the same
+    basin with nz levels: device time (HIP events) and the host-array call;
+    the oracle's one-thread C on the same call as the CPU baseline."""
+    rng = np.random.default_rng(23)
+    sh = (b.ny, b.nx)
+    s3 = (nz,) + sh
+    f = {"z": np.sort(rng.random(nz)), "tt": 2.0 + 25.0 * rng.random(s3), "ss": 5.0 + 30.0 * rng.random(s3),
+         "hhq": 10.0 + 90.0 * rng.random(sh), "den": np.zeros(s3), "den_pot": np.zeros(s3)}
+    args = (f["z"], f["tt"], f["ss"], f["hhq"], f["den"], f["den_pot"])
+    m.density(*args)
+    t0 = time.perf_counter()
+    m.density(*args)
+    host_s = time.perf_counter() - t0
+    m.density_timed(nz, 2)
+    avg_s = m.density_timed(nz, nrep) * 1e-3 / nrep
+    alg = (32 * nz + 12) * b.nx * b.ny
+    achieved = alg / avg_s / 1e9
+    log(f"density nz={nz}: device {avg_s * 1e3:.3f} ms ({achieved:.0f} GB/s), host-array call "
+        f"{host_s * 1e3:.1f} ms")
+    return f, {"levels": nz, "value": b.n_wet * nz / avg_s, "unit": "wet-cell levels of the density definition/s",
+               "device_ms": avg_s * 1e3, "host_call_ms": host_s * 1e3,
+               "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                            "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_from_profiles("density"),
+                            "bytes_per_call": alg}}
+
+
+def density_cpu(b, f):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    og = oracle.OracleGrid(b)
+    t0 = time.perf_counter()
+    oracle.density(og, f)
+    dt = time.perf_counter() - t0
+    nz = f["tt"].shape[0]
+    return {"value": b.n_wet * nz / dt, "unit": "wet-cell levels of the density definition/s", "cores": 1,
+            "kind": "port", "sample": f"one call on the same basin with {nz} levels, {dt:.2f} s, "
+                                      "oracle/bt_oracle.c (bto_density, one thread)"}
+
+
 def pressure_cpu(b, f):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -765,9 +809,10 @@ def main():
             baroclinic, baroclinic_inp = baroclinic_leg(m, b, args.baroclinic_levels, args.baroclinic_reps)
         vertical = vertical_leg(m, b, args.baroclinic_levels, args.baroclinic_reps) \
             if args.baroclinic_levels > 0 else None
-        pressure = None
+        pressure = density = None
         if args.baroclinic_levels > 0:
             pressure_inp, pressure = pressure_leg(m, b, args.baroclinic_levels, args.baroclinic_reps)
+            density_inp, density = density_leg(m, b, args.baroclinic_levels, args.baroclinic_reps)
         snapshot = snapshot_leg(m, b, args.snapshots) if args.snapshots > 0 else None
 
     errs = {}
@@ -830,6 +875,11 @@ def main():
             pressure["cpu_baseline"] = pressure_cpu(b, pressure_inp)
         del pressure_inp
         line["pressure_gradients"] = pressure
+    if density is not None:
+        if not args.no_cpu_baseline:
+            density["cpu_baseline"] = density_cpu(b, density_inp)
+        del density_inp
+        line["density"] = density
     if snapshot is not None:
         line["snapshot"] = snapshot
     if args.ensemble_steps > 0:

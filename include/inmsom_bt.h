@@ -411,6 +411,25 @@ int inmsom_bt_pressure_gradients_dev(inmsom_bt_ctx *ctx, int nz, const double *z
  * reference. */
 int inmsom_bt_pressure_gradients_timed(inmsom_bt_ctx *ctx, int nz, int nrep, double *ms);
 
+/* The density definition of the 3-D step, Control/dyncall.f90:437-449,
+ * with the equation of state of Service/oc_algorithms.f90:138-161 (Brydon
+ * et al. 1999): on lu points of m = 2..nx-1, n = 2..ny-1 and every level,
+ * den = density(tt, ss, FreeFallAcc * RefDen * hhq * z(k)) and den_pot =
+ * density(tt, ss, 0), deviations from 1020 kg/m^3; tt, ss, den, den_pot
+ * (nz, ny, nx), hhq (ny, nx), z[nz].  den, den_pot are read and written
+ * (cells the reference does not write keep their values); den is what
+ * inmsom_bt_pressure_gradients reads.  Host arrays; the _dev variant takes
+ * device addresses (z a host array) and returns after the work has
+ * finished. */
+int inmsom_bt_density(inmsom_bt_ctx *ctx, int nz, const double *z, const double *tt, const double *ss,
+                      const double *hhq, double *den, double *den_pot);
+int inmsom_bt_density_dev(inmsom_bt_ctx *ctx, int nz, const double *z, const double *tt, const double *ss,
+                          const double *hhq, double *den, double *den_pot);
+/* Timing: nrep repetitions of the device work on the fields the last
+ * inmsom_bt_density call with nz levels staged; *ms = HIP-event time of
+ * all.  No counterpart in the reference. */
+int inmsom_bt_density_timed(inmsom_bt_ctx *ctx, int nz, int nrep, double *ms);
+
 /* Timing of the coupling's device work: nrep repetitions of the kernels of
  * inmsom_bt_remove_barotropic then inmsom_bt_internal_update on the fields
  * the last inmsom_bt_internal_update call left on the device (it must have

@@ -125,6 +125,10 @@ struct inmsom_bt_ctx {
     size_t pg_n = 0;               // doubles in pg
     int pg_nz = 0;                 // levels of the staged fields (0 = none)
     double *pg_z = nullptr;        // z of the device-address call (up to 1000 levels)
+    double *eos = nullptr;         // staging of the density definition (bt_density.hip)
+    size_t eos_n = 0;              // doubles in eos
+    int eos_nz = 0;                // levels of the staged fields (0 = none)
+    double *eos_z = nullptr;       // z of the device-address call (up to 1000 levels)
     float *snap_dev = nullptr;     // packed snapshot fields (bt_snapshot.hip)
     float *snap_host = nullptr;    // pinned copy of snap_dev
     size_t snap_n = 0;             // floats in snap_dev / snap_host

@@ -272,6 +272,21 @@ interface
     integer(c_int), value :: nz
     type(c_ptr), value :: z, den, hhq, hhu, hhv, rhsx, rhsy
   end function
+  ! the density definition (Control/dyncall.f90:437-449)
+  integer(c_int) function inmsom_bt_density(ctx, nz, z, tt, ss, hhq, den, den_pot) &
+      bind(C, name="inmsom_bt_density")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: nz
+    type(c_ptr), value :: z, tt, ss, hhq, den, den_pot
+  end function
+  integer(c_int) function inmsom_bt_density_dev(ctx, nz, z, tt, ss, hhq, den, den_pot) &
+      bind(C, name="inmsom_bt_density_dev")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: nz
+    type(c_ptr), value :: z, tt, ss, hhq, den, den_pot
+  end function
   function c_strlen(s) bind(C, name="strlen")
     import :: c_ptr, c_size_t
     type(c_ptr), value :: s
@@ -417,5 +432,18 @@ subroutine pressure_gradients_gpu(nz, den, RHSx, RHSy, z, hhq, hhu, hhv)
   call bt_check(inmsom_bt_pressure_gradients(bt_ctx, int(nz, c_int), c_loc(z), c_loc(den), c_loc(hhq), &
                 c_loc(hhu), c_loc(hhv), c_loc(RHSx), c_loc(RHSy)), 'pressure_gradients_gpu')
 end subroutine pressure_gradients_gpu
+
+! The density definition of Control/dyncall.f90:437-449 (the loop over lu
+! points calling Service/oc_algorithms.f90:138-161) on the context bt_ctx:
+! den at the level's pressure and den_pot at zero pressure from tt, ss, with
+! the sigma levels z(nz) and the depth hhq.  den, den_pot are read and
+! written.
+subroutine density_gpu(nz, tt, ss, den, den_pot, z, hhq)
+  integer, intent(in) :: nz
+  real(8), target, intent(in) :: tt(*), ss(*), z(*), hhq(*)
+  real(8), target, intent(inout) :: den(*), den_pot(*)
+  call bt_check(inmsom_bt_density(bt_ctx, int(nz, c_int), c_loc(z), c_loc(tt), c_loc(ss), c_loc(hhq), &
+                c_loc(den), c_loc(den_pot)), 'density_gpu')
+end subroutine density_gpu
 
 end module inmsom_bt

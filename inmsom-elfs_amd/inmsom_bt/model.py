@@ -459,6 +459,45 @@ class BarotropicModel:
               "inmsom_bt_pressure_gradients_timed")
         return ms.value
 
+    def density(self, z, tt, ss, hhq, den, den_pot):
+        """Control/dyncall.f90:437-449 with Service/oc_algorithms.f90:138-161
+        on host arrays: tt, ss, den, den_pot (nz, ny, nx), hhq (ny, nx), z[nz].
+        Returns the new (den, den_pot) as new arrays (cells the reference
+        does not write keep their values)."""
+        t = np.ascontiguousarray(tt, dtype=np.float64)
+        nz = t.shape[0]
+        s = np.ascontiguousarray(ss, dtype=np.float64)
+        d = np.array(den, dtype=np.float64, order="C")
+        dp = np.array(den_pot, dtype=np.float64, order="C")
+        if t.shape != (nz, self.ny, self.nx) or any(a.shape != t.shape for a in (s, d, dp)):
+            raise ValueError("tt, ss, den, den_pot must be (nz, ny, nx)")
+        h = np.ascontiguousarray(hhq, dtype=np.float64)
+        if h.shape != (self.ny, self.nx):
+            raise ValueError("hhq must be (ny, nx)")
+        zc = np.ascontiguousarray(z, dtype=np.float64)
+        if zc.shape != (nz,):
+            raise ValueError("z must have nz values")
+        check(lib().inmsom_bt_density(self._h, nz, _d(zc), _d(t), _d(s), _d(h), _d(d), _d(dp)),
+              "inmsom_bt_density")
+        return d, dp
+
+    def density_dev(self, z, tt, ss, hhq, den, den_pot):
+        """As density on DeviceArray arguments; den, den_pot updated in place."""
+        zc = np.ascontiguousarray(z, dtype=np.float64)
+        nz = zc.shape[0]
+        s3, s2 = (nz, self.ny, self.nx), (self.ny, self.nx)
+        check(lib().inmsom_bt_density_dev(self._h, nz, _d(zc), self._dev(tt, s3), self._dev(ss, s3),
+                                          self._dev(hhq, s2), self._dev(den, s3), self._dev(den_pot, s3)),
+              "inmsom_bt_density_dev")
+
+    def density_timed(self, nz: int, nrep: int) -> float:
+        """HIP-event time (ms) of nrep repetitions of the density
+        definition's device work on the fields the last host-array call
+        staged."""
+        ms = C.c_double()
+        check(lib().inmsom_bt_density_timed(self._h, int(nz), int(nrep), C.byref(ms)), "inmsom_bt_density_timed")
+        return ms.value
+
     def forcing_3d_timed(self, nz: int, nrep: int) -> float:
         """HIP-event time (ms) of nrep repetitions of the forcing's device
         work on the fields the last forcing_3d call left on the device."""
