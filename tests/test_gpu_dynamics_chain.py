@@ -1,10 +1,11 @@
 """The dynamics of one 3-D model step in the reference's order
-(Control/dyncall.f90:37-366 without the vertical mixing, tracers and
+(Control/dyncall.f90:37-449 without the vertical mixing, tracers and
 turbulence), every block on the GPU through the C ABI, against the same
-chain of oracle routines, bit for bit at every hand-over:
-pressure_gradients -> forcing of the barotropic step -> barotropic_dynamics
-(the drop-in) -> removal of the barotropic part -> vertical_velocity ->
-internal update (ssh_internal, re-add, pgrx) -> baroclinic_dynamics."""
+chain of oracle routines, bit for bit at every hand-over: the density
+definition -> pressure_gradients -> forcing of the barotropic step ->
+barotropic_dynamics (the drop-in) -> removal of the barotropic part ->
+vertical_velocity -> internal update (ssh_internal, re-add, pgrx) ->
+baroclinic_dynamics."""
 import ctypes as C
 
 import numpy as np
@@ -36,7 +37,8 @@ def test_dynamics_step_chain(nz, metrics):
     f = _fields(b, nz, seed=nz + 3)
     wflux = np.ascontiguousarray(b.wflux, dtype=np.float64)
     z = np.sort(rng.random(nz))
-    den = 1025.0 + 0.3 * np.arange(1, nz + 1)[:, None, None] + rng.standard_normal(s3)
+    tt = 2.0 + 25.0 * rng.random(s3)
+    ss = 5.0 + 30.0 * rng.random(s3)
     zero3, zero2 = np.zeros(s3), np.zeros(sh)
     coupling_in = {"slpr": f["slpr"], "wflux": wflux, "ssh_i": 0.2 * rng.standard_normal(sh) * b.wet,
                    "sshp_i": 0.2 * rng.standard_normal(sh) * b.wet}
@@ -49,7 +51,8 @@ def test_dynamics_step_chain(nz, metrics):
 
     # ---- the oracle chain
     L = oracle.lib()
-    pg = oracle.pressure_gradients(og, {"z": z, "den": den, "rhsx": zero3, "rhsy": zero3, "hhq": f["hhq"],
+    eos = oracle.density(og, {"z": z, "tt": tt, "ss": ss, "hhq": f["hhq"], "den": zero3, "den_pot": zero3})
+    pg = oracle.pressure_gradients(og, {"z": z, "den": eos["den"], "rhsx": zero3, "rhsy": zero3, "hhq": f["hhq"],
                                         "hhu": f["hhu"], "hhv": f["hhv"]})
     frc = oracle.forcing_3d(og, dict(f, pgx=pg["rhsx"], pgy=pg["rhsy"], rhsx2d=zero2, rhsy2d=zero2), b.cb_nl,
                             b.ebottom)
@@ -71,6 +74,7 @@ def test_dynamics_step_chain(nz, metrics):
 
     # ---- the same chain on the GPU
     with BarotropicModel(b, rdis=rd) as m:
+        den, den_pot = m.density(z, tt, ss, f["hhq"], zero3, zero3)
         pgx, pgy = m.pressure_gradients(z, den, zero3, zero3, f["hhq"], f["hhu"], f["hhv"])
         frc_g = m.forcing_3d(dict(f, taux=b.taux, tauy=b.tauy), f["dz"], zero2, zero2, pgx, pgy)
         st_g = {k: np.ascontiguousarray(getattr(b, k)).copy() for k in ("u", "up", "v", "vp", "ssh", "sshp")}
@@ -91,6 +95,7 @@ def test_dynamics_step_chain(nz, metrics):
                                               pgry=cpl["pgry"], hhu=f["hhu"], hhun=cpl["hhun"], hhv=f["hhv"],
                                               hhvn=cpl["hhvn"]))
 
+    assert_bitwise({"den": den, "den_pot": den_pot}, eos, ("den", "den_pot"), "density")
     assert_bitwise({"pgx": pgx, "pgy": pgy}, {"pgx": pg["rhsx"], "pgy": pg["rhsy"]}, ("pgx", "pgy"), "pressure")
     assert_bitwise(frc_g, frc, ("rhsx2d", "rhsy2d"), "forcing")
     assert_bitwise(st_g, st_o, tuple(st_o), "barotropic state")
