@@ -6,7 +6,7 @@
 ! reference prints (time_barotrop).
 !
 ! usage: inmsom_bt_host <par dir> <case.bin> <out.bin> [literal|fp64_exact|fp64|fp32] [g] [coupling.bin|-]
-!                       [forcing.bin|-] [baroclinic.bin|-] [pressure.bin]
+!                       [forcing.bin|-] [baroclinic.bin|-] [pressure.bin|-] [density.bin]
 !
 !   ocean_run.par  line 2 time_step [s], line 3 run_duration [days],
 !                  line 9 nstep_barotrop                 (inmsom_head.f90:47-55)
@@ -39,7 +39,11 @@
 !   pressure.bin   optional: int32 nz, real8 z(nz), then den, RHSx, RHSy
 !                  (nx x ny x nz), hhq, hhu, hhv (nx x ny).  The host runs
 !                  pressure_gradients (Function/vel_ssh.f90:320-376) and
-!                  appends RHSx, RHSy to out.bin.
+!                  appends RHSx, RHSy to out.bin ('-': none).
+!   density.bin    optional: int32 nz, real8 z(nz), then tt, ss, den, den_pot
+!                  (nx x ny x nz), hhq (nx x ny).  The host runs the density
+!                  definition (Control/dyncall.f90:437-449) and appends den,
+!                  den_pot to out.bin.
 !
 ! Mode 'literal' runs the stage-by-stage device path (any metrics, the
 ! biharmonic term); the fused modes need uniform metrics.
@@ -48,7 +52,7 @@ use iso_c_binding
 use inmsom_bt
 implicit none
 
-character(512) :: pardir, case_path, out_path, arg, cpl_path, frc_path, bcl_path, prs_path
+character(512) :: pardir, case_path, out_path, arg, cpl_path, frc_path, bcl_path, prs_path, eos_path
 character(256) :: lines(64)
 character(16) :: mode_name
 integer :: nl, ierr, magic, version, nx, ny, nsteps_case, nstep_case, ksw4_case, cmode, nout
@@ -66,6 +70,8 @@ real(8), allocatable, target :: f3(:,:,:,:), f2(:,:,:), dzv(:)
 real(8), allocatable, target :: b3(:,:,:,:), bw(:,:,:,:), b2(:,:,:), bdz(:), bhzt(:)
 ! pressure gradients: p3 = den RHSx RHSy; p2 = hhq hhu hhv
 real(8), allocatable, target :: p3(:,:,:,:), p2(:,:,:), pz(:)
+! density: e3 = tt ss den den_pot; e2 = hhq
+real(8), allocatable, target :: e3(:,:,:,:), e2(:,:), ez(:)
 integer(c_long_long) :: bad
 integer(8) :: c0, c1, crate
 real(8) :: time_barotrop, vol0, vol1
@@ -201,8 +207,15 @@ if (command_argument_count() >= 8) then
 endif
 if (command_argument_count() >= 9) then
   call get_command_argument(9, prs_path)
-  call pressure_step(trim(prs_path))
-  write(23) p3(:,:,:,2:3)
+  if (trim(prs_path) /= '-') then
+    call pressure_step(trim(prs_path))
+    write(23) p3(:,:,:,2:3)
+  endif
+endif
+if (command_argument_count() >= 10) then
+  call get_command_argument(10, eos_path)
+  call density_step(trim(eos_path))
+  write(23) e3(:,:,:,3:4)
 endif
 close(23)
 write(*,'(a,es12.5)') ' Time barotropic: ', time_barotrop
@@ -302,6 +315,25 @@ subroutine pressure_step(path)
   call pressure_gradients_gpu(nzp, p3(:,:,:,1), p3(:,:,:,2), p3(:,:,:,3), pz, p2(:,:,1), p2(:,:,2), p2(:,:,3))
   write(*,'(a,i0,a)') ' pressure gradients done (', nzp, ' levels)'
 end subroutine pressure_step
+
+! The density definition (Control/dyncall.f90:437-449) on the fields of `path`.
+subroutine density_step(path)
+  character(*), intent(in) :: path
+  integer :: io, nze
+  open(28, file=path, access='stream', form='unformatted', status='old', iostat=io)
+  if (io /= 0) then
+    write(*,*) 'cannot open ', path
+    error stop 5
+  endif
+  read(28) nze
+  allocate(ez(nze), e3(nx, ny, nze, 4), e2(nx, ny))
+  read(28) ez
+  read(28) e3
+  read(28) e2
+  close(28)
+  call density_gpu(nze, e3(:,:,:,1), e3(:,:,:,2), e3(:,:,:,3), e3(:,:,:,4), ez, e2)
+  write(*,'(a,i0,a)') ' density done (', nze, ' levels)'
+end subroutine density_step
 
 ! Service/read_write_parameters.f90:2-41: the file's lines, one per entry.
 subroutine read_lines(path, buf, n)

@@ -237,3 +237,37 @@ def test_fortran_host_pressure_gradients_bitwise():
     got = dict(zip(("rhsx", "rhsy"), raw[6 * per:].reshape(2, nz, b.ny, b.nx)))
     assert "pressure gradients done (20 levels)" in p.stdout
     assert_bitwise(got, want, ("rhsx", "rhsy"), "fortran host pressure gradients")
+
+
+def test_fortran_host_density_bitwise():
+    """The host's density step (density_gpu through the ISO_C_BINDING
+    module, Control/dyncall.f90:437-449 with Service/oc_algorithms.f90:
+    138-161) against the oracle, bit for bit."""
+    b = randomize_state(make_basin(66, 48, seed=43, sigma=5.0, metrics="stretched"), 4)
+    rd = oracle.OracleGrid(b).rayleigh()
+    nz = 20
+    rng = np.random.default_rng(33)
+    sh = (b.ny, b.nx)
+    s3 = (nz,) + sh
+    f = {"z": np.sort(rng.random(nz)), "tt": 2.0 + 25.0 * rng.random(s3), "ss": 5.0 + 30.0 * rng.random(s3),
+         "den": np.full(s3, 3.0), "den_pot": np.full(s3, -3.0), "hhq": 10.0 + 90.0 * rng.random(sh)}
+    want = oracle.density(oracle.OracleGrid(b), f)
+    with tempfile.TemporaryDirectory() as tmp:
+        case = os.path.join(tmp, "case.bin")
+        eos = os.path.join(tmp, "density.bin")
+        out = os.path.join(tmp, "out.bin")
+        write_case(case, b, rd, 1, 2, 0, 0, [])
+        with open(eos, "wb") as fh:
+            np.array([nz], dtype="<i4").tofile(fh)
+            f["z"].astype("<f8").tofile(fh)
+            for k in ("tt", "ss", "den", "den_pot", "hhq"):
+                np.ascontiguousarray(f[k], dtype="<f8").tofile(fh)
+        p = subprocess.run([HOST, PARDIR, case, out, "literal", repr(b.g), "-", "-", "-", "-", eos],
+                           capture_output=True, text=True, timeout=120)
+        print(p.stdout, p.stderr)
+        assert p.returncode == 0, p.stdout + p.stderr
+        raw = np.fromfile(out, dtype="<f8")
+    per = b.nx * b.ny
+    got = dict(zip(("den", "den_pot"), raw[6 * per:].reshape(2, nz, b.ny, b.nx)))
+    assert "density done (20 levels)" in p.stdout
+    assert_bitwise(got, want, ("den", "den_pot"), "fortran host density")
