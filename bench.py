@@ -511,7 +511,7 @@ def vertical_leg(m, b, nz, nrep):
     achieved = alg / avg_s / 1e9
     log(f"vertical velocity nz={nz}: device {avg_s * 1e3:.3f} ms ({achieved:.0f} GB/s), host-array call "
         f"{host_s * 1e3:.1f} ms")
-    return {"levels": nz, "value": b.n_wet * nz / avg_s, "unit": "wet-cell levels of vertical_velocity/s",
+    return f, {"levels": nz, "value": b.n_wet * nz / avg_s, "unit": "wet-cell levels of vertical_velocity/s",
             "device_ms": avg_s * 1e3, "host_call_ms": host_s * 1e3,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_from_profiles("vertical_velocity"),
@@ -559,6 +559,20 @@ def density_cpu(b, f):
     return {"value": b.n_wet * nz / dt, "unit": "wet-cell levels of the density definition/s", "cores": 1,
             "kind": "port", "sample": f"one call on the same basin with {nz} levels, {dt:.2f} s, "
                                       "oracle/bt_oracle.c (bto_density, one thread)"}
+
+
+def vertical_cpu(b, f):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    og = oracle.OracleGrid(b)
+    t0 = time.perf_counter()
+    oracle.vertical_velocity(og, f)
+    dt = time.perf_counter() - t0
+    nz = f["uu"].shape[0]
+    return {"value": b.n_wet * nz / dt, "unit": "wet-cell levels of vertical_velocity/s", "cores": 1,
+            "kind": "port", "sample": f"one call on the same basin with {nz} levels, {dt:.2f} s, "
+                                      "oracle/bt_oracle.c (bto_vertical_velocity, one thread)"}
 
 
 def pressure_cpu(b, f):
@@ -807,8 +821,9 @@ def main():
         baroclinic = None
         if args.baroclinic_levels > 0:
             baroclinic, baroclinic_inp = baroclinic_leg(m, b, args.baroclinic_levels, args.baroclinic_reps)
-        vertical = vertical_leg(m, b, args.baroclinic_levels, args.baroclinic_reps) \
-            if args.baroclinic_levels > 0 else None
+        vertical = None
+        if args.baroclinic_levels > 0:
+            vertical_inp, vertical = vertical_leg(m, b, args.baroclinic_levels, args.baroclinic_reps)
         pressure = density = None
         if args.baroclinic_levels > 0:
             pressure_inp, pressure = pressure_leg(m, b, args.baroclinic_levels, args.baroclinic_reps)
@@ -869,6 +884,9 @@ def main():
         del baroclinic_inp
         line["baroclinic"] = baroclinic
     if vertical is not None:
+        if not args.no_cpu_baseline:
+            vertical["cpu_baseline"] = vertical_cpu(b, vertical_inp)
+        del vertical_inp
         line["vertical_velocity"] = vertical
     if pressure is not None:
         if not args.no_cpu_baseline:
