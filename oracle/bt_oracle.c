@@ -13,6 +13,23 @@
  * (Service/oc_algorithms.f90:50-135) is restated as the copy a
  * decomposed run performs: column mm -> mmm-1 and mmm -> mm+1 (rows
  * likewise).
+ *
+ * Pinning (what checks each function, all under tests/):
+ *   - the barotropic step and its helpers: bit for bit against the
+ *     compiled reference through oracle/ref_harness.f90 modes 0 and 1
+ *     (test_oracle_golden.py, committed digests; test_oracle_reference_live.py)
+ *     and by closed forms (test_oracle_invariants.py);
+ *   - bto_remove_barotropic / bto_internal_update (mode 2), bto_forcing_3d
+ *     (mode 3), bto_baroclinic_dynamics (mode 4), bto_vertical_velocity
+ *     (mode 5), bto_pressure_gradients (mode 6), bto_density (mode 7): bit
+ *     for bit against the reference's own routines
+ *     (test_oracle_reference_live.py) and by closed forms or invariants
+ *     (test_oracle_invariants.py);
+ *   - bto_rayleigh_init: by its closed form on a linear slope only (the
+ *     reference computes it inside an initialisation that needs input
+ *     files it does not ship);
+ *   - the periodic wrap: by translation invariance only (the shipped
+ *     reference has periodicity 0).
  */
 #include "bt_oracle.h"
 
