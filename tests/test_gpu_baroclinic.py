@@ -47,7 +47,8 @@ def _channel(nxi, nyi, seed, all_wet=False):
 CASES = [("closed nz=20", lambda: randomize_state(make_basin(150, 90, seed=3, sigma=5.0), 4), (0, 0), 20),
          ("stretched nz=7", lambda: make_basin(77, 61, seed=5, sigma=4.0, metrics="stretched"), (0, 0), 7),
          ("channel px nz=20", lambda: _channel(96, 40, 6), (1, 0), 20),
-         ("doubly periodic nz=3", lambda: _channel(40, 30, 7, all_wet=True), (1, 1), 3)]
+         ("doubly periodic nz=3", lambda: _channel(40, 30, 7, all_wet=True), (1, 1), 3),
+         ("largest nz=128", lambda: make_basin(30, 24, seed=8, sigma=3.0), (0, 0), 128)]
 
 
 @pytest.mark.parametrize("name,make,periodicity,nz", CASES, ids=[c[0] for c in CASES])

@@ -31,7 +31,8 @@ def _channel(nxi, nyi, seed, all_wet=False):
 CASES = [("closed stretched nz=20", lambda: randomize_state(make_basin(120, 88, seed=4, sigma=5.0,
                                                                        metrics="stretched"), 5), (0, 0), 20),
          ("channel px nz=7", lambda: _channel(90, 40, 6), (1, 0), 7),
-         ("doubly periodic nz=1", lambda: _channel(40, 30, 7, all_wet=True), (1, 1), 1)]
+         ("doubly periodic nz=1", lambda: _channel(40, 30, 7, all_wet=True), (1, 1), 1),
+         ("largest nz=1000", lambda: make_basin(24, 18, seed=8, sigma=3.0), (0, 0), 1000)]
 
 
 @pytest.mark.parametrize("name,make,periodicity,nz", CASES, ids=[c[0] for c in CASES])
