@@ -38,7 +38,7 @@ def _device(m, tau, inp):
 
 @pytest.mark.parametrize("nz,periodicity,metrics,ffs", [(1, (0, 0), "uniform", 1), (20, (0, 0), "stretched", 1),
                                                          (5, (1, 0), "uniform", 1), (3, (1, 1), "stretched", 1),
-                                                         (4, (0, 0), "uniform", 0)])
+                                                         (4, (0, 0), "uniform", 0), (400, (1, 0), "stretched", 1)])
 def test_coupling_matches_oracle(nz, periodicity, metrics, ffs):
     print(f"coupling nz={nz} periodicity={periodicity} {metrics} ffs={ffs}", flush=True)
     b = randomize_state(make_basin(70, 50, seed=31 + nz, sigma=4.0, metrics=metrics), 5)

@@ -34,7 +34,8 @@ def _fields(b, nz, seed):
 
 
 @pytest.mark.parametrize("nz,periodicity,metrics,pg", [(1, (0, 0), "uniform", True), (20, (0, 0), "stretched", True),
-                                                       (6, (1, 0), "uniform", False), (5, (1, 1), "stretched", True)])
+                                                       (6, (1, 0), "uniform", False), (5, (1, 1), "stretched", True),
+                                                       (1000, (0, 0), "uniform", True)])
 def test_forcing_3d_matches_oracle(nz, periodicity, metrics, pg):
     print(f"forcing 3-D nz={nz} periodicity={periodicity} {metrics} pg={pg}", flush=True)
     b = randomize_state(make_basin(70, 54, seed=60 + nz, sigma=4.0, metrics=metrics), 3)
