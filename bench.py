@@ -619,6 +619,32 @@ def pressure_leg(m, b, nz, nrep):
                          "bytes_per_call": alg}}
 
 
+def dropin_leg(m, b, calls=3):
+    """The reference's own boundary as a host-resident caller uses it: one
+    call of barotropic_dynamics (Function/vel_ssh.f90:379-679,
+    nstep_barotrop substeps with their look-ahead) on host arrays, the
+    state, forcing and outputs crossing PCIe (pageable memory) every call.
+    The PCIe-inclusive rate, reported beside the headline, never as it."""
+    sh = (b.ny, b.nx)
+    st = [np.ascontiguousarray(getattr(b, k), dtype=np.float64).copy() for k in ("u", "up", "v", "vp", "ssh", "sshp")]
+    acc = [np.zeros(sh) for _ in range(4)]
+    rhs = [np.zeros(sh), np.zeros(sh)]
+    frc = [np.ascontiguousarray(a, dtype=np.float64) for a in (b.wflux, b.mu, b.mu4, m.rdis)]
+    args = (b.dt, NSTEP_BAROTROP, 0, *st, *acc, *rhs, *frc)
+    m.barotropic_dynamics(*args)
+    times = []
+    for _ in range(calls):
+        t0 = time.perf_counter()
+        m.barotropic_dynamics(*args)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    log(f"drop-in host-array call: {t * 1e3:.1f} ms per call ({NSTEP_BAROTROP} substeps), "
+        f"{b.n_wet * NSTEP_BAROTROP / t:.3e} wet-cell steps/s PCIe included")
+    return {"value": b.n_wet * NSTEP_BAROTROP / t, "unit": "wet-cell steps/s (host arrays, PCIe included)",
+            "call_ms": t * 1e3, "calls": calls, "substeps_per_call": NSTEP_BAROTROP,
+            "entry": "inmsom_bt_barotropic_dynamics"}
+
+
 def snapshot_leg(m, b, count):
     """The state written in the reference's output formats (ssh, ubrtr,
     vbrtr as DIOGIN float32 records with GrADS descriptors, one record per
@@ -829,6 +855,7 @@ def main():
             pressure_inp, pressure = pressure_leg(m, b, args.baroclinic_levels, args.baroclinic_reps)
             density_inp, density = density_leg(m, b, args.baroclinic_levels, args.baroclinic_reps)
         snapshot = snapshot_leg(m, b, args.snapshots) if args.snapshots > 0 else None
+        dropin = dropin_leg(m, b) if "fp64" in modes else None
 
     errs = {}
     if "fp64" in states:
@@ -898,6 +925,8 @@ def main():
             density["cpu_baseline"] = density_cpu(b, density_inp)
         del density_inp
         line["density"] = density
+    if dropin is not None:
+        line["dropin_host_arrays"] = dropin
     if snapshot is not None:
         line["snapshot"] = snapshot
     if args.ensemble_steps > 0:
