@@ -61,7 +61,7 @@ __global__ __launch_bounds__(256) void k_density(EosParams p)
     double *__restrict__ den = p.den;
     double *__restrict__ dpot = p.den_pot;
     const double gh = g.g * g.rho0 * p.hhq[i];
-#pragma unroll 4
+#pragma unroll 8
     for (int k = 0; k < p.nz; ++k) {
         const size_t o = (size_t)k * pl + i;
         const double t = tt[o], s = ss[o];
