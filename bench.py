@@ -34,18 +34,27 @@ from inmsom_bt.workloads import NSTEP_BAROTROP, single_basin  # noqa: E402
 METRIC = "wet-cell steps/s on 1 GPU: plain FP64 vs FP32/batched/learned at fixed SSH err"
 PEAK_HBM_GBS = 8000.0
 
-# Algorithmic bytes per streamed cell per substep, nstep_barotrop = 2: level
-# n and filtered level n-1 of (u, v, ssh) read and written, the depth at rest
-# and the Rayleigh coefficient read, and the two RHS components frozen over
-# the call (written by the first substep, read by the second: one 8-byte
-# (FP32: 4-byte) array access per component per substep on average).  A
-# launch of two substeps is credited twice this; it moves far less, since
-# the intermediate level never leaves the registers (roofline.traffic).
+# Algorithmic bytes per streamed cell per substep (the survey's minimal
+# count): level n and filtered level n-1 of (u, v, ssh) read and written
+# (FP64: 12 x 8 B; FP32 stencils: level n 6 x 4 B, the FP64 filtered level
+# 6 x 8 B), the depth at rest and the Rayleigh coefficient read (2 x 8 B;
+# FP32 2 x 4 B).  Wind stress, Coriolis and viscosity are row tables or
+# constants (0 B).  A launch of two substeps is credited twice this; it
+# moves far less, since the intermediate level never leaves the registers
+# (roofline.traffic).
 BYTES_PER_CELL = {
-    "fp64": 12 * 8 + 8 + 8 + 2 * 8,          # 128
-    "fp64_exact": 12 * 8 + 8 + 8 + 2 * 8,    # 128
-    "fp32": 6 * 4 + 6 * 8 + 4 + 4 + 2 * 4,   # 88
+    "fp64": 12 * 8 + 8 + 8,                # 112
+    "fp64_exact": 12 * 8 + 8 + 8,          # 112
+    "fp32": 6 * 4 + 6 * 8 + 4 + 4,         # 80
 }
+# The same count with the two RHS components frozen over a call, which a
+# substep of the reference reads (written by the first substep of a call,
+# read by the second: one access per component per substep on average with
+# nstep_barotrop = 2); reported beside the minimal count, never as it.
+BYTES_PER_CELL_WITH_RHS = {"fp64": 128, "fp64_exact": 128, "fp32": 88}
+# Warm-up before a timed region: at least the requested warm-up steps and
+# at least this many seconds of them, so that the clocks have settled.
+WARMUP_SECONDS = 2.0
 
 
 def log(msg):
@@ -161,11 +170,25 @@ def traffic_from_profiles(mode):
         return None
 
 
+def warm_up(run, sync, steps, seconds=WARMUP_SECONDS):
+    """Runs at least ``steps`` warm-up steps and keeps running them, in
+    doubling batches, until ``seconds`` have passed; returns (steps run,
+    seconds)."""
+    t0 = time.perf_counter()
+    done, batch = 0, max(int(steps), 1)
+    while True:
+        run(batch)
+        sync()
+        done += batch
+        if done >= steps and time.perf_counter() - t0 >= seconds:
+            return done, time.perf_counter() - t0
+        batch = min(batch * 2, 4096)
+
+
 def timed_mode(m, b, mode, steps, warmup, ncell):
+    warm_steps, warm_s = warm_up(lambda k: m.run(k, mode, nstep=NSTEP_BAROTROP), m.synchronize, warmup)
     m.set_state(b.u, b.up, b.v, b.vp, b.ssh, b.sshp)
     vol0 = m.volume()
-    if warmup:
-        m.run(warmup, mode, nstep=NSTEP_BAROTROP)
     m.synchronize()
     t0 = time.perf_counter()
     ms = m.run_timed(steps, mode, nstep=NSTEP_BAROTROP)
@@ -186,9 +209,12 @@ def timed_mode(m, b, mode, steps, warmup, ncell):
         "wall_s": wall,
         "avg_launch_us": avg_launch_s * 1e6,
         "substeps_per_launch": subs,
+        "warmup_steps_run": warm_steps, "warmup_s": warm_s,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "bytes_per_cell_substep": bpc, "cells_per_launch": ncell, "substeps_per_launch": subs},
+                     "bytes_per_cell_substep": bpc, "cells_per_launch": ncell, "substeps_per_launch": subs,
+                     "frac_with_frozen_rhs": BYTES_PER_CELL_WITH_RHS[mode] * subs * ncell / avg_launch_s / 1e9
+                     / PEAK_HBM_GBS},
         "range_check_bad_cells": bad,
         "volume_rel_drift": abs(vol1 - vol0) / max(abs(vol0), 1e-300),
     }
@@ -207,11 +233,16 @@ def cpu_baseline(b, steps):
     t0 = time.perf_counter()
     oracle.run(b, steps, NSTEP_BAROTROP, out_steps=[], rdis=rdis, grid=og)
     dt = time.perf_counter() - t0
-    return {"value": b.n_wet * steps * NSTEP_BAROTROP / dt, "unit": "wet-cell steps/s",
+    # the oracle runs the reference's barotropic_dynamics: 2 * nstep_barotrop
+    # substeps per outer step (nstep state-advancing, nstep look-ahead), each
+    # the work of one fused substep; every substep it runs is credited
+    subs = 2 * NSTEP_BAROTROP
+    return {"value": b.n_wet * steps * subs / dt, "unit": "wet-cell steps/s",
             "cores": threads, "kind": "port",
-            "sample": f"{steps} outer steps (nstep_barotrop={NSTEP_BAROTROP}, with the reference's "
-                      f"look-ahead substeps) of the same 2048x1536 basin, {dt:.1f} s, oracle/bt_oracle.c "
-                      f"with OpenMP"}
+            "sample": f"{steps} outer steps of the same 2048x1536 basin through the oracle's "
+                      f"barotropic_dynamics (oracle/bt_oracle.c, OpenMP), {dt:.1f} s; each outer step runs "
+                      f"{subs} leap-frog substeps ({NSTEP_BAROTROP} state-advancing + {NSTEP_BAROTROP} "
+                      f"look-ahead) and all {subs} are counted"}
 
 
 def ensemble_leg(steps, warmup, nmembers):
@@ -227,10 +258,9 @@ def ensemble_leg(steps, warmup, nmembers):
         nmem, wet, cells = ens.size()
         tune = ens.autotune(nstep=NSTEP_BAROTROP, trial_steps=10)
         log(f"ensemble autotune: rows/wave {tune['rows_per_wave']}, waves/XCD {tune['waves_per_xcd']}")
+        warm_steps, warm_s = warm_up(lambda k: ens.run(k, NSTEP_BAROTROP), lambda: None, warmup)
         ens.reset()
         v0 = [ens.volume(k) for k in range(nmem)]
-        if warmup:
-            ens.run(warmup, NSTEP_BAROTROP)
         ms = ens.run_timed(steps, NSTEP_BAROTROP)
         bad = ens.range_check()
         drift = max(abs(ens.volume(k) - v0[k]) / abs(v0[k]) for k in range(nmem))
@@ -244,9 +274,12 @@ def ensemble_leg(steps, warmup, nmembers):
            "members": nmem, "wet_cells": wet, "array_cells": cells, "steps": steps,
            "ms_per_step": ms / steps, "avg_launch_us": avg * 1e6, "substeps_per_launch": subs,
            "launch": {"rows_per_wave": tune["rows_per_wave"], "waves_per_xcd": tune["waves_per_xcd"]},
+           "warmup_steps_run": warm_steps, "warmup_s": warm_s,
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_from_profiles("ensemble_fp32"),
-                        "bytes_per_cell_substep": bpc, "cells_per_launch": cells, "substeps_per_launch": subs},
+                        "bytes_per_cell_substep": bpc, "cells_per_launch": cells, "substeps_per_launch": subs,
+                        "frac_with_frozen_rhs": BYTES_PER_CELL_WITH_RHS["fp32"] * subs * cells / avg / 1e9
+                        / PEAK_HBM_GBS},
            "range_check_bad_cells": bad, "max_volume_rel_drift": drift}
     log(f"ensemble fp32: {ms:.1f} ms for {steps} steps ({avg * 1e6:.1f} us/launch), "
         f"{res['value']:.3e} wet-cell steps/s, {achieved:.0f} GB/s")

@@ -89,6 +89,15 @@ void infer_destroy(Infer *inf);
 void closure_launch(const inmsom_bt_cnn *net, hipStream_t s, bool f32, ClosureMember *members, Infer &inf,
                     double dt, double ratio);
 int net_device(const inmsom_bt_cnn *net);
+// Changes whenever something that closure_launch passes by value (the
+// normalisation and output kind) changes: graphs captured under an older
+// version replay stale arguments and must be captured again.
+unsigned net_version(const inmsom_bt_cnn *net);
+// Holders of the network (ensembles that use it as their closure): on
+// inmsom_bt_cnn_destroy every holder still attached is told through
+// detach(owner) before the network's memory is freed.
+void net_attach(inmsom_bt_cnn *net, void *owner, void (*detach)(void *owner));
+void net_detach(inmsom_bt_cnn *net, void *owner);
 
 }  // namespace cnn
 }  // namespace ibt

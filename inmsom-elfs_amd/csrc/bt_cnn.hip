@@ -657,6 +657,9 @@ struct inmsom_bt_cnn {
     Layer L[NL];
     size_t nparam = 0;
     Norm norm;
+    unsigned version = 0;  // bumped when norm changes (captured graphs hold it by value)
+    // ensembles using this network as their closure, told on destroy
+    std::vector<std::pair<void *, void (*)(void *)>> holders;
     float *wm = nullptr, *am = nullptr, *av = nullptr, *grad = nullptr;  // master, Adam m/v, gradient
     bf16 *wf = nullptr, *wt = nullptr;                                   // BF16 forward / transposed
     size_t wf_off[NL] = {}, wt_off[NL] = {};
@@ -929,6 +932,20 @@ void closure_launch(const inmsom_bt_cnn *net, hipStream_t st, bool f32, ClosureM
 }
 
 int net_device(const inmsom_bt_cnn *net) { return net->device; }
+unsigned net_version(const inmsom_bt_cnn *net) { return net->version; }
+void net_attach(inmsom_bt_cnn *net, void *owner, void (*detach)(void *owner))
+{
+    net->holders.emplace_back(owner, detach);
+}
+void net_detach(inmsom_bt_cnn *net, void *owner)
+{
+    auto &h = net->holders;
+    for (size_t i = 0; i < h.size();)
+        if (h[i].first == owner)
+            h.erase(h.begin() + (long)i);
+        else
+            ++i;
+}
 
 }  // namespace cnn
 }  // namespace ibt
@@ -996,6 +1013,12 @@ int inmsom_bt_cnn_create(int batch, int ny, int nx, int device, inmsom_bt_cnn **
 int inmsom_bt_cnn_destroy(inmsom_bt_cnn *c)
 {
     if (!c) return INMSOM_BT_OK;
+    // ensembles still holding the network drop it (and their graphs) first
+    while (!c->holders.empty()) {
+        auto h = c->holders.back();
+        c->holders.pop_back();
+        h.second(h.first);
+    }
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (void *p : {(void *)c->wm, (void *)c->am, (void *)c->av, (void *)c->grad, (void *)c->wf,
@@ -1027,6 +1050,7 @@ int inmsom_bt_cnn_set_norm(inmsom_bt_cnn *c, const float *in_shift, const float 
     }
     c->norm.oscale[0] = out_scale[0];
     c->norm.oscale[1] = out_scale[1];
+    ++c->version;
     return INMSOM_BT_OK;
 }
 
@@ -1035,6 +1059,7 @@ int inmsom_bt_cnn_set_output(inmsom_bt_cnn *c, int kind)
     if (!c) return cfail("null pointer", INMSOM_BT_EINVAL);
     if (kind != OUT_UV && kind != OUT_PSI) return cfail("unknown output kind", INMSOM_BT_EINVAL);
     c->norm.kind = kind;
+    ++c->version;
     return INMSOM_BT_OK;
 }
 

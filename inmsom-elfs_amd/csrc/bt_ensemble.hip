@@ -132,6 +132,7 @@ struct inmsom_bt_ens {
     int subs = 0;           // substeps per launch: 0 = by mode, 1, 2
     int built_chunk = -1;
     bool built_two = false;
+    unsigned table_gen = 0;  // bumped whenever build_tables replaces table[] / item_*
     int phase = 0;          // buffer set holding the current state
     // graph of one outer step for (tau, nstep, start phase)
     // graphs of 1 and of GRAPH_STEPS outer steps, per start phase
@@ -139,12 +140,17 @@ struct inmsom_bt_ens {
     double g_tau = -1.0;
     int g_nstep = -1, g_chunk = -1, g_wpx = -1;
     bool g_two = false;
+    // what the graphs' kernel nodes point at or hold by value: the tables'
+    // generation, and the closure network and its version (0 / null: none)
+    unsigned g_table_gen = 0, g_net_version = 0;
+    const inmsom_bt_cnn *g_net = nullptr;
     double *rows = nullptr;
     unsigned long long *cnt = nullptr;
     // snapshot of every member's state (inmsom_bt_ens_save / _restore)
     std::vector<void *> snap;
     // learned closure applied after every outer step
     inmsom_bt_cnn *closure = nullptr;
+    bool closure_lost = false;  // the attached network was destroyed: runs fail until set_closure
     double closure_ratio = 3.0;
     ibt::cnn::Infer inf;
     ibt::cnn::ClosureMember *cl_mem[2] = {};  // device tables per buffer set
@@ -409,6 +415,7 @@ int build_tables(inmsom_bt_ens *e, int chunk, bool two)
     e->total = (int)im.size();
     e->built_chunk = chunk;
     e->built_two = two;
+    ++e->table_gen;
     return INMSOM_BT_OK;
 }
 
@@ -443,6 +450,7 @@ void outer_step(inmsom_bt_ens *e, int phase, double tau, int nstep, int wpx, boo
 
 void drop_closure(inmsom_bt_ens *e)
 {
+    if (e->closure) ibt::cnn::net_detach(e->closure, e);
     ibt::cnn::infer_destroy(&e->inf);
     for (auto &t : e->cl_mem)
         if (t) {
@@ -468,11 +476,33 @@ void drop_graphs(inmsom_bt_ens *e)
     e->g_nstep = -1;
 }
 
+// The graphs' kernel nodes hold the table and work-list addresses of the
+// build they were captured under, and the closure's normalisation by
+// value: they are valid only while none of those has changed.
+bool graphs_current(const inmsom_bt_ens *e, double tau, int nstep, int chunk, int wpx, bool two)
+{
+    const unsigned nv = e->closure ? ibt::cnn::net_version(e->closure) : 0u;
+    return e->gexec[0][0] && e->g_tau == tau && e->g_nstep == nstep && e->g_chunk == chunk &&
+           e->g_wpx == wpx && e->g_two == two && e->g_table_gen == e->table_gen && e->g_net == e->closure &&
+           e->g_net_version == nv;
+}
+
+// inmsom_bt_cnn_destroy on the attached network: its memory is about to
+// be freed, so the graphs that point at it and the inference buffers go now.
+void closure_destroyed(void *owner)
+{
+    auto *e = static_cast<inmsom_bt_ens *>(owner);
+    hipSetDevice(e->device);
+    if (e->stream) hipStreamSynchronize(e->stream);
+    drop_graphs(e);
+    e->closure = nullptr;  // already detached by the network
+    drop_closure(e);
+    e->closure_lost = true;
+}
+
 int ensure_graphs(inmsom_bt_ens *e, double tau, int nstep, int chunk, int wpx, bool two)
 {
-    if (e->gexec[0][0] && e->g_tau == tau && e->g_nstep == nstep && e->g_chunk == chunk &&
-        e->g_wpx == wpx && e->g_two == two)
-        return INMSOM_BT_OK;
+    if (graphs_current(e, tau, nstep, chunk, wpx, two)) return INMSOM_BT_OK;
     drop_graphs(e);
     const bool flip = launches_per_step(nstep, two) % 2;
     const int nph = flip ? 2 : 1;
@@ -496,6 +526,9 @@ int ensure_graphs(inmsom_bt_ens *e, double tau, int nstep, int chunk, int wpx, b
     e->g_chunk = chunk;
     e->g_wpx = wpx;
     e->g_two = two;
+    e->g_table_gen = e->table_gen;
+    e->g_net = e->closure;
+    e->g_net_version = e->closure ? ibt::cnn::net_version(e->closure) : 0u;
     return INMSOM_BT_OK;
 }
 
@@ -510,6 +543,9 @@ void launch_shape(const inmsom_bt_ens *e, bool *two, int *wpx, int *chunk)
 
 int run_steps(inmsom_bt_ens *e, double tau, int nstep, int nsteps, int use_graph)
 {
+    if (e->closure_lost)
+        return fail("the closure network of this ensemble was destroyed: attach another or detach it "
+                    "(inmsom_bt_ens_set_closure)", INMSOM_BT_ESTATE);
     bool two;
     int wpx, chunk;
     launch_shape(e, &two, &wpx, &chunk);
@@ -841,6 +877,7 @@ int inmsom_bt_ens_set_closure(inmsom_bt_ens *e, inmsom_bt_cnn *net, double ratio
     EHC(hipStreamSynchronize(e->stream));
     drop_graphs(e);
     drop_closure(e);
+    e->closure_lost = false;
     if (!net) return INMSOM_BT_OK;
     if (ibt::cnn::net_device(net) != e->device) return fail("network and ensemble on different devices",
                                                             INMSOM_BT_EINVAL);
@@ -874,6 +911,7 @@ int inmsom_bt_ens_set_closure(inmsom_bt_ens *e, inmsom_bt_cnn *net, double ratio
         }
     }
     e->closure = net;
+    ibt::cnn::net_attach(net, e, closure_destroyed);
     e->closure_ratio = ratio;
     return INMSOM_BT_OK;
 }

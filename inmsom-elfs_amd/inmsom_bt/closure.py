@@ -56,7 +56,8 @@ class ClosureNet:
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
-            lib().inmsom_bt_cnn_destroy(self._h)
+            # refused (and the network kept) while an ensemble holds it as its closure
+            check(lib().inmsom_bt_cnn_destroy(self._h), "inmsom_bt_cnn_destroy")
             self._h = None
 
     def __del__(self):

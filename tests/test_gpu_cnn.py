@@ -272,3 +272,40 @@ def test_ensemble_closure_psi_matches_oracle():
             dv = dt * NORM["out_scale"][0] * t[..., 1] * lcv
             assert rel_l2(a["u"] - b["u"], du) <= FWD_TOL, rel_l2(a["u"] - b["u"], du)
             assert rel_l2(a["v"] - b["v"], dv) <= FWD_TOL, rel_l2(a["v"] - b["v"], dv)
+
+
+def test_closure_graphs_follow_norm_changes():
+    """The closure's normalisation is captured into the ensemble's graphs by
+    value: after set_norm (and set_output) on an attached network, graph
+    replays equal direct launches bit for bit.  Destroying an attached
+    network detaches it: the ensemble's next run fails with a clear error
+    (no replay of freed memory) until a closure is set or cleared."""
+    from inmsom_bt._capi import BtError
+
+    basins = _closure_members()
+    p = co.init_params(12, out_scale=0.3)
+    other = dict(shift=[0.01, -0.02, 0.0, 90.0], scale=[4.0, 6.0, 2.0, 0.02], out_scale=[2e-5, 0.5e-5])
+    with Ensemble(basins, mode="fp64") as g, Ensemble(basins, mode="fp64") as d:
+        net = ClosureNet(1, 8, 32)
+        net.set_params(p)
+        net.set_norm(**NORM)
+        g.set_closure(net, ratio=1e30)
+        d.set_closure(net, ratio=1e30)
+        for stage, change in enumerate((None, "norm", "output", None)):
+            print(f"closure graph stage {stage}: {change}", flush=True)
+            if change == "norm":
+                net.set_norm(**other)
+            elif change == "output":
+                net.set_output("psi")
+            g.run(9, nstep=2, graph=True)
+            d.run(9, nstep=2, graph=False)
+            for k in range(len(basins)):
+                a, b = g.get_state(k), d.get_state(k)
+                for f in ("ssh", "sshp", "u", "up", "v", "vp"):
+                    assert np.array_equal(a[f].view(np.uint64), b[f].view(np.uint64)), (stage, k, f)
+        net.close()
+        for ens, graph in ((g, True), (d, False)):
+            with pytest.raises(BtError, match="was destroyed"):
+                ens.run(1, nstep=2, graph=graph)
+            ens.set_closure(None)
+            ens.run(1, nstep=2, graph=graph)

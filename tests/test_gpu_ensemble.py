@@ -98,3 +98,24 @@ def test_autotune_restores_state_and_conserves_volume(tmp_path):
         again = ens.autotune(nstep=2, trial_steps=3, rows=(12, 24), waves=(128, 384),
                              cache=str(tmp_path / "tune.json"))
         assert again["cached"]
+
+
+def test_graphs_follow_table_rebuilds():
+    """Graph run at launch shape A, a direct run at shape B (which rebuilds
+    the member tables and work list the graphs point at), then a graph run
+    at A again: every stage equals direct launches bit for bit (graphs
+    captured under replaced tables are captured again, not replayed)."""
+    basins = _members(4, seed=13)
+    with Ensemble(basins, mode="fp64") as g, Ensemble(basins, mode="fp64") as d:
+        for stage, (rows, waves, graph) in enumerate(((0, 0, True), (5, 61, False), (0, 0, True),
+                                                      (5, 61, True), (0, 0, True))):
+            print(f"graph rebuild stage {stage}: rows {rows} waves {waves} graph {graph}", flush=True)
+            g.set_launch(rows, waves)
+            d.set_launch(rows, waves)
+            g.run(GRAPH_STEPS + 1, nstep=2, graph=graph)
+            d.run(GRAPH_STEPS + 1, nstep=2, graph=False)
+            for k in range(len(basins)):
+                assert_bitwise(g.get_state(k), d.get_state(k), STATE, f"stage {stage} member {k}")
+
+
+GRAPH_STEPS = 8  # outer steps per replay of the ensemble's long graph

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import oracle
+from _util import periodic_pair
 from inmsom_bt.cases import make_basin, randomize_state
 
 
@@ -543,3 +544,29 @@ def test_density_special_cases():
     a = oracle.density(og, dict(col, z=z, hhq=np.full(sh, 40.0), den=zero, den_pot=zero))
     b2 = oracle.density(og, dict(col, z=z * 0.5, hhq=np.full(sh, 80.0), den=zero, den_pot=zero))
     assert np.allclose(a["den"][:, inner], b2["den"][:, inner], rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("periodicity", [(1, 0), (1, 1)])
+def test_periodic_wrap_equals_doubled_domain(periodicity):
+    """Periodic wrap (Service/oc_algorithms.f90:50-135, as a decomposed run
+    applies it: column mm is copied to mmm-1 and mmm to mm+1, likewise for
+    rows): a periodic basin of width W advanced N steps equals, bit for bit,
+    each copy inside a basin of width 2W (and height 2H when y is periodic
+    too) holding the W basin's fields tiled twice, where the copies meet
+    without a wrap.  A wrap that copies a column to its own neighbour, or is
+    off by one, breaks the equality at the seam.  Bathymetry, state and
+    water flux vary along x (no symmetry the wrap could hide behind)."""
+    b, b2, (tx, ty) = periodic_pair(periodicity)
+    steps = 15
+    rd1 = oracle.OracleGrid(b, periodicity).rayleigh()
+    rd2 = oracle.OracleGrid(b2, periodicity).rayleigh()
+    o1 = oracle.run(b, steps, 2, out_steps=[steps], rdis=rd1, periodicity=periodicity)[steps]
+    o2 = oracle.run(b2, steps, 2, out_steps=[steps], rdis=rd2, periodicity=periodicity)[steps]
+    W, H = b.nx - 4, b.ny - 4
+    for k in ("ssh", "sshp", "u", "up", "v", "vp", "ubrtr_i", "vbrtr_i", "ssh4gradx", "ssh4grady"):
+        a = o1[k][2:-2, 2:-2]
+        for i in range(tx):
+            for j in range(ty):
+                c = o2[k][2 + j * H:2 + (j + 1) * H, 2 + i * W:2 + (i + 1) * W]
+                assert np.array_equal(a.view(np.uint64), c.view(np.uint64)), (k, i, j)
+    assert np.abs(o1["u"][2:-2, 2:-2]).max() > 0
