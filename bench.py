@@ -73,6 +73,7 @@ TRAFFIC_KERNELS = {"fp64": ("k_fused_step2<double, false, true>",),
                    "fp64_exact": ("k_fused_step<double, true, true>", "k_fused_step<double, true, false>"),
                    "ensemble_fp32": ("k_fused_batch2<float, false, true>",),
                    "forcing": ("k_forcing<",),
+                   "dropin": ("k_dropin<true, true>", "k_dropin<false, true>"),
                    "vertical_velocity": ("k_vertical(",),
                    "pressure_gradients": ("k_pressure(",),
                    "density": ("k_density(",)}
@@ -678,6 +679,57 @@ def dropin_leg(m, b, calls=3):
             "entry": "inmsom_bt_barotropic_dynamics"}
 
 
+def dropin_bytes_per_cell(nstep, row_metrics=True):
+    """Algorithmic HBM bytes per array cell of one fused barotropic_dynamics
+    call (2 * nstep launches): every launch reads the entering state (6 x 8 B),
+    the depth at rest, Rayleigh coefficient, viscosity, water flux and the
+    caller's two RHS components (6 x 8 B); all but the last write the state
+    (48 B) and the two filtered U/V depths they carry (16 B; read by all but
+    the first); the four averages are written by every launch and read by all
+    but the first (32 B each way); launch nstep + 1 writes the call's result
+    (48 B).  Per-cell metrics and Coriolis parameter add 9 x 8 B per launch
+    (row tables: 0)."""
+    n = 2 * nstep
+    per = 6 * 8 + 6 * 8
+    total = n * per + (n - 1) * (48 + 16) + (n - 1) * 16 + n * 32 + (n - 1) * 32 + 48
+    if not row_metrics:
+        total += n * 9 * 8
+    return total
+
+
+def dropin_device_leg(m, b, calls=20, nstep=NSTEP_BAROTROP):
+    """The reference's boundary, device-resident: calls of barotropic_dynamics
+    (Function/vel_ssh.f90:379-679: nstep state-advancing and nstep look-ahead
+    substeps, the four averages, the state after substep nstep returned) on
+    the model's state with the RHS of its wind stress and drag, through the
+    fused kernel and through one kernel per reference sweep.  A call is
+    credited its nstep state-advancing substeps."""
+    out = {}
+    m.set_state(b.u, b.up, b.v, b.vp, b.ssh, b.sshp)
+    for path in ("fused", "literal"):
+        m.set_dropin_path(path)
+        n = calls if path == "fused" else max(calls // 4, 2)
+        m.dropin_timed(2, nstep=nstep)
+        ms = m.dropin_timed(n, nstep=nstep)
+        call_s = ms * 1e-3 / n
+        out[path] = {"call_ms": call_s * 1e3, "calls": n,
+                     "value": b.n_wet * nstep / call_s, "unit": "wet-cell steps/s (device-resident drop-in)"}
+    m.set_dropin_path("auto")
+    m.set_state(b.u, b.up, b.v, b.vp, b.ssh, b.sshp)
+    bpc = dropin_bytes_per_cell(nstep, row_metrics=True)
+    ncell = b.nx * b.ny
+    achieved = bpc * ncell / (out["fused"]["call_ms"] * 1e-3) / 1e9
+    out["fused"]["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                "frac": achieved / PEAK_HBM_GBS, "bytes_per_cell_call": bpc,
+                                "cells": ncell, "launches_per_call": 2 * nstep,
+                                "traffic": (lambda t: None if t is None else t * 2 * nstep)(
+                                    traffic_from_profiles("dropin"))}
+    out["speedup_fused_vs_literal"] = out["literal"]["call_ms"] / out["fused"]["call_ms"]
+    log(f"drop-in device-resident, nstep={nstep}: fused {out['fused']['call_ms']:.3f} ms/call "
+        f"({achieved:.0f} GB/s), stage-by-stage {out['literal']['call_ms']:.3f} ms/call")
+    return out
+
+
 def snapshot_leg(m, b, count):
     """The state written in the reference's output formats (ssh, ubrtr,
     vbrtr as DIOGIN float32 records with GrADS descriptors, one record per
@@ -889,6 +941,7 @@ def main():
             density_inp, density = density_leg(m, b, args.baroclinic_levels, args.baroclinic_reps)
         snapshot = snapshot_leg(m, b, args.snapshots) if args.snapshots > 0 else None
         dropin = dropin_leg(m, b) if "fp64" in modes else None
+        dropin_dev = dropin_device_leg(m, b) if "fp64" in modes else None
 
     errs = {}
     if "fp64" in states:
@@ -960,6 +1013,8 @@ def main():
         line["density"] = density
     if dropin is not None:
         line["dropin_host_arrays"] = dropin
+    if dropin_dev is not None:
+        line["dropin_device"] = dropin_dev
     if snapshot is not None:
         line["snapshot"] = snapshot
     if args.ensemble_steps > 0:

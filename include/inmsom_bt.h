@@ -94,7 +94,12 @@ int inmsom_bt_finalize(inmsom_bt_ctx *ctx);
  * caller (Control/dyncall.f90:39,65,72-84) and are not returned.  The
  * context's external-mode depths are re-initialised from ssh_e/sshp_e at
  * the end, as hh_init does (:668-675); read them with
- * inmsom_bt_get_depths. */
+ * inmsom_bt_get_depths.
+ * Path: one fused kernel launch per substep (bit for bit the reference)
+ * when ksw4 = 0, full_free_surface = 1, the grid is not periodic and the
+ * two halo cells of every side are land; otherwise one kernel per
+ * reference sweep (also bit for bit).  inmsom_bt_set_dropin_path
+ * chooses. */
 int inmsom_bt_barotropic_dynamics(inmsom_bt_ctx *ctx, double tau, int nstep, int ksw4,
                                   double *ubrtr_e, double *ubrtrp_e, double *vbrtr_e,
                                   double *vbrtrp_e, double *ssh_e, double *sshp_e,
@@ -102,6 +107,32 @@ int inmsom_bt_barotropic_dynamics(inmsom_bt_ctx *ctx, double tau, int nstep, int
                                   double *ssh4grady, const double *RHSx, const double *RHSy,
                                   const double *wflux, const double *mu, const double *mu4,
                                   const double *rdis);
+
+/* As inmsom_bt_barotropic_dynamics (Function/vel_ssh.f90:379-679, called
+ * at Control/dyncall.f90:137-164) with every array a device address on the
+ * context's GPU (nx*ny doubles, ready: synchronise the producing stream
+ * first): no host copies.  The context's own state and depths are not
+ * touched.  Returns when the call has completed. */
+int inmsom_bt_barotropic_dynamics_dev(inmsom_bt_ctx *ctx, double tau, int nstep, int ksw4,
+                                      double *ubrtr_e, double *ubrtrp_e, double *vbrtr_e,
+                                      double *vbrtrp_e, double *ssh_e, double *sshp_e,
+                                      double *ubrtr_i, double *vbrtr_i, double *ssh4gradx,
+                                      double *ssh4grady, const double *RHSx, const double *RHSy,
+                                      const double *wflux, const double *mu, const double *mu4,
+                                      const double *rdis);
+
+/* Path of the barotropic_dynamics entries: 0 = fused where the call allows
+ * it (default), 1 = one kernel per reference sweep, 2 = fused (calls it
+ * does not cover fail with INMSOM_BT_ESTATE).  Results are the same bit
+ * for bit.  get: the path (1 or 2) a call with this ksw4 takes. */
+int inmsom_bt_set_dropin_path(inmsom_bt_ctx *ctx, int path);
+int inmsom_bt_get_dropin_path(inmsom_bt_ctx *ctx, int ksw4, int *path);
+
+/* ncalls device-resident calls of barotropic_dynamics on the context's
+ * state (inmsom_bt_set_state / _get_state) with the RHS of the current
+ * state's wind stress and bottom drag, the context's water flux, mu, mu4
+ * and Rayleigh coefficient; *ms = their HIP-event time. */
+int inmsom_bt_dropin_timed(inmsom_bt_ctx *ctx, double tau, int nstep, int ksw4, int ncalls, double *ms);
 
 /* The twelve external-mode depths hhq_e, hhqp_e, hhqn_e, hhu_e, hhup_e,
  * hhun_e, hhv_e, hhvp_e, hhvn_e, hhh_e, hhhp_e, hhhn_e
@@ -118,7 +149,8 @@ int inmsom_bt_set_state(inmsom_bt_ctx *ctx, const double *u, const double *up,
                         const double *sshp);
 int inmsom_bt_get_state(inmsom_bt_ctx *ctx, double *u, double *up, double *v, double *vp,
                         double *ssh, double *sshp);
-/* ubrtr_i, vbrtr_i, ssh4gradx, ssh4grady of the last literal step. */
+/* ubrtr_i, vbrtr_i, ssh4gradx, ssh4grady of the last barotropic_dynamics
+ * call on the context's state. */
 int inmsom_bt_get_means(inmsom_bt_ctx *ctx, double *ubrtr_i, double *vbrtr_i,
                         double *ssh4gradx, double *ssh4grady);
 
@@ -127,7 +159,8 @@ int inmsom_bt_get_means(inmsom_bt_ctx *ctx, double *ubrtr_i, double *vbrtr_i,
  * (Function/fluxes.f90:29-37), quadratic bottom drag from the current
  * external-mode state (:226-272), RHS assembly (dyncall.f90:114-126 with
  * the 3-D terms zero) and barotropic_dynamics(tau, nstep, ksw4), all on
- * the device in the reference's operation order. */
+ * the device in the reference's operation order; barotropic_dynamics takes
+ * the path inmsom_bt_barotropic_dynamics takes (inmsom_bt_set_dropin_path). */
 int inmsom_bt_model_steps(inmsom_bt_ctx *ctx, double tau, int nstep, int ksw4, int nsteps);
 
 /* nsteps outer steps with ksw4 = 0, each through nstep launches of one

@@ -4,6 +4,7 @@
 #include <string>
 #include <vector>
 
+#include "bt_dropin.h"
 #include "bt_fused.h"
 #include "bt_internal.h"
 #include "bt_literal.h"
@@ -313,6 +314,7 @@ int inmsom_bt_finalize(inmsom_bt_ctx *c)
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     fused_release(c);
+    dropin_release(c);
     auto fr = [](void *p) {
         if (p) hipFree(p);
     };
@@ -339,6 +341,53 @@ int inmsom_bt_finalize(inmsom_bt_ctx *c)
     return INMSOM_BT_OK;
 }
 
+// Which path takes a barotropic_dynamics call: the fused drop-in where the
+// context allows it and the caller did not ask for the stage-by-stage path.
+static int dropin_route(inmsom_bt_ctx *c, int ksw4, bool *fused)
+{
+    std::string why;
+    const bool ok = c->dropin_path != 1 && dropin_eligible(c, ksw4, &why);
+    if (c->dropin_path == 2 && !ok) {
+        set_error("fused barotropic_dynamics requested but not available: " + why);
+        return INMSOM_BT_ESTATE;
+    }
+    *fused = ok;
+    return INMSOM_BT_OK;
+}
+
+// dep[] back to hh_init of the context's state after fused drop-in calls
+// changed it (the stage-by-stage path reads dep[] as the reference reads its
+// module depths)
+static void refresh_depths(inmsom_bt_ctx *c)
+{
+    if (!c->dep_stale) return;
+    lit_hh_init(c, c->st[4], c->st[5]);
+    c->dep_stale = false;
+}
+
+// One call on the context's device arrays (state st[], averages acc[]) with
+// the inputs at the given device addresses, by either path.
+static int dynamics_on_context(inmsom_bt_ctx *c, double tau, int nstep, int ksw4, const double *rx,
+                               const double *ry, const double *wf, const double *mu, const double *mu4,
+                               const double *rd)
+{
+    bool fused = false;
+    TRY(dropin_route(c, ksw4, &fused));
+    if (fused) {
+        TRY(dropin_call(c, tau, nstep, c->st, c->acc, rx, ry, wf, mu, rd));
+        c->dep_stale = true;
+    } else {
+        // the depths a reference caller holds at entry are hh_init(ssh_e, sshp_e):
+        // every call ends with hh_init (vel_ssh.f90:668-675), as does set-up
+        // (init_pars.f90:426-430)
+        lit_hh_init(c, c->st[4], c->st[5]);
+        c->dep_stale = false;
+        lit_barotropic_dynamics(c, tau, nstep, ksw4, rx, ry, wf, mu, mu4, rd);
+    }
+    HIPCHECK(hipGetLastError());
+    return INMSOM_BT_OK;
+}
+
 int inmsom_bt_barotropic_dynamics(inmsom_bt_ctx *c, double tau, int nstep, int ksw4,
                                   double *ubrtr_e, double *ubrtrp_e, double *vbrtr_e,
                                   double *vbrtrp_e, double *ssh_e, double *sshp_e,
@@ -348,7 +397,7 @@ int inmsom_bt_barotropic_dynamics(inmsom_bt_ctx *c, double tau, int nstep, int k
                                   const double *rdis)
 {
     REQUIRE(c, "null context");
-    REQUIRE(nstep >= 1, "nstep must be >= 1");
+    REQUIRE(nstep >= 1 && nstep <= 1000, "nstep must be in 1..1000");
     REQUIRE(tau > 0.0, "tau must be positive");
     REQUIRE(ubrtr_e && ubrtrp_e && vbrtr_e && vbrtrp_e && ssh_e && sshp_e && ubrtr_i && vbrtr_i &&
                 ssh4gradx && ssh4grady && RHSx && RHSy && wflux && mu && rdis,
@@ -359,20 +408,14 @@ int inmsom_bt_barotropic_dynamics(inmsom_bt_ctx *c, double tau, int nstep, int k
     const size_t bd = c->n * sizeof(double);
     double *st_in[6] = {ubrtr_e, ubrtrp_e, vbrtr_e, vbrtrp_e, ssh_e, sshp_e};
     for (int k = 0; k < 6; ++k) TRY(upload(c, c->st[k], st_in[k], bd));
-    // inputs go to scratch arrays that the literal path does not otherwise use
+    // inputs go to scratch arrays that the stage-by-stage path does not otherwise use
     TRY(upload(c, c->rhsx, RHSx, bd));
     TRY(upload(c, c->rhsy, RHSy, bd));
     TRY(upload(c, c->surf_x, wflux, bd));
     TRY(upload(c, c->surf_y, mu, bd));
     TRY(upload(c, c->bot_x, mu4 ? mu4 : mu, bd));
     TRY(upload(c, c->bot_y, rdis, bd));
-    // the depths a reference caller holds at entry are hh_init(ssh_e, sshp_e):
-    // every call ends with hh_init (vel_ssh.f90:668-675), as does set-up
-    // (init_pars.f90:426-430)
-    lit_hh_init(c, c->st[4], c->st[5]);
-    lit_barotropic_dynamics(c, tau, nstep, ksw4, c->rhsx, c->rhsy, c->surf_x, c->surf_y, c->bot_x,
-                            c->bot_y);
-    HIPCHECK(hipGetLastError());
+    TRY(dynamics_on_context(c, tau, nstep, ksw4, c->rhsx, c->rhsy, c->surf_x, c->surf_y, c->bot_x, c->bot_y));
     double *st_out[6] = {ubrtr_e, ubrtrp_e, vbrtr_e, vbrtrp_e, ssh_e, sshp_e};
     for (int k = 0; k < 6; ++k)
         HIPCHECK(hipMemcpyAsync(st_out[k], c->st[k], bd, hipMemcpyDeviceToHost, c->stream));
@@ -383,11 +426,98 @@ int inmsom_bt_barotropic_dynamics(inmsom_bt_ctx *c, double tau, int nstep, int k
     return INMSOM_BT_OK;
 }
 
+int inmsom_bt_barotropic_dynamics_dev(inmsom_bt_ctx *c, double tau, int nstep, int ksw4,
+                                      double *ubrtr_e, double *ubrtrp_e, double *vbrtr_e,
+                                      double *vbrtrp_e, double *ssh_e, double *sshp_e,
+                                      double *ubrtr_i, double *vbrtr_i, double *ssh4gradx,
+                                      double *ssh4grady, const double *RHSx, const double *RHSy,
+                                      const double *wflux, const double *mu, const double *mu4,
+                                      const double *rdis)
+{
+    REQUIRE(c, "null context");
+    REQUIRE(nstep >= 1 && nstep <= 1000, "nstep must be in 1..1000");
+    REQUIRE(tau > 0.0, "tau must be positive");
+    REQUIRE(ubrtr_e && ubrtrp_e && vbrtr_e && vbrtrp_e && ssh_e && sshp_e && ubrtr_i && vbrtr_i &&
+                ssh4gradx && ssh4grady && RHSx && RHSy && wflux && mu && rdis,
+            "null array pointer");
+    REQUIRE(ksw4 <= 0 || mu4, "mu4 is required when ksw4 > 0");
+    HIPCHECK(hipSetDevice(c->device));
+    bool fused = false;
+    TRY(dropin_route(c, ksw4, &fused));
+    double *st[6] = {ubrtr_e, ubrtrp_e, vbrtr_e, vbrtrp_e, ssh_e, sshp_e};
+    double *acc[4] = {ubrtr_i, vbrtr_i, ssh4gradx, ssh4grady};
+    if (fused) {
+        TRY(dropin_call(c, tau, nstep, st, acc, RHSx, RHSy, wflux, mu, rdis));
+    } else {
+        // the stage-by-stage path runs on the context's own arrays: the
+        // caller's state goes in and comes back by device copies
+        TRY(fused_flush(c));
+        const size_t bd = c->n * sizeof(double);
+        for (int k = 0; k < 6; ++k)
+            HIPCHECK(hipMemcpyAsync(c->st[k], st[k], bd, hipMemcpyDeviceToDevice, c->stream));
+        lit_hh_init(c, c->st[4], c->st[5]);
+        c->dep_stale = false;
+        lit_barotropic_dynamics(c, tau, nstep, ksw4, RHSx, RHSy, wflux, mu, mu4 ? mu4 : mu, rdis);
+        for (int k = 0; k < 6; ++k)
+            HIPCHECK(hipMemcpyAsync(st[k], c->st[k], bd, hipMemcpyDeviceToDevice, c->stream));
+        for (int k = 0; k < 4; ++k)
+            HIPCHECK(hipMemcpyAsync(acc[k], c->acc[k], bd, hipMemcpyDeviceToDevice, c->stream));
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_set_dropin_path(inmsom_bt_ctx *c, int path)
+{
+    REQUIRE(c, "null context");
+    REQUIRE(path >= 0 && path <= 2, "path must be 0 (automatic), 1 (stage by stage) or 2 (fused)");
+    c->dropin_path = path;
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_get_dropin_path(inmsom_bt_ctx *c, int ksw4, int *path)
+{
+    REQUIRE(c && path, "null pointer");
+    HIPCHECK(hipSetDevice(c->device));
+    bool fused = false;
+    TRY(dropin_route(c, ksw4, &fused));
+    *path = fused ? 2 : 1;
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_dropin_timed(inmsom_bt_ctx *c, double tau, int nstep, int ksw4, int ncalls, double *ms)
+{
+    REQUIRE(c && ms, "null pointer");
+    REQUIRE(nstep >= 1 && nstep <= 1000 && ncalls >= 0 && tau > 0.0, "bad tau / nstep / ncalls");
+    REQUIRE(c->have_forcing, "inmsom_bt_set_forcing must be called first");
+    HIPCHECK(hipSetDevice(c->device));
+    TRY(fused_flush(c));
+    refresh_depths(c);
+    // the RHS of a model step (wind stress and bottom drag of the current state)
+    lit_forcing(c);
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    HIPCHECK(hipEventRecord(e0, c->stream));
+    for (int k = 0; k < ncalls; ++k)
+        TRY(dynamics_on_context(c, tau, nstep, ksw4, c->rhsx, c->rhsy, c->wflux, c->mu, c->mu4, c->rdis));
+    HIPCHECK(hipEventRecord(e1, c->stream));
+    HIPCHECK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHECK(hipEventElapsedTime(&t, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *ms = t;
+    return INMSOM_BT_OK;
+}
+
 int inmsom_bt_get_depths(inmsom_bt_ctx *c, double *const h[12])
 {
     REQUIRE(c && h, "null pointer");
     HIPCHECK(hipSetDevice(c->device));
     TRY(fused_flush(c));
+    refresh_depths(c);
     for (int k = 0; k < 12; ++k) {
         REQUIRE(h[k], "null depth pointer");
         HIPCHECK(hipMemcpyAsync(h[k], c->dep[k], c->n * sizeof(double), hipMemcpyDeviceToHost,
@@ -429,6 +559,7 @@ int inmsom_bt_set_state(inmsom_bt_ctx *c, const double *u, const double *up, con
     c->state_on_fused = false;
     c->fused_mode = -1;
     lit_hh_init(c, c->st[4], c->st[5]);
+    c->dep_stale = false;
     HIPCHECK(hipStreamSynchronize(c->stream));
     return INMSOM_BT_OK;
 }
@@ -468,9 +599,10 @@ int inmsom_bt_model_steps(inmsom_bt_ctx *c, double tau, int nstep, int ksw4, int
     HIPCHECK(hipSetDevice(c->device));
     TRY(fused_flush(c));
     for (int s = 0; s < nsteps; ++s) {
+        // the forcing reads the depths hh_init left (Control/dyncall.f90:110-130)
+        refresh_depths(c);
         lit_forcing(c);
-        lit_barotropic_dynamics(c, tau, nstep, ksw4, c->rhsx, c->rhsy, c->wflux, c->mu, c->mu4,
-                                c->rdis);
+        TRY(dynamics_on_context(c, tau, nstep, ksw4, c->rhsx, c->rhsy, c->wflux, c->mu, c->mu4, c->rdis));
     }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(c->stream));

@@ -1,0 +1,24 @@
+// Host entry points of the fused drop-in of barotropic_dynamics.
+#pragma once
+
+#include <string>
+
+#include "bt_internal.h"
+
+namespace ibt {
+
+// Whether the fused path can take a call with this ksw4 on this context's
+// grid (see bt_dropin.hip); *why says why not.
+bool dropin_eligible(inmsom_bt_ctx *c, int ksw4, std::string *why);
+// One call of barotropic_dynamics (Function/vel_ssh.f90:379-679) on device
+// arrays: the state (u, up, v, vp, ssh, sshp) is replaced by the state
+// after substep nstep, acc (ubrtr_i, vbrtr_i, ssh4gradx, ssh4grady) receives
+// the averages.  Queued on the context's stream.
+int dropin_call(inmsom_bt_ctx *c, double tau, int nstep, double *const st[6], double *const acc[4],
+                const double *rhsx, const double *rhsy, const double *wflux, const double *mu, const double *rdis);
+void dropin_release(inmsom_bt_ctx *c);
+// The launch the fused drop-in uses: rows per wave and whether metrics come
+// from row tables (after the first call; 0 / false before).
+void dropin_shape(const inmsom_bt_ctx *c, int *rows_per_wave, bool *row_metrics);
+
+}  // namespace ibt

@@ -52,7 +52,7 @@ void set_error(const std::string &msg);
 
 }  // namespace ibt
 
-namespace ibt { struct FusedAlloc; }
+namespace ibt { struct FusedAlloc; struct DropinAlloc; }
 
 // Context: every device allocation the library makes for one basin.
 struct inmsom_bt_ctx {
@@ -134,4 +134,8 @@ struct inmsom_bt_ctx {
     size_t snap_n = 0;             // floats in snap_dev / snap_host
     size_t f3_halo_n = 0;          // doubles in f3_halo
     long long *cnt = nullptr;
+    // fused drop-in of barotropic_dynamics (bt_dropin.hip)
+    ibt::DropinAlloc *da = nullptr;
+    int dropin_path = 0;           // 0 = fused where eligible, 1 = stage-by-stage, 2 = fused (error if not eligible)
+    bool dep_stale = false;        // dep[] no longer hh_init(st[4], st[5]) (a fused drop-in call changed st)
 };

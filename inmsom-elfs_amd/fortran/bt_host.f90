@@ -62,6 +62,9 @@ real(8) :: time_step, run_duration, tau_case, cb_nl, ebottom, g
 real(8), allocatable, target :: f(:,:,:)
 real(8), allocatable, target :: st(:,:,:)
 real(8), allocatable, target :: rdis(:,:)
+! the 'dropin' mode: the caller-side arrays of barotropic_dynamics (its RHS,
+! the four averages and the scratch arguments it does not write)
+real(8), allocatable, target :: rhs(:,:,:), acc(:,:,:), scr(:,:,:)
 ! coupling: c2 = ubrtr_i vbrtr_i pgrx pgry hhu hhv slpr uu2d vv2d ssh_i sshp_i hhqn hhun hhvn hhhn
 real(8), allocatable, target :: c2(:,:,:), uu(:,:,:), vv(:,:,:)
 ! forcing: f3 = uu vv uup vvp amuv pgx pgy; f2 = uu2d vv2d uup2d vvp2d amuv2d hhq hhu hhv hhh slpr RHSx2d RHSy2d
@@ -120,6 +123,7 @@ nsteps = nint(run_duration * 86400.0d0 / time_step)
 if (nsteps_case > 0) nsteps = nsteps_case
 select case (trim(mode_name))
 case ('literal');    mode = -1
+case ('dropin');     mode = -2
 case ('fp64_exact'); mode = INMSOM_BT_FP64_EXACT
 case ('fp64');       mode = INMSOM_BT_FP64
 case ('fp32');       mode = INMSOM_BT_FP32
@@ -157,11 +161,30 @@ write(*,'(a,i0,a,i0,a,f8.2,a,i0,a,i0,a,a)') ' grid ', nx, ' x ', ny, ', time ste
       ' s, nstep_barotrop ', nstep_barotrop, ', steps ', nsteps, ', mode ', trim(mode_name)
 
 ! ---- time loop (inmsom_head.f90:236-316, dyncall.f90:109-166)
+if (mode == -2) then
+  ! the caller's RHS of the 'dropin' mode: the wind stress on U and V faces
+  ! as a face force divided by the reference density, fixed over the run
+  allocate(rhs(nx, ny, 2), acc(nx, ny, 4), scr(nx, ny, 9))
+  rhs(:,:,1) = f(:,:,11) * f(:,:,4) * f(:,:,7) / 1025.0d0
+  rhs(:,:,2) = f(:,:,12) * f(:,:,6) * f(:,:,5) / 1025.0d0
+  acc = 0.0d0
+  scr = 0.0d0
+endif
 time_barotrop = 0.0d0
 call system_clock(count_rate=crate)
 do step = 1, nsteps
   call system_clock(c0)
-  if (mode < 0) then
+  if (mode == -2) then
+    ! the reference's own call (Control/dyncall.f90:137-164) on host arrays
+    call barotropic_dynamics_gpu(time_step, nstep_barotrop, 0, st(:,:,3), st(:,:,4), st(:,:,5), st(:,:,6), &
+                                 st(:,:,1), st(:,:,2), acc(:,:,1), acc(:,:,2), acc(:,:,3), acc(:,:,4), &
+                                 rhs(:,:,1), rhs(:,:,2), f(:,:,13), f(:,:,14), f(:,:,15), scr(:,:,1), &
+                                 scr(:,:,2), scr(:,:,3), scr(:,:,4), scr(:,:,5), rdis, scr(:,:,6), &
+                                 scr(:,:,7), scr(:,:,8), scr(:,:,9))
+    ! (the call leaves its result in the context's state too: the range
+    ! check below and the final read see it)
+    rc = INMSOM_BT_OK
+  else if (mode < 0) then
     rc = inmsom_bt_model_steps(bt_ctx, time_step, int(nstep_barotrop, c_int), &
                                int(ksw_lat4, c_int), 1_c_int)
   else
@@ -184,6 +207,7 @@ call bt_check(inmsom_bt_get_state(bt_ctx, c_loc(st(1,1,3)), c_loc(st(1,1,4)), c_
 call bt_check(inmsom_bt_volume(bt_ctx, vol1), 'inmsom_bt_volume')
 open(23, file=trim(out_path), access='stream', form='unformatted', status='replace')
 write(23) st
+if (mode == -2) write(23) acc
 if (command_argument_count() >= 6) then
   call get_command_argument(6, cpl_path)
   if (trim(cpl_path) /= '-') then

@@ -46,6 +46,11 @@ SIGNATURES = {
     "inmsom_bt_finalize": ([C.c_void_p], C.c_int),
     "inmsom_bt_barotropic_dynamics": ([C.c_void_p, C.c_double, C.c_int, C.c_int] + [_dp] * 16,
                                       C.c_int),
+    "inmsom_bt_barotropic_dynamics_dev": ([C.c_void_p, C.c_double, C.c_int, C.c_int] + [_dp] * 16,
+                                          C.c_int),
+    "inmsom_bt_set_dropin_path": ([C.c_void_p, C.c_int], C.c_int),
+    "inmsom_bt_get_dropin_path": ([C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
+    "inmsom_bt_dropin_timed": ([C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int, _dp], C.c_int),
     "inmsom_bt_get_depths": ([C.c_void_p, C.POINTER(_dp)], C.c_int),
     "inmsom_bt_set_forcing": ([C.c_void_p, C.POINTER(Forcing)], C.c_int),
     "inmsom_bt_set_state": ([C.c_void_p] + [_dp] * 6, C.c_int),

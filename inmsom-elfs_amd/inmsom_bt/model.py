@@ -170,6 +170,41 @@ class BarotropicModel:
                                                   *[_d(a) for a in arrs]),
               "inmsom_bt_barotropic_dynamics")
 
+    def barotropic_dynamics_dev(self, tau, nstep, ksw4, ubrtr_e, ubrtrp_e, vbrtr_e, vbrtrp_e, ssh_e,
+                                sshp_e, ubrtr_i, vbrtr_i, ssh4gradx, ssh4grady, RHSx, RHSy, wflux,
+                                mu, mu4, rdis):
+        """As barotropic_dynamics on DeviceArray arguments (inmsom_bt.hipmem)
+        of this model's shape and GPU: the first ten are updated in place,
+        no host copies."""
+        arrs = [ubrtr_e, ubrtrp_e, vbrtr_e, vbrtrp_e, ssh_e, sshp_e, ubrtr_i, vbrtr_i, ssh4gradx,
+                ssh4grady, RHSx, RHSy, wflux, mu, mu4, rdis]
+        s2 = (self.ny, self.nx)
+        check(lib().inmsom_bt_barotropic_dynamics_dev(self._h, float(tau), int(nstep), int(ksw4),
+                                                      *[self._dev(a, s2) for a in arrs]),
+              "inmsom_bt_barotropic_dynamics_dev")
+
+    DROPIN_PATHS = {"auto": 0, "literal": 1, "fused": 2}
+
+    def set_dropin_path(self, path: str = "auto"):
+        """Path of barotropic_dynamics: 'auto' (fused where the call allows
+        it), 'literal' (one kernel per reference sweep) or 'fused'."""
+        check(lib().inmsom_bt_set_dropin_path(self._h, self.DROPIN_PATHS[path]), "inmsom_bt_set_dropin_path")
+
+    def dropin_path(self, ksw4: int = 0) -> str:
+        """The path ('literal' or 'fused') a barotropic_dynamics call with
+        this ksw4 takes."""
+        k = C.c_int()
+        check(lib().inmsom_bt_get_dropin_path(self._h, int(ksw4), C.byref(k)), "inmsom_bt_get_dropin_path")
+        return {1: "literal", 2: "fused"}[k.value]
+
+    def dropin_timed(self, ncalls: int, nstep: int = 1, ksw4: int = 0, tau: Optional[float] = None) -> float:
+        """ncalls device-resident barotropic_dynamics calls on the model's
+        state (RHS of its wind stress and drag); HIP-event time in ms."""
+        ms = C.c_double()
+        check(lib().inmsom_bt_dropin_timed(self._h, float(self.b.dt if tau is None else tau), int(nstep),
+                                           int(ksw4), int(ncalls), C.byref(ms)), "inmsom_bt_dropin_timed")
+        return ms.value
+
     def model_steps(self, nsteps: int, nstep: int = 1, ksw4: int = 0, tau: Optional[float] = None):
         check(lib().inmsom_bt_model_steps(self._h, float(self.b.dt if tau is None else tau),
                                           int(nstep), int(ksw4), int(nsteps)),

@@ -271,3 +271,34 @@ def test_fortran_host_density_bitwise():
     got = dict(zip(("den", "den_pot"), raw[6 * per:].reshape(2, nz, b.ny, b.nx)))
     assert "density done (20 levels)" in p.stdout
     assert_bitwise(got, want, ("den", "den_pot"), "fortran host density")
+
+
+@pytest.mark.parametrize("metrics", ["uniform", "stretched"])
+def test_fortran_host_dropin_bitwise(metrics):
+    """The host's 'dropin' mode calls barotropic_dynamics_gpu, the
+    reference's own argument list (Function/vel_ssh.f90:379-439) through the
+    ISO_C_BINDING module, every step on host arrays with a fixed RHS (the
+    wind stress as a face force over the reference density): the state after
+    the steps and the last call's averages equal the oracle's calls bit for
+    bit."""
+    b = randomize_state(make_basin(96, 64, seed=55, sigma=5.0, metrics=metrics), 6, wflux_amp=1e-6)
+    rd = oracle.OracleGrid(b).rayleigh()
+    steps, nstep = 7, 2  # ocean_run.par: nstep_barotrop = 2
+    og = oracle.OracleGrid(b)
+    rhsx = b.taux * b.dxt * b.dyh / 1025.0
+    rhsy = b.tauy * b.dxh * b.dyt / 1025.0
+    st = {k: np.ascontiguousarray(getattr(b, k)).copy() for k in ("u", "up", "v", "vp", "ssh", "sshp")}
+    for _ in range(steps):
+        acc = {k: np.zeros((b.ny, b.nx)) for k in ("ubrtr_i", "vbrtr_i", "ssh4gradx", "ssh4grady")}
+        oracle.barotropic_dynamics(og, b.dt, nstep, 0, st, acc, rhsx, rhsy, b.wflux, b.mu, b.mu4, rd)
+    with tempfile.TemporaryDirectory() as tmp:
+        case = os.path.join(tmp, "case.bin")
+        out = os.path.join(tmp, "out.bin")
+        write_case(case, b, rd, steps, nstep, 0, 0, [])
+        p = subprocess.run([HOST, PARDIR, case, out, "dropin", repr(b.g)], capture_output=True, text=True,
+                           timeout=120)
+        print(p.stdout, p.stderr)
+        assert p.returncode == 0, p.stdout + p.stderr
+        raw = np.fromfile(out, dtype="<f8").reshape(10, b.ny, b.nx)
+    got = dict(zip(("ssh", "sshp", "u", "up", "v", "vp", "ubrtr_i", "vbrtr_i", "ssh4gradx", "ssh4grady"), raw))
+    assert_bitwise(got, {**st, **acc}, STATE + tuple(acc), f"fortran host drop-in ({metrics})")
