@@ -111,8 +111,9 @@ int inmsom_bt_barotropic_dynamics(inmsom_bt_ctx *ctx, double tau, int nstep, int
 /* As inmsom_bt_barotropic_dynamics (Function/vel_ssh.f90:379-679, called
  * at Control/dyncall.f90:137-164) with every array a device address on the
  * context's GPU (nx*ny doubles, ready: synchronise the producing stream
- * first): no host copies.  The context's own state and depths are not
- * touched.  Returns when the call has completed. */
+ * first): no host copies.  The context's own state, depths and averages
+ * (inmsom_bt_get_state / _get_depths / _get_means) are not changed.
+ * Returns when the call has completed. */
 int inmsom_bt_barotropic_dynamics_dev(inmsom_bt_ctx *ctx, double tau, int nstep, int ksw4,
                                       double *ubrtr_e, double *ubrtrp_e, double *vbrtr_e,
                                       double *vbrtrp_e, double *ssh_e, double *sshp_e,

@@ -324,6 +324,7 @@ int inmsom_bt_finalize(inmsom_bt_ctx *c)
     for (auto *p : c->w) fr(p);
     for (auto *p : c->acc) fr(p);
     for (auto *p : c->dep) fr(p);
+    for (auto *p : c->st_save) fr(p);
     for (void *p : {(void *)c->rlh_s, (void *)c->hr, (void *)c->taux, (void *)c->tauy,
                     (void *)c->wflux, (void *)c->mu, (void *)c->mu4, (void *)c->rdis,
                     (void *)c->vort, (void *)c->str_t, (void *)c->str_s, (void *)c->fx,
@@ -449,19 +450,33 @@ int inmsom_bt_barotropic_dynamics_dev(inmsom_bt_ctx *c, double tau, int nstep, i
     if (fused) {
         TRY(dropin_call(c, tau, nstep, st, acc, RHSx, RHSy, wflux, mu, rdis));
     } else {
-        // the stage-by-stage path runs on the context's own arrays: the
-        // caller's state goes in and comes back by device copies
+        // the stage-by-stage path runs on the context's own arrays: its
+        // state is set aside, the caller's goes in and comes back by device
+        // copies, then the context's own is put back (depths re-derived
+        // from it when next needed)
         TRY(fused_flush(c));
         const size_t bd = c->n * sizeof(double);
-        for (int k = 0; k < 6; ++k)
-            HIPCHECK(hipMemcpyAsync(c->st[k], st[k], bd, hipMemcpyDeviceToDevice, c->stream));
+        double *own[10], *theirs[10];
+        for (int k = 0; k < 6; ++k) {
+            own[k] = c->st[k];
+            theirs[k] = st[k];
+        }
+        for (int k = 0; k < 4; ++k) {
+            own[6 + k] = c->acc[k];
+            theirs[6 + k] = acc[k];
+        }
+        for (int k = 0; k < 10; ++k) {
+            if (!c->st_save[k]) DALLOC(c->st_save[k], bd);
+            HIPCHECK(hipMemcpyAsync(c->st_save[k], own[k], bd, hipMemcpyDeviceToDevice, c->stream));
+            if (k < 6) HIPCHECK(hipMemcpyAsync(own[k], theirs[k], bd, hipMemcpyDeviceToDevice, c->stream));
+        }
         lit_hh_init(c, c->st[4], c->st[5]);
-        c->dep_stale = false;
         lit_barotropic_dynamics(c, tau, nstep, ksw4, RHSx, RHSy, wflux, mu, mu4 ? mu4 : mu, rdis);
-        for (int k = 0; k < 6; ++k)
-            HIPCHECK(hipMemcpyAsync(st[k], c->st[k], bd, hipMemcpyDeviceToDevice, c->stream));
-        for (int k = 0; k < 4; ++k)
-            HIPCHECK(hipMemcpyAsync(acc[k], c->acc[k], bd, hipMemcpyDeviceToDevice, c->stream));
+        for (int k = 0; k < 10; ++k) {
+            HIPCHECK(hipMemcpyAsync(theirs[k], own[k], bd, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHECK(hipMemcpyAsync(own[k], c->st_save[k], bd, hipMemcpyDeviceToDevice, c->stream));
+        }
+        c->dep_stale = true;
     }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(c->stream));

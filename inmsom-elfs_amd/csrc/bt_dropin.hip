@@ -581,19 +581,22 @@ int check_grid(inmsom_bt_ctx *c, DropinAlloc *a)
                 return INMSOM_BT_OK;
             }
         }
-    // metrics and rlh_s uniform along x: row tables
+    // metrics and rlh_s uniform along x on columns 2..nx-1: row tables.  With
+    // land on the halo the outputs never depend on columns 1 and nx (the
+    // reference's own grid set-up leaves its metrics zero there and rlh_s
+    // unscaled, Service/basin_parameters.f90:250-300).
     std::vector<double> f(n), tab((size_t)9 * ny);
     bool rows = true;
     for (int k = 0; k < 9 && rows; ++k) {
         DHC(hipMemcpy(f.data(), k < 8 ? c->metric[k] : c->rlh_s, n * 8, hipMemcpyDeviceToHost));
         for (int r = 0; r < ny && rows; ++r) {
             const double *row = &f[(size_t)r * nx];
-            for (int m = 1; m < nx; ++m)
-                if (std::memcmp(&row[m], &row[0], sizeof(double)) != 0) {
+            for (int m = 2; m < nx - 1; ++m)
+                if (std::memcmp(&row[m], &row[1], sizeof(double)) != 0) {
                     rows = false;
                     break;
                 }
-            tab[(size_t)k * ny + r] = row[0];
+            tab[(size_t)k * ny + r] = row[1];
         }
     }
     a->rowmet_ok = rows;

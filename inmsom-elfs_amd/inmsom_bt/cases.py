@@ -328,22 +328,65 @@ def read_mask(path: str, nx: int, ny: int) -> np.ndarray:
     return np.ascontiguousarray(lb[::-1])
 
 
-def spherical_metrics(nx: int, ny: int, lon0: float, lat0: float, dlon: float, dlat: float,
-                      r: float = EARTH_RADIUS) -> dict:
-    """Metric coefficients of a regular longitude-latitude grid (rx = R cos(lat),
-    ry = R, Inc/basinpar.fi:31-32): x-distances at the latitude of each
-    staggered row (T and U rows at lat_t, V and H rows half a step north),
-    y-distances R * dlat.  This project's construction for the shipped
-    grid; the reference's basinpar is not on the path."""
-    deg = math.pi / 180.0
-    lat_t = lat0 + (np.arange(ny, dtype=np.float64) - 2.0) * dlat
-    lat_v = lat_t + 0.5 * dlat
-    cx_t = r * np.cos(lat_t * deg) * dlon * deg
-    cx_v = r * np.cos(lat_v * deg) * dlon * deg
-    ry = r * dlat * deg
-    ones = np.ones((ny, nx))
-    return {"dx": cx_t[:, None] * ones, "dy": ry * ones, "dxt": cx_t[:, None] * ones, "dyt": ry * ones,
-            "dxh": cx_v[:, None] * ones, "dyh": ry * ones, "dxb": cx_v[:, None] * ones, "dyb": ry * ones}
+def reference_grid(nx: int = None, ny: int = None, rlon: float = None, rlat: float = None, dxst: float = None,
+                   dyst: float = None, curve_grid: int = 1, radius: float = None, omega: float = None) -> dict:
+    """The reference's grid set-up basinpar (Service/basin_parameters.f90:
+    191-300) for a regular grid (xgr_type = ygr_type = 0) with the Cartesian
+    (grid_parameters_carthesian, :764-833) or the unrotated spherical
+    (grid_parameters_spherical, :836-949) metric: the eight metrics and the
+    Coriolis parameter rlh_s, in its operation order.  Defaults: the shipped
+    Inc/basinpar.fi (289 x 163 points, 27.525 E, 40.94 N, 0.05 x 0.04
+    degrees, spherical).
+
+    The metrics are set on points 2..nx-1 x 2..ny-1 (zero outside), rlh_s
+    is 2 Omega everywhere and, on H points of that range, 2 Omega times the
+    sine of the latitude (spherical) or 2 Omega / sqrt(2) (Cartesian).  The
+    degree-to-radian factor is the reference's 3.1415926535897/180.0d0, whose
+    literal is a default (single precision) real.  cos and sin are the C
+    library's (math.cos / math.sin), as the compiled reference calls them."""
+    nx = REF_NX if nx is None else nx
+    ny = REF_NY if ny is None else ny
+    rlon = REF_LON0 if rlon is None else rlon
+    rlat = REF_LAT0 if rlat is None else rlat
+    dxst = REF_DLON if dxst is None else dxst
+    dyst = REF_DLAT if dyst is None else dyst
+    radius = EARTH_RADIUS if radius is None else radius
+    omega = EARTH_OMEGA if omega is None else omega
+    dpip180 = float(np.float32(3.1415926535897)) / 180.0
+    mmm = nnn = 3
+    xt = [rlon + float(m - mmm) * dxst for m in range(1, nx + 1)]   # xt(m) at [m - 1]
+    yt = [rlat + float(n - nnn) * dyst for n in range(1, ny + 1)]
+    xu = [(xt[i] + xt[i + 1]) / 2.0 for i in range(nx - 1)]          # xu(m), m = 1..nx-1
+    yv = [(yt[i] + yt[i + 1]) / 2.0 for i in range(ny - 1)]
+    cx = dxst * dpip180 * radius
+    cy = dyst * dpip180 * radius
+    out = {k: np.zeros((ny, nx)) for k in ("dx", "dy", "dxt", "dyt", "dxh", "dyh", "dxb", "dyb")}
+    rlh_s = np.full((ny, nx), 2.0 * omega)
+    lat_extr = 89.99999
+    sinlat_extr = math.sin(dpip180 * lat_extr)
+    for n in range(2, ny):          # 1-based n = 2..ny-1 (ny_start-1..ny_end+1)
+        j = n - 1
+        for k in ("dx", "dxt", "dxh", "dxb"):
+            out[k][j, 1:nx - 1] = cx
+        for k in ("dy", "dyt", "dyh", "dyb"):
+            out[k][j, 1:nx - 1] = cy
+        if curve_grid == 1:
+            # T and U points on the latitude yt(n), V and H points on yv(n)
+            for k, y in (("dx", yt[j]), ("dxt", yt[j]), ("dxh", yv[j]), ("dxb", yv[j])):
+                lat_mod = max(min(y, lat_extr), -lat_extr)
+                out[k][j, 1:nx - 1] = cx * math.cos(dpip180 * lat_mod)
+            for m in range(2, nx):
+                xm = xu[m - 1]
+                sin_lat = (math.sin(dpip180 * yv[j]) * math.cos(dpip180 * 0.0) +
+                           math.cos(dpip180 * xm) * math.cos(dpip180 * yv[j]) * math.sin(dpip180 * 0.0))
+                sin_lat = min(max(sin_lat, -sinlat_extr), sinlat_extr)
+                rlh_s[j, m - 1] = (2.0 * omega) * sin_lat
+        elif curve_grid == 0:
+            rlh_s[j, 1:nx - 1] = (2.0 * omega) / math.sqrt(2.0)
+        else:
+            raise ValueError("curve_grid must be 0 (Cartesian) or 1 (spherical)")
+    out["rlh_s"] = rlh_s
+    return out
 
 
 def reference_basin(mask_path: str, topo_dir: str, topo_name: str = "topo.dat", *, metrics: str = "spherical",
@@ -362,13 +405,14 @@ def reference_basin(mask_path: str, topo_dir: str, topo_name: str = "topo.dat", 
     rdstd(topo_dir, topo_name, 1, topo, lu, (3, nx - 2, 3, ny - 2, 1, 1))
     hhq_rest = topo.astype(np.float64)
     if metrics == "spherical":
-        met = spherical_metrics(nx, ny, REF_LON0, REF_LAT0, REF_DLON, REF_DLAT)
+        # the reference's own construction (basinpar), metrics and Coriolis
+        met = reference_grid(nx, ny)
+        rlh_s = met.pop("rlh_s")
     else:
         met = uniform_metrics(nx, ny, d)
-    deg = math.pi / 180.0
-    lat_v = REF_LAT0 + (np.arange(ny, dtype=np.float64) - 2.0 + 0.5) * REF_DLAT
-    lat_t = lat_v - 0.5 * REF_DLAT
-    rlh_s = np.repeat((2.0 * EARTH_OMEGA * np.sin(lat_v * deg))[:, None], nx, axis=1)
+        lat_v = REF_LAT0 + (np.arange(ny, dtype=np.float64) - 2.0 + 0.5) * REF_DLAT
+        rlh_s = np.repeat((2.0 * EARTH_OMEGA * np.sin(lat_v * math.pi / 180.0))[:, None], nx, axis=1)
+    lat_t = REF_LAT0 + (np.arange(ny, dtype=np.float64) - 2.0) * REF_DLAT
     y = (lat_t - lat_t[2]) / (lat_t[-3] - lat_t[2])
     taux = np.repeat((tau0 * np.cos(math.pi * y))[:, None], nx, axis=1)
     zero = np.zeros((ny, nx))

@@ -25,11 +25,16 @@
  *     for bit against the reference's own routines
  *     (test_oracle_reference_live.py) and by closed forms or invariants
  *     (test_oracle_invariants.py);
- *   - bto_rayleigh_init: by its closed form on a linear slope only (the
- *     reference computes it inside an initialisation that needs input
- *     files it does not ship);
- *   - the periodic wrap: by translation invariance only (the shipped
- *     reference has periodicity 0).
+ *   - bto_rayleigh_init: by closed forms (the reference computes it inside
+ *     an initialisation that needs input files it does not ship): a
+ *     linear slope, and a depth quadratic in the distances along x and y
+ *     on a grid whose eight metrics all differ, with coasts of one and of
+ *     two open faces (test_oracle_invariants.py; a wrong metric, face side
+ *     or mask fails it);
+ *   - the periodic wrap: by translation invariance and by the equality of
+ *     a periodic basin with each copy of it tiled twice, bit for bit (the
+ *     shipped reference has periodicity 0; a wrap that copies a column to
+ *     its own neighbour or is off by one fails the latter).
  */
 #include "bt_oracle.h"
 

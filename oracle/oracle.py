@@ -338,6 +338,11 @@ def run_reference(case_path: str, out_path: str, nx: int, ny: int, mode: int,
                    stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, cwd=os.path.dirname(out_path))
     raw = np.fromfile(out_path, dtype="<f8")
     per = nx * ny
+    if mode == 8:
+        # masks, then dx, dy, dxt, dyt, dxh, dyh, dxb, dyb, rlh_s of basinpar
+        assert raw.size == 16 * per
+        names = ("dx", "dy", "dxt", "dyt", "dxh", "dyh", "dxb", "dyb", "rlh_s")
+        return {k: raw[(7 + i) * per:(8 + i) * per].reshape(ny, nx).copy() for i, k in enumerate(names)}
     if mode == 7:
         # masks, z (nz), tt, ss, den, den_pot (nz planes), hhq, den, den_pot out
         nz = (raw.size - 8 * per) // (1 + 6 * per)

@@ -58,6 +58,10 @@
 !   own density function (Service/oc_algorithms.f90:138-161) on synthetic
 !   tt, ss, a non-uniform sigma grid and the depths of hh_init: the inputs
 !   (z, tt, ss, the initial den, den_pot, hhq) and den, den_pot are written.
+! mode 8: the reference's grid set-up basinpar (Service/basin_parameters.f90:
+!   191-300 with grid_parameters_spherical / _carthesian, :764-949) for the
+!   compiled Inc/basinpar.fi, on zeroed metric arrays: dx, dy, dxt, dyt, dxh,
+!   dyh, dxb, dyb and rlh_s are written (the case's fields are not used).
 program ref_harness
 use main_basin_pars
 use mpi_parallel_tools
@@ -404,6 +408,14 @@ else if (mode == 7) then
     enddo
   enddo
   write(23) den, den_pot
+else if (mode == 8) then
+  ! the reference's own grid set-up (Service/basin_parameters.f90, basinpar)
+  ! for the compiled Inc/basinpar.fi, from zeroed metric arrays
+  dx = 0.0d0; dy = 0.0d0; dxt = 0.0d0; dyt = 0.0d0
+  dxh = 0.0d0; dyh = 0.0d0; dxb = 0.0d0; dyb = 0.0d0
+  rlh_s = 0.0d0
+  call basinpar
+  write(23) dx, dy, dxt, dyt, dxh, dyh, dxb, dyb, rlh_s
 else
   ! helper routines, one call each, on the case's state arrays
   ! hh_init from (ssh_e, sshp_e)

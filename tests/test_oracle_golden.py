@@ -76,3 +76,18 @@ def test_outputs_bitwise(case):
                 a = outs[int(step)][k]
                 bad.append(f"step {step} {k}: l2 {np.linalg.norm(a):.17g} vs {d['l2']:.17g}")
     assert not bad, f"{name}: " + "; ".join(bad[:6])
+
+
+def test_reference_grid_against_fixture():
+    """The restated grid set-up (inmsom_bt.cases.reference_grid) equals, bit
+    for bit, the metrics and Coriolis parameter the reference's own basinpar
+    computed for its shipped configuration (fixture written by
+    tests/golden/make_grid_golden.py from the compiled reference)."""
+    from inmsom_bt.cases import reference_grid
+
+    with open(os.path.join(HERE, "golden", "reference_grid.json")) as f:
+        gold = json.load(f)
+    got = reference_grid(gold["nx"], gold["ny"])
+    for k, rec in gold["fields"].items():
+        a = got[k]
+        assert sha(a) == rec["sha256"], (k, [float(a[r, c]) for r, c in gold["probes"]], rec["probe_values"])

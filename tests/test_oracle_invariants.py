@@ -570,3 +570,84 @@ def test_periodic_wrap_equals_doubled_domain(periodicity):
                 c = o2[k][2 + j * H:2 + (j + 1) * H, 2 + i * W:2 + (i + 1) * W]
                 assert np.array_equal(a.view(np.uint64), c.view(np.uint64)), (k, i, j)
     assert np.abs(o1["u"][2:-2, 2:-2]).max() > 0
+
+
+def test_reference_grid_closed_forms():
+    """The restated grid set-up in the forms its definitions fix:
+    Cartesian: every metric is dxst (dyst) * pi/180 * R on points
+    2..nx-1 x 2..ny-1 and zero outside, rlh_s = 2 Omega / sqrt(2) there and
+    2 Omega outside; spherical: y-metrics as Cartesian, x-metrics of T and U
+    rows scaled by the cosine of the row latitude and of V and H rows by
+    the cosine half a step north (so dx/dxh is the ratio of those cosines),
+    rlh_s = 2 Omega sin(latitude of the H row)."""
+    from inmsom_bt.cases import EARTH_OMEGA, EARTH_RADIUS, reference_grid
+
+    nx, ny, lon0, lat0, dl, dp = 20, 14, 10.0, 30.0, 0.5, 0.25
+    pif = float(np.float32(math.pi))
+    c = reference_grid(nx, ny, lon0, lat0, dl, dp, curve_grid=0)
+    inner = np.zeros((ny, nx), dtype=bool)
+    inner[1:-1, 1:-1] = True
+    for k in ("dx", "dxt", "dxh", "dxb"):
+        np.testing.assert_allclose(c[k][inner], dl * pif / 180.0 * EARTH_RADIUS, rtol=1e-15)
+        assert np.all(c[k][~inner] == 0.0)
+    for k in ("dy", "dyt", "dyh", "dyb"):
+        np.testing.assert_allclose(c[k][inner], dp * pif / 180.0 * EARTH_RADIUS, rtol=1e-15)
+    np.testing.assert_allclose(c["rlh_s"][inner], 2 * EARTH_OMEGA / math.sqrt(2.0), rtol=1e-15)
+    assert np.all(c["rlh_s"][~inner] == 2 * EARTH_OMEGA)
+    s = reference_grid(nx, ny, lon0, lat0, dl, dp, curve_grid=1)
+    lat_t = lat0 + (np.arange(ny) - 2) * dp
+    lat_v = lat_t + dp / 2
+    rad = pif / 180.0
+    for k in ("dy", "dyt", "dyh", "dyb"):
+        np.testing.assert_array_equal(s[k], c[k])
+    for k, lat in (("dx", lat_t), ("dxt", lat_t), ("dxh", lat_v), ("dxb", lat_v)):
+        want = c[k] * np.cos(lat * rad)[:, None]
+        np.testing.assert_allclose(s[k][inner], want[inner], rtol=1e-14)
+    np.testing.assert_allclose(s["rlh_s"][inner], (2 * EARTH_OMEGA * np.sin(lat_v * rad)[:, None]
+                                                   * np.ones((1, nx)))[inner], rtol=1e-14)
+
+
+def test_rayleigh_stretched_metrics_and_coasts():
+    """Rayleigh coefficient (Control/init_pars.f90:459-471) on a grid whose
+    eight metrics all differ, with a depth quadratic in the distance along
+    x and y built from dxt and dyt alone: the slope on the U face between m
+    and m+1 is then a (X_m + X_{m+1}) and on the V face b (Y_n + Y_{n+1}).
+    Interior cells average the squared slopes of their two faces per
+    direction, coastal cells use the one open face: a metric other than
+    dxt / dyt, a face taken from the wrong side or lcu and lcv exchanged
+    fail.  The basin has coasts with one and with both faces open."""
+    b = _flat(22, 18, h=0.0)
+    nyh, nxh = b.ny, b.nx
+    mm = np.arange(nxh, dtype=np.float64)[None, :] * np.ones((nyh, 1))
+    nn = np.arange(nyh, dtype=np.float64)[:, None] * np.ones((1, nxh))
+    for k, name in enumerate(("dx", "dy", "dxt", "dyt", "dxh", "dyh", "dxb", "dyb")):
+        setattr(b, name, 1000.0 * (1.0 + 0.05 * k + 0.2 * np.sin(0.7 * mm + k) * (name[1] == "x")
+                                   + 0.2 * np.cos(0.9 * nn + k) * (name[1] == "y")))
+    # dxt depends on m only, dyt on n only: distances X(m), Y(n) along the axes
+    X = np.concatenate([[0.0], np.cumsum(b.dxt[0, :-1])])
+    Y = np.concatenate([[0.0], np.cumsum(b.dyt[:-1, 0])])
+    a, c = 2e-9, -3e-9
+    wet = np.zeros((nyh, nxh), dtype=bool)
+    wet[2:-2, 2:-2] = True
+    wet[7:10, 9:12] = False        # an island: coasts with one open face
+    wet[13, 5] = False              # a single land cell
+    b.lbasins = np.where(wet, 0, 1).astype(np.int32)
+    b.hhq_rest = np.where(wet, 50.0 + a * X[None, :] ** 2 + c * Y[:, None] ** 2, 0.0)
+    r = oracle.OracleGrid(b).rayleigh(1e-4)
+    lcu = wet[:, :-1] & wet[:, 1:]   # U face between m and m+1 (0-based m)
+    lcv = wet[:-1, :] & wet[1:, :]
+    checked = {1: 0, 2: 0}
+    for j in range(2, nyh - 2):
+        for i in range(2, nxh - 2):
+            if not wet[j, i]:
+                continue
+            sx = ([(a * (X[i] + X[i + 1])) ** 2] * int(lcu[j, i]) +
+                  [(a * (X[i - 1] + X[i])) ** 2] * int(lcu[j, i - 1]))
+            sy = ([(c * (Y[j] + Y[j + 1])) ** 2] * int(lcv[j, i]) +
+                  [(c * (Y[j - 1] + Y[j])) ** 2] * int(lcv[j - 1, i]))
+            if not sx or not sy:
+                continue
+            want = 1e-4 * math.sqrt(sum(sx) / len(sx) + sum(sy) / len(sy))
+            assert math.isclose(r[j, i], want, rel_tol=1e-10), (j, i, r[j, i], want)
+            checked[min(len(sx), len(sy))] += 1
+    assert checked[1] > 10 and checked[2] > 100, checked

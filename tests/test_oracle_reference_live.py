@@ -150,3 +150,17 @@ def test_density_bitwise(seed, metrics):
     assert_bitwise(got, want, ("den", "den_pot"), f"density seed {seed} {metrics}")
     assert np.any(want["den"] == 7.0) and np.any(want["den"] != 7.0)
     assert np.any(want["den"] != want["den_pot"])
+
+
+def test_reference_grid_bitwise():
+    """inmsom_bt.cases.reference_grid against the reference's own basinpar
+    (Service/basin_parameters.f90, harness mode 8) for the shipped
+    configuration: every metric and rlh_s bit for bit."""
+    from inmsom_bt.cases import REF_NX, REF_NY, reference_grid
+
+    b = make_basin(REF_NX - 4, REF_NY - 4, seed=3, sigma=4.0)
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "case.bin")
+        write_case(path, b, np.zeros((b.ny, b.nx)), 1, 1, 0, 8)
+        ref = oracle.run_reference(path, path + ".out", b.nx, b.ny, 8)
+    assert_bitwise(reference_grid(), ref, tuple(ref), "basinpar")
