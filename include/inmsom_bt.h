@@ -122,6 +122,24 @@ int inmsom_bt_barotropic_dynamics_dev(inmsom_bt_ctx *ctx, double tau, int nstep,
                                       const double *wflux, const double *mu, const double *mu4,
                                       const double *rdis);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  As inmsom_bt_barotropic_dynamics_dev with ksw4 = 0 (so
+ * no mu4), through the fused kernel only, with its scratch (the substeps'
+ * two state sets and carried depths, 14 planes of nx*ny doubles) in a fixed
+ * 384 MiB device array the library holds on every GPU it runs on until the
+ * process ends: grids above 3,595,117 cells (nx*ny) fail with
+ * INMSOM_BT_EINVAL, calls the fused kernel does not cover with
+ * INMSOM_BT_ESTATE.  Calls no hipMalloc or hipFree; the work is queued on
+ * the context's stream and the entry returns without waiting for it
+ * (inmsom_bt_synchronize waits).  All calls on one GPU share the scratch:
+ * calls that overlap on one GPU race. */
+int inmsom_bt_barotropic_dynamics_noalloc(inmsom_bt_ctx *ctx, double tau, int nstep, double *ubrtr_e,
+                                          double *ubrtrp_e, double *vbrtr_e, double *vbrtrp_e, double *ssh_e,
+                                          double *sshp_e, double *ubrtr_i, double *vbrtr_i, double *ssh4gradx,
+                                          double *ssh4grady, const double *RHSx, const double *RHSy,
+                                          const double *wflux, const double *mu, const double *rdis);
+
 /* Path of the barotropic_dynamics entries: 0 = fused where the call allows
  * it (default), 1 = one kernel per reference sweep, 2 = fused (calls it
  * does not cover fail with INMSOM_BT_ESTATE).  Results are the same bit

@@ -298,6 +298,10 @@ int inmsom_bt_init(const inmsom_bt_grid *gr, int device, inmsom_bt_ctx **out)
         // depths of a state at rest until set_state is called
         lit_hh_init(c, c->st[4], c->st[5]);
         HIPCHECK(hipStreamSynchronize(c->stream));
+        // what the fused barotropic_dynamics needs of the static grid, and the
+        // address of the allocation-free entry's scratch on this device
+        TRY(dropin_setup(c));
+        TRY(noalloc_scratch_setup(device));
         return INMSOM_BT_OK;
     }();
     if (rc != INMSOM_BT_OK) {
@@ -480,6 +484,41 @@ int inmsom_bt_barotropic_dynamics_dev(inmsom_bt_ctx *c, double tau, int nstep, i
     }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(c->stream));
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_barotropic_dynamics_noalloc(inmsom_bt_ctx *c, double tau, int nstep, double *ubrtr_e,
+                                          double *ubrtrp_e, double *vbrtr_e, double *vbrtrp_e, double *ssh_e,
+                                          double *sshp_e, double *ubrtr_i, double *vbrtr_i, double *ssh4gradx,
+                                          double *ssh4grady, const double *RHSx, const double *RHSy,
+                                          const double *wflux, const double *mu, const double *rdis)
+{
+    REQUIRE(c, "null context");
+    REQUIRE(nstep >= 1 && nstep <= 1000, "nstep must be in 1..1000");
+    REQUIRE(tau > 0.0, "tau must be positive");
+    REQUIRE(ubrtr_e && ubrtrp_e && vbrtr_e && vbrtrp_e && ssh_e && sshp_e && ubrtr_i && vbrtr_i &&
+                ssh4gradx && ssh4grady && RHSx && RHSy && wflux && mu && rdis,
+            "null array pointer");
+    size_t have = 0;
+    double *scr = noalloc_scratch(c->device, &have);
+    if (!scr) {
+        set_error("the scratch of the allocation-free entry is not set up on this device");
+        return INMSOM_BT_ESTATE;
+    }
+    if (dropin_scratch_doubles(c) > have) {
+        set_error("grid too large for the allocation-free entry: it needs 14 planes of nx*ny doubles "
+                  "within its fixed scratch of " + std::to_string(have * 8 >> 20) + " MiB");
+        return INMSOM_BT_EINVAL;
+    }
+    if (c->dropin_path == 1 || !dropin_eligible(c, 0, nullptr)) {
+        set_error("the allocation-free entry runs the fused barotropic_dynamics only, which this context's grid "
+                  "or path setting does not allow");
+        return INMSOM_BT_ESTATE;
+    }
+    HIPCHECK(hipSetDevice(c->device));
+    double *st[6] = {ubrtr_e, ubrtrp_e, vbrtr_e, vbrtrp_e, ssh_e, sshp_e};
+    double *acc[4] = {ubrtr_i, vbrtr_i, ssh4gradx, ssh4grady};
+    TRY(dropin_call(c, tau, nstep, st, acc, RHSx, RHSy, wflux, mu, rdis, scr));
     return INMSOM_BT_OK;
 }
 

@@ -14,8 +14,21 @@ bool dropin_eligible(inmsom_bt_ctx *c, int ksw4, std::string *why);
 // arrays: the state (u, up, v, vp, ssh, sshp) is replaced by the state
 // after substep nstep, acc (ubrtr_i, vbrtr_i, ssh4gradx, ssh4grady) receives
 // the averages.  Queued on the context's stream.
+// scratch: 14 planes of the substeps' buffers (dropin_scratch_doubles), or
+// null for library-owned buffers allocated at the first such call.
 int dropin_call(inmsom_bt_ctx *c, double tau, int nstep, double *const st[6], double *const acc[4],
-                const double *rhsx, const double *rhsy, const double *wflux, const double *mu, const double *rdis);
+                const double *rhsx, const double *rhsy, const double *wflux, const double *mu, const double *rdis,
+                double *scratch = nullptr);
+// The grid analysis (eligibility, mask bits, row tables, launch shape),
+// done once per context at inmsom_bt_init.
+int dropin_setup(inmsom_bt_ctx *c);
+size_t dropin_scratch_doubles(const inmsom_bt_ctx *c);
+// The fixed scratch array of the entry point that allocates nothing: its
+// address on the current device, looked up once per device at set-up
+// (inmsom_bt_init), and (noalloc_scratch) that address and the array's
+// size in doubles (null before set-up).
+int noalloc_scratch_setup(int device);
+double *noalloc_scratch(int device, size_t *doubles);
 void dropin_release(inmsom_bt_ctx *c);
 // The launch the fused drop-in uses: rows per wave and whether metrics come
 // from row tables (after the first call; 0 / false before).

@@ -33,6 +33,7 @@
 // grids): then they are read from row tables instead of 8 + 1 arrays.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "bt_dropin.h"
@@ -526,6 +527,9 @@ struct DropinAlloc {
 
 namespace {
 
+// doubles per plane of the substeps' buffers, rounded up to 256 bytes
+size_t scratch_plane(size_t n) { return (n + 31) / 32 * 32; }
+
 #define DHC(expr)                                                                              \
     do {                                                                                       \
         hipError_t e_ = (expr);                                                                \
@@ -608,10 +612,6 @@ int check_grid(inmsom_bt_ctx *c, DropinAlloc *a)
     hipLaunchKernelGGL(k_dropin_masks, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n, c->mask[0],
                        c->mask[1], c->mask[2], c->mask[3], c->mask[4], c->mask[5], c->mask[6], a->mb);
     DHC(hipGetLastError());
-    for (int b = 0; b < 2; ++b)
-        for (int k = 0; k < 6; ++k) DHC(hipMalloc(&a->S[b][k], n * 8));
-    DHC(hipMalloc(&a->hupc, n * 8));
-    DHC(hipMalloc(&a->hvpc, n * 8));
     // launch shape: rows per wave filling whole rounds of the device's wave slots
     int dev = 0, cus = 0, per_cu = 0;
     DHC(hipGetDevice(&dev));
@@ -626,13 +626,26 @@ int check_grid(inmsom_bt_ctx *c, DropinAlloc *a)
     return INMSOM_BT_OK;
 }
 
+// the substeps' two state sets and carried depths (14 planes of n doubles)
+// in library-owned memory, allocated at the first call that needs them
+int ensure_buffers(inmsom_bt_ctx *c, DropinAlloc *a)
+{
+    const size_t n = c->n;
+    for (int b = 0; b < 2; ++b)
+        for (int k = 0; k < 6; ++k)
+            if (!a->S[b][k]) DHC(hipMalloc(&a->S[b][k], n * 8));
+    if (!a->hvpc) DHC(hipMalloc(&a->hvpc, n * 8));
+    if (!a->hupc) DHC(hipMalloc(&a->hupc, n * 8));
+    return INMSOM_BT_OK;
+}
+
 template <bool FIRST>
 void launch_dropin(inmsom_bt_ctx *c, const DropinParams &p, bool rowmet)
 {
     dim3 grid((unsigned)(p.per_xcd * NXCD)), blk(WAVE);
     const void *k = rowmet ? (const void *)k_dropin<FIRST, true> : (const void *)k_dropin<FIRST, false>;
     void *args[] = {const_cast<DropinParams *>(&p)};
-    hipLaunchKernel(k, grid, blk, args, 0, c->stream);
+    (void)hipLaunchKernel(k, grid, blk, args, 0, c->stream);
 }
 
 }  // namespace
@@ -661,8 +674,13 @@ bool dropin_eligible(inmsom_bt_ctx *c, int ksw4, std::string *why)
     return true;
 }
 
+int dropin_setup(inmsom_bt_ctx *c) { return check_grid(c, dalloc(c)); }
+
+size_t dropin_scratch_doubles(const inmsom_bt_ctx *c) { return 14 * scratch_plane(c->n); }
+
 int dropin_call(inmsom_bt_ctx *c, double tau, int nstep, double *const st[6], double *const acc[4],
-                const double *rhsx, const double *rhsy, const double *wflux, const double *mu, const double *rdis)
+                const double *rhsx, const double *rhsy, const double *wflux, const double *mu, const double *rdis,
+                double *scratch)
 {
     DropinAlloc *a = dalloc(c);
     int rc = check_grid(c, a);
@@ -670,6 +688,21 @@ int dropin_call(inmsom_bt_ctx *c, double tau, int nstep, double *const st[6], do
     if (!a->eligible) {
         set_error("the fused barotropic_dynamics does not cover this grid: " + a->why);
         return INMSOM_BT_ESTATE;
+    }
+    // the substeps' state sets and carried depths: 14 planes
+    double *S[2][6], *hupc, *hvpc;
+    if (scratch) {
+        const size_t plane = scratch_plane(c->n);
+        for (int b = 0; b < 2; ++b)
+            for (int k = 0; k < 6; ++k) S[b][k] = scratch + (size_t)(6 * b + k) * plane;
+        hupc = scratch + 12 * plane;
+        hvpc = scratch + 13 * plane;
+    } else {
+        if ((rc = ensure_buffers(c, a))) return rc;
+        for (int b = 0; b < 2; ++b)
+            for (int k = 0; k < 6; ++k) S[b][k] = a->S[b][k];
+        hupc = a->hupc;
+        hvpc = a->hvpc;
     }
     DropinParams p{};
     p.nx = c->nx;
@@ -688,8 +721,8 @@ int dropin_call(inmsom_bt_ctx *c, double tau, int nstep, double *const st[6], do
     p.ry = rhsy;
     for (int k = 0; k < 9; ++k)
         p.met[k] = a->rowmet_ok ? a->rowmet + (size_t)k * c->ny : (k < 8 ? c->metric[k] : c->rlh_s);
-    p.hupc = a->hupc;
-    p.hvpc = a->hvpc;
+    p.hupc = hupc;
+    p.hvpc = hvpc;
     for (int k = 0; k < 4; ++k) p.acc[k] = acc[k];
     p.g = c->g.g;
     p.rho0 = c->g.rho0;
@@ -702,14 +735,14 @@ int dropin_call(inmsom_bt_ctx *c, double tau, int nstep, double *const st[6], do
     for (int k = 1; k <= last; ++k) {
         // substep k reads the caller's state (k = 1) or the buffer substep
         // k - 1 wrote, and writes S[k % 2] (nothing after the last substep)
-        const double *const *in = k == 1 ? (const double *const *)st : (const double *const *)a->S[(k - 1) % 2];
+        const double *const *in = k == 1 ? (const double *const *)st : (const double *const *)S[(k - 1) % 2];
         p.u = in[0];
         p.up = in[1];
         p.v = in[2];
         p.vp = in[3];
         p.s = in[4];
         p.sp = in[5];
-        double *const *out = a->S[k % 2];
+        double *const *out = S[k % 2];
         const bool wr = k < last;
         p.uo = wr ? out[0] : nullptr;
         p.upo = wr ? out[1] : nullptr;
@@ -736,15 +769,56 @@ int dropin_call(inmsom_bt_ctx *c, double tau, int nstep, double *const st[6], do
     return INMSOM_BT_OK;
 }
 
+// Scratch of the entry point that allocates nothing
+// (inmsom_bt_barotropic_dynamics_noalloc): the substeps' two state sets and
+// carried depths, 14 planes of nx*ny doubles, in a fixed array every GPU the
+// library runs on holds until the process ends.  Calls on one GPU share it:
+// calls that overlap on one GPU race.  Nothing in it needs to start at
+// zero (every plane is written before it is read within a call).
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)384 << 20;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[MAX_DEVICES] = {};
+}  // namespace
+
+int noalloc_scratch_setup(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES) {
+        set_error("device index beyond the scratch table");
+        return INMSOM_BT_EINVAL;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return INMSOM_BT_OK;
+    void *ptr = nullptr;  // the array's address on the current device (each GPU has its own copy)
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        return INMSOM_BT_EDEVICE;
+    }
+    g_scratch[device] = static_cast<double *>(ptr);
+    return INMSOM_BT_OK;
+}
+
+double *noalloc_scratch(int device, size_t *doubles)
+{
+    *doubles = NOALLOC_SCRATCH_BYTES / sizeof(double);
+    if (device < 0 || device >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch[device];
+}
+
 void dropin_release(inmsom_bt_ctx *c)
 {
     DropinAlloc *a = c->da;
     if (!a) return;
     for (auto &b : a->S)
         for (double *q : b)
-            if (q) hipFree(q);
+            if (q) (void)hipFree(q);
     for (void *q : {(void *)a->hupc, (void *)a->hvpc, (void *)a->rowmet, (void *)a->mb})
-        if (q) hipFree(q);
+        if (q) (void)hipFree(q);
     delete a;
     c->da = nullptr;
 }

@@ -48,6 +48,7 @@ SIGNATURES = {
                                       C.c_int),
     "inmsom_bt_barotropic_dynamics_dev": ([C.c_void_p, C.c_double, C.c_int, C.c_int] + [_dp] * 16,
                                           C.c_int),
+    "inmsom_bt_barotropic_dynamics_noalloc": ([C.c_void_p, C.c_double, C.c_int] + [_dp] * 15, C.c_int),
     "inmsom_bt_set_dropin_path": ([C.c_void_p, C.c_int], C.c_int),
     "inmsom_bt_get_dropin_path": ([C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
     "inmsom_bt_dropin_timed": ([C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int, _dp], C.c_int),

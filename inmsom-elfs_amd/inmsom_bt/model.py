@@ -183,6 +183,19 @@ class BarotropicModel:
                                                       *[self._dev(a, s2) for a in arrs]),
               "inmsom_bt_barotropic_dynamics_dev")
 
+    def barotropic_dynamics_noalloc(self, tau, nstep, ubrtr_e, ubrtrp_e, vbrtr_e, vbrtrp_e, ssh_e, sshp_e,
+                                    ubrtr_i, vbrtr_i, ssh4gradx, ssh4grady, RHSx, RHSy, wflux, mu, rdis):
+        """As barotropic_dynamics_dev with ksw4 = 0 through the entry that
+        allocates no device memory (its scratch is a fixed array of the
+        library); queued on the model's stream: call synchronize() before
+        reading the results."""
+        arrs = [ubrtr_e, ubrtrp_e, vbrtr_e, vbrtrp_e, ssh_e, sshp_e, ubrtr_i, vbrtr_i, ssh4gradx, ssh4grady,
+                RHSx, RHSy, wflux, mu, rdis]
+        s2 = (self.ny, self.nx)
+        check(lib().inmsom_bt_barotropic_dynamics_noalloc(self._h, float(tau), int(nstep),
+                                                          *[self._dev(a, s2) for a in arrs]),
+              "inmsom_bt_barotropic_dynamics_noalloc")
+
     DROPIN_PATHS = {"auto": 0, "literal": 1, "fused": 2}
 
     def set_dropin_path(self, path: str = "auto"):
