@@ -781,6 +781,49 @@ def snapshot_leg(m, b, count):
 CLOSURE_PARAMS = os.path.join(ROOT, "inmsom-elfs_amd", "inmsom_bt", "data", "closure_cfg1.npz")
 
 
+# Multiply-adds per pixel of one forward pass of the closure network, as
+# the layers are defined (3x3 taps): 4 -> 32, 32 -> 32, 32 -> 32, 32 -> 2.
+CNN_MACS_PER_PIXEL = 9 * (4 * 32 + 32 * 32 + 32 * 32 + 32 * 2)
+PEAK_BF16_TFLOPS = 2500.0
+
+
+def cnn_train_leg(steps=20, batch=32, ny=388, nx=516):
+    """Training of the closure network on the device (BF16 MFMA forward,
+    data and weight gradients, FP32 Adam) on batches of the learned
+    closure's shape (32 coarse 512 x 384 basins with halo), synthetic
+    inputs.  Algorithmic work of a step: forward + data gradient + weight
+    gradient = 3 x the forward's 2 x CNN_MACS_PER_PIXEL flops per pixel
+    (the first layer's data gradient, never formed, counted anyway)."""
+    from inmsom_bt.closure import ClosureNet
+    from inmsom_bt.closure_experiment import load_params
+
+    params, nm, meta = load_params(CLOSURE_PARAMS)
+    rng = np.random.default_rng(3)
+    nsamp = batch + 4
+    with ClosureNet(batch, ny, nx) as net:
+        net.set_output(meta.get("output", "uv"))
+        net.set_params(params)
+        net.dataset_alloc(nsamp)
+        for i in range(nsamp):
+            x = np.zeros((ny, nx, 8), dtype=np.float32)
+            x[..., :4] = rng.standard_normal((ny, nx, 4))
+            net.dataset_set(i, x, 1e-2 * rng.standard_normal((ny, nx, 2)), np.full((ny, nx), 3, dtype=np.uint8))
+        idx = [list(rng.choice(nsamp, size=batch, replace=False)) for _ in range(steps + 2)]
+        for k in range(2):
+            net.train_step(idx[k])
+        t0 = time.perf_counter()
+        for k in range(steps):
+            net.train_step(idx[2 + k])
+        dt = (time.perf_counter() - t0) / steps
+    flops = 3 * 2 * CNN_MACS_PER_PIXEL * batch * ny * nx
+    tf = flops / dt / 1e12
+    log(f"cnn training: {dt * 1e3:.2f} ms/step (batch {batch} x {ny} x {nx}), {tf:.0f} TFLOP/s")
+    return {"ms_per_step": dt * 1e3, "batch": batch, "ny": ny, "nx": nx, "steps": steps,
+            "value": batch / dt, "unit": "training samples/s (512x384 coarse basins with halo)",
+            "roofline": {"bound": "mfma", "achieved": tf, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                         "frac": tf / PEAK_BF16_TFLOPS, "flops_per_step": flops}}
+
+
 def learned_leg(fine, rollout_steps, members, member_steps, ratio=3.0, rollout_ratio=1e300):
     """cfg3/cfg4: the trained closure (parameters trained on the device by
     inmsom_bt.closure_experiment, committed) on the 4x-coarsened basin:
@@ -888,6 +931,11 @@ def main():
     ap.add_argument("--learned-steps", type=int, default=2000,
                     help="coarse 40 s steps of the learned-closure rollout (0 = skip)")
     ap.add_argument("--closure-members", type=int, default=128)
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="ensembles launch every step directly instead of replaying HIP graphs (for profilers: "
+                         "rocprofv3's kernel trace of the closure ensembles' graph replays ends the process)")
+    ap.add_argument("--cnn-train-steps", type=int, default=20,
+                    help="timed training steps of the closure network (batch 32; 0 = skip)")
     ap.add_argument("--closure-member-steps", type=int, default=1000)
     ap.add_argument("--coupling-levels", type=int, default=COUPLING_NZ,
                     help="levels of the 2-D -> 3-D coupling leg (0: skip it)")
@@ -904,6 +952,9 @@ def main():
                     help="write the per-launch HBM traffic of the fused kernels from two rocprofv3 "
                          "--pmc outputs (counter_collection.csv or results.db; FETCH_SIZE, WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.no_graphs:
+        from inmsom_bt.ensemble import Ensemble
+        Ensemble.USE_GRAPHS = False
     if args.traffic_from:
         print(json.dumps(write_traffic(*args.traffic_from), indent=1))
         return
@@ -1022,6 +1073,8 @@ def main():
                                              args.ensemble_members)
     if args.learned_steps > 0 and args.workload == "cfg1":
         line["learned"] = learned_leg(b, args.learned_steps, args.closure_members, args.closure_member_steps)
+    if args.cnn_train_steps > 0 and os.path.exists(CLOSURE_PARAMS):
+        line["cnn_training"] = cnn_train_leg(args.cnn_train_steps)
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         line["cpu_baseline"] = cpu_baseline(b, args.cpu_steps)

@@ -108,13 +108,20 @@ class Ensemble:
               "inmsom_bt_ens_get_substeps_per_launch")
         return k.value
 
-    def run(self, nsteps: int, nstep: int = 2, tau: Optional[float] = None, graph: bool = True):
+    # Graph replays by default; False makes run()/run_timed() without an
+    # explicit graph argument launch directly (same results bit for bit),
+    # for tools that do not follow graph launches.
+    USE_GRAPHS = True
+
+    def run(self, nsteps: int, nstep: int = 2, tau: Optional[float] = None, graph: Optional[bool] = None):
         tau = self.basins[0].dt if tau is None else tau
+        graph = self.USE_GRAPHS if graph is None else graph
         check(lib().inmsom_bt_ens_run(self._h, float(tau), nstep, nsteps, int(graph)), "inmsom_bt_ens_run")
 
     def run_timed(self, nsteps: int, nstep: int = 2, tau: Optional[float] = None,
-                  graph: bool = True) -> float:
+                  graph: Optional[bool] = None) -> float:
         tau = self.basins[0].dt if tau is None else tau
+        graph = self.USE_GRAPHS if graph is None else graph
         ms = C.c_double()
         check(lib().inmsom_bt_ens_run_timed(self._h, float(tau), nstep, nsteps, int(graph), C.byref(ms)),
               "inmsom_bt_ens_run_timed")
