@@ -213,6 +213,16 @@ int inmsom_bt_get_launch(inmsom_bt_ctx *ctx, int *rows_per_wave);
  * from 2^20 cells on, 1 in FP64_EXACT and on smaller grids).  Results do
  * not depend on it. */
 int inmsom_bt_set_substeps_per_launch(inmsom_bt_ctx *ctx, int substeps);
+/* inmsom_bt_run in FP64 and FP32: stages = K in 2..4 runs K consecutive
+ * substeps per launch (across outer steps) as a pipeline of K waves per
+ * workgroup, one substep per wave, the rows handed over in LDS; 1 = off
+ * (inmsom_bt_set_substeps_per_launch applies); 0 = automatic (default:
+ * 2 stages from 2^20 cells on while the substeps per launch are automatic
+ * too).  Results agree with one launch per substep to rounding (the order
+ * of contracted multiply-adds may differ). */
+int inmsom_bt_set_pipeline(inmsom_bt_ctx *ctx, int stages);
+/* the pipeline stages the next inmsom_bt_run in this mode uses (0: none) */
+int inmsom_bt_get_pipeline(inmsom_bt_ctx *ctx, int mode, int *stages);
 /* the substeps per launch the fused step uses in `mode` (1 or 2) */
 int inmsom_bt_get_substeps_per_launch(inmsom_bt_ctx *ctx, int mode, int *substeps);
 

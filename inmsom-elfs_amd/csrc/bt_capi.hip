@@ -679,6 +679,21 @@ int inmsom_bt_run_timed(inmsom_bt_ctx *c, double tau, int nstep, int nsteps, int
     return INMSOM_BT_OK;
 }
 
+int inmsom_bt_set_pipeline(inmsom_bt_ctx *c, int stages)
+{
+    REQUIRE(c, "null context");
+    TRY(fused_set_pipeline(c, stages));
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_get_pipeline(inmsom_bt_ctx *c, int mode, int *stages)
+{
+    REQUIRE(c && stages, "null pointer");
+    REQUIRE(mode == INMSOM_BT_FP64_EXACT || mode == INMSOM_BT_FP64 || mode == INMSOM_BT_FP32, "unknown mode");
+    *stages = pipe_stages(c, mode);
+    return INMSOM_BT_OK;
+}
+
 int inmsom_bt_set_launch(inmsom_bt_ctx *c, int rows_per_wave)
 {
     REQUIRE(c, "null context");

@@ -381,6 +381,8 @@ FusedParams member_params(const inmsom_bt_ens *e, const EnsMember &mb, int phase
     p.d = mb.d;
     p.frx = mb.frx;
     p.fry = mb.fry;
+    p.frx_in = mb.frx;
+    p.fry_in = mb.fry;
     p.hup_c = mb.hup_c;
     p.hvp_c = mb.hvp_c;
     return p;

@@ -21,6 +21,10 @@ int fused_volume(inmsom_bt_ctx *c, double *vol);
 int fused_coarsen(inmsom_bt_ctx *c, int factor, double *u, double *v, double *ssh);
 int fused_set_shape(inmsom_bt_ctx *c, int chunk);
 int fused_set_substeps(inmsom_bt_ctx *c, int k);
+// K substeps per launch as a K-wave pipeline (FP64 and FP32; 0 =
+// automatic, 1 = off) and the stages the next launches use (0: none)
+int fused_set_pipeline(inmsom_bt_ctx *c, int k);
+int pipe_stages(const inmsom_bt_ctx *c, int mode);
 int subs_per_launch(const inmsom_bt_ctx *c, int mode);
 
 }  // namespace ibt

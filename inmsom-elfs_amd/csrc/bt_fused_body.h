@@ -28,7 +28,8 @@ struct FusedParams {
     // sub-cycling: substep `sub` of `nsub` (tau = tau_inner = outer tau / nsub)
     int sub, nsub;
     double dn;             // dfloat(nstep)
-    void *frx, *fry;       // forcing frozen over the substeps of one call (stencil type)
+    void *frx, *fry;       // forcing frozen over the substeps of one call (stencil type), written
+    const void *frx_in, *fry_in;  // the same, read (a separate buffer when a launch forms new forcing)
     double *hup_c, *hvp_c; // filtered U/V depths carried between substeps (FP64_EXACT only)
 };
 
@@ -542,7 +543,7 @@ struct Loader {
         const R *S = static_cast<const R *>(p.s), *Hh = static_cast<const R *>(p.H);
         const R *U = static_cast<const R *>(p.u), *V = static_cast<const R *>(p.v);
         const R *RD = static_cast<const R *>(p.rd);
-        const R *FX = static_cast<const R *>(p.frx), *FY = static_cast<const R *>(p.fry);
+        const R *FX = static_cast<const R *>(p.frx_in), *FY = static_cast<const R *>(p.fry_in);
         const size_t i2 = ra.at(n + 2), i1 = ra.at(n + 1), i0 = ra.at(n);
         q.s = S[i2];
         q.H = Hh[i2];
@@ -744,6 +745,145 @@ __device__ __forceinline__ void fused_chunk2(const FusedParams &p, int chunk_id,
         up_prev = okn ? oa.up : 0.0;
         vp_prev = okn ? oa.vp : 0.0;
         sp_prev = okn ? oa.sp : 0.0;
+    }
+}
+
+// ------------------------------------------------------------------
+// K substeps per launch as a pipeline of K waves (one workgroup of 64 K
+// threads per work item): wave w runs substep sub + w on row n - 3 w at
+// iteration n, on what wave w - 1 produced in the three iterations before
+// (its rows n - 3 w + 2, + 1 and + 0), handed over through a ring of
+// PIPE_SLOTS rows in LDS with one workgroup barrier per iteration.  Each
+// wave holds one substep's window (registers of one stage), the
+// intermediate levels never leave the chip, and wave 0 alone reads the
+// arrays; the last wave writes them.  The results are those of K launches
+// of fused_chunk bit for bit (the same Stage and Loader code).
+
+constexpr int PIPE_SLOTS = 4;
+// output columns per strip with K stages: stage w's results are valid on
+// lanes w + 1 .. 61 - 2 w (a stage's result at lane j needs its inputs at
+// j - 1 .. j + 2); the last stage writes lanes K .. 63 - 2 K
+__host__ __device__ constexpr int pipe_width(int K) { return WAVE - 3 * K; }
+// iterations of a work item beyond its rows: the first stage starts
+// 2 K + 1 rows above the chunk, the last ends 3 (K - 1) rows later
+__host__ __device__ constexpr int pipe_extra(int K) { return 5 * K - 2; }
+
+// What stage w hands to stage w + 1 for one row (each lane its column).
+template <typename R>
+struct PipeSlot {
+    R s[WAVE], u[WAVE], v[WAVE], fx[WAVE], fy[WAVE], H[WAVE], rd[WAVE];
+    double sp[WAVE], up[WAVE], vp[WAVE];
+    uint8_t mb[WAVE];
+};
+
+template <typename R, int K, bool LOAD, bool FORM>
+__device__ __forceinline__ void fused_pipe_stage(const FusedParams &p, int chunk_id, int strip, int w,
+                                                 bool save_forcing, PipeSlot<R> (*ring)[PIPE_SLOTS])
+{
+    constexpr int SWK = pipe_width(K);
+    const int lane = threadIdx.x % WAVE;
+    const int m = strip * SWK + lane - (K - 1);
+    const bool colok = (m >= 1 && m <= p.nx);
+    const bool writer = colok && lane >= K && lane <= WAVE - 1 - 2 * K;
+    const int r0 = chunk_id * p.chunk + 1;
+    const int r1 = min(r0 + p.chunk, p.ny + 1);
+    const int n0 = r0 - 2 * K - 1, n1 = r1 - 1 + 3 * (K - 1);  // iterations n0..n1
+    const int valid_from = n0 + 3 + 2 * w;                      // first row of valid results
+    const bool last = (w == K - 1);
+
+    Stage<R, false, FORM> st(p, m);
+    Loader<R, false, !FORM> ld(p, m);
+    StageIn<R> in;
+    StageOut<R> o;
+    // the row's own static fields for the next stage (inputs arrive two rows,
+    // and the Rayleigh coefficient one row, ahead)
+    R H_d1 = 0, H_d2 = 0, rd_d1 = 0;
+    int mb_d1 = 0, mb_d2 = 0;
+    if constexpr (LOAD) ld.issue(n0);
+    for (int n = n0; n <= n1; ++n) {
+        const int r = n - 3 * w;
+        if constexpr (LOAD) {
+            ld.take(r, in);
+            if (r + 1 <= n1) ld.issue(r + 1);
+        } else {
+            const PipeSlot<R> &a = ring[w - 1][(n - 1) & (PIPE_SLOTS - 1)];  // row r + 2
+            const PipeSlot<R> &b = ring[w - 1][(n - 2) & (PIPE_SLOTS - 1)];  // row r + 1
+            const PipeSlot<R> &c = ring[w - 1][(n - 3) & (PIPE_SLOTS - 1)];  // row r
+            in.s2 = a.s[lane];
+            in.H2 = a.H[lane];
+            in.mb2 = a.mb[lane];
+            in.u1 = b.u[lane];
+            in.v1 = b.v[lane];
+            in.up1 = b.up[lane];
+            in.vp1 = b.vp[lane];
+            in.sp1 = b.sp[lane];
+            in.rd1 = b.rd[lane];
+            in.fx0 = c.fx[lane];
+            in.fy0 = c.fy[lane];
+            in.hupc0 = in.hvpc0 = 0.0;
+            in.f0 = (r >= 1 && r <= p.ny) ? R(ld.tab(p.row_f, r)) : R(0);
+            in.sx0 = R(ld.tab(p.row_sx, r));
+            in.sy0 = R(ld.tab(p.row_sy, r));
+            in.wf1 = (r + 1 >= 1 && r + 1 <= p.ny) ? R(ld.tab(p.row_wf, r + 1)) : R(0);
+        }
+        const bool keep = last ? (r >= r0) : (r >= valid_from);
+        st.row(r, in, keep, o);
+        if (!last) {
+            // row r for stage w + 1, zero outside the array or before the
+            // results are valid (what its loads would have read there)
+            PipeSlot<R> &q = ring[w][n & (PIPE_SLOTS - 1)];
+            const bool ok = ld.rowok(r) && keep;
+            q.s[lane] = ok ? o.s : R(0);
+            q.u[lane] = ok ? o.u : R(0);
+            q.v[lane] = ok ? o.v : R(0);
+            q.sp[lane] = ok ? o.sp : 0.0;
+            q.up[lane] = ok ? o.up : 0.0;
+            q.vp[lane] = ok ? o.vp : 0.0;
+            q.fx[lane] = keep ? o.rhs_u : R(0);
+            q.fy[lane] = keep ? o.rhs_v : R(0);
+            q.H[lane] = H_d2;
+            q.rd[lane] = rd_d1;
+            q.mb[lane] = (uint8_t)mb_d2;
+        } else if (keep && writer && r <= p.ny) {
+            const size_t io = (size_t)(r - 1) * p.nx + (m - 1);
+            // the forcing of a call that continues after this launch
+            if (save_forcing) {
+                st_out<false, R>(p.frx, io, o.rhs_u);
+                st_out<false, R>(p.fry, io, o.rhs_v);
+            }
+            store_state<R>(p, io, o);
+        }
+        H_d2 = H_d1;
+        H_d1 = in.H2;
+        mb_d2 = mb_d1;
+        mb_d1 = in.mb2;
+        rd_d1 = in.rd1;
+        __syncthreads();
+    }
+}
+
+// One work item of the K-stage pipeline; every wave of the workgroup
+// calls it.  form_mask bit w: stage w is the first substep of its call
+// (forms the wind and drag forcing); save_forcing: the last stage's call
+// continues after this launch.
+template <typename R, int K>
+__device__ __forceinline__ void fused_pipe(const FusedParams &p, int chunk_id, int strip, int form_mask,
+                                           bool save_forcing, PipeSlot<R> (*ring)[PIPE_SLOTS])
+{
+    // the stage index as a scalar: each wave takes one branch, the same
+    // number of barriers in every branch
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    const bool form = (form_mask >> w) & 1;
+    if (w == 0) {
+        if (form)
+            fused_pipe_stage<R, K, true, true>(p, chunk_id, strip, w, save_forcing, ring);
+        else
+            fused_pipe_stage<R, K, true, false>(p, chunk_id, strip, w, save_forcing, ring);
+    } else {
+        if (form)
+            fused_pipe_stage<R, K, false, true>(p, chunk_id, strip, w, save_forcing, ring);
+        else
+            fused_pipe_stage<R, K, false, false>(p, chunk_id, strip, w, save_forcing, ring);
     }
 }
 

@@ -100,6 +100,8 @@ struct inmsom_bt_ctx {
     int fz_cur = 0;
     int chunk = 0;                 // output rows per wave of the fused step (0 = per-mode default)
     int subs_per_launch = 0;       // fused substeps per launch (0 = by mode, 1, 2 = two in one march)
+    int pipe_k = 0;                // K substeps per launch as a K-wave pipeline (0 = off)
+    long long pipe_slots[2][5] = {};  // workgroups of the pipeline kernel the device holds (FP64 / FP32, K)
     long long slots[6] = {};       // waves the device holds at once, per fused kernel (0 = not queried)
     double *scal = nullptr;        // device scalars (reductions)
     double *cpl = nullptr;         // staging of the 2-D -> 3-D coupling (bt_coupling.hip)

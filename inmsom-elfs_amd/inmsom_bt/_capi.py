@@ -61,6 +61,8 @@ SIGNATURES = {
     "inmsom_bt_run": ([C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int], C.c_int),
     "inmsom_bt_run_timed": ([C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int, _dp], C.c_int),
     "inmsom_bt_set_launch": ([C.c_void_p, C.c_int], C.c_int),
+    "inmsom_bt_set_pipeline": ([C.c_void_p, C.c_int], C.c_int),
+    "inmsom_bt_get_pipeline": ([C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
     "inmsom_bt_get_launch": ([C.c_void_p, C.POINTER(C.c_int)], C.c_int),
     "inmsom_bt_set_substeps_per_launch": ([C.c_void_p, C.c_int], C.c_int),
     "inmsom_bt_get_substeps_per_launch": ([C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),

@@ -242,6 +242,18 @@ class BarotropicModel:
         substep, 0 = by mode (default); results do not depend on it."""
         check(lib().inmsom_bt_set_substeps_per_launch(self._h, int(k)), "inmsom_bt_set_substeps_per_launch")
 
+    def set_pipeline(self, stages: int):
+        """FP64 / FP32 fused step: stages = K in 2..4 runs K substeps per
+        launch as a K-wave pipeline, 1 = off, 0 = automatic; results agree
+        to rounding."""
+        check(lib().inmsom_bt_set_pipeline(self._h, int(stages)), "inmsom_bt_set_pipeline")
+
+    def pipeline(self, mode: str) -> int:
+        """Pipeline stages the next run in this mode uses (0: none)."""
+        k = C.c_int()
+        check(lib().inmsom_bt_get_pipeline(self._h, _capi.MODES[mode], C.byref(k)), "inmsom_bt_get_pipeline")
+        return k.value
+
     def substeps_per_launch(self, mode: str) -> int:
         k = C.c_int()
         check(lib().inmsom_bt_get_substeps_per_launch(self._h, _capi.MODES[mode], C.byref(k)),

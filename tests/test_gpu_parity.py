@@ -103,6 +103,35 @@ def test_two_substeps_per_launch_bitwise(mode, shape, nstep, chunk):
     assert_bitwise(out[2], out[1], STATE, f"two vs one substep per launch, {mode} {shape} nstep={nstep}")
 
 
+@pytest.mark.parametrize("mode", ["fp64", "fp32"])
+@pytest.mark.parametrize("stages", [2, 3, 4])
+@pytest.mark.parametrize("shape,nstep,steps,chunk", [((150, 110), 2, 9, 0), ((150, 110), 3, 7, 5),
+                                                     ((61, 7), 2, 5, 0), ((300, 64), 1, 6, 3),
+                                                     ((116, 91), 4, 3, 7)])
+def test_pipeline_bitwise(mode, stages, shape, nstep, steps, chunk):
+    """K substeps per launch as a K-wave pipeline (across outer steps, with
+    one- and two-substep launches for the remainder) give what one launch
+    per substep gives, over strip and chunk edges, odd nstep and a ragged
+    last chunk: to rounding (FP64 relative L2 <= 1e-14, FP32 <= 1e-6; the
+    modes contract multiply-adds as the compiler finds them, which the
+    pipeline's code may do differently)."""
+    b = randomize_state(make_basin(*shape, seed=34, sigma=5.0), 10)
+    print(f"pipeline {stages} {mode} {shape} nstep={nstep} steps={steps} chunk={chunk}", flush=True)
+    out = {}
+    for k in (1, stages):
+        with BarotropicModel(b) as m:
+            m.set_launch(chunk)
+            m.set_substeps_per_launch(1)
+            m.set_pipeline(k)
+            m.run(steps, mode, nstep=nstep)
+            out[k] = m.get_state()
+    tol = 1e-14 if mode == "fp64" else 1e-6
+    for k in STATE:
+        e = rel_l2(out[stages][k], out[1][k])
+        print(f"  {k}: rel L2 {e:.2e}")
+        assert e <= tol, (k, e)
+
+
 def test_fused_fp64_within_1e12():
     """Reassociated FP64 step: relative L2 <= 1e-12 after 200 outer steps."""
     b = make_basin(200, 150, seed=41, sigma=8.0)
