@@ -266,6 +266,12 @@ int inmsom_bt_ens_set_launch(inmsom_bt_ens *ens, int rows_per_wave, int waves_pe
 /* substeps per launch, as inmsom_bt_set_substeps_per_launch (0 = by mode) */
 int inmsom_bt_ens_set_substeps_per_launch(inmsom_bt_ens *ens, int substeps);
 int inmsom_bt_ens_get_substeps_per_launch(inmsom_bt_ens *ens, int *substeps);
+/* substep pairs (FP64 and FP32) as a two-wave pipeline, as
+ * inmsom_bt_set_pipeline: 2 = on, 1 = off, 0 = by mode (default: on);
+ * waves_per_xcd of inmsom_bt_ens_set_launch then counts two-wave
+ * workgroups.  get: the stages the next run uses (0: none) */
+int inmsom_bt_ens_set_pipeline(inmsom_bt_ens *ens, int stages);
+int inmsom_bt_ens_get_pipeline(inmsom_bt_ens *ens, int *stages);
 /* nsteps outer steps of tau, nstep substeps each; use_graph = 1 replays a
  * captured graph per outer step, 0 launches directly */
 int inmsom_bt_ens_run(inmsom_bt_ens *ens, double tau, int nstep, int nsteps, int use_graph);

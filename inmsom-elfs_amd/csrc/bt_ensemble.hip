@@ -63,6 +63,36 @@ __global__ void __launch_bounds__(WAVE)
     }
 }
 
+// Substeps sub and sub + 1 of every item as a two-wave pipeline
+// (fused_pipe, 58-column strips as the pair march): the two waves of a
+// workgroup walk the work list together, one substep each.
+static_assert(SW2 == pipe_width(2), "the pipeline and the pair march share the strip width");
+template <typename R>
+__global__ void __launch_bounds__(2 * WAVE)
+    k_fused_batch_pipe(const FusedParams *__restrict__ table, const int *__restrict__ item_member,
+                       const int *__restrict__ item_local, int total, int per_xcd, int groups_per_xcd,
+                       double tau_inner, int sub, int nsub)
+{
+    __shared__ PipeSlot<R> ring[1][PIPE_SLOTS];
+    const int w = blockIdx.x;
+    const int xcd = w % NXCD, j = w / NXCD;
+    const int t0 = xcd * per_xcd, t1 = min(t0 + per_xcd, total);
+    // the first pair of a call forms the forcing; the later pairs read it,
+    // so the first pair saves it when the call goes on
+    const int form = sub == 1 ? 1 : 0;
+    const bool save = sub == 1 && sub + 1 < nsub;
+    for (int t = t0 + j; t < t1; t += groups_per_xcd) {
+        FusedParams p = table[item_member[t]];
+        p.tau = tau_inner;
+        p.sub = sub;
+        p.nsub = nsub;
+        p.dn = (double)nsub;
+        const int local = item_local[t];
+        const int chunk_id = local / p.strips, strip = local - chunk_id * p.strips;
+        fused_pipe<R, 2>(p, chunk_id, strip, form, save, ring);
+    }
+}
+
 template <class R>
 __global__ void k_member_range(int nx, int ny, const uint8_t *mb, const R *ssh, const R *u, const R *v,
                                unsigned long long *bad)
@@ -130,6 +160,7 @@ struct inmsom_bt_ens {
     int chunk = 0;          // rows per wave, 0 = per-mode default
     int waves_per_xcd = 0;  // persistent grid, 0 = default
     int subs = 0;           // substeps per launch: 0 = by mode, 1, 2
+    int pipe = 0;           // substep pairs as a two-wave pipeline: 0 = by mode, 1 = off, 2 = on
     int built_chunk = -1;
     bool built_two = false;
     unsigned table_gen = 0;  // bumped whenever build_tables replaces table[] / item_*
@@ -139,7 +170,7 @@ struct inmsom_bt_ens {
     hipGraphExec_t gexec[2][2] = {};
     double g_tau = -1.0;
     int g_nstep = -1, g_chunk = -1, g_wpx = -1;
-    bool g_two = false;
+    bool g_two = false, g_pipe = false;
     // what the graphs' kernel nodes point at or hold by value: the tables'
     // generation, and the closure network and its version (0 / null: none)
     unsigned g_table_gen = 0, g_net_version = 0;
@@ -184,12 +215,21 @@ bool two_per_launch(const inmsom_bt_ens *e)
     return e->mode != INMSOM_BT_FP64_EXACT && cells >= TWO_SUBSTEP_CELLS;
 }
 
-// Waves of the batch kernel one XCD holds at once: its CUs x the kernel's
-// waves per CU from the occupancy calculator.
+// substep pairs as a two-wave pipeline (FP64 and FP32): as set, or by default
+bool pipe_pairs(const inmsom_bt_ens *e)
+{
+    if (e->mode == INMSOM_BT_FP64_EXACT || !two_per_launch(e)) return false;
+    return e->pipe != 1;
+}
+
 // the batch kernel for (mode, two substeps, forming substep)
 template <bool FORM>
-const void *batch_kernel(int mode, bool two)
+const void *batch_kernel(int mode, bool two, bool pipe = false)
 {
+    if (two && pipe) {
+        if (mode == INMSOM_BT_FP64) return (const void *)&k_fused_batch_pipe<double>;
+        return (const void *)&k_fused_batch_pipe<float>;
+    }
     if (two) {
         if (mode == INMSOM_BT_FP64_EXACT) return (const void *)&k_fused_batch2<double, true, FORM>;
         if (mode == INMSOM_BT_FP64) return (const void *)&k_fused_batch2<double, false, FORM>;
@@ -200,13 +240,17 @@ const void *batch_kernel(int mode, bool two)
     return (const void *)&k_fused_batch<float, false, FORM>;
 }
 
+// Workgroups of the batch kernel one XCD holds at once (one wave each, two
+// for the pipeline): its CUs x the kernel's workgroups per CU from the
+// occupancy calculator.
 int default_waves(const inmsom_bt_ens *e, bool two)
 {
     int cus = 0, per_cu = 0;
+    const bool pipe = two && pipe_pairs(e);
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device) != hipSuccess || cus < NXCD)
         cus = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<true>(e->mode, two), WAVE, 0) !=
-            hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<true>(e->mode, two, pipe),
+                                                     pipe ? 2 * WAVE : WAVE, 0) != hipSuccess ||
         per_cu < 1)
         per_cu = 4;
     return cus / NXCD * per_cu;
@@ -216,7 +260,7 @@ int default_waves(const inmsom_bt_ens *e, bool two)
 // rows) over all members' items for the persistent grid's wave slots.
 int default_chunk(const inmsom_bt_ens *e, bool two, int wpx)
 {
-    const int sw = two ? SW2 : SW, warm = two ? 7 : 3;
+    const int sw = two ? SW2 : SW, warm = two ? (pipe_pairs(e) ? pipe_extra(2) : 7) : 3;
     const long long slots = (long long)wpx * NXCD;
     int best = 16;
     long long best_cost = -1;
@@ -424,12 +468,13 @@ int build_tables(inmsom_bt_ens *e, int chunk, bool two)
 void launch_batch(inmsom_bt_ens *e, int phase, double tau_inner, int sub, int nsub, int wpx, bool two)
 {
     int per_xcd = (e->total + NXCD - 1) / NXCD;
-    dim3 grid((unsigned)(wpx * NXCD)), blk(WAVE);
+    const bool pipe = two && pipe_pairs(e);
+    dim3 grid((unsigned)(wpx * NXCD)), blk(pipe ? 2 * WAVE : WAVE);
     const FusedParams *tb = e->table[phase];
     const int *im = e->item_member, *il = e->item_local;
     int total = e->total;
     void *args[] = {&tb, &im, &il, &total, &per_xcd, &wpx, &tau_inner, &sub, &nsub};
-    const void *k = sub == 1 ? batch_kernel<true>(e->mode, two) : batch_kernel<false>(e->mode, two);
+    const void *k = sub == 1 ? batch_kernel<true>(e->mode, two, pipe) : batch_kernel<false>(e->mode, two, pipe);
     hipLaunchKernel(k, grid, blk, args, 0, e->stream);
 }
 
@@ -485,7 +530,7 @@ bool graphs_current(const inmsom_bt_ens *e, double tau, int nstep, int chunk, in
 {
     const unsigned nv = e->closure ? ibt::cnn::net_version(e->closure) : 0u;
     return e->gexec[0][0] && e->g_tau == tau && e->g_nstep == nstep && e->g_chunk == chunk &&
-           e->g_wpx == wpx && e->g_two == two && e->g_table_gen == e->table_gen && e->g_net == e->closure &&
+           e->g_wpx == wpx && e->g_two == two && e->g_pipe == pipe_pairs(e) && e->g_table_gen == e->table_gen && e->g_net == e->closure &&
            e->g_net_version == nv;
 }
 
@@ -528,6 +573,7 @@ int ensure_graphs(inmsom_bt_ens *e, double tau, int nstep, int chunk, int wpx, b
     e->g_chunk = chunk;
     e->g_wpx = wpx;
     e->g_two = two;
+    e->g_pipe = pipe_pairs(e);
     e->g_table_gen = e->table_gen;
     e->g_net = e->closure;
     e->g_net_version = e->closure ? ibt::cnn::net_version(e->closure) : 0u;
@@ -728,6 +774,22 @@ int inmsom_bt_ens_set_substeps_per_launch(inmsom_bt_ens *e, int substeps)
     if (!e) return fail("null ensemble", INMSOM_BT_EINVAL);
     if (substeps < 0 || substeps > 2) return fail("substeps per launch must be 0 (by mode), 1 or 2", INMSOM_BT_EINVAL);
     e->subs = substeps;
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_ens_set_pipeline(inmsom_bt_ens *e, int stages)
+{
+    if (!e) return fail("null ensemble", INMSOM_BT_EINVAL);
+    if (stages < 0 || stages > 2)
+        return fail("ensemble pipeline stages must be 0 (by mode), 1 (off) or 2", INMSOM_BT_EINVAL);
+    e->pipe = stages;
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_ens_get_pipeline(inmsom_bt_ens *e, int *stages)
+{
+    if (!e || !stages) return fail("null pointer", INMSOM_BT_EINVAL);
+    *stages = pipe_pairs(e) ? 2 : 0;
     return INMSOM_BT_OK;
 }
 

@@ -110,6 +110,8 @@ SIGNATURES = {
     "inmsom_bt_ens_get_state": ([C.c_void_p, C.c_int] + [_dp] * 6, C.c_int),
     "inmsom_bt_ens_set_launch": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
     "inmsom_bt_ens_set_substeps_per_launch": ([C.c_void_p, C.c_int], C.c_int),
+    "inmsom_bt_ens_set_pipeline": ([C.c_void_p, C.c_int], C.c_int),
+    "inmsom_bt_ens_get_pipeline": ([C.c_void_p, C.POINTER(C.c_int)], C.c_int),
     "inmsom_bt_ens_get_substeps_per_launch": ([C.c_void_p, C.POINTER(C.c_int)], C.c_int),
     "inmsom_bt_ens_run": ([C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int], C.c_int),
     "inmsom_bt_ens_run_timed": ([C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int, _dp], C.c_int),

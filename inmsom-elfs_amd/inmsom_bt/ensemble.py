@@ -102,6 +102,16 @@ class Ensemble:
         check(lib().inmsom_bt_ens_set_substeps_per_launch(self._h, int(k)),
               "inmsom_bt_ens_set_substeps_per_launch")
 
+    def set_pipeline(self, stages: int):
+        """2 = substep pairs as a two-wave pipeline, 1 = off, 0 = by mode
+        (default); results agree to rounding."""
+        check(lib().inmsom_bt_ens_set_pipeline(self._h, int(stages)), "inmsom_bt_ens_set_pipeline")
+
+    def pipeline(self) -> int:
+        k = C.c_int()
+        check(lib().inmsom_bt_ens_get_pipeline(self._h, C.byref(k)), "inmsom_bt_ens_get_pipeline")
+        return k.value
+
     def substeps_per_launch(self) -> int:
         k = C.c_int()
         check(lib().inmsom_bt_ens_get_substeps_per_launch(self._h, C.byref(k)),
@@ -168,7 +178,7 @@ class Ensemble:
     # ------------------------------------------------------------ autotuner
     def signature(self) -> str:
         shapes = ",".join(f"{b.nx}x{b.ny}" for b in self.basins)
-        return hashlib.sha1(f"{self.mode}|{shapes}".encode()).hexdigest()[:16]
+        return hashlib.sha1(f"{self.mode}|pipeline {self.pipeline()}|{shapes}".encode()).hexdigest()[:16]
 
     def autotune(self, nstep: int = 2, trial_steps: int = 20,
                  rows: Sequence[int] = DEFAULT_ROWS, waves: Sequence[int] = DEFAULT_WAVES,

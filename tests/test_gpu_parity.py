@@ -108,7 +108,7 @@ def test_two_substeps_per_launch_bitwise(mode, shape, nstep, chunk):
 @pytest.mark.parametrize("shape,nstep,steps,chunk", [((150, 110), 2, 9, 0), ((150, 110), 3, 7, 5),
                                                      ((61, 7), 2, 5, 0), ((300, 64), 1, 6, 3),
                                                      ((116, 91), 4, 3, 7)])
-def test_pipeline_bitwise(mode, stages, shape, nstep, steps, chunk):
+def test_pipeline_matches_one_launch_per_substep(mode, stages, shape, nstep, steps, chunk):
     """K substeps per launch as a K-wave pipeline (across outer steps, with
     one- and two-substep launches for the remainder) give what one launch
     per substep gives, over strip and chunk edges, odd nstep and a ragged
