@@ -68,10 +68,10 @@ def rel_l2(a, b):
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "fused_step_cfg1_traffic.json")
 # the kernels each leg launches (two substeps per launch in FP64 and FP32; the
 # last template argument marks the call's first, forcing-forming substep)
-TRAFFIC_KERNELS = {"fp64": ("k_fused_step2<double, false, true>",),
-                   "fp32": ("k_fused_step2<float, false, true>",),
+TRAFFIC_KERNELS = {"fp64": ("k_fused_pipe<double, 2>",),
+                   "fp32": ("k_fused_pipe<float, 2>",),
                    "fp64_exact": ("k_fused_step<double, true, true>", "k_fused_step<double, true, false>"),
-                   "ensemble_fp32": ("k_fused_batch2<float, false, true>",),
+                   "ensemble_fp32": ("k_fused_batch_pipe<float>",),
                    "forcing": ("k_forcing<",),
                    "dropin": ("k_dropin<true, true>", "k_dropin<false, true>"),
                    "vertical_velocity": ("k_vertical(",),

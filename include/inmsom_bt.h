@@ -223,7 +223,8 @@ int inmsom_bt_set_substeps_per_launch(inmsom_bt_ctx *ctx, int substeps);
 int inmsom_bt_set_pipeline(inmsom_bt_ctx *ctx, int stages);
 /* the pipeline stages the next inmsom_bt_run in this mode uses (0: none) */
 int inmsom_bt_get_pipeline(inmsom_bt_ctx *ctx, int mode, int *stages);
-/* the substeps per launch the fused step uses in `mode` (1 or 2) */
+/* the substeps per launch the fused step uses in `mode` (1 or 2, or the
+ * pipeline's stages when it is on) */
 int inmsom_bt_get_substeps_per_launch(inmsom_bt_ctx *ctx, int mode, int *substeps);
 
 /* Range check of Control/dyncall.f90:452-516 on the device state:

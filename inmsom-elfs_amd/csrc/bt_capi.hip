@@ -712,7 +712,8 @@ int inmsom_bt_get_substeps_per_launch(inmsom_bt_ctx *c, int mode, int *substeps)
 {
     REQUIRE(c && substeps, "null pointer");
     REQUIRE(mode == INMSOM_BT_FP64_EXACT || mode == INMSOM_BT_FP64 || mode == INMSOM_BT_FP32, "unknown mode");
-    *substeps = subs_per_launch(c, mode);
+    const int k = pipe_stages(c, mode);
+    *substeps = k > 0 ? k : subs_per_launch(c, mode);
     return INMSOM_BT_OK;
 }
 

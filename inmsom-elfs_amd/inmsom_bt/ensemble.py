@@ -23,6 +23,8 @@ from .model import STATE, _d, _grid_struct, rayleigh
 
 DEFAULT_ROWS = (0, 12, 16, 24, 32, 40, 48, 64)
 DEFAULT_WAVES = (0, 256, 384, 512)
+# two-wave workgroups per XCD when substep pairs run as a pipeline
+DEFAULT_GROUPS = (0, 96, 128, 160, 192)
 
 
 class Ensemble:
@@ -181,12 +183,15 @@ class Ensemble:
         return hashlib.sha1(f"{self.mode}|pipeline {self.pipeline()}|{shapes}".encode()).hexdigest()[:16]
 
     def autotune(self, nstep: int = 2, trial_steps: int = 20,
-                 rows: Sequence[int] = DEFAULT_ROWS, waves: Sequence[int] = DEFAULT_WAVES,
+                 rows: Sequence[int] = DEFAULT_ROWS, waves: Optional[Sequence[int]] = None,
                  cache: Optional[str] = None) -> dict:
         """Times every (rows per wave, waves per XCD) candidate for
         trial_steps outer steps from the same saved state (restored after
         each trial) and keeps the fastest.  Results never depend on the
-        shape; only the time does."""
+        shape; only the time does.  waves defaults to DEFAULT_WAVES, or to
+        DEFAULT_GROUPS (two-wave workgroups) under the pipeline."""
+        if waves is None:
+            waves = DEFAULT_GROUPS if self.pipeline() == 2 else DEFAULT_WAVES
         key = self.signature()
         if cache and os.path.exists(cache):
             with open(cache) as f:
