@@ -83,7 +83,7 @@ TRAFFIC_KERNELS = {"fp64": ("k_fused_pipe<double, 2>",),
 COUPLING_SEQUENCE = {"k_barotropic_part(": 2, "k_internal_uv(": 1, "k_ssh_internal(ibt::DevGrid, double": 1,
                      "k_ssh_internal_shift(": 1, "k_hh_update(": 1}
 # kernels of one baroclinic_dynamics call (nz = 20)
-BAROCLINIC_SEQUENCE = {"k_bc_solve<20>": 1, "k_bc_update(": 1}
+BAROCLINIC_SEQUENCE = {"k_bc_solve<20, true>": 1, "k_bc_frame(": 1}
 SEQUENCES = {"coupling": COUPLING_SEQUENCE, "baroclinic": BAROCLINIC_SEQUENCE}
 
 

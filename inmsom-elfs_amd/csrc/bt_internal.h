@@ -117,7 +117,8 @@ struct inmsom_bt_ctx {
     double *bc = nullptr;          // staging of baroclinic_dynamics (bt_baroclinic.hip)
     size_t bc_n = 0;               // doubles in bc
     int bc_nz = 0;                 // levels of the staged fields (0 = none)
-    double *bc_dev = nullptr;      // un, vn and the vertical grid of the device-address call
+    int bc_cur = 0;                // staged level n of uu, vv: 0 = slots UU, VV; 1 = UN, VN
+    double *bc_dev = nullptr;      // new uu, vv and the vertical grid of the device-address call
     size_t bc_dev_n = 0;           // doubles in bc_dev
     double *vv = nullptr;          // staging of vertical_velocity (bt_vertical.hip)
     size_t vv_n = 0;               // doubles in vv
