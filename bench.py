@@ -246,6 +246,34 @@ def cpu_baseline(b, steps):
                       f"look-ahead) and all {subs} are counted"}
 
 
+def cpu_reference_fortran(steps):
+    """The reference's own Fortran (oracle/_ref/ref_harness, built from its
+    sources; the timing mode uses its start_timer / end_timer) and the C
+    restatement on the same basin at the reference's compiled 289 x 163,
+    one rank x the host's threads (at most 16): the calibration of the
+    port-based cpu_baseline.  None where the harness is not built."""
+    if steps <= 0:
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import time_reference
+
+    if not oracle.have_reference():
+        log("cpu reference: oracle/_ref/ref_harness not built, skipped")
+        return None
+    threads = min(os.cpu_count() or 1, 16)
+    r = time_reference.measure(steps, [threads])
+    t = r["threads"][str(threads)]
+    log(f"cpu reference: Fortran {t['fortran_rate']:.3e}, C {t['c_rate']:.3e} wet-cell steps/s on {threads} "
+        f"threads ({r['basin']})")
+    return {"value": t["fortran_rate"], "unit": r["unit"], "cores": threads, "kind": "reference",
+            "ranks": 1, "c_port_value": t["c_rate"], "c_over_fortran": t["c_over_fortran"],
+            "sample": f"{steps} outer steps of the {r['basin']} (the reference's compiled grid) through "
+                      f"oracle/_ref/ref_harness mode 9, timed with the reference's start_timer/end_timer; "
+                      f"{r['substeps_per_outer_step']} leap-frog substeps counted per outer step; the C "
+                      f"restatement on the same steps and threads beside it"}
+
+
 def ensemble_leg(steps, warmup, nmembers):
     """cfg2: uneven basins advanced together in FP32 (one persistent,
     graph-replayed launch per substep), launch shape from the autotuner."""
@@ -946,6 +974,8 @@ def main():
     ap.add_argument("--baroclinic-levels", type=int, default=COUPLING_NZ,
                     help="levels of the baroclinic_dynamics leg (0: skip it)")
     ap.add_argument("--baroclinic-reps", type=int, default=20)
+    ap.add_argument("--reference-steps", type=int, default=200,
+                    help="outer steps of the reference Fortran timing at 289x163 (0: skip)")
     ap.add_argument("--snapshots", type=int, default=3,
                     help="snapshots of the state written in the reference's output formats (0: skip)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH", "WRITE"),
@@ -1080,6 +1110,9 @@ def main():
         line["cpu_baseline"] = cpu_baseline(b, args.cpu_steps)
         log(f"cpu baseline: {line['cpu_baseline']['value']:.3e} wet-cell steps/s on "
             f"{line['cpu_baseline']['cores']} threads")
+        ref = cpu_reference_fortran(args.reference_steps)
+        if ref is not None:
+            line["cpu_reference_fortran"] = ref
     print(json.dumps(line), flush=True)
 
 

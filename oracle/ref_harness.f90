@@ -58,11 +58,16 @@
 !   own density function (Service/oc_algorithms.f90:138-161) on synthetic
 !   tt, ss, a non-uniform sigma grid and the depths of hh_init: the inputs
 !   (z, tt, ss, the initial den, den_pot, hhq) and den, den_pot are written.
+! mode 9: the time loop of mode 0 without output, timed by the reference's
+!   own start_timer / end_timer (mpi_wtime, Modules/mod_mpi_parallel_tools.f90)
+!   around the nsteps steps: after the masks it writes int32 nsteps, int32
+!   OpenMP threads, real8 seconds.
 ! mode 8: the reference's grid set-up basinpar (Service/basin_parameters.f90:
 !   191-300 with grid_parameters_spherical / _carthesian, :764-949) for the
 !   compiled Inc/basinpar.fi, on zeroed metric arrays: dx, dy, dxt, dyt, dxh,
 !   dyh, dxb, dyb and rlh_s are written (the case's fields are not used).
 program ref_harness
+use omp_lib
 use main_basin_pars
 use mpi_parallel_tools
 use basin_grid
@@ -73,7 +78,7 @@ character(512) :: case_path, out_path, mask_path
 integer :: ierr, magic, version, cnx, cny, nsteps, nstep, ksw4, mode, nout
 integer :: step, k, m, n, io
 integer, allocatable :: out_steps(:), lb(:,:)
-real(8) :: tau, cb_nl_in, ebottom_in
+real(8) :: tau, cb_nl_in, ebottom_in, tsec
 real(8), allocatable :: wflux(:,:), mu(:,:), mu4(:,:), rdis(:,:)
 real(8), allocatable :: ssh4gx(:,:), ssh4gy(:,:), vort(:,:), strt(:,:), strs(:,:)
 real(8), allocatable :: fxa(:,:), fya(:,:), radv(:,:), rady(:,:), rdfx(:,:), rdfy(:,:)
@@ -127,13 +132,14 @@ call gridcon(trim(mask_path))
 open(23, file=trim(out_path), access='stream', form='unformatted', status='replace')
 call write_masks()
 
-if (mode == 0) then
+if (mode == 0 .or. mode == 9) then
   ksw_ssbc = 1
   ksw_lat4 = ksw4
   call hh_init(hhq_e, hhqp_e, hhqn_e, hhu_e, hhup_e, hhun_e, hhv_e, hhvp_e, hhvn_e, &
                hhh_e, hhhp_e, hhhn_e, ssh_e, sshp_e, hhq_rest)
   r_diss = rdis
   k = 1
+  if (mode == 9) call start_timer(tsec)
   do step = 1, nsteps
     ! the 2-D flow is its own bottom layer; depths are the external-mode ones
     uu(:,:,nz)  = ubrtr_e
@@ -171,6 +177,10 @@ if (mode == 0) then
       endif
     endif
   enddo
+  if (mode == 9) then
+    call end_timer(tsec)
+    write(23) nsteps, omp_get_max_threads(), tsec
+  endif
 else if (mode == 2) then
   ksw_ssbc = 1
   ksw_lat4 = ksw4

@@ -164,3 +164,20 @@ def test_reference_grid_bitwise():
         write_case(path, b, np.zeros((b.ny, b.nx)), 1, 1, 0, 8)
         ref = oracle.run_reference(path, path + ".out", b.nx, b.ny, 8)
     assert_bitwise(reference_grid(), ref, tuple(ref), "basinpar")
+
+
+def test_reference_timing_mode():
+    """Mode 9 of the harness (the reference's time loop under its own
+    start_timer / end_timer) reports the steps and threads it ran and a
+    positive time, and the recipe that times it beside the C restatement
+    runs end to end on a few steps."""
+    import time_reference
+
+    b = time_reference.basin()
+    rdis = oracle.OracleGrid(b).rayleigh()
+    sec = time_reference.fortran_seconds(b, rdis, 3, 2)
+    assert 0.0 < sec < 60.0
+    r = time_reference.measure(2, [1])
+    t = r["threads"]["1"]
+    assert t["fortran_rate"] > 0 and t["c_rate"] > 0
+    assert r["substeps_per_outer_step"] == 2 * time_reference.NSTEP
