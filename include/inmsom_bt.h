@@ -571,6 +571,13 @@ int inmsom_bt_cnn_forward(inmsom_bt_cnn *net, float *out);
 int inmsom_bt_cnn_get_activation(inmsom_bt_cnn *net, int layer, float *out);
 /* loss and gradient of the batch without an update */
 int inmsom_bt_cnn_loss_grad(inmsom_bt_cnn *net, double *loss);
+/* keep dL/dz of every layer of the next backward passes (on = 1: four
+ * batch-sized BF16 buffers; 0 frees them), and read layer `layer`'s
+ * (0..3; z_3 = the outputs, z_0..2 before the ReLU) as float, [B, H, W,
+ * 32] channels-last.  No reference counterpart: the network's per-layer
+ * parity tests read them */
+int inmsom_bt_cnn_keep_backprop(inmsom_bt_cnn *net, int on);
+int inmsom_bt_cnn_get_backprop(inmsom_bt_cnn *net, int layer, float *out);
 /* device-resident training set of nsamples images and one Adam step on the
  * samples idx[0..batch-1]; *loss is the batch loss before the step */
 int inmsom_bt_cnn_dataset_alloc(inmsom_bt_cnn *net, int nsamples);

@@ -666,6 +666,9 @@ struct inmsom_bt_cnn {
     int adam_t = 0;
     // training batch buffers
     bf16 *x = nullptr, *act[3] = {}, *dy = nullptr, *dx = nullptr;
+    // inmsom_bt_cnn_keep_backprop: dL/dz of every layer in its own buffer
+    // (otherwise the backward pass ping-pongs dy / dx)
+    bf16 *gz[NL] = {};
     float *pred = nullptr, *tgt = nullptr, *loss_part = nullptr;
     float *part_w = nullptr, *part_b = nullptr;  // weight-gradient chunk records, group records
     uint8_t *mask = nullptr;
@@ -824,16 +827,18 @@ int backward(inmsom_bt_cnn *c, long long count, double *loss)
     const long long np = (long long)c->B * c->H * c->W;
     const float inv = count > 0 ? 1.0f / (float)count : 0.0f;
     const int nb = (int)((np + 255) / 256);
+    // dL/dz_l of layer l (z_3: the outputs; z_0..2: before the ReLU)
+    bf16 *g3 = c->gz[3] ? c->gz[3] : c->dy, *g2 = c->gz[2] ? c->gz[2] : c->dx;
+    bf16 *g1 = c->gz[1] ? c->gz[1] : c->dy, *g0 = c->gz[0] ? c->gz[0] : c->dx;
     hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, c->stream, np, c->H, c->W, c->norm.kind, c->pred, c->tgt,
-                       c->mask, inv, c->dy,
-                       c->loss_part);
-    wgrad_launch(c, 3, c->dy, c->act[2]);
-    conv_launch(c, c->stream, 3, true, c->imgs, c->tile_img, c->tile_local, c->ntiles, c->dy, c->act[2], c->dx);
-    wgrad_launch(c, 2, c->dx, c->act[1]);
-    conv_launch(c, c->stream, 2, true, c->imgs, c->tile_img, c->tile_local, c->ntiles, c->dx, c->act[1], c->dy);
-    wgrad_launch(c, 1, c->dy, c->act[0]);
-    conv_launch(c, c->stream, 1, true, c->imgs, c->tile_img, c->tile_local, c->ntiles, c->dy, c->act[0], c->dx);
-    wgrad_launch(c, 0, c->dx, c->x);
+                       c->mask, inv, g3, c->loss_part);
+    wgrad_launch(c, 3, g3, c->act[2]);
+    conv_launch(c, c->stream, 3, true, c->imgs, c->tile_img, c->tile_local, c->ntiles, g3, c->act[2], g2);
+    wgrad_launch(c, 2, g2, c->act[1]);
+    conv_launch(c, c->stream, 2, true, c->imgs, c->tile_img, c->tile_local, c->ntiles, g2, c->act[1], g1);
+    wgrad_launch(c, 1, g1, c->act[0]);
+    conv_launch(c, c->stream, 1, true, c->imgs, c->tile_img, c->tile_local, c->ntiles, g1, c->act[0], g0);
+    wgrad_launch(c, 0, g0, c->x);
     CHC(hipGetLastError());
     if (loss) {
         std::vector<float> part(nb);
@@ -1026,7 +1031,7 @@ int inmsom_bt_cnn_destroy(inmsom_bt_cnn *c)
                     (void *)c->dy, (void *)c->dx, (void *)c->pred, (void *)c->tgt, (void *)c->mask,
                     (void *)c->loss_part, (void *)c->part_w, (void *)c->part_b, (void *)c->imgs,
                     (void *)c->tile_img, (void *)c->tile_local, (void *)c->ds_x, (void *)c->ds_t,
-                    (void *)c->ds_m})
+                    (void *)c->ds_m, (void *)c->gz[0], (void *)c->gz[1], (void *)c->gz[2], (void *)c->gz[3]})
         if (p) hipFree(p);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -1135,6 +1140,38 @@ int inmsom_bt_cnn_get_activation(inmsom_bt_cnn *c, int layer, float *out)
     const size_t n = (size_t)c->B * c->H * c->W * CH;
     std::vector<bf16> a(n);
     CHC(hipMemcpy(a.data(), c->act[layer], n * 2, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) out[i] = (float)a[i];
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_cnn_keep_backprop(inmsom_bt_cnn *c, int on)
+{
+    if (!c) return cfail("null pointer", INMSOM_BT_EINVAL);
+    CHC(hipSetDevice(c->device));
+    CHC(hipStreamSynchronize(c->stream));
+    const size_t n = (size_t)c->B * c->H * c->W * CH;
+    for (int l = 0; l < NL; ++l) {
+        if (on && !c->gz[l]) {
+            CTRY(dmalloc(&c->gz[l], n));
+            CHC(hipMemset(c->gz[l], 0, n * sizeof(bf16)));
+        } else if (!on && c->gz[l]) {
+            CHC(hipFree(c->gz[l]));
+            c->gz[l] = nullptr;
+        }
+    }
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_cnn_get_backprop(inmsom_bt_cnn *c, int layer, float *out)
+{
+    if (!c || !out || layer < 0 || layer >= NL) return cfail("bad arguments", INMSOM_BT_EINVAL);
+    if (!c->gz[layer]) return cfail("back-propagated values are kept only after inmsom_bt_cnn_keep_backprop",
+                                    INMSOM_BT_ESTATE);
+    CHC(hipSetDevice(c->device));
+    CHC(hipStreamSynchronize(c->stream));
+    const size_t n = (size_t)c->B * c->H * c->W * CH;
+    std::vector<bf16> a(n);
+    CHC(hipMemcpy(a.data(), c->gz[layer], n * 2, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < n; ++i) out[i] = (float)a[i];
     return INMSOM_BT_OK;
 }

@@ -131,6 +131,8 @@ SIGNATURES = {
     "inmsom_bt_cnn_forward": ([C.c_void_p, _fp], C.c_int),
     "inmsom_bt_cnn_get_activation": ([C.c_void_p, C.c_int, _fp], C.c_int),
     "inmsom_bt_cnn_loss_grad": ([C.c_void_p, _dp], C.c_int),
+    "inmsom_bt_cnn_keep_backprop": ([C.c_void_p, C.c_int], C.c_int),
+    "inmsom_bt_cnn_get_backprop": ([C.c_void_p, C.c_int, C.POINTER(C.c_float)], C.c_int),
     "inmsom_bt_cnn_dataset_alloc": ([C.c_void_p, C.c_int], C.c_int),
     "inmsom_bt_cnn_dataset_set": ([C.c_void_p, C.c_int, _fp, _fp, _u8p], C.c_int),
     "inmsom_bt_cnn_train_step": ([C.c_void_p, _ip, C.c_float, _dp], C.c_int),

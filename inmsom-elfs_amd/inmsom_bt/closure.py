@@ -56,7 +56,7 @@ class ClosureNet:
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
-            # refused (and the network kept) while an ensemble holds it as its closure
+            # an ensemble holding it as its closure drops it first (its next run fails)
             check(lib().inmsom_bt_cnn_destroy(self._h), "inmsom_bt_cnn_destroy")
             self._h = None
 
@@ -128,6 +128,16 @@ class ClosureNet:
     def activation(self, layer: int) -> np.ndarray:
         out = np.zeros((self.batch, self.ny, self.nx, CH), dtype=np.float32)
         check(lib().inmsom_bt_cnn_get_activation(self._h, layer, _f(out)), "inmsom_bt_cnn_get_activation")
+        return out
+
+    def keep_backprop(self, on: bool = True):
+        """Keep dL/dz of every layer of the next backward passes."""
+        check(lib().inmsom_bt_cnn_keep_backprop(self._h, int(on)), "inmsom_bt_cnn_keep_backprop")
+
+    def backprop(self, layer: int) -> np.ndarray:
+        """dL/dz of layer 0..3 of the last backward pass, [B, H, W, 32]."""
+        out = np.empty((self.batch, self.ny, self.nx, 32), dtype=np.float32)
+        check(lib().inmsom_bt_cnn_get_backprop(self._h, int(layer), _f(out)), "inmsom_bt_cnn_get_backprop")
         return out
 
     def loss_grad(self) -> float:

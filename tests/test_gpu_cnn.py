@@ -1,9 +1,16 @@
 """The closure network's GPU kernels against the CPU oracle.
 
-Tolerances: the device and the oracle round to BF16 at the same points
-(inputs, weights, hidden activations, back-propagated tendencies) but
-accumulate in different orders (FP32 matrix cores vs float64), so single
-BF16 roundings can flip.  Forward outputs agree to relative L2 <= 1e-2,
+Layer by layer (test_layer_*): the oracle is fed the device's own BF16
+inputs of each layer (its activations, its back-propagated dL/dz), so only
+the FP32 accumulation order differs from the float64 sums.  Every element
+must lie within half a BF16 ulp of the float64 value plus the FP32
+accumulation bound 2^-24 K S (K terms, S the sum of their magnitudes: a
+relative 1.7e-5 at K = 288), at least 99 % must equal the oracle's own
+BF16 rounding, and the FP32 outputs and weight gradients agree to relative
+L2 2e-5; shapes include widths that are not multiples of 32.
+
+End to end (the network run by each side alone): single BF16 roundings
+can flip and propagate, so forward outputs agree to relative L2 <= 1e-2,
 each layer's gradient to <= 2e-2, the Adam update to float32 rounding."""
 import numpy as np
 import pytest
@@ -40,6 +47,98 @@ def _params(seed):
 
 # shapes: one tile, ragged tiles (width not a multiple of 32), several images
 SHAPES = [(1, 8, 32), (2, 13, 45), (3, 20, 70)]
+
+
+# one tile, ragged tiles (widths 33, 45, 70, 97: not multiples of 32), several images
+LAYER_SHAPES = [(1, 8, 32), (2, 13, 45), (1, 7, 33), (3, 20, 70), (1, 5, 97)]
+EPS32 = 2.0 ** -24
+
+
+def _bf16_ulp(a):
+    """Spacing of BF16 numbers at |a| (float64)."""
+    m = np.maximum(np.abs(a), 2.0 ** -126)
+    return 2.0 ** (np.floor(np.log2(m)) - 7)
+
+
+def _check_bf16(got, exact, err, label):
+    """got: the device's BF16 values; exact: the float64 value it rounded
+    (to within err, the FP32 accumulation bound)."""
+    got = got.astype(np.float64)
+    bound = 0.5 * _bf16_ulp(np.abs(exact) + err) + err
+    bad = np.abs(got - exact) > bound
+    assert not bad.any(), (label, int(bad.sum()), np.argwhere(bad)[:5].tolist())
+    same = got == co.bf16(exact.astype(np.float32)).astype(np.float64)
+    assert same.mean() >= 0.99, (label, same.mean())
+
+
+def _conv_abs(x, w, b):
+    """Sum of the magnitudes of the terms of each output of conv(x, w, b)."""
+    return co.conv(np.abs(x), np.abs(w), np.abs(b))
+
+
+@pytest.mark.parametrize("b,h,w", LAYER_SHAPES)
+def test_layer_forward_on_device_inputs(b, h, w):
+    print(f"cnn layer forward {b}x{h}x{w}", flush=True)
+    p = _params(11)
+    x, _, _ = _batch(12, b, h, w)
+    with ClosureNet(b, h, w) as net:
+        net.set_params(p)
+        net.set_batch(x)
+        pred = net.forward()
+        acts = [co.bf16(x)] + [net.activation(l) for l in range(3)]
+    for l in range(4):
+        wl, bl = co.weights(p, l)
+        wq = co.bf16(wl)
+        z = co.conv(acts[l], wq, bl)
+        err = EPS32 * (9 * co.CIN[l] + 1) * _conv_abs(acts[l], wq, bl)
+        if l < 3:
+            _check_bf16(acts[l + 1], np.maximum(z, 0.0), err, f"activation {l}")
+        else:
+            exact = z[..., :co.NOUT]
+            dev = np.abs(pred - exact)
+            assert (dev <= err[..., :co.NOUT] + EPS32 * np.abs(exact)).all(), ("output", dev.max())
+            assert rel_l2(pred, exact) <= 2e-5, rel_l2(pred, exact)
+
+
+@pytest.mark.parametrize("b,h,w", LAYER_SHAPES)
+def test_layer_backward_on_device_inputs(b, h, w):
+    print(f"cnn layer backward {b}x{h}x{w}", flush=True)
+    p = _params(13)
+    x, tgt, mask = _batch(14, b, h, w)
+    with ClosureNet(b, h, w) as net:
+        net.set_params(p)
+        net.keep_backprop(True)
+        net.set_batch(x, tgt, mask)
+        net.loss_grad()
+        pred = net.forward()
+        acts = [co.bf16(x)] + [net.activation(l) for l in range(3)]
+        gz = [net.backprop(l) for l in range(4)]
+        g = net.grad()
+    # dL/dz_3 from the device's outputs, in the device's FP32 operations
+    m = np.stack([(mask & 1) != 0, (mask & 2) != 0], axis=-1).astype(np.float32)
+    count = int(m.sum())
+    d = (pred.astype(np.float32) - tgt) * m
+    want3 = co.bf16(np.float32(2.0) * d * (np.float32(1.0) / np.float32(count)))
+    assert np.array_equal(gz[3][..., :co.NOUT], want3), "dL/dz_3"
+    assert not gz[3][..., co.NOUT:].any()
+    for l in range(3, -1, -1):
+        woff, boff, kp = co.layout()[l]
+        gz64 = gz[l].astype(np.float64)
+        cols = co._cols(acts[l].astype(np.float64))
+        dw = np.einsum("bhwo,bhwtc->otc", gz64, cols).reshape(co.CH, -1)
+        gw = g[woff:woff + co.CH * kp].reshape(co.CH, kp)
+        assert rel_l2(gw[:, :9 * co.CIN[l]], dw) <= 2e-5, (l, "W", rel_l2(gw[:, :9 * co.CIN[l]], dw))
+        assert not gw[:, 9 * co.CIN[l]:].any()
+        db = gz64.sum(axis=(0, 1, 2))
+        assert rel_l2(g[boff:boff + co.CH], db) <= 2e-5, (l, "b")
+        if l > 0:
+            wq = co.bf16(co.weights(p, l)[0]).astype(np.float64)
+            dx = co._col2im(np.einsum("bhwo,otc->bhwtc", gz64, wq))
+            sx = co._col2im(np.einsum("bhwo,otc->bhwtc", np.abs(gz64), np.abs(wq)))
+            live = acts[l] > 0
+            exact = dx * live
+            err = EPS32 * 9 * co.CH * sx * live
+            _check_bf16(gz[l - 1], exact, err, f"dL/dz_{l - 1}")
 
 
 @pytest.mark.parametrize("b,h,w", SHAPES)
