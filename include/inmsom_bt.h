@@ -596,6 +596,11 @@ int inmsom_bt_cnn_synchronize(inmsom_bt_cnn *net);
  * those keep the plain step (per step).  The network must outlive the
  * attachment; its weights are read at every step. */
 int inmsom_bt_ens_set_closure(inmsom_bt_ens *ens, inmsom_bt_cnn *net, double ratio);
+/* the closure after every `every`-th outer step (1 = every step, the
+ * default; 2, 4 or 8), adding `every` steps' worth of its tendency
+ * (u += every * tau * tendency); the count restarts at set_closure,
+ * set_state and restore */
+int inmsom_bt_ens_set_closure_interval(inmsom_bt_ens *ens, int every);
 /* per member: steps that fell back to plain viscosity, and the closure and
  * viscous tendency norms of the last step (either pointer may be NULL) */
 int inmsom_bt_ens_closure_stats(inmsom_bt_ens *ens, int *nfallback, double *norms);

@@ -168,6 +168,7 @@ struct inmsom_bt_ens {
     // graph of one outer step for (tau, nstep, start phase)
     // graphs of 1 and of GRAPH_STEPS outer steps, per start phase
     hipGraphExec_t gexec[2][2] = {};
+    hipGraphExec_t gexec_plain[2] = {};  // one outer step without the closure (interval > 1)
     double g_tau = -1.0;
     int g_nstep = -1, g_chunk = -1, g_wpx = -1;
     bool g_two = false, g_pipe = false;
@@ -183,6 +184,10 @@ struct inmsom_bt_ens {
     inmsom_bt_cnn *closure = nullptr;
     bool closure_lost = false;  // the attached network was destroyed: runs fail until set_closure
     double closure_ratio = 3.0;
+    // the closure runs after every cl_every-th outer step (1, 2, 4 or 8),
+    // applying cl_every steps' worth of its tendency; cl_phase = outer steps
+    // since it last ran
+    int cl_every = 1, cl_phase = 0, g_every = 1;
     ibt::cnn::Infer inf;
     ibt::cnn::ClosureMember *cl_mem[2] = {};  // device tables per buffer set
 };
@@ -481,8 +486,9 @@ void launch_batch(inmsom_bt_ens *e, int phase, double tau_inner, int sub, int ns
 // launches per outer step: substep pairs, and an odd last substep alone
 int launches_per_step(int nstep, bool two) { return two ? nstep / 2 + nstep % 2 : nstep; }
 
-// launches of one outer step starting from buffer set `phase`
-void outer_step(inmsom_bt_ens *e, int phase, double tau, int nstep, int wpx, bool two)
+// launches of one outer step starting from buffer set `phase`, the
+// closure after it when `closure` (and one is attached)
+void outer_step(inmsom_bt_ens *e, int phase, double tau, int nstep, int wpx, bool two, bool closure = true)
 {
     for (int sub = 1; sub <= nstep;) {
         const bool pair = two && sub + 1 <= nstep;
@@ -490,9 +496,9 @@ void outer_step(inmsom_bt_ens *e, int phase, double tau, int nstep, int wpx, boo
         phase = 1 - phase;
         sub += pair ? 2 : 1;
     }
-    if (e->closure)
-        ibt::cnn::closure_launch(e->closure, e->stream, e->mode == INMSOM_BT_FP32, e->cl_mem[phase], e->inf, tau,
-                                 e->closure_ratio);
+    if (e->closure && closure)
+        ibt::cnn::closure_launch(e->closure, e->stream, e->mode == INMSOM_BT_FP32, e->cl_mem[phase], e->inf,
+                                 tau * e->cl_every, e->closure_ratio);
 }
 
 void drop_closure(inmsom_bt_ens *e)
@@ -520,6 +526,11 @@ void drop_graphs(inmsom_bt_ens *e)
                 hipGraphExecDestroy(g);
                 g = nullptr;
             }
+    for (auto &g : e->gexec_plain)
+        if (g) {
+            hipGraphExecDestroy(g);
+            g = nullptr;
+        }
     e->g_nstep = -1;
 }
 
@@ -531,7 +542,7 @@ bool graphs_current(const inmsom_bt_ens *e, double tau, int nstep, int chunk, in
     const unsigned nv = e->closure ? ibt::cnn::net_version(e->closure) : 0u;
     return e->gexec[0][0] && e->g_tau == tau && e->g_nstep == nstep && e->g_chunk == chunk &&
            e->g_wpx == wpx && e->g_two == two && e->g_pipe == pipe_pairs(e) && e->g_table_gen == e->table_gen && e->g_net == e->closure &&
-           e->g_net_version == nv;
+           e->g_net_version == nv && e->g_every == e->cl_every;
 }
 
 // inmsom_bt_cnn_destroy on the attached network: its memory is about to
@@ -547,23 +558,31 @@ void closure_destroyed(void *owner)
     e->closure_lost = true;
 }
 
+// Graphs per starting buffer set: one outer step (with the closure), 8
+// outer steps starting right after a closure application (the closure
+// after every cl_every-th of them; cl_every divides 8), and with an
+// interval above 1 one outer step without the closure.
 int ensure_graphs(inmsom_bt_ens *e, double tau, int nstep, int chunk, int wpx, bool two)
 {
     if (graphs_current(e, tau, nstep, chunk, wpx, two)) return INMSOM_BT_OK;
     drop_graphs(e);
     const bool flip = launches_per_step(nstep, two) % 2;
     const int nph = flip ? 2 : 1;
-    for (int len = 0; len < 2; ++len)
+    const int kinds = (e->closure && e->cl_every > 1) ? 3 : 2;
+    for (int kind = 0; kind < kinds; ++kind)
         for (int ph = 0; ph < nph; ++ph) {
             hipGraph_t graph = nullptr;
             EHC(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
             int phase = ph;
-            for (int k = 0; k < (len ? GRAPH_STEPS : 1); ++k) {
-                outer_step(e, phase, tau, nstep, wpx, two);
+            const int len = kind == 1 ? GRAPH_STEPS : 1;
+            for (int k = 0; k < len; ++k) {
+                const bool cl = kind == 0 || (kind == 1 && (k + 1) % e->cl_every == 0);
+                outer_step(e, phase, tau, nstep, wpx, two, cl);
                 if (flip) phase = 1 - phase;
             }
             EHC(hipStreamEndCapture(e->stream, &graph));
-            hipError_t r = hipGraphInstantiate(&e->gexec[len][ph], graph, nullptr, nullptr, 0);
+            hipGraphExec_t *dst = kind == 2 ? &e->gexec_plain[ph] : &e->gexec[kind][ph];
+            hipError_t r = hipGraphInstantiate(dst, graph, nullptr, nullptr, 0);
             hipGraphDestroy(graph);
             if (r != hipSuccess)
                 return fail(std::string("hipGraphInstantiate: ") + hipGetErrorString(r), INMSOM_BT_EDEVICE);
@@ -577,6 +596,7 @@ int ensure_graphs(inmsom_bt_ens *e, double tau, int nstep, int chunk, int wpx, b
     e->g_table_gen = e->table_gen;
     e->g_net = e->closure;
     e->g_net_version = e->closure ? ibt::cnn::net_version(e->closure) : 0u;
+    e->g_every = e->cl_every;
     return INMSOM_BT_OK;
 }
 
@@ -600,21 +620,33 @@ int run_steps(inmsom_bt_ens *e, double tau, int nstep, int nsteps, int use_graph
     int rc = build_tables(e, chunk, two);
     if (rc) return rc;
     const bool flip = launches_per_step(nstep, two) % 2;
+    const int every = e->closure ? e->cl_every : 1;
+    // one outer step; the closure after it when it completes an interval
+    auto single = [&](bool graph) -> int {
+        const bool cl = (e->cl_phase + 1) % every == 0;
+        if (graph) {
+            hipGraphExec_t g = cl ? e->gexec[0][flip ? e->phase : 0] : e->gexec_plain[flip ? e->phase : 0];
+            EHC(hipGraphLaunch(g, e->stream));
+        } else {
+            outer_step(e, e->phase, tau, nstep, wpx, two, cl);
+        }
+        if (flip) e->phase = 1 - e->phase;
+        e->cl_phase = (e->cl_phase + 1) % every;
+        return INMSOM_BT_OK;
+    };
     if (use_graph) {
         rc = ensure_graphs(e, tau, nstep, chunk, wpx, two);
         if (rc) return rc;
         int s = 0;
+        for (; s < nsteps && e->cl_phase != 0; ++s)  // to the start of an interval
+            if ((rc = single(true))) return rc;
         for (; s + GRAPH_STEPS <= nsteps; s += GRAPH_STEPS)  // GRAPH_STEPS is even: the phase comes back
             EHC(hipGraphLaunch(e->gexec[1][flip ? e->phase : 0], e->stream));
-        for (; s < nsteps; ++s) {
-            EHC(hipGraphLaunch(e->gexec[0][flip ? e->phase : 0], e->stream));
-            if (flip) e->phase = 1 - e->phase;
-        }
+        for (; s < nsteps; ++s)
+            if ((rc = single(true))) return rc;
     } else {
-        for (int s = 0; s < nsteps; ++s) {
-            outer_step(e, e->phase, tau, nstep, wpx, two);
-            if (flip) e->phase = 1 - e->phase;
-        }
+        for (int s = 0; s < nsteps; ++s)
+            if ((rc = single(false))) return rc;
     }
     EHC(hipGetLastError());
     return INMSOM_BT_OK;
@@ -717,6 +749,7 @@ int inmsom_bt_ens_set_state(inmsom_bt_ens *e, int k, const double *u, const doub
                             const double *vp, const double *ssh, const double *sshp)
 {
     if (!e || k < 0 || k >= (int)e->mem.size()) return fail("bad member index", INMSOM_BT_EINVAL);
+    e->cl_phase = 0;
     if (!u || !up || !v || !vp || !ssh || !sshp) return fail("null pointer", INMSOM_BT_EINVAL);
     EHC(hipSetDevice(e->device));
     EHC(hipStreamSynchronize(e->stream));
@@ -871,6 +904,7 @@ int inmsom_bt_ens_restore(inmsom_bt_ens *e)
     if (!e) return fail("null ensemble", INMSOM_BT_EINVAL);
     if (e->snap.empty()) return fail("inmsom_bt_ens_save was not called", INMSOM_BT_ESTATE);
     EHC(hipSetDevice(e->device));
+    e->cl_phase = 0;
     const size_t es = esz(e->mode);
     size_t j = 0;
     for (auto &mb : e->mem)
@@ -933,6 +967,19 @@ int inmsom_bt_ens_volume(inmsom_bt_ens *e, int k, double *vol)
     return INMSOM_BT_OK;
 }
 
+int inmsom_bt_ens_set_closure_interval(inmsom_bt_ens *e, int every)
+{
+    if (!e) return fail("null ensemble", INMSOM_BT_EINVAL);
+    if (every != 1 && every != 2 && every != 4 && every != 8)
+        return fail("the closure interval must be 1, 2, 4 or 8 outer steps", INMSOM_BT_EINVAL);
+    EHC(hipSetDevice(e->device));
+    EHC(hipStreamSynchronize(e->stream));
+    drop_graphs(e);
+    e->cl_every = every;
+    e->cl_phase = 0;
+    return INMSOM_BT_OK;
+}
+
 int inmsom_bt_ens_set_closure(inmsom_bt_ens *e, inmsom_bt_cnn *net, double ratio)
 {
     if (!e) return fail("null ensemble", INMSOM_BT_EINVAL);
@@ -942,6 +989,7 @@ int inmsom_bt_ens_set_closure(inmsom_bt_ens *e, inmsom_bt_cnn *net, double ratio
     drop_graphs(e);
     drop_closure(e);
     e->closure_lost = false;
+    e->cl_phase = 0;
     if (!net) return INMSOM_BT_OK;
     if (ibt::cnn::net_device(net) != e->device) return fail("network and ensemble on different devices",
                                                             INMSOM_BT_EINVAL);

@@ -139,6 +139,7 @@ SIGNATURES = {
     "inmsom_bt_cnn_eval": ([C.c_void_p, _ip, _dp], C.c_int),
     "inmsom_bt_cnn_synchronize": ([C.c_void_p], C.c_int),
     "inmsom_bt_ens_set_closure": ([C.c_void_p, C.c_void_p, C.c_double], C.c_int),
+    "inmsom_bt_ens_set_closure_interval": ([C.c_void_p, C.c_int], C.c_int),
     "inmsom_bt_ens_closure_stats": ([C.c_void_p, _ip, _dp], C.c_int),
 }
 

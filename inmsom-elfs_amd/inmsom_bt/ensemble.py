@@ -158,13 +158,16 @@ class Ensemble:
         return v.value
 
     # ------------------------------------------------------------ closure
-    def set_closure(self, net, ratio: float = 3.0):
+    def set_closure(self, net, ratio: float = 3.0, interval: int = 1):
         """Attach a learned closure (a ``ClosureNet``; None detaches): after
-        every outer step the network's tendency is added to each member's u
-        and v, except for members whose closure-tendency norm exceeds
-        ``ratio`` times their plain viscous tendency (plain step instead)."""
+        every ``interval``-th outer step the network's tendency times
+        ``interval`` steps is added to each member's u and v, except for
+        members whose closure-tendency norm exceeds ``ratio`` times their
+        plain viscous tendency (plain step instead)."""
         h = None if net is None else net._h
         check(lib().inmsom_bt_ens_set_closure(self._h, h, float(ratio)), "inmsom_bt_ens_set_closure")
+        check(lib().inmsom_bt_ens_set_closure_interval(self._h, int(interval)),
+              "inmsom_bt_ens_set_closure_interval")
         self.closure = net  # the network must outlive the attachment
 
     def closure_stats(self):

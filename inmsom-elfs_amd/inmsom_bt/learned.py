@@ -204,10 +204,11 @@ def rel(a: np.ndarray, b: np.ndarray, m: np.ndarray) -> float:
 
 def rollout(cb: Basin, start: Snapshot, nsteps: int, truth: Optional[Dict[int, Dict[str, np.ndarray]]] = None,
             net: Optional[ClosureNet] = None, ratio: float = 3.0, mode: str = "fp64", nstep: int = 2,
-            every: int = 0, timed: bool = False) -> dict:
+            interval: int = 1, timed: bool = False) -> dict:
     """Coarse run of ``nsteps`` outer steps from ``start`` (C(t), C(t - dt_c / nstep)).
     ``truth`` maps coarse step -> coarsened fine state; errors are scored at
-    those steps.  Returns errors, the final state, fallback counts and the
+    those steps.  ``interval``: the closure after every interval-th step
+    (with interval steps' worth of its tendency).  Returns errors, the final state, fallback counts and the
     run's wall time (ms) when ``timed`` (after the ensemble's autotuner has
     picked the launch shape from the same state; results do not depend on
     the shape)."""
@@ -218,7 +219,7 @@ def rollout(cb: Basin, start: Snapshot, nsteps: int, truth: Optional[Dict[int, D
         ens.set_state(0, start.now["u"], start.prev["u"], start.now["v"], start.prev["v"], start.now["ssh"],
                       start.prev["ssh"])
         if net is not None:
-            ens.set_closure(net, ratio)
+            ens.set_closure(net, ratio, interval)
         if timed:
             ens.autotune(nstep=nstep, trial_steps=20, rows=(0, 2, 3, 4, 6, 8), waves=(0, 256, 384, 512))
         marks = sorted(truth) if truth else []

@@ -3,11 +3,11 @@
 The network committed with the package was trained on the device (by
 inmsom_bt.closure_experiment) on snapshots of the 2048 x 1536, 1 km FP64
 run taken before fine step meta['t_start'].  From that (held-out) time, the
-4 km coarse model runs with and without the closure and both are scored
-against the coarsened 1 km run.  The closure acts on momentum: it must
-reduce the u and v errors while the SSH error stays close to the plain
-model's (its strength was chosen on the training period as the largest
-that does not raise the mean SSH error there)."""
+4 km coarse model runs 10^4 steps of 40 s with and without the closure
+(applied after every second step, two steps' worth of its tendency), and
+both are scored against the coarsened 1 km run: at the end the closure's
+SSH error is no larger than the plain model's and its u and v errors are
+at least 3 % lower."""
 import os
 
 import numpy as np
@@ -22,7 +22,8 @@ from inmsom_bt.workloads import single_basin
 pytestmark = pytest.mark.gpu
 
 PARAMS = os.path.join(ROOT, "inmsom-elfs_amd", "inmsom_bt", "data", "closure_cfg1.npz")
-STEPS = 2000  # coarse 40 s steps (22 simulated hours)
+STEPS = 10000  # coarse 40 s steps (4.6 simulated days)
+INTERVAL = 2
 
 
 @pytest.mark.skipif(not os.path.exists(PARAMS), reason="no trained closure parameters committed")
@@ -42,12 +43,12 @@ def test_trained_closure_beats_plain_viscosity():
         net.set_params(params)
         net.set_norm(nm.shift, nm.scale, nm.out_scale)
         plain = rollout(cb, start, STEPS, tr, None, mode="fp64")
-        learned = rollout(cb, start, STEPS, tr, net, ratio=1e300, mode="fp64")
-    print(f"ssh rel err vs coarsened truth after {STEPS} steps: plain {plain['ssh'][-1]:.4f}, "
-          f"learned {learned['ssh'][-1]:.4f} (u {plain['u'][-1]:.4f} -> {learned['u'][-1]:.4f}, "
-          f"v {plain['v'][-1]:.4f} -> {learned['v'][-1]:.4f}); fallback steps {learned['fallback_steps']}",
+        learned = rollout(cb, start, STEPS, tr, net, ratio=1e300, mode="fp64", interval=INTERVAL)
+    print(f"rel err vs coarsened truth after {STEPS} steps: ssh {plain['ssh'][-1]:.5f} -> "
+          f"{learned['ssh'][-1]:.5f}, u {plain['u'][-1]:.5f} -> {learned['u'][-1]:.5f}, "
+          f"v {plain['v'][-1]:.5f} -> {learned['v'][-1]:.5f}; fallback steps {learned['fallback_steps']}",
           flush=True)
     assert learned["bad"] == 0
-    assert learned["u"][-1] < 0.98 * plain["u"][-1]
-    assert learned["v"][-1] < 0.98 * plain["v"][-1]
-    assert learned["ssh"][-1] < 1.15 * plain["ssh"][-1]
+    assert learned["ssh"][-1] <= plain["ssh"][-1]
+    assert learned["u"][-1] <= 0.97 * plain["u"][-1]
+    assert learned["v"][-1] <= 0.97 * plain["v"][-1]

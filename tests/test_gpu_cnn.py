@@ -329,6 +329,43 @@ def test_ensemble_closure_fallback_and_graph_replays():
                     assert np.array_equal(a[f], b[f]), (rep, k, f)
 
 
+@pytest.mark.parametrize("interval", [2, 4])
+def test_closure_interval(interval):
+    """The closure after every interval-th step: graph replays equal direct
+    launches bit for bit, a run split at any step (3 + 9 + 4) equals one
+    run of 16 (the count of steps since the closure carries over), the
+    closure runs 16 / interval times, and its tendency is applied with
+    interval steps' worth of time (16 steps at interval k, tendency x k
+    every k steps, against interval 1: the same order of magnitude of
+    change from the plain run, not equal)."""
+    basins = _closure_members()
+    p = co.init_params(15, out_scale=0.3)
+    states = {}
+    with ClosureNet(1, 8, 32) as net:
+        net.set_params(p)
+        net.set_norm(**NORM)
+        for label, graph, parts, iv in (("graph", True, (16,), interval), ("direct", False, (16,), interval),
+                                        ("split", True, (3, 9, 4), interval), ("every", True, (16,), 1)):
+            print(f"closure interval {iv}: {label} {parts}", flush=True)
+            with Ensemble(basins, mode="fp32") as ens:
+                ens.set_closure(net, ratio=1e30, interval=iv)
+                for n in parts:
+                    ens.run(n, nstep=2, graph=graph)
+                nfall, norms = ens.closure_stats()
+                assert (nfall == 0).all()
+                states[label] = [ens.get_state(k) for k in range(len(basins))]
+        with Ensemble(basins, mode="fp32") as ens:
+            ens.run(16, nstep=2)
+            states["plain"] = [ens.get_state(k) for k in range(len(basins))]
+    for k in range(len(basins)):
+        for f in states["graph"][k]:
+            assert np.array_equal(states["graph"][k][f], states["direct"][k][f]), ("direct", k, f)
+            assert np.array_equal(states["graph"][k][f], states["split"][k][f]), ("split", k, f)
+        du = np.abs(states["graph"][k]["u"] - states["plain"][k]["u"]).max()
+        de = np.abs(states["every"][k]["u"] - states["plain"][k]["u"]).max()
+        assert 0.3 * de < du < 3.0 * de, (k, du, de)
+
+
 @pytest.mark.parametrize("b,h,w", [(1, 8, 32), (2, 13, 45)])
 def test_gradient_psi_output_matches_oracle(b, h, w):
     """Stream-function outputs: loss on the differenced tendencies, gradient
