@@ -862,7 +862,7 @@ CLOSURE_INTERVAL = 2
 CLOSURE_ENSEMBLE_SCALES = (1.0, 0.5, 0.25)
 
 
-def learned_leg(fine, rollout_steps, members, member_steps, ratio=3.0, rollout_ratio=1e300,
+def learned_leg(fine, rollout_steps, members, member_steps, ratio=3.0, rollout_ratio=3.0,
                 interval=CLOSURE_INTERVAL, score_every=1000):
     """cfg3/cfg4: the trained closure (parameters trained on the device by
     inmsom_bt.closure_experiment, committed) on the 4x-coarsened basin:
@@ -908,6 +908,8 @@ def learned_leg(fine, rollout_steps, members, member_steps, ratio=3.0, rollout_r
                       "v_by_step": r["v"], "range_check_bad_cells": r["bad"]}
         if use:
             runs[name]["fallback_steps"] = r["fallback_steps"]
+            runs[name]["active_fraction"] = 1.0 - r["fallback_steps"] / max(rollout_steps // interval, 1)
+            runs[name]["last_norm_ratio"] = r["norms"][0] / max(r["norms"][1], 1e-300)
     gain = {}
     for m in ("fp64", "fp32"):
         p, q = runs[f"plain_{m}"], runs[f"learned_{m}"]
@@ -930,7 +932,7 @@ def learned_leg(fine, rollout_steps, members, member_steps, ratio=3.0, rollout_r
     out.update({"workload": f"{cb.nx - 4}x{cb.ny - 4} at 4 km (4x block average of the 1 km basin), "
                             f"dt=40 s, nstep_barotrop=2; {rollout_steps} coarse steps from fine step "
                             f"{t_start} scored every {score_every} steps against the coarsened FP64 1 km run; "
-                            f"the closure after every {interval}-th step",
+                            f"the closure after every {interval}-th step, under the {rollout_ratio:g}x rule",
                 "error_vs_coarsened_truth": runs, "gain_vs_plain": gain, "throughput": tm})
     for k, v in runs.items():
         log(f"learned {k}: ssh {v['ssh_rel_l2']:.5f} u {v['u_rel_l2']:.5f} v {v['v_rel_l2']:.5f}")
