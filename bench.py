@@ -76,7 +76,8 @@ TRAFFIC_KERNELS = {"fp64": ("k_fused_pipe<double, 2>",),
                    "dropin": ("k_dropin<true, true>", "k_dropin<false, true>"),
                    "vertical_velocity": ("k_vertical(",),
                    "pressure_gradients": ("k_pressure(",),
-                   "density": ("k_density(",)}
+                   "density": ("k_density(",),
+                   "smagorinsky": ("k_smag(",)}
 
 
 # kernels of one 2-D -> 3-D coupling and their launches per coupling
@@ -609,6 +610,52 @@ def density_leg(m, b, nz, nrep):
                             "bytes_per_call": alg}}
 
 
+# Algorithmic bytes of the Smagorinsky viscosity per array cell: str_t,
+# str_s read and amts, amuv written (32 B x nz); amuv2d written, dx, dy read
+# (24 B) and lu as float (4 B).
+def smagorinsky_leg(m, b, nz, nrep):
+    """The Smagorinsky viscosity and its depth integral
+    (Control/dyncall.f90:40-43) on the same basin with nz levels: device
+    time (HIP events) and the host-array call; the oracle's one-thread C on
+    the same call as the CPU baseline."""
+    rng = np.random.default_rng(29)
+    sh = (b.ny, b.nx)
+    s3 = (nz,) + sh
+    dz = 0.5 + rng.random(nz)
+    f = {"dz": dz / dz.sum(), "stress_t": 1e-5 * rng.standard_normal(s3), "stress_s": 1e-5 * rng.standard_normal(s3),
+         "amts": np.zeros(s3), "amuv": np.zeros(s3), "amuv2d": np.zeros(sh)}
+    args = (f["dz"], f["stress_t"], f["stress_s"], f["amts"], f["amuv"], f["amuv2d"], 1.0e3, 5.0e2)
+    m.smagorinsky(*args)
+    t0 = time.perf_counter()
+    m.smagorinsky(*args)
+    host_s = time.perf_counter() - t0
+    m.smagorinsky_timed(nz, 2)
+    avg_s = m.smagorinsky_timed(nz, nrep) * 1e-3 / nrep
+    alg = (32 * nz + 28) * b.nx * b.ny
+    achieved = alg / avg_s / 1e9
+    log(f"smagorinsky nz={nz}: device {avg_s * 1e3:.3f} ms ({achieved:.0f} GB/s), host-array call "
+        f"{host_s * 1e3:.1f} ms")
+    return f, {"levels": nz, "value": b.n_wet * nz / avg_s, "unit": "wet-cell levels of the Smagorinsky viscosity/s",
+               "device_ms": avg_s * 1e3, "host_call_ms": host_s * 1e3,
+               "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                            "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_from_profiles("smagorinsky"),
+                            "bytes_per_call": alg}}
+
+
+def smagorinsky_cpu(b, f):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    og = oracle.OracleGrid(b)
+    t0 = time.perf_counter()
+    oracle.smagorinsky(og, f, 1.0e3, 5.0e2)
+    dt = time.perf_counter() - t0
+    nz = f["stress_t"].shape[0]
+    return {"value": b.n_wet * nz / dt, "unit": "wet-cell levels of the Smagorinsky viscosity/s", "cores": 1,
+            "kind": "port", "sample": f"one call on the same basin with {nz} levels, {dt:.2f} s, "
+                                      "oracle/bt_oracle.c (bto_smagorinsky, one thread)"}
+
+
 def density_cpu(b, f):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -1054,10 +1101,11 @@ def main():
         vertical = None
         if args.baroclinic_levels > 0:
             vertical_inp, vertical = vertical_leg(m, b, args.baroclinic_levels, args.baroclinic_reps)
-        pressure = density = None
+        pressure = density = smag = None
         if args.baroclinic_levels > 0:
             pressure_inp, pressure = pressure_leg(m, b, args.baroclinic_levels, args.baroclinic_reps)
             density_inp, density = density_leg(m, b, args.baroclinic_levels, args.baroclinic_reps)
+            smag_inp, smag = smagorinsky_leg(m, b, args.baroclinic_levels, args.baroclinic_reps)
         snapshot = snapshot_leg(m, b, args.snapshots) if args.snapshots > 0 else None
         dropin = dropin_leg(m, b) if "fp64" in modes else None
         dropin_dev = dropin_device_leg(m, b) if "fp64" in modes else None
@@ -1130,6 +1178,11 @@ def main():
             density["cpu_baseline"] = density_cpu(b, density_inp)
         del density_inp
         line["density"] = density
+    if smag is not None:
+        if not args.no_cpu_baseline:
+            smag["cpu_baseline"] = smagorinsky_cpu(b, smag_inp)
+        del smag_inp
+        line["smagorinsky"] = smag
     if dropin is not None:
         line["dropin_host_arrays"] = dropin
     if dropin_dev is not None:

@@ -499,6 +499,29 @@ int inmsom_bt_density_dev(inmsom_bt_ctx *ctx, int nz, const double *z, const dou
  * all.  No counterpart in the reference. */
 int inmsom_bt_density_timed(inmsom_bt_ctx *ctx, int nz, int nrep, double *ms);
 
+/* The lateral viscosity the 3-D step hands the barotropic step when
+ * ksw_lat > 0, Control/dyncall.f90:40-43: smagorinsky_coeff
+ * (Function/mixing.f90:171-219) and depth_ave(amuv, amuv2d, lu, 0)
+ * (Function/depth.f90:223-254).  On lu points of the interior and every
+ * level, mu = (0.4 / pi)^2 dx dy sqrt(str_t^2 + mean of str_s^2 over the
+ * four H corners of the T cell), amuv = visc2 + mu, amts = dift + 0.1 mu
+ * (visc2, dift: lvisc_2, ldiff_ts of the reference); both wrapped on
+ * periodic boundaries; then amuv2d = sum_k amuv dz on lu points of
+ * m = 2..nx-1, n = 2..ny-1 -- the mu argument of barotropic_dynamics.
+ * str_t, str_s (nz, ny, nx) from stress_components; amts, amuv (nz, ny, nx)
+ * and amuv2d (ny, nx) are read and written (cells the reference does not
+ * write keep their values).  Host arrays; the _dev variant takes device
+ * addresses (dz a host array) and returns after the work has finished. */
+int inmsom_bt_smagorinsky(inmsom_bt_ctx *ctx, int nz, double visc2, double dift, const double *str_t,
+                          const double *str_s, const double *dz, double *amts, double *amuv, double *amuv2d);
+int inmsom_bt_smagorinsky_dev(inmsom_bt_ctx *ctx, int nz, double visc2, double dift, const double *str_t,
+                              const double *str_s, const double *dz, double *amts, double *amuv,
+                              double *amuv2d);
+/* Timing: nrep repetitions of the device work on the fields the last
+ * inmsom_bt_smagorinsky call with nz levels staged; *ms = HIP-event time
+ * of all.  No counterpart in the reference. */
+int inmsom_bt_smagorinsky_timed(inmsom_bt_ctx *ctx, int nz, int nrep, double *ms);
+
 /* Timing of the coupling's device work: nrep repetitions of the kernels of
  * inmsom_bt_remove_barotropic then inmsom_bt_internal_update on the fields
  * the last inmsom_bt_internal_update call left on the device (it must have

@@ -132,6 +132,11 @@ struct inmsom_bt_ctx {
     size_t eos_n = 0;              // doubles in eos
     int eos_nz = 0;                // levels of the staged fields (0 = none)
     double *eos_z = nullptr;       // z of the device-address call (up to 1000 levels)
+    double *sm = nullptr;          // staging of the Smagorinsky viscosity (bt_smagorinsky.hip)
+    size_t sm_n = 0;               // doubles in sm
+    int sm_nz = 0;                 // levels of the staged fields (0 = none)
+    double sm_visc2 = 0.0, sm_dift = 0.0;  // coefficients of the staged call
+    double *sm_dz = nullptr;       // dz of the device-address call (up to 1000 levels)
     float *snap_dev = nullptr;     // packed snapshot fields (bt_snapshot.hip)
     float *snap_host = nullptr;    // pinned copy of snap_dev
     size_t snap_n = 0;             // floats in snap_dev / snap_host

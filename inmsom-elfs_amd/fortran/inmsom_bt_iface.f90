@@ -287,6 +287,23 @@ interface
     integer(c_int), value :: nz
     type(c_ptr), value :: z, tt, ss, hhq, den, den_pot
   end function
+  ! smagorinsky_coeff + depth_ave (Control/dyncall.f90:40-43)
+  integer(c_int) function inmsom_bt_smagorinsky(ctx, nz, visc2, dift, str_t, str_s, dz, amts, amuv, amuv2d) &
+      bind(C, name="inmsom_bt_smagorinsky")
+    import :: c_int, c_double, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: nz
+    real(c_double), value :: visc2, dift
+    type(c_ptr), value :: str_t, str_s, dz, amts, amuv, amuv2d
+  end function
+  integer(c_int) function inmsom_bt_smagorinsky_dev(ctx, nz, visc2, dift, str_t, str_s, dz, amts, amuv, amuv2d) &
+      bind(C, name="inmsom_bt_smagorinsky_dev")
+    import :: c_int, c_double, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: nz
+    real(c_double), value :: visc2, dift
+    type(c_ptr), value :: str_t, str_s, dz, amts, amuv, amuv2d
+  end function
   function c_strlen(s) bind(C, name="strlen")
     import :: c_ptr, c_size_t
     type(c_ptr), value :: s
@@ -445,5 +462,19 @@ subroutine density_gpu(nz, tt, ss, den, den_pot, z, hhq)
   call bt_check(inmsom_bt_density(bt_ctx, int(nz, c_int), c_loc(z), c_loc(tt), c_loc(ss), c_loc(hhq), &
                 c_loc(den), c_loc(den_pot)), 'density_gpu')
 end subroutine density_gpu
+
+! Control/dyncall.f90:40-43 on the context bt_ctx: smagorinsky_coeff
+! (Function/mixing.f90:171-219) with the reference's arguments, then
+! depth_ave(amuv, amuv2d, lu, 0) (Function/depth.f90:223-254) with the
+! vertical grid dz(nz) it reads.  amts, amuv, amuv2d are read and written;
+! amuv2d is the mu argument of barotropic_dynamics_gpu.
+subroutine smagorinsky_gpu(nz, visc2, dift, str_t, str_s, amts, amuv, amuv2d, dz)
+  integer, intent(in) :: nz
+  real(8), intent(in) :: visc2, dift
+  real(8), target, intent(in) :: str_t(*), str_s(*), dz(*)
+  real(8), target, intent(inout) :: amts(*), amuv(*), amuv2d(*)
+  call bt_check(inmsom_bt_smagorinsky(bt_ctx, int(nz, c_int), real(visc2, c_double), real(dift, c_double), &
+                c_loc(str_t), c_loc(str_s), c_loc(dz), c_loc(amts), c_loc(amuv), c_loc(amuv2d)), 'smagorinsky_gpu')
+end subroutine smagorinsky_gpu
 
 end module inmsom_bt

@@ -558,6 +558,50 @@ class BarotropicModel:
         check(lib().inmsom_bt_density_timed(self._h, int(nz), int(nrep), C.byref(ms)), "inmsom_bt_density_timed")
         return ms.value
 
+    def smagorinsky(self, dz, str_t, str_s, amts, amuv, amuv2d, visc2: float, dift: float):
+        """Control/dyncall.f90:40-43 on host arrays: smagorinsky_coeff
+        (Function/mixing.f90:171-219) then depth_ave(amuv, amuv2d, lu, 0)
+        (Function/depth.f90:223-254).  str_t, str_s, amts, amuv (nz, ny, nx),
+        amuv2d (ny, nx), dz[nz]; visc2, dift are lvisc_2 and ldiff_ts.
+        Returns the new (amts, amuv, amuv2d) as new arrays (cells the
+        reference does not write keep their values)."""
+        t = np.ascontiguousarray(str_t, dtype=np.float64)
+        nz = t.shape[0]
+        s = np.ascontiguousarray(str_s, dtype=np.float64)
+        ts = np.array(amts, dtype=np.float64, order="C")
+        uv = np.array(amuv, dtype=np.float64, order="C")
+        uv2 = np.array(amuv2d, dtype=np.float64, order="C")
+        if t.shape != (nz, self.ny, self.nx) or any(a.shape != t.shape for a in (s, ts, uv)):
+            raise ValueError("str_t, str_s, amts, amuv must be (nz, ny, nx)")
+        if uv2.shape != (self.ny, self.nx):
+            raise ValueError("amuv2d must be (ny, nx)")
+        dzc = np.ascontiguousarray(dz, dtype=np.float64)
+        if dzc.shape != (nz,):
+            raise ValueError("dz must have nz values")
+        check(lib().inmsom_bt_smagorinsky(self._h, nz, float(visc2), float(dift), _d(t), _d(s), _d(dzc), _d(ts),
+                                          _d(uv), _d(uv2)), "inmsom_bt_smagorinsky")
+        return ts, uv, uv2
+
+    def smagorinsky_dev(self, dz, str_t, str_s, amts, amuv, amuv2d, visc2: float, dift: float):
+        """As smagorinsky on DeviceArray arguments; amts, amuv, amuv2d
+        updated in place."""
+        dzc = np.ascontiguousarray(dz, dtype=np.float64)
+        nz = dzc.shape[0]
+        s3, s2 = (nz, self.ny, self.nx), (self.ny, self.nx)
+        check(lib().inmsom_bt_smagorinsky_dev(self._h, nz, float(visc2), float(dift), self._dev(str_t, s3),
+                                              self._dev(str_s, s3), _d(dzc), self._dev(amts, s3),
+                                              self._dev(amuv, s3), self._dev(amuv2d, s2)),
+              "inmsom_bt_smagorinsky_dev")
+
+    def smagorinsky_timed(self, nz: int, nrep: int) -> float:
+        """HIP-event time (ms) of nrep repetitions of the Smagorinsky
+        viscosity's device work on the fields the last host-array call
+        staged."""
+        ms = C.c_double()
+        check(lib().inmsom_bt_smagorinsky_timed(self._h, int(nz), int(nrep), C.byref(ms)),
+              "inmsom_bt_smagorinsky_timed")
+        return ms.value
+
     def forcing_3d_timed(self, nz: int, nrep: int) -> float:
         """HIP-event time (ms) of nrep repetitions of the forcing's device
         work on the fields the last forcing_3d call left on the device."""

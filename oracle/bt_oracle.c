@@ -1019,6 +1019,51 @@ void bto_vertical_velocity(const bto_grid *g, int nz, const double *dz, const do
     for (int k = 0; k <= nz; ++k) cyclize(g, w + (size_t)k * sz);
 }
 
+/* Function/mixing.f90:171-219 (smagorinsky_coeff) followed by
+ * Function/depth.f90:223-254 (depth_ave(amuv, amuv2d, lu, kbcl = 0)), the
+ * pair Control/dyncall.f90:40-43 runs when ksw_lat > 0.  On lu interior
+ * points each level's viscosity is visc2 plus the Smagorinsky term
+ * (kappa / pi)^2 dx dy sqrt(str_t^2 + <str_s^2>), <str_s^2> the mean of
+ * the squared shear at the four H corners of the T cell (kappa = 0.4,
+ * pi = 3.141592653589793, Inc/basinpar.fi:72,89), and the diffusivity is
+ * dift plus a tenth of that term; both are wrapped on periodic
+ * boundaries; then amuv2d = sum_k amuv dz on lu points of the interior and
+ * its one-cell ring.  str_t, str_s, amts, amuv (nz, ny, nx); cells not
+ * computed keep their values. */
+void bto_smagorinsky(const bto_grid *g, int nz, double visc2, double dift, const double *str_t,
+                     const double *str_s, const double *dz, double *amts, double *amuv, double *amuv2d)
+{
+    const size_t sz = (size_t)g->nx * g->ny;
+    const double kp = 0.4 / 3.141592653589793;
+    const double coef = kp * kp, prandtl = 0.1;
+    for (int n = NYS; n <= NYE; ++n)
+        for (int m = NXS; m <= NXE; ++m) {
+            const size_t i = IX(m, n), iw = IX(m - 1, n), is = IX(m, n - 1), isw = IX(m - 1, n - 1);
+            if (!(g->lu[i] > 0.5f)) continue;
+            for (int k = 0; k < nz; ++k) {
+                const size_t o = (size_t)k * sz;
+                const double a = str_s[o + i], b = str_s[o + iw], c = str_s[o + is], d = str_s[o + isw];
+                const double s2 = (a * a + b * b + c * c + d * d) / 4.0;
+                const double t = str_t[o + i];
+                const double mu = coef * g->dx[i] * g->dy[i] * sqrt(t * t + s2);
+                amuv[o + i] = visc2 + mu;
+                amts[o + i] = dift + mu * prandtl;
+            }
+        }
+    for (int k = 0; k < nz; ++k) {
+        cyclize(g, amuv + (size_t)k * sz);
+        cyclize(g, amts + (size_t)k * sz);
+    }
+    for (int n = 2; n <= g->ny - 1; ++n)
+        for (int m = 2; m <= g->nx - 1; ++m) {
+            const size_t i = IX(m, n);
+            if (!(g->lu[i] > 0.5f)) continue;
+            double acc = 0.0;
+            for (int k = 0; k < nz; ++k) acc = acc + amuv[(size_t)k * sz + i] * dz[k];
+            amuv2d[i] = acc;
+        }
+}
+
 /* Function/vel_ssh.f90:320-376 (pressure_gradients): the hydrostatic
  * pressure of density den (nz, ny, nx) integrated down the sigma levels
  * z[nz] between the two T columns of each lcu (x) and lcv (y) interior

@@ -181,6 +181,9 @@ void bto_baroclinic_dynamics(const bto_grid *g, int nz, double tau, const double
 
 /* Function/vel_ssh.f90:1030-1076 (vertical_velocity): u, v (nz, ny, nx),
  * w (nz+1, ny, nx) read and written, dz[nz]; hu, hv, wflux (ny, nx). */
+/* Function/mixing.f90:171-219 + Function/depth.f90:223-254 (kbcl = 0) */
+void bto_smagorinsky(const bto_grid *g, int nz, double visc2, double dift, const double *str_t,
+                     const double *str_s, const double *dz, double *amts, double *amuv, double *amuv2d);
 void bto_vertical_velocity(const bto_grid *g, int nz, const double *dz, const double *u, const double *v, double *w,
                            const double *hu, const double *hv, const double *wflux);
 

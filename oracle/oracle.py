@@ -304,6 +304,17 @@ def vertical_velocity(og: "OracleGrid", inp: Dict[str, np.ndarray]) -> np.ndarra
     return w["ww"]
 
 
+def smagorinsky(og: "OracleGrid", inp: Dict[str, np.ndarray], visc2: float, dift: float) -> Dict[str, np.ndarray]:
+    """bto_smagorinsky on copies of ``inp`` (dz, stress_t, stress_s, amts,
+    amuv (nz, ny, nx), amuv2d); returns the new amts, amuv, amuv2d."""
+    L = lib()
+    w = {k: np.ascontiguousarray(inp[k], dtype=np.float64).copy() for k in inp}
+    nz = w["amuv"].shape[0]
+    L.bto_smagorinsky(C.byref(og.g), nz, C.c_double(visc2), C.c_double(dift), _d(w["stress_t"]),
+                      _d(w["stress_s"]), _d(w["dz"]), _d(w["amts"]), _d(w["amuv"]), _d(w["amuv2d"]))
+    return {k: w[k] for k in ("amts", "amuv", "amuv2d")}
+
+
 def pressure_gradients(og: "OracleGrid", inp: Dict[str, np.ndarray]) -> Dict[str, np.ndarray]:
     """bto_pressure_gradients on copies of ``inp`` (z, den, rhsx, rhsy as
     (nz, ny, nx), hhq, hhu, hhv); returns the updated rhsx, rhsy."""
@@ -381,6 +392,28 @@ def run_reference(case_path: str, out_path: str, nx: int, ny: int, mode: int,
             o += nz * per
         assert o == raw.size
         res["pg_in"], res["pg_out"] = inp, out
+        return res
+    if mode == 10:
+        # masks, dz (nz), stress_t, stress_s, amts, amuv (nz planes), amuv2d,
+        # then amts, amuv (nz planes), amuv2d out
+        nz = (raw.size - 7 * per - 2 * per) // (1 + 6 * per)
+        res = {"masks": {k: raw[i * per:(i + 1) * per].reshape(ny, nx) for i, k in enumerate(MASKS)}}
+        o = 7 * per
+        inp = {"dz": raw[o:o + nz].copy()}
+        o += nz
+        for k in ("stress_t", "stress_s", "amts", "amuv"):
+            inp[k] = raw[o:o + nz * per].reshape(nz, ny, nx).copy()
+            o += nz * per
+        inp["amuv2d"] = raw[o:o + per].reshape(ny, nx).copy()
+        o += per
+        out = {}
+        for k in ("amts", "amuv"):
+            out[k] = raw[o:o + nz * per].reshape(nz, ny, nx).copy()
+            o += nz * per
+        out["amuv2d"] = raw[o:o + per].reshape(ny, nx).copy()
+        o += per
+        assert o == raw.size
+        res["smag_in"], res["smag_out"] = inp, out
         return res
     if mode == 5:
         # masks, dz (nz), uu, vv (nz planes), ww (nz+1), hu, hv, wflux, ww out (nz+1)

@@ -62,6 +62,14 @@
 !   own start_timer / end_timer (mpi_wtime, Modules/mod_mpi_parallel_tools.f90)
 !   around the nsteps steps: after the masks it writes int32 nsteps, int32
 !   OpenMP threads, real8 seconds.
+! mode 10: the Smagorinsky viscosity of Control/dyncall.f90:40-43 (ksw_lat >
+!   0): the reference's stress_components (Function/mixing.f90:115-168) of
+!   synthetic 3-D velocities at the compiled nz, then smagorinsky_coeff
+!   (:171-219) with lvisc_2 = 1e3, ldiff_ts = 5e2 and depth_ave(amuv, amuv2d,
+!   lu, 0) (Function/depth.f90:223-254) from initial amts, amuv, amuv2d the
+!   routines must leave where they do not write: the inputs (dz, stress_t,
+!   stress_s, amts, amuv, amuv2d) and the outputs (amts, amuv, amuv2d) are
+!   written.
 ! mode 8: the reference's grid set-up basinpar (Service/basin_parameters.f90:
 !   191-300 with grid_parameters_spherical / _carthesian, :764-949) for the
 !   compiled Inc/basinpar.fi, on zeroed metric arrays: dx, dy, dxt, dyt, dxh,
@@ -418,6 +426,23 @@ else if (mode == 7) then
     enddo
   enddo
   write(23) den, den_pot
+else if (mode == 10) then
+  do k = 1, nz
+    dz(k) = (0.5d0 + k)/(0.5d0*nz + 0.5d0*nz*(nz + 1))
+    uup(:,:,k) = (ubrtr_e*(1.0d0 + 0.05d0*k) + 0.01d0*k)*lcu
+    vvp(:,:,k) = (vbrtr_e*(1.0d0 - 0.03d0*k) - 0.02d0*k)*lcv
+    amts(:,:,k) = 3.0d0 + 0.25d0*k + ssh_e
+    amuv(:,:,k) = 5.0d0 - 0.5d0*k + sshp_e
+  enddo
+  amuv2d = -7.0d0 + ubrtr_e
+  call stress_components(uup, vvp, stress_t, stress_s, nz)
+  lvisc_2 = 1.0d3
+  ldiff_ts = 5.0d2
+  write(23) dz
+  write(23) stress_t, stress_s, amts, amuv, amuv2d
+  call smagorinsky_coeff(lvisc_2, ldiff_ts, stress_t, stress_s, amts, amuv)
+  call depth_ave(amuv, amuv2d, lu, 0)
+  write(23) amts, amuv, amuv2d
 else if (mode == 8) then
   ! the reference's own grid set-up (Service/basin_parameters.f90, basinpar)
   ! for the compiled Inc/basinpar.fi, from zeroed metric arrays

@@ -546,6 +546,70 @@ def test_density_special_cases():
     assert np.allclose(a["den"][:, inner], b2["den"][:, inner], rtol=0, atol=1e-12)
 
 
+@pytest.mark.parametrize("periodicity", [(0, 0), (1, 1)])
+def test_smagorinsky_closed_forms(periodicity):
+    """Smagorinsky viscosity (Function/mixing.f90:171-219) and its depth
+    integral (Function/depth.f90:223-254) from the formula: with no strain
+    amuv = visc2 and amts = dift exactly; a single shear value s at the H
+    corner (m0, n0) raises exactly the four T cells m0..m0+1 x n0..n0+1 by
+    mu = (0.4/pi)^2 dx dy |s| / 2 (a stencil that looks at the wrong corners
+    raises other cells), amts by mu / 10; a single tension value t raises its
+    own cell by (0.4/pi)^2 dx dy |t|; amuv2d = sum_k amuv dz on the lu cells
+    of 2..nx-1 x 2..ny-1; cells outside those regions keep their values and
+    the periodic wrap copies the interior's opposite edge."""
+    b = make_basin(40, 30, seed=10, sigma=3.0, metrics="stretched")
+    if periodicity != (0, 0):
+        b.lbasins[:, :] = 0
+    og = oracle.OracleGrid(b, periodicity=periodicity)
+    rng = np.random.default_rng(10)
+    nz, sh = 5, (b.ny, b.nx)
+    lu = og.masks["lu"] > 0.5
+    inner = np.zeros(sh, dtype=bool)
+    inner[2:-2, 2:-2] = True
+    dz = 0.5 + rng.random(nz)
+    visc2, dift = 1.0e3, 5.0e2
+    coef = (0.4 / math.pi) ** 2
+    base = {"dz": dz, "stress_t": np.zeros((nz,) + sh), "stress_s": np.zeros((nz,) + sh),
+            "amts": rng.random((nz,) + sh), "amuv": rng.random((nz,) + sh), "amuv2d": rng.random(sh)}
+    calm = oracle.smagorinsky(og, base, visc2, dift)
+    comp = inner & lu
+    assert np.all(calm["amuv"][:, comp] == visc2) and np.all(calm["amts"][:, comp] == dift)
+    m0, n0, k0, s, t = 17, 12, 3, 0.37, -0.21           # 1-based (m, n) of the corner / cell
+    spike = dict(base, stress_s=base["stress_s"].copy(), stress_t=base["stress_t"].copy())
+    spike["stress_s"][k0, n0 - 1, m0 - 1] = s
+    spike["stress_t"][k0 - 2, 5, 8] = t
+    out = oracle.smagorinsky(og, spike, visc2, dift)
+    want_uv = calm["amuv"].copy()
+    want_ts = calm["amts"].copy()
+    for m, n in ((m0, n0), (m0 + 1, n0), (m0, n0 + 1), (m0 + 1, n0 + 1)):
+        if comp[n - 1, m - 1]:
+            mu = coef * b.dx[n - 1, m - 1] * b.dy[n - 1, m - 1] * abs(s) / 2
+            want_uv[k0, n - 1, m - 1] = visc2 + mu
+            want_ts[k0, n - 1, m - 1] = dift + mu / 10
+    assert comp[5, 8]
+    want_uv[k0 - 2, 5, 8] = visc2 + coef * b.dx[5, 8] * b.dy[5, 8] * abs(t)
+    want_ts[k0 - 2, 5, 8] = dift + coef * b.dx[5, 8] * b.dy[5, 8] * abs(t) / 10
+    assert np.allclose(out["amuv"], want_uv, rtol=1e-14, atol=0)
+    assert np.allclose(out["amts"], want_ts, rtol=1e-14, atol=0)
+    assert np.count_nonzero(out["amuv"] != calm["amuv"]) == 1 + sum(
+        comp[n - 1, m - 1] for m, n in ((m0, n0), (m0 + 1, n0), (m0, n0 + 1), (m0 + 1, n0 + 1)))
+    ring = np.zeros(sh, dtype=bool)
+    ring[1:-1, 1:-1] = True
+    ring &= lu
+    for o in (calm, out):
+        assert np.allclose(o["amuv2d"][ring], np.tensordot(dz, o["amuv"], axes=1)[ring], rtol=1e-14, atol=0)
+        assert np.array_equal(o["amuv2d"][~ring], base["amuv2d"][~ring])
+    if periodicity == (0, 0):
+        keep = ~comp
+        assert np.array_equal(out["amuv"][:, keep], base["amuv"][:, keep])
+        assert np.array_equal(out["amts"][:, keep], base["amts"][:, keep])
+    else:
+        for f in ("amuv", "amts"):
+            a = out[f]
+            assert np.array_equal(a[:, :, 1], a[:, :, -3]) and np.array_equal(a[:, :, -2], a[:, :, 2])
+            assert np.array_equal(a[:, 1, :], a[:, -3, :]) and np.array_equal(a[:, -2, :], a[:, 2, :])
+
+
 @pytest.mark.parametrize("periodicity", [(1, 0), (1, 1)])
 def test_periodic_wrap_equals_doubled_domain(periodicity):
     """Periodic wrap (Service/oc_algorithms.f90:50-135, as a decomposed run

@@ -111,6 +111,28 @@ def test_vertical_velocity_bitwise(seed, metrics):
     assert np.any(want != inp["ww"]) and np.any(want == inp["ww"])
 
 
+@pytest.mark.parametrize("seed,metrics", [(41, "uniform"), (42, "stretched")])
+def test_smagorinsky_bitwise(seed, metrics):
+    """smagorinsky_coeff (Function/mixing.f90:171-219) and depth_ave of its
+    viscosity (Function/depth.f90:223-254), as Control/dyncall.f90:40-43
+    runs them, called by the harness on the reference's own stress
+    components, against the oracle on the inputs it wrote; cells the
+    routines do not compute keep their initial values."""
+    b = make_basin(285, 159, seed=seed, sigma=4.0, g=9.8, hmax=120.0, metrics=metrics)
+    b = randomize_state(b, seed + 1, amp_uv=0.05)
+    rd = oracle.OracleGrid(b).rayleigh()
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "case.bin")
+        write_case(path, b, rd, 1, 2, 0, 10)
+        ref = oracle.run_reference(path, path + ".out", b.nx, b.ny, 10)
+    inp, want = ref["smag_in"], ref["smag_out"]
+    assert inp["amuv"].shape[0] == 20
+    got = oracle.smagorinsky(oracle.OracleGrid(b), inp, 1.0e3, 5.0e2)
+    assert_bitwise(got, want, ("amts", "amuv", "amuv2d"), f"smagorinsky seed {seed} {metrics}")
+    for k in ("amts", "amuv", "amuv2d"):
+        assert np.any(want[k] != inp[k]) and np.any(want[k] == inp[k]), k
+
+
 @pytest.mark.parametrize("seed,metrics", [(21, "uniform"), (22, "stretched")])
 def test_pressure_gradients_bitwise(seed, metrics):
     """pressure_gradients (Function/vel_ssh.f90:320-376) called by the
