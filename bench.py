@@ -859,6 +859,16 @@ CLOSURE_PARAMS = os.path.join(ROOT, "inmsom-elfs_amd", "inmsom_bt", "data", "clo
 # Multiply-adds per pixel of one forward pass of the closure network, as
 # the layers are defined (3x3 taps): 4 -> 32, 32 -> 32, 32 -> 32, 32 -> 2.
 CNN_MACS_PER_PIXEL = 9 * (4 * 32 + 32 * 32 + 32 * 32 + 32 * 2)
+# Algorithmic HBM bytes per pixel of one training step, layer by layer with
+# the stored precisions (BF16 activations and gradients of 32 channels,
+# 64 B; the input's 4 channels 8 B; the two outputs and targets FP32, 8 B
+# each; the mask 1 B; dL/dpred of the two outputs in BF16, 4 B):
+#   forward        8 + 64 | 64 + 64 | 64 + 64 | 64 + 8            = 400
+#   loss           8 + 8 + 1 read, 4 written                       =  21
+#   weight grads   layer 3: 4 + 64, layers 2, 1: 64 + 64, layer 0: 64 + 8
+#   data grads     layer 3: 4 + 64 (ReLU mask) + 64, layers 2, 1: 64 + 64 + 64
+#                                                                  = 912
+CNN_TRAIN_BYTES_PER_PIXEL = 400 + 21 + (68 + 128 + 128 + 72) + (132 + 192 + 192)
 PEAK_BF16_TFLOPS = 2500.0
 
 
@@ -890,13 +900,21 @@ def cnn_train_leg(steps=20, batch=32, ny=388, nx=516):
         for k in range(steps):
             net.train_step(idx[2 + k])
         dt = (time.perf_counter() - t0) / steps
-    flops = 3 * 2 * CNN_MACS_PER_PIXEL * batch * ny * nx
+    npx = batch * ny * nx
+    flops = 3 * 2 * CNN_MACS_PER_PIXEL * npx
     tf = flops / dt / 1e12
-    log(f"cnn training: {dt * 1e3:.2f} ms/step (batch {batch} x {ny} x {nx}), {tf:.0f} TFLOP/s")
+    nbytes = CNN_TRAIN_BYTES_PER_PIXEL * npx
+    gbs = nbytes / dt / 1e9
+    # the step's floor: its bytes at the HBM peak take longer than its flops
+    # at the dense BF16 MFMA peak (144 flops per byte against 312)
+    log(f"cnn training: {dt * 1e3:.2f} ms/step (batch {batch} x {ny} x {nx}), {gbs:.0f} GB/s, {tf:.0f} TFLOP/s")
     return {"ms_per_step": dt * 1e3, "batch": batch, "ny": ny, "nx": nx, "steps": steps,
             "value": batch / dt, "unit": "training samples/s (512x384 coarse basins with halo)",
-            "roofline": {"bound": "mfma", "achieved": tf, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
-                         "frac": tf / PEAK_BF16_TFLOPS, "flops_per_step": flops}}
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": gbs / PEAK_HBM_GBS, "bytes_per_step": nbytes,
+                         "bytes_per_pixel": CNN_TRAIN_BYTES_PER_PIXEL},
+            "mfma": {"achieved": tf, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s", "frac": tf / PEAK_BF16_TFLOPS,
+                     "flops_per_step": flops}}
 
 
 # The closure runs after every CLOSURE_INTERVAL-th coarse step with that

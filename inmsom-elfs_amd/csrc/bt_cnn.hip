@@ -319,80 +319,162 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// Partial weight and bias gradients of one chunk of pixels (uniform batch:
-// B images of H x W).  NCB column blocks of 32 cover the 9*CIN columns.
-template <int CIN, int NCB>
-__global__ void __launch_bounds__(64)
-    k_wgrad(int H, int W, long long npix, int chunk, const bf16 *__restrict__ dy,
-            const bf16 *__restrict__ x, float *__restrict__ part)
+// Partial weight and bias gradients (uniform batch: B images of H x W).
+// dW[co][t][ci] = sum_p dy[p, co] x[p + o_t, ci] reduces over pixels, so
+// the MFMA's k axis is pixels and each lane's operand is 8 pixels of one
+// channel.  Rows of x and dy are staged in LDS as they lie in memory
+// (pixel-major, 16-byte loads and stores) and transposed on the way into
+// the operands, 2-byte LDS reads: operand element e of lane half h is
+// pixel 2 e + h of the 16 (the same pairing in A and B, so the product
+// still sums the 16 pixels), which puts the two halves' reads of one
+// element 64 bytes apart, on different banks.
+// A block of three waves walks work items of WG_R rows x WG_TW columns of
+// one image; wave ky multiplies the row's dy (A = dy[px][co]) by row
+// y + ky - 1 of x at the three column shifts kx (B = x[px + kx - 1][ci]):
+// for CIN = 32 one accumulator per tap (its 32 x 32 block of dW), for
+// CIN = 8 one accumulator holding the three taps of its row (column
+// 8 kx + ci).  x rows are kept in a ring of four LDS slots (rows y - 1 ..
+// y + 1 in use while row y + 2 is staged).  The next row's global loads are
+// issued before the current row's matrix products and written to LDS after
+// them; one barrier per row.  Blocks take consecutive runs of work items (a
+// fixed assignment: the record of block k is always the same sum, so the
+// gradient is deterministic).
+constexpr int WG_TW = 64;            // columns of a work item
+constexpr int WG_R = 16;             // rows of a work item
+constexpr int WG_SLOTS = 4;          // x rows in the ring
+
+template <int CIN>
+struct WgradLds {
+    bf16 xs[WG_SLOTS][WG_TW + 2][CIN];  // x row: [p][ci] = x[x0 - 1 + p][ci]
+    bf16 dys[2][WG_TW][CH];             // dy row: [p][co] = dy[x0 + p][co]
+};
+
+template <int CIN>
+__global__ void __launch_bounds__(3 * 64)
+    k_wgrad(int H, int W, int nitems, int per, const bf16 *__restrict__ dy, const bf16 *__restrict__ x,
+            float *__restrict__ part)
 {
-    const int lane = threadIdx.x, j = lane & 31, half = lane >> 5;
-    const long long p0 = (long long)blockIdx.x * chunk;
-    const long long p1 = p0 + chunk < npix ? p0 + chunk : npix;
-    f32x16 acc[NCB];
+    constexpr int NACC = CIN == CH ? 3 : 1;
+    constexpr int XCPP = CIN / 8, YCPP = CH / 8;                 // 16-byte chunks per pixel
+    constexpr int XCH = (WG_TW + 2) * XCPP, YCH = WG_TW * YCPP;  // chunks per staged row
+    constexpr int XPT = (XCH + 191) / 192, YPT = (YCH + 191) / 192;
+    __shared__ __attribute__((aligned(16))) WgradLds<CIN> L;
+
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // ky
+    const int lane = threadIdx.x & 63, i = lane & 31, half = lane >> 5;
+    const int nsx = (W + WG_TW - 1) / WG_TW, nry = (H + WG_R - 1) / WG_R;
+    f32x16 acc[NACC];
 #pragma unroll
-    for (int c = 0; c < NCB; ++c) acc[c] = f32x16{};
+    for (int a = 0; a < NACC; ++a) acc[a] = f32x16{};
     float bsum = 0.0f;
-    const long long hw = (long long)H * W;
-    for (long long p = p0; p < p1; p += 16) {
-        // the lane's 8 consecutive pixels, decoded once and stepped along x
-        bf16x8 a;
-        int yb[8], xb[8];
-        long long bb[8];
-        bool ok[8];
-        long long q0 = p + 8 * half;
-        if (q0 >= p1) q0 = p0;
-        long long b0 = q0 / hw;
-        long long r0 = q0 - b0 * hw;
-        int y0 = (int)(r0 / W), x0 = (int)(r0 - (long long)y0 * W);
+
+    bf16x8 rx[XPT], ry[YPT];
+    // global loads of x row yy and dy row yd of image b, strip x0 (zero
+    // outside the image; yd < 0: no dy row)
+    auto fetch = [&](int b, int x0, int yy, int yd) {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            ok[e] = p + 8 * half + e < p1;
-            bb[e] = b0;
-            yb[e] = y0;
-            xb[e] = x0;
-            const long long q = (b0 * H + y0) * W + x0;
-            const bf16 v = ok[e] ? dy[q * CH + j] : (bf16)0.0f;
-            a[e] = v;
-            bsum += (float)v;
-            if (++x0 == W) {
-                x0 = 0;
-                if (++y0 == H) {
-                    y0 = 0;
-                    ++b0;
-                }
-            }
+        for (int j = 0; j < XPT; ++j) {
+            const int c = threadIdx.x + 192 * j;
+            const int px = c / XCPP, prt = c - px * XCPP, xx = x0 - 1 + px;
+            rx[j] = bf16x8{};
+            if (c < XCH && yy >= 0 && yy < H && xx >= 0 && xx < W)
+                rx[j] = *reinterpret_cast<const bf16x8 *>(x + (((long long)b * H + yy) * W + xx) * CIN + prt * 8);
         }
 #pragma unroll
-        for (int cb = 0; cb < NCB; ++cb) {
-            const int c = cb * 32 + j;
-            const int t = c / CIN, ci = c - t * CIN;
-            bf16x8 b;
+        for (int j = 0; j < YPT; ++j) {
+            const int c = threadIdx.x + 192 * j;
+            const int px = c / YCPP, prt = c - px * YCPP, xx = x0 + px;
+            ry[j] = bf16x8{};
+            if (c < YCH && yd >= 0 && xx < W)
+                ry[j] = *reinterpret_cast<const bf16x8 *>(dy + (((long long)b * H + yd) * W + xx) * CH + prt * 8);
+        }
+    };
+    // the fetched rows into x slot xsl and dy buffer dsl (dsl < 0: none)
+    auto commit = [&](int xsl, int dsl) {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                bf16 v = (bf16)0.0f;
-                if (ok[e] && t < 9) {
-                    const int yy = yb[e] + t / 3 - 1, xx = xb[e] + t % 3 - 1;
-                    if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-                        v = x[((bb[e] * H + yy) * W + xx) * CIN + ci];
+        for (int j = 0; j < XPT; ++j) {
+            const int c = threadIdx.x + 192 * j;
+            const int px = c / XCPP, prt = c - px * XCPP;
+            if (c < XCH) *reinterpret_cast<bf16x8 *>(&L.xs[xsl][px][prt * 8]) = rx[j];
+        }
+        if (dsl < 0) return;
+#pragma unroll
+        for (int j = 0; j < YPT; ++j) {
+            const int c = threadIdx.x + 192 * j;
+            const int px = c / YCPP, prt = c - px * YCPP;
+            if (c < YCH) *reinterpret_cast<bf16x8 *>(&L.dys[dsl][px][prt * 8]) = ry[j];
+        }
+    };
+
+    const int it0 = blockIdx.x * per, it1 = min(it0 + per, nitems);
+    for (int it = it0; it < it1; ++it) {
+        const int b = it / (nsx * nry), rem = it - b * nsx * nry;
+        const int ry0 = rem / nsx, sx = rem - ry0 * nsx;
+        const int x0 = sx * WG_TW, y0 = ry0 * WG_R, y1 = min(y0 + WG_R, H);
+        // prime: x rows y0 - 1, y0, y0 + 1 and dy row y0
+        fetch(b, x0, y0 - 1, -1);
+        commit((y0 - 1 + WG_SLOTS) % WG_SLOTS, -1);
+        fetch(b, x0, y0, -1);
+        commit(y0 % WG_SLOTS, -1);
+        fetch(b, x0, y0 + 1, y0);
+        commit((y0 + 1) % WG_SLOTS, y0 & 1);
+        __syncthreads();
+        for (int y = y0; y < y1; ++y) {
+            const bool more = y + 1 < y1;
+            if (more) fetch(b, x0, y + 2, y + 1);
+            const int xsl = (y + wv - 1 + WG_SLOTS) % WG_SLOTS, dsl = y & 1;
+#pragma unroll
+            for (int ks = 0; ks < WG_TW / 16; ++ks) {
+                bf16x8 a;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) a[e] = L.dys[dsl][16 * ks + 2 * e + half][i];
+                if (wv == 0) {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) bsum += (float)a[e];
                 }
-                b[e] = v;
+                if constexpr (CIN == CH) {
+#pragma unroll
+                    for (int kx = 0; kx < 3; ++kx) {
+                        bf16x8 bb;
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) bb[e] = L.xs[xsl][16 * ks + 2 * e + half + kx][i];
+                        acc[kx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc[kx], 0, 0, 0);
+                    }
+                } else {
+                    // column j = 8 kx + ci (j < 24)
+                    const int kx = i >> 3, ci = i & 7;
+                    bf16x8 bb = bf16x8{};
+                    if (i < 24) {
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) bb[e] = L.xs[xsl][16 * ks + 2 * e + half + kx][ci];
+                    }
+                    acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc[0], 0, 0, 0);
+                }
             }
-            acc[cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[cb], 0, 0, 0);
+            if (more) commit((y + 2) % WG_SLOTS, (y + 1) & 1);
+            __syncthreads();
         }
     }
-    // record of this chunk: W partials [32][ncol], then the 32 bias partials
-    const int ncol = NCB * 32;
+    // record of this block: W partials [32][ncol] (column t CIN + ci), then
+    // the 32 bias partials
+    constexpr int ncol = CIN == CH ? 9 * CH : 96;
     float *rec = part + (size_t)blockIdx.x * (32 * ncol + 32);
 #pragma unroll
-    for (int cb = 0; cb < NCB; ++cb)
+    for (int a = 0; a < NACC; ++a)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int co = 8 * (r >> 2) + 4 * half + (r & 3);
-            rec[co * ncol + cb * 32 + j] = acc[cb][r];
+            if constexpr (CIN == CH) {
+                rec[co * ncol + (3 * wv + a) * CH + i] = acc[a][r];
+            } else if (i < 24) {
+                rec[co * ncol + 24 * wv + i] = acc[a][r];
+                if (wv == 2) rec[co * ncol + 72 + i] = 0.0f;  // padding columns 72..95
+            }
         }
-    bsum += __shfl_xor(bsum, 32);
-    if (half == 0) rec[32 * ncol + j] = bsum;
+    if (wv == 0) {
+        bsum += __shfl_xor(bsum, 32);
+        if (half == 0) rec[32 * ncol + i] = bsum;
+    }
 }
 
 // Fixed-order reduction of the chunk records (rec floats each) in two
@@ -675,7 +757,7 @@ struct inmsom_bt_cnn {
     Img *imgs = nullptr;
     int *tile_img = nullptr, *tile_local = nullptr;
     int ntiles = 0;
-    int wg_chunk = 2048, wg_nchunks = 0;  // pixels per weight-gradient wave
+    int wg_chunk = 2048, wg_nchunks = 0;  // weight-gradient records: one per wg_chunk pixels of the batch
     // device-resident training set
     int nsamples = 0;
     bf16 *ds_x = nullptr;
@@ -776,15 +858,18 @@ void conv_launch(const inmsom_bt_cnn *c, hipStream_t st, int l, bool bwd, const 
 
 void wgrad_launch(inmsom_bt_cnn *c, int l, const bf16 *dy, const bf16 *x)
 {
-    const long long npix = (long long)c->B * c->H * c->W;
-    const int nch = (int)((npix + c->wg_chunk - 1) / c->wg_chunk);
+    // work items (image, 16-row band, 64-column strip) in runs of `per`
+    // per block, at most one record per block of the record buffer
+    const int nitems = c->B * ((c->H + WG_R - 1) / WG_R) * ((c->W + WG_TW - 1) / WG_TW);
+    const int per = (nitems + c->wg_nchunks - 1) / c->wg_nchunks;
+    const int nch = (nitems + per - 1) / per;
     const Layer &Ly = c->L[l];
     if (l == 0)
-        hipLaunchKernelGGL((k_wgrad<CIN0, 3>), dim3(nch), dim3(64), 0, c->stream, c->H, c->W, npix,
-                           c->wg_chunk, dy, x, c->part_w);
+        hipLaunchKernelGGL(k_wgrad<CIN0>, dim3(nch), dim3(3 * 64), 0, c->stream, c->H, c->W, nitems, per, dy, x,
+                           c->part_w);
     else
-        hipLaunchKernelGGL((k_wgrad<CH, 9>), dim3(nch), dim3(64), 0, c->stream, c->H, c->W, npix, c->wg_chunk,
-                           dy, x, c->part_w);
+        hipLaunchKernelGGL(k_wgrad<CH>, dim3(nch), dim3(3 * 64), 0, c->stream, c->H, c->W, nitems, per, dy, x,
+                           c->part_w);
     const int ncol = l == 0 ? 96 : 288;
     const int rec = 32 * ncol + 32;
     const int ngroups = (nch + RG - 1) / RG;

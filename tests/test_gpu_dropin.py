@@ -78,7 +78,7 @@ def test_fused_dropin_matches_oracle(kind, nstep, calls):
             assert_bitwise(got, want, STATE + ACC + DEPTHS, f"{kind} call {call}")
 
 
-@pytest.mark.parametrize("nstep", [1, 2, 4])
+@pytest.mark.parametrize("nstep", [1, 2, 3, 4])
 def test_fused_equals_stage_by_stage(nstep):
     """Five outer steps of the 2-D model (wind stress and drag formed from
     each step's state, then barotropic_dynamics) through either path."""
