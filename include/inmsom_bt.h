@@ -168,6 +168,13 @@ int inmsom_bt_set_state(inmsom_bt_ctx *ctx, const double *u, const double *up,
                         const double *sshp);
 int inmsom_bt_get_state(inmsom_bt_ctx *ctx, double *u, double *up, double *v, double *vp,
                         double *ssh, double *sshp);
+/* No counterpart in the reference (its restarts go through files): a copy
+ * of the context's state kept on the device, and its restore, which is
+ * inmsom_bt_set_state with the saved arrays (depths re-initialised)
+ * without host copies; INMSOM_BT_ESTATE if nothing was saved.  For callers
+ * that rerun from one state, e.g. perturbation runs or timing. */
+int inmsom_bt_save_state(inmsom_bt_ctx *ctx);
+int inmsom_bt_restore_state(inmsom_bt_ctx *ctx);
 /* ubrtr_i, vbrtr_i, ssh4gradx, ssh4grady of the last barotropic_dynamics
  * call on the context's state. */
 int inmsom_bt_get_means(inmsom_bt_ctx *ctx, double *ubrtr_i, double *vbrtr_i,

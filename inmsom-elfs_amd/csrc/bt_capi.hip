@@ -329,6 +329,7 @@ int inmsom_bt_finalize(inmsom_bt_ctx *c)
     for (auto *p : c->acc) fr(p);
     for (auto *p : c->dep) fr(p);
     for (auto *p : c->st_save) fr(p);
+    for (auto *p : c->st_snap) fr(p);
     for (void *p : {(void *)c->rlh_s, (void *)c->hr, (void *)c->taux, (void *)c->tauy,
                     (void *)c->wflux, (void *)c->mu, (void *)c->mu4, (void *)c->rdis,
                     (void *)c->vort, (void *)c->str_t, (void *)c->str_s, (void *)c->fx,
@@ -611,6 +612,40 @@ int inmsom_bt_set_state(inmsom_bt_ctx *c, const double *u, const double *up, con
     const size_t bd = c->n * sizeof(double);
     const double *in[6] = {u, up, v, vp, ssh, sshp};
     for (int k = 0; k < 6; ++k) TRY(upload(c, c->st[k], in[k], bd));
+    c->state_on_fused = false;
+    c->fused_mode = -1;
+    lit_hh_init(c, c->st[4], c->st[5]);
+    c->dep_stale = false;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_save_state(inmsom_bt_ctx *c)
+{
+    REQUIRE(c, "null pointer");
+    HIPCHECK(hipSetDevice(c->device));
+    TRY(fused_flush(c));
+    const size_t bd = c->n * sizeof(double);
+    for (int k = 0; k < 6; ++k) {
+        if (!c->st_snap[k]) HIPCHECK(hipMalloc(&c->st_snap[k], bd));
+        HIPCHECK(hipMemcpyAsync(c->st_snap[k], c->st[k], bd, hipMemcpyDeviceToDevice, c->stream));
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    c->snap_ok = true;
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_restore_state(inmsom_bt_ctx *c)
+{
+    REQUIRE(c, "null pointer");
+    if (!c->snap_ok) {
+        set_error("no saved state: call inmsom_bt_save_state first");
+        return INMSOM_BT_ESTATE;
+    }
+    HIPCHECK(hipSetDevice(c->device));
+    const size_t bd = c->n * sizeof(double);
+    for (int k = 0; k < 6; ++k)
+        HIPCHECK(hipMemcpyAsync(c->st[k], c->st_snap[k], bd, hipMemcpyDeviceToDevice, c->stream));
     c->state_on_fused = false;
     c->fused_mode = -1;
     lit_hh_init(c, c->st[4], c->st[5]);

@@ -526,13 +526,16 @@ __global__ void k_loss(long long n, int H, int W, int kind, const float *__restr
     float e2 = 0.0f;
     if (p < n) {
         const uint8_t mk = mask[p];
+        // the pixel's 32 channels, written as four 16-byte stores (the
+        // channels past the two outputs are zero)
+        bf16x8 g0 = bf16x8{};
         if (kind == OUT_UV) {
 #pragma unroll
             for (int c = 0; c < NOUT; ++c) {
                 const float on = (mk >> c) & 1 ? 1.0f : 0.0f;
                 const float d = (pred[p * NOUT + c] - tgt[p * NOUT + c]) * on;
                 e2 += d * d;
-                dpred[p * CH + c] = (bf16)(2.0f * d * inv_count);
+                g0[c] = (bf16)(2.0f * d * inv_count);
             }
         } else {
             const long long hw = (long long)H * W;
@@ -553,10 +556,12 @@ __global__ void k_loss(long long n, int H, int W, int kind, const float *__restr
             e2 = ru * ru + rv * rv;
             const float ru_n = y + 1 < H ? res(x, y + 1, 0) : 0.0f;
             const float rv_e = x + 1 < W ? res(x + 1, y, 1) : 0.0f;
-            dpred[p * CH + 0] = (bf16)(2.0f * inv_count * (-ru + ru_n + rv - rv_e));
-            dpred[p * CH + 1] = (bf16)0.0f;
+            g0[0] = (bf16)(2.0f * inv_count * (-ru + ru_n + rv - rv_e));
         }
-        for (int c = NOUT; c < CH; ++c) dpred[p * CH + c] = (bf16)0.0f;
+        bf16x8 *o = reinterpret_cast<bf16x8 *>(dpred + p * CH);
+        o[0] = g0;
+#pragma unroll
+        for (int g = 1; g < CH / 8; ++g) o[g] = bf16x8{};
     }
     red[threadIdx.x] = e2;
     __syncthreads();

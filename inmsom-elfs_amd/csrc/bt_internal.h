@@ -147,4 +147,6 @@ struct inmsom_bt_ctx {
     int dropin_path = 0;           // 0 = fused where eligible, 1 = stage-by-stage, 2 = fused (error if not eligible)
     bool dep_stale = false;        // dep[] no longer hh_init(st[4], st[5]) (a fused drop-in call changed st)
     double *st_save[10] = {};      // the context's state and averages kept across a stage-by-stage device-address call
+    double *st_snap[6] = {};       // inmsom_bt_save_state's copy of the state
+    bool snap_ok = false;          // st_snap holds a saved state
 };

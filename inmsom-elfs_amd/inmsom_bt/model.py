@@ -142,6 +142,15 @@ class BarotropicModel:
         a = [np.ascontiguousarray(x, dtype=np.float64) for x in (u, up, v, vp, ssh, sshp)]
         check(lib().inmsom_bt_set_state(self._h, *[_d(x) for x in a]), "inmsom_bt_set_state")
 
+    def save_state(self):
+        """Keeps a copy of the current state on the device."""
+        check(lib().inmsom_bt_save_state(self._h), "inmsom_bt_save_state")
+
+    def restore_state(self):
+        """The saved state back as the current one (set_state without host
+        copies)."""
+        check(lib().inmsom_bt_restore_state(self._h), "inmsom_bt_restore_state")
+
     def get_state(self) -> Dict[str, np.ndarray]:
         out = {k: np.empty((self.ny, self.nx)) for k in STATE}
         check(lib().inmsom_bt_get_state(self._h, *[_d(out[k]) for k in STATE]), "inmsom_bt_get_state")

@@ -205,3 +205,32 @@ def test_gpu_against_reference_vectors(name, path):
             bad = [k for k, d in rec["outputs"][str(step)].items()
                    if k in got and _sha(got[k]) != d["sha256"]]
             assert not bad, f"{name} step {step} via {path}: {bad}"
+
+
+@pytest.mark.parametrize("mode", ["fp64_exact", "fp32"])
+def test_save_restore_state_reruns_bitwise(mode):
+    """inmsom_bt_save_state / _restore_state: a run from the restored state
+    equals, bit for bit, the same run from the state uploaded with
+    set_state (and the exact path still equals the oracle), also when the
+    state was on the fused path when saved; restoring before any save is
+    refused."""
+    b = make_basin(120, 90, seed=43, sigma=6.0)
+    steps, nstep = 6, 2
+    with BarotropicModel(b) as m:
+        with pytest.raises(Exception):
+            m.restore_state()
+        m.run(3, mode, nstep=nstep)              # the saved state lives on the fused path
+        m.save_state()
+        mid = m.get_state()
+        m.run(steps, mode, nstep=nstep)
+        first = m.get_state()
+        m.restore_state()
+        assert_bitwise(m.get_state(), mid, STATE, f"{mode} restored state")
+        m.run(steps, mode, nstep=nstep)
+        assert_bitwise(m.get_state(), first, STATE, f"{mode} rerun from the restored state")
+        m.set_state(mid["u"], mid["up"], mid["v"], mid["vp"], mid["ssh"], mid["sshp"])
+        m.run(steps, mode, nstep=nstep)
+        assert_bitwise(m.get_state(), first, STATE, f"{mode} rerun from set_state")
+    if mode == "fp64_exact":
+        want = oracle.run(b, 3 + steps, nstep, out_steps=[3 + steps])[3 + steps]
+        assert_bitwise(first, want, STATE, "exact path against the oracle")
