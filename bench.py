@@ -299,9 +299,14 @@ def ensemble_leg(steps, warmup, nmembers):
         nmem, wet, cells = ens.size()
         tune = ens.autotune(nstep=NSTEP_BAROTROP, trial_steps=10)
         log(f"ensemble autotune: rows/wave {tune['rows_per_wave']}, waves/XCD {tune['waves_per_xcd']}")
-        warm_steps, warm_s = warm_up(lambda k: ens.run(k, NSTEP_BAROTROP), lambda: None, warmup)
+        # the members' initial volumes first (one short reduction and host
+        # sync each): the timed steps follow the warm-up and the device-side
+        # reset without the GPU idling in between
         ens.reset()
+        ens.save()
         v0 = [ens.volume(k) for k in range(nmem)]
+        warm_steps, warm_s = warm_up(lambda k: ens.run(k, NSTEP_BAROTROP), lambda: None, warmup)
+        ens.restore()
         ms = ens.run_timed(steps, NSTEP_BAROTROP)
         bad = ens.range_check()
         drift = max(abs(ens.volume(k) - v0[k]) / abs(v0[k]) for k in range(nmem))
