@@ -34,8 +34,8 @@ from inmsom_bt.workloads import NSTEP_BAROTROP, single_basin  # noqa: E402
 METRIC = "wet-cell steps/s on 1 GPU: plain FP64 vs FP32/batched/learned at fixed SSH err"
 PEAK_HBM_GBS = 8000.0
 
-# Algorithmic bytes per streamed cell per substep (the survey's minimal
-# count): level n and filtered level n-1 of (u, v, ssh) read and written
+# Algorithmic bytes per streamed cell per substep (the minimal count of
+# what a substep must move): level n and filtered level n-1 of (u, v, ssh) read and written
 # (FP64: 12 x 8 B; FP32 stencils: level n 6 x 4 B, the FP64 filtered level
 # 6 x 8 B), the depth at rest and the Rayleigh coefficient read (2 x 8 B;
 # FP32 2 x 4 B).  Wind stress, Coriolis and viscosity are row tables or
@@ -1113,8 +1113,8 @@ def main():
         return
 
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
+              "(the step is not sharded across GPUs).", file=sys.stderr)
         sys.exit(2)
 
     t0 = time.perf_counter()

@@ -54,3 +54,17 @@ def test_error_path_sets_message():
     assert b"at least 5" in L.inmsom_bt_last_error()
     code = L.inmsom_bt_rayleigh(None, 1e-4, None)
     assert code == _capi.EINVAL and b"null" in L.inmsom_bt_last_error()
+
+
+@pytest.mark.parametrize("n", [2, 8])
+def test_bench_refuses_more_than_one_gpu(n):
+    """bench.py measures one GPU: --gpus above 1 exits with status 2 and
+    says so, before touching a device."""
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    p = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", str(n)], capture_output=True,
+                       text=True, timeout=120)
+    assert p.returncode == 2
+    assert "one GPU" in p.stderr and p.stdout == ""
