@@ -585,6 +585,12 @@ int inmsom_bt_cnn_set_norm(inmsom_bt_cnn *net, const float *in_shift, const floa
 #define INMSOM_BT_CNN_UV 0
 #define INMSOM_BT_CNN_PSI 1
 int inmsom_bt_cnn_set_output(inmsom_bt_cnn *net, int kind);
+/* The closure's inference in the step: 2 runs the four layers in one pass
+ * per tile (the halo rings recomputed: fewer launches, more products), 1
+ * one pass per layer (the first two fused), 0 (default) the one pass for
+ * batches of at most 4096 tiles of 8 x 32 pixels.  The tendencies are the
+ * same bits either way.  No counterpart in the reference. */
+int inmsom_bt_cnn_set_fused(inmsom_bt_cnn *net, int mode);
 /* set (and reset the Adam moments) / read the FP32 master parameters */
 int inmsom_bt_cnn_set_params(inmsom_bt_cnn *net, const float *params);
 int inmsom_bt_cnn_get_params(inmsom_bt_cnn *net, float *params);

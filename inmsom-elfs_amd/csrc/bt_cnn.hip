@@ -49,6 +49,9 @@ enum { EPI_RELU_BF16 = 0, EPI_OUT_F32 = 1, EPI_MASK_BF16 = 2 };
 // Rows of one block tile (RW rows per wave, 4 waves) and its column width.
 constexpr int RW = 2;
 constexpr int TR = 4 * RW;
+// Inference batches of at most this many tiles run the four layers in one
+// pass (k_closure4) unless the network's fuse setting says otherwise.
+constexpr int FUSED_MAX_TILES = 4096;
 constexpr int TW = 32;
 
 // LDS pixel stride (bf16 elements) of a staged input tile: 32-channel
@@ -316,6 +319,148 @@ __global__ void __launch_bounds__(256)
                 *reinterpret_cast<bf16x4 *>(out + pp * CH + co0) = o;
             }
         }
+    }
+}
+
+// One layer of the fused inference on a region of Ho x Wo pixels whose
+// top-left pixel is (oy0, ox0) of the image: its 3 x 3 inputs are the
+// (Ho + 2) x (Wo + 2) region in LDS `in` (pixel stride LIN), its outputs
+// go to LDS `outl` (pixel stride LS, ReLU, zero outside the image, as the
+// next layer's padding) or, for the output layer, to memory.  The region
+// is walked in blocks of 32 consecutive pixels of its row-major order (one
+// MFMA column each, so a region need not be 32 wide); per pixel the
+// products and their order are k_conv's, so the results are the same bits.
+template <int CIN, int KS, int LIN, bool LAST>
+__device__ __forceinline__ void fused_layer(const bf16 *__restrict__ in, int Ho, int Wo, const bf16x8 (&b)[KS],
+                                            const float (&bc)[16], bf16 *__restrict__ outl,
+                                            float *__restrict__ outg, const Img &im, int oy0, int ox0, int wv,
+                                            int i, int half)
+{
+    constexpr int LS = lds_stride<CH>();
+    const int np = Ho * Wo, nb = (np + 31) / 32, Wi = Wo + 2;
+    for (int j = wv; j < nb; j += 4) {
+        const int p = 32 * j + i;
+        const int ry = p / Wo, rx = p - ry * Wo;
+        f32x16 acc = f32x16{};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int k0 = 16 * s + 8 * half;
+            const int t = k0 / CIN, ci0 = k0 - t * CIN;
+            bf16x8 a = {};
+            if (t < 9 && p < np) {
+                const int oy = t / 3 - 1, ox = t % 3 - 1;
+                a = *reinterpret_cast<const bf16x8 *>(in + ((ry + 1 + oy) * Wi + (rx + 1 + ox)) * LIN + ci0);
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[s], a, acc, 0, 0, 0);
+        }
+        if (p >= np) continue;
+        const int y = oy0 + ry, x = ox0 + rx;
+        const bool in_img = y >= 0 && y < im.H && x >= 0 && x < im.W;
+        if constexpr (LAST) {
+            if (in_img && half == 0) {
+                f32x2 o = {acc[0] + bc[0], acc[1] + bc[1]};
+                *reinterpret_cast<f32x2 *>(outg + (im.pix + (long long)y * im.W + x) * NOUT) = o;
+            }
+        } else {
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                bf16x4 o;
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    o[e] = in_img ? (bf16)fmaxf(acc[4 * g + e] + bc[4 * g + e], 0.0f) : (bf16)0.0f;
+                *reinterpret_cast<bf16x4 *>(outl + p * LS + 8 * g + 4 * half) = o;
+            }
+        }
+    }
+}
+
+// The pointer, hidden from the optimiser: weight loads through it stay where
+// they are written (inside the tile loop) instead of being hoisted out of
+// it, where all four layers' weights would have to stay in registers.
+template <class T>
+__device__ __forceinline__ const T *opaque(const T *p)
+{
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
+template <int KS>
+__device__ __forceinline__ void fused_weights(const bf16 *__restrict__ w, const float *__restrict__ bias,
+                                              bf16x8 (&b)[KS], float (&bc)[16], int i, int half)
+{
+#pragma unroll
+    for (int s = 0; s < KS; ++s) b[s] = *reinterpret_cast<const bf16x8 *>(w + (size_t)i * (16 * KS) + 16 * s + 8 * half);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) bc[r] = bias[8 * (r >> 2) + 4 * half + (r & 3)];
+}
+
+// All four layers of the closure in one pass (inference), for batches too
+// small to fill the chip with one layer at a time, where each launch costs
+// more than its products.  For each TR x 32 tile of outputs the block
+// stages the members' normalised (u, v, eta, depth) on the (TR + 8) x 40
+// pixels the four layers reach, then forms layer 0 on (TR + 6) x 38
+// pixels, layer 1 on (TR + 4) x 36, layer 2 on (TR + 2) x 34 in LDS and
+// layer 3 on the tile, written as the tendency.  Region A holds layer 0's
+// outputs and later layer 2's, region B the features and later layer 1's
+// outputs: 77 KB, two blocks per CU.  About 1.44 x the products of the
+// layer-by-layer launches (the halo rings), the same activations bit for
+// bit.
+template <int FT>
+__global__ void __launch_bounds__(256)
+    k_closure4(const Img *__restrict__ imgs, const int *__restrict__ tile_img, const int *__restrict__ tile_local,
+               int ntiles, const bf16 *__restrict__ w0, const float *__restrict__ bias0, const bf16 *__restrict__ w1,
+               const float *__restrict__ bias1, const bf16 *__restrict__ w2, const float *__restrict__ bias2,
+               const bf16 *__restrict__ w3, const float *__restrict__ bias3, float *__restrict__ tend,
+               const ClosureMember *__restrict__ mem, Norm nm)
+{
+    using FT_T = typename std::conditional<FT == 2, double, float>::type;
+    constexpr int LSS = lds_stride<CIN0>(), LS = lds_stride<CH>();
+    constexpr int KS0 = (9 * CIN0 + 15) / 16, KS1 = 9 * CH / 16;
+    constexpr int FH = TR + 8, FW = TW + 8;
+    constexpr int SZA = (TR + 6) * (TW + 6) * LS;
+    constexpr int SZB = (TR + 4) * (TW + 4) * LS > FH * FW * LSS ? (TR + 4) * (TW + 4) * LS : FH * FW * LSS;
+    __shared__ __attribute__((aligned(16))) bf16 lA[SZA];
+    __shared__ __attribute__((aligned(16))) bf16 lB[SZB];
+
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, i = lane & 31, half = lane >> 5;
+    bf16x8 b0[KS0], b[KS1];
+    float bc0[16], bc[16];
+    fused_weights<KS0>(w0, bias0, b0, bc0, i, half);
+
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const Img im = imgs[tile_img[tile]];
+        const int tl = tile_local[tile];
+        const int tx = (im.W + TW - 1) / TW;
+        const int ty = tl / tx;
+        const int y0 = ty * TR, x0 = (tl - ty * tx) * TW;
+        __syncthreads();  // the previous tile's reads of both regions are done
+        // features of pixels (y0 - 4 + fy, x0 - 4 + fx) into region B
+        const ClosureMember &m = mem[tile_img[tile]];
+        for (int q = threadIdx.x; q < FH * FW; q += 256) {
+            const int fy = q / FW, fx = q - fy * FW;
+            const int y = y0 - 4 + fy, x = x0 - 4 + fx;
+            bf16x8 v = {};
+            if (y >= 0 && y < im.H && x >= 0 && x < im.W) {
+                const long long a = (long long)y * m.nx + x;
+                const FT_T f[NIN] = {static_cast<const FT_T *>(m.u)[a], static_cast<const FT_T *>(m.v)[a],
+                                     static_cast<const FT_T *>(m.s)[a], static_cast<const FT_T *>(m.H)[a]};
+#pragma unroll
+                for (int c = 0; c < NIN; ++c) v[c] = (bf16)(((float)f[c] - nm.shift[c]) * nm.scale[c]);
+            }
+            *reinterpret_cast<bf16x8 *>(lB + q * LSS) = v;
+        }
+        fused_weights<KS1>(opaque(w1), opaque(bias1), b, bc, i, half);
+        __syncthreads();
+        fused_layer<CIN0, KS0, LSS, false>(lB, TR + 6, TW + 6, b0, bc0, lA, nullptr, im, y0 - 3, x0 - 3, wv, i,
+                                           half);
+        __syncthreads();
+        fused_layer<CH, KS1, LS, false>(lA, TR + 4, TW + 4, b, bc, lB, nullptr, im, y0 - 2, x0 - 2, wv, i, half);
+        fused_weights<KS1>(opaque(w2), opaque(bias2), b, bc, i, half);
+        __syncthreads();
+        fused_layer<CH, KS1, LS, false>(lB, TR + 2, TW + 2, b, bc, lA, nullptr, im, y0 - 1, x0 - 1, wv, i, half);
+        fused_weights<KS1>(opaque(w3), opaque(bias3), b, bc, i, half);
+        __syncthreads();
+        fused_layer<CH, KS1, LS, true>(lA, TR, TW, b, bc, nullptr, tend, im, y0, x0, wv, i, half);
     }
 }
 
@@ -744,7 +889,8 @@ struct inmsom_bt_cnn {
     Layer L[NL];
     size_t nparam = 0;
     Norm norm;
-    unsigned version = 0;  // bumped when norm changes (captured graphs hold it by value)
+    unsigned version = 0;  // bumped when norm or fuse changes (captured graphs hold them)
+    int fuse = 0;          // inference layers in one pass: 0 = by batch size, 1 = never, 2 = always
     // ensembles using this network as their closure, told on destroy
     std::vector<std::pair<void *, void (*)(void *)>> holders;
     float *wm = nullptr, *am = nullptr, *av = nullptr, *grad = nullptr;  // master, Adam m/v, gradient
@@ -996,23 +1142,42 @@ void infer_destroy(Infer *inf)
     *inf = Infer();
 }
 
+bool closure_fused(const inmsom_bt_cnn *net, const Infer &inf)
+{
+    return net->fuse == 2 || (net->fuse == 0 && inf.ntiles <= FUSED_MAX_TILES);
+}
+
 void closure_launch(const inmsom_bt_cnn *net, hipStream_t st, bool f32, ClosureMember *members, Infer &inf,
                     double dt, double ratio)
 {
     const dim3 rows((unsigned)inf.nrows), blk(256);
     const dim3 grid((unsigned)(inf.ntiles < 2048 ? inf.ntiles : 2048));
     const Layer &L0 = net->L[0];
-    // layers 0 and 1 from the members' state in one pass (-> a1), then a1 -> a0 -> tend
     const Layer &L1 = net->L[1];
     const bf16 *w0 = net->wf + net->wf_off[0], *w1 = net->wf + net->wf_off[1];
-    if (f32)
-        hipLaunchKernelGGL((k_conv01<1>), grid, blk, 0, st, inf.imgs, inf.tile_img, inf.tile_local, inf.ntiles, w0,
-                           net->wm + L0.boff, w1, net->wm + L1.boff, inf.a1, members, net->norm);
-    else
-        hipLaunchKernelGGL((k_conv01<2>), grid, blk, 0, st, inf.imgs, inf.tile_img, inf.tile_local, inf.ntiles, w0,
-                           net->wm + L0.boff, w1, net->wm + L1.boff, inf.a1, members, net->norm);
-    conv_launch(net, st, 2, false, inf.imgs, inf.tile_img, inf.tile_local, inf.ntiles, inf.a1, nullptr, inf.a0);
-    conv_launch(net, st, 3, false, inf.imgs, inf.tile_img, inf.tile_local, inf.ntiles, inf.a0, nullptr, inf.tend);
+    if (closure_fused(net, inf)) {
+        // the four layers in one pass (-> tend)
+        const bf16 *w2 = net->wf + net->wf_off[2], *w3 = net->wf + net->wf_off[3];
+        const float *bs[NL] = {net->wm + L0.boff, net->wm + L1.boff, net->wm + net->L[2].boff,
+                               net->wm + net->L[3].boff};
+        if (f32)
+            hipLaunchKernelGGL((k_closure4<1>), grid, blk, 0, st, inf.imgs, inf.tile_img, inf.tile_local, inf.ntiles,
+                               w0, bs[0], w1, bs[1], w2, bs[2], w3, bs[3], inf.tend, members, net->norm);
+        else
+            hipLaunchKernelGGL((k_closure4<2>), grid, blk, 0, st, inf.imgs, inf.tile_img, inf.tile_local, inf.ntiles,
+                               w0, bs[0], w1, bs[1], w2, bs[2], w3, bs[3], inf.tend, members, net->norm);
+    } else {
+        // layers 0 and 1 from the members' state in one pass (-> a1), then a1 -> a0 -> tend
+        if (f32)
+            hipLaunchKernelGGL((k_conv01<1>), grid, blk, 0, st, inf.imgs, inf.tile_img, inf.tile_local, inf.ntiles,
+                               w0, net->wm + L0.boff, w1, net->wm + L1.boff, inf.a1, members, net->norm);
+        else
+            hipLaunchKernelGGL((k_conv01<2>), grid, blk, 0, st, inf.imgs, inf.tile_img, inf.tile_local, inf.ntiles,
+                               w0, net->wm + L0.boff, w1, net->wm + L1.boff, inf.a1, members, net->norm);
+        conv_launch(net, st, 2, false, inf.imgs, inf.tile_img, inf.tile_local, inf.ntiles, inf.a1, nullptr, inf.a0);
+        conv_launch(net, st, 3, false, inf.imgs, inf.tile_img, inf.tile_local, inf.ntiles, inf.a0, nullptr,
+                    inf.tend);
+    }
     if (f32) {
         hipLaunchKernelGGL(k_cl_norm<float>, rows, blk, 0, st, members, inf.row_member, inf.row_y, net->norm,
                            inf.tend, inf.part);
@@ -1145,6 +1310,16 @@ int inmsom_bt_cnn_set_norm(inmsom_bt_cnn *c, const float *in_shift, const float 
     }
     c->norm.oscale[0] = out_scale[0];
     c->norm.oscale[1] = out_scale[1];
+    ++c->version;
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_cnn_set_fused(inmsom_bt_cnn *c, int mode)
+{
+    if (!c) return cfail("null pointer", INMSOM_BT_EINVAL);
+    if (mode < 0 || mode > 2) return cfail("fused mode must be 0 (by batch size), 1 (never) or 2 (always)",
+                                           INMSOM_BT_EINVAL);
+    c->fuse = mode;
     ++c->version;
     return INMSOM_BT_OK;
 }

@@ -102,6 +102,7 @@ SIGNATURES = {
     "inmsom_bt_density_dev": ([C.c_void_p, C.c_int] + [_dp] * 6, C.c_int),
     "inmsom_bt_density_timed": ([C.c_void_p, C.c_int, C.c_int, _dp], C.c_int),
     "inmsom_bt_save_state": ([C.c_void_p], C.c_int),
+    "inmsom_bt_cnn_set_fused": ([C.c_void_p, C.c_int], C.c_int),
     "inmsom_bt_restore_state": ([C.c_void_p], C.c_int),
     "inmsom_bt_smagorinsky": ([C.c_void_p, C.c_int, C.c_double, C.c_double] + [_dp] * 6, C.c_int),
     "inmsom_bt_smagorinsky_dev": ([C.c_void_p, C.c_int, C.c_double, C.c_double] + [_dp] * 6, C.c_int),

@@ -87,6 +87,13 @@ class ClosureNet:
         check(lib().inmsom_bt_cnn_set_output(self._h, OUTPUT_KINDS[kind]), "inmsom_bt_cnn_set_output")
         self.kind = kind
 
+    def set_fused(self, mode: str = "auto"):
+        """The closure's inference in the step: 'always' the four layers in
+        one pass per tile, 'never' one pass per layer, 'auto' by batch size
+        (inmsom_bt_cnn_set_fused); the same tendencies either way."""
+        modes = {"auto": 0, "never": 1, "always": 2}
+        check(lib().inmsom_bt_cnn_set_fused(self._h, modes[mode]), "inmsom_bt_cnn_set_fused")
+
     def set_params(self, p: np.ndarray):
         p = np.ascontiguousarray(p, dtype=np.float32)
         if p.size != self.nparam:

@@ -445,3 +445,40 @@ def test_closure_graphs_follow_norm_changes():
                 ens.run(1, nstep=2, graph=graph)
             ens.set_closure(None)
             ens.run(1, nstep=2, graph=graph)
+
+
+@pytest.mark.parametrize("mode", ["fp32", "fp64"])
+def test_fused_inference_equals_layer_by_layer(mode):
+    """The closure's four layers in one pass per tile (k_closure4) give the
+    tendencies of one pass per layer bit for bit: 12 outer steps of a
+    ragged 4-member ensemble (sizes that are not multiples of the 8 x 32
+    tiles) with the closure always applied and under the 3x rule, graph
+    replays and direct launches, the same states and closure statistics."""
+    basins = _closure_members() + [randomize_state(ensemble(1, seed=5, size_lo=(33, 9), size_hi=(33, 9),
+                                                            sigma=3.0)[0], 9)]
+    p = co.init_params(17, out_scale=0.3)
+    out = {}
+    with ClosureNet(1, 8, 32) as net:
+        net.set_params(p)
+        net.set_norm(**NORM)
+        for fused in ("always", "never"):
+            net.set_fused(fused)
+            for ratio in (1e30, 3.0):
+                for graph in (True, False):
+                    print(f"{mode} closure inference {fused}, ratio {ratio:g}, graph {graph}", flush=True)
+                    with Ensemble(basins, mode=mode) as ens:
+                        ens.set_closure(net, ratio=ratio)
+                        ens.run(12, nstep=2, graph=graph)
+                        nfall, norms = ens.closure_stats()
+                        out[(fused, ratio, graph)] = ([ens.get_state(k) for k in range(len(basins))],
+                                                      np.array(nfall), np.array(norms))
+        net.set_fused("auto")
+    for (fused, ratio, graph), (st, nf, nr) in out.items():
+        ref_st, ref_nf, ref_nr = out[("never", ratio, True)]
+        assert np.array_equal(nf, ref_nf) and np.array_equal(nr.view(np.uint64), ref_nr.view(np.uint64)), \
+            (fused, ratio, graph)
+        for k in range(len(basins)):
+            for f in ref_st[k]:
+                assert np.array_equal(st[k][f].view(np.uint64), ref_st[k][f].view(np.uint64)), (fused, ratio, graph,
+                                                                                               k, f)
+    assert (out[("never", 3.0, True)][2][:, 0] > 0).all()              # every member had a closure tendency
