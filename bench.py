@@ -651,11 +651,25 @@ def smagorinsky_leg(m, b, nz, nrep):
     achieved = alg / avg_s / 1e9
     log(f"smagorinsky nz={nz}: device {avg_s * 1e3:.3f} ms ({achieved:.0f} GB/s), host-array call "
         f"{host_s * 1e3:.1f} ms")
+    # its input, stress_components of the 3-D velocities (Control/dyncall.f90:36):
+    # u, v read and str_t, str_s written (32 B x nz); eight metrics read
+    # (64 B) and lu, luu as float (8 B)
+    uv = {"u": 0.1 * rng.standard_normal(s3), "v": 0.1 * rng.standard_normal(s3)}
+    m.stress_components(uv["u"], uv["v"], f["stress_t"], f["stress_s"])
+    warm_up(lambda k: m.stress_components_timed(nz, k), lambda: None, 2, LEG_WARMUP_SECONDS)
+    st_s = m.stress_components_timed(nz, nrep) * 1e-3 / nrep
+    st_alg = (32 * nz + 72) * b.nx * b.ny
+    log(f"stress_components nz={nz}: device {st_s * 1e3:.3f} ms ({st_alg / st_s / 1e9:.0f} GB/s)")
     return f, {"levels": nz, "value": b.n_wet * nz / avg_s, "unit": "wet-cell levels of the Smagorinsky viscosity/s",
                "device_ms": avg_s * 1e3, "host_call_ms": host_s * 1e3,
                "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                             "frac": achieved / PEAK_HBM_GBS, "traffic": traffic_from_profiles("smagorinsky"),
-                            "bytes_per_call": alg}}
+                            "bytes_per_call": alg},
+               "stress_components": {"device_ms": st_s * 1e3,
+                                     "roofline": {"bound": "hbm", "achieved": st_alg / st_s / 1e9,
+                                                  "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                                  "frac": st_alg / st_s / 1e9 / PEAK_HBM_GBS,
+                                                  "bytes_per_call": st_alg}}}
 
 
 def smagorinsky_cpu(b, f):

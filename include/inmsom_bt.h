@@ -506,6 +506,23 @@ int inmsom_bt_density_dev(inmsom_bt_ctx *ctx, int nz, const double *z, const dou
  * all.  No counterpart in the reference. */
 int inmsom_bt_density_timed(inmsom_bt_ctx *ctx, int nz, int nrep, double *ms);
 
+/* stress_components, Function/mixing.f90:115-168, of the 3-D velocities u,
+ * v (nz, ny, nx), as Control/dyncall.f90:36 calls it for the Smagorinsky
+ * viscosity: on lu points of the interior str_t = dy/dx (u/dyh - u_w/dyh_w)
+ * - dx/dy (v/dxh - v_s/dxh_s), on luu points str_s = dxb/dyb (u_n/dxt_n -
+ * u/dxt) + dyb/dxb (v_e/dyt_e - v/dyt), every level, then both wrapped on
+ * periodic boundaries.  str_t, str_s (nz, ny, nx) are read and written
+ * (cells the reference does not write keep their values).  Host arrays;
+ * the _dev variant takes device addresses and returns after the work has
+ * finished; _timed repeats the device work on the fields the last
+ * host-array call staged (*ms = HIP-event time of all; no counterpart in
+ * the reference). */
+int inmsom_bt_stress_components(inmsom_bt_ctx *ctx, int nz, const double *u, const double *v, double *str_t,
+                                double *str_s);
+int inmsom_bt_stress_components_dev(inmsom_bt_ctx *ctx, int nz, const double *u, const double *v,
+                                    double *str_t, double *str_s);
+int inmsom_bt_stress_components_timed(inmsom_bt_ctx *ctx, int nz, int nrep, double *ms);
+
 /* The lateral viscosity the 3-D step hands the barotropic step when
  * ksw_lat > 0, Control/dyncall.f90:40-43: smagorinsky_coeff
  * (Function/mixing.f90:171-219) and depth_ave(amuv, amuv2d, lu, 0)

@@ -340,7 +340,7 @@ int inmsom_bt_finalize(inmsom_bt_ctx *c)
                     (void *)c->f3_mb, (void *)c->f3_halo, (void *)c->f3_dz,
                     (void *)c->snap_dev, (void *)c->bc, (void *)c->bc_dev, (void *)c->vv, (void *)c->vv_dz,
                     (void *)c->pg, (void *)c->pg_z, (void *)c->eos, (void *)c->eos_z,
-                    (void *)c->sm, (void *)c->sm_dz})
+                    (void *)c->sm, (void *)c->sm_dz, (void *)c->sst})
         fr(p);
     if (c->snap_host) hipHostFree(c->snap_host);
     if (c->stream) hipStreamDestroy(c->stream);

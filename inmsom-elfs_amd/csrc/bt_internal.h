@@ -137,6 +137,9 @@ struct inmsom_bt_ctx {
     int sm_nz = 0;                 // levels of the staged fields (0 = none)
     double sm_visc2 = 0.0, sm_dift = 0.0;  // coefficients of the staged call
     double *sm_dz = nullptr;       // dz of the device-address call (up to 1000 levels)
+    double *sst = nullptr;         // staging of the 3-D stress_components (bt_smagorinsky.hip)
+    size_t sst_n = 0;              // doubles in sst
+    int sst_nz = 0;                // levels of the staged fields (0 = none)
     float *snap_dev = nullptr;     // packed snapshot fields (bt_snapshot.hip)
     float *snap_host = nullptr;    // pinned copy of snap_dev
     size_t snap_n = 0;             // floats in snap_dev / snap_host

@@ -24,6 +24,13 @@
 // nothing is read back.  k_smag_ring sums the one-cell ring of the
 // interior after the wraps (values copied there, or the caller's).
 // Cells the reference does not compute keep the caller's values.
+//
+// stress_components (Function/mixing.f90:115-168) of the 3-D velocities,
+// the input of the pair (Control/dyncall.f90:36): tension on lu points and
+// shear on luu points of the interior, every level, in the reference's
+// operation order, then both wrapped on periodic boundaries.  k_stress3
+// walks each interior column like k_smag; the per-point metric ratios are
+// the same at every level and formed once.
 #include <string>
 
 #include "bt_internal.h"
@@ -79,6 +86,45 @@ __global__ __launch_bounds__(256) void k_smag(SmParams p)
         acc = acc + v * dz[k];
     }
     p.amuv2d[i] = acc;
+}
+
+struct StParams {
+    DevGrid g;
+    int nz;
+    int tiles_x, tiles_y;
+    const double *u, *v;      // (nz, ny, nx)
+    double *str_t, *str_s;    // (nz, ny, nx)
+};
+
+__global__ __launch_bounds__(256) void k_stress3(StParams p)
+{
+    const DevGrid &g = p.g;
+    int bx, by;
+    xcd_tile(p.tiles_x, p.tiles_y, (int)blockIdx.x, bx, by);
+    const int m = bx * 64 + (int)(threadIdx.x & 63) + 3;
+    const int n = by * 4 + (int)(threadIdx.x >> 6) + 3;
+    if (m > g.nx - 2 || n > g.ny - 2) return;
+    const size_t pl = (size_t)g.nx * g.ny, i = I(m, n), iw = I(m - 1, n), is = I(m, n - 1), in = I(m, n + 1),
+                 ie = I(m + 1, n);
+    const bool tw = g.lu[i] > 0.5f, sw = g.luu[i] > 0.5f;
+    if (!tw && !sw) return;
+    const double *__restrict__ u = p.u;
+    const double *__restrict__ v = p.v;
+    double *__restrict__ st = p.str_t;
+    double *__restrict__ ss = p.str_s;
+    // tension: dy/dx (u/dyh - u_w/dyh_w) - dx/dy (v/dxh - v_s/dxh_s)
+    const double ryx = g.dy[i] / g.dx[i], rxy = g.dx[i] / g.dy[i];
+    const double dyh0 = g.dyh[i], dyhw = g.dyh[iw], dxh0 = g.dxh[i], dxhs = g.dxh[is];
+    // shear: dxb/dyb (u_n/dxt_n - u/dxt) + dyb/dxb (v_e/dyt_e - v/dyt)
+    const double rbxy = g.dxb[i] / g.dyb[i], rbyx = g.dyb[i] / g.dxb[i];
+    const double dxtn = g.dxt[in], dxt0 = g.dxt[i], dyte = g.dyt[ie], dyt0 = g.dyt[i];
+#pragma unroll 2
+    for (int k = 0; k < p.nz; ++k) {
+        const size_t o = (size_t)k * pl;
+        const double u0 = u[o + i], v0 = v[o + i];
+        if (tw) st[o + i] = ryx * (u0 / dyh0 - u[o + iw] / dyhw) - rxy * (v0 / dxh0 - v[o + is] / dxhs);
+        if (sw) ss[o + i] = rbxy * (u[o + in] / dxtn - u0 / dxt0) + rbyx * (v[o + ie] / dyte - v0 / dyt0);
+    }
 }
 
 // column mm -> 2, 3 -> nx - 1 (x); row nn -> 2, 3 -> ny - 1 (y); both
@@ -158,6 +204,33 @@ void launch(inmsom_bt_ctx *c, SmParams p)
                            c->stream, c->g, p.nz, p.amuv, p.amts);
     const int ring = 2 * (c->nx - 2) + 2 * (c->ny - 4);
     hipLaunchKernelGGL(k_smag_ring, dim3((unsigned)((ring + 255) / 256)), dim3(256), 0, c->stream, p);
+}
+
+void launch_stress(inmsom_bt_ctx *c, StParams p)
+{
+    p.tiles_x = (c->nx - 4 + 63) / 64;
+    p.tiles_y = (c->ny - 4 + 3) / 4;
+    hipLaunchKernelGGL(k_stress3, dim3((unsigned)(p.tiles_x * p.tiles_y)), dim3(256), 0, c->stream, p);
+    if (c->g.px)
+        hipLaunchKernelGGL(k_smag_wrap_x, dim3((unsigned)((c->ny + 255) / 256), (unsigned)(2 * p.nz)), dim3(256), 0,
+                           c->stream, c->g, p.nz, p.str_t, p.str_s);
+    if (c->g.py)
+        hipLaunchKernelGGL(k_smag_wrap_y, dim3((unsigned)((c->nx + 255) / 256), (unsigned)(2 * p.nz)), dim3(256), 0,
+                           c->stream, c->g, p.nz, p.str_t, p.str_s);
+}
+
+// staging of stress_components: u, v, str_t, str_s (nz planes each)
+StParams staged_stress(inmsom_bt_ctx *c, int nz)
+{
+    const size_t vol = (size_t)nz * c->n;
+    StParams p{};
+    p.g = c->g;
+    p.nz = nz;
+    p.u = c->sst;
+    p.v = c->sst + vol;
+    p.str_t = c->sst + 2 * vol;
+    p.str_s = c->sst + 3 * vol;
+    return p;
 }
 
 int check(inmsom_bt_ctx *c, int nz)
@@ -260,6 +333,81 @@ int inmsom_bt_smagorinsky_dev(inmsom_bt_ctx *c, int nz, double visc2, double dif
     launch(c, p);
     CHS(hipGetLastError());
     CHS(hipStreamSynchronize(c->stream));
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_stress_components(inmsom_bt_ctx *c, int nz, const double *u, const double *v, double *str_t,
+                                double *str_s)
+{
+    int rc = check(c, nz);
+    if (rc) return rc;
+    if (!u || !v || !str_t || !str_s) return sfail("null pointer", INMSOM_BT_EINVAL);
+    CHS(hipSetDevice(c->device));
+    const size_t vol = (size_t)nz * c->n, need = 4 * vol;
+    if (!c->sst || c->sst_n < need) {
+        if (c->sst) CHS(hipFree(c->sst));
+        c->sst = nullptr;
+        c->sst_n = 0;
+        c->sst_nz = 0;
+        CHS(hipMalloc(&c->sst, need * sizeof(double)));
+        c->sst_n = need;
+    }
+    const StParams p = staged_stress(c, nz);
+    const size_t bytes = vol * sizeof(double);
+    CHS(hipMemcpyAsync(const_cast<double *>(p.u), u, bytes, hipMemcpyHostToDevice, c->stream));
+    CHS(hipMemcpyAsync(const_cast<double *>(p.v), v, bytes, hipMemcpyHostToDevice, c->stream));
+    CHS(hipMemcpyAsync(p.str_t, str_t, bytes, hipMemcpyHostToDevice, c->stream));
+    CHS(hipMemcpyAsync(p.str_s, str_s, bytes, hipMemcpyHostToDevice, c->stream));
+    launch_stress(c, p);
+    CHS(hipGetLastError());
+    CHS(hipMemcpyAsync(str_t, p.str_t, bytes, hipMemcpyDeviceToHost, c->stream));
+    CHS(hipMemcpyAsync(str_s, p.str_s, bytes, hipMemcpyDeviceToHost, c->stream));
+    CHS(hipStreamSynchronize(c->stream));
+    c->sst_nz = nz;
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_stress_components_dev(inmsom_bt_ctx *c, int nz, const double *u, const double *v, double *str_t,
+                                    double *str_s)
+{
+    int rc = check(c, nz);
+    if (rc) return rc;
+    if (!u || !v || !str_t || !str_s) return sfail("null pointer", INMSOM_BT_EINVAL);
+    CHS(hipSetDevice(c->device));
+    StParams p{};
+    p.g = c->g;
+    p.nz = nz;
+    p.u = u;
+    p.v = v;
+    p.str_t = str_t;
+    p.str_s = str_s;
+    launch_stress(c, p);
+    CHS(hipGetLastError());
+    CHS(hipStreamSynchronize(c->stream));
+    return INMSOM_BT_OK;
+}
+
+int inmsom_bt_stress_components_timed(inmsom_bt_ctx *c, int nz, int nrep, double *ms)
+{
+    if (!c || !ms) return sfail("null pointer", INMSOM_BT_EINVAL);
+    if (nrep < 1) return sfail("nrep must be >= 1", INMSOM_BT_EINVAL);
+    if (!c->sst || c->sst_nz != nz)
+        return sfail("no fields staged for nz levels: call inmsom_bt_stress_components first", INMSOM_BT_ESTATE);
+    CHS(hipSetDevice(c->device));
+    const StParams p = staged_stress(c, nz);
+    hipEvent_t e0, e1;
+    CHS(hipEventCreate(&e0));
+    CHS(hipEventCreate(&e1));
+    CHS(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < nrep; ++r) launch_stress(c, p);
+    CHS(hipGetLastError());
+    CHS(hipEventRecord(e1, c->stream));
+    CHS(hipEventSynchronize(e1));
+    float t = 0.0f;
+    CHS(hipEventElapsedTime(&t, e0, e1));
+    CHS(hipEventDestroy(e0));
+    CHS(hipEventDestroy(e1));
+    *ms = t;
     return INMSOM_BT_OK;
 }
 

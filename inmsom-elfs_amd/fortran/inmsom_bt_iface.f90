@@ -287,6 +287,21 @@ interface
     integer(c_int), value :: nz
     type(c_ptr), value :: z, tt, ss, hhq, den, den_pot
   end function
+  ! stress_components of the 3-D velocities (Function/mixing.f90:115-168)
+  integer(c_int) function inmsom_bt_stress_components(ctx, nz, u, v, str_t, str_s) &
+      bind(C, name="inmsom_bt_stress_components")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: nz
+    type(c_ptr), value :: u, v, str_t, str_s
+  end function
+  integer(c_int) function inmsom_bt_stress_components_dev(ctx, nz, u, v, str_t, str_s) &
+      bind(C, name="inmsom_bt_stress_components_dev")
+    import :: c_int, c_ptr
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: nz
+    type(c_ptr), value :: u, v, str_t, str_s
+  end function
   ! smagorinsky_coeff + depth_ave (Control/dyncall.f90:40-43)
   integer(c_int) function inmsom_bt_smagorinsky(ctx, nz, visc2, dift, str_t, str_s, dz, amts, amuv, amuv2d) &
       bind(C, name="inmsom_bt_smagorinsky")
@@ -462,6 +477,16 @@ subroutine density_gpu(nz, tt, ss, den, den_pot, z, hhq)
   call bt_check(inmsom_bt_density(bt_ctx, int(nz, c_int), c_loc(z), c_loc(tt), c_loc(ss), c_loc(hhq), &
                 c_loc(den), c_loc(den_pot)), 'density_gpu')
 end subroutine density_gpu
+
+! Function/mixing.f90:115-168 on the context bt_ctx for nz levels, the
+! reference's arguments (its nlev is nz); str_t, str_s are read and written.
+subroutine stress_components_gpu(u, v, str_t, str_s, nz)
+  integer, intent(in) :: nz
+  real(8), target, intent(in) :: u(*), v(*)
+  real(8), target, intent(inout) :: str_t(*), str_s(*)
+  call bt_check(inmsom_bt_stress_components(bt_ctx, int(nz, c_int), c_loc(u), c_loc(v), c_loc(str_t), &
+                c_loc(str_s)), 'stress_components_gpu')
+end subroutine stress_components_gpu
 
 ! Control/dyncall.f90:40-43 on the context bt_ctx: smagorinsky_coeff
 ! (Function/mixing.f90:171-219) with the reference's arguments, then

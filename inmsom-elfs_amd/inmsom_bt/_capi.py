@@ -107,6 +107,9 @@ SIGNATURES = {
     "inmsom_bt_smagorinsky": ([C.c_void_p, C.c_int, C.c_double, C.c_double] + [_dp] * 6, C.c_int),
     "inmsom_bt_smagorinsky_dev": ([C.c_void_p, C.c_int, C.c_double, C.c_double] + [_dp] * 6, C.c_int),
     "inmsom_bt_smagorinsky_timed": ([C.c_void_p, C.c_int, C.c_int, _dp], C.c_int),
+    "inmsom_bt_stress_components": ([C.c_void_p, C.c_int] + [_dp] * 4, C.c_int),
+    "inmsom_bt_stress_components_dev": ([C.c_void_p, C.c_int] + [_dp] * 4, C.c_int),
+    "inmsom_bt_stress_components_timed": ([C.c_void_p, C.c_int, C.c_int, _dp], C.c_int),
     "inmsom_bt_ens_create": ([C.c_int, C.POINTER(Grid), C.POINTER(Forcing), C.c_int, C.c_int,
                               C.POINTER(C.c_void_p)], C.c_int),
     "inmsom_bt_ens_destroy": ([C.c_void_p], C.c_int),

@@ -567,6 +567,37 @@ class BarotropicModel:
         check(lib().inmsom_bt_density_timed(self._h, int(nz), int(nrep), C.byref(ms)), "inmsom_bt_density_timed")
         return ms.value
 
+    def stress_components(self, u, v, str_t, str_s):
+        """Function/mixing.f90:115-168 on host arrays of 3-D velocities
+        u, v and stress components str_t, str_s (nz, ny, nx) (cells not
+        computed keep their values).  Returns the new (str_t, str_s)."""
+        uc = np.ascontiguousarray(u, dtype=np.float64)
+        nz = uc.shape[0]
+        vc = np.ascontiguousarray(v, dtype=np.float64)
+        st = np.array(str_t, dtype=np.float64, order="C")
+        ss = np.array(str_s, dtype=np.float64, order="C")
+        if uc.shape != (nz, self.ny, self.nx) or any(a.shape != uc.shape for a in (vc, st, ss)):
+            raise ValueError("u, v, str_t, str_s must be (nz, ny, nx)")
+        check(lib().inmsom_bt_stress_components(self._h, nz, _d(uc), _d(vc), _d(st), _d(ss)),
+              "inmsom_bt_stress_components")
+        return st, ss
+
+    def stress_components_dev(self, nz: int, u, v, str_t, str_s):
+        """As stress_components on DeviceArray arguments; str_t, str_s
+        updated in place."""
+        s3 = (int(nz), self.ny, self.nx)
+        check(lib().inmsom_bt_stress_components_dev(self._h, int(nz), self._dev(u, s3), self._dev(v, s3),
+                                                    self._dev(str_t, s3), self._dev(str_s, s3)),
+              "inmsom_bt_stress_components_dev")
+
+    def stress_components_timed(self, nz: int, nrep: int) -> float:
+        """HIP-event time (ms) of nrep repetitions of stress_components'
+        device work on the fields the last host-array call staged."""
+        ms = C.c_double()
+        check(lib().inmsom_bt_stress_components_timed(self._h, int(nz), int(nrep), C.byref(ms)),
+              "inmsom_bt_stress_components_timed")
+        return ms.value
+
     def smagorinsky(self, dz, str_t, str_s, amts, amuv, amuv2d, visc2: float, dift: float):
         """Control/dyncall.f90:40-43 on host arrays: smagorinsky_coeff
         (Function/mixing.f90:171-219) then depth_ave(amuv, amuv2d, lu, 0)

@@ -11,7 +11,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import subprocess
-from typing import Dict, Optional, Sequence
+from typing import Dict, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -304,6 +304,24 @@ def vertical_velocity(og: "OracleGrid", inp: Dict[str, np.ndarray]) -> np.ndarra
     return w["ww"]
 
 
+def stress_components_3d(og: "OracleGrid", u: np.ndarray, v: np.ndarray, str_t: np.ndarray,
+                         str_s: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+    """Function/mixing.f90:115-168 with nlev = nz levels (u, v, str_t, str_s
+    (nz, ny, nx)): every level is the one-level bto_stress_components (the
+    routine's loops and its wrap are per level); cells it does not write keep
+    the values of the str_t, str_s passed.  Returns new arrays."""
+    L = lib()
+    uc = np.ascontiguousarray(u, dtype=np.float64)
+    vc = np.ascontiguousarray(v, dtype=np.float64)
+    st = np.array(str_t, dtype=np.float64, order="C")
+    ss = np.array(str_s, dtype=np.float64, order="C")
+    for k in range(uc.shape[0]):
+        t, s = np.ascontiguousarray(st[k]), np.ascontiguousarray(ss[k])
+        L.bto_stress_components(og.ref, _d(uc[k]), _d(vc[k]), _d(t), _d(s))
+        st[k], ss[k] = t, s
+    return st, ss
+
+
 def smagorinsky(og: "OracleGrid", inp: Dict[str, np.ndarray], visc2: float, dift: float) -> Dict[str, np.ndarray]:
     """bto_smagorinsky on copies of ``inp`` (dz, stress_t, stress_s, amts,
     amuv (nz, ny, nx), amuv2d); returns the new amts, amuv, amuv2d."""
@@ -394,14 +412,15 @@ def run_reference(case_path: str, out_path: str, nx: int, ny: int, mode: int,
         res["pg_in"], res["pg_out"] = inp, out
         return res
     if mode == 10:
-        # masks, dz (nz), stress_t, stress_s, amts, amuv (nz planes), amuv2d,
-        # then amts, amuv (nz planes), amuv2d out
-        nz = (raw.size - 7 * per - 2 * per) // (1 + 6 * per)
+        # masks, dz (nz), uup, vvp (the velocities stress_components read),
+        # stress_t, stress_s, amts, amuv (nz planes), amuv2d, then amts, amuv
+        # (nz planes), amuv2d out
+        nz = (raw.size - 7 * per - 2 * per) // (1 + 8 * per)
         res = {"masks": {k: raw[i * per:(i + 1) * per].reshape(ny, nx) for i, k in enumerate(MASKS)}}
         o = 7 * per
         inp = {"dz": raw[o:o + nz].copy()}
         o += nz
-        for k in ("stress_t", "stress_s", "amts", "amuv"):
+        for k in ("uup", "vvp", "stress_t", "stress_s", "amts", "amuv"):
             inp[k] = raw[o:o + nz * per].reshape(nz, ny, nx).copy()
             o += nz * per
         inp["amuv2d"] = raw[o:o + per].reshape(ny, nx).copy()

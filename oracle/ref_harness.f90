@@ -67,9 +67,9 @@
 !   synthetic 3-D velocities at the compiled nz, then smagorinsky_coeff
 !   (:171-219) with lvisc_2 = 1e3, ldiff_ts = 5e2 and depth_ave(amuv, amuv2d,
 !   lu, 0) (Function/depth.f90:223-254) from initial amts, amuv, amuv2d the
-!   routines must leave where they do not write: the inputs (dz, stress_t,
-!   stress_s, amts, amuv, amuv2d) and the outputs (amts, amuv, amuv2d) are
-!   written.
+!   routines must leave where they do not write: the inputs (dz, the
+!   velocities uup, vvp, their stress_t, stress_s, amts, amuv, amuv2d) and
+!   the outputs (amts, amuv, amuv2d) are written.
 ! mode 8: the reference's grid set-up basinpar (Service/basin_parameters.f90:
 !   191-300 with grid_parameters_spherical / _carthesian, :764-949) for the
 !   compiled Inc/basinpar.fi, on zeroed metric arrays: dx, dy, dxt, dyt, dxh,
@@ -439,6 +439,7 @@ else if (mode == 10) then
   lvisc_2 = 1.0d3
   ldiff_ts = 5.0d2
   write(23) dz
+  write(23) uup, vvp
   write(23) stress_t, stress_s, amts, amuv, amuv2d
   call smagorinsky_coeff(lvisc_2, ldiff_ts, stress_t, stress_s, amts, amuv)
   call depth_ave(amuv, amuv2d, lu, 0)

@@ -89,3 +89,52 @@ def test_smagorinsky_device_arrays_feed_the_step():
         m.model_steps(steps, nstep=nstep)
         st = m.get_state()
     assert_bitwise(st, ref, ("ssh", "u", "v"), "step with the Smagorinsky mu")
+
+
+@pytest.mark.parametrize("name,make,periodicity,nz", CASES, ids=[c[0] for c in CASES])
+def test_stress_components_matches_oracle(name, make, periodicity, nz):
+    """stress_components of 3-D velocities (Function/mixing.f90:115-168) on
+    the GPU against the oracle's level-by-level restatement, bit for bit;
+    cells it does not write keep the caller's values."""
+    print(f"3-D stress {name}", flush=True)
+    b = make()
+    rng = np.random.default_rng(nz + 100)
+    s3 = (nz, b.ny, b.nx)
+    u, v = 0.1 * rng.standard_normal(s3), 0.1 * rng.standard_normal(s3)
+    st0, ss0 = rng.random(s3), rng.random(s3)
+    og = oracle.OracleGrid(b, periodicity=periodicity)
+    want_t, want_s = oracle.stress_components_3d(og, u, v, st0, ss0)
+    with BarotropicModel(b, periodicity=periodicity) as m:
+        got_t, got_s = m.stress_components(u, v, st0, ss0)
+        assert m.stress_components_timed(nz, 2) > 0.0
+    assert_bitwise({"str_t": got_t, "str_s": got_s}, {"str_t": want_t, "str_s": want_s}, ("str_t", "str_s"),
+                   f"3-D stress {name}")
+    assert np.any(got_t == st0) and np.any(got_t != st0)
+
+
+def test_viscosity_chain_on_device():
+    """Control/dyncall.f90:36-43 on HBM-resident arrays: the 3-D stress of
+    the level n-1 velocities, then the Smagorinsky viscosity from it, equal
+    the oracle's chain bit for bit (a stretched basin, 20 levels)."""
+    from inmsom_bt.hipmem import DeviceArray
+    b = make_basin(150, 110, seed=12, sigma=5.0, metrics="stretched")
+    nz = 20
+    rng = np.random.default_rng(12)
+    s3, s2 = (nz, b.ny, b.nx), (b.ny, b.nx)
+    dz = 0.5 + rng.random(nz)
+    dz = dz / dz.sum()
+    f = {"uup": 0.1 * rng.standard_normal(s3), "vvp": 0.1 * rng.standard_normal(s3), "stress_t": np.zeros(s3),
+         "stress_s": np.zeros(s3), "amts": np.zeros(s3), "amuv": np.zeros(s3), "amuv2d": np.zeros(s2)}
+    og = oracle.OracleGrid(b)
+    st, ss = oracle.stress_components_3d(og, f["uup"], f["vvp"], f["stress_t"], f["stress_s"])
+    want = oracle.smagorinsky(og, dict(f, dz=dz, stress_t=st, stress_s=ss), 1.0e3, 5.0e2)
+    with BarotropicModel(b) as m:
+        d = {k: DeviceArray.from_host(np.ascontiguousarray(f[k]), m.device) for k in f}
+        try:
+            m.stress_components_dev(nz, d["uup"], d["vvp"], d["stress_t"], d["stress_s"])
+            m.smagorinsky_dev(dz, d["stress_t"], d["stress_s"], d["amts"], d["amuv"], d["amuv2d"], 1.0e3, 5.0e2)
+            got = {k: d[k].to_host() for k in FIELDS}
+        finally:
+            for a in d.values():
+                a.free()
+    assert_bitwise(got, want, FIELDS, "stress -> Smagorinsky chain")

@@ -133,6 +133,27 @@ def test_smagorinsky_bitwise(seed, metrics):
         assert np.any(want[k] != inp[k]) and np.any(want[k] == inp[k]), k
 
 
+@pytest.mark.parametrize("seed,metrics", [(41, "uniform"), (42, "stretched")])
+def test_stress_components_3d_bitwise(seed, metrics):
+    """stress_components (Function/mixing.f90:115-168) of 3-D velocities at
+    the compiled nz, as Control/dyncall.f90:36 calls it before the
+    Smagorinsky viscosity, called by the harness (mode 10) on zeroed
+    stress arrays, against the oracle's level-by-level restatement."""
+    b = make_basin(285, 159, seed=seed, sigma=4.0, g=9.8, hmax=120.0, metrics=metrics)
+    b = randomize_state(b, seed + 1, amp_uv=0.05)
+    rd = oracle.OracleGrid(b).rayleigh()
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "case.bin")
+        write_case(path, b, rd, 1, 2, 0, 10)
+        ref = oracle.run_reference(path, path + ".out", b.nx, b.ny, 10)
+    inp = ref["smag_in"]
+    z = np.zeros_like(inp["uup"])
+    st, ss = oracle.stress_components_3d(oracle.OracleGrid(b), inp["uup"], inp["vvp"], z, z)
+    assert_bitwise({"stress_t": st, "stress_s": ss}, inp, ("stress_t", "stress_s"),
+                   f"3-D stress seed {seed} {metrics}")
+    assert np.any(st != 0) and np.any(ss != 0)
+
+
 @pytest.mark.parametrize("seed,metrics", [(21, "uniform"), (22, "stretched")])
 def test_pressure_gradients_bitwise(seed, metrics):
     """pressure_gradients (Function/vel_ssh.f90:320-376) called by the
