@@ -80,7 +80,7 @@ TRAFFIC_KERNELS = {"fp64": ("k_fused_pipe<double, 2>",),
                    "fp64_exact": ("k_fused_step<double, true, true>", "k_fused_step<double, true, false>"),
                    "ensemble_fp32": ("k_fused_batch_pipe<float>",),
                    "forcing": ("k_forcing<",),
-                   "dropin": ("k_dropin<true, true>", "k_dropin<false, true>"),
+                   "dropin": ("k_dropin<true, true, false>", "k_dropin<false, true, false>"),
                    "vertical_velocity": ("k_vertical(",),
                    "pressure_gradients": ("k_pressure(",),
                    "density": ("k_density(",),
@@ -830,8 +830,21 @@ def dropin_device_leg(m, b, calls=20, nstep=NSTEP_BAROTROP):
                                 "traffic": (lambda t: None if t is None else t * 2 * nstep)(
                                     traffic_from_profiles("dropin"))}
     out["speedup_fused_vs_literal"] = out["literal"]["call_ms"] / out["fused"]["call_ms"]
+    # the same calls with the biharmonic viscosity on (ksw4 = 1): uv_diff4 by
+    # reference-order kernels before each fused substep, or stage by stage
+    for path in ("fused", "literal"):
+        m.set_dropin_path(path)
+        n = max(calls // 2, 2) if path == "fused" else max(calls // 4, 2)
+        warm_up(lambda k: m.dropin_timed(k, nstep=nstep, ksw4=1), lambda: None, 2, LEG_WARMUP_SECONDS)
+        ms = m.dropin_timed(n, nstep=nstep, ksw4=1)
+        call_s = ms * 1e-3 / n
+        out[path + "_ksw4"] = {"call_ms": call_s * 1e3, "calls": n,
+                               "value": b.n_wet * nstep / call_s, "unit": "wet-cell steps/s (ksw4 = 1)"}
+    m.set_dropin_path("auto")
+    m.set_state(b.u, b.up, b.v, b.vp, b.ssh, b.sshp)
     log(f"drop-in device-resident, nstep={nstep}: fused {out['fused']['call_ms']:.3f} ms/call "
-        f"({achieved:.0f} GB/s), stage-by-stage {out['literal']['call_ms']:.3f} ms/call")
+        f"({achieved:.0f} GB/s), stage-by-stage {out['literal']['call_ms']:.3f} ms/call; ksw4 = 1: fused "
+        f"{out['fused_ksw4']['call_ms']:.3f}, stage-by-stage {out['literal_ksw4']['call_ms']:.3f} ms/call")
     return out
 
 

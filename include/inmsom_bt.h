@@ -96,10 +96,11 @@ int inmsom_bt_finalize(inmsom_bt_ctx *ctx);
  * the end, as hh_init does (:668-675); read them with
  * inmsom_bt_get_depths.
  * Path: one fused kernel launch per substep (bit for bit the reference)
- * when ksw4 = 0, full_free_surface = 1, the grid is not periodic and the
- * two halo cells of every side are land; otherwise one kernel per
- * reference sweep (also bit for bit).  inmsom_bt_set_dropin_path
- * chooses. */
+ * when full_free_surface = 1, the grid is not periodic and the two halo
+ * cells of every side are land (with ksw4 > 0 the biharmonic term of each
+ * substep comes from a few reference-order kernels before its launch);
+ * otherwise one kernel per reference sweep (also bit for bit).
+ * inmsom_bt_set_dropin_path chooses. */
 int inmsom_bt_barotropic_dynamics(inmsom_bt_ctx *ctx, double tau, int nstep, int ksw4,
                                   double *ubrtr_e, double *ubrtrp_e, double *vbrtr_e,
                                   double *vbrtrp_e, double *ssh_e, double *sshp_e,

@@ -381,7 +381,7 @@ static int dynamics_on_context(inmsom_bt_ctx *c, double tau, int nstep, int ksw4
     bool fused = false;
     TRY(dropin_route(c, ksw4, &fused));
     if (fused) {
-        TRY(dropin_call(c, tau, nstep, c->st, c->acc, rx, ry, wf, mu, rd));
+        TRY(dropin_call(c, tau, nstep, c->st, c->acc, rx, ry, wf, mu, rd, nullptr, ksw4, mu4));
         c->dep_stale = true;
     } else {
         // the depths a reference caller holds at entry are hh_init(ssh_e, sshp_e):
@@ -454,7 +454,7 @@ int inmsom_bt_barotropic_dynamics_dev(inmsom_bt_ctx *c, double tau, int nstep, i
     double *st[6] = {ubrtr_e, ubrtrp_e, vbrtr_e, vbrtrp_e, ssh_e, sshp_e};
     double *acc[4] = {ubrtr_i, vbrtr_i, ssh4gradx, ssh4grady};
     if (fused) {
-        TRY(dropin_call(c, tau, nstep, st, acc, RHSx, RHSy, wflux, mu, rdis));
+        TRY(dropin_call(c, tau, nstep, st, acc, RHSx, RHSy, wflux, mu, rdis, nullptr, ksw4, mu4));
     } else {
         // the stage-by-stage path runs on the context's own arrays: its
         // state is set aside, the caller's goes in and comes back by device

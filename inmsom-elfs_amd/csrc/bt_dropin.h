@@ -8,7 +8,9 @@
 namespace ibt {
 
 // Whether the fused path can take a call with this ksw4 on this context's
-// grid (see bt_dropin.hip); *why says why not.
+// grid (see bt_dropin.hip); *why says why not.  ksw4 > 0 is covered: the
+// biharmonic term comes from a pre-pass of reference-order kernels per
+// substep (lit_diff4_terms).
 bool dropin_eligible(inmsom_bt_ctx *c, int ksw4, std::string *why);
 // One call of barotropic_dynamics (Function/vel_ssh.f90:379-679) on device
 // arrays: the state (u, up, v, vp, ssh, sshp) is replaced by the state
@@ -18,7 +20,7 @@ bool dropin_eligible(inmsom_bt_ctx *c, int ksw4, std::string *why);
 // null for library-owned buffers allocated at the first such call.
 int dropin_call(inmsom_bt_ctx *c, double tau, int nstep, double *const st[6], double *const acc[4],
                 const double *rhsx, const double *rhsy, const double *wflux, const double *mu, const double *rdis,
-                double *scratch = nullptr);
+                double *scratch = nullptr, int ksw4 = 0, const double *mu4 = nullptr);
 // The grid analysis (eligibility, mask bits, row tables, launch shape),
 // done once per context at inmsom_bt_init.
 int dropin_setup(inmsom_bt_ctx *c);

@@ -25,9 +25,14 @@
 // order of an addition of two terms (fx_m(m) = fx_p(m-1), fy_m(n) =
 // fy_p(n-1)); everything else is evaluated as written.
 //
-// Eligible calls: ksw4 = 0, full_free_surface = 1, no periodicity, and land
-// on the two halo cells of every side (the reference's loop bounds then
-// coincide with its masks).  Other calls take the stage-by-stage path.
+// Eligible calls: full_free_surface = 1, no periodicity, and land on the
+// two halo cells of every side (the reference's loop bounds then coincide
+// with its masks).  Other calls take the stage-by-stage path.  With ksw4 >
+// 0 the biharmonic term uv_diff4 (vel_ssh.f90:187-317) of each substep is
+// formed before its launch by the reference-order kernels of the
+// stage-by-stage path from the substep's entering state (its two terms per
+// component, lit_diff4_terms) and added by the kernel to its uv_diff2 result
+// in the reference's order.
 // Metrics: per cell (any grid), or per row when every metric and the
 // Coriolis parameter are uniform along x (Cartesian and longitude-latitude
 // grids): then they are read from row tables instead of 8 + 1 arrays.
@@ -55,6 +60,7 @@ struct DropinParams {
     double *hupc, *hvpc;                     // filtered U/V depths carried in place (substeps > 1 read)
     bool carry_out;                          // write the filtered U/V depths for the next substep
     double *acc[4];                          // ubrtr_i, vbrtr_i, ssh4gradx, ssh4grady, in place
+    const double *d4[4];                     // ksw4 > 0: uv_diff4's terms px, qx, py, qy of the substep
     double g, rho0, ts, tau, ffs, dn, d4n;
 };
 
@@ -68,6 +74,7 @@ struct DIn {
     int mb2;
     double u1, v1, up1, vp1, sp1, rd1, mu1, wf1, dxt1, dyh1, dxh1, dyt1;
     double f0, rx0, ry0, dxb0, dyb0, hupc0, hvpc0, a0, a1, a2, a3;
+    double d4[4];  // uv_diff4's terms of row n (D4)
 };
 
 // What one substep produces for row n.
@@ -79,8 +86,10 @@ struct DOut {
 
 // One substep marching down the rows of one strip.  Suffixes: m1 = row
 // n-1, 0 = row n, p1 = row n+1, p2 = row n+2.  FIRST: the call's first
-// substep (filtered depths from hh_init, accumulators start at zero).
-template <bool FIRST>
+// substep (filtered depths from hh_init, accumulators start at zero).  D4:
+// the biharmonic viscosity is on (ksw4 > 0); its terms come precomputed
+// per cell and are added to the uv_diff2 result in the reference's order.
+template <bool FIRST, bool D4>
 struct DStage {
     const DropinParams &p;
     const bool inx, inx1;
@@ -239,9 +248,10 @@ struct DStage {
             const double fx_m = xl(fx_p);
             const double radv = -(fx_p - fx_m + fyU_0 - fyU_m1) + (Cc_0 + Cc_m1) / 4.0;
             const double muh_m = (mu_0 + xr(mu_0) + mu_m1 + xr(mu_m1)) / 4.0;
-            const double rdfx = (xr(P_0) - P_0) / dyh_0 +
-                                (dxb_0 * dxb_0 * muh_p * hh_0 * sts_0 - dxb_m1 * dxb_m1 * muh_m * hh_m1 * sts_m1) /
-                                    dxt_0;
+            double rdfx = (xr(P_0) - P_0) / dyh_0 +
+                          (dxb_0 * dxb_0 * muh_p * hh_0 * sts_0 - dxb_m1 * dxb_m1 * muh_m * hh_m1 * sts_m1) / dxt_0;
+            // uv_diff4 adds to uv_diff2's result (vel_ssh.f90:292-295)
+            if constexpr (D4) rdfx = rdfx + q.d4[0] + q.d4[1];
             const double bp = hun_0 * dxt_0 * dyh_0 / 2.0 / tau;
             const double bp0 = hupc * dxt_0 * dyh_0 / 2.0 / tau;
             const double slx = -G * (xr(s_0) - s_0) * dyh_0 * hu_0;
@@ -257,8 +267,10 @@ struct DStage {
             const double rady = -(fxV_p - fxV_m + fyV_0 - fyV_m1) - (D + xl(D)) / 4.0;
             const double muhv_m = (mu_0 + xl(mu_0) + mu_p1 + xl(mu_p1)) / 4.0;
             const double dybl = xl(dyb_0), hhl = xl(hh_0), stsl = xl(sts_0);
-            const double rdfy = -(Px_p1 - Px_0) / dxh_0 +
-                                (dyb_0 * dyb_0 * muh_p * hh_0 * sts_0 - dybl * dybl * muhv_m * hhl * stsl) / dyt_0;
+            double rdfy = -(Px_p1 - Px_0) / dxh_0 +
+                          (dyb_0 * dyb_0 * muh_p * hh_0 * sts_0 - dybl * dybl * muhv_m * hhl * stsl) / dyt_0;
+            // (vel_ssh.f90:306-309)
+            if constexpr (D4) rdfy = rdfy - q.d4[2] + q.d4[3];
             const double bq = hvn_0 * dyt_0 * dxh_0 / 2.0 / tau;
             const double bq0 = hvpc * dyt_0 * dxh_0 / 2.0 / tau;
             const double sly = -G * (s_p1 - s_0) * dxh_0 * hv_0;
@@ -361,7 +373,7 @@ struct DStage {
 };
 
 // Loads of one iteration, issued one iteration ahead of their use.
-template <bool FIRST, bool ROWMET>
+template <bool FIRST, bool ROWMET, bool D4>
 struct DLoader {
     const DropinParams &p;
     const RowAddr ra;
@@ -369,6 +381,7 @@ struct DLoader {
     struct Pre {
         double s, H, dx, dy, u, v, up, vp, sp, rd, mu, wf, dxt, dyh, dxh, dyt, f, rx, ry, dxb, dyb, hupc, hvpc;
         double a[4];
+        double d4[D4 ? 4 : 1];
         int mb;
     } q;
 
@@ -410,6 +423,8 @@ struct DLoader {
         q.f = met(8, n, i0);
         q.rx = p.rx[i0];
         q.ry = p.ry[i0];
+        if constexpr (D4)
+            for (int k = 0; k < 4; ++k) q.d4[k] = p.d4[k][i0];
         if constexpr (!FIRST) {
             q.hupc = p.hupc[i0];
             q.hvpc = p.hvpc[i0];
@@ -441,6 +456,8 @@ struct DLoader {
         in.ry0 = r0 ? q.ry : 0.0;
         in.dxb0 = r0 ? q.dxb : 0.0;
         in.dyb0 = r0 ? q.dyb : 0.0;
+        if constexpr (D4)
+            for (int k = 0; k < 4; ++k) in.d4[k] = q.d4[k];
         if constexpr (!FIRST) {
             in.hupc0 = q.hupc;
             in.hvpc0 = q.hvpc;
@@ -457,7 +474,7 @@ struct DLoader {
 // Two waves per SIMD (the later substeps would take 256 VGPRs + 4 AGPRs,
 // one wave; bounded they spill 12 bytes per lane): at 2048 x 1536 on
 // MI355X a call of 4 launches takes 0.782 ms instead of 0.867.
-template <bool FIRST, bool ROWMET>
+template <bool FIRST, bool ROWMET, bool D4>
 __global__ void __launch_bounds__(WAVE, 2) k_dropin(DropinParams p)
 {
     const int w = blockIdx.x;
@@ -471,8 +488,8 @@ __global__ void __launch_bounds__(WAVE, 2) k_dropin(DropinParams p)
     const int r0 = chunk_id * p.chunk + 1;
     const int r1 = min(r0 + p.chunk, p.ny + 1);
 
-    DStage<FIRST> st(p, m);
-    DLoader<FIRST, ROWMET> ld(p, m);
+    DStage<FIRST, D4> st(p, m);
+    DLoader<FIRST, ROWMET, D4> ld(p, m);
     DIn in;
     DOut o;
     ld.issue(r0 - 3);
@@ -526,6 +543,7 @@ struct DropinAlloc {
     bool rowmet_ok = false;
     int chunk = 0;
     long long slots = 0;
+    double *b4[12] = {};  // work planes of the biharmonic pre-pass (ksw4 > 0)
 };
 
 namespace {
@@ -619,7 +637,7 @@ int check_grid(inmsom_bt_ctx *c, DropinAlloc *a)
     int dev = 0, cus = 0, per_cu = 0;
     DHC(hipGetDevice(&dev));
     DHC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const void *kf = rows ? (const void *)k_dropin<false, true> : (const void *)k_dropin<false, false>;
+    const void *kf = rows ? (const void *)k_dropin<false, true, false> : (const void *)k_dropin<false, false, false>;
     DHC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kf, WAVE, 0));
     a->slots = (long long)cus * (per_cu > 0 ? per_cu : 1);
     const int strips = (nx + SW - 1) / SW;
@@ -642,11 +660,24 @@ int ensure_buffers(inmsom_bt_ctx *c, DropinAlloc *a)
     return INMSOM_BT_OK;
 }
 
+// the work planes of the biharmonic pre-pass (lit_diff4_terms), zeroed once
+int ensure_diff4(inmsom_bt_ctx *c, DropinAlloc *a)
+{
+    for (double *&q : a->b4)
+        if (!q) {
+            DHC(hipMalloc(&q, c->n * 8));
+            DHC(hipMemsetAsync(q, 0, c->n * 8, c->stream));
+        }
+    return INMSOM_BT_OK;
+}
+
 template <bool FIRST>
 void launch_dropin(inmsom_bt_ctx *c, const DropinParams &p, bool rowmet)
 {
     dim3 grid((unsigned)(p.per_xcd * NXCD)), blk(WAVE);
-    const void *k = rowmet ? (const void *)k_dropin<FIRST, true> : (const void *)k_dropin<FIRST, false>;
+    const bool d4 = p.d4[0] != nullptr;
+    const void *k = rowmet ? (d4 ? (const void *)k_dropin<FIRST, true, true> : (const void *)k_dropin<FIRST, true, false>)
+                           : (d4 ? (const void *)k_dropin<FIRST, false, true> : (const void *)k_dropin<FIRST, false, false>);
     void *args[] = {const_cast<DropinParams *>(&p)};
     (void)hipLaunchKernel(k, grid, blk, args, 0, c->stream);
 }
@@ -670,10 +701,7 @@ bool dropin_eligible(inmsom_bt_ctx *c, int ksw4, std::string *why)
         if (why) *why = a->why;
         return false;
     }
-    if (ksw4 > 0) {
-        if (why) *why = "ksw4 > 0 (biharmonic viscosity)";
-        return false;
-    }
+    (void)ksw4;  // the biharmonic term is covered by a pre-pass per substep
     return true;
 }
 
@@ -683,7 +711,7 @@ size_t dropin_scratch_doubles(const inmsom_bt_ctx *c) { return 14 * scratch_plan
 
 int dropin_call(inmsom_bt_ctx *c, double tau, int nstep, double *const st[6], double *const acc[4],
                 const double *rhsx, const double *rhsy, const double *wflux, const double *mu, const double *rdis,
-                double *scratch)
+                double *scratch, int ksw4, const double *mu4)
 {
     DropinAlloc *a = dalloc(c);
     int rc = check_grid(c, a);
@@ -734,6 +762,13 @@ int dropin_call(inmsom_bt_ctx *c, double tau, int nstep, double *const st[6], do
     p.ffs = (double)c->g.ffs;
     p.dn = (double)nstep;
     p.d4n = (double)(4 * nstep);
+    if (ksw4 > 0) {
+        if (!mu4) {
+            set_error("mu4 is required when ksw4 > 0");
+            return INMSOM_BT_EINVAL;
+        }
+        if ((rc = ensure_diff4(c, a))) return rc;
+    }
     const int last = 2 * nstep;
     for (int k = 1; k <= last; ++k) {
         // substep k reads the caller's state (k = 1) or the buffer substep
@@ -755,6 +790,12 @@ int dropin_call(inmsom_bt_ctx *c, double tau, int nstep, double *const st[6], do
         p.spo = wr ? out[5] : nullptr;
         // substep nstep + 1 enters with the state after substep nstep: the
         // call's result (vel_ssh.f90:638-648)
+        // uv_diff4 of this substep from its entering state (level-n surface,
+        // filtered velocities), added by the kernel to its uv_diff2 result
+        if (ksw4 > 0) {
+            lit_diff4_terms(c, in[1], in[3], in[4], mu4, a->b4);
+            for (int q = 0; q < 4; ++q) p.d4[q] = a->b4[8 + q];
+        }
         const bool snap = k == nstep + 1;
         p.cu = snap ? st[0] : nullptr;
         p.cup = snap ? st[1] : nullptr;
@@ -821,6 +862,8 @@ void dropin_release(inmsom_bt_ctx *c)
         for (double *q : b)
             if (q) (void)hipFree(q);
     for (void *q : {(void *)a->hupc, (void *)a->hvpc, (void *)a->rowmet, (void *)a->mb})
+        if (q) (void)hipFree(q);
+    for (double *q : a->b4)
         if (q) (void)hipFree(q);
     delete a;
     c->da = nullptr;

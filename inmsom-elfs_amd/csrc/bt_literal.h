@@ -19,5 +19,13 @@ void lit_forcing(inmsom_bt_ctx *c);
 // (Function/depth.f90:108-170) of (hqn, hun, hvn, hhn) from the surface sh
 void lit_cyclize(inmsom_bt_ctx *c, double *f);
 void lit_hh_update(inmsom_bt_ctx *c, const double *sh, double *hqn, double *hun, double *hvn, double *hhn);
+// uv_diff4 (Function/vel_ssh.f90:187-317) of a substep with level-n surface
+// sh and filtered velocities up, vp, as its two terms per component:
+// t[8..11] = px, qx, py, qy on lcu / lcv interior points, which the caller
+// adds to its uv_diff2 result as RHSx_dif + px + qx, RHSy_dif - py + qy.
+// t[0..7] are work planes (level-n depths hq, hu, hv, hh, stress, fx, fy);
+// all twelve are n doubles, zero where never written.
+void lit_diff4_terms(inmsom_bt_ctx *c, const double *up, const double *vp, const double *sh, const double *mu4,
+                     double *const t[12]);
 
 }  // namespace ibt

@@ -348,6 +348,30 @@ __global__ void k_diff4_add(DevGrid g, const double *__restrict__ mu, const doub
     }
 }
 
+// last sweep of uv_diff4 as its two terms per component, for a caller that
+// adds them itself in the reference's order: RHSx_dif + px + qx and
+// RHSy_dif - py + qy (Function/vel_ssh.f90:274-312)
+__global__ void k_diff4_terms(DevGrid g, const double *__restrict__ mu, const double *__restrict__ str_t,
+                              const double *__restrict__ str_s, const double *__restrict__ hq,
+                              const double *__restrict__ hh, double *__restrict__ px, double *__restrict__ qx,
+                              double *__restrict__ py, double *__restrict__ qy)
+{
+    KERNEL_MN
+    if (!IN_INTERIOR(g)) return;
+    const size_t i = I(m, n);
+    double p, q;
+    if (g.lcu[i] > 0.5f) {
+        visc_u(g, mu, str_t, str_s, hq, hh, m, n, p, q);
+        px[i] = p;
+        qx[i] = q;
+    }
+    if (g.lcv[i] > 0.5f) {
+        visc_v(g, mu, str_t, str_s, hq, hh, m, n, p, q);
+        py[i] = p;
+        qy[i] = q;
+    }
+}
+
 // momentum with Coriolis, pressure gradient, Rayleigh friction and the
 // time-mean accumulators (Function/vel_ssh.f90:531-573)
 struct MomArgs {
@@ -490,6 +514,28 @@ void lit_hh_update(inmsom_bt_ctx *c, const double *sh, double *hqn, double *hun,
     cyclize(c, hun);
     cyclize(c, hvn);
     cyclize(c, hhn);
+}
+
+void lit_diff4_terms(inmsom_bt_ctx *c, const double *up, const double *vp, const double *sh, const double *mu4,
+                     double *const t[12])
+{
+    const DevGrid &g = c->g;
+    double *hq = t[0], *hu = t[1], *hv = t[2], *hh = t[3], *st = t[4], *ss = t[5], *fx = t[6], *fy = t[7];
+    // the substep's level-n depths, then uv_diff4 as the stage-by-stage path runs it
+    lit_hh_update(c, sh, hq, hu, hv, hh);
+    hipLaunchKernelGGL(k_stress, grid2(g), BLK2, 0, c->stream, g, up, vp, st, ss);
+    cyclize(c, st);
+    cyclize(c, ss);
+    fill(c, fx, 0.0);
+    fill(c, fy, 0.0);
+    hipLaunchKernelGGL(k_diff4_f, grid2(g), BLK2, 0, c->stream, g, mu4, st, ss, hq, hu, hv, hh, fx, fy);
+    cyclize(c, fx);
+    cyclize(c, fy);
+    hipLaunchKernelGGL(k_stress, grid2(g), BLK2, 0, c->stream, g, fx, fy, st, ss);
+    cyclize(c, st);
+    cyclize(c, ss);
+    hipLaunchKernelGGL(k_diff4_terms, grid2(g), BLK2, 0, c->stream, g, mu4, st, ss, hq, hh, t[8], t[9], t[10],
+                       t[11]);
 }
 
 // Device array of the 12 depth pointers (k_hh_init and k_shift index

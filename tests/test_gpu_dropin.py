@@ -3,9 +3,9 @@ substeps and averages included) behind the reference's own boundary, bit
 for bit against the oracle and against the stage-by-stage path: uniform,
 stretched (every metric a different 2-D field) and longitude-latitude
 metrics, random states, water flux and caller RHS, nstep 1..3, chained
-calls, the device-address entry, the full 2048 x 1536 basin, and the rule
-that sends other calls (biharmonic, periodic, open halo) down the
-stage-by-stage path."""
+calls, the biharmonic viscosity (ksw4 = 1), the device-address entry, the
+full 2048 x 1536 basin, and the rule that sends other calls (periodic,
+open halo) down the stage-by-stage path."""
 import os
 
 import numpy as np
@@ -49,49 +49,51 @@ def _inputs(b, seed):
             "mu4": np.ascontiguousarray(b.mu4)}
 
 
-def _call_model(m, b, nstep, st, acc, inp, rdis):
-    m.barotropic_dynamics(b.dt, nstep, 0, st["u"], st["up"], st["v"], st["vp"], st["ssh"], st["sshp"],
+def _call_model(m, b, nstep, st, acc, inp, rdis, ksw4=0):
+    m.barotropic_dynamics(b.dt, nstep, ksw4, st["u"], st["up"], st["v"], st["vp"], st["ssh"], st["sshp"],
                           acc["ubrtr_i"], acc["vbrtr_i"], acc["ssh4gradx"], acc["ssh4grady"], inp["RHSx"],
                           inp["RHSy"], inp["wflux"], inp["mu"], inp["mu4"], rdis)
 
 
-@pytest.mark.parametrize("kind,nstep,calls", [("uniform", 2, 3), ("stretched", 3, 2), ("spherical", 2, 2),
-                                              ("ragged", 1, 3), ("wide", 2, 1)])
-def test_fused_dropin_matches_oracle(kind, nstep, calls):
+@pytest.mark.parametrize("kind,nstep,calls,ksw4", [("uniform", 2, 3, 0), ("stretched", 3, 2, 0), ("spherical", 2, 2, 0),
+                                                   ("ragged", 1, 3, 0), ("wide", 2, 1, 0), ("uniform", 2, 2, 1),
+                                                   ("stretched", 3, 2, 1), ("spherical", 2, 2, 1),
+                                                   ("ragged", 1, 2, 1)])
+def test_fused_dropin_matches_oracle(kind, nstep, calls, ksw4):
     b = _basin(kind)
     og = oracle.OracleGrid(b)
     rdis = og.rayleigh()
     st_o = {k: np.ascontiguousarray(getattr(b, k)).copy() for k in ("u", "up", "v", "vp", "ssh", "sshp")}
     st_g = {k: v.copy() for k, v in st_o.items()}
     with BarotropicModel(b, rdis=rdis) as m:
-        assert m.dropin_path(0) == "fused", kind
+        assert m.dropin_path(ksw4) == "fused", kind
         for call in range(calls):
-            print(f"fused drop-in {kind} {b.nx}x{b.ny} nstep={nstep} call {call}", flush=True)
+            print(f"fused drop-in {kind} {b.nx}x{b.ny} nstep={nstep} ksw4={ksw4} call {call}", flush=True)
             inp = _inputs(b, 100 + call)
             acc_o = {k: np.zeros((b.ny, b.nx)) for k in ACC}
             acc_g = {k: np.full((b.ny, b.nx), np.nan) for k in ACC}  # outputs only
-            dep_o = oracle.barotropic_dynamics(og, b.dt, nstep, 0, st_o, acc_o, inp["RHSx"], inp["RHSy"],
+            dep_o = oracle.barotropic_dynamics(og, b.dt, nstep, ksw4, st_o, acc_o, inp["RHSx"], inp["RHSy"],
                                                inp["wflux"], inp["mu"], inp["mu4"], rdis)
-            _call_model(m, b, nstep, st_g, acc_g, inp, rdis)
+            _call_model(m, b, nstep, st_g, acc_g, inp, rdis, ksw4)
             got = {**st_g, **acc_g, **m.depths()}
             want = {**st_o, **acc_o, **dep_o}
-            assert_bitwise(got, want, STATE + ACC + DEPTHS, f"{kind} call {call}")
+            assert_bitwise(got, want, STATE + ACC + DEPTHS, f"{kind} ksw4={ksw4} call {call}")
 
 
-@pytest.mark.parametrize("nstep", [1, 2, 3, 4])
-def test_fused_equals_stage_by_stage(nstep):
+@pytest.mark.parametrize("nstep,ksw4", [(1, 0), (2, 0), (3, 0), (4, 0), (1, 1), (3, 1)])
+def test_fused_equals_stage_by_stage(nstep, ksw4):
     """Five outer steps of the 2-D model (wind stress and drag formed from
     each step's state, then barotropic_dynamics) through either path."""
     b = randomize_state(single_basin("cfg0"), 4, wflux_amp=1e-6)
     out = {}
     for path in ("literal", "fused"):
-        print(f"256x256 model steps, nstep={nstep}, {path}", flush=True)
+        print(f"256x256 model steps, nstep={nstep}, ksw4={ksw4}, {path}", flush=True)
         with BarotropicModel(b) as m:
             m.set_dropin_path(path)
-            assert m.dropin_path(0) == path
-            m.model_steps(5, nstep=nstep)
+            assert m.dropin_path(ksw4) == path
+            m.model_steps(5, nstep=nstep, ksw4=ksw4)
             out[path] = {**m.get_state(), **m.means(), **m.depths()}
-    assert_bitwise(out["fused"], out["literal"], STATE + ACC + DEPTHS, f"nstep={nstep}")
+    assert_bitwise(out["fused"], out["literal"], STATE + ACC + DEPTHS, f"nstep={nstep} ksw4={ksw4}")
 
 
 @pytest.mark.parametrize("path", ["fused", "literal"])
@@ -146,15 +148,16 @@ def test_routing():
     b = make_basin(40, 30, seed=5, sigma=3.0)
     with BarotropicModel(b) as m:
         assert m.dropin_path(0) == "fused"
-        assert m.dropin_path(1) == "literal"  # biharmonic viscosity
-        m.set_dropin_path("fused")
-        with pytest.raises(BtError, match="ksw4"):
-            m.model_steps(1, nstep=2, ksw4=1)
+        assert m.dropin_path(1) == "fused"  # biharmonic viscosity: pre-pass per substep
         m.set_dropin_path("literal")
         assert m.dropin_path(0) == "literal"
     pb, _, _ = periodic_pair((1, 0))
     with BarotropicModel(pb, periodicity=(1, 0)) as m:
         assert m.dropin_path(0) == "literal"
+        assert m.dropin_path(1) == "literal"
+        m.set_dropin_path("fused")
+        with pytest.raises(BtError, match="periodic"):
+            m.model_steps(1, nstep=2)
     ob = make_basin(40, 30, seed=6, sigma=3.0)
     ob.lbasins[:, :] = 0  # ocean on the halo: the loop bounds differ from the masks
     with BarotropicModel(ob) as m:
