@@ -98,7 +98,7 @@ int inmsom_bt_finalize(inmsom_bt_ctx *ctx);
  * Path: one fused kernel launch per substep (bit for bit the reference)
  * when full_free_surface = 1, the grid is not periodic and the two halo
  * cells of every side are land (with ksw4 > 0 the biharmonic term of each
- * substep comes from a few reference-order kernels before its launch);
+ * substep comes from two reference-order sweeps before its launch);
  * otherwise one kernel per reference sweep (also bit for bit).
  * inmsom_bt_set_dropin_path chooses. */
 int inmsom_bt_barotropic_dynamics(inmsom_bt_ctx *ctx, double tau, int nstep, int ksw4,

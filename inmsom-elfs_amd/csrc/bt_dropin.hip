@@ -543,7 +543,7 @@ struct DropinAlloc {
     bool rowmet_ok = false;
     int chunk = 0;
     long long slots = 0;
-    double *b4[12] = {};  // work planes of the biharmonic pre-pass (ksw4 > 0)
+    double *b4[6] = {};  // fx, fy, px, qx, py, qy of the biharmonic pre-pass (ksw4 > 0)
 };
 
 namespace {
@@ -794,7 +794,7 @@ int dropin_call(inmsom_bt_ctx *c, double tau, int nstep, double *const st[6], do
         // filtered velocities), added by the kernel to its uv_diff2 result
         if (ksw4 > 0) {
             lit_diff4_terms(c, in[1], in[3], in[4], mu4, a->b4);
-            for (int q = 0; q < 4; ++q) p.d4[q] = a->b4[8 + q];
+            for (int q = 0; q < 4; ++q) p.d4[q] = a->b4[2 + q];
         }
         const bool snap = k == nstep + 1;
         p.cu = snap ? st[0] : nullptr;

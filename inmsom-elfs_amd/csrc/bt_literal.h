@@ -20,12 +20,12 @@ void lit_forcing(inmsom_bt_ctx *c);
 void lit_cyclize(inmsom_bt_ctx *c, double *f);
 void lit_hh_update(inmsom_bt_ctx *c, const double *sh, double *hqn, double *hun, double *hvn, double *hhn);
 // uv_diff4 (Function/vel_ssh.f90:187-317) of a substep with level-n surface
-// sh and filtered velocities up, vp, as its two terms per component:
-// t[8..11] = px, qx, py, qy on lcu / lcv interior points, which the caller
-// adds to its uv_diff2 result as RHSx_dif + px + qx, RHSy_dif - py + qy.
-// t[0..7] are work planes (level-n depths hq, hu, hv, hh, stress, fx, fy);
-// all twelve are n doubles, zero where never written.
+// sh and filtered velocities up, vp on a grid that is not periodic, as its
+// two terms per component: t[2..5] = px, qx, py, qy on lcu / lcv interior
+// points (0 elsewhere), which the caller adds to its uv_diff2 result as
+// RHSx_dif + px + qx, RHSy_dif - py + qy.  t[0], t[1] receive the first
+// sweep's fx, fy.  Two launches; all six planes are n doubles.
 void lit_diff4_terms(inmsom_bt_ctx *c, const double *up, const double *vp, const double *sh, const double *mu4,
-                     double *const t[12]);
+                     double *const t[6]);
 
 }  // namespace ibt

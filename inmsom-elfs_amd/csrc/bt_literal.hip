@@ -24,6 +24,13 @@ namespace ibt {
 #define I(mm, nn) IDX(g, (mm), (nn))
 
 // ---------------------------------------------------------------- helpers
+// a plane read at (a, b)
+struct Plane {
+    const DevGrid *g;
+    const double *f;
+    __device__ double operator()(int a, int b) const { return f[IDX(*g, a, b)]; }
+};
+
 __global__ void k_fill(double *a, size_t n, double v)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -196,19 +203,33 @@ __global__ void k_continuity(DevGrid g, double tau_inner, const double *__restri
                            (g.dx[i] * g.dy[i]));
 }
 
-// stress_components (Function/mixing.f90:115-168)
+// stress_components (Function/mixing.f90:115-168) at (m, n) of the
+// velocity pair given by accessors, for a point of the interior: tension on
+// lu points, shear on luu points.
+template <class A, class B>
+__device__ inline double tension_at(const DevGrid &g, A u, B v, int m, int n)
+{
+    const size_t i = I(m, n);
+    return g.dy[i] / g.dx[i] * (u(m, n) / g.dyh[i] - u(m - 1, n) / g.dyh[I(m - 1, n)]) -
+           g.dx[i] / g.dy[i] * (v(m, n) / g.dxh[i] - v(m, n - 1) / g.dxh[I(m, n - 1)]);
+}
+
+template <class A, class B>
+__device__ inline double shear_at(const DevGrid &g, A u, B v, int m, int n)
+{
+    const size_t i = I(m, n);
+    return g.dxb[i] / g.dyb[i] * (u(m, n + 1) / g.dxt[I(m, n + 1)] - u(m, n) / g.dxt[i]) +
+           g.dyb[i] / g.dxb[i] * (v(m + 1, n) / g.dyt[I(m + 1, n)] - v(m, n) / g.dyt[i]);
+}
+
 __global__ void k_stress(DevGrid g, const double *__restrict__ u, const double *__restrict__ v,
                          double *__restrict__ str_t, double *__restrict__ str_s)
 {
     KERNEL_MN
     if (!IN_INTERIOR(g)) return;
     const size_t i = I(m, n);
-    if (g.lu[i] > 0.5f)
-        str_t[i] = g.dy[i] / g.dx[i] * (u[i] / g.dyh[i] - u[I(m - 1, n)] / g.dyh[I(m - 1, n)]) -
-                   g.dx[i] / g.dy[i] * (v[i] / g.dxh[i] - v[I(m, n - 1)] / g.dxh[I(m, n - 1)]);
-    if (g.luu[i] > 0.5f)
-        str_s[i] = g.dxb[i] / g.dyb[i] * (u[I(m, n + 1)] / g.dxt[I(m, n + 1)] - u[i] / g.dxt[i]) +
-                   g.dyb[i] / g.dxb[i] * (v[I(m + 1, n)] / g.dyt[I(m + 1, n)] - v[i] / g.dyt[i]);
+    if (g.lu[i] > 0.5f) str_t[i] = tension_at(g, Plane{&g, u}, Plane{&g, v}, m, n);
+    if (g.luu[i] > 0.5f) str_s[i] = shear_at(g, Plane{&g, u}, Plane{&g, v}, m, n);
 }
 
 // metric vorticity of uv_trans (Function/vel_ssh.f90:30-42)
@@ -225,33 +246,48 @@ __global__ void k_vort(DevGrid g, const double *__restrict__ u, const double *__
 }
 
 // Tension part p and shear part q of the viscous stencil,
-// Function/vel_ssh.f90:150-156 (u) and :166-172 (v).
-__device__ inline void visc_u(const DevGrid &g, const double *mu, const double *str_t,
-                              const double *str_s, const double *hq, const double *hh, int m, int n,
-                              double &p, double &q)
+// Function/vel_ssh.f90:150-156 (u) and :166-172 (v); the stress, depths
+// at T and H points come from accessors (a, b) -> value.
+template <class ST, class SS, class HQ, class HH>
+__device__ inline void visc_u_at(const DevGrid &g, const double *mu, ST str_t, SS str_s, HQ hq, HH hh, int m,
+                                 int n, double &p, double &q)
 {
     double muh_p = (mu[I(m, n)] + mu[I(m + 1, n)] + mu[I(m, n + 1)] + mu[I(m + 1, n + 1)]) / 4.0;
     double muh_m = (mu[I(m, n)] + mu[I(m + 1, n)] + mu[I(m, n - 1)] + mu[I(m + 1, n - 1)]) / 4.0;
     double dy1 = g.dy[I(m + 1, n)], dy0 = g.dy[I(m, n)];
     double b1 = g.dxb[I(m, n)], b0 = g.dxb[I(m, n - 1)];
-    p = (dy1 * dy1 * mu[I(m + 1, n)] * hq[I(m + 1, n)] * str_t[I(m + 1, n)] -
-         dy0 * dy0 * mu[I(m, n)] * hq[I(m, n)] * str_t[I(m, n)]) / g.dyh[I(m, n)];
-    q = (b1 * b1 * muh_p * hh[I(m, n)] * str_s[I(m, n)] -
-         b0 * b0 * muh_m * hh[I(m, n - 1)] * str_s[I(m, n - 1)]) / g.dxt[I(m, n)];
+    p = (dy1 * dy1 * mu[I(m + 1, n)] * hq(m + 1, n) * str_t(m + 1, n) -
+         dy0 * dy0 * mu[I(m, n)] * hq(m, n) * str_t(m, n)) / g.dyh[I(m, n)];
+    q = (b1 * b1 * muh_p * hh(m, n) * str_s(m, n) -
+         b0 * b0 * muh_m * hh(m, n - 1) * str_s(m, n - 1)) / g.dxt[I(m, n)];
+}
+
+template <class ST, class SS, class HQ, class HH>
+__device__ inline void visc_v_at(const DevGrid &g, const double *mu, ST str_t, SS str_s, HQ hq, HH hh, int m,
+                                 int n, double &p, double &q)
+{
+    double muh_p = (mu[I(m, n)] + mu[I(m + 1, n)] + mu[I(m, n + 1)] + mu[I(m + 1, n + 1)]) / 4.0;
+    double muh_m = (mu[I(m, n)] + mu[I(m - 1, n)] + mu[I(m, n + 1)] + mu[I(m - 1, n + 1)]) / 4.0;
+    double dx1 = g.dx[I(m, n + 1)], dx0 = g.dx[I(m, n)];
+    double b1 = g.dyb[I(m, n)], b0 = g.dyb[I(m - 1, n)];
+    p = (dx1 * dx1 * mu[I(m, n + 1)] * hq(m, n + 1) * str_t(m, n + 1) -
+         dx0 * dx0 * mu[I(m, n)] * hq(m, n) * str_t(m, n)) / g.dxh[I(m, n)];
+    q = (b1 * b1 * muh_p * hh(m, n) * str_s(m, n) -
+         b0 * b0 * muh_m * hh(m - 1, n) * str_s(m - 1, n)) / g.dyt[I(m, n)];
+}
+
+__device__ inline void visc_u(const DevGrid &g, const double *mu, const double *str_t,
+                              const double *str_s, const double *hq, const double *hh, int m, int n,
+                              double &p, double &q)
+{
+    visc_u_at(g, mu, Plane{&g, str_t}, Plane{&g, str_s}, Plane{&g, hq}, Plane{&g, hh}, m, n, p, q);
 }
 
 __device__ inline void visc_v(const DevGrid &g, const double *mu, const double *str_t,
                               const double *str_s, const double *hq, const double *hh, int m, int n,
                               double &p, double &q)
 {
-    double muh_p = (mu[I(m, n)] + mu[I(m + 1, n)] + mu[I(m, n + 1)] + mu[I(m + 1, n + 1)]) / 4.0;
-    double muh_m = (mu[I(m, n)] + mu[I(m - 1, n)] + mu[I(m, n + 1)] + mu[I(m - 1, n + 1)]) / 4.0;
-    double dx1 = g.dx[I(m, n + 1)], dx0 = g.dx[I(m, n)];
-    double b1 = g.dyb[I(m, n)], b0 = g.dyb[I(m - 1, n)];
-    p = (dx1 * dx1 * mu[I(m, n + 1)] * hq[I(m, n + 1)] * str_t[I(m, n + 1)] -
-         dx0 * dx0 * mu[I(m, n)] * hq[I(m, n)] * str_t[I(m, n)]) / g.dxh[I(m, n)];
-    q = (b1 * b1 * muh_p * hh[I(m, n)] * str_s[I(m, n)] -
-         b0 * b0 * muh_m * hh[I(m - 1, n)] * str_s[I(m - 1, n)]) / g.dyt[I(m, n)];
+    visc_v_at(g, mu, Plane{&g, str_t}, Plane{&g, str_s}, Plane{&g, hq}, Plane{&g, hh}, m, n, p, q);
 }
 
 // flux-form advection of uv_trans (Function/vel_ssh.f90:54-109) and the
@@ -348,27 +384,129 @@ __global__ void k_diff4_add(DevGrid g, const double *__restrict__ mu, const doub
     }
 }
 
-// last sweep of uv_diff4 as its two terms per component, for a caller that
-// adds them itself in the reference's order: RHSx_dif + px + qx and
-// RHSy_dif - py + qy (Function/vel_ssh.f90:274-312)
-__global__ void k_diff4_terms(DevGrid g, const double *__restrict__ mu, const double *__restrict__ str_t,
-                              const double *__restrict__ str_s, const double *__restrict__ hq,
-                              const double *__restrict__ hh, double *__restrict__ px, double *__restrict__ qx,
-                              double *__restrict__ py, double *__restrict__ qy)
-{
-    KERNEL_MN
-    if (!IN_INTERIOR(g)) return;
-    const size_t i = I(m, n);
-    double p, q;
-    if (g.lcu[i] > 0.5f) {
-        visc_u(g, mu, str_t, str_s, hq, hh, m, n, p, q);
-        px[i] = p;
-        qx[i] = q;
+// uv_diff4 (Function/vel_ssh.f90:203-312) in two sweeps for a caller that
+// adds its terms itself in the reference's order (RHSx_dif + px + qx and
+// RHSy_dif - py + qy), on a grid that is not periodic.  Each sweep forms
+// what the stage-by-stage path keeps in planes (level-n depths of hh_update,
+// Function/depth.f90:108-170; the stresses of stress_components) at the
+// points its stencil reads, with the same expressions, and the value 0 where
+// the reference does not compute them (outside their index range or mask),
+// as the zero-initialised planes hold there.
+struct D4Depth {
+    const DevGrid *g;
+    const double *sh;
+    // hq = H + ssh at a T point
+    __device__ double hq(int a, int b) const
+    {
+        const size_t j = IDX(*g, a, b);
+        return g->hr[j] + sh[j];
     }
-    if (g.lcv[i] > 0.5f) {
-        visc_v(g, mu, str_t, str_s, hq, hh, m, n, p, q);
-        py[i] = p;
-        qy[i] = q;
+    __device__ bool in_range(int a, int b) const { return a >= 2 && a <= g->nx - 2 && b >= 2 && b <= g->ny - 2; }
+    __device__ double hu(int a, int b) const
+    {
+        const DevGrid &G = *g;
+        if (!in_range(a, b) || !(G.llu[IDX(G, a, b)] > 0.5f)) return 0.0;
+        return avg_u(G, [this](int x, int y) { return hq(x, y); }, a, b);
+    }
+    __device__ double hv(int a, int b) const
+    {
+        const DevGrid &G = *g;
+        if (!in_range(a, b) || !(G.llv[IDX(G, a, b)] > 0.5f)) return 0.0;
+        return avg_v(G, [this](int x, int y) { return hq(x, y); }, a, b);
+    }
+    __device__ double hh(int a, int b) const
+    {
+        const DevGrid &G = *g;
+        if (!in_range(a, b) || !(G.luh[IDX(G, a, b)] > 0.5f)) return 0.0;
+        return avg_h(G, [this](int x, int y) { return hq(x, y); }, a, b);
+    }
+};
+
+template <class A, class B>
+struct D4Stress {
+    const DevGrid *g;
+    A u;
+    B v;
+    __device__ bool interior(int a, int b) const { return a >= 3 && a <= g->nx - 2 && b >= 3 && b <= g->ny - 2; }
+    __device__ double t(int a, int b) const
+    {
+        if (!interior(a, b) || !(g->lu[IDX(*g, a, b)] > 0.5f)) return 0.0;
+        return tension_at(*g, u, v, a, b);
+    }
+    __device__ double s(int a, int b) const
+    {
+        if (!interior(a, b) || !(g->luu[IDX(*g, a, b)] > 0.5f)) return 0.0;
+        return shear_at(*g, u, v, a, b);
+    }
+};
+
+// One sweep per launch over tiles of D4_TX x D4_TY points (one thread per
+// column, D4_TY / D4_ROWS rows of threads): the tile first forms, once per
+// point, the tension on [m0, m0 + D4_TX] x [n0, n0 + D4_TY] and the shear
+// and H-point depths on [m0 - 1, m0 + D4_TX - 1] x [n0 - 1, n0 + D4_TY - 1]
+// in LDS, which is every point the viscous stencil of its own points reads.
+// FIRST: the sweep of :216-257 from the stress of (a, b) = (up, vp), giving
+// o0, o1 = fx, fy; else the sweep of :259-312 from the stress of
+// (a, b) = (fx, fy), giving o0..o3 = px, qx, py, qy.  Every point of the
+// array is written, 0 off lcu / lcv and outside the interior.
+constexpr int D4_TX = 64, D4_TY = 8, D4_ROWS = 4;
+constexpr int D4_W = D4_TX + 1, D4_H = D4_TY + 1;
+
+template <bool FIRST>
+__global__ __launch_bounds__(D4_TX *D4_ROWS) void k_d4_sweep(DevGrid g, const double *__restrict__ mu,
+                                                             const double *__restrict__ a,
+                                                             const double *__restrict__ b,
+                                                             const double *__restrict__ sh, double *__restrict__ o0,
+                                                             double *__restrict__ o1, double *__restrict__ o2,
+                                                             double *__restrict__ o3)
+{
+    __shared__ double Ts[D4_H][D4_W], Ss[D4_H][D4_W], Hs[D4_H][D4_W];
+    int bx, by;
+    xcd_tile((int)gridDim.x, (int)gridDim.y, (int)(blockIdx.y * gridDim.x + blockIdx.x), bx, by);
+    const int m0 = 1 + bx * D4_TX, n0 = 1 + by * D4_TY;
+    const int tid = (int)(threadIdx.y * D4_TX + threadIdx.x);
+    const D4Depth d{&g, sh};
+    const D4Stress<Plane, Plane> st{&g, Plane{&g, a}, Plane{&g, b}};
+    for (int k = tid; k < D4_W * D4_H; k += D4_TX * D4_ROWS) {
+        const int r = k / D4_W, c = k - r * D4_W;
+        Ts[r][c] = st.t(m0 + c, n0 + r);
+        Ss[r][c] = st.s(m0 - 1 + c, n0 - 1 + r);
+        Hs[r][c] = d.hh(m0 - 1 + c, n0 - 1 + r);
+    }
+    __syncthreads();
+    auto tt = [&](int x, int y) { return Ts[y - n0][x - m0]; };
+    auto ss = [&](int x, int y) { return Ss[y - n0 + 1][x - m0 + 1]; };
+    auto hh = [&](int x, int y) { return Hs[y - n0 + 1][x - m0 + 1]; };
+    auto hq = [&](int x, int y) { return d.hq(x, y); };
+    const int m = m0 + (int)threadIdx.x;
+    if (m > g.nx) return;
+    for (int j = (int)threadIdx.y; j < D4_TY; j += D4_ROWS) {
+        const int n = n0 + j;
+        if (n > g.ny) return;
+        const size_t i = I(m, n);
+        double pu = 0.0, qu = 0.0, pv = 0.0, qv = 0.0;
+        const bool in = IN_INTERIOR(g);
+        const bool cu = in && g.lcu[i] > 0.5f, cv = in && g.lcv[i] > 0.5f;
+        if (cu) visc_u_at(g, mu, tt, ss, hq, hh, m, n, pu, qu);
+        if (cv) visc_v_at(g, mu, tt, ss, hq, hh, m, n, pv, qv);
+        if constexpr (FIRST) {
+            double vx = 0.0, vy = 0.0;
+            if (cu) {
+                double f = pu + qu;
+                vx = -f / d.hu(m, n) / g.dxt[i] / g.dyh[i];
+            }
+            if (cv) {
+                double f = -pv + qv;
+                vy = -f / d.hv(m, n) / g.dxh[i] / g.dyt[i];
+            }
+            o0[i] = vx;
+            o1[i] = vy;
+        } else {
+            o0[i] = pu;
+            o1[i] = qu;
+            o2[i] = pv;
+            o3[i] = qv;
+        }
     }
 }
 
@@ -517,25 +655,13 @@ void lit_hh_update(inmsom_bt_ctx *c, const double *sh, double *hqn, double *hun,
 }
 
 void lit_diff4_terms(inmsom_bt_ctx *c, const double *up, const double *vp, const double *sh, const double *mu4,
-                     double *const t[12])
+                     double *const t[6])
 {
     const DevGrid &g = c->g;
-    double *hq = t[0], *hu = t[1], *hv = t[2], *hh = t[3], *st = t[4], *ss = t[5], *fx = t[6], *fy = t[7];
-    // the substep's level-n depths, then uv_diff4 as the stage-by-stage path runs it
-    lit_hh_update(c, sh, hq, hu, hv, hh);
-    hipLaunchKernelGGL(k_stress, grid2(g), BLK2, 0, c->stream, g, up, vp, st, ss);
-    cyclize(c, st);
-    cyclize(c, ss);
-    fill(c, fx, 0.0);
-    fill(c, fy, 0.0);
-    hipLaunchKernelGGL(k_diff4_f, grid2(g), BLK2, 0, c->stream, g, mu4, st, ss, hq, hu, hv, hh, fx, fy);
-    cyclize(c, fx);
-    cyclize(c, fy);
-    hipLaunchKernelGGL(k_stress, grid2(g), BLK2, 0, c->stream, g, fx, fy, st, ss);
-    cyclize(c, st);
-    cyclize(c, ss);
-    hipLaunchKernelGGL(k_diff4_terms, grid2(g), BLK2, 0, c->stream, g, mu4, st, ss, hq, hh, t[8], t[9], t[10],
-                       t[11]);
+    const dim3 grid((unsigned)((g.nx + D4_TX - 1) / D4_TX), (unsigned)((g.ny + D4_TY - 1) / D4_TY));
+    const dim3 blk(D4_TX, D4_ROWS);
+    hipLaunchKernelGGL(k_d4_sweep<true>, grid, blk, 0, c->stream, g, mu4, up, vp, sh, t[0], t[1], nullptr, nullptr);
+    hipLaunchKernelGGL(k_d4_sweep<false>, grid, blk, 0, c->stream, g, mu4, t[0], t[1], sh, t[2], t[3], t[4], t[5]);
 }
 
 // Device array of the 12 depth pointers (k_hh_init and k_shift index
