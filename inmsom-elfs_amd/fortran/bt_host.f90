@@ -176,7 +176,7 @@ do step = 1, nsteps
   call system_clock(c0)
   if (mode == -2) then
     ! the reference's own call (Control/dyncall.f90:137-164) on host arrays
-    call barotropic_dynamics_gpu(time_step, nstep_barotrop, 0, st(:,:,3), st(:,:,4), st(:,:,5), st(:,:,6), &
+    call barotropic_dynamics_gpu(time_step, nstep_barotrop, ksw_lat4, st(:,:,3), st(:,:,4), st(:,:,5), st(:,:,6), &
                                  st(:,:,1), st(:,:,2), acc(:,:,1), acc(:,:,2), acc(:,:,3), acc(:,:,4), &
                                  rhs(:,:,1), rhs(:,:,2), f(:,:,13), f(:,:,14), f(:,:,15), scr(:,:,1), &
                                  scr(:,:,2), scr(:,:,3), scr(:,:,4), scr(:,:,5), rdis, scr(:,:,6), &

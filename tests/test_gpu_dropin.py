@@ -124,7 +124,8 @@ def test_device_address_entry(path):
     assert_bitwise(got, {**st_h, **acc_h}, STATE + ACC, f"device vs host entry ({path})")
 
 
-def test_full_size_call_matches_oracle():
+@pytest.mark.parametrize("ksw4", [0, 1])
+def test_full_size_call_matches_oracle(ksw4):
     b = single_basin("cfg1")
     og = oracle.OracleGrid(b)
     rdis = og.rayleigh()
@@ -133,13 +134,13 @@ def test_full_size_call_matches_oracle():
     st_g = {k: v.copy() for k, v in st_o.items()}
     acc_o = {k: np.zeros((b.ny, b.nx)) for k in ACC}
     acc_g = {k: np.zeros((b.ny, b.nx)) for k in ACC}
-    print("2048x1536 drop-in call, nstep=2", flush=True)
-    oracle.barotropic_dynamics(og, b.dt, 2, 0, st_o, acc_o, inp["RHSx"], inp["RHSy"], inp["wflux"], inp["mu"],
+    print(f"2048x1536 drop-in call, nstep=2, ksw4={ksw4}", flush=True)
+    oracle.barotropic_dynamics(og, b.dt, 2, ksw4, st_o, acc_o, inp["RHSx"], inp["RHSy"], inp["wflux"], inp["mu"],
                                inp["mu4"], rdis)
     with BarotropicModel(b, rdis=rdis) as m:
-        assert m.dropin_path(0) == "fused"
-        _call_model(m, b, 2, st_g, acc_g, inp, rdis)
-    assert_bitwise({**st_g, **acc_g}, {**st_o, **acc_o}, STATE + ACC, "2048x1536")
+        assert m.dropin_path(ksw4) == "fused"
+        _call_model(m, b, 2, st_g, acc_g, inp, rdis, ksw4)
+    assert_bitwise({**st_g, **acc_g}, {**st_o, **acc_o}, STATE + ACC, f"2048x1536 ksw4={ksw4}")
 
 
 def test_routing():

@@ -1,6 +1,7 @@
 """The Fortran host (ISO_C_BINDING module + driver in the reference's
 shape) on the GPU: its final state equals the oracle's bit for bit."""
 import os
+import shutil
 import subprocess
 import tempfile
 
@@ -273,14 +274,16 @@ def test_fortran_host_density_bitwise():
     assert_bitwise(got, want, ("den", "den_pot"), "fortran host density")
 
 
-@pytest.mark.parametrize("metrics", ["uniform", "stretched"])
-def test_fortran_host_dropin_bitwise(metrics):
+@pytest.mark.parametrize("metrics,ksw4", [("uniform", 0), ("stretched", 0), ("uniform", 1), ("stretched", 1)])
+def test_fortran_host_dropin_bitwise(metrics, ksw4):
     """The host's 'dropin' mode calls barotropic_dynamics_gpu, the
     reference's own argument list (Function/vel_ssh.f90:379-439) through the
     ISO_C_BINDING module, every step on host arrays with a fixed RHS (the
     wind stress as a face force over the reference density): the state after
     the steps and the last call's averages equal the oracle's calls bit for
-    bit."""
+    bit.  ksw4 is the parameter file's ksw_lat4 (phys_proc.par line 6), which
+    the host passes on as the reference's caller does; 1 adds the biharmonic
+    viscosity."""
     b = randomize_state(make_basin(96, 64, seed=55, sigma=5.0, metrics=metrics), 6, wflux_amp=1e-6)
     rd = oracle.OracleGrid(b).rayleigh()
     steps, nstep = 7, 2  # ocean_run.par: nstep_barotrop = 2
@@ -290,15 +293,22 @@ def test_fortran_host_dropin_bitwise(metrics):
     st = {k: np.ascontiguousarray(getattr(b, k)).copy() for k in ("u", "up", "v", "vp", "ssh", "sshp")}
     for _ in range(steps):
         acc = {k: np.zeros((b.ny, b.nx)) for k in ("ubrtr_i", "vbrtr_i", "ssh4gradx", "ssh4grady")}
-        oracle.barotropic_dynamics(og, b.dt, nstep, 0, st, acc, rhsx, rhsy, b.wflux, b.mu, b.mu4, rd)
+        oracle.barotropic_dynamics(og, b.dt, nstep, ksw4, st, acc, rhsx, rhsy, b.wflux, b.mu, b.mu4, rd)
     with tempfile.TemporaryDirectory() as tmp:
         case = os.path.join(tmp, "case.bin")
         out = os.path.join(tmp, "out.bin")
         write_case(case, b, rd, steps, nstep, 0, 0, [])
-        p = subprocess.run([HOST, PARDIR, case, out, "dropin", repr(b.g)], capture_output=True, text=True,
+        pardir = os.path.join(tmp, "par")
+        shutil.copytree(PARDIR, pardir)
+        pp = os.path.join(pardir, "phys_proc.par")
+        lines = open(pp).read().split("\n")
+        lines[5] = f"           {ksw4}  : Lateral 4th order momentum mixing: 0 - no, 1 - yes"
+        with open(pp, "w") as fh:
+            fh.write("\n".join(lines))
+        p = subprocess.run([HOST, pardir, case, out, "dropin", repr(b.g)], capture_output=True, text=True,
                            timeout=120)
         print(p.stdout, p.stderr)
         assert p.returncode == 0, p.stdout + p.stderr
         raw = np.fromfile(out, dtype="<f8").reshape(10, b.ny, b.nx)
     got = dict(zip(("ssh", "sshp", "u", "up", "v", "vp", "ubrtr_i", "vbrtr_i", "ssh4gradx", "ssh4grady"), raw))
-    assert_bitwise(got, {**st, **acc}, STATE + tuple(acc), f"fortran host drop-in ({metrics})")
+    assert_bitwise(got, {**st, **acc}, STATE + tuple(acc), f"fortran host drop-in ({metrics}, ksw4={ksw4})")
