@@ -793,7 +793,7 @@ int dropin_call(inmsom_bt_ctx *c, double tau, int nstep, double *const st[6], do
         // uv_diff4 of this substep from its entering state (level-n surface,
         // filtered velocities), added by the kernel to its uv_diff2 result
         if (ksw4 > 0) {
-            lit_diff4_terms(c, in[1], in[3], in[4], mu4, a->b4);
+            lit_diff4_terms(c, in[1], in[3], in[4], mu4, a->b4, a->rowmet_ok ? a->rowmet : nullptr, a->mb);
             for (int q = 0; q < 4; ++q) p.d4[q] = a->b4[2 + q];
         }
         const bool snap = k == nstep + 1;

@@ -24,8 +24,11 @@ void lit_hh_update(inmsom_bt_ctx *c, const double *sh, double *hqn, double *hun,
 // two terms per component: t[2..5] = px, qx, py, qy on lcu / lcv interior
 // points (0 elsewhere), which the caller adds to its uv_diff2 result as
 // RHSx_dif + px + qx, RHSy_dif - py + qy.  t[0], t[1] receive the first
-// sweep's fx, fy.  Two launches; all six planes are n doubles.
+// sweep's fx, fy.  Two launches; all six planes are n doubles.  rowmet:
+// the metrics as a 9 x ny row table (dx dy dxt dyt dxh dyh dxb dyb rlh_s)
+// when they are uniform along x on columns 2..nx-1, with mb the packed mask
+// bits (MB_*) of every cell; null for the grid's per-cell planes.
 void lit_diff4_terms(inmsom_bt_ctx *c, const double *up, const double *vp, const double *sh, const double *mu4,
-                     double *const t[6]);
+                     double *const t[6], const double *rowmet = nullptr, const uint8_t *mb = nullptr);
 
 }  // namespace ibt

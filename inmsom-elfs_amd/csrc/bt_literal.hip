@@ -64,37 +64,104 @@ __global__ void k_cyclize_y(DevGrid g, double *f)
     f[I(m, g.ny - 1)] = b;
 }
 
+// Metrics and masks of the grid at (a, b), for the stencil helpers below.
+// CellView reads the grid's per-cell planes; RowView (for grids whose
+// metrics are uniform along x on columns 2..nx-1) reads a table of nine
+// rows-long metric vectors (dx dy dxt dyt dxh dyh dxb dyb rlh_s) and packed
+// mask bits (MB_*) for the same values.
+struct CellView {
+    const DevGrid *g;
+    __device__ size_t at(int a, int b) const { return IDX(*g, a, b); }
+    __device__ double dx(int a, int b) const { return g->dx[at(a, b)]; }
+    __device__ double dy(int a, int b) const { return g->dy[at(a, b)]; }
+    __device__ double dxt(int a, int b) const { return g->dxt[at(a, b)]; }
+    __device__ double dyt(int a, int b) const { return g->dyt[at(a, b)]; }
+    __device__ double dxh(int a, int b) const { return g->dxh[at(a, b)]; }
+    __device__ double dyh(int a, int b) const { return g->dyh[at(a, b)]; }
+    __device__ double dxb(int a, int b) const { return g->dxb[at(a, b)]; }
+    __device__ double dyb(int a, int b) const { return g->dyb[at(a, b)]; }
+    __device__ float lu(int a, int b) const { return g->lu[at(a, b)]; }
+    __device__ bool is(const float *mask, int a, int b) const { return mask[at(a, b)] > 0.5f; }
+    __device__ bool is_lu(int a, int b) const { return is(g->lu, a, b); }
+    __device__ bool is_luu(int a, int b) const { return is(g->luu, a, b); }
+    __device__ bool is_luh(int a, int b) const { return is(g->luh, a, b); }
+    __device__ bool is_llu(int a, int b) const { return is(g->llu, a, b); }
+    __device__ bool is_llv(int a, int b) const { return is(g->llv, a, b); }
+    __device__ bool is_lcu(int a, int b) const { return is(g->lcu, a, b); }
+    __device__ bool is_lcv(int a, int b) const { return is(g->lcv, a, b); }
+};
+
+struct RowView {
+    const DevGrid *g;
+    const double *__restrict__ rm;
+    const uint8_t *__restrict__ mb;
+    __device__ size_t at(int a, int b) const { return IDX(*g, a, b); }
+    __device__ double met(int k, int b) const { return rm[(size_t)k * (size_t)g->ny + (size_t)(b - 1)]; }
+    __device__ double dx(int, int b) const { return met(0, b); }
+    __device__ double dy(int, int b) const { return met(1, b); }
+    __device__ double dxt(int, int b) const { return met(2, b); }
+    __device__ double dyt(int, int b) const { return met(3, b); }
+    __device__ double dxh(int, int b) const { return met(4, b); }
+    __device__ double dyh(int, int b) const { return met(5, b); }
+    __device__ double dxb(int, int b) const { return met(6, b); }
+    __device__ double dyb(int, int b) const { return met(7, b); }
+    __device__ float lu(int a, int b) const { return (mb[at(a, b)] & MB_LU) ? 1.0f : 0.0f; }
+    __device__ bool is_lu(int a, int b) const { return mb[at(a, b)] & MB_LU; }
+    __device__ bool is_luu(int a, int b) const { return mb[at(a, b)] & MB_LUU; }
+    __device__ bool is_luh(int a, int b) const { return mb[at(a, b)] & MB_LUH; }
+    __device__ bool is_llu(int a, int b) const { return mb[at(a, b)] & MB_LLU; }
+    __device__ bool is_llv(int a, int b) const { return mb[at(a, b)] & MB_LLV; }
+    __device__ bool is_lcu(int a, int b) const { return mb[at(a, b)] & MB_LCU; }
+    __device__ bool is_lcv(int a, int b) const { return mb[at(a, b)] & MB_LCV; }
+};
+
 // Depth on U, V and H points from T-point depths hq(m, n) given by Q,
 // Function/depth.f90:29-66.
+template <class V, class Q>
+__device__ inline double avg_u_w(const V &w, Q q, int m, int n)
+{
+    double slu = (double)(w.lu(m, n) + w.lu(m + 1, n));
+    return (q(m, n) * w.dx(m, n) * w.dy(m, n) * (double)w.lu(m, n) +
+            q(m + 1, n) * w.dx(m + 1, n) * w.dy(m + 1, n) * (double)w.lu(m + 1, n)) /
+           slu / w.dxt(m, n) / w.dyh(m, n);
+}
+
+template <class V, class Q>
+__device__ inline double avg_v_w(const V &w, Q q, int m, int n)
+{
+    double slu = (double)(w.lu(m, n) + w.lu(m, n + 1));
+    return (q(m, n) * w.dx(m, n) * w.dy(m, n) * (double)w.lu(m, n) +
+            q(m, n + 1) * w.dx(m, n + 1) * w.dy(m, n + 1) * (double)w.lu(m, n + 1)) /
+           slu / w.dxh(m, n) / w.dyt(m, n);
+}
+
+template <class V, class Q>
+__device__ inline double avg_h_w(const V &w, Q q, int m, int n)
+{
+    double slu = (double)(w.lu(m, n) + w.lu(m + 1, n) + w.lu(m, n + 1) + w.lu(m + 1, n + 1));
+    return (q(m, n) * w.dx(m, n) * w.dy(m, n) * (double)w.lu(m, n) +
+            q(m + 1, n) * w.dx(m + 1, n) * w.dy(m + 1, n) * (double)w.lu(m + 1, n) +
+            q(m, n + 1) * w.dx(m, n + 1) * w.dy(m, n + 1) * (double)w.lu(m, n + 1) +
+            q(m + 1, n + 1) * w.dx(m + 1, n + 1) * w.dy(m + 1, n + 1) * (double)w.lu(m + 1, n + 1)) /
+           slu / w.dxb(m, n) / w.dyb(m, n);
+}
+
 template <class Q>
 __device__ inline double avg_u(const DevGrid &g, Q q, int m, int n)
 {
-    double slu = (double)(g.lu[I(m, n)] + g.lu[I(m + 1, n)]);
-    return (q(m, n) * g.dx[I(m, n)] * g.dy[I(m, n)] * (double)g.lu[I(m, n)] +
-            q(m + 1, n) * g.dx[I(m + 1, n)] * g.dy[I(m + 1, n)] * (double)g.lu[I(m + 1, n)]) /
-           slu / g.dxt[I(m, n)] / g.dyh[I(m, n)];
+    return avg_u_w(CellView{&g}, q, m, n);
 }
 
 template <class Q>
 __device__ inline double avg_v(const DevGrid &g, Q q, int m, int n)
 {
-    double slu = (double)(g.lu[I(m, n)] + g.lu[I(m, n + 1)]);
-    return (q(m, n) * g.dx[I(m, n)] * g.dy[I(m, n)] * (double)g.lu[I(m, n)] +
-            q(m, n + 1) * g.dx[I(m, n + 1)] * g.dy[I(m, n + 1)] * (double)g.lu[I(m, n + 1)]) /
-           slu / g.dxh[I(m, n)] / g.dyt[I(m, n)];
+    return avg_v_w(CellView{&g}, q, m, n);
 }
 
 template <class Q>
 __device__ inline double avg_h(const DevGrid &g, Q q, int m, int n)
 {
-    double slu = (double)(g.lu[I(m, n)] + g.lu[I(m + 1, n)] + g.lu[I(m, n + 1)] +
-                          g.lu[I(m + 1, n + 1)]);
-    return (q(m, n) * g.dx[I(m, n)] * g.dy[I(m, n)] * (double)g.lu[I(m, n)] +
-            q(m + 1, n) * g.dx[I(m + 1, n)] * g.dy[I(m + 1, n)] * (double)g.lu[I(m + 1, n)] +
-            q(m, n + 1) * g.dx[I(m, n + 1)] * g.dy[I(m, n + 1)] * (double)g.lu[I(m, n + 1)] +
-            q(m + 1, n + 1) * g.dx[I(m + 1, n + 1)] * g.dy[I(m + 1, n + 1)] *
-                (double)g.lu[I(m + 1, n + 1)]) /
-           slu / g.dxb[I(m, n)] / g.dyb[I(m, n)];
+    return avg_h_w(CellView{&g}, q, m, n);
 }
 
 // ------------------------------------------------------------ depth.f90
@@ -206,20 +273,18 @@ __global__ void k_continuity(DevGrid g, double tau_inner, const double *__restri
 // stress_components (Function/mixing.f90:115-168) at (m, n) of the
 // velocity pair given by accessors, for a point of the interior: tension on
 // lu points, shear on luu points.
-template <class A, class B>
-__device__ inline double tension_at(const DevGrid &g, A u, B v, int m, int n)
+template <class V, class A, class B>
+__device__ inline double tension_at(const V &w, A u, B v, int m, int n)
 {
-    const size_t i = I(m, n);
-    return g.dy[i] / g.dx[i] * (u(m, n) / g.dyh[i] - u(m - 1, n) / g.dyh[I(m - 1, n)]) -
-           g.dx[i] / g.dy[i] * (v(m, n) / g.dxh[i] - v(m, n - 1) / g.dxh[I(m, n - 1)]);
+    return w.dy(m, n) / w.dx(m, n) * (u(m, n) / w.dyh(m, n) - u(m - 1, n) / w.dyh(m - 1, n)) -
+           w.dx(m, n) / w.dy(m, n) * (v(m, n) / w.dxh(m, n) - v(m, n - 1) / w.dxh(m, n - 1));
 }
 
-template <class A, class B>
-__device__ inline double shear_at(const DevGrid &g, A u, B v, int m, int n)
+template <class V, class A, class B>
+__device__ inline double shear_at(const V &w, A u, B v, int m, int n)
 {
-    const size_t i = I(m, n);
-    return g.dxb[i] / g.dyb[i] * (u(m, n + 1) / g.dxt[I(m, n + 1)] - u(m, n) / g.dxt[i]) +
-           g.dyb[i] / g.dxb[i] * (v(m + 1, n) / g.dyt[I(m + 1, n)] - v(m, n) / g.dyt[i]);
+    return w.dxb(m, n) / w.dyb(m, n) * (u(m, n + 1) / w.dxt(m, n + 1) - u(m, n) / w.dxt(m, n)) +
+           w.dyb(m, n) / w.dxb(m, n) * (v(m + 1, n) / w.dyt(m + 1, n) - v(m, n) / w.dyt(m, n));
 }
 
 __global__ void k_stress(DevGrid g, const double *__restrict__ u, const double *__restrict__ v,
@@ -228,8 +293,8 @@ __global__ void k_stress(DevGrid g, const double *__restrict__ u, const double *
     KERNEL_MN
     if (!IN_INTERIOR(g)) return;
     const size_t i = I(m, n);
-    if (g.lu[i] > 0.5f) str_t[i] = tension_at(g, Plane{&g, u}, Plane{&g, v}, m, n);
-    if (g.luu[i] > 0.5f) str_s[i] = shear_at(g, Plane{&g, u}, Plane{&g, v}, m, n);
+    if (g.lu[i] > 0.5f) str_t[i] = tension_at(CellView{&g}, Plane{&g, u}, Plane{&g, v}, m, n);
+    if (g.luu[i] > 0.5f) str_s[i] = shear_at(CellView{&g}, Plane{&g, u}, Plane{&g, v}, m, n);
 }
 
 // metric vorticity of uv_trans (Function/vel_ssh.f90:30-42)
@@ -248,46 +313,46 @@ __global__ void k_vort(DevGrid g, const double *__restrict__ u, const double *__
 // Tension part p and shear part q of the viscous stencil,
 // Function/vel_ssh.f90:150-156 (u) and :166-172 (v); the stress, depths
 // at T and H points come from accessors (a, b) -> value.
-template <class ST, class SS, class HQ, class HH>
-__device__ inline void visc_u_at(const DevGrid &g, const double *mu, ST str_t, SS str_s, HQ hq, HH hh, int m,
-                                 int n, double &p, double &q)
+template <class V, class ST, class SS, class HQ, class HH>
+__device__ inline void visc_u_at(const V &w, const double *mu, ST str_t, SS str_s, HQ hq, HH hh, int m, int n,
+                                 double &p, double &q)
 {
-    double muh_p = (mu[I(m, n)] + mu[I(m + 1, n)] + mu[I(m, n + 1)] + mu[I(m + 1, n + 1)]) / 4.0;
-    double muh_m = (mu[I(m, n)] + mu[I(m + 1, n)] + mu[I(m, n - 1)] + mu[I(m + 1, n - 1)]) / 4.0;
-    double dy1 = g.dy[I(m + 1, n)], dy0 = g.dy[I(m, n)];
-    double b1 = g.dxb[I(m, n)], b0 = g.dxb[I(m, n - 1)];
-    p = (dy1 * dy1 * mu[I(m + 1, n)] * hq(m + 1, n) * str_t(m + 1, n) -
-         dy0 * dy0 * mu[I(m, n)] * hq(m, n) * str_t(m, n)) / g.dyh[I(m, n)];
-    q = (b1 * b1 * muh_p * hh(m, n) * str_s(m, n) -
-         b0 * b0 * muh_m * hh(m, n - 1) * str_s(m, n - 1)) / g.dxt[I(m, n)];
+    auto M = [&](int a, int b) { return mu[w.at(a, b)]; };
+    double muh_p = (M(m, n) + M(m + 1, n) + M(m, n + 1) + M(m + 1, n + 1)) / 4.0;
+    double muh_m = (M(m, n) + M(m + 1, n) + M(m, n - 1) + M(m + 1, n - 1)) / 4.0;
+    double dy1 = w.dy(m + 1, n), dy0 = w.dy(m, n);
+    double b1 = w.dxb(m, n), b0 = w.dxb(m, n - 1);
+    p = (dy1 * dy1 * M(m + 1, n) * hq(m + 1, n) * str_t(m + 1, n) - dy0 * dy0 * M(m, n) * hq(m, n) * str_t(m, n)) /
+        w.dyh(m, n);
+    q = (b1 * b1 * muh_p * hh(m, n) * str_s(m, n) - b0 * b0 * muh_m * hh(m, n - 1) * str_s(m, n - 1)) / w.dxt(m, n);
 }
 
-template <class ST, class SS, class HQ, class HH>
-__device__ inline void visc_v_at(const DevGrid &g, const double *mu, ST str_t, SS str_s, HQ hq, HH hh, int m,
-                                 int n, double &p, double &q)
+template <class V, class ST, class SS, class HQ, class HH>
+__device__ inline void visc_v_at(const V &w, const double *mu, ST str_t, SS str_s, HQ hq, HH hh, int m, int n,
+                                 double &p, double &q)
 {
-    double muh_p = (mu[I(m, n)] + mu[I(m + 1, n)] + mu[I(m, n + 1)] + mu[I(m + 1, n + 1)]) / 4.0;
-    double muh_m = (mu[I(m, n)] + mu[I(m - 1, n)] + mu[I(m, n + 1)] + mu[I(m - 1, n + 1)]) / 4.0;
-    double dx1 = g.dx[I(m, n + 1)], dx0 = g.dx[I(m, n)];
-    double b1 = g.dyb[I(m, n)], b0 = g.dyb[I(m - 1, n)];
-    p = (dx1 * dx1 * mu[I(m, n + 1)] * hq(m, n + 1) * str_t(m, n + 1) -
-         dx0 * dx0 * mu[I(m, n)] * hq(m, n) * str_t(m, n)) / g.dxh[I(m, n)];
-    q = (b1 * b1 * muh_p * hh(m, n) * str_s(m, n) -
-         b0 * b0 * muh_m * hh(m - 1, n) * str_s(m - 1, n)) / g.dyt[I(m, n)];
+    auto M = [&](int a, int b) { return mu[w.at(a, b)]; };
+    double muh_p = (M(m, n) + M(m + 1, n) + M(m, n + 1) + M(m + 1, n + 1)) / 4.0;
+    double muh_m = (M(m, n) + M(m - 1, n) + M(m, n + 1) + M(m - 1, n + 1)) / 4.0;
+    double dx1 = w.dx(m, n + 1), dx0 = w.dx(m, n);
+    double b1 = w.dyb(m, n), b0 = w.dyb(m - 1, n);
+    p = (dx1 * dx1 * M(m, n + 1) * hq(m, n + 1) * str_t(m, n + 1) - dx0 * dx0 * M(m, n) * hq(m, n) * str_t(m, n)) /
+        w.dxh(m, n);
+    q = (b1 * b1 * muh_p * hh(m, n) * str_s(m, n) - b0 * b0 * muh_m * hh(m - 1, n) * str_s(m - 1, n)) / w.dyt(m, n);
 }
 
 __device__ inline void visc_u(const DevGrid &g, const double *mu, const double *str_t,
                               const double *str_s, const double *hq, const double *hh, int m, int n,
                               double &p, double &q)
 {
-    visc_u_at(g, mu, Plane{&g, str_t}, Plane{&g, str_s}, Plane{&g, hq}, Plane{&g, hh}, m, n, p, q);
+    visc_u_at(CellView{&g}, mu, Plane{&g, str_t}, Plane{&g, str_s}, Plane{&g, hq}, Plane{&g, hh}, m, n, p, q);
 }
 
 __device__ inline void visc_v(const DevGrid &g, const double *mu, const double *str_t,
                               const double *str_s, const double *hq, const double *hh, int m, int n,
                               double &p, double &q)
 {
-    visc_v_at(g, mu, Plane{&g, str_t}, Plane{&g, str_s}, Plane{&g, hq}, Plane{&g, hh}, m, n, p, q);
+    visc_v_at(CellView{&g}, mu, Plane{&g, str_t}, Plane{&g, str_s}, Plane{&g, hq}, Plane{&g, hh}, m, n, p, q);
 }
 
 // flux-form advection of uv_trans (Function/vel_ssh.f90:54-109) and the
@@ -392,112 +457,121 @@ __global__ void k_diff4_add(DevGrid g, const double *__restrict__ mu, const doub
 // points its stencil reads, with the same expressions, and the value 0 where
 // the reference does not compute them (outside their index range or mask),
 // as the zero-initialised planes hold there.
+template <class V>
 struct D4Depth {
-    const DevGrid *g;
+    const V w;
     const double *sh;
     // hq = H + ssh at a T point
     __device__ double hq(int a, int b) const
     {
-        const size_t j = IDX(*g, a, b);
-        return g->hr[j] + sh[j];
+        const size_t j = w.at(a, b);
+        return w.g->hr[j] + sh[j];
     }
-    __device__ bool in_range(int a, int b) const { return a >= 2 && a <= g->nx - 2 && b >= 2 && b <= g->ny - 2; }
+    __device__ bool in_range(int a, int b) const
+    {
+        return a >= 2 && a <= w.g->nx - 2 && b >= 2 && b <= w.g->ny - 2;
+    }
     __device__ double hu(int a, int b) const
     {
-        const DevGrid &G = *g;
-        if (!in_range(a, b) || !(G.llu[IDX(G, a, b)] > 0.5f)) return 0.0;
-        return avg_u(G, [this](int x, int y) { return hq(x, y); }, a, b);
+        if (!in_range(a, b) || !w.is_llu(a, b)) return 0.0;
+        return avg_u_w(w, [this](int x, int y) { return hq(x, y); }, a, b);
     }
     __device__ double hv(int a, int b) const
     {
-        const DevGrid &G = *g;
-        if (!in_range(a, b) || !(G.llv[IDX(G, a, b)] > 0.5f)) return 0.0;
-        return avg_v(G, [this](int x, int y) { return hq(x, y); }, a, b);
+        if (!in_range(a, b) || !w.is_llv(a, b)) return 0.0;
+        return avg_v_w(w, [this](int x, int y) { return hq(x, y); }, a, b);
     }
     __device__ double hh(int a, int b) const
     {
-        const DevGrid &G = *g;
-        if (!in_range(a, b) || !(G.luh[IDX(G, a, b)] > 0.5f)) return 0.0;
-        return avg_h(G, [this](int x, int y) { return hq(x, y); }, a, b);
+        if (!in_range(a, b) || !w.is_luh(a, b)) return 0.0;
+        return avg_h_w(w, [this](int x, int y) { return hq(x, y); }, a, b);
     }
 };
 
-template <class A, class B>
+template <class V>
 struct D4Stress {
-    const DevGrid *g;
-    A u;
-    B v;
-    __device__ bool interior(int a, int b) const { return a >= 3 && a <= g->nx - 2 && b >= 3 && b <= g->ny - 2; }
+    const V w;
+    Plane u, v;
+    __device__ bool interior(int a, int b) const
+    {
+        return a >= 3 && a <= w.g->nx - 2 && b >= 3 && b <= w.g->ny - 2;
+    }
     __device__ double t(int a, int b) const
     {
-        if (!interior(a, b) || !(g->lu[IDX(*g, a, b)] > 0.5f)) return 0.0;
-        return tension_at(*g, u, v, a, b);
+        if (!interior(a, b) || !w.is_lu(a, b)) return 0.0;
+        return tension_at(w, u, v, a, b);
     }
     __device__ double s(int a, int b) const
     {
-        if (!interior(a, b) || !(g->luu[IDX(*g, a, b)] > 0.5f)) return 0.0;
-        return shear_at(*g, u, v, a, b);
+        if (!interior(a, b) || !w.is_luu(a, b)) return 0.0;
+        return shear_at(w, u, v, a, b);
     }
 };
 
 // One sweep per launch over tiles of D4_TX x D4_TY points (one thread per
-// column, D4_TY / D4_ROWS rows of threads): the tile first forms, once per
-// point, the tension on [m0, m0 + D4_TX] x [n0, n0 + D4_TY] and the shear
-// and H-point depths on [m0 - 1, m0 + D4_TX - 1] x [n0 - 1, n0 + D4_TY - 1]
-// in LDS, which is every point the viscous stencil of its own points reads.
+// column, D4_ROWS rows of threads, a wave per row of threads): the tile
+// first forms, once per point, the tension on [m0, m0 + D4_TX] x
+// [n0, n0 + D4_TY] and the shear and H-point depths on
+// [m0 - 1, m0 + D4_TX - 1] x [n0 - 1, n0 + D4_TY - 1] in LDS, which is every
+// point the viscous stencil of its own points reads.
 // FIRST: the sweep of :216-257 from the stress of (a, b) = (up, vp), giving
 // o0, o1 = fx, fy; else the sweep of :259-312 from the stress of
 // (a, b) = (fx, fy), giving o0..o3 = px, qx, py, qy.  Every point of the
-// array is written, 0 off lcu / lcv and outside the interior.
+// array is written, 0 off lcu / lcv and outside the interior.  A wave's
+// points share their row, so with RowView its metric reads are uniform.
 constexpr int D4_TX = 64, D4_TY = 8, D4_ROWS = 4;
 constexpr int D4_W = D4_TX + 1, D4_H = D4_TY + 1;
 
-template <bool FIRST>
-__global__ __launch_bounds__(D4_TX *D4_ROWS) void k_d4_sweep(DevGrid g, const double *__restrict__ mu,
-                                                             const double *__restrict__ a,
-                                                             const double *__restrict__ b,
-                                                             const double *__restrict__ sh, double *__restrict__ o0,
-                                                             double *__restrict__ o1, double *__restrict__ o2,
-                                                             double *__restrict__ o3)
+template <bool FIRST, class V>
+__device__ __forceinline__ void d4_sweep(const V &w, const double *__restrict__ mu, const double *__restrict__ a,
+                                         const double *__restrict__ b, const double *__restrict__ sh,
+                                         double *__restrict__ o0, double *__restrict__ o1, double *__restrict__ o2,
+                                         double *__restrict__ o3)
 {
     __shared__ double Ts[D4_H][D4_W], Ss[D4_H][D4_W], Hs[D4_H][D4_W];
+    const DevGrid &g = *w.g;
     int bx, by;
     xcd_tile((int)gridDim.x, (int)gridDim.y, (int)(blockIdx.y * gridDim.x + blockIdx.x), bx, by);
     const int m0 = 1 + bx * D4_TX, n0 = 1 + by * D4_TY;
-    const int tid = (int)(threadIdx.y * D4_TX + threadIdx.x);
-    const D4Depth d{&g, sh};
-    const D4Stress<Plane, Plane> st{&g, Plane{&g, a}, Plane{&g, b}};
-    for (int k = tid; k < D4_W * D4_H; k += D4_TX * D4_ROWS) {
-        const int r = k / D4_W, c = k - r * D4_W;
-        Ts[r][c] = st.t(m0 + c, n0 + r);
-        Ss[r][c] = st.s(m0 - 1 + c, n0 - 1 + r);
-        Hs[r][c] = d.hh(m0 - 1 + c, n0 - 1 + r);
+    const int tx = (int)threadIdx.x, tid = (int)(threadIdx.y * D4_TX + threadIdx.x);
+    const D4Depth<V> d{w, sh};
+    const D4Stress<V> st{w, Plane{&g, a}, Plane{&g, b}};
+    for (int r = (int)threadIdx.y; r < D4_H; r += D4_ROWS) {
+        const int ru = __builtin_amdgcn_readfirstlane(r);  // one row per wave
+        Ts[ru][tx] = st.t(m0 + tx, n0 + ru);
+        Ss[ru][tx] = st.s(m0 - 1 + tx, n0 - 1 + ru);
+        Hs[ru][tx] = d.hh(m0 - 1 + tx, n0 - 1 + ru);
+    }
+    if (tid < D4_H) {  // the tile's last column
+        Ts[tid][D4_TX] = st.t(m0 + D4_TX, n0 + tid);
+        Ss[tid][D4_TX] = st.s(m0 - 1 + D4_TX, n0 - 1 + tid);
+        Hs[tid][D4_TX] = d.hh(m0 - 1 + D4_TX, n0 - 1 + tid);
     }
     __syncthreads();
     auto tt = [&](int x, int y) { return Ts[y - n0][x - m0]; };
     auto ss = [&](int x, int y) { return Ss[y - n0 + 1][x - m0 + 1]; };
     auto hh = [&](int x, int y) { return Hs[y - n0 + 1][x - m0 + 1]; };
     auto hq = [&](int x, int y) { return d.hq(x, y); };
-    const int m = m0 + (int)threadIdx.x;
+    const int m = m0 + tx;
     if (m > g.nx) return;
     for (int j = (int)threadIdx.y; j < D4_TY; j += D4_ROWS) {
-        const int n = n0 + j;
+        const int n = __builtin_amdgcn_readfirstlane(n0 + j);  // one row per wave
         if (n > g.ny) return;
-        const size_t i = I(m, n);
+        const size_t i = w.at(m, n);
         double pu = 0.0, qu = 0.0, pv = 0.0, qv = 0.0;
         const bool in = IN_INTERIOR(g);
-        const bool cu = in && g.lcu[i] > 0.5f, cv = in && g.lcv[i] > 0.5f;
-        if (cu) visc_u_at(g, mu, tt, ss, hq, hh, m, n, pu, qu);
-        if (cv) visc_v_at(g, mu, tt, ss, hq, hh, m, n, pv, qv);
+        const bool cu = in && w.is_lcu(m, n), cv = in && w.is_lcv(m, n);
+        if (cu) visc_u_at(w, mu, tt, ss, hq, hh, m, n, pu, qu);
+        if (cv) visc_v_at(w, mu, tt, ss, hq, hh, m, n, pv, qv);
         if constexpr (FIRST) {
             double vx = 0.0, vy = 0.0;
             if (cu) {
                 double f = pu + qu;
-                vx = -f / d.hu(m, n) / g.dxt[i] / g.dyh[i];
+                vx = -f / d.hu(m, n) / w.dxt(m, n) / w.dyh(m, n);
             }
             if (cv) {
                 double f = -pv + qv;
-                vy = -f / d.hv(m, n) / g.dxh[i] / g.dyt[i];
+                vy = -f / d.hv(m, n) / w.dxh(m, n) / w.dyt(m, n);
             }
             o0[i] = vx;
             o1[i] = vy;
@@ -508,6 +582,24 @@ __global__ __launch_bounds__(D4_TX *D4_ROWS) void k_d4_sweep(DevGrid g, const do
             o3[i] = qv;
         }
     }
+}
+
+// rm: the row table (9 x ny) when the metrics are uniform along x, with mb
+// the packed masks; null for the grid's per-cell planes
+template <bool FIRST>
+__global__ __launch_bounds__(D4_TX *D4_ROWS) void k_d4_sweep(DevGrid g, const double *__restrict__ rm,
+                                                             const uint8_t *__restrict__ mb,
+                                                             const double *__restrict__ mu,
+                                                             const double *__restrict__ a,
+                                                             const double *__restrict__ b,
+                                                             const double *__restrict__ sh, double *__restrict__ o0,
+                                                             double *__restrict__ o1, double *__restrict__ o2,
+                                                             double *__restrict__ o3)
+{
+    if (rm)
+        d4_sweep<FIRST>(RowView{&g, rm, mb}, mu, a, b, sh, o0, o1, o2, o3);
+    else
+        d4_sweep<FIRST>(CellView{&g}, mu, a, b, sh, o0, o1, o2, o3);
 }
 
 // momentum with Coriolis, pressure gradient, Rayleigh friction and the
@@ -655,13 +747,16 @@ void lit_hh_update(inmsom_bt_ctx *c, const double *sh, double *hqn, double *hun,
 }
 
 void lit_diff4_terms(inmsom_bt_ctx *c, const double *up, const double *vp, const double *sh, const double *mu4,
-                     double *const t[6])
+                     double *const t[6], const double *rowmet, const uint8_t *mb)
 {
     const DevGrid &g = c->g;
     const dim3 grid((unsigned)((g.nx + D4_TX - 1) / D4_TX), (unsigned)((g.ny + D4_TY - 1) / D4_TY));
     const dim3 blk(D4_TX, D4_ROWS);
-    hipLaunchKernelGGL(k_d4_sweep<true>, grid, blk, 0, c->stream, g, mu4, up, vp, sh, t[0], t[1], nullptr, nullptr);
-    hipLaunchKernelGGL(k_d4_sweep<false>, grid, blk, 0, c->stream, g, mu4, t[0], t[1], sh, t[2], t[3], t[4], t[5]);
+    const double *rm = rowmet && mb ? rowmet : nullptr;
+    hipLaunchKernelGGL(k_d4_sweep<true>, grid, blk, 0, c->stream, g, rm, mb, mu4, up, vp, sh, t[0], t[1], nullptr,
+                       nullptr);
+    hipLaunchKernelGGL(k_d4_sweep<false>, grid, blk, 0, c->stream, g, rm, mb, mu4, t[0], t[1], sh, t[2], t[3], t[4],
+                       t[5]);
 }
 
 // Device array of the 12 depth pointers (k_hh_init and k_shift index
