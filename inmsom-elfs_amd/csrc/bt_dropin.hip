@@ -473,9 +473,11 @@ struct DLoader {
 
 // Two waves per SIMD (the later substeps would take 256 VGPRs + 4 AGPRs,
 // one wave; bounded they spill 12 bytes per lane): at 2048 x 1536 on
-// MI355X a call of 4 launches takes 0.782 ms instead of 0.867.
+// MI355X a call of 4 launches takes 0.782 ms instead of 0.867.  With the
+// biharmonic terms (D4) one wave, whose registers hold what two waves
+// would spill.
 template <bool FIRST, bool ROWMET, bool D4>
-__global__ void __launch_bounds__(WAVE, 2) k_dropin(DropinParams p)
+__global__ void __launch_bounds__(WAVE, D4 ? 1 : 2) k_dropin(DropinParams p)
 {
     const int w = blockIdx.x;
     const int t = (w % NXCD) * p.per_xcd + (w / NXCD);
