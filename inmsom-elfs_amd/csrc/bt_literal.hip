@@ -584,9 +584,10 @@ __device__ __forceinline__ void d4_sweep(const V &w, const double *__restrict__ 
     }
 }
 
-// rm: the row table (9 x ny) when the metrics are uniform along x, with mb
-// the packed masks; null for the grid's per-cell planes
-template <bool FIRST>
+// ROW: metrics from rm, the row table (9 x ny) of a grid whose metrics are
+// uniform along x, and masks from mb, the packed mask bits; else the grid's
+// per-cell planes
+template <bool FIRST, bool ROW>
 __global__ __launch_bounds__(D4_TX *D4_ROWS) void k_d4_sweep(DevGrid g, const double *__restrict__ rm,
                                                              const uint8_t *__restrict__ mb,
                                                              const double *__restrict__ mu,
@@ -596,7 +597,7 @@ __global__ __launch_bounds__(D4_TX *D4_ROWS) void k_d4_sweep(DevGrid g, const do
                                                              double *__restrict__ o1, double *__restrict__ o2,
                                                              double *__restrict__ o3)
 {
-    if (rm)
+    if constexpr (ROW)
         d4_sweep<FIRST>(RowView{&g, rm, mb}, mu, a, b, sh, o0, o1, o2, o3);
     else
         d4_sweep<FIRST>(CellView{&g}, mu, a, b, sh, o0, o1, o2, o3);
@@ -752,11 +753,17 @@ void lit_diff4_terms(inmsom_bt_ctx *c, const double *up, const double *vp, const
     const DevGrid &g = c->g;
     const dim3 grid((unsigned)((g.nx + D4_TX - 1) / D4_TX), (unsigned)((g.ny + D4_TY - 1) / D4_TY));
     const dim3 blk(D4_TX, D4_ROWS);
-    const double *rm = rowmet && mb ? rowmet : nullptr;
-    hipLaunchKernelGGL(k_d4_sweep<true>, grid, blk, 0, c->stream, g, rm, mb, mu4, up, vp, sh, t[0], t[1], nullptr,
-                       nullptr);
-    hipLaunchKernelGGL(k_d4_sweep<false>, grid, blk, 0, c->stream, g, rm, mb, mu4, t[0], t[1], sh, t[2], t[3], t[4],
-                       t[5]);
+    if (rowmet && mb) {
+        hipLaunchKernelGGL((k_d4_sweep<true, true>), grid, blk, 0, c->stream, g, rowmet, mb, mu4, up, vp, sh, t[0],
+                           t[1], nullptr, nullptr);
+        hipLaunchKernelGGL((k_d4_sweep<false, true>), grid, blk, 0, c->stream, g, rowmet, mb, mu4, t[0], t[1], sh,
+                           t[2], t[3], t[4], t[5]);
+    } else {
+        hipLaunchKernelGGL((k_d4_sweep<true, false>), grid, blk, 0, c->stream, g, nullptr, nullptr, mu4, up, vp, sh,
+                           t[0], t[1], nullptr, nullptr);
+        hipLaunchKernelGGL((k_d4_sweep<false, false>), grid, blk, 0, c->stream, g, nullptr, nullptr, mu4, t[0], t[1],
+                           sh, t[2], t[3], t[4], t[5]);
+    }
 }
 
 // Device array of the 12 depth pointers (k_hh_init and k_shift index

@@ -3,9 +3,10 @@ substeps and averages included) behind the reference's own boundary, bit
 for bit against the oracle and against the stage-by-stage path: uniform,
 stretched (every metric a different 2-D field) and longitude-latitude
 metrics, random states, water flux and caller RHS, nstep 1..3, chained
-calls, the biharmonic viscosity (ksw4 = 1), the device-address entry, the
-full 2048 x 1536 basin, and the rule that sends other calls (periodic,
-open halo) down the stage-by-stage path."""
+calls, the biharmonic viscosity (ksw4 = 1), the shipped run's
+nstep_barotrop = 50 (ocean_run.par:9) with ksw4 = 1 on the Black Sea grid,
+the device-address entry, the full 2048 x 1536 basin, and the rule that
+sends other calls (periodic, open halo) down the stage-by-stage path."""
 import os
 
 import numpy as np
@@ -58,7 +59,7 @@ def _call_model(m, b, nstep, st, acc, inp, rdis, ksw4=0):
 @pytest.mark.parametrize("kind,nstep,calls,ksw4", [("uniform", 2, 3, 0), ("stretched", 3, 2, 0), ("spherical", 2, 2, 0),
                                                    ("ragged", 1, 3, 0), ("wide", 2, 1, 0), ("uniform", 2, 2, 1),
                                                    ("stretched", 3, 2, 1), ("spherical", 2, 2, 1),
-                                                   ("ragged", 1, 2, 1)])
+                                                   ("ragged", 1, 2, 1), ("spherical", 50, 1, 1)])
 def test_fused_dropin_matches_oracle(kind, nstep, calls, ksw4):
     b = _basin(kind)
     og = oracle.OracleGrid(b)
