@@ -519,7 +519,7 @@ struct D4Stress {
 // (a, b) = (fx, fy), giving o0..o3 = px, qx, py, qy.  Every point of the
 // array is written, 0 off lcu / lcv and outside the interior.  A wave's
 // points share their row, so with RowView its metric reads are uniform.
-constexpr int D4_TX = 64, D4_TY = 8, D4_ROWS = 4;
+constexpr int D4_TX = 64, D4_TY = 15, D4_ROWS = 4;
 constexpr int D4_W = D4_TX + 1, D4_H = D4_TY + 1;
 
 template <bool FIRST, class V>
