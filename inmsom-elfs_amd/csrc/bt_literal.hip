@@ -457,6 +457,26 @@ __global__ void k_diff4_add(DevGrid g, const double *__restrict__ mu, const doub
 // points its stencil reads, with the same expressions, and the value 0 where
 // the reference does not compute them (outside their index range or mask),
 // as the zero-initialised planes hold there.
+// A view whose reads are moved into the array ([1, nx] x [1, ny]): the
+// stencil of a point outside the interior can then be formed without a
+// branch and discarded; at interior points nothing moves.
+template <class V>
+struct ClampView {
+    V w;
+    const DevGrid *g;
+    __device__ int cx(int a) const { return min(max(a, 1), g->nx); }
+    __device__ int cy(int b) const { return min(max(b, 1), g->ny); }
+    __device__ size_t at(int a, int b) const { return w.at(cx(a), cy(b)); }
+    __device__ double dx(int a, int b) const { return w.dx(cx(a), cy(b)); }
+    __device__ double dy(int a, int b) const { return w.dy(cx(a), cy(b)); }
+    __device__ double dxt(int a, int b) const { return w.dxt(cx(a), cy(b)); }
+    __device__ double dyt(int a, int b) const { return w.dyt(cx(a), cy(b)); }
+    __device__ double dxh(int a, int b) const { return w.dxh(cx(a), cy(b)); }
+    __device__ double dyh(int a, int b) const { return w.dyh(cx(a), cy(b)); }
+    __device__ double dxb(int a, int b) const { return w.dxb(cx(a), cy(b)); }
+    __device__ double dyb(int a, int b) const { return w.dyb(cx(a), cy(b)); }
+};
+
 template <class V>
 struct D4Depth {
     const V w;
@@ -486,6 +506,27 @@ struct D4Depth {
         if (!in_range(a, b) || !w.is_luh(a, b)) return 0.0;
         return avg_h_w(w, [this](int x, int y) { return hq(x, y); }, a, b);
     }
+    __device__ double hu_nb(int a, int b) const
+    {
+        const int ac = min(max(a, 1), w.g->nx - 1), bc = min(max(b, 1), w.g->ny - 1);
+        const double h = avg_u_w(w, [this](int x, int y) { return hq(x, y); }, ac, bc);
+        return in_range(a, b) && w.is_llu(ac, bc) ? h : 0.0;
+    }
+    __device__ double hv_nb(int a, int b) const
+    {
+        const int ac = min(max(a, 1), w.g->nx - 1), bc = min(max(b, 1), w.g->ny - 1);
+        const double h = avg_v_w(w, [this](int x, int y) { return hq(x, y); }, ac, bc);
+        return in_range(a, b) && w.is_llv(ac, bc) ? h : 0.0;
+    }
+    // the same values without a branch: the average is formed at (a, b)
+    // moved into [1, nx-1] x [1, ny-1] (every point it reads in the array;
+    // unchanged where the value is used) and selected
+    __device__ double hh_nb(int a, int b) const
+    {
+        const int ac = min(max(a, 1), w.g->nx - 1), bc = min(max(b, 1), w.g->ny - 1);
+        const double h = avg_h_w(w, [this](int x, int y) { return hq(x, y); }, ac, bc);
+        return in_range(a, b) && w.is_luh(ac, bc) ? h : 0.0;
+    }
 };
 
 template <class V>
@@ -505,6 +546,20 @@ struct D4Stress {
     {
         if (!interior(a, b) || !w.is_luu(a, b)) return 0.0;
         return shear_at(w, u, v, a, b);
+    }
+    // t and s without a branch, as D4Depth::hh_nb: tension reads (a-1..a,
+    // b-1..b), shear (a..a+1, b..b+1)
+    __device__ double t_nb(int a, int b) const
+    {
+        const int ac = min(max(a, 2), w.g->nx - 1), bc = min(max(b, 2), w.g->ny - 1);
+        const double t = tension_at(w, u, v, ac, bc);
+        return interior(a, b) && w.is_lu(ac, bc) ? t : 0.0;
+    }
+    __device__ double s_nb(int a, int b) const
+    {
+        const int ac = min(max(a, 1), w.g->nx - 1), bc = min(max(b, 1), w.g->ny - 1);
+        const double sv = shear_at(w, u, v, ac, bc);
+        return interior(a, b) && w.is_luu(ac, bc) ? sv : 0.0;
     }
 };
 
@@ -538,9 +593,9 @@ __device__ __forceinline__ void d4_sweep(const V &w, const double *__restrict__ 
     const D4Stress<V> st{w, Plane{&g, a}, Plane{&g, b}};
     for (int r = (int)threadIdx.y; r < D4_H; r += D4_ROWS) {
         const int ru = __builtin_amdgcn_readfirstlane(r);  // one row per wave
-        Ts[ru][tx] = st.t(m0 + tx, n0 + ru);
-        Ss[ru][tx] = st.s(m0 - 1 + tx, n0 - 1 + ru);
-        Hs[ru][tx] = d.hh(m0 - 1 + tx, n0 - 1 + ru);
+        Ts[ru][tx] = st.t_nb(m0 + tx, n0 + ru);
+        Ss[ru][tx] = st.s_nb(m0 - 1 + tx, n0 - 1 + ru);
+        Hs[ru][tx] = d.hh_nb(m0 - 1 + tx, n0 - 1 + ru);
     }
     if (tid < D4_H) {  // the tile's last column
         Ts[tid][D4_TX] = st.t(m0 + D4_TX, n0 + tid);
@@ -551,35 +606,34 @@ __device__ __forceinline__ void d4_sweep(const V &w, const double *__restrict__ 
     auto tt = [&](int x, int y) { return Ts[y - n0][x - m0]; };
     auto ss = [&](int x, int y) { return Ss[y - n0 + 1][x - m0 + 1]; };
     auto hh = [&](int x, int y) { return Hs[y - n0 + 1][x - m0 + 1]; };
-    auto hq = [&](int x, int y) { return d.hq(x, y); };
+    // phase 2 forms both stencils at every point of the tile without a
+    // branch (LDS reads stay in the tile; array reads go through cw) and
+    // keeps them on lcu / lcv points of the interior
+    const ClampView<V> cw{w, &g};
+    auto hq = [&](int x, int y) { return d.hq(cw.cx(x), cw.cy(y)); };
     const int m = m0 + tx;
     if (m > g.nx) return;
     for (int j = (int)threadIdx.y; j < D4_TY; j += D4_ROWS) {
         const int n = __builtin_amdgcn_readfirstlane(n0 + j);  // one row per wave
         if (n > g.ny) return;
         const size_t i = w.at(m, n);
-        double pu = 0.0, qu = 0.0, pv = 0.0, qv = 0.0;
+        double pu, qu, pv, qv;
         const bool in = IN_INTERIOR(g);
         const bool cu = in && w.is_lcu(m, n), cv = in && w.is_lcv(m, n);
-        if (cu) visc_u_at(w, mu, tt, ss, hq, hh, m, n, pu, qu);
-        if (cv) visc_v_at(w, mu, tt, ss, hq, hh, m, n, pv, qv);
+        visc_u_at(cw, mu, tt, ss, hq, hh, m, n, pu, qu);
+        visc_v_at(cw, mu, tt, ss, hq, hh, m, n, pv, qv);
         if constexpr (FIRST) {
-            double vx = 0.0, vy = 0.0;
-            if (cu) {
-                double f = pu + qu;
-                vx = -f / d.hu(m, n) / w.dxt(m, n) / w.dyh(m, n);
-            }
-            if (cv) {
-                double f = -pv + qv;
-                vy = -f / d.hv(m, n) / w.dxh(m, n) / w.dyt(m, n);
-            }
-            o0[i] = vx;
-            o1[i] = vy;
+            const double fu = pu + qu;
+            const double vx = -fu / d.hu_nb(m, n) / cw.dxt(m, n) / cw.dyh(m, n);
+            const double fv = -pv + qv;
+            const double vy = -fv / d.hv_nb(m, n) / cw.dxh(m, n) / cw.dyt(m, n);
+            o0[i] = cu ? vx : 0.0;
+            o1[i] = cv ? vy : 0.0;
         } else {
-            o0[i] = pu;
-            o1[i] = qu;
-            o2[i] = pv;
-            o3[i] = qv;
+            o0[i] = cu ? pu : 0.0;
+            o1[i] = cu ? qu : 0.0;
+            o2[i] = cv ? pv : 0.0;
+            o3[i] = cv ? qv : 0.0;
         }
     }
 }
